@@ -1,0 +1,249 @@
+/* mb3d_hip.h -- C ABI of the MI355X (gfx950) HIP backend of mb3d-rs's
+ * per-pixel distance-estimated (DE) fractal ray marcher.
+ *
+ * The backend is a drop-in sibling of the reference's wgpu backend
+ * (`src/gpu.rs`): it takes the renderer's derived march parameters
+ * (`CalcParams`, flattened into `mb3d_hip_params`) and the post job
+ * (`PostJob`: hard shadows, DE ambient occlusion) and returns the G-buffer
+ * (the 20-byte card record of `src/gpu.rs:3452-3463`) and/or the raw
+ * buffers: f32 Z-buffer, u16 iteration counts, f32 normals.
+ *
+ * Conventions follow the reference's C ABI (capi/include/mb3d.h:1-16,
+ * capi/src/lib.rs:18-41): plain structs and pointers, an int status, and a
+ * thread-local text for the last failure (mb3d_hip_last_error).  No C++
+ * exception crosses this boundary.  Pointers are not kept after a call
+ * returns.  One device per process (mb3d_hip_init); calls from several
+ * threads are serialised inside the library.
+ */
+#ifndef MB3D_HIP_H
+#define MB3D_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MB3D_HIP_ABI_VERSION 1
+#define MB3D_HIP_MAX_SLOTS 6
+#define MB3D_HIP_MAX_LIGHTS 6
+
+/* ---- status codes (the reference's Option/Result of gpu::march,
+ *      src/gpu.rs:2296-2404, as integers) ---- */
+#define MB3D_HIP_OK 0
+#define MB3D_HIP_CANCELLED 1      /* the cancel callback asked to stop ("cancelled") */
+#define MB3D_HIP_UNSUPPORTED 2    /* the scene needs a feature this backend lacks: run the CPU path */
+#define MB3D_HIP_OVERFLOW 3       /* > 1/500 of the pixels overflowed single precision (src/gpu.rs:2350-2375) */
+#define MB3D_HIP_ERR_INVALID (-1) /* bad argument */
+#define MB3D_HIP_ERR_DEVICE (-2)  /* HIP runtime error, no device */
+#define MB3D_HIP_ERR_NOT_INIT (-3)
+
+/* ---- formula kinds of a hybrid slot (src/formulas.rs:28-68) ---- */
+#define MB3D_HIP_FORMULA_NONE 0
+#define MB3D_HIP_FORMULA_INTPOW 1     /* 'Integer Power' (triplex sine bulb); power 8 on the card */
+#define MB3D_HIP_FORMULA_AMAZINGBOX 2 /* 'Amazing Box' (Mandelbox) */
+
+/* ---- DE modes ---- */
+#define MB3D_HIP_DE_REFERENCE 0   /* the reference's: numeric gradient (option 0) or analytic box (11) */
+#define MB3D_HIP_DE_ANALYTIC_P8 1 /* extension: running derivative, 0.5 r ln r / dr (one IntPow slot only) */
+
+/* ---- iterate precision of the march ---- */
+#define MB3D_HIP_PREC_AUTO 0 /* FP32 unless the zoom is beyond it (the reference's needs_double rule) */
+#define MB3D_HIP_PREC_F32 1  /* FP32 iterate; FP64 ray position, stop decision, bisection, normals */
+#define MB3D_HIP_PREC_F64 2  /* FP64 everywhere */
+
+/* One slot of the alternating hybrid (`FormulaEntry` + `Formula`,
+ * src/scene.rs, src/formulas.rs:28-68). */
+typedef struct mb3d_hip_formula {
+    int32_t kind;       /* MB3D_HIP_FORMULA_* */
+    int32_t iterations; /* consecutive iterations of this slot (iItCount); < 0: not counted */
+    int32_t power;      /* IntPow: the integer power */
+    int32_t reserved;
+    double z_mul;       /* IntPow: Z multiplier */
+    double scale;       /* AmazingBox: scale */
+    double min_r;       /* AmazingBox: Min R */
+    double fold;        /* AmazingBox: fold */
+} mb3d_hip_formula;
+
+/* The subset of the reference's `Scene` (src/scene.rs:398-532) that feeds
+ * `CalcParams::new` for the built-in formulas, plus two declared
+ * extensions (`de_mode`, `step_cap`).  Defaults: mb3d_hip_scene_default. */
+typedef struct mb3d_hip_scene {
+    int32_t abi_version;
+    int32_t width, height;
+    int32_t iterations;        /* max iterations */
+    int32_t min_iterations;
+    int32_t smooth_normals;    /* 0..8 */
+    int32_t bin_search_steps;  /* bStepsafterDEStop */
+    int32_t first_step_random; /* 0/1 */
+    int32_t step_sub_de_stop;  /* 0/1 */
+    int32_t vary_de_stop_on_fov; /* 0/1 */
+    int32_t normals_on_de;     /* 0/1 */
+    int32_t color_option;      /* 0..5 */
+    int32_t dfog_on_it;
+    int32_t repeat_from;
+    int32_t n_formulas;        /* 1..6 */
+    int32_t de_mode;           /* MB3D_HIP_DE_* */
+    int32_t step_cap;          /* extension: max march steps per ray (0: the reference's guard of 2e6) */
+    int32_t reserved;
+    double rstop;              /* bailout radius (not squared); <= 0: the formulas' default */
+    double zoom;
+    double mid[3];
+    double z_start, z_end;
+    double vgrads[3][3];       /* camera matrix rows: screen x, screen y, view direction */
+    double fov_y;              /* degrees */
+    double de_stop;
+    double z_step_div;         /* raystep */
+    double raystep_limiter;
+    double color_mul;
+    mb3d_hip_formula formulas[MB3D_HIP_MAX_SLOTS];
+} mb3d_hip_scene;
+
+/* The derived march parameters: the reference's `CalcParams`
+ * (src/calc.rs:21-135) for the built-in formulas, with its f32-rounded
+ * fields kept as f32.  Filled by mb3d_hip_params_from_scene (a
+ * restatement of CalcParams::new, src/calc.rs:375-898) or by the caller
+ * from its own CalcParams. */
+typedef struct mb3d_hip_params {
+    int32_t abi_version;
+    int32_t width, height;
+    int32_t n_slots;
+    int32_t slot_kind[MB3D_HIP_MAX_SLOTS];
+    int32_t slot_iters[MB3D_HIP_MAX_SLOTS];
+    int32_t slot_uncounted[MB3D_HIP_MAX_SLOTS];
+    int32_t slot_ade[MB3D_HIP_MAX_SLOTS];
+    int32_t slot_power[MB3D_HIP_MAX_SLOTS];
+    int32_t de_option;       /* 0 numeric, 11 analytic box, 32 analytic p8 (extension) */
+    int32_t is_custom_de;
+    int32_t end_to, repeat_from;
+    int32_t max_it, min_it;
+    int32_t sm_normals, de_add_steps;
+    int32_t first_step_random, normals_on_de;
+    int32_t color_option, dfog_on_it;
+    int32_t step_cap;
+    int32_t reserved;
+    double slot_z_mul[MB3D_HIP_MAX_SLOTS];
+    double slot_scale[MB3D_HIP_MAX_SLOTS];
+    double slot_min_r[MB3D_HIP_MAX_SLOTS];
+    double slot_fold[MB3D_HIP_MAX_SLOTS];
+    float d_de_scale, de_stop, z_step_div, mh04zsd;
+    float de_stop_factor, ln_rstop, de_offset, de_offset006;
+    float mcts_m, ms_de_sub, d_col_plus, col_var_de_stop_mul;
+    float fovx_off, fovx_mul, pl_optic_z, mct_color_mul;
+    float fhln[MB3D_HIP_MAX_SLOTS];
+    double d_rstop, rstop3d, fov_y, zcorr, zc_mul, zend, step_width;
+    double vgrads[3][3];
+    double ystart[3];
+} mb3d_hip_params;
+
+/* The post passes that run with the march (`PostJob`, src/calc.rs:187-205;
+ * `ShadowJob`, `HsLight` src/calc.rs:2705-2732; `DeaoParams`
+ * src/deao.rs:14-29).  Directional lights only. */
+typedef struct mb3d_hip_post_job {
+    int32_t n_lights;                      /* 0: no shadow pass */
+    int32_t light_idx[MB3D_HIP_MAX_LIGHTS]; /* header light index: bit 0x400 << idx of `shadow` */
+    double light_vec[MB3D_HIP_MAX_LIGHTS][3]; /* HSvecs: world direction towards the light, length -step_width */
+    float shadow_max_len_mul;
+    int32_t deao;            /* 0: no DE ambient occlusion */
+    int32_t deao_quality;    /* 0..3 = 3, 7, 17, 33 rays (src/deao.rs:75-105); 4 = the 8-ray set (extension) */
+    float deao_max_len;
+    int32_t deao_first_step_random;
+    int32_t reserved;
+} mb3d_hip_post_job;
+
+/* Output buffers of rows [row0, row0 + rows) of the picture, each row
+ * `width` pixels.  Any pointer may be NULL (that output is skipped).  With
+ * `on_device` = 0 they are host memory, else device memory of the
+ * library's device. */
+typedef struct mb3d_hip_out {
+    float *depth;       /* march depth mzz in step widths (x step_width = scene units along the ray);
+                           +inf: background; 0: the ray starts inside the set */
+    double *depth64;    /* the same in f64 */
+    uint16_t *iters;    /* it_result at the surface point (after the bisection); 0 background */
+    float *normals;     /* 3 per pixel: unit normal in the view (vgrads) frame; 0 background */
+    uint32_t *gbuf;     /* 5 words per pixel: the card record of src/gpu.rs:3452-3463 */
+    int32_t on_device;
+    int32_t reserved;
+} mb3d_hip_out;
+
+/* Launch and precision options of one call (zeros: the defaults). */
+typedef struct mb3d_hip_options {
+    int32_t precision;     /* MB3D_HIP_PREC_* */
+    int32_t tile_w, tile_h; /* pixel tile of one wave's work item (default 8 x 8) */
+    int32_t waves_per_cu;  /* persistent grid: waves per CU (default 8) */
+    int32_t it_split;      /* extension: march steps per resident turn before a ray yields (0: none) */
+    float guard_band;      /* FP32 stop decisions within (1 + guard_band) x DE stop go to FP64 (default 0.25) */
+    int32_t reserved[4];
+} mb3d_hip_options;
+
+/* Work counters of the last call on this thread: the algorithmic work that
+ * the roofline figures are computed from (each lane counts its own). */
+typedef struct mb3d_hip_counters {
+    uint64_t pixels;
+    uint64_t hits;
+    uint64_t steps;          /* march steps */
+    uint64_t de_evals_f32;   /* DE evaluations with the FP32 iterate */
+    uint64_t de_evals_f64;   /* DE evaluations with the FP64 iterate */
+    uint64_t its_p8_f32, its_box_f32;  /* formula iterations by kind and precision */
+    uint64_t its_p8_f64, its_box_f64;
+    uint64_t overflow;       /* pixels whose FP32 values overflowed */
+    double kernel_ms;        /* device time of the march kernels (HIP events) */
+    double post_ms;          /* device time of the post kernels */
+} mb3d_hip_counters;
+
+typedef void (*mb3d_hip_progress_fn)(void *user, int64_t done, int64_t total);
+typedef int (*mb3d_hip_cancel_fn)(void *user); /* non-zero: stop */
+
+/* Why the last failing call on this thread failed (capi/src/lib.rs:18-26).
+ * Valid until the next failing call on the thread; not to be freed. */
+const char *mb3d_hip_last_error(void);
+
+/* The ABI version the library was built with (MB3D_HIP_ABI_VERSION). */
+int32_t mb3d_hip_abi_version(void);
+
+/* Opens HIP device `device` (the reference's one global device, OnceLock,
+ * src/gpu.rs:1042-1088).  Idempotent for the same device. */
+int32_t mb3d_hip_init(int32_t device);
+
+/* Releases the library's device buffers. */
+void mb3d_hip_shutdown(void);
+
+/* Scene defaults (Scene::default, src/scene.rs:626-700) with one Integer
+ * Power p8 slot (src/formulas.rs:266-306). */
+void mb3d_hip_scene_default(mb3d_hip_scene *scene);
+
+/* CalcParams::new (src/calc.rs:375-898) for the built-in formulas. */
+int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *scene, mb3d_hip_params *params);
+
+/* gpu::unsupported (src/gpu.rs:445-477): 0 when this backend can march
+ * `params`, else MB3D_HIP_UNSUPPORTED with the reason in `reason`. */
+int32_t mb3d_hip_supported(const mb3d_hip_params *params, char *reason, size_t reason_len);
+
+/* pixel_seed (src/calc.rs:1125-1140): the random state of pixel (x, y). */
+int32_t mb3d_hip_pixel_seed(int64_t y, int64_t x);
+
+/* gpu::march / gpu::march_bands (src/gpu.rs:2296, 1586): marches rows
+ * [row0, row0 + rows) of the picture, runs the post job on them and fills
+ * `out`.  `progress` and `cancel` (may be NULL) are called on the calling
+ * thread between kernel slices. */
+int32_t mb3d_hip_march(const mb3d_hip_params *params, const mb3d_hip_post_job *post,
+                       int32_t row0, int32_t rows, const mb3d_hip_out *out,
+                       const mb3d_hip_options *options, mb3d_hip_progress_fn progress,
+                       mb3d_hip_cancel_fn cancel, void *user);
+
+/* gpu::post_on_card (src/gpu.rs:1649): the shadow and DE-AO passes on a
+ * G-buffer of rows [row0, row0 + rows) marched elsewhere (5 words per
+ * pixel, in and out). */
+int32_t mb3d_hip_post(const mb3d_hip_params *params, const mb3d_hip_post_job *post,
+                      int32_t row0, int32_t rows, uint32_t *gbuf, int32_t on_device,
+                      const mb3d_hip_options *options);
+
+/* The work counters of the last march/post call on this thread. */
+int32_t mb3d_hip_counters_get(mb3d_hip_counters *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MB3D_HIP_H */

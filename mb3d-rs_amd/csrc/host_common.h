@@ -1,0 +1,18 @@
+// host_common.h -- host-side helpers shared by the backend's sources: the
+// thread-local last error (as capi/src/lib.rs:18-26) and the capability
+// check (gpu::unsupported, src/gpu.rs:445-477).
+#pragma once
+
+#include <string>
+
+#include "../../include/mb3d_hip.h"
+
+namespace mb3d {
+
+// Records `msg` as this thread's last error and returns `code`.
+int set_error(int code, const char *msg);
+
+// Why this backend cannot march `p` ("" when it can).
+std::string unsupported_reason(const mb3d_hip_params *p);
+
+}  // namespace mb3d

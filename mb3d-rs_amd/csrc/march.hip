@@ -1,0 +1,1242 @@
+// march.hip -- the HIP kernels of the DE ray marcher for gfx950 and the C ABI
+// that launches them (include/mb3d_hip.h).
+//
+// Passes of one call (all on one stream, no host round trip in between):
+//  1. k_march32  persistent, wave64 lanes refilled from a tile queue: the
+//                FP32-iterate march (march_pixel_from + march_from,
+//                src/calc.rs:2231-2322,2533-2657) up to the guard band
+//                around the DE stop; background rays finish here, rays near
+//                the surface are handed to pass 2 through a compacted list.
+//  2. k_finish64 the reference's f64 marcher from the hand-off state: the
+//                stop decision, bisection (src/calc.rs:1706-1792), normals
+//                (1813-1858), colour and depth (2471-2517, 2194-2205).
+//  3. k_shadow   per surface pixel: hs_surface_start + the march towards each
+//                light (src/calc.rs:2741-2899, without the row shortcut).
+//  4. k_deao     per surface pixel: surface_point + DE AO rays
+//                (src/calc.rs:2157-2192, src/deao.rs:268-418).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mb3d_hip.h"
+#include "device_de.h"
+#include "host_common.h"
+
+namespace mb3d {
+
+// ---------------------------------------------------------------------------
+// device buffers of one call
+
+// State of a ray handed from the FP32 march to the FP64 finish.
+struct Cand {
+    double c[3], v[3];
+    double mzz, last_step, last_de;
+    float rsf, step_count;
+    int32_t pix, flags, nsteps, seed;
+};
+enum : int32_t { CF_START = 1, CF_FIRST = 2, CF_STEPPED = 4 };
+
+// counter words of Bufs::ctr
+enum : int {
+    C_QUEUE = 0,   // next work item of the march queue
+    C_CAND = 1,    // hand-off list length
+    C_HITS = 2,    // surface pixel list length
+    C_STATS = 8,   // 9 work counters from here
+    C_WORDS = 24
+};
+
+struct Bufs {
+    float *depth;
+    double *depth64;
+    uint16_t *iters;
+    float *normals;
+    uint32_t *gbuf;
+    Cand *cand;
+    int32_t *hits;
+    unsigned long long *ctr;
+};
+
+// DE AO ray set (build_rays, src/deao.rs:75-105), computed on the host
+struct AoRays {
+    float dir[33][3];
+    int count;
+    float d_step_mul, d_min_a_dif, correction_weight, max_len;
+};
+
+struct ShadowJob {
+    int n;
+    int bit[MB3D_HIP_MAX_LIGHTS];
+    double vec[MB3D_HIP_MAX_LIGHTS][3];
+    float max_len_mul;
+};
+
+// ---------------------------------------------------------------------------
+// wave helpers
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ void flush_counters(const Cnt &c, unsigned long long *ctr) {
+    const unsigned long long v[9] = {c.steps, c.de32, c.de64, c.p8_32, c.box_32, c.p8_64, c.box_64, c.hits, c.overflow};
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const unsigned long long s = wave_sum(v[k]);
+        if (lane_id() == 0 && s) atomicAdd(&ctr[C_STATS + k], s);
+    }
+}
+
+// appends one entry per lane with `want` set: returns the lane's slot
+__device__ __forceinline__ long long wave_append(bool want, unsigned long long *counter) {
+    const unsigned long long m = __ballot(want);
+    if (!m) return -1;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (!want) return -1;
+    return (long long)(base + __popcll(m & ((1ull << lane_id()) - 1ull)));
+}
+
+__device__ void write_background(const DevP &P, const Bufs &B, int pix, float step_count) {
+    const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
+    if (B.depth) B.depth[pix] = __builtin_inff();
+    if (B.depth64) B.depth64[pix] = __builtin_inf();
+    if (B.iters) B.iters[pix] = 0;
+    if (B.normals) {
+        B.normals[3 * pix] = 0.0f;
+        B.normals[3 * pix + 1] = 0.0f;
+        B.normals[3 * pix + 2] = 0.0f;
+    }
+    if (B.gbuf) {
+        uint32_t *g = B.gbuf + 5 * (size_t)pix;
+        g[0] = 0;
+        g[1] = sh << 16;
+        g[2] = 32768u << 16;
+        g[3] = 0;
+        g[4] = 5000;
+    }
+}
+
+// pixel_seed (src/calc.rs:1125-1140)
+__device__ int32_t dev_pixel_seed(int64_t y, int64_t x) {
+    uint32_t s = (uint32_t)(int32_t)(0x24563487LL + (y + 1) * 0x324594A1LL);
+    uint32_t mul = 214013u, add = 2531011u, am = 1u, aa = 0u;
+    for (uint64_t n = x > 0 ? (uint64_t)x : 0; n; n >>= 1) {
+        if (n & 1) {
+            aa = aa * mul + add;
+            am *= mul;
+        }
+        add = add * mul + add;
+        mul *= mul;
+    }
+    return (int32_t)(am * s + aa);
+}
+
+// ---------------------------------------------------------------------------
+// pass 1: the FP32 march, persistent, lanes refilled one by one
+
+template <int KIND, int DEM>
+__global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
+    Cnt cnt;
+    const int lane = lane_id();
+    bool have = false, exhausted = false;
+    int pix = 0, px = 0, py = 0;
+    double c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    double mzz = 0, last_step = 0, last_de = 0, dtmp = 0;
+    float ms = 0, rsf = 1, scount = 0;
+    int nsteps = 0;
+    bool start = false, first = false;
+    int32_t seed = 0;
+    const double guard = (double)P.guard;
+    const int tile_n = P.tile_w * P.tile_h;
+    for (;;) {
+        // refill: every idle lane takes the next work item (one atomic per wave)
+        const bool need = !have && !exhausted;
+        const unsigned long long m = __ballot(need);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&B.ctr[C_QUEUE], (unsigned long long)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const unsigned long long q = base + __popcll(m & ((1ull << lane) - 1ull));
+                if ((long long)q >= P.nslots_q) {
+                    exhausted = true;
+                } else {
+                    const long long tile = (long long)q / tile_n;
+                    const int k = (int)((long long)q - tile * tile_n);
+                    const int x = (int)(tile % P.tiles_x) * P.tile_w + k % P.tile_w;
+                    const int yr = (int)(tile / P.tiles_x) * P.tile_h + k / P.tile_w;
+                    if (x < P.width && yr < P.rows) {
+                        have = true;
+                        px = x;
+                        py = P.row0 + yr;
+                        pix = yr * P.width + x;
+                        pixel_ray(P, px, py, c, v);
+                        mzz = 0.0;
+                        ms = P.de_stop;
+                        rsf = 1.0f;
+                        scount = 0.0f;
+                        nsteps = 0;
+                        start = true;
+                        first = P.first_step_random != 0;
+                        seed = first ? dev_pixel_seed(py, px) : 0;
+                    }
+                }
+            }
+        }
+        if (__all(exhausted)) break;
+        bool handoff = false, background = false;
+        if (have) {
+            int it = 0;
+            const float d32 = de32<KIND, DEM>(P, c, ms, it, cnt);
+            bool top = true;
+            if (!(d32 < __builtin_inff())) {  // single precision overflow: f64 decides
+                cnt.overflow++;
+                handoff = true;
+                top = false;
+            } else if (start) {
+                if (it >= P.max_it || !((double)d32 >= (double)ms * guard)) {
+                    handoff = true;
+                    top = false;
+                } else {
+                    dtmp = (double)d32;
+                    last_step = dtmp * (double)P.z_step_div;
+                }
+            } else {
+                dtmp = (double)d32;
+                if (dtmp > last_de + last_step) dtmp = last_de + last_step;
+                if (last_de > dtmp + 1e-30) {
+                    const double t = last_step / (last_de - dtmp);
+                    rsf = t < 1.0 ? (float)fmax(0.5, t) : 1.0f;
+                } else {
+                    rsf = 1.0f;
+                }
+            }
+            if (top) {
+                if (it >= P.max_it) {
+                    handoff = true;  // the half step back (src/calc.rs:2545-2552) is f64 work
+                } else if (it < P.min_it || dtmp >= (double)ms * guard) {
+                    if (P.step_cap > 0 && nsteps >= P.step_cap) {
+                        background = true;
+                    } else {
+                        last_de = dtmp;
+                        double d = fmax(0.11, (dtmp - (double)P.ms_de_sub * (double)ms) * (double)P.z_step_div *
+                                                  (double)rsf);
+                        const double s1 = (double)(fmaxf(ms, 0.4f) * P.mh04zsd);
+                        const bool count = P.dfog_on_it == 0 || it == P.dfog_on_it;
+                        if (s1 < d) {
+                            if (count) scount += (float)(s1 / d);
+                            d = s1;
+                        } else if (count) {
+                            scount += 1.0f;
+                        }
+                        if (first) {
+                            first = false;
+                            seed = (int32_t)((uint32_t)seed * 214013u + 2531011u);
+                            d *= (double)(seed & 0x7FFFFFFF) * (1.0 / (double)0x7FFFFFFF);
+                        }
+                        mzz += d;
+                        if (mzz > P.zend) {
+                            background = true;
+                        } else {
+                            nsteps++;
+                            cnt.steps++;
+                            last_step = d;
+                            c[0] += v[0] * d;
+                            c[1] += v[1] * d;
+                            c[2] += v[2] * d;
+                            ms = (float)((double)P.de_stop * (1.0 + mzz * (double)P.de_stop_factor));
+                            start = false;
+                        }
+                    }
+                } else {
+                    handoff = true;
+                }
+            }
+        }
+        const long long slot = wave_append(handoff, &B.ctr[C_CAND]);
+        if (handoff) {
+            Cand &o = B.cand[slot];
+            o.c[0] = c[0];
+            o.c[1] = c[1];
+            o.c[2] = c[2];
+            o.v[0] = v[0];
+            o.v[1] = v[1];
+            o.v[2] = v[2];
+            o.mzz = mzz;
+            o.last_step = last_step;
+            o.last_de = last_de;
+            o.rsf = rsf;
+            o.step_count = scount;
+            o.pix = pix;
+            o.flags = (start ? CF_START : 0) | (first ? CF_FIRST : 0) | (nsteps > 0 ? CF_STEPPED : 0);
+            o.nsteps = nsteps;
+            o.seed = seed;
+            have = false;
+        }
+        if (background) {
+            write_background(P, B, pix, scount);
+            have = false;
+        }
+    }
+    flush_counters(cnt, B.ctr);
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: the f64 finish
+
+__device__ void write_surface(const DevP &P, const Bufs &B, int pix, const M64 &m, const double n[3], int16_t nq[3],
+                              float nn, bool de_limited, uint16_t iters, float step_count, uint32_t zf) {
+    // si_gradient (src/calc.rs:2497-2502)
+    float g;
+    if (de_limited) {
+        const float lg = (float)log((double)fmaxf(P.de_stop, m.ms_de_stop) * P.step_width);
+        g = 32767.0f - (nn + P.d_col_plus + P.col_var * lg) * P.mcts_m;
+    } else {
+        g = 32767.0f - nn * P.mcts_m;
+    }
+    const uint32_t si_gradient = clip15(g);
+    const uint32_t otrap = clip15((float)(m.otrap * 4096.0));  // do_color, option 0
+    const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
+    const double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (B.depth) B.depth[pix] = (float)m.mzz;
+    if (B.depth64) B.depth64[pix] = m.mzz;
+    if (B.iters) B.iters[pix] = iters;
+    if (B.normals) {
+        for (int k = 0; k < 3; k++) B.normals[3 * pix + k] = l > 0 ? (float)(n[k] / l) : 0.0f;
+    }
+    if (B.gbuf) {
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        w[0] = (uint32_t)(uint16_t)nq[0] | ((uint32_t)(uint16_t)nq[1] << 16);
+        w[1] = (uint32_t)(uint16_t)nq[2] | (sh << 16);
+        w[2] = zf;
+        w[3] = si_gradient | (otrap << 16);
+        w[4] = 5000;
+    }
+}
+
+// store_normal (src/calc.rs:1794-1804)
+__device__ void store_normal(const double n[3], int16_t out[3]) {
+    const double d = 32767.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] + 1e-100);
+    for (int k = 0; k < 3; k++) {
+        const double v = rte(n[k] * d);
+        out[k] = isnan(v) ? 0 : (int16_t)fmin(fmax(v, -32767.0), 32767.0);
+    }
+    if (out[0] == 0 && out[1] == 0) out[2] = out[2] > 0 ? 32767 : -32767;
+}
+
+__global__ __launch_bounds__(256) void k_finish64(DevP P, Bufs B, int all_pixels) {
+    Cnt cnt;
+    const long long n = all_pixels ? P.npix : (long long)B.ctr[C_CAND];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        M64 m;
+        m.init(P);
+        int pix;
+        int flags;
+        double last_step = 0, last_de = 0;
+        float rsf = 1.0f, scount = 0.0f;
+        int nsteps = 0;
+        int32_t seed = 0;
+        if (all_pixels) {
+            pix = (int)i;
+            const int x = pix % P.width, y = P.row0 + pix / P.width;
+            pixel_ray(P, x, y, m.c, m.vfov);
+            flags = CF_START | (P.first_step_random ? CF_FIRST : 0);
+            seed = P.first_step_random ? dev_pixel_seed(y, x) : 0;
+        } else {
+            const Cand &o = B.cand[i];
+            pix = o.pix;
+            for (int k = 0; k < 3; k++) {
+                m.c[k] = o.c[k];
+                m.vfov[k] = o.v[k];
+            }
+            m.mzz = o.mzz;
+            last_step = o.last_step;
+            last_de = o.last_de;
+            rsf = o.rsf;
+            scount = o.step_count;
+            flags = o.flags;
+            nsteps = o.nsteps;
+            seed = o.seed;
+        }
+        bool first = (flags & CF_FIRST) != 0;
+        double dtmp;
+        if (flags & CF_START) {
+            m.mzz = 0.0;
+            m.ms_de_stop = P.de_stop;
+            dtmp = m.calc_de(P, cnt);
+            if (m.it_result >= m.max_its_result || dtmp < (double)m.ms_de_stop) {
+                // inside the set at the start plane (src/calc.rs:2277-2293)
+                const double t = fmin(fmax(m.rout / P.d_rstop, 0.0), 1.0);
+                const uint32_t sg = 32768u + (uint32_t)rte(32767.0 * t);
+                const uint32_t otrap = clip15((float)(m.otrap * 4096.0));
+                if (B.depth) B.depth[pix] = 0.0f;
+                if (B.depth64) B.depth64[pix] = 0.0;
+                if (B.iters) B.iters[pix] = (uint16_t)max(m.it_result, 0);
+                if (B.normals) {
+                    B.normals[3 * pix] = 0.0f;
+                    B.normals[3 * pix + 1] = 0.0f;
+                    B.normals[3 * pix + 2] = -1.0f;
+                }
+                if (B.gbuf) {
+                    uint32_t *w = B.gbuf + 5 * (size_t)pix;
+                    w[0] = 0;
+                    w[1] = (uint32_t)(uint16_t)(int16_t)-32767;
+                    w[2] = 0x7FFF0000u;
+                    w[3] = sg | (otrap << 16);
+                    w[4] = 5000;
+                }
+                continue;
+            }
+            last_step = dtmp * (double)P.z_step_div;
+        } else {
+            m.update_de_stop(P);
+            dtmp = m.calc_de(P, cnt);
+            if (flags & CF_STEPPED) {
+                if (dtmp > last_de + last_step) dtmp = last_de + last_step;
+                if (last_de > dtmp + 1e-30) {
+                    const double t = last_step / (last_de - dtmp);
+                    rsf = t < 1.0 ? (float)fmax(0.5, t) : 1.0f;
+                } else {
+                    rsf = 1.0f;
+                }
+            } else {
+                last_step = dtmp * (double)P.z_step_div;
+            }
+        }
+        // the rest of march_from (src/calc.rs:2533-2657) in f64
+        bool hit = false;
+        for (uint32_t guard = 1; guard <= 2000000u; guard++) {
+            if (m.it_result >= m.max_its_result) {
+                const double dt1 = -0.5 * last_step;
+                m.mzz += dt1;
+                m.step(dt1);
+                m.update_de_stop(P);
+                dtmp = m.calc_de(P, cnt);
+                last_step = -dt1;
+            }
+            if (m.it_result < P.min_it || (m.it_result < m.max_its_result && dtmp >= (double)m.ms_de_stop)) {
+                if (P.step_cap > 0 && nsteps >= P.step_cap) break;
+                last_de = dtmp;
+                double d = fmax(0.11, (dtmp - (double)P.ms_de_sub * (double)m.ms_de_stop) * (double)P.z_step_div *
+                                          (double)rsf);
+                const double s1 = (double)(fmaxf(m.ms_de_stop, 0.4f) * P.mh04zsd);
+                const bool count = P.dfog_on_it == 0 || m.it_result == P.dfog_on_it;
+                if (s1 < d) {
+                    if (count) scount += (float)(s1 / d);
+                    d = s1;
+                } else if (count) {
+                    scount += 1.0f;
+                }
+                if (first) {
+                    first = false;
+                    seed = (int32_t)((uint32_t)seed * 214013u + 2531011u);
+                    d *= (double)(seed & 0x7FFFFFFF) * (1.0 / (double)0x7FFFFFFF);
+                }
+                m.mzz += d;
+                if (m.mzz > P.zend) break;
+                nsteps++;
+                cnt.steps++;
+                last_step = d;
+                m.step(d);
+                m.update_de_stop(P);
+                dtmp = m.calc_de(P, cnt);
+                if (dtmp > last_de + last_step) dtmp = last_de + last_step;
+                if (last_de > dtmp + 1e-30) {
+                    const double t = last_step / (last_de - dtmp);
+                    rsf = t < 1.0 ? (float)fmax(0.5, t) : 1.0f;
+                } else {
+                    rsf = 1.0f;
+                }
+            } else {
+                hit = true;
+                break;
+            }
+        }
+        if (!hit) {
+            write_background(P, B, pix, scount);
+            continue;
+        }
+        // surface_found (src/calc.rs:2471-2517)
+        const bool de_limited = m.it_result < m.max_its_result || dtmp < (double)m.ms_de_stop;
+        if (P.de_add_steps != 0) {
+            if (de_limited)
+                m.bin_search(P, dtmp, last_step, cnt);
+            else
+                m.bin_search_it(P, P.de_add_steps, cnt);
+        }
+        const uint16_t iters = (uint16_t)max(m.it_result, 0);
+        double nv[3];
+        const float nn = m.normals(P, nv, cnt);
+        int16_t nq[3];
+        store_normal(nv, nq);
+        write_surface(P, B, pix, m, nv, nq, nn, de_limited, iters, scount, m.zpos_fine(P));
+        cnt.hits++;
+        const long long h = (long long)atomicAdd(&B.ctr[C_HITS], 1ull);
+        B.hits[h] = pix;
+    }
+    flush_counters(cnt, B.ctr);
+}
+
+// ---------------------------------------------------------------------------
+// the post passes
+
+// hs_surface_start / surface_point: the f64 surface point of a G-buffer
+// pixel from its depth code (src/calc.rs:2741-2765, 2157-2192)
+__device__ void surface_from_code(const DevP &P, M64 &m, int x, int y, uint32_t zpos_fine, bool ao, Cnt &cnt) {
+    pixel_ray(P, x, y, m.c, m.vfov);
+    m.calc_sit = false;
+    const double zf = (double)(zpos_fine >> 8);
+    m.mzz = depth_of_code(P, zf, 8388351.5);
+    m.step(m.mzz);
+    m.update_de_stop(P);
+    if (!ao) {
+        double d = m.calc_de(P, cnt);
+        const bool de_limited = m.it_result < m.max_its_result || d < (double)m.ms_de_stop;
+        if (de_limited)
+            m.bin_search(P, d, depth_of_code(P, zf, 8388351.9) - m.mzz, cnt);
+        else
+            m.bin_search_it(P, P.de_add_steps, cnt);
+        return;
+    }
+    double d = fmin(m.calc_de(P, cnt), (double)m.ms_de_stop * 2.0);
+    if (m.it_result < m.max_its_result || d < (double)m.ms_de_stop) {
+        double dt1 = depth_of_code(P, zf, 8388351.9) - m.mzz;
+        int itmp = 5;
+        while (itmp > 0 && fabs(d - (double)m.ms_de_stop) > 0.004) {
+            m.mzz += dt1;
+            m.step(dt1);
+            m.update_de_stop(P);
+            itmp -= 1;
+            if (itmp > 0) {
+                d = m.calc_de(P, cnt);
+                dt1 = d < (double)m.ms_de_stop ? fabs(dt1) * -0.55 : fabs(dt1) * 0.55;
+            }
+        }
+    } else {
+        m.bin_search_it(P, 5, cnt);
+        m.ms_de_stop = (float)((double)P.de_stop * (1.0 + fabs(m.mzz) * (double)P.de_stop_factor));
+    }
+}
+
+// The DE of the bulk precision, or f64 where the decision is close: DE
+// within the guard band of the DE stop, or at the iteration limit.
+template <int KIND, int DEM, bool F32>
+__device__ double banded_de(const DevP &P, M64 &m, float ms, int &it, int &max_its, Cnt &cnt) {
+    if (F32) {
+        const float d32 = de32<KIND, DEM>(P, m.c, ms, it, cnt);
+        if (d32 < __builtin_inff() && it < P.max_it && (double)d32 > (double)ms * (double)P.guard) {
+            max_its = P.max_it;
+            return (double)d32;
+        }
+    }
+    m.ms_de_stop = ms;
+    const double d = m.calc_de(P, cnt);
+    it = m.it_result;
+    max_its = m.max_its_result;
+    return d;
+}
+
+template <int KIND, int DEM, bool F32>
+__global__ __launch_bounds__(256) void k_shadow(DevP P, Bufs B, ShadowJob J) {
+    Cnt cnt;
+    const long long n = (long long)B.ctr[C_HITS];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    uint16_t clear_mask = 0;
+    for (int l = 0; l < J.n; l++) clear_mask |= (uint16_t)J.bit[l];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int pix = B.hits[i];
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        uint32_t shadow = (w[1] >> 16) & ~(uint32_t)clear_mask;
+        const int x = pix % P.width, y = P.row0 + pix / P.width;
+        if ((w[2] >> 16) >= 32768u || (w[3] & 0xFFFF) >= 32768u) continue;
+        const double nv0 = (double)(int16_t)(w[0] & 0xFFFF) / 32767.0;
+        const double nv1 = (double)(int16_t)(w[0] >> 16) / 32767.0;
+        const double nv2 = (double)(int16_t)(w[1] & 0xFFFF) / 32767.0;
+        double nvec[3];
+        for (int k = 0; k < 3; k++) nvec[k] = nv0 * P.vg[0][k] + nv1 * P.vg[1][k] + nv2 * P.vg[2][k];
+        M64 m;
+        m.init(P);
+        surface_from_code(P, m, x, y, w[2], false, cnt);
+        m.mzz = fmax(m.mzz - 0.1, 0.0);
+        m.step(-0.1);
+        m.update_de_stop(P);
+        const double ic[3] = {m.c[0], m.c[1], m.c[2]};
+        // hs_max_len (src/calc.rs:2767-2775)
+        const float mz = (float)fmin(m.mzz, P.zend * 0.4);
+        const double max_lhs = (double)((float)(P.width + y) * 0.6f *
+                                        (1.0f + 0.5f * mz * (float)fmax(P.fov_y, 0.0) / (float)P.height) *
+                                        J.max_len_mul);
+        const double vv = m.vfov[0] * m.vfov[0] + m.vfov[1] * m.vfov[1] + m.vfov[2] * m.vfov[2];
+        for (int l = 0; l < J.n; l++) {
+            const double *lv = J.vec[l];
+            m.c[0] = ic[0];
+            m.c[1] = ic[1];
+            m.c[2] = ic[2];
+            double zz2 = m.mzz;
+            const double ll = lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2];
+            const double zz2mul =
+                -((lv[0] * m.vfov[0] + lv[1] * m.vfov[1] + lv[2] * m.vfov[2]) / fmax(sqrt(ll * vv), 1e-300));
+            if (nvec[0] * lv[0] + nvec[1] * lv[1] + nvec[2] * lv[2] > 0.0) {
+                shadow |= (uint32_t)J.bit[l];
+                continue;
+            }
+            double dt1 = max_lhs;
+            if (dt1 > 0.0) {
+                float ms = (float)((double)P.de_stop * (1.0 + fabs(zz2) * (double)P.de_stop_factor));
+                float rsf = 2.0f;
+                int it = 0, max_its = P.max_it;
+                double d = banded_de<KIND, DEM, F32>(P, m, ms, it, max_its, cnt);
+                for (int guard = 1; guard <= 1000000; guard++) {
+                    const double last_de = d;
+                    const float a = fmaxf(0.11f, (float)((d - (double)P.ms_de_sub * (double)ms) *
+                                                         (double)P.z_step_div * (double)rsf));
+                    const float b = fmaxf(ms, 0.4f) * P.mh04zsd;
+                    const double st = (double)fminf(a, b);
+                    dt1 -= st;
+                    m.c[0] += lv[0] * -st;
+                    m.c[1] += lv[1] * -st;
+                    m.c[2] += lv[2] * -st;
+                    zz2 += st * zz2mul;
+                    ms = (float)((double)P.de_stop * (1.0 + fabs(zz2) * (double)P.de_stop_factor));
+                    d = banded_de<KIND, DEM, F32>(P, m, ms, it, max_its, cnt);
+                    cnt.steps++;
+                    if (it >= max_its || d <= (double)ms) break;
+                    if (d > last_de + st) d = last_de + st;
+                    if (last_de > d + 1e-30) {
+                        const float t = (float)(st / (last_de - d));
+                        rsf = t < 1.0f ? fmaxf(t, 0.5f) : 1.0f;
+                    } else {
+                        rsf = 1.0f;
+                    }
+                    if (dt1 < 0.0) break;
+                }
+            }
+            if (dt1 > 0.0) shadow |= (uint32_t)J.bit[l];
+        }
+        w[1] = (w[1] & 0xFFFFu) | (shadow << 16);
+    }
+    flush_counters(cnt, B.ctr);
+}
+
+// normal_matrix (src/deao.rs:42-58)
+__device__ void normal_matrix(const float n[3], float mo[3][3]) {
+    const double a = acos(fmin(fmax(-(double)n[2], -1.0), 1.0)) * 0.5;
+    double sa = sin(a);
+    const double ca = cos(a);
+    const double nn = sqrt((double)(n[1] * n[1] + n[0] * n[0]));
+    double q0 = 0, q1 = 0, q2 = 0;
+    if (!(nn < 1e-25)) {
+        sa /= nn;
+        q0 = -(double)n[1] * sa;
+        q1 = (double)n[0] * sa;
+    }
+    const double q3 = ca;
+    mo[0][0] = (float)(1.0 - 2.0 * (q1 * q1 + q2 * q2));
+    mo[0][1] = (float)(2.0 * (q0 * q1 + q2 * q3));
+    mo[0][2] = (float)(2.0 * (q0 * q2 - q1 * q3));
+    mo[1][0] = (float)(2.0 * (q0 * q1 - q2 * q3));
+    mo[1][1] = (float)(1.0 - 2.0 * (q0 * q0 + q2 * q2));
+    mo[1][2] = (float)(2.0 * (q2 * q1 + q0 * q3));
+    mo[2][0] = (float)(2.0 * (q0 * q2 + q1 * q3));
+    mo[2][1] = (float)(2.0 * (q1 * q2 - q0 * q3));
+    mo[2][2] = (float)(1.0 - 2.0 * (q0 * q0 + q1 * q1));
+}
+
+template <int KIND, int DEM, bool F32>
+__global__ __launch_bounds__(256) void k_deao(DevP P, Bufs B, AoRays R) {
+    Cnt cnt;
+    const long long n = (long long)B.ctr[C_HITS];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const double sw = P.step_width;
+    const double s_max_d =
+        (double)R.max_len * 0.5 * sqrt((double)P.height * (double)P.height + (double)P.width * (double)P.width);
+    const double de_stop = (double)P.de_stop;
+    const double dsm = (double)R.d_step_mul;
+    const int rc = R.count;
+    const double de_mul0 = sqrt((double)rc * 0.5);
+    const double abr = 1.2 / asin(fmin(fmax(1.0 / de_mul0, -1.0), 1.0));
+    // rows of the camera matrix scaled to the step width (normalise_matrix_to)
+    double rot_v[3][3];
+    for (int i = 0; i < 3; i++) {
+        const double d = sw / sqrt(P.vg[i][0] * P.vg[i][0] + P.vg[i][1] * P.vg[i][1] + P.vg[i][2] * P.vg[i][2]);
+        for (int j = 0; j < 3; j++) rot_v[i][j] = P.vg[i][j] * d;
+    }
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int pix = B.hits[i];
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        const int x = pix % P.width, y = P.row0 + pix / P.width;
+        if ((w[2] >> 16) >= 32768u || (w[3] & 0xFFFF) >= 32768u) {
+            w[4] = (w[4] & ~0xFFFFu);
+            continue;
+        }
+        M64 m;
+        m.init(P);
+        surface_from_code(P, m, x, y, w[2], true, cnt);
+        double ms_de_stop = fmin((double)m.ms_de_stop, 1e6);
+        const double ic[3] = {m.c[0], m.c[1], m.c[2]};
+        const double step_ao = ms_de_stop / de_stop;
+        const double max_dist = s_max_d * sqrt(step_ao);
+        ms_de_stop = P.de_stop_factor == 0.0f ? de_stop / (dsm * dsm) : ms_de_stop / (dsm * dsm);
+        const double md_d10 = 0.1 / (max_dist * de_mul0);
+        const float nf[3] = {(float)(int16_t)(w[0] & 0xFFFF), (float)(int16_t)(w[0] >> 16),
+                             (float)(int16_t)(w[1] & 0xFFFF)};
+        const float nl = fmaxf(sqrtf(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]), 1e-30f);
+        const float nu[3] = {nf[0] / nl, nf[1] / nl, nf[2] / nl};
+        float rw[3][3];
+        normal_matrix(nu, rw);
+        float min_ra[33];
+        for (int k = 0; k < rc; k++) {
+            const float *d = R.dir[k];
+            const float vv0 = rw[0][0] * d[0] + rw[0][1] * d[1] + rw[0][2] * d[2];
+            const float vv1 = rw[1][0] * d[0] + rw[1][1] * d[1] + rw[1][2] * d[2];
+            const float vv2 = rw[2][0] * d[0] + rw[2][1] * d[1] + rw[2][2] * d[2];
+            double sv[3];
+            for (int q = 0; q < 3; q++)
+                sv[q] = (double)vv0 * rot_v[0][q] + (double)vv1 * rot_v[1][q] + (double)vv2 * rot_v[2][q];
+            double dt1 = step_ao * dsm;
+            double s_tmp = 1.0;
+            bool b_end = false;
+            for (int guard = 0; guard < 100000; guard++) {
+                if (dt1 > max_dist) {
+                    dt1 = max_dist;
+                    b_end = true;
+                }
+                m.c[0] = ic[0] + sv[0] * dt1;
+                m.c[1] = ic[1] + sv[1] * dt1;
+                m.c[2] = ic[2] + sv[2] * dt1;
+                double dt2;
+                if (F32) {
+                    int it;
+                    const float d32 = de32<KIND, DEM>(P, m.c, (float)ms_de_stop, it, cnt);
+                    dt2 = d32 < __builtin_inff() ? (double)d32 : m.calc_de(P, cnt);
+                } else {
+                    dt2 = m.calc_de(P, cnt);
+                }
+                cnt.steps++;
+                s_tmp = fmin(s_tmp, (dt2 - ms_de_stop + dt1 * md_d10) / dt1);
+                if (s_tmp < 0.02) break;
+                dt1 += fmax(dt2, dt1 * dsm);
+                if (b_end) break;
+            }
+            min_ra[k] = (float)(fmax(s_tmp, 0.0) * de_mul0);
+        }
+        double amount = 0.0;
+        for (int k = 0; k < rc; k++) {
+            float s_add = 0.0f;
+            if (min_ra[k] < 1.0f) {
+                const float max_add = 1.0f - min_ra[k];
+                for (int k2 = 0; k2 < rc; k2++) {
+                    if (k2 == k) continue;
+                    const float dt = R.dir[k][0] * R.dir[k2][0] + R.dir[k][1] * R.dir[k2][1] + R.dir[k][2] * R.dir[k2][2];
+                    if (dt > R.d_min_a_dif) {
+                        const float ac = acosf(fminf(fmaxf(dt, -1.0f), 1.0f));
+                        const float ov = min_ra[k2] - (float)((double)ac * abr) + 1.0f;
+                        if (ov > 0.0f) s_add += fminf(max_add, ov) * R.correction_weight;
+                    }
+                }
+            }
+            amount += (double)fminf(s_add + min_ra[k], 1.0f);
+        }
+        const double a = rte(16383.0 * (1.0 - amount / (double)rc));
+        const uint32_t amb = (uint32_t)fmax(a, 0.0);
+        w[4] = (w[4] & ~0xFFFFu) | amb;
+    }
+    flush_counters(cnt, B.ctr);
+}
+
+// the surface pixels of a G-buffer marched elsewhere
+__global__ void k_collect_hits(DevP P, Bufs B) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < P.npix; i += stride) {
+        const uint32_t *w = B.gbuf + 5 * (size_t)i;
+        const bool surf = (w[2] >> 16) < 32768u && (w[3] & 0xFFFF) < 32768u;
+        const long long s = wave_append(surf, &B.ctr[C_HITS]);
+        if (surf) B.hits[s] = (int32_t)i;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+thread_local std::string t_error;
+thread_local mb3d_hip_counters t_counters;
+
+struct Ctx {
+    std::mutex mu;
+    int device = -1;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // grow-only device buffers
+    void *d_out = nullptr;
+    size_t out_bytes = 0;
+    void *d_work = nullptr;
+    size_t work_bytes = 0;
+    unsigned long long *d_ctr = nullptr;
+};
+Ctx g_ctx;
+
+int set_error(int code, const char *msg) {
+    t_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            char b_[256];                                                                         \
+            std::snprintf(b_, sizeof b_, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return set_error(MB3D_HIP_ERR_DEVICE, b_);                                            \
+        }                                                                                         \
+    } while (0)
+
+std::string unsupported_reason(const mb3d_hip_params *p) {
+    if (!p) return "no parameters";
+    if (p->abi_version != MB3D_HIP_ABI_VERSION) return "parameter ABI version mismatch";
+    if (p->n_slots < 1 || p->n_slots > MB3D_HIP_MAX_SLOTS) return "bad hybrid slot count";
+    for (int i = 0; i < p->n_slots; i++) {
+        if (p->slot_kind[i] == MB3D_HIP_FORMULA_INTPOW && p->slot_power[i] != 8 && p->slot_iters[i] != 0)
+            return "Integer Power other than power 8 (not on this backend)";
+        if (p->slot_kind[i] != MB3D_HIP_FORMULA_INTPOW && p->slot_kind[i] != MB3D_HIP_FORMULA_AMAZINGBOX)
+            return "formula not on this backend";
+        if (p->slot_uncounted[i] && p->slot_iters[i] != 0) return "uncounted hybrid slots (not on this backend)";
+    }
+    if (!(p->de_option == 0 || p->de_option == 11 || p->de_option == 32))
+        return "this DE option (not on this backend)";
+    if (p->sm_normals != 0) return "smooth normals (not on this backend yet)";
+    if (p->color_option != 0) return "colour option other than 0 (not on this backend yet)";
+    if (!p->normals_on_de) return "normals on the smooth iteration (not on this backend yet)";
+    return "";
+}
+
+// needs_double (src/gpu.rs:569-576, 1019-1040): the FP32 iterate cannot
+// resolve the numeric gradient offset or the pixel step
+bool needs_f64(const mb3d_hip_params &p) {
+    const double coord = std::max({std::fabs(p.ystart[0]), std::fabs(p.ystart[1]), std::fabs(p.ystart[2])}) +
+                         p.step_width * (p.width + p.height) + 2.0;
+    const double eps = 1.1920928955078125e-07;
+    const double precision = (double)p.de_offset / (coord * eps);
+    const double step_precision = p.step_width / (coord * eps);
+    const bool numeric = !p.is_custom_de;
+    return (numeric && precision < 3.0) || step_precision < 64.0;
+}
+
+void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, int rows, DevP &d) {
+    std::memset(&d, 0, sizeof d);
+    d.width = p.width;
+    d.height = p.height;
+    d.row0 = row0;
+    d.rows = rows;
+    d.n_slots = p.n_slots;
+    d.end_to = p.end_to;
+    d.repeat_from = p.repeat_from;
+    d.max_it = p.max_it;
+    d.min_it = p.min_it;
+    d.de_option = p.de_option;
+    d.is_custom_de = p.is_custom_de;
+    d.de_add_steps = p.de_add_steps;
+    d.dfog_on_it = p.dfog_on_it;
+    d.first_step_random = p.first_step_random;
+    d.step_cap = p.step_cap;
+    for (int i = 0; i < MB3D_HIP_MAX_SLOTS; i++) {
+        d.slot_kind[i] = p.slot_kind[i];
+        d.slot_iters[i] = std::max(p.slot_iters[i], 0);
+        d.slot_ade[i] = p.slot_ade[i];
+        d.slot_z_mul[i] = p.slot_z_mul[i];
+        d.slot_scale[i] = p.slot_scale[i];
+        d.slot_min_r[i] = p.slot_min_r[i];
+        d.slot_fold[i] = p.slot_fold[i];
+        d.fz_mul[i] = (float)p.slot_z_mul[i];
+        d.fscale[i] = (float)p.slot_scale[i];
+        const double m = std::fmax(p.slot_min_r[i], 1e-40);
+        d.fmin_r2[i] = (float)(m * m);
+        d.fscale_min_r2[i] = (float)(p.slot_scale[i] / (m * m));
+        d.ffold[i] = (float)p.slot_fold[i];
+        d.fhln[i] = p.fhln[i];
+    }
+    d.d_de_scale = p.d_de_scale;
+    d.de_stop = p.de_stop;
+    d.z_step_div = p.z_step_div;
+    d.mh04zsd = p.mh04zsd;
+    d.de_stop_factor = p.de_stop_factor;
+    d.ln_rstop = p.ln_rstop;
+    d.de_offset = p.de_offset;
+    d.de_offset006 = p.de_offset006;
+    d.mcts_m = p.mcts_m;
+    d.ms_de_sub = p.ms_de_sub;
+    d.d_col_plus = p.d_col_plus;
+    d.col_var = p.col_var_de_stop_mul;
+    d.fovx_off = p.fovx_off;
+    d.fovx_mul = p.fovx_mul;
+    d.guard = 1.0f + (o.guard_band > 0.0f ? o.guard_band : 0.25f);
+    d.f_d_rstop = (float)p.d_rstop;
+    d.f_rstop3d = (float)p.rstop3d;
+    d.d_rstop = p.d_rstop;
+    d.rstop3d = p.rstop3d;
+    d.fov_y = p.fov_y;
+    d.zcorr = p.zcorr;
+    d.zc_mul = p.zc_mul;
+    d.zend = p.zend;
+    d.step_width = p.step_width;
+    std::memcpy(d.vg, p.vgrads, sizeof d.vg);
+    std::memcpy(d.ystart, p.ystart, sizeof d.ystart);
+    d.tile_w = o.tile_w > 0 ? o.tile_w : 8;
+    d.tile_h = o.tile_h > 0 ? o.tile_h : 8;
+    d.tiles_x = (p.width + d.tile_w - 1) / d.tile_w;
+    const long long tiles_y = (rows + d.tile_h - 1) / d.tile_h;
+    d.npix = (long long)p.width * rows;
+    d.nslots_q = (long long)d.tiles_x * tiles_y * d.tile_w * d.tile_h;
+}
+
+// build_rays (src/deao.rs:75-105); quality 4 is the 8-ray set (extension):
+// the quality-1 geometry with 7 rays on its ring
+void build_rays(int quality, float max_len, AoRays &r) {
+    std::memset(&r, 0, sizeof r);
+    const double TAU = 6.283185307179586;
+    auto push = [&](double ya, double za) {
+        const double sy = std::sin(ya), cy = std::cos(ya), sz = std::sin(za), cz = std::cos(za);
+        r.dir[r.count][0] = (float)(sy * cz);
+        r.dir[r.count][1] = (float)(-sy * sz);
+        r.dir[r.count][2] = (float)(-cy);
+        r.count++;
+    };
+    const int q = std::min(std::max(quality, 0), 4);
+    r.max_len = max_len;
+    if (q == 0) {
+        const double abr = 60.0 * (3.141592653589793 / 180.0);
+        for (int x = 0; x < 3; x++) push(0.5 * abr, x * TAU / 3.0);
+        r.d_step_mul = 1.8f;
+        r.d_min_a_dif = -1.0f;
+        r.correction_weight = 0.3f;
+        return;
+    }
+    const int qg = q == 4 ? 1 : q;
+    push(0.0, 0.0);
+    const double abr = 1.5707963267948966 / ((double)qg + 0.9);
+    for (int y = 1; y <= qg; y++) {
+        const double dt1 = y * abr;
+        int n = (int)std::nearbyint(std::sin(dt1) * TAU / abr);
+        if (q == 4) n = 7;
+        for (int x = 0; x < n; x++) push(dt1, x * TAU / n);
+    }
+    r.d_step_mul = (float)(1.0 + std::sin(abr));
+    r.d_min_a_dif = (float)std::cos(abr * 1.2);
+    r.correction_weight = qg == 1 ? 0.2f : 0.1666f;
+}
+
+int ensure(void **buf, size_t *have, size_t need) {
+    if (*have >= need) return 0;
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    size_t n = std::max(need, (size_t)1 << 20);
+    HIP_TRY(hipMalloc(buf, n));
+    *have = n;
+    return 0;
+}
+
+int kind_of(const mb3d_hip_params &p) {
+    bool p8 = false, box = false;
+    for (int i = 0; i < p.n_slots; i++) {
+        if (p.slot_iters[i] == 0) continue;
+        if (p.slot_kind[i] == MB3D_HIP_FORMULA_INTPOW) p8 = true;
+        if (p.slot_kind[i] == MB3D_HIP_FORMULA_AMAZINGBOX) box = true;
+    }
+    if (p8 && box) return K_HYB;
+    return box ? K_BOX : K_P8;
+}
+
+int dem_of(const mb3d_hip_params &p) {
+    if (p.de_option == 11) return DEM_BOX;
+    if (p.de_option == 32) return DEM_P8;
+    return DEM_NUMERIC;
+}
+
+template <int KIND, int DEM>
+void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b) {
+    hipLaunchKernelGGL((k_march32<KIND, DEM>), dim3(grid), dim3(256), 0, s, d, b);
+}
+template <int KIND, int DEM, bool F32>
+void launch_shadow(int grid, hipStream_t s, const DevP &d, const Bufs &b, const ShadowJob &j) {
+    hipLaunchKernelGGL((k_shadow<KIND, DEM, F32>), dim3(grid), dim3(256), 0, s, d, b, j);
+}
+template <int KIND, int DEM, bool F32>
+void launch_deao(int grid, hipStream_t s, const DevP &d, const Bufs &b, const AoRays &r) {
+    hipLaunchKernelGGL((k_deao<KIND, DEM, F32>), dim3(grid), dim3(256), 0, s, d, b, r);
+}
+
+// dispatch over the (formula kind, DE mode) pairs that exist
+template <class F>
+void dispatch(int kind, int dem, F &&f) {
+    if (kind == K_P8 && dem == DEM_NUMERIC) f(std::integral_constant<int, K_P8>(), std::integral_constant<int, DEM_NUMERIC>());
+    else if (kind == K_P8 && dem == DEM_P8) f(std::integral_constant<int, K_P8>(), std::integral_constant<int, DEM_P8>());
+    else if (kind == K_BOX && dem == DEM_BOX) f(std::integral_constant<int, K_BOX>(), std::integral_constant<int, DEM_BOX>());
+    else if (kind == K_BOX && dem == DEM_NUMERIC) f(std::integral_constant<int, K_BOX>(), std::integral_constant<int, DEM_NUMERIC>());
+    else f(std::integral_constant<int, K_HYB>(), std::integral_constant<int, DEM_NUMERIC>());
+}
+
+int init_locked(int device) {
+    Ctx &c = g_ctx;
+    if (c.device == device) return 0;
+    if (c.device >= 0) return set_error(MB3D_HIP_ERR_INVALID, "the library is bound to another device");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_error(MB3D_HIP_ERR_DEVICE, "no such HIP device");
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, device));
+    HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    for (auto &e : c.ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipMalloc((void **)&c.d_ctr, C_WORDS * sizeof(unsigned long long)));
+    c.device = device;
+    return 0;
+}
+
+struct OutLayout {
+    size_t depth, depth64, iters, normals, gbuf, total;
+};
+
+OutLayout layout_for(size_t npix) {
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    OutLayout l;
+    l.depth = 0;
+    l.depth64 = al(l.depth + npix * 4);
+    l.iters = al(l.depth64 + npix * 8);
+    l.normals = al(l.iters + npix * 2);
+    l.gbuf = al(l.normals + npix * 12);
+    l.total = al(l.gbuf + npix * 20);
+    return l;
+}
+
+int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, int row0, int rows,
+                 const mb3d_hip_out *out, const mb3d_hip_options *opt_in, mb3d_hip_progress_fn progress,
+                 mb3d_hip_cancel_fn cancel, void *user, uint32_t *post_gbuf, int post_on_device) {
+    Ctx &c = g_ctx;
+    if (c.device < 0) return set_error(MB3D_HIP_ERR_NOT_INIT, "mb3d_hip_init was not called");
+    if (!params) return set_error(MB3D_HIP_ERR_INVALID, "params is NULL");
+    const mb3d_hip_params &p = *params;
+    const std::string why = unsupported_reason(params);
+    if (!why.empty()) return set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
+    if (p.width <= 0 || p.height <= 0) return set_error(MB3D_HIP_ERR_INVALID, "bad picture size");
+    if (row0 < 0 || rows <= 0 || row0 + rows > p.height) return set_error(MB3D_HIP_ERR_INVALID, "rows out of the picture");
+    if (post && (post->n_lights < 0 || post->n_lights > MB3D_HIP_MAX_LIGHTS))
+        return set_error(MB3D_HIP_ERR_INVALID, "bad light count");
+    mb3d_hip_options o;
+    std::memset(&o, 0, sizeof o);
+    if (opt_in) o = *opt_in;
+    if (o.tile_w < 0 || o.tile_h < 0 || o.tile_w * o.tile_h > 4096)
+        return set_error(MB3D_HIP_ERR_INVALID, "bad tile shape");
+    if (cancel && cancel(user)) return set_error(MB3D_HIP_CANCELLED, "cancelled");
+    const bool f64 = o.precision == MB3D_HIP_PREC_F64 || (o.precision == MB3D_HIP_PREC_AUTO && needs_f64(p));
+    DevP d;
+    fill_devp(p, o, row0, rows, d);
+    const size_t npix = (size_t)d.npix;
+    const bool post_only = post_gbuf != nullptr;
+    const bool want_shadow = post && post->n_lights > 0;
+    const bool want_ao = post && post->deao;
+
+    // buffers: outputs on the device (the caller's or ours), hand-off list, hit list
+    const OutLayout L = layout_for(npix);
+    const bool dev_out = post_only ? post_on_device != 0 : (out && out->on_device);
+    if (!dev_out) {
+        if (int e = ensure(&c.d_out, &c.out_bytes, L.total)) return e;
+    }
+    const size_t work_need = npix * sizeof(Cand) + npix * sizeof(int32_t) + 512;
+    if (int e = ensure(&c.d_work, &c.work_bytes, work_need)) return e;
+    Bufs b;
+    std::memset(&b, 0, sizeof b);
+    char *ob = (char *)c.d_out;
+    if (post_only) {
+        b.gbuf = dev_out ? post_gbuf : (uint32_t *)(ob + L.gbuf);
+    } else if (dev_out) {
+        b.depth = out->depth;
+        b.depth64 = out->depth64;
+        b.iters = out->iters;
+        b.normals = out->normals;
+        b.gbuf = out->gbuf;
+        if (!b.gbuf && (want_shadow || want_ao)) {
+            if (int e = ensure(&c.d_out, &c.out_bytes, L.total)) return e;
+            b.gbuf = (uint32_t *)((char *)c.d_out + L.gbuf);
+        }
+    } else {
+        b.depth = out && out->depth ? (float *)(ob + L.depth) : nullptr;
+        b.depth64 = out && out->depth64 ? (double *)(ob + L.depth64) : nullptr;
+        b.iters = out && out->iters ? (uint16_t *)(ob + L.iters) : nullptr;
+        b.normals = out && out->normals ? (float *)(ob + L.normals) : nullptr;
+        b.gbuf = (out && out->gbuf) || want_shadow || want_ao ? (uint32_t *)(ob + L.gbuf) : nullptr;
+    }
+    b.cand = (Cand *)c.d_work;
+    b.hits = (int32_t *)((char *)c.d_work + npix * sizeof(Cand));
+    b.ctr = c.d_ctr;
+
+    HIP_TRY(hipMemsetAsync(c.d_ctr, 0, C_WORDS * sizeof(unsigned long long), c.stream));
+    if (post_only && !dev_out)
+        HIP_TRY(hipMemcpyAsync(b.gbuf, post_gbuf, npix * 20, hipMemcpyHostToDevice, c.stream));
+    const int kind = kind_of(p), dem = dem_of(p);
+    hipStream_t stream = c.stream;
+    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 8;
+    const int grid_march = std::max(1, c.cus * wpc / 4);
+    const int grid_flat = std::max(1, c.cus * 4);
+    HIP_TRY(hipEventRecord(c.ev[0], c.stream));
+    if (!post_only) {
+        if (f64) {
+            hipLaunchKernelGGL(k_finish64, dim3(grid_flat * 2), dim3(256), 0, c.stream, d, b, 1);
+        } else {
+            dispatch(kind, dem, [&](auto K, auto D) { launch_march<decltype(K)::value, decltype(D)::value>(grid_march, stream, d, b); });
+            hipLaunchKernelGGL(k_finish64, dim3(grid_flat), dim3(256), 0, c.stream, d, b, 0);
+        }
+        HIP_TRY(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(k_collect_hits, dim3(grid_flat), dim3(256), 0, c.stream, d, b);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(c.ev[1], c.stream));
+    if (want_shadow || want_ao) {
+        HIP_TRY(hipStreamSynchronize(c.stream));
+        if (progress) progress(user, 1, 2);
+        if (cancel && cancel(user)) return set_error(MB3D_HIP_CANCELLED, "cancelled");
+    }
+    if (want_shadow) {
+        ShadowJob j;
+        std::memset(&j, 0, sizeof j);
+        j.n = post->n_lights;
+        for (int l = 0; l < j.n; l++) {
+            if (post->light_idx[l] < 0 || post->light_idx[l] > 5) return set_error(MB3D_HIP_ERR_INVALID, "bad light index");
+            j.bit[l] = 0x400 << post->light_idx[l];
+            for (int k = 0; k < 3; k++) j.vec[l][k] = post->light_vec[l][k];
+        }
+        j.max_len_mul = post->shadow_max_len_mul;
+        dispatch(kind, dem, [&](auto K, auto D) {
+            if (f64) launch_shadow<decltype(K)::value, decltype(D)::value, false>(grid_flat, stream, d, b, j);
+            else launch_shadow<decltype(K)::value, decltype(D)::value, true>(grid_flat, stream, d, b, j);
+        });
+        HIP_TRY(hipGetLastError());
+    }
+    if (want_ao) {
+        if (post->deao_first_step_random) return set_error(MB3D_HIP_UNSUPPORTED, "DE AO with a random first step (not on this backend)");
+        AoRays r;
+        build_rays(post->deao_quality, post->deao_max_len, r);
+        DevP d5 = d;
+        d5.de_add_steps = 5;  // DeaoRows::new (src/deao.rs:254-260)
+        dispatch(kind, dem, [&](auto K, auto D) {
+            if (f64) launch_deao<decltype(K)::value, decltype(D)::value, false>(grid_flat, stream, d5, b, r);
+            else launch_deao<decltype(K)::value, decltype(D)::value, true>(grid_flat, stream, d5, b, r);
+        });
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(c.ev[2], c.stream));
+    // results to the caller
+    if (!dev_out) {
+        if (post_only) {
+            HIP_TRY(hipMemcpyAsync(post_gbuf, b.gbuf, npix * 20, hipMemcpyDeviceToHost, c.stream));
+        } else if (out) {
+            if (out->depth) HIP_TRY(hipMemcpyAsync(out->depth, b.depth, npix * 4, hipMemcpyDeviceToHost, c.stream));
+            if (out->depth64) HIP_TRY(hipMemcpyAsync(out->depth64, b.depth64, npix * 8, hipMemcpyDeviceToHost, c.stream));
+            if (out->iters) HIP_TRY(hipMemcpyAsync(out->iters, b.iters, npix * 2, hipMemcpyDeviceToHost, c.stream));
+            if (out->normals) HIP_TRY(hipMemcpyAsync(out->normals, b.normals, npix * 12, hipMemcpyDeviceToHost, c.stream));
+            if (out->gbuf) HIP_TRY(hipMemcpyAsync(out->gbuf, b.gbuf, npix * 20, hipMemcpyDeviceToHost, c.stream));
+        }
+    }
+    unsigned long long ctr[C_WORDS];
+    HIP_TRY(hipMemcpyAsync(ctr, c.d_ctr, sizeof ctr, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    float ms_march = 0, ms_post = 0;
+    HIP_TRY(hipEventElapsedTime(&ms_march, c.ev[0], c.ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ms_post, c.ev[1], c.ev[2]));
+    mb3d_hip_counters &t = t_counters;
+    std::memset(&t, 0, sizeof t);
+    t.pixels = npix;
+    t.steps = ctr[C_STATS + 0];
+    t.de_evals_f32 = ctr[C_STATS + 1];
+    t.de_evals_f64 = ctr[C_STATS + 2];
+    t.its_p8_f32 = ctr[C_STATS + 3];
+    t.its_box_f32 = ctr[C_STATS + 4];
+    t.its_p8_f64 = ctr[C_STATS + 5];
+    t.its_box_f64 = ctr[C_STATS + 6];
+    t.hits = ctr[C_STATS + 7];
+    t.overflow = ctr[C_STATS + 8];
+    t.kernel_ms = ms_march;
+    t.post_ms = ms_post;
+    if (progress) progress(user, 2, 2);
+    // the overflow verdict (src/gpu.rs:2350-2375): the f64 finish already
+    // recomputed those pixels, so it is reported, not failed
+    return MB3D_HIP_OK;
+}
+
+}  // namespace mb3d
+
+using namespace mb3d;
+
+extern "C" const char *mb3d_hip_last_error(void) { return t_error.c_str(); }
+
+extern "C" int32_t mb3d_hip_abi_version(void) { return MB3D_HIP_ABI_VERSION; }
+
+extern "C" int32_t mb3d_hip_init(int32_t device) {
+    try {
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        return init_locked(device);
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_init");
+    }
+}
+
+extern "C" void mb3d_hip_shutdown(void) {
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    Ctx &c = g_ctx;
+    if (c.device < 0) return;
+    (void)hipStreamSynchronize(c.stream);
+    if (c.d_out) (void)hipFree(c.d_out);
+    if (c.d_work) (void)hipFree(c.d_work);
+    if (c.d_ctr) (void)hipFree(c.d_ctr);
+    for (auto &e : c.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.d_out = c.d_work = nullptr;
+    c.d_ctr = nullptr;
+    c.out_bytes = c.work_bytes = 0;
+    c.stream = nullptr;
+    for (auto &e : c.ev) e = nullptr;
+    c.device = -1;
+}
+
+extern "C" int32_t mb3d_hip_march(const mb3d_hip_params *params, const mb3d_hip_post_job *post, int32_t row0,
+                                  int32_t rows, const mb3d_hip_out *out, const mb3d_hip_options *options,
+                                  mb3d_hip_progress_fn progress, mb3d_hip_cancel_fn cancel, void *user) {
+    try {
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        return march_locked(params, post, row0, rows, out, options, progress, cancel, user, nullptr, 0);
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_march");
+    }
+}
+
+extern "C" int32_t mb3d_hip_post(const mb3d_hip_params *params, const mb3d_hip_post_job *post, int32_t row0,
+                                 int32_t rows, uint32_t *gbuf, int32_t on_device, const mb3d_hip_options *options) {
+    try {
+        if (!gbuf || !post) return set_error(MB3D_HIP_ERR_INVALID, "NULL argument");
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        return march_locked(params, post, row0, rows, nullptr, options, nullptr, nullptr, nullptr, gbuf, on_device);
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_post");
+    }
+}
+
+extern "C" int32_t mb3d_hip_counters_get(mb3d_hip_counters *out) {
+    if (!out) return set_error(MB3D_HIP_ERR_INVALID, "NULL argument");
+    *out = t_counters;
+    return MB3D_HIP_OK;
+}

@@ -1,0 +1,301 @@
+// params.cpp -- host side of the HIP backend: scene defaults, the derived
+// march parameters (the reference's CalcParams::new for the built-in
+// formulas), and the backend's capability check.
+//
+//   mb3d_hip_scene_default      Scene::default          src/scene.rs:626-700
+//   mb3d_hip_params_from_scene  CalcParams::new         src/calc.rs:375-898
+//   mb3d_hip_supported          gpu::unsupported        src/gpu.rs:445-477
+//   mb3d_hip_pixel_seed         pixel_seed              src/calc.rs:1125-1140
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/mb3d_hip.h"
+#include "host_common.h"
+
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kRadPerDeg = kPi / 180.0;  // f64::to_radians
+
+// The built-in formulas' metadata (src/formulas.rs:492-560): DE option,
+// DE scale, default bailout, smooth-iteration power, first option.
+struct FormulaInfo {
+    int de_option;
+    double de_scale;
+    double default_rstop;
+    double si_pow;
+    double first_option;
+    bool first_option_double;
+};
+
+FormulaInfo info_of(const mb3d_hip_formula &f) {
+    if (f.kind == MB3D_HIP_FORMULA_AMAZINGBOX) return {11, 0.2, 1024.0, 2.0, f.scale, true};
+    double sp = (double)std::min(std::max(f.power, 2), 8);
+    return {0, 1.0, 16.0, sp, f.z_mul, false};
+}
+
+// combine_de_options (src/calc.rs:318-372) for DE options 0 and 11
+int combine_de(int acc, int de) {
+    if (acc == -1 || acc == 21 || acc == 22) return de;
+    if (acc == 11) {
+        if (de == 2 || de == 5) return 2;
+        return de != 6 ? 0 : 11;
+    }
+    return acc;
+}
+
+}  // namespace
+
+extern "C" void mb3d_hip_scene_default(mb3d_hip_scene *s) {
+    if (!s) return;
+    std::memset(s, 0, sizeof *s);
+    s->abi_version = MB3D_HIP_ABI_VERSION;
+    s->width = 800;
+    s->height = 600;
+    s->iterations = 60;
+    s->min_iterations = 1;
+    s->bin_search_steps = 8;
+    s->first_step_random = 1;
+    s->vary_de_stop_on_fov = 1;
+    s->normals_on_de = 1;
+    s->n_formulas = 1;
+    s->zoom = 1.0;
+    s->z_start = -2.0;
+    s->z_end = 30.0;
+    for (int i = 0; i < 3; i++) s->vgrads[i][i] = 1.0;
+    s->fov_y = 30.0;
+    s->de_stop = 1.0;
+    s->z_step_div = 0.5;
+    s->raystep_limiter = 1.0;
+    s->color_mul = 1.0;
+    s->formulas[0].kind = MB3D_HIP_FORMULA_INTPOW;
+    s->formulas[0].iterations = 1;
+    s->formulas[0].power = 8;
+    s->formulas[0].z_mul = -1.0;
+}
+
+extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_params *out) {
+    using mb3d::set_error;
+    if (!sc || !out) return set_error(MB3D_HIP_ERR_INVALID, "NULL argument");
+    if (sc->abi_version != MB3D_HIP_ABI_VERSION) return set_error(MB3D_HIP_ERR_INVALID, "scene ABI version mismatch");
+    const int nf = sc->n_formulas;
+    if (nf < 1 || nf > MB3D_HIP_MAX_SLOTS) return set_error(MB3D_HIP_ERR_INVALID, "a hybrid has 1..6 formulas");
+    if (sc->width <= 0 || sc->height <= 0) return set_error(MB3D_HIP_ERR_INVALID, "bad picture size");
+    bool any_its = false;
+    for (int i = 0; i < nf; i++) {
+        const int k = sc->formulas[i].kind;
+        if (k != MB3D_HIP_FORMULA_INTPOW && k != MB3D_HIP_FORMULA_AMAZINGBOX)
+            return set_error(MB3D_HIP_ERR_INVALID, "unknown formula kind");
+        any_its |= sc->formulas[i].iterations != 0;
+    }
+    if (!any_its) return set_error(MB3D_HIP_ERR_INVALID, "no formula with iterations > 0");
+
+    mb3d_hip_params p;
+    std::memset(&p, 0, sizeof p);
+    p.abi_version = MB3D_HIP_ABI_VERSION;
+    p.width = sc->width;
+    p.height = sc->height;
+    const double W = sc->width, Hh = sc->height;
+
+    // the hybrid ends at the last slot with iterations (CheckHybridOptions)
+    int last = nf - 1;
+    while (last > 0 && sc->formulas[last].iterations == 0) --last;
+    p.n_slots = last + 1;
+    FormulaInfo fi[MB3D_HIP_MAX_SLOTS];
+    for (int i = 0; i < p.n_slots; i++) {
+        const mb3d_hip_formula &f = sc->formulas[i];
+        fi[i] = info_of(f);
+        p.slot_kind[i] = f.kind;
+        p.slot_iters[i] = f.iterations < 0 ? -f.iterations : f.iterations;
+        p.slot_uncounted[i] = f.iterations < 0 || fi[i].de_option < 0 || fi[i].de_option > 20;
+        p.slot_power[i] = f.power;
+        p.slot_z_mul[i] = f.z_mul;
+        p.slot_scale[i] = f.scale;
+        p.slot_min_r[i] = f.min_r;
+        p.slot_fold[i] = f.fold;
+    }
+    p.end_to = last;
+    int rf = std::min(sc->repeat_from, last);
+    while (rf > 0 && p.slot_iters[rf] <= 0) --rf;
+    p.repeat_from = rf;
+
+    int de_option = -1;
+    for (int i = 0; i <= last; i++)
+        if (p.slot_iters[i] != 0 && fi[i].de_option != de_option) de_option = combine_de(de_option, fi[i].de_option);
+    if (sc->de_mode == MB3D_HIP_DE_ANALYTIC_P8) {
+        if (p.n_slots != 1 || sc->formulas[0].kind != MB3D_HIP_FORMULA_INTPOW)
+            return set_error(MB3D_HIP_ERR_INVALID, "analytic p8 DE needs one Integer Power slot");
+        de_option = 32;  // extension: running derivative
+    } else if (sc->de_mode != MB3D_HIP_DE_REFERENCE) {
+        return set_error(MB3D_HIP_ERR_INVALID, "unknown DE mode");
+    }
+    const bool custom = de_option == 2 || de_option == 5 || de_option == 6 || de_option == 11 || de_option == 20 ||
+                        de_option == 32;
+    p.de_option = de_option;
+    p.is_custom_de = custom;
+    for (int i = 0; i < p.n_slots; i++) p.slot_ade[i] = custom;
+
+    // DE scale, colour offset and smooth-iteration logs (src/calc.rs:534-600)
+    const int max_it = sc->iterations;
+    for (float &h : p.fhln) h = (float)(1.0 / std::log(2.0));
+    double cnt_all = 0, scale_sum = 0, abs_first = 0, cnt_ade = 0;
+    for (int i = 0; i < p.n_slots; i++) {
+        if (p.slot_iters[i] == 0) continue;
+        const double its = p.slot_iters[i];
+        if (!p.slot_uncounted[i]) {
+            const double z1 = fi[i].si_pow > 1.0 ? fi[i].si_pow : fi[i].first_option;
+            p.fhln[i] = (float)(1.0 / std::log(std::fmax(2.0, std::fabs(z1))));
+        }
+        const int j = fi[i].de_option;
+        if (custom && !p.slot_uncounted[i] && (j == 5 || j == 11)) {
+            cnt_all += its;
+            cnt_ade += its;
+            double x2 = fi[i].first_option;
+            if (x2 == 0.0) x2 = 1e-40;
+            scale_sum += x2 < 0.0 ? 0.65 * its : (x2 * 1.2 + 1.0) / x2 * its;
+            abs_first += std::fabs(x2) * its;
+        } else if (j >= 0) {
+            cnt_all += its;
+            scale_sum += (custom ? 1.0 : fi[i].de_scale) * its;
+        }
+    }
+    const double de_scale0 = (cnt_all > 0.0 && de_option != 20) ? scale_sum / cnt_all : 1.0;
+    float col_plus = 0.0f;
+    if (cnt_all > 0.0 && cnt_ade > 0.0) col_plus = (float)(std::pow(std::fabs(abs_first / cnt_ade), 0.25) * 40.0 - 49.0);
+
+    // mctColVarDEstopMul (src/calc.rs:600-670) for one hybrid part
+    float col_var = 0.6f;
+    {
+        double num = 0, den = 0;
+        for (int i = 0; i < p.n_slots; i++) {
+            if (!(p.slot_iters[i] > 0 && !p.slot_uncounted[i])) continue;
+            const double its = p.slot_iters[i];
+            const int de = fi[i].de_option;
+            if (de == 0 || de == 4) {
+                num += its * 0.6;
+                den += its;
+            } else if (((de >= 1 && de <= 3) || (de >= 5 && de <= 40)) && fi[i].first_option_double) {
+                double v = fi[i].first_option;
+                if (v > 1.0) {
+                    if (v < 1.2) v = 1.1 + (v - 1.0) * (v - 1.0) * 2.5;
+                    num += its * 0.5 / (std::log(v) + 0.03);
+                    den += its;
+                } else if (v < -1.0) {
+                    if (v > -1.2) v = -1.1 - (v + 1.0) * (v + 1.0) * 2.5;
+                    num += its * 0.5 / (std::log(-v) + 0.03);
+                    den += its;
+                }
+            }
+        }
+        if (den > 1e-3) col_var = (float)(num / den);
+    }
+
+    // bailout (Scene::effective_rstop, src/scene.rs:799-809)
+    double rstop = sc->rstop;
+    if (!(rstop > 0.0)) {
+        rstop = 0.0;
+        for (int i = 0; i < nf; i++)
+            if (sc->formulas[i].iterations > 0) rstop = std::fmax(rstop, info_of(sc->formulas[i]).default_rstop);
+        rstop = std::fmax(rstop, 1.0001);
+    }
+    const double sw = 2.1345 / (sc->zoom * W);  // Scene::step_width
+    const double fov = sc->fov_y * kRadPerDeg;
+    const float zsd0 = (float)std::fmax(sc->z_step_div, 0.0001);
+    const float de_stop = (float)std::fmax(sc->de_stop, 0.001);
+    const double half = std::fmin(std::fmax(fov * 0.5, 0.01), 1.5);
+    p.pl_optic_z = (float)(std::cos(half) * half / std::sin(half));
+    p.fovx_off = 0.5f * (float)sc->width;
+    p.fovx_mul = (float)(fov / Hh);
+
+    // CalcPPZvals: the depth code's constants
+    const double zcorr = std::sin(std::fmax(sc->fov_y * kRadPerDeg, 1.0) / Hh);
+    const double depth_range = sc->z_end - sc->z_start;
+    const double zc_mul = 32767.0 * 256.0 / (std::sqrt(depth_range * zcorr / sw + 1.0) - 0.999999999);
+    const double zend = std::fmax(1e-10, depth_range / sw);
+    const float mh04zsd = (float)((double)std::max(sc->width, sc->height) * 0.5 * std::sqrt((double)zsd0 + 0.001) *
+                                  std::fmax(0.01, sc->raystep_limiter));
+    // GetDEstopFactor
+    float dsf = 0.0f;
+    if (sc->vary_de_stop_on_fov) {
+        const double ze = std::fmax(1e-16, depth_range / sw);
+        const double a = 0.01 * Hh / (std::sin(0.01) * std::fmax(1.0 / 65535.0, sc->fov_y * kRadPerDeg));
+        const double b = sw * (a + ze) / a;
+        dsf = (float)((b - sw) / (sw * std::fmax(1e-6, ze)));
+    }
+    // camera: rows scaled to the step width, start of pixel (0, 0)
+    const double z1 = (sc->z_start - sc->mid[2]) / sw;
+    for (int i = 0; i < 3; i++) {
+        const double *r = sc->vgrads[i];
+        const double d = sw / std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        for (int j = 0; j < 3; j++) p.vgrads[i][j] = r[j] * d;
+    }
+    for (int k = 0; k < 3; k++)
+        p.ystart[k] = sc->mid[k] + z1 * p.vgrads[2][k] - (Hh * 0.5) * p.vgrads[1][k] - (W * 0.5) * p.vgrads[0][k];
+
+    float de_offset = std::fmin(de_stop * 0.1f, 0.004f);
+    p.d_de_scale = custom ? (float)((double)(float)de_scale0 / sw) : (float)de_scale0 * de_offset;
+    p.de_offset006 = de_offset * 0.06f;
+    p.de_offset = (float)((double)de_offset * sw);
+
+    p.z_step_div = zsd0;
+    p.ms_de_sub = 0.0f;
+    if (sc->step_sub_de_stop) {
+        float s = zsd0;
+        s = s * s + (1.2f * s) * (1.0f - s);
+        p.z_step_div = s;
+        p.ms_de_sub = std::fmin(std::sqrt(s), 0.9f);
+    }
+    p.ln_rstop = (float)std::log(std::log(rstop));
+    p.d_rstop = (double)(float)(rstop * rstop);
+    p.rstop3d = (double)(float)((p.d_rstop * p.d_rstop) * 64.0);
+    p.mcts_m = 32767.0f / (float)std::max(max_it + 1, 1);
+    p.d_col_plus = col_plus + (float)max_it * 0.1f;
+    p.col_var_de_stop_mul = col_var;
+
+    p.sm_normals = std::min(std::max(sc->smooth_normals, 0), 8);
+    p.de_stop = de_stop;
+    p.de_add_steps = sc->bin_search_steps;
+    p.fov_y = fov;
+    p.max_it = max_it;
+    p.min_it = std::min(sc->min_iterations, max_it);
+    p.color_option = std::min(sc->color_option, 5);
+    p.mct_color_mul = (float)(sc->color_mul * 512.0);
+    p.dfog_on_it = sc->dfog_on_it;
+    p.zcorr = zcorr;
+    p.zc_mul = zc_mul;
+    p.zend = zend;
+    p.step_width = sw;
+    p.first_step_random = sc->first_step_random != 0;
+    p.mh04zsd = mh04zsd;
+    p.de_stop_factor = dsf;
+    p.normals_on_de = sc->normals_on_de || custom;
+    p.step_cap = std::max(sc->step_cap, 0);
+    *out = p;
+    return MB3D_HIP_OK;
+}
+
+extern "C" int32_t mb3d_hip_supported(const mb3d_hip_params *p, char *reason, size_t reason_len) {
+    std::string why = mb3d::unsupported_reason(p);
+    if (why.empty()) return MB3D_HIP_OK;
+    if (reason && reason_len) std::snprintf(reason, reason_len, "%s", why.c_str());
+    mb3d::set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
+    return MB3D_HIP_UNSUPPORTED;
+}
+
+extern "C" int32_t mb3d_hip_pixel_seed(int64_t y, int64_t x) {
+    // the row seed advanced x steps of s -> 214013 s + 2531011, by squaring
+    uint32_t s = (uint32_t)(int32_t)(0x24563487LL + (y + 1) * 0x324594A1LL);
+    uint32_t mul = 214013u, add = 2531011u, acc_mul = 1u, acc_add = 0u;
+    for (uint64_t n = x > 0 ? (uint64_t)x : 0; n; n >>= 1) {
+        if (n & 1) {
+            acc_add = acc_add * mul + add;
+            acc_mul *= mul;
+        }
+        add = add * mul + add;
+        mul *= mul;
+    }
+    return (int32_t)(acc_mul * s + acc_add);
+}

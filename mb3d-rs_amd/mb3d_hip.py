@@ -1,0 +1,467 @@
+"""ctypes mirror of include/mb3d_hip.h and a thin Python front of the C ABI.
+
+The product is the C-ABI library ``libmb3d_hip.so`` built next to this file
+(HIP kernels for gfx950 plus the C++ host side).  This module only declares
+the structs, loads the library (it fails loudly when it is missing) and
+offers the scene builders of the measured workloads.
+
+Scene fields mirror the reference's ``Scene`` (src/scene.rs:398-532) and its
+defaults (src/scene.rs:626-700); camera matrices follow
+``vgrads_from_angles`` (src/math.rs:102-123).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmb3d_hip.so")
+
+ABI_VERSION = 1
+MAX_SLOTS = 6
+MAX_LIGHTS = 6
+
+OK, CANCELLED, UNSUPPORTED, OVERFLOW = 0, 1, 2, 3
+FORMULA_NONE, FORMULA_INTPOW, FORMULA_AMAZINGBOX = 0, 1, 2
+DE_REFERENCE, DE_ANALYTIC_P8 = 0, 1
+PREC_AUTO, PREC_F32, PREC_F64 = 0, 1, 2
+
+
+class Formula(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("iterations", C.c_int32),
+        ("power", C.c_int32),
+        ("reserved", C.c_int32),
+        ("z_mul", C.c_double),
+        ("scale", C.c_double),
+        ("min_r", C.c_double),
+        ("fold", C.c_double),
+    ]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("iterations", C.c_int32),
+        ("min_iterations", C.c_int32),
+        ("smooth_normals", C.c_int32),
+        ("bin_search_steps", C.c_int32),
+        ("first_step_random", C.c_int32),
+        ("step_sub_de_stop", C.c_int32),
+        ("vary_de_stop_on_fov", C.c_int32),
+        ("normals_on_de", C.c_int32),
+        ("color_option", C.c_int32),
+        ("dfog_on_it", C.c_int32),
+        ("repeat_from", C.c_int32),
+        ("n_formulas", C.c_int32),
+        ("de_mode", C.c_int32),
+        ("step_cap", C.c_int32),
+        ("reserved", C.c_int32),
+        ("rstop", C.c_double),
+        ("zoom", C.c_double),
+        ("mid", C.c_double * 3),
+        ("z_start", C.c_double),
+        ("z_end", C.c_double),
+        ("vgrads", (C.c_double * 3) * 3),
+        ("fov_y", C.c_double),
+        ("de_stop", C.c_double),
+        ("z_step_div", C.c_double),
+        ("raystep_limiter", C.c_double),
+        ("color_mul", C.c_double),
+        ("formulas", Formula * MAX_SLOTS),
+    ]
+
+
+S6i = C.c_int32 * MAX_SLOTS
+S6d = C.c_double * MAX_SLOTS
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("n_slots", C.c_int32),
+        ("slot_kind", S6i),
+        ("slot_iters", S6i),
+        ("slot_uncounted", S6i),
+        ("slot_ade", S6i),
+        ("slot_power", S6i),
+        ("de_option", C.c_int32),
+        ("is_custom_de", C.c_int32),
+        ("end_to", C.c_int32),
+        ("repeat_from", C.c_int32),
+        ("max_it", C.c_int32),
+        ("min_it", C.c_int32),
+        ("sm_normals", C.c_int32),
+        ("de_add_steps", C.c_int32),
+        ("first_step_random", C.c_int32),
+        ("normals_on_de", C.c_int32),
+        ("color_option", C.c_int32),
+        ("dfog_on_it", C.c_int32),
+        ("step_cap", C.c_int32),
+        ("reserved", C.c_int32),
+        ("slot_z_mul", S6d),
+        ("slot_scale", S6d),
+        ("slot_min_r", S6d),
+        ("slot_fold", S6d),
+        ("d_de_scale", C.c_float),
+        ("de_stop", C.c_float),
+        ("z_step_div", C.c_float),
+        ("mh04zsd", C.c_float),
+        ("de_stop_factor", C.c_float),
+        ("ln_rstop", C.c_float),
+        ("de_offset", C.c_float),
+        ("de_offset006", C.c_float),
+        ("mcts_m", C.c_float),
+        ("ms_de_sub", C.c_float),
+        ("d_col_plus", C.c_float),
+        ("col_var_de_stop_mul", C.c_float),
+        ("fovx_off", C.c_float),
+        ("fovx_mul", C.c_float),
+        ("pl_optic_z", C.c_float),
+        ("mct_color_mul", C.c_float),
+        ("fhln", C.c_float * MAX_SLOTS),
+        ("d_rstop", C.c_double),
+        ("rstop3d", C.c_double),
+        ("fov_y", C.c_double),
+        ("zcorr", C.c_double),
+        ("zc_mul", C.c_double),
+        ("zend", C.c_double),
+        ("step_width", C.c_double),
+        ("vgrads", (C.c_double * 3) * 3),
+        ("ystart", C.c_double * 3),
+    ]
+
+
+class PostJob(C.Structure):
+    _fields_ = [
+        ("n_lights", C.c_int32),
+        ("light_idx", C.c_int32 * MAX_LIGHTS),
+        ("light_vec", (C.c_double * 3) * MAX_LIGHTS),
+        ("shadow_max_len_mul", C.c_float),
+        ("deao", C.c_int32),
+        ("deao_quality", C.c_int32),
+        ("deao_max_len", C.c_float),
+        ("deao_first_step_random", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class Out(C.Structure):
+    _fields_ = [
+        ("depth", C.c_void_p),
+        ("depth64", C.c_void_p),
+        ("iters", C.c_void_p),
+        ("normals", C.c_void_p),
+        ("gbuf", C.c_void_p),
+        ("on_device", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32),
+        ("tile_w", C.c_int32),
+        ("tile_h", C.c_int32),
+        ("waves_per_cu", C.c_int32),
+        ("it_split", C.c_int32),
+        ("guard_band", C.c_float),
+        ("reserved", C.c_int32 * 4),
+    ]
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("pixels", C.c_uint64),
+        ("hits", C.c_uint64),
+        ("steps", C.c_uint64),
+        ("de_evals_f32", C.c_uint64),
+        ("de_evals_f64", C.c_uint64),
+        ("its_p8_f32", C.c_uint64),
+        ("its_box_f32", C.c_uint64),
+        ("its_p8_f64", C.c_uint64),
+        ("its_box_f64", C.c_uint64),
+        ("overflow", C.c_uint64),
+        ("kernel_ms", C.c_double),
+        ("post_ms", C.c_double),
+    ]
+
+
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int64)
+CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+# every symbol include/mb3d_hip.h declares, with its ctypes signature
+SIGNATURES = {
+    "mb3d_hip_last_error": (C.c_char_p, []),
+    "mb3d_hip_abi_version": (C.c_int32, []),
+    "mb3d_hip_init": (C.c_int32, [C.c_int32]),
+    "mb3d_hip_shutdown": (None, []),
+    "mb3d_hip_scene_default": (None, [C.POINTER(Scene)]),
+    "mb3d_hip_params_from_scene": (C.c_int32, [C.POINTER(Scene), C.POINTER(Params)]),
+    "mb3d_hip_supported": (C.c_int32, [C.POINTER(Params), C.c_char_p, C.c_size_t]),
+    "mb3d_hip_pixel_seed": (C.c_int32, [C.c_int64, C.c_int64]),
+    "mb3d_hip_march": (
+        C.c_int32,
+        [C.POINTER(Params), C.POINTER(PostJob), C.c_int32, C.c_int32, C.POINTER(Out),
+         C.POINTER(Options), PROGRESS_FN, CANCEL_FN, C.c_void_p],
+    ),
+    "mb3d_hip_post": (
+        C.c_int32,
+        [C.POINTER(Params), C.POINTER(PostJob), C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+         C.POINTER(Options)],
+    ),
+    "mb3d_hip_counters_get": (C.c_int32, [C.POINTER(Counters)]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """The product library; raises when it was not built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libmb3d_hip.so is not built ({path}): run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    e = load().mb3d_hip_last_error()
+    return e.decode() if e else ""
+
+
+def check(status: int, what: str) -> int:
+    if status < 0 or status == UNSUPPORTED:
+        raise RuntimeError(f"{what} failed ({status}): {last_error()}")
+    return status
+
+
+# ---------------------------------------------------------------------------
+# scene helpers (harness side)
+
+
+def vgrads_from_angles(ax: float, ay: float, az: float) -> list:
+    """vgrads_from_angles (src/math.rs:102-123): rotate around X, then Y, then Z."""
+    s1, c1 = math.sin(ax), math.cos(ax)
+    s2, c2 = math.sin(ay), math.cos(ay)
+    s3, c3 = math.sin(az), math.cos(az)
+    m = []
+    for i in range(3):
+        x1, y1, z1 = (1.0, 0.0, 0.0) if i == 0 else (0.0, 1.0, 0.0) if i == 1 else (0.0, 0.0, 1.0)
+        x2 = x1
+        y2 = c1 * y1 + s1 * z1
+        z2 = c1 * z1 - s1 * y1
+        x1 = c2 * x2 + s2 * z2
+        y1 = y2
+        z1 = c2 * z2 - s2 * x2
+        m.append([c3 * x1 + s3 * y1, c3 * y1 - s3 * x1, z1])
+    return m
+
+
+def scene_default() -> Scene:
+    """Scene::default (src/scene.rs:626-700) with one Integer Power p8 slot."""
+    s = Scene()
+    s.abi_version = ABI_VERSION
+    s.width, s.height = 800, 600
+    s.iterations = 60
+    s.min_iterations = 1
+    s.smooth_normals = 0
+    s.bin_search_steps = 8
+    s.first_step_random = 1
+    s.step_sub_de_stop = 0
+    s.vary_de_stop_on_fov = 1
+    s.normals_on_de = 1
+    s.color_option = 0
+    s.dfog_on_it = 0
+    s.repeat_from = 0
+    s.n_formulas = 1
+    s.de_mode = DE_REFERENCE
+    s.step_cap = 0
+    s.rstop = 0.0
+    s.zoom = 1.0
+    s.mid[:] = [0.0, 0.0, 0.0]
+    s.z_start, s.z_end = -2.0, 30.0
+    for i in range(3):
+        for j in range(3):
+            s.vgrads[i][j] = 1.0 if i == j else 0.0
+    s.fov_y = 30.0
+    s.de_stop = 1.0
+    s.z_step_div = 0.5
+    s.raystep_limiter = 1.0
+    s.color_mul = 1.0
+    set_formulas(s, [intpow(8, 1)])
+    return s
+
+
+def intpow(power=8, iterations=1, z_mul=-1.0) -> Formula:
+    f = Formula()
+    f.kind, f.iterations, f.power, f.z_mul = FORMULA_INTPOW, iterations, power, z_mul
+    return f
+
+
+def amazing_box(scale=2.0, min_r=0.5, fold=1.0, iterations=1) -> Formula:
+    f = Formula()
+    f.kind, f.iterations, f.scale, f.min_r, f.fold = FORMULA_AMAZINGBOX, iterations, scale, min_r, fold
+    return f
+
+
+def set_formulas(s: Scene, fs: list) -> None:
+    for i in range(MAX_SLOTS):
+        s.formulas[i] = fs[i] if i < len(fs) else Formula()
+    s.n_formulas = len(fs)
+
+
+def set_camera(s: Scene, vg) -> None:
+    for i in range(3):
+        for j in range(3):
+            s.vgrads[i][j] = vg[i][j]
+
+
+def look_matrix(direction) -> list:
+    """Camera rows (screen x, screen y, view) for a view direction."""
+    d = np.asarray(direction, float)
+    d = d / np.linalg.norm(d)
+    up = np.array([0.0, 1.0, 0.0]) if abs(d[1]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    x = np.cross(up, d)
+    x /= np.linalg.norm(x)
+    y = np.cross(d, x)
+    return [list(x), list(y), list(d)]
+
+
+def scene_bulb(width, height, seed=1, de_mode=DE_ANALYTIC_P8, step_cap=500, bin_search=6) -> Scene:
+    """The power-8 Mandelbulb workload: 60 iterations, bailout r^2 = 16,
+    the camera's start plane 2.5 from the origin facing it (the reference's
+    Integer Power preset framing, zoom 0.8, src/scene.rs:811-838), FOV 30,
+    seeded yaw in [0, 2 pi) and pitch in [-0.5, 0.5]; no random first step;
+    6 bisection steps; 500-step cap."""
+    rng = np.random.default_rng(seed)
+    yaw = float(rng.uniform(0.0, 2.0 * math.pi))
+    pitch = float(rng.uniform(-0.5, 0.5))
+    s = scene_default()
+    s.width, s.height = width, height
+    s.iterations = 60
+    s.rstop = 4.0
+    s.zoom = 0.8
+    s.z_start, s.z_end = -2.5, 30.0
+    s.fov_y = 30.0
+    s.first_step_random = 0
+    s.bin_search_steps = bin_search
+    s.step_cap = step_cap
+    s.de_mode = de_mode
+    set_camera(s, vgrads_from_angles(pitch, yaw, 0.0))
+    set_formulas(s, [intpow(8, 1)])
+    return s
+
+
+def _box_de(p, scale, min_r, fold, its):
+    """Analytic Amazing Box DE |z|/|dr| at a point (harness: camera placement)."""
+    z = np.array(p, float)
+    c = z.copy()
+    w = 1.0
+    m2 = max(min_r, 1e-40) ** 2
+    for _ in range(its):
+        z = np.abs(z + fold) - (np.abs(z - fold) + z)
+        r = float(z @ z)
+        mul = scale / m2 if r < m2 else (scale if r > 1.0 else scale / r)
+        w *= mul
+        z = z * mul + c
+        if float(z @ z) > 1024.0 ** 2:
+            break
+    return math.sqrt(float(z @ z)) / abs(w)
+
+
+def scene_box(width, height, seed=1, step_cap=0) -> Scene:
+    """The Amazing Box workload: scale -1.5, min R 0.5, fold 1, 30
+    iterations, analytic DE; the camera's start plane centred at a seeded
+    point inside the radius-4 sphere (re-drawn until it lies outside the set,
+    its DE above 0.05) with a seeded view direction towards the inner half."""
+    rng = np.random.default_rng(seed)
+    for _ in range(1000):
+        p = rng.uniform(-4.0, 4.0, 3)
+        if p @ p > 16.0:
+            continue
+        if _box_de(p, -1.5, 0.5, 1.0, 30) < 0.05:
+            continue
+        break
+    target = rng.uniform(-1.0, 1.0, 3)
+    d = target - p
+    s = scene_default()
+    s.width, s.height = width, height
+    s.iterations = 30
+    s.zoom = 1.0
+    s.mid[:] = list(p)
+    s.z_start = float(p[2])
+    s.z_end = s.z_start + 12.0
+    s.first_step_random = 0
+    s.step_cap = step_cap
+    set_camera(s, look_matrix(d))
+    set_formulas(s, [amazing_box(-1.5, 0.5, 1.0, 1)])
+    return s
+
+
+def scene_hybrid(width, height, seed=1) -> Scene:
+    """The hybrid chain: Mandelbulb p8 and Amazing Box (scale 2) alternating
+    1:2, 45 iterations, numeric DE (the reference's combined option 0)."""
+    s = scene_bulb(width, height, seed=seed, de_mode=DE_REFERENCE, step_cap=500, bin_search=8)
+    s.iterations = 45
+    s.rstop = 0.0  # the formulas' default (1024 for the box)
+    s.zoom = 0.5
+    s.z_start = -4.0
+    set_formulas(s, [intpow(8, 1), amazing_box(2.0, 0.5, 1.0, 2)])
+    return s
+
+
+def post_job(lights=(), deao_quality=None, shadow_max_len_mul=1.0, params: Params = None) -> PostJob:
+    """A post job with directional lights given as view-space directions
+    towards the light (HSvecs = RotateVectorReverse(-StepWidth * LN, M),
+    src/render.rs:2383-2417)."""
+    j = PostJob()
+    j.n_lights = len(lights)
+    if lights:
+        assert params is not None
+        vg = np.array([[params.vgrads[i][k] for k in range(3)] for i in range(3)])
+        m = vg / np.linalg.norm(vg, axis=1, keepdims=True)
+        for i, ln in enumerate(lights):
+            ln = np.asarray(ln, float)
+            v = -ln / np.linalg.norm(ln) * params.step_width
+            w = v @ m  # rotate_vector_reverse: sum_i v[i] * M[i]
+            j.light_idx[i] = i
+            for k in range(3):
+                j.light_vec[i][k] = float(w[k])
+    j.shadow_max_len_mul = shadow_max_len_mul
+    if deao_quality is not None:
+        j.deao = 1
+        j.deao_quality = deao_quality
+        j.deao_max_len = 1.0
+    return j
+
+
+def params_from_scene(s: Scene) -> Params:
+    p = Params()
+    check(load().mb3d_hip_params_from_scene(C.byref(s), C.byref(p)), "params_from_scene")
+    return p
+
+
+def params_dict(p: Params) -> dict:
+    out = {}
+    for name, _ in Params._fields_:
+        v = getattr(p, name)
+        if hasattr(v, "__len__"):
+            v = [list(x) if hasattr(x, "__len__") else x for x in v]
+        out[name] = v
+    return out

@@ -1,0 +1,1387 @@
+// mb3d_oracle.cpp -- TEST INFRASTRUCTURE (the f64 CPU oracle), not product code.
+//
+// A restatement of mb3d-rs's CPU ray marcher for the built-in formulas
+// 'Integer Power' and 'Amazing Box' and their alternating hybrids, in the
+// reference's operation order and with its f32 roundings kept (so that the
+// results are those of the reference's own CPU path):
+//   parameter set-up   CalcParams::new          src/calc.rs:375-898
+//   iteration          Iteration::run_with ...  src/iteration.rs:140-154,268-315,521-585
+//   formulas           int_pow, amazing_box     src/formulas.rs:707-736,769-858
+//   distance estimate  calc_de_part             src/calc.rs:1495-1559
+//   march              march_pixel_from, march_from  src/calc.rs:2231-2322,2533-2657
+//   bisection          bin_search, bin_search_it     src/calc.rs:1706-1792
+//   normals            calculate_normals (smooth 0)  src/calc.rs:1794-1858
+//   surface            surface_found, calc_zpos_and_rough  src/calc.rs:2194-2205,2471-2517
+//   hard shadows       hard_shadow_row, per pixel (no row shortcut)  src/calc.rs:2741-2899
+//   DE AO              deao_row, surface_point   src/deao.rs:268-418, src/calc.rs:2157-2192
+// Built with -ffp-contract=off: no fused multiply-adds, like the reference.
+#include "mb3d_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char *msg) {
+    g_err = msg;
+    return MB3D_HIP_ERR_INVALID;
+}
+
+typedef double Vec3[3];
+typedef double Mat3[3][3];
+
+// math.rs helpers (src/math.rs:26-133)
+inline void add_weight(double a[3], const double b[3], double w) {
+    a[0] += b[0] * w;
+    a[1] += b[1] * w;
+    a[2] += b[2] * w;
+}
+inline double dot3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// rotate_vector_reverse (src/math.rs:63-69): V' = V * M
+inline void rot_rev(const double v[3], const double m[3][3], double r[3]) {
+    r[0] = v[0] * m[0][0] + v[1] * m[1][0] + v[2] * m[2][0];
+    r[1] = v[0] * m[0][1] + v[1] * m[1][1] + v[2] * m[2][1];
+    r[2] = v[0] * m[0][2] + v[1] * m[1][2] + v[2] * m[2][2];
+}
+
+// normalise_matrix_to (src/math.rs:78-87)
+void normalise_matrix_to(double n, const double m[3][3], double r[3][3]) {
+    for (int i = 0; i < 3; i++) {
+        double d = n / std::sqrt(dot3(m[i], m[i]));
+        for (int j = 0; j < 3; j++) r[i][j] = m[i][j] * d;
+    }
+}
+
+// build_view_vector_dfov (src/math.rs:127-133)
+void build_view_vector_dfov(double xa, double ya, double v[3]) {
+    double sx = std::sin(xa), cx = std::cos(xa);
+    double sy = std::sin(ya), cy = std::cos(ya);
+    double w[3] = {-sy, sx, cx * cy};
+    double n = 1.0 / std::sqrt(dot3(w, w));
+    v[0] = w[0] * n;
+    v[1] = w[1] * n;
+    v[2] = w[2] * n;
+}
+
+// min_max_clip_15bit (src/math.rs:246-255)
+uint16_t clip15(float s) {
+    if (std::isnan(s) || s < 0.0f) return 0;
+    if (s > 32767.0f) return 32767;
+    return (uint16_t)std::nearbyint(s);  // round half to even (default rounding mode)
+}
+
+inline double round_even(double x) { return std::nearbyint(x); }
+
+// ---------------------------------------------------------------------------
+// formulas (src/formulas.rs)
+
+// box_fold (src/formulas.rs:707-710)
+inline double box_fold(double x, double fold) { return std::fabs(x + fold) - (std::fabs(x - fold) + x); }
+
+// amazing_box (src/formulas.rs:713-736)
+inline void amazing_box(double scale, double min_r, double fold, double v[4], const double j[4], bool ade) {
+    double x = box_fold(v[0], fold);
+    double y = box_fold(v[1], fold);
+    double z = box_fold(v[2], fold);
+    double r = z * z + y * y + x * x;
+    double m = std::fmax(min_r, 1e-40);
+    double sqr_min_r = m * m;
+    double mul;
+    if (r < sqr_min_r)
+        mul = scale / sqr_min_r;
+    else if (1.0 < r)
+        mul = scale;
+    else
+        mul = scale / r;
+    if (ade) v[3] *= mul;
+    v[2] = z * mul + j[2];
+    v[1] = y * mul + j[1];
+    v[0] = x * mul + j[0];
+}
+
+// int_pow (src/formulas.rs:769-858)
+void int_pow(int power, double zmul, double v[4], const double j[4]) {
+    double x = v[0], y = v[1], z = v[2];
+    const double EPS = 1e-40;
+    if (power <= 2) {
+        double yy = y * y, xx = x * x, r = xx + yy;
+        v[2] = (std::sqrt(r) * zmul * z) * 2.0 + j[2];
+        double a = (r - z * z) / r;
+        v[0] = (xx - yy) * a + j[0];
+        v[1] = (x * y * a) * 2.0 + j[1];
+    } else if (power == 3) {
+        double sy = y * y, sx = x * x, r = sx + sy, sz = z * z;
+        double a = 1.0 - 3.0 * sz / (r + EPS);
+        v[0] = (sx - 3.0 * sy) * a * x + j[0];
+        v[2] = j[2] - (sz - 3.0 * r) * z * zmul;
+        v[1] = a * (3.0 * sx - sy) * y + j[1];
+    } else if (power == 4) {
+        double sy = y * y, sx = x * x, r = sx + sy, sz = z * z;
+        double a = 1.0 + sz * (sz - 6.0 * r) / (r * r + EPS);
+        v[0] = (sx * (sx - 6.0 * sy) + sy * sy) * a + j[0];
+        v[2] = std::sqrt(r) * (r - sz) * z * 4.0 * zmul + j[2];
+        v[1] = a * x * (sx - sy) * 4.0 * y + j[1];
+    } else if (power == 5) {
+        double sy = y * y, sx = x * x, r = sx + sy, sz = z * z;
+        double a = 1.0 + (sz * sz - sz * r * 2.0) * 5.0 / (r * r + EPS);
+        v[1] = (5.0 * sx * sx - sy * (10.0 * sx - sy)) * a * y + j[1];
+        v[2] = (sz * (sz - 10.0 * r) + 5.0 * r * r) * z * zmul + j[2];
+        v[0] = a * x * (sx * (sx - 10.0 * sy) + 5.0 * sy * sy) + j[0];
+    } else if (power == 6) {
+        double sy = y * y, sx = x * x, r = sx + sy, sz = z * z, rr = r * r;
+        double a = 1.0 - sz * (15.0 * rr + sz * (sz - r * 15.0)) / (rr * r + EPS);
+        v[1] = (3.0 * sy * sy + sx * (3.0 * sx - 10.0 * sy)) * a * y * x * 2.0 + j[1];
+        v[2] = (sz * (3.0 * sz - 10.0 * r) + 3.0 * rr) * std::sqrt(r) * z * zmul * 2.0 + j[2];
+        v[0] = (sy * sy * (15.0 * sx - sy) + sx * sx * (sx - 15.0 * sy)) * a + j[0];
+    } else if (power == 7) {
+        double sy = y * y, sx = x * x, r = sx + sy, sz = z * z, rr = r * r;
+        double a = 1.0 - 7.0 * (sz * (3.0 * rr + sz * (sz - 5.0 * r))) / (rr * r + EPS);
+        v[1] = (sx * (21.0 * sy * sy + sx * (7.0 * sx - 35.0 * sy)) - sy * sy * sy) * a * y + j[1];
+        v[2] = j[2] - (sz * sz * sz - 7.0 * r * (rr + sz * (3.0 * sz - 5.0 * r))) * z * zmul;
+        v[0] = a * x * (sx * (35.0 * sy * sy + sx * (sx - 21.0 * sy)) - 7.0 * sy * sy * sy) + j[0];
+    } else {
+        // HybridIntP8 (White's / Nylander's power 8)
+        double xx = x * x, yy = y * y, zz = z * z;
+        double r = xx + yy;
+        double rr = r * r;
+        double zzzz = zz * zz;
+        double t = (zzzz - 6.0 * r * zz + rr) * (zz - r) * std::sqrt(r);
+        v[2] = -(t * z * 8.0 * zmul) + j[2];
+        double num = (rr * 70.0 + zzzz) * zzzz - 28.0 * zz * r * (zzzz + rr);
+        double a = num / (rr * rr + EPS) + 1.0;
+        double xxxx = xx * xx, yyyy = yy * yy;
+        v[1] = (yyyy * (7.0 * xx - yy) + xxxx * (xx - 7.0 * yy)) * 8.0 * x * y * a + j[1];
+        v[0] = a * (xxxx * xxxx + yyyy * (70.0 * xxxx + yyyy) - 28.0 * (xx * yy * (yyyy + xxxx))) + j[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the iteration record and the hybrid loops (src/iteration.rs)
+
+struct Iter {
+    double v[4] = {0, 0, 0, 0};
+    double c[3] = {0, 0, 0};
+    double j[4] = {0, 0, 0, 0};
+    double rout = 0, rold = 0, otrap = 0;
+    int it_result = 0, max_it = 0;
+    double rstop = 0;
+    bool calc_sit = false;
+    float smooth_it = 0;
+    float ln_rstop = 0;
+    int de_option = 0;
+    double deriv1 = 1.0;
+    int start_from = 0, end_to = 0, repeat_from = 0;
+    float fhln[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t its_p8 = 0, its_box = 0;
+};
+
+struct Marcher {
+    const mb3d_hip_params &p;
+    Iter it;
+    float ms_de_stop;
+    double mzz = 0;
+    double vfov[3] = {0, 0, 1};
+    double cafy = 0;
+    int max_its_result;
+    int32_t seed = 0;
+    uint64_t de_evals = 0;
+
+    explicit Marcher(const mb3d_hip_params &pp) : p(pp) {
+        // CalcParams::new_iteration (src/calc.rs:1054-1087)
+        it.max_it = p.max_it;
+        it.rstop = p.d_rstop;
+        it.ln_rstop = p.ln_rstop;
+        it.de_option = p.de_option;
+        it.deriv1 = 1.0;
+        it.end_to = p.end_to;
+        it.repeat_from = p.repeat_from;
+        for (int k = 0; k < 6; k++) it.fhln[k] = p.fhln[k];
+        ms_de_stop = p.de_stop;
+        max_its_result = p.max_it;
+    }
+
+    // Formula::iterate through call_slot (src/iteration.rs:325-345,
+    // src/formulas.rs:576-641); the analytic power-8 extension keeps the
+    // running derivative dr <- p r^(p-1) dr + 1 in deriv1.
+    void call_slot(int n) {
+        int kind = p.slot_kind[n];
+        if (kind == MB3D_HIP_FORMULA_INTPOW) {
+            if (p.slot_ade[n]) {
+                double r2 = it.v[0] * it.v[0] + it.v[1] * it.v[1] + it.v[2] * it.v[2];
+                int pw = p.slot_power[n];
+                double rp = pw == 8 ? r2 * r2 * r2 * std::sqrt(r2) : std::pow(r2, 0.5 * (pw - 1));
+                it.deriv1 = (double)pw * rp * it.deriv1 + 1.0;
+            }
+            int_pow(p.slot_power[n], p.slot_z_mul[n], it.v, it.j);
+            it.its_p8++;
+        } else if (kind == MB3D_HIP_FORMULA_AMAZINGBOX) {
+            amazing_box(p.slot_scale[n], p.slot_min_r[n], p.slot_fold[n], it.v, it.j, p.slot_ade[n] != 0);
+            it.its_box++;
+        }
+    }
+
+    // Iteration::calc_smooth_iterations (src/iteration.rs:140-154)
+    void calc_smooth_iterations(int n) {
+        if (it.rout <= 1.0) {
+            it.smooth_it = (float)it.it_result;
+        } else if (it.rold < 1.0) {
+            double d = std::log(std::log(it.rout) * 0.5) * (double)it.fhln[n];
+            it.smooth_it = (float)((double)it.it_result + (double)it.ln_rstop - d);
+        } else {
+            double d = std::log(0.5 * std::log(it.rout));
+            double e = d - std::log(0.5 * std::log(it.rold));
+            it.smooth_it = (float)(((double)it.ln_rstop - d) / (e + 1e-100) + (double)it.it_result);
+        }
+    }
+
+    // Iteration::run_with (src/iteration.rs:268-315), 3D
+    int run() {
+        int n = it.start_from;
+        int btmp = std::max(p.slot_iters[n], 0);
+        it.it_result = 0;
+        uint32_t uncounted_guard = 0;
+        for (;;) {
+            it.rold = it.rout;
+            int guard = 0;
+            while (btmp <= 0) {
+                n += 1;
+                if (n > it.end_to) n = it.repeat_from;
+                btmp = std::max(p.slot_iters[n], 0);
+                guard += 1;
+                if (guard > 64) return n;
+            }
+            call_slot(n);
+            btmp -= 1;
+            if (p.slot_uncounted[n]) {
+                uncounted_guard += 1;
+                if (uncounted_guard > 100000) return n;
+                continue;
+            }
+            it.it_result += 1;
+            double x = it.v[0], y = it.v[1], z = it.v[2];
+            it.rout = x * x + y * y + z * z;
+            if (it.rout < it.otrap) it.otrap = it.rout;
+            // NaN compares false: keeps iterating
+            if (it.it_result >= it.max_it || it.rout > it.rstop) return n;
+        }
+    }
+
+    // Iteration::hybrid_3d (src/iteration.rs:521-537)
+    void hybrid_3d() {
+        const double *c = it.c;
+        it.j[0] = c[0];
+        it.j[1] = c[1];
+        it.j[2] = c[2];
+        it.v[0] = c[0];
+        it.v[1] = c[1];
+        it.v[2] = c[2];
+        it.v[3] = 0.0;
+        it.rout = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+        it.otrap = it.rout;
+        int n = run();
+        if (it.calc_sit) calc_smooth_iterations(n);
+    }
+
+    // Iteration::hybrid_3d_de (src/iteration.rs:539-585); de_option 32 is
+    // the analytic power-8 extension with the textbook 0.5 r ln r / dr.
+    double hybrid_3d_de() {
+        const double *c = it.c;
+        it.j[0] = c[0];
+        it.j[1] = c[1];
+        it.j[2] = c[2];
+        it.v[0] = c[0];
+        it.v[1] = c[1];
+        it.v[2] = c[2];
+        it.rout = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+        it.otrap = it.rout;
+        switch (it.de_option & 0x38) {
+            case 16: it.v[3] = it.rout; break;
+            case 32: it.deriv1 = 1.0; break;
+            default: it.v[3] = 1.0; break;
+        }
+        int n = run();
+        double w = it.v[3];
+        double result;
+        if ((it.de_option & 0x38) == 32) {
+            result = std::sqrt(it.rout) * 0.25 * std::log(it.rout) / it.deriv1;
+        } else if ((it.de_option & 7) == 4) {
+            result = std::fabs(it.v[1]) * std::log(std::fabs(it.v[1])) / w;
+        } else {
+            result = std::sqrt(it.rout) / std::fabs(w);
+        }
+        if (it.calc_sit) calc_smooth_iterations(n);
+        return result;
+    }
+
+    // Marcher::calc_de_part for one part without dIFS (src/calc.rs:1495-1559)
+    double calc_de() {
+        de_evals++;
+        double result;
+        if (p.is_custom_de) {
+            result = hybrid_3d_de() * (double)p.d_de_scale;
+        } else {
+            hybrid_3d();
+            if (it.rout < 1e-200) {
+                result = 0.0;
+            } else {
+                bool buf_sit = it.calc_sit;
+                int buf_max_it = it.max_it;
+                double buf_rout = it.rout;
+                it.calc_sit = false;
+                it.max_it = it.it_result;
+                it.rstop = p.rstop3d;
+                double off = (double)p.de_offset;
+                double g[3];
+                for (int k = 0; k < 3; k++) {
+                    double bufd = it.c[k];
+                    it.c[k] = bufd + off;
+                    hybrid_3d();
+                    g[k] = (buf_rout - it.rout) * (buf_rout - it.rout);
+                    it.c[k] = bufd;
+                }
+                result = buf_rout * std::log(buf_rout) * (double)p.d_de_scale /
+                         (std::sqrt(g[0] + g[1] + g[2]) + (double)p.de_offset006);
+                it.rout = buf_rout;
+                it.it_result = it.max_it;
+                it.max_it = buf_max_it;
+                it.calc_sit = buf_sit;
+                it.rstop = p.d_rstop;
+            }
+        }
+        max_its_result = it.max_it;
+        double mn = (double)ms_de_stop * 0.25;
+        if (result < mn || std::isnan(result)) result = mn;
+        return result;
+    }
+
+    // calc_vgrads_fov (src/calc.rs:1561-1575), Common optic
+    void calc_vgrads_fov(int ix) {
+        double cafx = (double)((p.fovx_off - (float)ix) * p.fovx_mul);
+        double v[3];
+        build_view_vector_dfov(cafy, cafx, v);
+        rot_rev(v, p.vgrads, vfov);
+    }
+
+    // calc_start_pos (src/calc.rs:1673-1690)
+    void calc_start_pos(int ix, int iy) {
+        double x = ix, y = iy;
+        for (int k = 0; k < 3; k++) it.c[k] = p.ystart[k] + p.vgrads[0][k] * x + p.vgrads[1][k] * y;
+    }
+
+    void setup_ray(int x, int y) {
+        cafy = ((double)y / (double)p.height - 0.5) * p.fov_y;
+        calc_vgrads_fov(x + 1);
+        calc_start_pos(x, y);
+    }
+
+    inline void step(double d) { add_weight(it.c, vfov, d); }
+
+    // update_de_stop (src/calc.rs:1698-1703)
+    inline void update_de_stop() {
+        ms_de_stop = (float)((double)p.de_stop * (1.0 + mzz * (double)p.de_stop_factor));
+    }
+
+    // bin_search (src/calc.rs:1706-1735), without the experimental secant
+    void bin_search(double &de, double last_step_width) {
+        int itmp = p.de_add_steps;
+        double dt1 = last_step_width * -0.5;
+        while (std::fabs(de - (double)ms_de_stop) > 0.001) {
+            mzz += dt1;
+            step(dt1);
+            update_de_stop();
+            itmp -= 1;
+            if (itmp <= 0) break;
+            de = calc_de();
+            if (it.it_result >= max_its_result) {
+                dt1 = -std::fabs(dt1);
+                continue;
+            }
+            double f = de - (double)ms_de_stop;
+            dt1 = f < 0.0 ? std::fabs(dt1) * -0.55 : std::fabs(dt1) * 0.55;
+        }
+    }
+
+    // bin_search_it (src/calc.rs:1737-1792)
+    void bin_search_it() {
+        float yp = (float)it.max_it - 0.99f;
+        int saved_max = it.max_it;
+        it.max_it += 1;
+        int itmp = p.de_add_steps;
+        it.calc_sit = true;
+        double dt1 = 0.0, dmul = 1.0;
+        bool first = true;
+        float last_si = 0.0f, last_dif = 0.0f;
+        for (;;) {
+            mzz += dt1;
+            step(dt1);
+            calc_de();
+            if (!first && last_dif < std::fabs(yp - it.smooth_it)) {
+                mzz -= dt1;
+                step(-dt1);
+                it.smooth_it = last_si;
+                if (dt1 > 0.0)
+                    dmul *= 0.5;
+                else
+                    dmul *= 0.7;
+            }
+            last_dif = std::fabs(yp - it.smooth_it);
+            last_si = it.smooth_it;
+            if (it.smooth_it > (float)it.max_it - 0.1f) {
+                dt1 = -3.0;
+            } else {
+                mzz -= 0.001;
+                step(-0.001);
+                calc_de();
+                float r = last_si - it.smooth_it;
+                if (std::fabs(r) < 1e-30f)
+                    dt1 = (last_si < it.smooth_it ? 1.0 : 0.0) - 0.5;
+                else if (r < 0.0f)
+                    dt1 = (double)((yp - it.smooth_it) / (r * 500.0f));
+                else
+                    dt1 = (double)((yp - it.smooth_it) / (r * 1000.0f));
+                if (dt1 > 4.0)
+                    dt1 = std::sqrt(dt1) * 2.0;
+                else if (dt1 < -9.0)
+                    dt1 = std::sqrt(-dt1) * -3.0;
+                dt1 = dt1 * dmul + 0.0005;
+            }
+            first = false;
+            itmp -= 1;
+            if (itmp < 0) break;
+        }
+        it.max_it = saved_max;
+    }
+
+    // calculate_normals with sm_normals = 0 (src/calc.rs:1813-1858); the
+    // raw gradient goes to n, the return value is NN (smooth iterations)
+    float calculate_normals(double n[3]) {
+        const auto &vg = p.vgrads;
+        it.calc_sit = true;
+        double noffset = (double)std::fmin(p.de_stop, 1.0f) * (1.0 + mzz * (double)p.de_stop_factor) * 0.15;
+        double ct1[3] = {it.c[0], it.c[1], it.c[2]};
+        calc_de();
+        float nn = it.smooth_it;
+        it.calc_sit = false;
+        const int order[3][2] = {{2, 2}, {0, 0}, {1, 1}};
+        for (auto &ka : order) {
+            int k = ka[0], axis = ka[1];
+            std::memcpy(it.c, ct1, sizeof ct1);
+            add_weight(it.c, vg[axis], noffset);
+            double a = calc_de();
+            add_weight(it.c, vg[axis], -2.0 * noffset);
+            n[k] = (a - calc_de()) * 0.5;
+        }
+        std::memcpy(it.c, ct1, sizeof ct1);
+        return nn;
+    }
+
+    // store_normal (src/calc.rs:1794-1804)
+    static void store_normal(int16_t out[3], const double n[3]) {
+        double d = 32767.0 / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] + 1e-100);
+        for (int k = 0; k < 3; k++) {
+            double v = round_even(n[k] * d);
+            out[k] = std::isnan(v) ? 0 : (int16_t)std::min(std::max(v, -32767.0), 32767.0);
+        }
+        if (out[0] == 0 && out[1] == 0) out[2] = out[2] > 0 ? 32767 : -32767;
+    }
+
+    // calc_zpos_and_rough (src/calc.rs:2194-2205), sm_normals = 0
+    uint32_t zpos_fine() const {
+        double zz = std::max(mzz, 0.0);
+        double vr = round_even(p.zc_mul * (std::sqrt(zz * p.zcorr + 1.0) - 1.0));
+        int64_t v = (int64_t)vr;
+        int64_t itmp = std::min<int64_t>(std::max<int64_t>(8388352 - v, 0), 8388352);
+        return (uint32_t)itmp << 8;
+    }
+};
+
+// The G-buffer record (SiLight, src/gbuffer.rs:4-40) with the raw values
+struct Px {
+    int16_t normal[3] = {0, 0, 0};
+    uint32_t zpos_fine = 32768u << 16;
+    uint16_t shadow = 0, amb_shadow = 0, si_gradient = 0, otrap = 0;
+    double depth = INFINITY;
+    double n[3] = {0, 0, 0};
+    uint16_t iters = 0;
+    int32_t steps = 0;
+};
+
+void to_words(const Px &s, uint32_t w[5]) {
+    w[0] = (uint32_t)(uint16_t)s.normal[0] | ((uint32_t)(uint16_t)s.normal[1] << 16);
+    w[1] = (uint32_t)(uint16_t)s.normal[2] | ((uint32_t)s.shadow << 16);
+    w[2] = s.zpos_fine;
+    w[3] = (uint32_t)s.si_gradient | ((uint32_t)s.otrap << 16);
+    w[4] = (uint32_t)s.amb_shadow;
+}
+
+void from_words(const uint32_t w[5], Px &s) {
+    s.normal[0] = (int16_t)(uint16_t)(w[0] & 0xFFFF);
+    s.normal[1] = (int16_t)(uint16_t)(w[0] >> 16);
+    s.normal[2] = (int16_t)(uint16_t)(w[1] & 0xFFFF);
+    s.shadow = (uint16_t)(w[1] >> 16);
+    s.zpos_fine = w[2];
+    s.si_gradient = (uint16_t)(w[3] & 0xFFFF);
+    s.otrap = (uint16_t)(w[3] >> 16);
+    s.amb_shadow = (uint16_t)(w[4] & 0xFFFF);
+}
+
+// do_color with color_option 0 (src/calc.rs:2068-2090)
+void do_color(const Marcher &m, Px &si) { si.otrap = clip15((float)(m.it.otrap * 4096.0)); }
+
+// surface_found (src/calc.rs:2471-2517)
+void surface_found(Marcher &m, Px &si, double dtmp, double last_step) {
+    const mb3d_hip_params &p = m.p;
+    bool de_limited = m.it.it_result < m.max_its_result || dtmp < (double)m.ms_de_stop;
+    if (p.de_add_steps != 0) {
+        if (de_limited)
+            m.bin_search(dtmp, last_step);
+        else
+            m.bin_search_it();
+    }
+    si.iters = (uint16_t)std::max(m.it.it_result, 0);
+    double n[3];
+    float nn = m.calculate_normals(n);
+    Marcher::store_normal(si.normal, n);
+    double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int k = 0; k < 3; k++) si.n[k] = l > 0 ? n[k] / l : 0.0;
+    float v;
+    if (de_limited) {
+        float lg = (float)std::log((double)std::fmax(p.de_stop, m.ms_de_stop) * p.step_width);
+        v = 32767.0f - (nn + p.d_col_plus + p.col_var_de_stop_mul * lg) * p.mcts_m;
+    } else {
+        v = 32767.0f - nn * p.mcts_m;
+    }
+    si.si_gradient = clip15(v);
+    do_color(m, si);
+    si.zpos_fine = m.zpos_fine();
+    si.depth = m.mzz;
+}
+
+// march_from (src/calc.rs:2533-2657) without volumetric light and relax;
+// the step cap is the declared extension (0: the reference's guard only)
+void march_from(Marcher &m, Px &si, double dtmp, bool first_step, float step_count) {
+    const mb3d_hip_params &p = m.p;
+    float rsf_mul = 1.0f;
+    double last_step = dtmp * (double)p.z_step_div;
+    double last_de;
+    uint32_t guard = 0;
+    int32_t nsteps = 0;
+    for (;;) {
+        guard += 1;
+        if (guard > 2000000) break;
+        if (m.it.it_result >= m.max_its_result) {
+            double dt1 = -0.5 * last_step;
+            m.mzz += dt1;
+            m.step(dt1);
+            m.update_de_stop();
+            dtmp = m.calc_de();
+            last_step = -dt1;
+        }
+        if (m.it.it_result < p.min_it || (m.it.it_result < m.max_its_result && dtmp >= (double)m.ms_de_stop)) {
+            if (p.step_cap > 0 && nsteps >= p.step_cap) break;  // extension: background
+            last_de = dtmp;
+            double d = std::fmax(0.11, (dtmp - (double)p.ms_de_sub * (double)m.ms_de_stop) * (double)p.z_step_div *
+                                           (double)rsf_mul);
+            double s1 = (double)(std::fmax(m.ms_de_stop, 0.4f) * p.mh04zsd);
+            bool count = p.dfog_on_it == 0 || m.it.it_result == p.dfog_on_it;
+            if (s1 < d) {
+                if (count) step_count += (float)(s1 / d);
+                d = s1;
+            } else if (count) {
+                step_count += 1.0f;
+            }
+            if (first_step) {
+                first_step = false;
+                m.seed = (int32_t)((uint32_t)m.seed * 214013u + 2531011u);
+                d *= (double)(m.seed & 0x7FFFFFFF) * (1.0 / (double)0x7FFFFFFF);
+            }
+            m.mzz += d;
+            if (m.mzz > p.zend) break;  // background
+            nsteps++;
+            last_step = d;
+            m.step(d);
+            m.update_de_stop();
+            dtmp = m.calc_de();
+            if (dtmp > last_de + last_step) dtmp = last_de + last_step;
+            if (last_de > dtmp + 1e-30) {
+                double t = last_step / (last_de - dtmp);
+                rsf_mul = t < 1.0 ? (float)std::fmax(0.5, t) : 1.0f;
+            } else {
+                rsf_mul = 1.0f;
+            }
+        } else {
+            surface_found(m, si, dtmp, last_step);
+            break;
+        }
+    }
+    float sc = std::fmin(std::fmax(step_count, 0.0f), 1023.0f);
+    si.shadow = (uint16_t)std::nearbyint(sc);
+    si.steps = nsteps;
+}
+
+// march_pixel / march_pixel_from without cone, cut planes, inside
+// rendering or ground (src/calc.rs:2207-2322)
+Px march_pixel(Marcher &m, int x, int y) {
+    const mb3d_hip_params &p = m.p;
+    Px si;
+    si.amb_shadow = 5000;
+    m.setup_ray(x, y);
+    m.it.calc_sit = false;
+    m.mzz = 0.0;
+    m.ms_de_stop = p.de_stop;
+    bool first_step = p.first_step_random != 0;
+    double dtmp = m.calc_de();
+    if (m.it.it_result >= m.max_its_result || dtmp < (double)m.ms_de_stop) {
+        // inside the set at the start plane: calc_nangles_for_cut(0)
+        si.zpos_fine = 0x7FFF0000u;
+        si.normal[0] = 0;
+        si.normal[1] = 0;
+        si.normal[2] = -32767;
+        si.n[2] = -1.0;
+        do_color(m, si);
+        double t = std::fmin(std::fmax(m.it.rout / p.d_rstop, 0.0), 1.0);
+        si.si_gradient = (uint16_t)(32768 + (int)round_even(32767.0 * t));
+        si.depth = 0.0;
+        si.iters = (uint16_t)std::max(m.it.it_result, 0);
+        return si;
+    }
+    march_from(m, si, dtmp, first_step, 0.0f);
+    return si;
+}
+
+// surface depth of a G-buffer pixel, the middle of its depth code's last
+// digit (src/calc.rs:2167, 2748-2750)
+inline double depth_of_code(const mb3d_hip_params &p, double zf, double frac) {
+    double a = (frac - zf) / p.zc_mul + 1.0;
+    return (a * a - 1.0) / p.zcorr;
+}
+
+// hs_surface_start (src/calc.rs:2741-2765)
+bool hs_surface_start(Marcher &m, int x, int y, const Px &si) {
+    const mb3d_hip_params &p = m.p;
+    if ((si.zpos_fine >> 16) >= 32768 || si.si_gradient >= 32768) return false;
+    m.setup_ray(x, y);
+    double zf = (double)(si.zpos_fine >> 8);
+    m.mzz = depth_of_code(p, zf, 8388351.5);
+    add_weight(m.it.c, m.vfov, m.mzz);
+    m.update_de_stop();
+    double d = m.calc_de();
+    bool de_limited = m.it.it_result < m.max_its_result || d < (double)m.ms_de_stop;
+    if (de_limited) {
+        double dt1 = depth_of_code(p, zf, 8388351.9) - m.mzz;
+        m.bin_search(d, dt1);
+    } else {
+        m.bin_search_it();
+    }
+    return true;
+}
+
+// dot_normalize (src/calc.rs:2734-2737)
+inline double dot_normalize(const double a[3], const double b[3]) {
+    return dot3(a, b) / std::fmax(std::sqrt(dot3(a, a) * dot3(b, b)), 1e-300);
+}
+
+// hard_shadow_row for one pixel, directional lights, without the row
+// neighbour shortcut (src/calc.rs:2777-2899; the card's variant)
+void hard_shadow_pixel(Marcher &m, int x, int y, Px &si, const mb3d_hip_post_job &job) {
+    const mb3d_hip_params &p = m.p;
+    uint16_t clear_mask = 0;
+    for (int l = 0; l < job.n_lights; l++) clear_mask |= (uint16_t)(0x400 << job.light_idx[l]);
+    si.shadow &= (uint16_t)~clear_mask;
+    double nv[3] = {si.normal[0] / 32767.0, si.normal[1] / 32767.0, si.normal[2] / 32767.0};
+    double nvec[3];
+    rot_rev(nv, p.vgrads, nvec);
+    if (!hs_surface_start(m, x, y, si)) return;
+    m.mzz = std::fmax(m.mzz - 0.1, 0.0);
+    add_weight(m.it.c, m.vfov, -0.1);
+    m.update_de_stop();
+    double ic[3] = {m.it.c[0], m.it.c[1], m.it.c[2]};
+    // hs_max_len (src/calc.rs:2767-2775)
+    float mz = (float)std::fmin(m.mzz, p.zend * 0.4);
+    double max_lhs = (double)((float)(p.width + y) * 0.6f *
+                              (1.0f + 0.5f * mz * (float)std::fmax(p.fov_y, 0.0) / (float)p.height) *
+                              job.shadow_max_len_mul);
+    for (int l = 0; l < job.n_lights; l++) {
+        int bit = 0x400 << job.light_idx[l];
+        std::memcpy(m.it.c, ic, sizeof ic);
+        double zz2 = m.mzz;
+        const double *lvec = job.light_vec[l];
+        double zz2mul = -dot_normalize(lvec, m.vfov);
+        if (dot3(nvec, lvec) > 0.0) {
+            si.shadow |= (uint16_t)bit;
+            continue;
+        }
+        double dt1 = max_lhs;
+        if (dt1 > 0.0) {
+            m.ms_de_stop = (float)((double)p.de_stop * (1.0 + std::fabs(zz2) * (double)p.de_stop_factor));
+            float rsf = 2.0f;
+            double d = m.calc_de();
+            int guard = 0;
+            for (;;) {
+                guard += 1;
+                if (guard > 1000000) break;
+                double last_de = d;
+                float a = std::fmax(0.11f, (float)((d - (double)p.ms_de_sub * (double)m.ms_de_stop) *
+                                                   (double)p.z_step_div * (double)rsf));
+                float b = std::fmax(m.ms_de_stop, 0.4f) * p.mh04zsd;
+                double st = (double)std::fmin(a, b);
+                dt1 -= st;
+                add_weight(m.it.c, lvec, -st);
+                zz2 += st * zz2mul;
+                m.ms_de_stop = (float)((double)p.de_stop * (1.0 + std::fabs(zz2) * (double)p.de_stop_factor));
+                d = m.calc_de();
+                if (m.it.it_result >= m.max_its_result || d <= (double)m.ms_de_stop) break;
+                if (d > last_de + st) d = last_de + st;
+                if (last_de > d + 1e-30) {
+                    float t = (float)(st / (last_de - d));
+                    rsf = t < 1.0f ? std::fmax(t, 0.5f) : 1.0f;
+                } else {
+                    rsf = 1.0f;
+                }
+                if (dt1 < 0.0) break;
+            }
+        }
+        if (dt1 > 0.0) si.shadow |= (uint16_t)bit;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// DE ambient occlusion (src/deao.rs)
+
+typedef float SVec[3];
+
+// ray_dir (src/deao.rs:35-40)
+void ray_dir(double ya, double za, float out[3]) {
+    double sy = std::sin(ya), cy = std::cos(ya);
+    double sz = std::sin(za), cz = std::cos(za);
+    out[0] = (float)(sy * cz);
+    out[1] = (float)(-sy * sz);
+    out[2] = (float)(-cy);
+}
+
+// normal_matrix (src/deao.rs:42-58)
+void normal_matrix(const float n[3], float m[3][3]) {
+    double a = std::acos(std::fmin(std::fmax(-(double)n[2], -1.0), 1.0)) * 0.5;
+    double sa = std::sin(a), ca = std::cos(a);
+    double nn = std::sqrt((double)(n[1] * n[1] + n[0] * n[0]));
+    double q[4];
+    if (nn < 1e-25) {
+        q[0] = q[1] = q[2] = 0.0;
+        q[3] = ca;
+    } else {
+        sa /= nn;
+        q[0] = -(double)n[1] * sa;
+        q[1] = (double)n[0] * sa;
+        q[2] = 0.0;
+        q[3] = ca;
+    }
+    double r[3][3] = {
+        {1.0 - 2.0 * (q[1] * q[1] + q[2] * q[2]), 2.0 * (q[0] * q[1] + q[2] * q[3]), 2.0 * (q[0] * q[2] - q[1] * q[3])},
+        {2.0 * (q[0] * q[1] - q[2] * q[3]), 1.0 - 2.0 * (q[0] * q[0] + q[2] * q[2]), 2.0 * (q[2] * q[1] + q[0] * q[3])},
+        {2.0 * (q[0] * q[2] + q[1] * q[3]), 2.0 * (q[1] * q[2] - q[0] * q[3]), 1.0 - 2.0 * (q[0] * q[0] + q[1] * q[1])},
+    };
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) m[i][j] = (float)r[i][j];
+}
+
+inline float sdot(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+struct Rays {
+    std::vector<std::array<float, 3>> dirs;
+    float d_step_mul, d_min_a_dif, correction_weight;
+};
+
+// build_rays (src/deao.rs:75-105); quality 4 is the 8-ray set (extension):
+// the quality-1 geometry with 7 rays on its ring
+Rays build_rays(int quality) {
+    Rays r;
+    const double TAU = 6.283185307179586;
+    int q = std::min(quality, 4);
+    if (q == 0) {
+        double abr = 60.0 * (3.141592653589793 / 180.0);
+        for (int x = 0; x < 3; x++) {
+            std::array<float, 3> d;
+            ray_dir(0.5 * abr, x * TAU / 3.0, d.data());
+            r.dirs.push_back(d);
+        }
+        r.d_step_mul = 1.8f;
+        r.d_min_a_dif = -1.0f;
+        r.correction_weight = 0.3f;
+        return r;
+    }
+    int qg = q == 4 ? 1 : q;
+    std::array<float, 3> d0;
+    ray_dir(0.0, 0.0, d0.data());
+    r.dirs.push_back(d0);
+    double abr = 1.5707963267948966 / ((double)qg + 0.9);
+    for (int y = 1; y <= qg; y++) {
+        double dt1 = y * abr;
+        int n = (int)std::nearbyint(std::sin(dt1) * TAU / abr);
+        if (q == 4) n = 7;
+        for (int x = 0; x < n; x++) {
+            std::array<float, 3> d;
+            ray_dir(dt1, x * TAU / n, d.data());
+            r.dirs.push_back(d);
+        }
+    }
+    r.d_step_mul = (float)(1.0 + std::sin(abr));
+    r.d_min_a_dif = (float)std::cos(abr * 1.2);
+    r.correction_weight = qg == 1 ? 0.2f : 0.1666f;
+    return r;
+}
+
+// surface_point (src/calc.rs:2157-2192)
+bool surface_point(Marcher &m, int x, int y, const Px &si, int steps, double thr) {
+    const mb3d_hip_params &p = m.p;
+    if ((si.zpos_fine >> 16) >= 32768 || si.si_gradient >= 32768) return false;
+    m.setup_ray(x, y);
+    m.it.calc_sit = false;
+    double zf = (double)(si.zpos_fine >> 8);
+    m.mzz = depth_of_code(p, zf, 8388351.5);
+    add_weight(m.it.c, m.vfov, m.mzz);
+    m.update_de_stop();
+    double d = std::fmin(m.calc_de(), (double)m.ms_de_stop * 2.0);
+    if (m.it.it_result < m.max_its_result || d < (double)m.ms_de_stop) {
+        double dt1 = depth_of_code(p, zf, 8388351.9) - m.mzz;
+        int itmp = steps;
+        while (itmp > 0 && std::fabs(d - (double)m.ms_de_stop) > thr) {
+            m.mzz += dt1;
+            add_weight(m.it.c, m.vfov, dt1);
+            m.update_de_stop();
+            itmp -= 1;
+            if (itmp > 0) {
+                d = m.calc_de();
+                dt1 = d < (double)m.ms_de_stop ? std::fabs(dt1) * -0.55 : std::fabs(dt1) * 0.55;
+            }
+        }
+    } else {
+        m.bin_search_it();
+        m.ms_de_stop = (float)((double)p.de_stop * (1.0 + std::fabs(m.mzz) * (double)p.de_stop_factor));
+    }
+    return true;
+}
+
+// deao_row for one pixel, no dither, no random first step
+// (src/deao.rs:268-418); the marcher's parameters have de_add_steps = 5
+void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_hip_post_job &job) {
+    const mb3d_hip_params &p = m.p;
+    double sw = p.step_width;
+    double rot_v[3][3];
+    normalise_matrix_to(sw, p.vgrads, rot_v);
+    double s_max_d = (double)job.deao_max_len * 0.5 *
+                     std::sqrt((double)p.height * (double)p.height + (double)p.width * (double)p.width);
+    double de_stop = (double)p.de_stop;
+    si.amb_shadow = 0;
+    if ((si.zpos_fine >> 16) >= 32768 || si.si_gradient >= 32768) return;
+    if (!surface_point(m, x, y, si, 5, 0.004)) return;
+    double ms_de_stop = std::fmin((double)m.ms_de_stop, 1e6);
+    double ic[3] = {m.it.c[0], m.it.c[1], m.it.c[2]};
+    double step_ao = ms_de_stop / de_stop;
+    double max_dist = s_max_d * std::sqrt(step_ao);
+    double dsm = (double)base.d_step_mul;
+    if (p.de_stop_factor == 0.0f)
+        ms_de_stop = de_stop / (dsm * dsm);
+    else
+        ms_de_stop = ms_de_stop / (dsm * dsm);
+    int ray_count = (int)base.dirs.size();
+    double de_mul = std::sqrt((double)ray_count * 0.5);
+    double abr = 1.2 / std::asin(std::fmin(std::fmax(1.0 / de_mul, -1.0), 1.0));
+    double md_d10 = 0.1 / (max_dist * de_mul);
+    float nvf[3] = {(float)si.normal[0], (float)si.normal[1], (float)si.normal[2]};
+    float l = std::fmax(std::sqrt(sdot(nvf, nvf)), 1e-30f);
+    float n[3] = {nvf[0] / l, nvf[1] / l, nvf[2] / l};
+    float rot_w[3][3];
+    normal_matrix(n, rot_w);
+    std::vector<float> min_ra(ray_count, 0.0f);
+    for (int k = 0; k < ray_count; k++) {
+        const float *d = base.dirs[k].data();
+        float v[3] = {sdot(rot_w[0], d), sdot(rot_w[1], d), sdot(rot_w[2], d)};
+        double vd[3] = {(double)v[0], (double)v[1], (double)v[2]};
+        double sv[3];
+        rot_rev(vd, rot_v, sv);
+        double dt1 = step_ao * dsm;
+        double s_tmp = 1.0;
+        bool b_end = false;
+        for (;;) {
+            if (dt1 > max_dist) {
+                dt1 = max_dist;
+                b_end = true;
+            }
+            std::memcpy(m.it.c, ic, sizeof ic);
+            add_weight(m.it.c, sv, dt1);
+            double dt2 = m.calc_de();
+            s_tmp = std::fmin(s_tmp, (dt2 - ms_de_stop + dt1 * md_d10) / dt1);
+            if (s_tmp < 0.02) break;
+            dt1 += std::fmax(dt2, dt1 * dsm);
+            if (b_end) break;
+        }
+        min_ra[k] = (float)(std::fmax(s_tmp, 0.0) * de_mul);
+    }
+    double amount = 0.0;
+    for (int k = 0; k < ray_count; k++) {
+        float s_add = 0.0f;
+        if (min_ra[k] < 1.0f) {
+            float max_add = 1.0f - min_ra[k];
+            for (int k2 = 0; k2 < ray_count; k2++) {
+                if (k2 == k) continue;
+                float dt = sdot(base.dirs[k].data(), base.dirs[k2].data());
+                if (dt > base.d_min_a_dif) {
+                    float ac = std::acos(std::fmin(std::fmax(dt, -1.0f), 1.0f));
+                    float overlap = min_ra[k2] - (float)((double)ac * abr) + 1.0f;
+                    if (overlap > 0.0f) s_add += std::fmin(max_add, overlap) * base.correction_weight;
+                }
+            }
+        }
+        amount += (double)std::fmin(s_add + min_ra[k], 1.0f);
+    }
+    double a = round_even(16383.0 * (1.0 - amount / (double)ray_count));
+    si.amb_shadow = (uint16_t)std::fmax(a, 0.0);
+}
+
+int nthreads(int t) {
+    if (t > 0) return t;
+    unsigned h = std::thread::hardware_concurrency();
+    return h ? (int)h : 1;
+}
+
+template <class F>
+void for_rows_interleaved(int rows, int threads, F f) {
+    threads = std::max(1, std::min(threads, rows));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++)
+        pool.emplace_back([=]() {
+            for (int r = t; r < rows; r += threads) f(r);
+        });
+    for (auto &th : pool) th.join();
+}
+
+int check_params(const mb3d_hip_params *p) {
+    if (!p) return fail("params is NULL");
+    if (p->abi_version != MB3D_HIP_ABI_VERSION) return fail("params ABI version mismatch");
+    if (p->width <= 0 || p->height <= 0) return fail("bad picture size");
+    if (p->n_slots < 1 || p->n_slots > MB3D_HIP_MAX_SLOTS) return fail("bad slot count");
+    if (p->sm_normals != 0) return fail("the oracle covers smooth_normals = 0 only");
+    if (p->color_option != 0) return fail("the oracle covers color_option 0 only");
+    return 0;
+}
+
+void run_post(const mb3d_hip_params &p, const mb3d_hip_post_job &job, int row0, int rows, std::vector<Px> &px,
+              int threads) {
+    int w = p.width;
+    if (job.n_lights > 0) {
+        for_rows_interleaved(rows, nthreads(threads), [&](int r) {
+            Marcher m(p);
+            int y = row0 + r;
+            for (int x = 0; x < w; x++) hard_shadow_pixel(m, x, y, px[(size_t)r * w + x], job);
+        });
+    }
+    if (job.deao) {
+        mb3d_hip_params pc = p;
+        pc.de_add_steps = 5;  // DeaoRows::new (src/deao.rs:254-260)
+        Rays base = build_rays(job.deao_quality);
+        for_rows_interleaved(rows, nthreads(threads), [&](int r) {
+            Marcher m(pc);
+            int y = row0 + r;
+            for (int x = 0; x < w; x++) deao_pixel(m, x, y, px[(size_t)r * w + x], base, job);
+        });
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// parameter set-up: CalcParams::new (src/calc.rs:375-898) for the built-in
+// formulas, one alternating 3D hybrid, no DE combination or interpolation
+
+namespace {
+
+int formula_de_option(const mb3d_hip_formula &f) {  // Formula::de_option (src/formulas.rs:492-499)
+    return f.kind == MB3D_HIP_FORMULA_AMAZINGBOX ? 11 : 0;
+}
+double formula_de_scale(const mb3d_hip_formula &f) {  // de_scale (src/formulas.rs:502-509)
+    return f.kind == MB3D_HIP_FORMULA_AMAZINGBOX ? 0.2 : 1.0;
+}
+double formula_default_rstop(const mb3d_hip_formula &f) {  // default_rstop (src/formulas.rs:520-528)
+    return f.kind == MB3D_HIP_FORMULA_AMAZINGBOX ? 1024.0 : 16.0;
+}
+double formula_si_pow(const mb3d_hip_formula &f) {  // si_pow (src/formulas.rs:531-545)
+    if (f.kind == MB3D_HIP_FORMULA_INTPOW) return (double)std::min(std::max(f.power, 2), 8);
+    return 2.0;
+}
+double formula_first_option(const mb3d_hip_formula &f) {  // first_option (src/formulas.rs:548-560)
+    return f.kind == MB3D_HIP_FORMULA_INTPOW ? f.z_mul : f.scale;
+}
+bool formula_first_option_double(const mb3d_hip_formula &f) {  // first_option_type
+    return f.kind != MB3D_HIP_FORMULA_INTPOW;
+}
+
+}  // namespace
+
+extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_params *out) {
+    if (!sc || !out) return fail("NULL argument");
+    if (sc->abi_version != MB3D_HIP_ABI_VERSION) return fail("scene ABI version mismatch");
+    int nf = sc->n_formulas;
+    if (nf < 1 || nf > MB3D_HIP_MAX_SLOTS) return fail("1..6 formulas");
+    for (int i = 0; i < nf; i++) {
+        int k = sc->formulas[i].kind;
+        if (k != MB3D_HIP_FORMULA_INTPOW && k != MB3D_HIP_FORMULA_AMAZINGBOX) return fail("unknown formula kind");
+    }
+    bool any = false;
+    for (int i = 0; i < nf; i++) any = any || sc->formulas[i].iterations != 0;
+    if (!any) return fail("no formula with iterations > 0");
+    mb3d_hip_params p;
+    std::memset(&p, 0, sizeof p);
+    p.abi_version = MB3D_HIP_ABI_VERSION;
+    int width = sc->width, height = sc->height;
+    p.width = width;
+    p.height = height;
+    // slots; the last with iterations ends the hybrid (CheckHybridOptions)
+    int last = nf - 1;
+    while (last > 0 && sc->formulas[last].iterations == 0) last--;
+    int ns = last + 1;
+    p.n_slots = ns;
+    for (int i = 0; i < ns; i++) {
+        const mb3d_hip_formula &f = sc->formulas[i];
+        p.slot_kind[i] = f.kind;
+        p.slot_iters[i] = std::abs(f.iterations);
+        p.slot_uncounted[i] = f.iterations < 0;
+        p.slot_power[i] = f.power;
+        p.slot_z_mul[i] = f.z_mul;
+        p.slot_scale[i] = f.scale;
+        p.slot_min_r[i] = f.min_r;
+        p.slot_fold[i] = f.fold;
+    }
+    int end_to = last;
+    int repeat_from = std::min(sc->repeat_from, end_to);
+    while (repeat_from > 0 && p.slot_iters[repeat_from] <= 0) repeat_from--;
+    // combine_de_options (src/calc.rs:318-372)
+    int de_option = -1;
+    for (int i = 0; i <= end_to; i++) {
+        if (p.slot_iters[i] == 0) continue;
+        int de = formula_de_option(sc->formulas[i]);
+        if (de < 0 || de == de_option) continue;
+        if (de_option == -1 || de_option == 21 || de_option == 22)
+            de_option = de;
+        else if (de_option == 11)
+            de_option = (de == 2 || de == 5) ? 2 : (de != 6 ? 0 : 11);
+        // (de_option 0 keeps 0)
+    }
+    if (sc->de_mode == MB3D_HIP_DE_ANALYTIC_P8) {
+        if (ns != 1 || sc->formulas[0].kind != MB3D_HIP_FORMULA_INTPOW) return fail("analytic p8 DE needs one Integer Power slot");
+        de_option = 32;
+    } else if (sc->de_mode != MB3D_HIP_DE_REFERENCE) {
+        return fail("unknown DE mode");
+    }
+    bool is_custom_de = de_option == 2 || de_option == 5 || de_option == 6 || de_option == 11 || de_option == 20 ||
+                        de_option == 32;
+    p.de_option = de_option;
+    p.is_custom_de = is_custom_de;
+    for (int i = 0; i < ns; i++) {
+        p.slot_ade[i] = is_custom_de;
+        int de = formula_de_option(sc->formulas[i]);
+        p.slot_uncounted[i] = p.slot_uncounted[i] || de < 0 || de > 20;
+    }
+    // DE scale, colour helpers, smooth-it logs
+    int max_it = sc->iterations;
+    float fhln_default = (float)(1.0 / std::log(2.0));
+    for (int k = 0; k < 6; k++) p.fhln[k] = fhln_default;
+    double n = 0, x1 = 0, y1 = 0, ii = 0;
+    for (int x = 0; x < ns; x++) {
+        if (p.slot_iters[x] == 0) continue;
+        const mb3d_hip_formula &f = sc->formulas[x];
+        if (!p.slot_uncounted[x]) {
+            double sip = formula_si_pow(f);
+            double z1 = sip > 1.0 ? sip : formula_first_option(f);
+            p.fhln[x] = (float)(1.0 / std::log(std::fmax(2.0, std::fabs(z1))));
+        }
+        double z1 = (double)p.slot_iters[x], cnt = (double)p.slot_iters[x];
+        int j = formula_de_option(f);
+        if (is_custom_de && !p.slot_uncounted[x] && (j == 5 || j == 11)) {
+            n += cnt;
+            ii += cnt;
+            double x2 = formula_first_option(f);
+            if (x2 == 0.0) x2 = 1e-40;
+            if (x2 < 0.0)
+                x1 += 0.65 * z1;
+            else
+                x1 += (x2 * 1.2 + 1.0) / x2 * z1;
+            y1 += std::fabs(x2) * z1;
+        } else if (j >= 0) {
+            n += cnt;
+            x1 += (is_custom_de ? 1.0 : formula_de_scale(f)) * z1;  // ade_scale is 1 for built-ins
+        }
+    }
+    float d_col_plus = 0.0f;
+    double d_de_scale = 1.0;
+    if (n > 0.0 && de_option != 20) d_de_scale = x1 / n;
+    if (n > 0.0 && ii > 0.0) d_col_plus = (float)(std::pow(std::fabs(y1 / ii), 0.25) * 40.0 - 49.0);
+    // mctColVarDEstopMul (src/calc.rs:610-670), one part
+    float col_var = 0.6f;
+    {
+        double yy1 = 0, zz1 = 0, zz2 = 0;
+        int ia = 0, itmp = 0;
+        for (int x = 0; x < ns; x++) {
+            bool counted = p.slot_iters[x] > 0 && !p.slot_uncounted[x];
+            if (!counted) {
+                if (p.slot_iters[x] > 0) {
+                    if (x > end_to)
+                        ia++;
+                    else
+                        itmp++;
+                }
+                continue;
+            }
+            const mb3d_hip_formula &f = sc->formulas[x];
+            double x2 = (double)p.slot_iters[x];
+            int de = formula_de_option(f);
+            if (de == 0 || de == 4) {
+                yy1 += x2 * 0.6;
+                zz1 += x2;
+            } else if ((de >= 1 && de <= 3) || (de >= 5 && de <= 40)) {
+                if (formula_first_option_double(f)) {
+                    double v = formula_first_option(f);
+                    if (v > 1.0) {
+                        if (v < 1.2) v = 1.1 + (v - 1.0) * (v - 1.0) * 2.5;
+                        yy1 += x2 * 0.5 / (std::log(v) + 0.03);
+                        zz1 += x2;
+                    } else if (v < -1.0) {
+                        if (v > -1.2) v = -1.1 - (v + 1.0) * (v + 1.0) * 2.5;
+                        yy1 += x2 * 0.5 / (std::log(-v) + 0.03);
+                        zz1 += x2;
+                    }
+                }
+            }
+        }
+        if ((zz1 >= zz2 && zz1 > 1e-3) || (ia > itmp && std::fabs(zz1 - zz2) < 1e-3 && zz1 > 1e-3))
+            col_var = (float)(yy1 / zz1);
+        else if (zz2 < 1e-3)
+            col_var = 0.6f;
+    }
+    // effective_rstop (src/scene.rs:799-809)
+    double rstop = sc->rstop;
+    if (!(rstop > 0.0)) {
+        rstop = 0.0;
+        for (int i = 0; i < nf; i++)
+            if (sc->formulas[i].iterations > 0) rstop = std::fmax(rstop, formula_default_rstop(sc->formulas[i]));
+        rstop = std::fmax(rstop, 1.0001);
+    }
+    double step_width = 2.1345 / (sc->zoom * (double)width);  // Scene::step_width (src/scene.rs:717-719)
+    double fov_y = sc->fov_y * (3.141592653589793 / 180.0);
+    float z_step_div0 = (float)std::fmax(sc->z_step_div, 0.0001);
+    float de_stop = (float)std::fmax(sc->de_stop, 0.001);
+    double x1c = std::fmin(std::fmax(fov_y * 0.5, 0.01), 1.5);
+    p.pl_optic_z = (float)(std::cos(x1c) * x1c / std::sin(x1c));
+    p.fovx_off = 0.5f * (float)width;  // stereo_xoff() = 0.5 for a mono picture
+    p.fovx_mul = (float)(fov_y / (double)height);
+    // CalcPPZvals
+    double zcorr = std::sin(std::fmax(sc->fov_y * (3.141592653589793 / 180.0), 1.0) / (double)height);
+    double zc_mul =
+        32767.0 * 256.0 / (std::sqrt((sc->z_end - sc->z_start) * zcorr / step_width + 1.0) - 0.999999999);
+    double zend = std::fmax(1e-10, (sc->z_end - sc->z_start) / step_width);
+    float mh04zsd = (float)((double)std::max(width, height) * 0.5 * std::sqrt((double)z_step_div0 + 0.001) *
+                            std::fmax(0.01, sc->raystep_limiter));
+    // GetDEstopFactor
+    float de_stop_factor = 0.0f;
+    if (sc->vary_de_stop_on_fov) {
+        double ze = std::fmax(1e-16, (sc->z_end - sc->z_start) / step_width);
+        double xx1 = 0.01 * (double)height /
+                     (std::sin(0.01) * std::fmax(1.0 / 65535.0, sc->fov_y * (3.141592653589793 / 180.0)));
+        double xx2 = step_width * (xx1 + ze) / xx1;
+        de_stop_factor = (float)((xx2 - step_width) / (step_width * std::fmax(1e-6, ze)));
+    }
+    double xh = (double)width * 0.5, yh = (double)height * 0.5;
+    double z1 = (sc->z_start - sc->mid[2]) / step_width;
+    normalise_matrix_to(step_width, sc->vgrads, p.vgrads);
+    for (int k = 0; k < 3; k++)
+        p.ystart[k] = sc->mid[k] + z1 * p.vgrads[2][k] - yh * p.vgrads[1][k] - xh * p.vgrads[0][k];
+    float de_offset = std::fmin(de_stop * 0.1f, 0.004f);
+    float dds;
+    if (is_custom_de)
+        dds = (float)((double)(float)d_de_scale / step_width);
+    else
+        dds = (float)d_de_scale * de_offset;
+    float de_offset006 = de_offset * 0.06f;
+    de_offset = (float)((double)de_offset * step_width);
+    float z_step_div = z_step_div0, ms_de_sub = 0.0f;
+    if (sc->step_sub_de_stop) {
+        float s = z_step_div0;
+        s = s * s + (1.2f * s) * (1.0f - s);
+        z_step_div = s;
+        ms_de_sub = std::fmin(std::sqrt(s), 0.9f);
+    }
+    float ln_rstop = (float)std::log(std::log(rstop));
+    double d_rstop = (double)(float)(rstop * rstop);
+    double rstop3d = (double)(float)((d_rstop * d_rstop) * 64.0);
+    float mcts_m = 32767.0f / (float)std::max(max_it + 1, 1);
+    d_col_plus = d_col_plus + (float)max_it * 0.1f;
+
+    p.d_de_scale = dds;
+    p.sm_normals = std::min(std::max(sc->smooth_normals, 0), 8);
+    p.de_stop = de_stop;
+    p.z_step_div = z_step_div;
+    p.de_add_steps = sc->bin_search_steps;
+    p.fov_y = fov_y;
+    p.max_it = max_it;
+    p.min_it = std::min(sc->min_iterations, max_it);
+    p.color_option = std::min(sc->color_option, 5);
+    p.mct_color_mul = (float)(sc->color_mul * 512.0);
+    p.d_rstop = d_rstop;
+    p.rstop3d = rstop3d;
+    p.dfog_on_it = sc->dfog_on_it;
+    p.d_col_plus = d_col_plus;
+    p.zcorr = zcorr;
+    p.zc_mul = zc_mul;
+    p.zend = zend;
+    p.step_width = step_width;
+    p.col_var_de_stop_mul = col_var;
+    p.first_step_random = sc->first_step_random;
+    p.mh04zsd = mh04zsd;
+    p.de_stop_factor = de_stop_factor;
+    p.ln_rstop = ln_rstop;
+    p.normals_on_de = sc->normals_on_de || is_custom_de;
+    p.de_offset = de_offset;
+    p.de_offset006 = de_offset006;
+    p.mcts_m = mcts_m;
+    p.ms_de_sub = ms_de_sub;
+    p.end_to = end_to;
+    p.repeat_from = repeat_from;
+    p.step_cap = std::max(sc->step_cap, 0);
+    *out = p;
+    return 0;
+}
+
+extern "C" const char *orc_last_error(void) { return g_err.c_str(); }
+
+// pixel_seed (src/calc.rs:1125-1140)
+extern "C" int32_t orc_pixel_seed(int64_t y, int64_t x) {
+    uint32_t row = (uint32_t)(int32_t)(int64_t)(0x24563487LL + (y + 1) * 0x324594A1LL);
+    uint32_t a = 214013u, c = 2531011u, ra = 1u, rc = 0u;
+    uint64_t n = x > 0 ? (uint64_t)x : 0;
+    while (n > 0) {
+        if (n & 1) {
+            rc = rc * a + c;
+            ra = ra * a;
+        }
+        c = c * a + c;
+        a = a * a;
+        n >>= 1;
+    }
+    return (int32_t)(ra * row + rc);
+}
+
+extern "C" void orc_iterate(const mb3d_hip_formula *f, double v[4], double j[4], double rout, int32_t ade) {
+    (void)rout;
+    if (f->kind == MB3D_HIP_FORMULA_INTPOW)
+        int_pow(f->power, f->z_mul, v, j);
+    else if (f->kind == MB3D_HIP_FORMULA_AMAZINGBOX)
+        amazing_box(f->scale, f->min_r, f->fold, v, j, ade != 0);
+}
+
+extern "C" double orc_de_at(const mb3d_hip_params *p, const double pos[3], int32_t *its) {
+    if (check_params(p)) return NAN;
+    Marcher m(*p);
+    m.it.c[0] = pos[0];
+    m.it.c[1] = pos[1];
+    m.it.c[2] = pos[2];
+    double d = m.calc_de();
+    if (its) *its = m.it.it_result;
+    return d * p->step_width;
+}
+
+extern "C" void orc_pixel_ray(const mb3d_hip_params *p, int32_t x, int32_t y, double c[3], double v[3]) {
+    Marcher m(*p);
+    m.setup_ray(x, y);
+    for (int k = 0; k < 3; k++) {
+        c[k] = m.it.c[k];
+        v[k] = m.vfov[k];
+    }
+}
+
+extern "C" double orc_march_ray(const mb3d_hip_params *p, const double origin[3], const double dir[3], double zz,
+                                int32_t *its, int32_t *steps) {
+    if (check_params(p)) return NAN;
+    Marcher m(*p);
+    for (int k = 0; k < 3; k++) {
+        m.it.c[k] = origin[k];
+        m.vfov[k] = dir[k];
+    }
+    m.mzz = zz;
+    m.it.calc_sit = false;
+    m.update_de_stop();
+    double d = m.calc_de();
+    Px si;
+    march_from(m, si, d, false, 0.0f);
+    if (its) *its = si.iters;
+    if (steps) *steps = si.steps;
+    return si.depth;
+}
+
+extern "C" int32_t orc_render(const mb3d_hip_params *pp, const mb3d_hip_post_job *post, int32_t row0, int32_t rows,
+                              const orc_out *out, int32_t threads, orc_counters *cnt) {
+    if (int e = check_params(pp)) return e;
+    if (!out) return fail("out is NULL");
+    const mb3d_hip_params &p = *pp;
+    if (row0 < 0 || rows < 0 || row0 + rows > p.height) return fail("rows out of the picture");
+    int w = p.width;
+    std::vector<Px> px((size_t)rows * w);
+    std::atomic<uint64_t> c_steps{0}, c_de{0}, c_p8{0}, c_box{0}, c_hits{0};
+    for_rows_interleaved(rows, nthreads(threads), [&](int r) {
+        Marcher m(p);
+        int y = row0 + r;
+        uint64_t hits = 0, steps = 0;
+        for (int x = 0; x < w; x++) {
+            m.seed = orc_pixel_seed(y, x);
+            Px s = march_pixel(m, x, y);
+            hits += std::isfinite(s.depth) && s.depth > 0 ? 1 : 0;
+            steps += (uint64_t)s.steps;
+            px[(size_t)r * w + x] = s;
+        }
+        c_steps += steps;
+        c_hits += hits;
+        c_de += m.de_evals;
+        c_p8 += m.it.its_p8;
+        c_box += m.it.its_box;
+    });
+    if (post) run_post(p, *post, row0, rows, px, threads);
+    for (size_t i = 0; i < px.size(); i++) {
+        const Px &s = px[i];
+        if (out->depth) out->depth[i] = s.depth;
+        if (out->iters) out->iters[i] = s.iters;
+        if (out->normals)
+            for (int k = 0; k < 3; k++) out->normals[3 * i + k] = s.n[k];
+        if (out->gbuf) to_words(s, out->gbuf + 5 * i);
+        if (out->steps) out->steps[i] = s.steps;
+    }
+    if (cnt) {
+        cnt->pixels = px.size();
+        cnt->hits = c_hits;
+        cnt->steps = c_steps;
+        cnt->de_evals = c_de;
+        cnt->its_p8 = c_p8;
+        cnt->its_box = c_box;
+    }
+    return 0;
+}
+
+extern "C" int32_t orc_post(const mb3d_hip_params *pp, const mb3d_hip_post_job *post, int32_t row0, int32_t rows,
+                            uint32_t *gbuf, int32_t threads) {
+    if (int e = check_params(pp)) return e;
+    if (!post || !gbuf) return fail("NULL argument");
+    const mb3d_hip_params &p = *pp;
+    int w = p.width;
+    std::vector<Px> px((size_t)rows * w);
+    for (size_t i = 0; i < px.size(); i++) from_words(gbuf + 5 * i, px[i]);
+    run_post(p, *post, row0, rows, px, threads);
+    for (size_t i = 0; i < px.size(); i++) to_words(px[i], gbuf + 5 * i);
+    return 0;
+}

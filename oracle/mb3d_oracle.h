@@ -1,0 +1,78 @@
+/* mb3d_oracle.h -- TEST INFRASTRUCTURE, not product code.
+ *
+ * The f64 CPU oracle: a restatement of mb3d-rs's CPU ray marcher
+ * (src/calc.rs, src/iteration.rs, src/formulas.rs, src/math.rs,
+ * src/deao.rs) for the built-in Integer Power and Amazing Box formulas and
+ * their alternating hybrids.  Only the tests, the smoke check and the CPU
+ * baseline leg of the benchmark load it, always as the checker.
+ *
+ * It shares the plain data layouts of include/mb3d_hip.h (scene,
+ * parameters, post job), and keeps its own restatement of the parameter
+ * set-up (CalcParams::new), so that the product's set-up is checked
+ * against it field by field.
+ */
+#ifndef MB3D_ORACLE_H
+#define MB3D_ORACLE_H
+
+#include <stdint.h>
+#include "../include/mb3d_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Work counters of an oracle run (same meaning as mb3d_hip_counters). */
+typedef struct orc_counters {
+    uint64_t pixels, hits, steps, de_evals;
+    uint64_t its_p8, its_box;
+} orc_counters;
+
+/* Oracle output per pixel of a rectangle; any pointer may be NULL. */
+typedef struct orc_out {
+    double *depth;      /* mzz (step widths); +inf background; 0 inside at the start */
+    uint16_t *iters;    /* it_result after the bisection (0 background) */
+    double *normals;    /* 3 per pixel, unit, view (vgrads) frame */
+    uint32_t *gbuf;     /* 5 words per pixel (the card record) */
+    int32_t *steps;     /* march steps per pixel */
+} orc_out;
+
+const char *orc_last_error(void);
+
+/* CalcParams::new restated (src/calc.rs:375-898), built-in formulas. */
+int32_t orc_params_from_scene(const mb3d_hip_scene *scene, mb3d_hip_params *params);
+
+/* pixel_seed (src/calc.rs:1125-1140). */
+int32_t orc_pixel_seed(int64_t y, int64_t x);
+
+/* One formula iteration (Formula::iterate, src/formulas.rs:576-641). */
+void orc_iterate(const mb3d_hip_formula *f, double v[4], double j[4], double rout, int32_t ade);
+
+/* de_at (src/calc.rs:2695-2703): the DE at a world point in world units,
+ * and the iteration count there. */
+double orc_de_at(const mb3d_hip_params *p, const double pos[3], int32_t *its);
+
+/* pixel_ray (src/calc.rs:2324-2355 via Marcher::pixel_ray): start and
+ * direction (one step long) of pixel (x, y). */
+void orc_pixel_ray(const mb3d_hip_params *p, int32_t x, int32_t y, double c[3], double v[3]);
+
+/* The G-buffer of rows [row0, row0+rows) by the CPU marcher with row
+ * interleaved threads (src/render.rs:757-803), then the post job on it
+ * (per-pixel hard shadows, DE AO).  threads <= 0: all cores. */
+int32_t orc_render(const mb3d_hip_params *p, const mb3d_hip_post_job *post, int32_t row0,
+                   int32_t rows, const orc_out *out, int32_t threads, orc_counters *cnt);
+
+/* The post job alone on a G-buffer (5 words per pixel, in and out). */
+int32_t orc_post(const mb3d_hip_params *p, const mb3d_hip_post_job *post, int32_t row0,
+                 int32_t rows, uint32_t *gbuf, int32_t threads);
+
+/* The march of one ray from `origin` with direction `dir` (one step long)
+ * at depth `zz`, as Marcher::march_ray_from (src/calc.rs:2519-2531): the
+ * depth reached (+inf: background), its iteration count and steps. */
+double orc_march_ray(const mb3d_hip_params *p, const double origin[3], const double dir[3],
+                     double zz, int32_t *its, int32_t *steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
