@@ -465,3 +465,56 @@ def params_dict(p: Params) -> dict:
             v = [list(x) if hasattr(x, "__len__") else x for x in v]
         out[name] = v
     return out
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=8, guard_band=0.25, it_split=0) -> Options:
+    o = Options()
+    o.precision = precision
+    o.tile_w, o.tile_h = tile
+    o.waves_per_cu = waves_per_cu
+    o.guard_band = guard_band
+    o.it_split = it_split
+    return o
+
+
+def march(p: Params, post: PostJob = None, row0=0, rows=None, opts: Options = None, outputs=None):
+    """mb3d_hip_march into new host arrays: dict of depth (f32), depth64,
+    iters (u16), normals (n x 3 f32), gbuf (n x 5 u32) and the counters."""
+    lib = load()
+    rows = p.height - row0 if rows is None else rows
+    n = rows * p.width
+    want = outputs or ("depth", "depth64", "iters", "normals", "gbuf")
+    res = {
+        "depth": np.zeros(n, np.float32) if "depth" in want else None,
+        "depth64": np.zeros(n, np.float64) if "depth64" in want else None,
+        "iters": np.zeros(n, np.uint16) if "iters" in want else None,
+        "normals": np.zeros((n, 3), np.float32) if "normals" in want else None,
+        "gbuf": np.zeros((n, 5), np.uint32) if "gbuf" in want else None,
+    }
+    o = Out(*(_p(res[k]) for k in ("depth", "depth64", "iters", "normals", "gbuf")), 0, 0)
+    st = lib.mb3d_hip_march(C.byref(p), C.byref(post) if post is not None else None, row0, rows, C.byref(o),
+                            C.byref(opts) if opts is not None else None, PROGRESS_FN(0), CANCEL_FN(0), None)
+    check(st, "mb3d_hip_march")
+    res["status"] = st
+    res["counters"] = counters()
+    return res
+
+
+def post(p: Params, job: PostJob, gbuf, row0=0, rows=None, opts: Options = None):
+    lib = load()
+    rows = p.height - row0 if rows is None else rows
+    g = np.ascontiguousarray(gbuf, dtype=np.uint32).copy()
+    st = lib.mb3d_hip_post(C.byref(p), C.byref(job), row0, rows, _p(g), 0,
+                           C.byref(opts) if opts is not None else None)
+    check(st, "mb3d_hip_post")
+    return g
+
+
+def counters() -> dict:
+    c = Counters()
+    load().mb3d_hip_counters_get(C.byref(c))
+    return {k: getattr(c, k) for k, _ in Counters._fields_}

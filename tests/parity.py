@@ -1,0 +1,58 @@
+"""Parity metrics of a G-buffer against the f64 oracle's (test and bench
+infrastructure).  Tolerances (BASELINE): hit/miss mask agreement >= 99.9 %,
+relative depth error <= 1e-4 of the scene scale, normal error < 0.5 degrees,
+iteration counts equal away from silhouette pixels (a pixel whose 8
+neighbours do not all share its mask)."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def masks(depth):
+    d = np.asarray(depth, np.float64)
+    obj = np.isfinite(d)          # object (surface or inside at the start)
+    hit = obj & (d > 0)           # surface found by the march
+    return obj, hit
+
+
+def silhouette(obj, w, h):
+    m = obj.reshape(h, w)
+    pad = np.pad(m, 1, mode="edge")
+    sil = np.zeros_like(m)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dx or dy:
+                sil |= pad[1 + dy:1 + dy + h, 1 + dx:1 + dx + w] != m
+    return sil.ravel()
+
+
+def compare(gpu: dict, ref: dict, w: int, h: int, step_width: float, scale: float) -> dict:
+    og, hg = masks(gpu["depth64"] if gpu.get("depth64") is not None else gpu["depth"])
+    oo, ho = masks(ref["depth"])
+    both = hg & ho
+    sil = silhouette(oo, w, h) | silhouette(og, w, h)
+    g64 = np.asarray(gpu["depth64"] if gpu.get("depth64") is not None else gpu["depth"], np.float64)
+    rel = np.abs(g64[both] - ref["depth"][both]) * step_width / scale
+    ng = np.asarray(gpu["normals"], np.float64)[both]
+    no = np.asarray(ref["normals"], np.float64)[both]
+    cosang = np.clip(np.sum(ng * no, axis=1) / np.maximum(
+        np.linalg.norm(ng, axis=1) * np.linalg.norm(no, axis=1), 1e-300), -1.0, 1.0)
+    ang = np.degrees(np.arccos(cosang))
+    inner = both & ~sil
+    it_eq = (gpu["iters"][inner] == ref["iters"][inner])
+    q = lambda a, p: float(np.percentile(a, p)) if a.size else 0.0
+    return {
+        "pixels": int(og.size),
+        "mask_agree": float((og == oo).mean()),
+        "hits_gpu": int(hg.sum()),
+        "hits_ref": int(ho.sum()),
+        "both": int(both.sum()),
+        "depth_rel_max": float(rel.max()) if rel.size else 0.0,
+        "depth_rel_p99": q(rel, 99),
+        "depth_rel_mean": float(rel.mean()) if rel.size else 0.0,
+        "normal_deg_max": float(ang.max()) if ang.size else 0.0,
+        "normal_deg_p99": q(ang, 99),
+        "normal_deg_mean": float(ang.mean()) if ang.size else 0.0,
+        "iters_equal_inner": float(it_eq.mean()) if it_eq.size else 1.0,
+        "inner": int(inner.sum()),
+    }

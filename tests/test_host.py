@@ -1,0 +1,127 @@
+"""Host-side checks of the C-ABI library without a GPU: it loads, exports
+every function include/mb3d_hip.h declares, derives the march parameters
+exactly as the oracle's restatement of CalcParams::new does, and rejects bad
+calls with the documented status and a last-error text."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mb3d_hip as H
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if f.endswith(".h"):
+            text = open(os.path.join(inc, f)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(mb3d_hip_\w+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = H.load()
+    names = declared_functions()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(H.SIGNATURES), "the ctypes mirror lists the header's functions"
+
+
+def test_abi_version_and_struct_sizes():
+    assert H.load().mb3d_hip_abi_version() == H.ABI_VERSION
+    # the sizes the header's layout gives on x86-64
+    assert C.sizeof(H.Formula) == 48
+    assert C.sizeof(H.Params) % 8 == 0 and C.sizeof(H.Scene) % 8 == 0
+
+
+def test_scene_default_matches_python_mirror():
+    s = H.Scene()
+    H.load().mb3d_hip_scene_default(C.byref(s))
+    t = H.scene_default()
+    assert bytes(s) == bytes(t)
+
+
+SCENES = {
+    "bulb-analytic-640": lambda: H.scene_bulb(640, 480, seed=1),
+    "bulb-numeric-1080": lambda: H.scene_bulb(1920, 1080, seed=2, de_mode=H.DE_REFERENCE),
+    "box-1080": lambda: H.scene_box(1920, 1080, seed=1),
+    "hybrid-1080": lambda: H.scene_hybrid(1920, 1080, seed=3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_params_match_the_oracle_bit_for_bit(name):
+    sc = SCENES[name]()
+    a = H.params_from_scene(sc)
+    b = O.params(sc)
+    assert bytes(a) == bytes(b)
+
+
+def test_params_of_the_hybrid_chain():
+    """p8 x1 + box x2 combine to the numeric DE (src/calc.rs:318-372) with
+    dDEscale = (1 * 1.0 + 2 * 0.2) / 3 times the DE offset."""
+    p = H.params_from_scene(H.scene_hybrid(640, 480))
+    assert p.de_option == 0 and not p.is_custom_de
+    de_offset = np.float32(min(np.float32(p.de_stop) * np.float32(0.1), np.float32(0.004)))
+    assert p.d_de_scale == np.float32(np.float32(1.4 / 3.0) * de_offset)
+    assert p.d_rstop == 1024.0 ** 2
+
+
+def test_params_of_the_box():
+    p = H.params_from_scene(H.scene_box(1920, 1080))
+    assert p.de_option == 11 and p.is_custom_de and p.slot_ade[0]
+    assert p.d_de_scale == np.float32(np.float64(np.float32(0.65)) / p.step_width)
+
+
+def test_pixel_seed_matches_oracle():
+    lib = H.load()
+    for y, x in [(0, 0), (7, 123), (1079, 1919), (1499, 2999)]:
+        assert lib.mb3d_hip_pixel_seed(y, x) == O.pixel_seed(y, x)
+
+
+def test_invalid_scenes_are_rejected():
+    lib = H.load()
+    s = H.scene_default()
+    s.n_formulas = 0
+    p = H.Params()
+    assert lib.mb3d_hip_params_from_scene(C.byref(s), C.byref(p)) == -1
+    assert "formula" in H.last_error()
+    s = H.scene_default()
+    s.formulas[0].iterations = 0
+    assert lib.mb3d_hip_params_from_scene(C.byref(s), C.byref(p)) == -1
+    s = H.scene_default()
+    s.abi_version = 99
+    assert lib.mb3d_hip_params_from_scene(C.byref(s), C.byref(p)) == -1
+    assert lib.mb3d_hip_params_from_scene(None, C.byref(p)) == -1
+
+
+def test_unsupported_scenes_name_the_reason():
+    lib = H.load()
+    s = H.scene_default()
+    s.formulas[0].power = 5
+    p = H.params_from_scene(s)
+    buf = C.create_string_buffer(200)
+    assert lib.mb3d_hip_supported(C.byref(p), buf, 200) == H.UNSUPPORTED
+    assert b"power" in buf.value
+    s = H.scene_default()
+    s.smooth_normals = 3
+    p = H.params_from_scene(s)
+    assert lib.mb3d_hip_supported(C.byref(p), buf, 200) == H.UNSUPPORTED
+    p = H.params_from_scene(H.scene_bulb(64, 48))
+    assert lib.mb3d_hip_supported(C.byref(p), buf, 200) == H.OK
+
+
+def test_march_before_init_fails_without_touching_a_device():
+    lib = H.load()
+    p = H.params_from_scene(H.scene_bulb(64, 48))
+    o = H.Out()
+    st = lib.mb3d_hip_march(C.byref(p), None, 0, 48, C.byref(o), None, H.PROGRESS_FN(0), H.CANCEL_FN(0), None)
+    assert st == -3 and "init" in H.last_error()
