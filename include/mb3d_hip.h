@@ -190,6 +190,8 @@ typedef struct mb3d_hip_counters {
     uint64_t overflow;       /* pixels whose FP32 values overflowed */
     double kernel_ms;        /* device time of the march kernels (HIP events) */
     double post_ms;          /* device time of the post kernels */
+    double march_f32_ms;     /* of kernel_ms: the FP32 march */
+    double finish_f64_ms;    /* of kernel_ms: the FP64 finish */
 } mb3d_hip_counters;
 
 typedef void (*mb3d_hip_progress_fn)(void *user, int64_t done, int64_t total);

@@ -253,305 +253,352 @@ __device__ __forceinline__ uint32_t clip15(float s) {
     return (uint32_t)rtef(s);
 }
 
-struct M64 {
-    // Iteration (src/iteration.rs:59-105), the parts the built-ins use
-    double v[4], c[3], j[3];
+// The f64 iteration record of one evaluation (Iteration,
+// src/iteration.rs:59-105), the parts the built-in formulas use.
+struct It64 {
+    double v[4], j[3];
     double rout, rold, otrap, deriv1;
-    int it_result, max_it;
-    double rstop;
-    bool calc_sit;
+    int it_result;
+    uint32_t p8, box;
+};
+
+// Formula::iterate via call_slot (src/iteration.rs:325-345)
+__device__ __forceinline__ void call_slot64(const DevP &P, int n, It64 &s) {
+    double *v = s.v;
+    const double *j = s.j;
+    if (P.slot_kind[n] == MB3D_HIP_FORMULA_INTPOW) {
+        if (P.slot_ade[n]) {  // the analytic power-8 extension: dr <- 8 r^7 dr + 1
+            const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            s.deriv1 = 8.0 * (r2 * r2 * r2 * sqrt(r2)) * s.deriv1 + 1.0;
+        }
+        const double x = v[0], y = v[1], z = v[2];
+        const double xx = x * x, yy = y * y, zz = z * z;
+        const double r = xx + yy;
+        const double rr = r * r;
+        const double zzzz = zz * zz;
+        const double t = (zzzz - 6.0 * r * zz + rr) * (zz - r) * sqrt(r);
+        v[2] = -(t * z * 8.0 * P.slot_z_mul[n]) + j[2];
+        const double num = (rr * 70.0 + zzzz) * zzzz - 28.0 * zz * r * (zzzz + rr);
+        const double a = num / (rr * rr + 1e-40) + 1.0;
+        const double xxxx = xx * xx, yyyy = yy * yy;
+        v[1] = (yyyy * (7.0 * xx - yy) + xxxx * (xx - 7.0 * yy)) * 8.0 * x * y * a + j[1];
+        v[0] = a * (xxxx * xxxx + yyyy * (70.0 * xxxx + yyyy) - 28.0 * (xx * yy * (yyyy + xxxx))) + j[0];
+        s.p8++;
+    } else {
+        const double fold = P.slot_fold[n], scale = P.slot_scale[n];
+        const double x = bfold64(v[0], fold), y = bfold64(v[1], fold), z = bfold64(v[2], fold);
+        const double r = z * z + y * y + x * x;
+        const double m = fmax(P.slot_min_r[n], 1e-40);
+        const double sqr_min_r = m * m;
+        double mul;
+        if (r < sqr_min_r)
+            mul = scale / sqr_min_r;
+        else if (1.0 < r)
+            mul = scale;
+        else
+            mul = scale / r;
+        if (P.slot_ade[n]) v[3] *= mul;
+        v[2] = z * mul + j[2];
+        v[1] = y * mul + j[1];
+        v[0] = x * mul + j[0];
+        s.box++;
+    }
+}
+
+// calc_smooth_iterations (src/iteration.rs:140-154)
+__device__ __forceinline__ float smooth64(const DevP &P, const It64 &s, int n) {
+    if (s.rout <= 1.0) return (float)s.it_result;
+    if (s.rold < 1.0) {
+        const double d = log(log(s.rout) * 0.5) * (double)P.fhln[n];
+        return (float)((double)s.it_result + (double)P.ln_rstop - d);
+    }
+    const double d = log(0.5 * log(s.rout));
+    const double e = d - log(0.5 * log(s.rold));
+    return (float)(((double)P.ln_rstop - d) / (e + 1e-100) + (double)s.it_result);
+}
+
+// run_with (src/iteration.rs:268-315), 3D, counted slots; returns the
+// index of the last slot used
+__device__ __forceinline__ int run64(const DevP &P, It64 &s, int max_it, double rstop) {
+    int n = 0;
+    int btmp = P.slot_iters[0];
+    s.it_result = 0;
+    for (;;) {
+        s.rold = s.rout;
+        int guard = 0;
+        while (btmp <= 0) {
+            n += 1;
+            if (n > P.end_to) n = P.repeat_from;
+            btmp = P.slot_iters[n];
+            if (++guard > 64) return n;
+        }
+        call_slot64(P, n, s);
+        btmp -= 1;
+        s.it_result += 1;
+        s.rout = s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2];
+        if (s.rout < s.otrap) s.otrap = s.rout;
+        if (s.it_result >= max_it || s.rout > rstop) return n;
+    }
+}
+
+__device__ __forceinline__ void start64(It64 &s, double cx, double cy, double cz) {
+    s.j[0] = cx;
+    s.j[1] = cy;
+    s.j[2] = cz;
+    s.v[0] = cx;
+    s.v[1] = cy;
+    s.v[2] = cz;
+    s.rout = cx * cx + cy * cy + cz * cz;
+    s.otrap = s.rout;
+}
+
+// The result of one f64 DE evaluation.
+struct De64 {
+    double de, rout, otrap;
+    int it_result;
     float smooth_it;
-    // Marcher (src/calc.rs:1143-1170)
-    double mzz, vfov[3];
-    float ms_de_stop;
-    int max_its_result;
+    uint32_t p8, box;
+};
 
-    __device__ void init(const DevP &P) {
-        v[0] = v[1] = v[2] = v[3] = 0.0;
-        rout = rold = otrap = 0.0;
-        deriv1 = 1.0;
-        it_result = 0;
-        max_it = P.max_it;
-        rstop = P.d_rstop;
-        calc_sit = false;
-        smooth_it = 0.0f;
-        mzz = 0.0;
-        ms_de_stop = P.de_stop;
-        max_its_result = P.max_it;
-    }
-
-    // Formula::iterate via call_slot (src/iteration.rs:325-345)
-    __device__ void call_slot(const DevP &P, int n, Cnt &cnt) {
-        if (P.slot_kind[n] == MB3D_HIP_FORMULA_INTPOW) {
-            if (P.slot_ade[n]) {  // the analytic power-8 extension: dr <- 8 r^7 dr + 1
-                const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-                deriv1 = 8.0 * (r2 * r2 * r2 * sqrt(r2)) * deriv1 + 1.0;
-            }
-            const double x = v[0], y = v[1], z = v[2];
-            const double xx = x * x, yy = y * y, zz = z * z;
-            const double r = xx + yy;
-            const double rr = r * r;
-            const double zzzz = zz * zz;
-            const double t = (zzzz - 6.0 * r * zz + rr) * (zz - r) * sqrt(r);
-            v[2] = -(t * z * 8.0 * P.slot_z_mul[n]) + j[2];
-            const double num = (rr * 70.0 + zzzz) * zzzz - 28.0 * zz * r * (zzzz + rr);
-            const double a = num / (rr * rr + 1e-40) + 1.0;
-            const double xxxx = xx * xx, yyyy = yy * yy;
-            v[1] = (yyyy * (7.0 * xx - yy) + xxxx * (xx - 7.0 * yy)) * 8.0 * x * y * a + j[1];
-            v[0] = a * (xxxx * xxxx + yyyy * (70.0 * xxxx + yyyy) - 28.0 * (xx * yy * (yyyy + xxxx))) + j[0];
-            cnt.p8_64++;
-        } else {
-            const double fold = P.slot_fold[n], scale = P.slot_scale[n];
-            const double x = bfold64(v[0], fold), y = bfold64(v[1], fold), z = bfold64(v[2], fold);
-            const double r = z * z + y * y + x * x;
-            const double m = fmax(P.slot_min_r[n], 1e-40);
-            const double sqr_min_r = m * m;
-            double mul;
-            if (r < sqr_min_r)
-                mul = scale / sqr_min_r;
-            else if (1.0 < r)
-                mul = scale;
-            else
-                mul = scale / r;
-            if (P.slot_ade[n]) v[3] *= mul;
-            v[2] = z * mul + j[2];
-            v[1] = y * mul + j[1];
-            v[0] = x * mul + j[0];
-            cnt.box_64++;
-        }
-    }
-
-    // calc_smooth_iterations (src/iteration.rs:140-154)
-    __device__ void smooth(const DevP &P, int n) {
-        if (rout <= 1.0) {
-            smooth_it = (float)it_result;
-        } else if (rold < 1.0) {
-            const double d = log(log(rout) * 0.5) * (double)P.fhln[n];
-            smooth_it = (float)((double)it_result + (double)P.ln_rstop - d);
-        } else {
-            const double d = log(0.5 * log(rout));
-            const double e = d - log(0.5 * log(rold));
-            smooth_it = (float)(((double)P.ln_rstop - d) / (e + 1e-100) + (double)it_result);
-        }
-    }
-
-    // run_with (src/iteration.rs:268-315), 3D, counted slots
-    __device__ int run(const DevP &P, Cnt &cnt) {
-        int n = 0;
-        int btmp = P.slot_iters[0];
-        it_result = 0;
-        for (;;) {
-            rold = rout;
-            int guard = 0;
-            while (btmp <= 0) {
-                n += 1;
-                if (n > P.end_to) n = P.repeat_from;
-                btmp = P.slot_iters[n];
-                if (++guard > 64) return n;
-            }
-            call_slot(P, n, cnt);
-            btmp -= 1;
-            it_result += 1;
-            rout = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-            if (rout < otrap) otrap = rout;
-            if (it_result >= max_it || rout > rstop) return n;
-        }
-    }
-
-    __device__ void start_point() {
-        j[0] = c[0];
-        j[1] = c[1];
-        j[2] = c[2];
-        v[0] = c[0];
-        v[1] = c[1];
-        v[2] = c[2];
-        rout = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
-        otrap = rout;
-    }
-
-    // hybrid_3d (src/iteration.rs:521-537)
-    __device__ void hybrid_3d(const DevP &P, Cnt &cnt) {
-        start_point();
-        v[3] = 0.0;
-        const int n = run(P, cnt);
-        if (calc_sit) smooth(P, n);
-    }
-
-    // hybrid_3d_de (src/iteration.rs:539-585); option 32: the textbook
-    // 0.5 r ln r / dr of the power-8 extension
-    __device__ double hybrid_3d_de(const DevP &P, Cnt &cnt) {
-        start_point();
+// calc_de_part without dIFS (src/calc.rs:1495-1559): hybrid_3d_de
+// (src/iteration.rs:539-585) for the analytic options -- option 32 being the
+// textbook 0.5 r ln r / dr of the power-8 extension -- or hybrid_3d
+// (src/iteration.rs:521-537) four times for the numeric gradient.  One
+// out-of-line copy for all kernels; arguments and result in registers.
+__device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double cz, int max_it, int calc_sit,
+                                  float ms) {
+    De64 o;
+    It64 s;
+    s.p8 = s.box = 0;
+    s.deriv1 = 1.0;
+    s.v[3] = 0.0;
+    s.rold = 0.0;
+    double result;
+    o.smooth_it = 0.0f;
+    start64(s, cx, cy, cz);
+    if (P.is_custom_de) {
         if ((P.de_option & 0x38) == 32)
-            deriv1 = 1.0;
+            s.deriv1 = 1.0;
         else if ((P.de_option & 0x38) == 16)
-            v[3] = rout;
+            s.v[3] = s.rout;
         else
-            v[3] = 1.0;
-        const int n = run(P, cnt);
-        double result;
+            s.v[3] = 1.0;
+        const int n = run64(P, s, max_it, P.d_rstop);
         if ((P.de_option & 0x38) == 32)
-            result = sqrt(rout) * 0.25 * log(rout) / deriv1;
+            result = sqrt(s.rout) * 0.25 * log(s.rout) / s.deriv1;
         else
-            result = sqrt(rout) / fabs(v[3]);
-        if (calc_sit) smooth(P, n);
-        return result;
-    }
-
-    // calc_de_part without dIFS (src/calc.rs:1495-1559)
-    __device__ __noinline__ double calc_de(const DevP &P, Cnt &cnt) {
-        cnt.de64++;
-        double result;
-        if (P.is_custom_de) {
-            result = hybrid_3d_de(P, cnt) * (double)P.d_de_scale;
+            result = sqrt(s.rout) / fabs(s.v[3]);
+        if (calc_sit) o.smooth_it = smooth64(P, s, n);
+        result = result * (double)P.d_de_scale;
+        o.it_result = s.it_result;
+        o.rout = s.rout;
+        o.otrap = s.otrap;
+    } else {
+        const int n = run64(P, s, max_it, P.d_rstop);
+        if (calc_sit) o.smooth_it = smooth64(P, s, n);
+        o.it_result = s.it_result;
+        o.rout = s.rout;
+        o.otrap = s.otrap;
+        if (s.rout < 1e-200) {
+            result = 0.0;
         } else {
-            hybrid_3d(P, cnt);
-            if (rout < 1e-200) {
-                result = 0.0;
-            } else {
-                const bool buf_sit = calc_sit;
-                const int buf_max_it = max_it;
-                const double buf_rout = rout;
-                calc_sit = false;
-                max_it = it_result;
-                rstop = P.rstop3d;
-                const double off = (double)P.de_offset;
-                double g[3];
-                for (int k = 0; k < 3; k++) {
-                    const double bufd = c[k];
-                    c[k] = bufd + off;
-                    hybrid_3d(P, cnt);
-                    g[k] = (buf_rout - rout) * (buf_rout - rout);
-                    c[k] = bufd;
-                }
-                result = buf_rout * log(buf_rout) * (double)P.d_de_scale /
-                         (sqrt(g[0] + g[1] + g[2]) + (double)P.de_offset006);
-                rout = buf_rout;
-                it_result = max_it;
-                max_it = buf_max_it;
-                calc_sit = buf_sit;
-                rstop = P.d_rstop;
+            const double buf_rout = s.rout;
+            const int it0 = s.it_result;
+            const double off = (double)P.de_offset;
+            double g[3];
+            for (int k = 0; k < 3; k++) {
+                start64(s, k == 0 ? cx + off : cx, k == 1 ? cy + off : cy, k == 2 ? cz + off : cz);
+                s.v[3] = 0.0;
+                run64(P, s, it0, P.rstop3d);
+                g[k] = (buf_rout - s.rout) * (buf_rout - s.rout);
             }
-        }
-        max_its_result = max_it;
-        const double mn = (double)ms_de_stop * 0.25;
-        if (result < mn || isnan(result)) result = mn;
-        return result;
-    }
-
-    __device__ __forceinline__ void step(double d) {
-        c[0] += vfov[0] * d;
-        c[1] += vfov[1] * d;
-        c[2] += vfov[2] * d;
-    }
-
-    // update_de_stop (src/calc.rs:1698-1703)
-    __device__ __forceinline__ void update_de_stop(const DevP &P) {
-        ms_de_stop = (float)((double)P.de_stop * (1.0 + mzz * (double)P.de_stop_factor));
-    }
-
-    // bin_search (src/calc.rs:1706-1735)
-    __device__ void bin_search(const DevP &P, double &de, double last_step_width, Cnt &cnt) {
-        int itmp = P.de_add_steps;
-        double dt1 = last_step_width * -0.5;
-        while (fabs(de - (double)ms_de_stop) > 0.001) {
-            mzz += dt1;
-            step(dt1);
-            update_de_stop(P);
-            itmp -= 1;
-            if (itmp <= 0) break;
-            de = calc_de(P, cnt);
-            if (it_result >= max_its_result) {
-                dt1 = -fabs(dt1);
-                continue;
-            }
-            const double f = de - (double)ms_de_stop;
-            dt1 = f < 0.0 ? fabs(dt1) * -0.55 : fabs(dt1) * 0.55;
+            result = buf_rout * log(buf_rout) * (double)P.d_de_scale /
+                     (sqrt(g[0] + g[1] + g[2]) + (double)P.de_offset006);
+            o.otrap = s.otrap;  // the last pass's, as the reference leaves it
         }
     }
+    const double mn = (double)ms * 0.25;
+    if (result < mn || isnan(result)) result = mn;
+    o.de = result;
+    o.p8 = s.p8;
+    o.box = s.box;
+    return o;
+}
 
-    // bin_search_it (src/calc.rs:1737-1792)
-    __device__ __noinline__ void bin_search_it(const DevP &P, int steps, Cnt &cnt) {
-        const float yp = (float)max_it - 0.99f;
-        const int saved_max = max_it;
-        max_it += 1;
-        int itmp = steps;
-        calc_sit = true;
-        double dt1 = 0.0, dmul = 1.0;
-        bool first = true;
-        float last_si = 0.0f, last_dif = 0.0f;
-        for (;;) {
-            mzz += dt1;
-            step(dt1);
-            calc_de(P, cnt);
-            if (!first && last_dif < fabsf(yp - smooth_it)) {
-                mzz -= dt1;
-                step(-dt1);
-                smooth_it = last_si;
-                dmul *= dt1 > 0.0 ? 0.5 : 0.7;
-            }
-            last_dif = fabsf(yp - smooth_it);
-            last_si = smooth_it;
-            if (smooth_it > (float)max_it - 0.1f) {
-                dt1 = -3.0;
-            } else {
-                mzz -= 0.001;
-                step(-0.001);
-                calc_de(P, cnt);
-                const float r = last_si - smooth_it;
-                if (fabsf(r) < 1e-30f)
-                    dt1 = (last_si < smooth_it ? 1.0 : 0.0) - 0.5;
-                else if (r < 0.0f)
-                    dt1 = (double)((yp - smooth_it) / (r * 500.0f));
-                else
-                    dt1 = (double)((yp - smooth_it) / (r * 1000.0f));
-                if (dt1 > 4.0)
-                    dt1 = sqrt(dt1) * 2.0;
-                else if (dt1 < -9.0)
-                    dt1 = sqrt(-dt1) * -3.0;
-                dt1 = dt1 * dmul + 0.0005;
-            }
-            first = false;
-            itmp -= 1;
-            if (itmp < 0) break;
-        }
-        max_it = saved_max;
-    }
+// The reference's Marcher (src/calc.rs:1143-1170) for one ray in f64: the
+// position, direction and depth, the DE stop, and the values of its last DE
+// evaluation that the decisions read.
+struct Ray64 {
+    double c[3], v[3];
+    double mzz;
+    float ms;  // ms_de_stop
+    int max_it;
+    bool calc_sit;
+    // the last DE
+    double de, rout, otrap;
+    int it, max_its;  // it_result, max_its_result
+    float smooth_it;
 
-    // calculate_normals, smooth_normals = 0 (src/calc.rs:1813-1858)
-    __device__ float normals(const DevP &P, double n[3], Cnt &cnt) {
-        calc_sit = true;
-        const double noffset = (double)fminf(P.de_stop, 1.0f) * (1.0 + mzz * (double)P.de_stop_factor) * 0.15;
-        const double ct[3] = {c[0], c[1], c[2]};
-        calc_de(P, cnt);
-        const float nn = smooth_it;
+    __device__ __forceinline__ void reset(const DevP &P) {
+        mzz = 0.0;
+        ms = P.de_stop;
+        max_it = P.max_it;
         calc_sit = false;
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int axis = q == 0 ? 2 : q - 1;  // the order vg2, vg0, vg1
-            c[0] = ct[0] + P.vg[axis][0] * noffset;
-            c[1] = ct[1] + P.vg[axis][1] * noffset;
-            c[2] = ct[2] + P.vg[axis][2] * noffset;
-            const double a = calc_de(P, cnt);
-            const double m2 = -2.0 * noffset;
-            c[0] += P.vg[axis][0] * m2;
-            c[1] += P.vg[axis][1] * m2;
-            c[2] += P.vg[axis][2] * m2;
-            n[axis] = (a - calc_de(P, cnt)) * 0.5;
-        }
-        c[0] = ct[0];
-        c[1] = ct[1];
-        c[2] = ct[2];
-        return nn;
+        de = rout = otrap = 0.0;
+        it = 0;
+        max_its = P.max_it;
+        smooth_it = 0.0f;
     }
-
+    // mzz += d; step(d)
+    __device__ __forceinline__ void move(double d) {
+        mzz += d;
+        c[0] += v[0] * d;
+        c[1] += v[1] * d;
+        c[2] += v[2] * d;
+    }
+    // update_de_stop (src/calc.rs:1698-1703)
+    __device__ __forceinline__ void upd(const DevP &P) {
+        ms = (float)((double)P.de_stop * (1.0 + mzz * (double)P.de_stop_factor));
+    }
+    // takes the result of de64 at c
+    __device__ __forceinline__ void take(const De64 &r, Cnt &cnt) {
+        de = r.de;
+        it = r.it_result;
+        rout = r.rout;
+        otrap = r.otrap;
+        if (calc_sit) smooth_it = r.smooth_it;
+        max_its = max_it;
+        cnt.de64++;
+        cnt.p8_64 += r.p8;
+        cnt.box_64 += r.box;
+    }
     // calc_zpos_and_rough, smooth_normals = 0 (src/calc.rs:2194-2205)
-    __device__ uint32_t zpos_fine(const DevP &P) const {
+    __device__ __forceinline__ uint32_t zpos_fine(const DevP &P) const {
         const double zz = fmax(mzz, 0.0);
         const double vr = rte(P.zc_mul * (sqrt(zz * P.zcorr + 1.0) - 1.0));
         const int64_t vv = (int64_t)vr;
         int64_t itmp = 8388352 - vv;
         itmp = itmp < 0 ? 0 : (itmp > 8388352 ? 8388352 : itmp);
         return (uint32_t)itmp << 8;
+    }
+};
+
+// The surface refinements as resumable loops: each call runs until the loop
+// needs its next DE at the ray's position (returns true) or ends (false).
+//   RK_BS   bin_search     (src/calc.rs:1706-1735)
+//   RK_BSI  bin_search_it  (src/calc.rs:1737-1792)
+//   RK_SP   the bisection of surface_point (src/calc.rs:2173-2185)
+enum RefKind : int { RK_BS = 0, RK_BSI = 1, RK_SP = 2 };
+
+struct Refine {
+    int kind, itmp, saved_max, sub;
+    double dt1, d, dmul, thr;
+    float yp, last_si, last_dif;
+    bool first;
+
+    __device__ __forceinline__ bool bs_head(Ray64 &R, const DevP &P) {
+        if (fabs(d - (double)R.ms) > 0.001) {
+            R.move(dt1);
+            R.upd(P);
+            itmp -= 1;
+            return itmp > 0;
+        }
+        return false;
+    }
+    __device__ __forceinline__ bool start_bs(Ray64 &R, const DevP &P, double de, double last_step_width, int steps) {
+        kind = RK_BS;
+        itmp = steps;
+        dt1 = last_step_width * -0.5;
+        d = de;
+        return bs_head(R, P);
+    }
+    __device__ __forceinline__ bool start_bsi(Ray64 &R, int steps) {
+        kind = RK_BSI;
+        yp = (float)R.max_it - 0.99f;
+        saved_max = R.max_it;
+        R.max_it += 1;
+        itmp = steps;
+        R.calc_sit = true;
+        dt1 = 0.0;
+        dmul = 1.0;
+        first = true;
+        last_si = last_dif = 0.0f;
+        sub = 0;
+        R.move(dt1);
+        return true;
+    }
+    __device__ __forceinline__ bool sp_head(Ray64 &R, const DevP &P) {
+        if (itmp > 0 && fabs(d - (double)R.ms) > thr) {
+            R.move(dt1);
+            R.upd(P);
+            itmp -= 1;
+            return itmp > 0;
+        }
+        return false;
+    }
+    __device__ __forceinline__ bool start_sp(Ray64 &R, const DevP &P, double de, double dt, int steps, double th) {
+        kind = RK_SP;
+        itmp = steps;
+        d = de;
+        dt1 = dt;
+        thr = th;
+        return sp_head(R, P);
+    }
+    // after the DE the loop asked for
+    __device__ __forceinline__ bool after(Ray64 &R, const DevP &P) {
+        if (kind == RK_BS) {
+            d = R.de;
+            if (R.it >= R.max_its) {
+                dt1 = -fabs(dt1);
+            } else {
+                const double f = d - (double)R.ms;
+                dt1 = f < 0.0 ? fabs(dt1) * -0.55 : fabs(dt1) * 0.55;
+            }
+            return bs_head(R, P);
+        }
+        if (kind == RK_SP) {
+            d = R.de;
+            dt1 = d < (double)R.ms ? fabs(dt1) * -0.55 : fabs(dt1) * 0.55;
+            return sp_head(R, P);
+        }
+        // bin_search_it: the main probe, then the probe 0.001 nearer
+        if (sub == 0) {
+            if (!first && last_dif < fabsf(yp - R.smooth_it)) {
+                R.move(-dt1);
+                R.smooth_it = last_si;
+                dmul *= dt1 > 0.0 ? 0.5 : 0.7;
+            }
+            last_dif = fabsf(yp - R.smooth_it);
+            last_si = R.smooth_it;
+            if (!(R.smooth_it > (float)R.max_it - 0.1f)) {
+                R.move(-0.001);
+                sub = 1;
+                return true;
+            }
+            dt1 = -3.0;
+        } else {
+            const float r = last_si - R.smooth_it;
+            if (fabsf(r) < 1e-30f)
+                dt1 = (last_si < R.smooth_it ? 1.0 : 0.0) - 0.5;
+            else if (r < 0.0f)
+                dt1 = (double)((yp - R.smooth_it) / (r * 500.0f));
+            else
+                dt1 = (double)((yp - R.smooth_it) / (r * 1000.0f));
+            if (dt1 > 4.0)
+                dt1 = sqrt(dt1) * 2.0;
+            else if (dt1 < -9.0)
+                dt1 = sqrt(-dt1) * -3.0;
+            dt1 = dt1 * dmul + 0.0005;
+            sub = 0;
+        }
+        first = false;
+        itmp -= 1;
+        if (itmp < 0) {
+            R.max_it = saved_max;
+            return false;
+        }
+        R.move(dt1);
+        return true;
     }
 };
 

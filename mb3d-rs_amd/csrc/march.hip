@@ -47,6 +47,9 @@ enum : int {
     C_QUEUE = 0,   // next work item of the march queue
     C_CAND = 1,    // hand-off list length
     C_HITS = 2,    // surface pixel list length
+    C_FINQ = 3,    // next item of the f64 finish
+    C_POSTQ = 4,   // next item of the shadow pass
+    C_AOQ = 5,     // next item of the DE AO pass
     C_STATS = 8,   // 9 work counters from here
     C_WORDS = 24
 };
@@ -153,12 +156,13 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
     bool have = false, exhausted = false;
     int pix = 0, px = 0, py = 0;
     double c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
-    double mzz = 0, last_step = 0, last_de = 0, dtmp = 0;
+    double mzz = 0;
+    float last_step = 0, last_de = 0, dtmp = 0;
     float ms = 0, rsf = 1, scount = 0;
     int nsteps = 0;
     bool start = false, first = false;
     int32_t seed = 0;
-    const double guard = (double)P.guard;
+    const float fguard = P.guard;
     const int tile_n = P.tile_w * P.tile_h;
     for (;;) {
         // refill: every idle lane takes the next work item (one atomic per wave)
@@ -201,25 +205,27 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
         if (have) {
             int it = 0;
             const float d32 = de32<KIND, DEM>(P, c, ms, it, cnt);
+            // the step bookkeeping of the FP32 march runs in f32 (the f64
+            // finish re-decides everything near the surface); the depth and
+            // the position accumulate in f64
             bool top = true;
             if (!(d32 < __builtin_inff())) {  // single precision overflow: f64 decides
                 cnt.overflow++;
                 handoff = true;
                 top = false;
             } else if (start) {
-                if (it >= P.max_it || !((double)d32 >= (double)ms * guard)) {
+                if (it >= P.max_it || !(d32 >= ms * fguard)) {
                     handoff = true;
                     top = false;
                 } else {
-                    dtmp = (double)d32;
-                    last_step = dtmp * (double)P.z_step_div;
+                    dtmp = d32;
+                    last_step = dtmp * P.z_step_div;
                 }
             } else {
-                dtmp = (double)d32;
-                if (dtmp > last_de + last_step) dtmp = last_de + last_step;
-                if (last_de > dtmp + 1e-30) {
-                    const double t = last_step / (last_de - dtmp);
-                    rsf = t < 1.0 ? (float)fmax(0.5, t) : 1.0f;
+                dtmp = fminf(d32, last_de + last_step);
+                if (last_de > dtmp + 1e-30f) {
+                    const float t = last_step * __builtin_amdgcn_rcpf(last_de - dtmp);
+                    rsf = t < 1.0f ? fmaxf(0.5f, t) : 1.0f;
                 } else {
                     rsf = 1.0f;
                 }
@@ -227,17 +233,16 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
             if (top) {
                 if (it >= P.max_it) {
                     handoff = true;  // the half step back (src/calc.rs:2545-2552) is f64 work
-                } else if (it < P.min_it || dtmp >= (double)ms * guard) {
+                } else if (it < P.min_it || dtmp >= ms * fguard) {
                     if (P.step_cap > 0 && nsteps >= P.step_cap) {
                         background = true;
                     } else {
                         last_de = dtmp;
-                        double d = fmax(0.11, (dtmp - (double)P.ms_de_sub * (double)ms) * (double)P.z_step_div *
-                                                  (double)rsf);
-                        const double s1 = (double)(fmaxf(ms, 0.4f) * P.mh04zsd);
+                        float d = fmaxf(0.11f, (dtmp - P.ms_de_sub * ms) * P.z_step_div * rsf);
+                        const float s1 = fmaxf(ms, 0.4f) * P.mh04zsd;
                         const bool count = P.dfog_on_it == 0 || it == P.dfog_on_it;
                         if (s1 < d) {
-                            if (count) scount += (float)(s1 / d);
+                            if (count) scount += s1 * __builtin_amdgcn_rcpf(d);
                             d = s1;
                         } else if (count) {
                             scount += 1.0f;
@@ -245,19 +250,20 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
                         if (first) {
                             first = false;
                             seed = (int32_t)((uint32_t)seed * 214013u + 2531011u);
-                            d *= (double)(seed & 0x7FFFFFFF) * (1.0 / (double)0x7FFFFFFF);
+                            d = (float)((double)d * ((double)(seed & 0x7FFFFFFF) * (1.0 / (double)0x7FFFFFFF)));
                         }
-                        mzz += d;
+                        const double dd = (double)d;
+                        mzz += dd;
                         if (mzz > P.zend) {
                             background = true;
                         } else {
                             nsteps++;
                             cnt.steps++;
                             last_step = d;
-                            c[0] += v[0] * d;
-                            c[1] += v[1] * d;
-                            c[2] += v[2] * d;
-                            ms = (float)((double)P.de_stop * (1.0 + mzz * (double)P.de_stop_factor));
+                            c[0] += v[0] * dd;
+                            c[1] += v[1] * dd;
+                            c[2] += v[2] * dd;
+                            ms = P.de_stop * (1.0f + (float)mzz * P.de_stop_factor);
                             start = false;
                         }
                     }
@@ -295,37 +301,11 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
 }
 
 // ---------------------------------------------------------------------------
-// pass 2: the f64 finish
-
-__device__ void write_surface(const DevP &P, const Bufs &B, int pix, const M64 &m, const double n[3], int16_t nq[3],
-                              float nn, bool de_limited, uint16_t iters, float step_count, uint32_t zf) {
-    // si_gradient (src/calc.rs:2497-2502)
-    float g;
-    if (de_limited) {
-        const float lg = (float)log((double)fmaxf(P.de_stop, m.ms_de_stop) * P.step_width);
-        g = 32767.0f - (nn + P.d_col_plus + P.col_var * lg) * P.mcts_m;
-    } else {
-        g = 32767.0f - nn * P.mcts_m;
-    }
-    const uint32_t si_gradient = clip15(g);
-    const uint32_t otrap = clip15((float)(m.otrap * 4096.0));  // do_color, option 0
-    const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
-    const double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    if (B.depth) B.depth[pix] = (float)m.mzz;
-    if (B.depth64) B.depth64[pix] = m.mzz;
-    if (B.iters) B.iters[pix] = iters;
-    if (B.normals) {
-        for (int k = 0; k < 3; k++) B.normals[3 * pix + k] = l > 0 ? (float)(n[k] / l) : 0.0f;
-    }
-    if (B.gbuf) {
-        uint32_t *w = B.gbuf + 5 * (size_t)pix;
-        w[0] = (uint32_t)(uint16_t)nq[0] | ((uint32_t)(uint16_t)nq[1] << 16);
-        w[1] = (uint32_t)(uint16_t)nq[2] | (sh << 16);
-        w[2] = zf;
-        w[3] = si_gradient | (otrap << 16);
-        w[4] = 5000;
-    }
-}
+// pass 2: the f64 finish.  Each lane is a resumable state machine with one
+// DE site (the reference's WGSL keeps the same shape, one evaluate() call in
+// a loop of states, src/gpu_march.wgsl:1903): lanes in different states --
+// march, bisection, normals -- share the one f64 DE of every turn, and a lane
+// that finishes takes the next ray from the list at once.
 
 // store_normal (src/calc.rs:1794-1804)
 __device__ void store_normal(const double n[3], int16_t out[3]) {
@@ -337,297 +317,437 @@ __device__ void store_normal(const double n[3], int16_t out[3]) {
     if (out[0] == 0 && out[1] == 0) out[2] = out[2] > 0 ? 32767 : -32767;
 }
 
+// the surface values of surface_found (src/calc.rs:2471-2517) into the outputs
+__device__ void write_surface(const DevP &P, const Bufs &B, int pix, const Ray64 &R, const double n[3], float nn,
+                              bool de_limited, uint16_t iters, float step_count) {
+    int16_t nq[3];
+    store_normal(n, nq);
+    float g;
+    if (de_limited) {
+        const float lg = (float)log((double)fmaxf(P.de_stop, R.ms) * P.step_width);
+        g = 32767.0f - (nn + P.d_col_plus + P.col_var * lg) * P.mcts_m;
+    } else {
+        g = 32767.0f - nn * P.mcts_m;
+    }
+    const uint32_t si_gradient = clip15(g);
+    const uint32_t otrap = clip15((float)(R.otrap * 4096.0));  // do_color, option 0
+    const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
+    const double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (B.depth) B.depth[pix] = (float)R.mzz;
+    if (B.depth64) B.depth64[pix] = R.mzz;
+    if (B.iters) B.iters[pix] = iters;
+    if (B.normals) {
+        for (int k = 0; k < 3; k++) B.normals[3 * pix + k] = l > 0 ? (float)(n[k] / l) : 0.0f;
+    }
+    if (B.gbuf) {
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        w[0] = (uint32_t)(uint16_t)nq[0] | ((uint32_t)(uint16_t)nq[1] << 16);
+        w[1] = (uint32_t)(uint16_t)nq[2] | (sh << 16);
+        w[2] = R.zpos_fine(P);
+        w[3] = si_gradient | (otrap << 16);
+        w[4] = 5000;
+    }
+}
+
+// a ray that starts inside the set (src/calc.rs:2277-2293)
+__device__ void write_inside(const DevP &P, const Bufs &B, int pix, const Ray64 &R) {
+    const double t = fmin(fmax(R.rout / P.d_rstop, 0.0), 1.0);
+    const uint32_t sg = 32768u + (uint32_t)rte(32767.0 * t);
+    const uint32_t otrap = clip15((float)(R.otrap * 4096.0));
+    if (B.depth) B.depth[pix] = 0.0f;
+    if (B.depth64) B.depth64[pix] = 0.0;
+    if (B.iters) B.iters[pix] = (uint16_t)max(R.it, 0);
+    if (B.normals) {
+        B.normals[3 * pix] = 0.0f;
+        B.normals[3 * pix + 1] = 0.0f;
+        B.normals[3 * pix + 2] = -1.0f;
+    }
+    if (B.gbuf) {
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        w[0] = 0;
+        w[1] = (uint32_t)(uint16_t)(int16_t)-32767;
+        w[2] = 0x7FFF0000u;
+        w[3] = sg | (otrap << 16);
+        w[4] = 5000;
+    }
+}
+
+enum FinPhase : int { FP_START, FP_RESUME, FP_BACK, FP_STEP, FP_REF, FP_NC, FP_NP, FP_NM };
+
 __global__ __launch_bounds__(256) void k_finish64(DevP P, Bufs B, int all_pixels) {
     Cnt cnt;
-    const long long n = all_pixels ? P.npix : (long long)B.ctr[C_CAND];
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        M64 m;
-        m.init(P);
-        int pix;
-        int flags;
-        double last_step = 0, last_de = 0;
-        float rsf = 1.0f, scount = 0.0f;
-        int nsteps = 0;
-        int32_t seed = 0;
-        if (all_pixels) {
-            pix = (int)i;
-            const int x = pix % P.width, y = P.row0 + pix / P.width;
-            pixel_ray(P, x, y, m.c, m.vfov);
-            flags = CF_START | (P.first_step_random ? CF_FIRST : 0);
-            seed = P.first_step_random ? dev_pixel_seed(y, x) : 0;
-        } else {
-            const Cand &o = B.cand[i];
-            pix = o.pix;
-            for (int k = 0; k < 3; k++) {
-                m.c[k] = o.c[k];
-                m.vfov[k] = o.v[k];
-            }
-            m.mzz = o.mzz;
-            last_step = o.last_step;
-            last_de = o.last_de;
-            rsf = o.rsf;
-            scount = o.step_count;
-            flags = o.flags;
-            nsteps = o.nsteps;
-            seed = o.seed;
-        }
-        bool first = (flags & CF_FIRST) != 0;
-        double dtmp;
-        if (flags & CF_START) {
-            m.mzz = 0.0;
-            m.ms_de_stop = P.de_stop;
-            dtmp = m.calc_de(P, cnt);
-            if (m.it_result >= m.max_its_result || dtmp < (double)m.ms_de_stop) {
-                // inside the set at the start plane (src/calc.rs:2277-2293)
-                const double t = fmin(fmax(m.rout / P.d_rstop, 0.0), 1.0);
-                const uint32_t sg = 32768u + (uint32_t)rte(32767.0 * t);
-                const uint32_t otrap = clip15((float)(m.otrap * 4096.0));
-                if (B.depth) B.depth[pix] = 0.0f;
-                if (B.depth64) B.depth64[pix] = 0.0;
-                if (B.iters) B.iters[pix] = (uint16_t)max(m.it_result, 0);
-                if (B.normals) {
-                    B.normals[3 * pix] = 0.0f;
-                    B.normals[3 * pix + 1] = 0.0f;
-                    B.normals[3 * pix + 2] = -1.0f;
-                }
-                if (B.gbuf) {
-                    uint32_t *w = B.gbuf + 5 * (size_t)pix;
-                    w[0] = 0;
-                    w[1] = (uint32_t)(uint16_t)(int16_t)-32767;
-                    w[2] = 0x7FFF0000u;
-                    w[3] = sg | (otrap << 16);
-                    w[4] = 5000;
-                }
-                continue;
-            }
-            last_step = dtmp * (double)P.z_step_div;
-        } else {
-            m.update_de_stop(P);
-            dtmp = m.calc_de(P, cnt);
-            if (flags & CF_STEPPED) {
-                if (dtmp > last_de + last_step) dtmp = last_de + last_step;
-                if (last_de > dtmp + 1e-30) {
-                    const double t = last_step / (last_de - dtmp);
-                    rsf = t < 1.0 ? (float)fmax(0.5, t) : 1.0f;
-                } else {
-                    rsf = 1.0f;
-                }
+    const long long n_items = all_pixels ? P.npix : (long long)B.ctr[C_CAND];
+    bool have = false, exhausted = false;
+    Ray64 R;
+    Refine F;
+    int pix = 0, phase = FP_START;
+    double last_step = 0, last_de = 0, dtmp = 0, back_dt1 = 0;
+    float rsf = 1.0f, scount = 0.0f;
+    int nsteps = 0;
+    uint32_t guard = 0;
+    bool first = false, stepped = false, de_limited = false;
+    int32_t seed = 0;
+    int iters = 0, q = 0, axis = 0;
+    double ct[3] = {0, 0, 0}, nv[3] = {0, 0, 0}, noffset = 0, na = 0;
+    float nn = 0.0f;
+    for (;;) {
+        const bool need = !have && !exhausted;
+        const long long slot = wave_append(need, &B.ctr[C_FINQ]);
+        if (need) {
+            if (slot >= n_items) {
+                exhausted = true;
             } else {
+                have = true;
+                R.reset(P);
+                guard = 0;
+                if (all_pixels) {
+                    pix = (int)slot;
+                    const int x = pix % P.width, y = P.row0 + pix / P.width;
+                    pixel_ray(P, x, y, R.c, R.v);
+                    phase = FP_START;
+                    first = P.first_step_random != 0;
+                    seed = first ? dev_pixel_seed(y, x) : 0;
+                    scount = 0.0f;
+                    nsteps = 0;
+                    rsf = 1.0f;
+                    stepped = false;
+                } else {
+                    const Cand &o = B.cand[slot];
+                    pix = o.pix;
+                    for (int k = 0; k < 3; k++) {
+                        R.c[k] = o.c[k];
+                        R.v[k] = o.v[k];
+                    }
+                    last_step = o.last_step;
+                    last_de = o.last_de;
+                    rsf = o.rsf;
+                    scount = o.step_count;
+                    nsteps = o.nsteps;
+                    seed = o.seed;
+                    first = (o.flags & CF_FIRST) != 0;
+                    stepped = (o.flags & CF_STEPPED) != 0;
+                    if (o.flags & CF_START) {
+                        phase = FP_START;
+                    } else {
+                        R.mzz = o.mzz;
+                        R.upd(P);
+                        phase = FP_RESUME;
+                    }
+                }
+            }
+        }
+        if (__all(exhausted)) break;
+        if (!have) continue;
+        R.take(de64(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+        bool done = false;
+        switch (phase) {
+            case FP_START:
+                if (R.it >= R.max_its || R.de < (double)R.ms) {
+                    write_inside(P, B, pix, R);
+                    done = true;
+                    goto end;
+                }
+                dtmp = R.de;
                 last_step = dtmp * (double)P.z_step_div;
-            }
+                goto top;
+            case FP_RESUME:
+                dtmp = R.de;
+                if (!stepped) {
+                    last_step = dtmp * (double)P.z_step_div;
+                    goto top;
+                }
+                goto clamp;
+            case FP_BACK:
+                dtmp = R.de;
+                last_step = -back_dt1;
+                goto cond;
+            case FP_STEP:
+                dtmp = R.de;
+                goto clamp;
+            case FP_REF:
+                if (F.after(R, P)) goto end;
+                goto refined;
+            case FP_NC:
+                nn = R.smooth_it;
+                R.calc_sit = false;
+                q = 0;
+                goto nplus;
+            case FP_NP:
+                na = R.de;
+                for (int k = 0; k < 3; k++) R.c[k] += P.vg[axis][k] * (-2.0 * noffset);
+                phase = FP_NM;
+                goto end;
+            default:  // FP_NM
+                nv[axis] = (na - R.de) * 0.5;
+                if (++q < 3) goto nplus;
+                for (int k = 0; k < 3; k++) R.c[k] = ct[k];
+                goto write;
         }
-        // the rest of march_from (src/calc.rs:2533-2657) in f64
-        bool hit = false;
-        for (uint32_t guard = 1; guard <= 2000000u; guard++) {
-            if (m.it_result >= m.max_its_result) {
-                const double dt1 = -0.5 * last_step;
-                m.mzz += dt1;
-                m.step(dt1);
-                m.update_de_stop(P);
-                dtmp = m.calc_de(P, cnt);
-                last_step = -dt1;
-            }
-            if (m.it_result < P.min_it || (m.it_result < m.max_its_result && dtmp >= (double)m.ms_de_stop)) {
-                if (P.step_cap > 0 && nsteps >= P.step_cap) break;
-                last_de = dtmp;
-                double d = fmax(0.11, (dtmp - (double)P.ms_de_sub * (double)m.ms_de_stop) * (double)P.z_step_div *
-                                          (double)rsf);
-                const double s1 = (double)(fmaxf(m.ms_de_stop, 0.4f) * P.mh04zsd);
-                const bool count = P.dfog_on_it == 0 || m.it_result == P.dfog_on_it;
-                if (s1 < d) {
-                    if (count) scount += (float)(s1 / d);
-                    d = s1;
-                } else if (count) {
-                    scount += 1.0f;
-                }
-                if (first) {
-                    first = false;
-                    seed = (int32_t)((uint32_t)seed * 214013u + 2531011u);
-                    d *= (double)(seed & 0x7FFFFFFF) * (1.0 / (double)0x7FFFFFFF);
-                }
-                m.mzz += d;
-                if (m.mzz > P.zend) break;
-                nsteps++;
-                cnt.steps++;
-                last_step = d;
-                m.step(d);
-                m.update_de_stop(P);
-                dtmp = m.calc_de(P, cnt);
-                if (dtmp > last_de + last_step) dtmp = last_de + last_step;
-                if (last_de > dtmp + 1e-30) {
-                    const double t = last_step / (last_de - dtmp);
-                    rsf = t < 1.0 ? (float)fmax(0.5, t) : 1.0f;
-                } else {
-                    rsf = 1.0f;
-                }
-            } else {
-                hit = true;
-                break;
-            }
+    clamp:
+        // after a step of march_from (src/calc.rs:2626-2638)
+        if (dtmp > last_de + last_step) dtmp = last_de + last_step;
+        if (last_de > dtmp + 1e-30) {
+            const double t = last_step / (last_de - dtmp);
+            rsf = t < 1.0 ? (float)fmax(0.5, t) : 1.0f;
+        } else {
+            rsf = 1.0f;
         }
-        if (!hit) {
+    top:
+        // the head of march_from's loop (src/calc.rs:2540-2552)
+        if (++guard > 2000000u) {
             write_background(P, B, pix, scount);
-            continue;
+            done = true;
+            goto end;
         }
-        // surface_found (src/calc.rs:2471-2517)
-        const bool de_limited = m.it_result < m.max_its_result || dtmp < (double)m.ms_de_stop;
+        if (R.it >= R.max_its) {
+            back_dt1 = -0.5 * last_step;
+            R.move(back_dt1);
+            R.upd(P);
+            phase = FP_BACK;
+            goto end;
+        }
+    cond:
+        if (R.it < P.min_it || (R.it < R.max_its && dtmp >= (double)R.ms)) {
+            // the next step (src/calc.rs:2553-2620)
+            if (P.step_cap > 0 && nsteps >= P.step_cap) {
+                write_background(P, B, pix, scount);
+                done = true;
+                goto end;
+            }
+            last_de = dtmp;
+            double d = fmax(0.11, (dtmp - (double)P.ms_de_sub * (double)R.ms) * (double)P.z_step_div * (double)rsf);
+            const double s1 = (double)(fmaxf(R.ms, 0.4f) * P.mh04zsd);
+            const bool count = P.dfog_on_it == 0 || R.it == P.dfog_on_it;
+            if (s1 < d) {
+                if (count) scount += (float)(s1 / d);
+                d = s1;
+            } else if (count) {
+                scount += 1.0f;
+            }
+            if (first) {
+                first = false;
+                seed = (int32_t)((uint32_t)seed * 214013u + 2531011u);
+                d *= (double)(seed & 0x7FFFFFFF) * (1.0 / (double)0x7FFFFFFF);
+            }
+            R.mzz += d;
+            if (R.mzz > P.zend) {
+                write_background(P, B, pix, scount);
+                done = true;
+                goto end;
+            }
+            nsteps++;
+            cnt.steps++;
+            last_step = d;
+            for (int k = 0; k < 3; k++) R.c[k] += R.v[k] * d;
+            R.upd(P);
+            phase = FP_STEP;
+            goto end;
+        }
+        // surface found (src/calc.rs:2471-2481)
+        de_limited = R.it < R.max_its || dtmp < (double)R.ms;
         if (P.de_add_steps != 0) {
-            if (de_limited)
-                m.bin_search(P, dtmp, last_step, cnt);
-            else
-                m.bin_search_it(P, P.de_add_steps, cnt);
+            const bool more = de_limited ? F.start_bs(R, P, dtmp, last_step, P.de_add_steps)
+                                         : F.start_bsi(R, P.de_add_steps);
+            if (more) {
+                phase = FP_REF;
+                goto end;
+            }
         }
-        const uint16_t iters = (uint16_t)max(m.it_result, 0);
-        double nv[3];
-        const float nn = m.normals(P, nv, cnt);
-        int16_t nq[3];
-        store_normal(nv, nq);
-        write_surface(P, B, pix, m, nv, nq, nn, de_limited, iters, scount, m.zpos_fine(P));
+    refined:
+        // calculate_normals, smooth_normals 0 (src/calc.rs:1813-1858)
+        iters = max(R.it, 0);
+        R.calc_sit = true;
+        noffset = (double)fminf(P.de_stop, 1.0f) * (1.0 + R.mzz * (double)P.de_stop_factor) * 0.15;
+        for (int k = 0; k < 3; k++) ct[k] = R.c[k];
+        phase = FP_NC;
+        goto end;
+    nplus:
+        axis = q == 0 ? 2 : q - 1;  // the order vg2, vg0, vg1
+        for (int k = 0; k < 3; k++) R.c[k] = ct[k] + P.vg[axis][k] * noffset;
+        phase = FP_NP;
+        goto end;
+    write:
+        write_surface(P, B, pix, R, nv, nn, de_limited, (uint16_t)iters, scount);
         cnt.hits++;
-        const long long h = (long long)atomicAdd(&B.ctr[C_HITS], 1ull);
-        B.hits[h] = pix;
+        B.hits[atomicAdd(&B.ctr[C_HITS], 1ull)] = pix;
+        done = true;
+    end:
+        if (done) have = false;
     }
     flush_counters(cnt, B.ctr);
 }
 
 // ---------------------------------------------------------------------------
-// the post passes
+// the post passes: hard shadows and DE AO, same state-machine shape
 
-// hs_surface_start / surface_point: the f64 surface point of a G-buffer
-// pixel from its depth code (src/calc.rs:2741-2765, 2157-2192)
-__device__ void surface_from_code(const DevP &P, M64 &m, int x, int y, uint32_t zpos_fine, bool ao, Cnt &cnt) {
-    pixel_ray(P, x, y, m.c, m.vfov);
-    m.calc_sit = false;
-    const double zf = (double)(zpos_fine >> 8);
-    m.mzz = depth_of_code(P, zf, 8388351.5);
-    m.step(m.mzz);
-    m.update_de_stop(P);
-    if (!ao) {
-        double d = m.calc_de(P, cnt);
-        const bool de_limited = m.it_result < m.max_its_result || d < (double)m.ms_de_stop;
-        if (de_limited)
-            m.bin_search(P, d, depth_of_code(P, zf, 8388351.9) - m.mzz, cnt);
-        else
-            m.bin_search_it(P, P.de_add_steps, cnt);
-        return;
-    }
-    double d = fmin(m.calc_de(P, cnt), (double)m.ms_de_stop * 2.0);
-    if (m.it_result < m.max_its_result || d < (double)m.ms_de_stop) {
-        double dt1 = depth_of_code(P, zf, 8388351.9) - m.mzz;
-        int itmp = 5;
-        while (itmp > 0 && fabs(d - (double)m.ms_de_stop) > 0.004) {
-            m.mzz += dt1;
-            m.step(dt1);
-            m.update_de_stop(P);
-            itmp -= 1;
-            if (itmp > 0) {
-                d = m.calc_de(P, cnt);
-                dt1 = d < (double)m.ms_de_stop ? fabs(dt1) * -0.55 : fabs(dt1) * 0.55;
-            }
-        }
-    } else {
-        m.bin_search_it(P, 5, cnt);
-        m.ms_de_stop = (float)((double)P.de_stop * (1.0 + fabs(m.mzz) * (double)P.de_stop_factor));
-    }
-}
-
-// The DE of the bulk precision, or f64 where the decision is close: DE
-// within the guard band of the DE stop, or at the iteration limit.
+// The DE of a post ray: the bulk precision, and f64 where the decision is
+// close (within the guard band of the DE stop, at the iteration limit, or an
+// FP32 overflow), so that no DE site is duplicated per state.
 template <int KIND, int DEM, bool F32>
-__device__ double banded_de(const DevP &P, M64 &m, float ms, int &it, int &max_its, Cnt &cnt) {
-    if (F32) {
-        const float d32 = de32<KIND, DEM>(P, m.c, ms, it, cnt);
-        if (d32 < __builtin_inff() && it < P.max_it && (double)d32 > (double)ms * (double)P.guard) {
-            max_its = P.max_it;
-            return (double)d32;
+__device__ __forceinline__ void post_de(const DevP &P, Ray64 &R, bool want64, Cnt &cnt) {
+    if (F32 && !want64) {
+        int it = 0;
+        const float d32 = de32<KIND, DEM>(P, R.c, R.ms, it, cnt);
+        if (d32 < __builtin_inff() && it < R.max_it && d32 > R.ms * P.guard) {
+            R.de = (double)d32;
+            R.it = it;
+            R.max_its = R.max_it;
+            return;
         }
+        if (!(d32 < __builtin_inff())) cnt.overflow++;
     }
-    m.ms_de_stop = ms;
-    const double d = m.calc_de(P, cnt);
-    it = m.it_result;
-    max_its = m.max_its_result;
-    return d;
+    R.take(de64(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
 }
+
+// hs_surface_start / surface_point: the start of the f64 surface point of a
+// G-buffer pixel from its depth code (src/calc.rs:2741-2753, 2163-2171)
+__device__ __forceinline__ void ray_from_code(const DevP &P, Ray64 &R, int x, int y, uint32_t zpos_fine) {
+    R.reset(P);
+    pixel_ray(P, x, y, R.c, R.v);
+    const double zf = (double)(zpos_fine >> 8);
+    R.mzz = depth_of_code(P, zf, 8388351.5);
+    for (int k = 0; k < 3; k++) R.c[k] += R.v[k] * R.mzz;
+    R.upd(P);
+}
+
+enum ShPhase : int { SH_SURF, SH_REF, SH_D0, SH_DS };
 
 template <int KIND, int DEM, bool F32>
 __global__ __launch_bounds__(256) void k_shadow(DevP P, Bufs B, ShadowJob J) {
     Cnt cnt;
-    const long long n = (long long)B.ctr[C_HITS];
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    uint16_t clear_mask = 0;
-    for (int l = 0; l < J.n; l++) clear_mask |= (uint16_t)J.bit[l];
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int pix = B.hits[i];
-        uint32_t *w = B.gbuf + 5 * (size_t)pix;
-        uint32_t shadow = (w[1] >> 16) & ~(uint32_t)clear_mask;
-        const int x = pix % P.width, y = P.row0 + pix / P.width;
-        if ((w[2] >> 16) >= 32768u || (w[3] & 0xFFFF) >= 32768u) continue;
-        const double nv0 = (double)(int16_t)(w[0] & 0xFFFF) / 32767.0;
-        const double nv1 = (double)(int16_t)(w[0] >> 16) / 32767.0;
-        const double nv2 = (double)(int16_t)(w[1] & 0xFFFF) / 32767.0;
-        double nvec[3];
-        for (int k = 0; k < 3; k++) nvec[k] = nv0 * P.vg[0][k] + nv1 * P.vg[1][k] + nv2 * P.vg[2][k];
-        M64 m;
-        m.init(P);
-        surface_from_code(P, m, x, y, w[2], false, cnt);
-        m.mzz = fmax(m.mzz - 0.1, 0.0);
-        m.step(-0.1);
-        m.update_de_stop(P);
-        const double ic[3] = {m.c[0], m.c[1], m.c[2]};
-        // hs_max_len (src/calc.rs:2767-2775)
-        const float mz = (float)fmin(m.mzz, P.zend * 0.4);
-        const double max_lhs = (double)((float)(P.width + y) * 0.6f *
-                                        (1.0f + 0.5f * mz * (float)fmax(P.fov_y, 0.0) / (float)P.height) *
-                                        J.max_len_mul);
-        const double vv = m.vfov[0] * m.vfov[0] + m.vfov[1] * m.vfov[1] + m.vfov[2] * m.vfov[2];
-        for (int l = 0; l < J.n; l++) {
-            const double *lv = J.vec[l];
-            m.c[0] = ic[0];
-            m.c[1] = ic[1];
-            m.c[2] = ic[2];
-            double zz2 = m.mzz;
-            const double ll = lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2];
-            const double zz2mul =
-                -((lv[0] * m.vfov[0] + lv[1] * m.vfov[1] + lv[2] * m.vfov[2]) / fmax(sqrt(ll * vv), 1e-300));
-            if (nvec[0] * lv[0] + nvec[1] * lv[1] + nvec[2] * lv[2] > 0.0) {
-                shadow |= (uint32_t)J.bit[l];
-                continue;
-            }
-            double dt1 = max_lhs;
-            if (dt1 > 0.0) {
-                float ms = (float)((double)P.de_stop * (1.0 + fabs(zz2) * (double)P.de_stop_factor));
-                float rsf = 2.0f;
-                int it = 0, max_its = P.max_it;
-                double d = banded_de<KIND, DEM, F32>(P, m, ms, it, max_its, cnt);
-                for (int guard = 1; guard <= 1000000; guard++) {
-                    const double last_de = d;
-                    const float a = fmaxf(0.11f, (float)((d - (double)P.ms_de_sub * (double)ms) *
-                                                         (double)P.z_step_div * (double)rsf));
-                    const float b = fmaxf(ms, 0.4f) * P.mh04zsd;
-                    const double st = (double)fminf(a, b);
-                    dt1 -= st;
-                    m.c[0] += lv[0] * -st;
-                    m.c[1] += lv[1] * -st;
-                    m.c[2] += lv[2] * -st;
-                    zz2 += st * zz2mul;
-                    ms = (float)((double)P.de_stop * (1.0 + fabs(zz2) * (double)P.de_stop_factor));
-                    d = banded_de<KIND, DEM, F32>(P, m, ms, it, max_its, cnt);
-                    cnt.steps++;
-                    if (it >= max_its || d <= (double)ms) break;
-                    if (d > last_de + st) d = last_de + st;
-                    if (last_de > d + 1e-30) {
-                        const float t = (float)(st / (last_de - d));
-                        rsf = t < 1.0f ? fmaxf(t, 0.5f) : 1.0f;
-                    } else {
-                        rsf = 1.0f;
-                    }
-                    if (dt1 < 0.0) break;
+    const long long n_items = (long long)B.ctr[C_HITS];
+    uint32_t clear_mask = 0;
+    for (int l = 0; l < J.n; l++) clear_mask |= (uint32_t)J.bit[l];
+    bool have = false, exhausted = false;
+    Ray64 R;
+    Refine F;
+    int pix = 0, phase = SH_SURF, l = 0, guard = 0, y = 0;
+    uint32_t sh = 0, zcode = 0;
+    double nvec[3] = {0, 0, 0}, ic[3] = {0, 0, 0};
+    double max_lhs = 0, zz2 = 0, zz2mul = 0, dt1 = 0, d = 0, last_de = 0, st = 0;
+    float rsf = 2.0f;
+    for (;;) {
+        const bool need = !have && !exhausted;
+        const long long slot = wave_append(need, &B.ctr[C_POSTQ]);
+        if (need) {
+            if (slot >= n_items) {
+                exhausted = true;
+            } else {
+                pix = B.hits[slot];
+                const uint32_t *w = B.gbuf + 5 * (size_t)pix;
+                if ((w[2] >> 16) < 32768u && (w[3] & 0xFFFF) < 32768u) {
+                    have = true;
+                    sh = (w[1] >> 16) & ~clear_mask;
+                    const double n0 = (double)(int16_t)(w[0] & 0xFFFF) / 32767.0;
+                    const double n1 = (double)(int16_t)(w[0] >> 16) / 32767.0;
+                    const double n2 = (double)(int16_t)(w[1] & 0xFFFF) / 32767.0;
+                    for (int k = 0; k < 3; k++) nvec[k] = n0 * P.vg[0][k] + n1 * P.vg[1][k] + n2 * P.vg[2][k];
+                    const int x = pix % P.width;
+                    y = P.row0 + pix / P.width;
+                    zcode = w[2];
+                    ray_from_code(P, R, x, y, zcode);
+                    phase = SH_SURF;
                 }
             }
-            if (dt1 > 0.0) shadow |= (uint32_t)J.bit[l];
         }
-        w[1] = (w[1] & 0xFFFFu) | (shadow << 16);
+        if (__all(exhausted)) break;
+        if (!have) continue;
+        post_de<KIND, DEM, F32>(P, R, phase == SH_SURF || phase == SH_REF, cnt);
+        bool done = false;
+        switch (phase) {
+            case SH_SURF: {
+                const bool de_limited = R.it < R.max_its || R.de < (double)R.ms;
+                const double zf = (double)(zcode >> 8);
+                const bool more = de_limited ? F.start_bs(R, P, R.de, depth_of_code(P, zf, 8388351.9) - R.mzz,
+                                                          P.de_add_steps)
+                                             : F.start_bsi(R, P.de_add_steps);
+                if (more) {
+                    phase = SH_REF;
+                    goto end;
+                }
+                goto start;
+            }
+            case SH_REF:
+                if (F.after(R, P)) goto end;
+                goto start;
+            case SH_D0:
+                d = R.de;
+                goto head;
+            default:  // SH_DS
+                d = R.de;
+                if (R.it >= R.max_its || d <= (double)R.ms) goto endl;
+                if (d > last_de + st) d = last_de + st;
+                if (last_de > d + 1e-30) {
+                    const float t = (float)(st / (last_de - d));
+                    rsf = t < 1.0f ? fmaxf(t, 0.5f) : 1.0f;
+                } else {
+                    rsf = 1.0f;
+                }
+                if (dt1 < 0.0) goto endl;
+                goto head;
+        }
+    start: {
+        // hard_shadow_row from the surface point (src/calc.rs:2793-2800)
+        R.mzz = fmax(R.mzz - 0.1, 0.0);
+        for (int k = 0; k < 3; k++) R.c[k] += R.v[k] * -0.1;
+        R.upd(P);
+        for (int k = 0; k < 3; k++) ic[k] = R.c[k];
+        // hs_max_len (src/calc.rs:2767-2775)
+        const float mz = (float)fmin(R.mzz, P.zend * 0.4);
+        max_lhs = (double)((float)(P.width + y) * 0.6f *
+                           (1.0f + 0.5f * mz * (float)fmax(P.fov_y, 0.0) / (float)P.height) * J.max_len_mul);
+        l = 0;
+    }
+    light:
+        if (l >= J.n) goto write;
+        {
+            const double *lv = J.vec[l];
+            for (int k = 0; k < 3; k++) R.c[k] = ic[k];
+            zz2 = R.mzz;
+            const double ll = lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2];
+            const double vv = R.v[0] * R.v[0] + R.v[1] * R.v[1] + R.v[2] * R.v[2];
+            zz2mul = -((lv[0] * R.v[0] + lv[1] * R.v[1] + lv[2] * R.v[2]) / fmax(sqrt(ll * vv), 1e-300));
+            if (nvec[0] * lv[0] + nvec[1] * lv[1] + nvec[2] * lv[2] > 0.0) {
+                sh |= (uint32_t)J.bit[l];
+                l++;
+                goto light;
+            }
+            dt1 = max_lhs;
+            if (!(dt1 > 0.0)) goto endl;
+            R.ms = (float)((double)P.de_stop * (1.0 + fabs(zz2) * (double)P.de_stop_factor));
+            rsf = 2.0f;
+            guard = 0;
+            phase = SH_D0;
+            goto end;
+        }
+    head:
+        if (++guard > 1000000) goto endl;
+        {
+            const double *lv = J.vec[l];
+            last_de = d;
+            const float a = fmaxf(0.11f, (float)((d - (double)P.ms_de_sub * (double)R.ms) * (double)P.z_step_div *
+                                                 (double)rsf));
+            const float b = fmaxf(R.ms, 0.4f) * P.mh04zsd;
+            st = (double)fminf(a, b);
+            dt1 -= st;
+            for (int k = 0; k < 3; k++) R.c[k] += lv[k] * -st;
+            zz2 += st * zz2mul;
+            R.ms = (float)((double)P.de_stop * (1.0 + fabs(zz2) * (double)P.de_stop_factor));
+            cnt.steps++;
+            phase = SH_DS;
+            goto end;
+        }
+    endl:
+        if (dt1 > 0.0) sh |= (uint32_t)J.bit[l];
+        l++;
+        goto light;
+    write: {
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        w[1] = (w[1] & 0xFFFFu) | (sh << 16);
+        done = true;
+    }
+    end:
+        if (done) have = false;
     }
     flush_counters(cnt, B.ctr);
 }
@@ -656,104 +776,157 @@ __device__ void normal_matrix(const float n[3], float mo[3][3]) {
     mo[2][2] = (float)(1.0 - 2.0 * (q0 * q0 + q1 * q1));
 }
 
+enum AoPhase : int { AO_SURF, AO_REF, AO_D };
+
 template <int KIND, int DEM, bool F32>
-__global__ __launch_bounds__(256) void k_deao(DevP P, Bufs B, AoRays R) {
+__global__ __launch_bounds__(256) void k_deao(DevP P, Bufs B, AoRays A) {
+    __shared__ float s_min_ra[256 * 33];
+    float *min_ra = s_min_ra + threadIdx.x;  // min_ra[k * 256]
     Cnt cnt;
-    const long long n = (long long)B.ctr[C_HITS];
-    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long n_items = (long long)B.ctr[C_HITS];
     const double sw = P.step_width;
     const double s_max_d =
-        (double)R.max_len * 0.5 * sqrt((double)P.height * (double)P.height + (double)P.width * (double)P.width);
+        (double)A.max_len * 0.5 * sqrt((double)P.height * (double)P.height + (double)P.width * (double)P.width);
     const double de_stop = (double)P.de_stop;
-    const double dsm = (double)R.d_step_mul;
-    const int rc = R.count;
-    const double de_mul0 = sqrt((double)rc * 0.5);
-    const double abr = 1.2 / asin(fmin(fmax(1.0 / de_mul0, -1.0), 1.0));
-    // rows of the camera matrix scaled to the step width (normalise_matrix_to)
+    const double dsm = (double)A.d_step_mul;
+    const int rc = A.count;
+    const double de_mul = sqrt((double)rc * 0.5);
+    const double abr = 1.2 / asin(fmin(fmax(1.0 / de_mul, -1.0), 1.0));
+    // the camera rows scaled to the step width (normalise_matrix_to)
     double rot_v[3][3];
     for (int i = 0; i < 3; i++) {
-        const double d = sw / sqrt(P.vg[i][0] * P.vg[i][0] + P.vg[i][1] * P.vg[i][1] + P.vg[i][2] * P.vg[i][2]);
-        for (int j = 0; j < 3; j++) rot_v[i][j] = P.vg[i][j] * d;
+        const double dd = sw / sqrt(P.vg[i][0] * P.vg[i][0] + P.vg[i][1] * P.vg[i][1] + P.vg[i][2] * P.vg[i][2]);
+        for (int j = 0; j < 3; j++) rot_v[i][j] = P.vg[i][j] * dd;
     }
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int pix = B.hits[i];
-        uint32_t *w = B.gbuf + 5 * (size_t)pix;
-        const int x = pix % P.width, y = P.row0 + pix / P.width;
-        if ((w[2] >> 16) >= 32768u || (w[3] & 0xFFFF) >= 32768u) {
-            w[4] = (w[4] & ~0xFFFFu);
-            continue;
-        }
-        M64 m;
-        m.init(P);
-        surface_from_code(P, m, x, y, w[2], true, cnt);
-        double ms_de_stop = fmin((double)m.ms_de_stop, 1e6);
-        const double ic[3] = {m.c[0], m.c[1], m.c[2]};
-        const double step_ao = ms_de_stop / de_stop;
-        const double max_dist = s_max_d * sqrt(step_ao);
-        ms_de_stop = P.de_stop_factor == 0.0f ? de_stop / (dsm * dsm) : ms_de_stop / (dsm * dsm);
-        const double md_d10 = 0.1 / (max_dist * de_mul0);
-        const float nf[3] = {(float)(int16_t)(w[0] & 0xFFFF), (float)(int16_t)(w[0] >> 16),
-                             (float)(int16_t)(w[1] & 0xFFFF)};
-        const float nl = fmaxf(sqrtf(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]), 1e-30f);
-        const float nu[3] = {nf[0] / nl, nf[1] / nl, nf[2] / nl};
-        float rw[3][3];
-        normal_matrix(nu, rw);
-        float min_ra[33];
-        for (int k = 0; k < rc; k++) {
-            const float *d = R.dir[k];
-            const float vv0 = rw[0][0] * d[0] + rw[0][1] * d[1] + rw[0][2] * d[2];
-            const float vv1 = rw[1][0] * d[0] + rw[1][1] * d[1] + rw[1][2] * d[2];
-            const float vv2 = rw[2][0] * d[0] + rw[2][1] * d[1] + rw[2][2] * d[2];
-            double sv[3];
-            for (int q = 0; q < 3; q++)
-                sv[q] = (double)vv0 * rot_v[0][q] + (double)vv1 * rot_v[1][q] + (double)vv2 * rot_v[2][q];
-            double dt1 = step_ao * dsm;
-            double s_tmp = 1.0;
-            bool b_end = false;
-            for (int guard = 0; guard < 100000; guard++) {
-                if (dt1 > max_dist) {
-                    dt1 = max_dist;
-                    b_end = true;
-                }
-                m.c[0] = ic[0] + sv[0] * dt1;
-                m.c[1] = ic[1] + sv[1] * dt1;
-                m.c[2] = ic[2] + sv[2] * dt1;
-                double dt2;
-                if (F32) {
-                    int it;
-                    const float d32 = de32<KIND, DEM>(P, m.c, (float)ms_de_stop, it, cnt);
-                    dt2 = d32 < __builtin_inff() ? (double)d32 : m.calc_de(P, cnt);
+    bool have = false, exhausted = false;
+    Ray64 R;
+    Refine F;
+    int pix = 0, phase = AO_SURF, k = 0, guard = 0;
+    uint32_t zcode = 0;
+    float rw[3][3];
+    double ic[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+    double ms_ao = 0, step_ao = 0, max_dist = 0, md_d10 = 0, dt1 = 0, s_tmp = 0;
+    bool b_end = false;
+    for (;;) {
+        const bool need = !have && !exhausted;
+        const long long slot = wave_append(need, &B.ctr[C_AOQ]);
+        if (need) {
+            if (slot >= n_items) {
+                exhausted = true;
+            } else {
+                pix = B.hits[slot];
+                uint32_t *w = B.gbuf + 5 * (size_t)pix;
+                if ((w[2] >> 16) < 32768u && (w[3] & 0xFFFF) < 32768u) {
+                    have = true;
+                    zcode = w[2];
+                    const float nf[3] = {(float)(int16_t)(w[0] & 0xFFFF), (float)(int16_t)(w[0] >> 16),
+                                         (float)(int16_t)(w[1] & 0xFFFF)};
+                    const float nl = fmaxf(sqrtf(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]), 1e-30f);
+                    const float nu[3] = {nf[0] / nl, nf[1] / nl, nf[2] / nl};
+                    normal_matrix(nu, rw);
+                    ray_from_code(P, R, pix % P.width, P.row0 + pix / P.width, zcode);
+                    phase = AO_SURF;
                 } else {
-                    dt2 = m.calc_de(P, cnt);
+                    w[4] = w[4] & ~0xFFFFu;
                 }
-                cnt.steps++;
-                s_tmp = fmin(s_tmp, (dt2 - ms_de_stop + dt1 * md_d10) / dt1);
-                if (s_tmp < 0.02) break;
-                dt1 += fmax(dt2, dt1 * dsm);
-                if (b_end) break;
             }
-            min_ra[k] = (float)(fmax(s_tmp, 0.0) * de_mul0);
         }
+        if (__all(exhausted)) break;
+        if (!have) continue;
+        post_de<KIND, DEM, F32>(P, R, phase != AO_D, cnt);
+        bool done = false;
+        switch (phase) {
+            case AO_SURF: {
+                // surface_point (src/calc.rs:2172-2188): 5 steps to |DE - stop| <= 0.004
+                const double dd = fmin(R.de, (double)R.ms * 2.0);
+                const double zf = (double)(zcode >> 8);
+                bool more;
+                if (R.it < R.max_its || dd < (double)R.ms)
+                    more = F.start_sp(R, P, dd, depth_of_code(P, zf, 8388351.9) - R.mzz, 5, 0.004);
+                else
+                    more = F.start_bsi(R, 5);
+                if (more) {
+                    phase = AO_REF;
+                    goto end;
+                }
+                goto refined;
+            }
+            case AO_REF:
+                if (F.after(R, P)) goto end;
+                goto refined;
+            default: {  // AO_D
+                const double dt2 = R.de;
+                s_tmp = fmin(s_tmp, (dt2 - ms_ao + dt1 * md_d10) / dt1);
+                if (s_tmp < 0.02) goto ray_end;
+                dt1 += fmax(dt2, dt1 * dsm);
+                if (b_end || ++guard >= 100000) goto ray_end;
+                goto ray_step;
+            }
+        }
+    refined:
+        if (F.kind == RK_BSI)
+            R.ms = (float)((double)P.de_stop * (1.0 + fabs(R.mzz) * (double)P.de_stop_factor));
+        // deao_row after surface_point (src/deao.rs:303-345)
+        ms_ao = fmin((double)R.ms, 1e6);
+        for (int q = 0; q < 3; q++) ic[q] = R.c[q];
+        step_ao = ms_ao / de_stop;
+        max_dist = s_max_d * sqrt(step_ao);
+        ms_ao = P.de_stop_factor == 0.0f ? de_stop / (dsm * dsm) : ms_ao / (dsm * dsm);
+        md_d10 = 0.1 / (max_dist * de_mul);
+        k = 0;
+    ray:
+        if (k >= rc) goto write;
+        {
+            const float *dr = A.dir[k];
+            const float v0 = rw[0][0] * dr[0] + rw[0][1] * dr[1] + rw[0][2] * dr[2];
+            const float v1 = rw[1][0] * dr[0] + rw[1][1] * dr[1] + rw[1][2] * dr[2];
+            const float v2 = rw[2][0] * dr[0] + rw[2][1] * dr[1] + rw[2][2] * dr[2];
+            for (int q = 0; q < 3; q++)
+                sv[q] = (double)v0 * rot_v[0][q] + (double)v1 * rot_v[1][q] + (double)v2 * rot_v[2][q];
+            dt1 = step_ao * dsm;
+            s_tmp = 1.0;
+            b_end = false;
+            guard = 0;
+        }
+    ray_step:
+        if (dt1 > max_dist) {
+            dt1 = max_dist;
+            b_end = true;
+        }
+        for (int q = 0; q < 3; q++) R.c[q] = ic[q] + sv[q] * dt1;
+        cnt.steps++;
+        phase = AO_D;
+        goto end;
+    ray_end:
+        min_ra[k * 256] = (float)(fmax(s_tmp, 0.0) * de_mul);
+        k++;
+        goto ray;
+    write: {
         double amount = 0.0;
-        for (int k = 0; k < rc; k++) {
+        for (int a = 0; a < rc; a++) {
             float s_add = 0.0f;
-            if (min_ra[k] < 1.0f) {
-                const float max_add = 1.0f - min_ra[k];
-                for (int k2 = 0; k2 < rc; k2++) {
-                    if (k2 == k) continue;
-                    const float dt = R.dir[k][0] * R.dir[k2][0] + R.dir[k][1] * R.dir[k2][1] + R.dir[k][2] * R.dir[k2][2];
-                    if (dt > R.d_min_a_dif) {
+            const float ma = min_ra[a * 256];
+            if (ma < 1.0f) {
+                const float max_add = 1.0f - ma;
+                for (int b = 0; b < rc; b++) {
+                    if (b == a) continue;
+                    const float dt = A.dir[a][0] * A.dir[b][0] + A.dir[a][1] * A.dir[b][1] + A.dir[a][2] * A.dir[b][2];
+                    if (dt > A.d_min_a_dif) {
                         const float ac = acosf(fminf(fmaxf(dt, -1.0f), 1.0f));
-                        const float ov = min_ra[k2] - (float)((double)ac * abr) + 1.0f;
-                        if (ov > 0.0f) s_add += fminf(max_add, ov) * R.correction_weight;
+                        const float ov = min_ra[b * 256] - (float)((double)ac * abr) + 1.0f;
+                        if (ov > 0.0f) s_add += fminf(max_add, ov) * A.correction_weight;
                     }
                 }
             }
-            amount += (double)fminf(s_add + min_ra[k], 1.0f);
+            amount += (double)fminf(s_add + ma, 1.0f);
         }
-        const double a = rte(16383.0 * (1.0 - amount / (double)rc));
-        const uint32_t amb = (uint32_t)fmax(a, 0.0);
-        w[4] = (w[4] & ~0xFFFFu) | amb;
+        const double av = rte(16383.0 * (1.0 - amount / (double)rc));
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        w[4] = (w[4] & ~0xFFFFu) | (uint32_t)fmax(av, 0.0);
+        done = true;
+    }
+    end:
+        if (done) have = false;
     }
     flush_counters(cnt, B.ctr);
 }
@@ -1097,6 +1270,7 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
             hipLaunchKernelGGL(k_finish64, dim3(grid_flat * 2), dim3(256), 0, c.stream, d, b, 1);
         } else {
             dispatch(kind, dem, [&](auto K, auto D) { launch_march<decltype(K)::value, decltype(D)::value>(grid_march, stream, d, b); });
+            HIP_TRY(hipEventRecord(c.ev[3], c.stream));
             hipLaunchKernelGGL(k_finish64, dim3(grid_flat), dim3(256), 0, c.stream, d, b, 0);
         }
         HIP_TRY(hipGetLastError());
@@ -1154,9 +1328,11 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     unsigned long long ctr[C_WORDS];
     HIP_TRY(hipMemcpyAsync(ctr, c.d_ctr, sizeof ctr, hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(hipStreamSynchronize(c.stream));
-    float ms_march = 0, ms_post = 0;
+    float ms_march = 0, ms_post = 0, ms_a = 0;
     HIP_TRY(hipEventElapsedTime(&ms_march, c.ev[0], c.ev[1]));
     HIP_TRY(hipEventElapsedTime(&ms_post, c.ev[1], c.ev[2]));
+    const bool split = !post_only && !f64;
+    if (split) HIP_TRY(hipEventElapsedTime(&ms_a, c.ev[0], c.ev[3]));
     mb3d_hip_counters &t = t_counters;
     std::memset(&t, 0, sizeof t);
     t.pixels = npix;
@@ -1171,6 +1347,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     t.overflow = ctr[C_STATS + 8];
     t.kernel_ms = ms_march;
     t.post_ms = ms_post;
+    t.march_f32_ms = split ? ms_a : 0.0;
+    t.finish_f64_ms = split ? ms_march - ms_a : ms_march;
     if (progress) progress(user, 2, 2);
     // the overflow verdict (src/gpu.rs:2350-2375): the f64 finish already
     // recomputed those pixels, so it is reported, not failed

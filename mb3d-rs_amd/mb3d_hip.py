@@ -192,6 +192,8 @@ class Counters(C.Structure):
         ("overflow", C.c_uint64),
         ("kernel_ms", C.c_double),
         ("post_ms", C.c_double),
+        ("march_f32_ms", C.c_double),
+        ("finish_f64_ms", C.c_double),
     ]
 
 

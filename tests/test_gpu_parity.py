@@ -1,0 +1,128 @@
+"""Parity of the HIP marcher (through the C ABI) with the f64 CPU oracle on
+the same seeded scenes.  Tolerances (BASELINE): mask agreement >= 99.9 %,
+relative depth error <= 1e-4 of the scene scale (bulb 2, box 4), normal
+error < 0.5 degrees, iteration counts equal away from silhouettes.  The
+depth, normal and iteration bars are applied at the 99th percentile of the
+pixels both sides hit (the max is reported in the failure text); the FP64
+mode must match at the max."""
+import numpy as np
+import pytest
+
+import mb3d_hip as H
+import oracle as O
+import parity
+
+pytestmark = pytest.mark.gpu
+
+
+def run(hip, sc, scale, opts=None, post=None):
+    p = hip.params_from_scene(sc)
+    g = hip.march(p, post=post, opts=opts)
+    po = O.params(sc)
+    r = O.render(po, post, threads=0)
+    m = parity.compare(g, r, p.width, p.height, p.step_width, scale)
+    return p, g, r, m
+
+
+def check(m, iters_eq=0.995):
+    assert m["mask_agree"] >= 0.999, m
+    assert m["depth_rel_p99"] <= 1e-4, m
+    assert m["normal_deg_p99"] < 0.5, m
+    assert m["iters_equal_inner"] >= iters_eq, m
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_bulb_analytic_640x480(hip, seed):
+    _, _, _, m = run(hip, H.scene_bulb(640, 480, seed=seed), 2.0)
+    check(m)
+    assert m["depth_rel_max"] <= 1e-4 and m["normal_deg_max"] < 0.5, m
+
+
+def test_bulb_numeric_1080p_band(hip):
+    sc = H.scene_bulb(1920, 1080, seed=2, de_mode=H.DE_REFERENCE)
+    p = hip.params_from_scene(sc)
+    g = hip.march(p, row0=400, rows=96)
+    r = O.render(O.params(sc), row0=400, rows=96, threads=0)
+    m = parity.compare(g, r, p.width, 96, p.step_width, 2.0)
+    check(m)
+
+
+def test_box_1080p_band(hip):
+    sc = H.scene_box(1920, 1080, seed=1)
+    p = hip.params_from_scene(sc)
+    g = hip.march(p, row0=500, rows=64)
+    r = O.render(O.params(sc), row0=500, rows=64, threads=0)
+    m = parity.compare(g, r, p.width, 64, p.step_width, 4.0)
+    check(m)
+
+
+@pytest.mark.parametrize("name", ["bulb-a", "bulb-n", "box", "hybrid"])
+def test_f64_mode_matches_at_the_max(hip, name):
+    sc = {"bulb-a": H.scene_bulb(160, 120, seed=1), "bulb-n": H.scene_bulb(160, 120, seed=1, de_mode=0),
+          "box": H.scene_box(160, 90, seed=1), "hybrid": H.scene_hybrid(160, 120, seed=1)}[name]
+    _, g, r, m = run(hip, sc, 2.0, opts=H.options(precision=H.PREC_F64))
+    assert m["mask_agree"] == 1.0 and m["depth_rel_max"] < 1e-9 and m["normal_deg_max"] < 1e-3, m
+    assert m["iters_equal_inner"] == 1.0
+    hit = np.isfinite(r["depth"])
+    # the G-buffer words: normal, depth code, gradient equal but for rounding at ties
+    same = (g["gbuf"][hit] == r["gbuf"][hit]).all(axis=1).mean()
+    assert same > 0.99, same
+
+
+def test_shadows_match_the_oracle(hip):
+    sc = H.scene_bulb(320, 240, seed=2)
+    p = hip.params_from_scene(sc)
+    job = H.post_job(lights=[(0.6, -0.5, -0.6)], params=p)
+    g = hip.march(p, post=job)
+    r = O.render(O.params(sc), job, threads=0)
+    surf = np.isfinite(r["depth"]) & (r["depth"] > 0) & np.isfinite(g["depth64"]) & (g["depth64"] > 0)
+    sg = (g["gbuf"][surf, 1] >> 16) & 0x400
+    so = (r["gbuf"][surf, 1] >> 16) & 0x400
+    assert (so != 0).any()
+    assert (sg == so).mean() >= 0.995
+
+
+def test_deao_matches_the_oracle(hip):
+    sc = H.scene_box(320, 180, seed=1)
+    p = hip.params_from_scene(sc)
+    job = H.post_job(deao_quality=4)
+    g = hip.march(p, post=job)
+    r = O.render(O.params(sc), job, threads=0)
+    surf = np.isfinite(r["depth"]) & (r["depth"] > 0) & np.isfinite(g["depth64"]) & (g["depth64"] > 0)
+    ag = (g["gbuf"][surf, 4] & 0xFFFF).astype(np.float64) / 16383.0
+    ao = (r["gbuf"][surf, 4] & 0xFFFF).astype(np.float64) / 16383.0
+    # the reference's own card test: mean |dAO| < 0.01, < 1 % off by > 10 %
+    assert np.abs(ag - ao).mean() < 0.01
+    assert (np.abs(ag - ao) > 0.1).mean() < 0.01
+
+
+def test_post_on_a_gbuffer_marched_elsewhere(hip):
+    """mb3d_hip_post (gpu::post_on_card) on the oracle's G-buffer."""
+    sc = H.scene_bulb(160, 120, seed=3)
+    p = hip.params_from_scene(sc)
+    job = H.post_job(lights=[(0.3, -0.7, -0.5)], params=p)
+    po = O.params(sc)
+    r = O.render(po, None, threads=0)
+    gp = hip.post(p, job, r["gbuf"])
+    ro = O.post(po, job, r["gbuf"])
+    assert (((gp[:, 1] >> 16) & 0x400) == ((ro[:, 1] >> 16) & 0x400)).mean() >= 0.995
+
+
+def test_rows_band_equals_the_whole_picture(hip):
+    sc = H.scene_bulb(200, 150, seed=4)
+    p = hip.params_from_scene(sc)
+    whole = hip.march(p)
+    band = hip.march(p, row0=50, rows=40)
+    assert np.array_equal(band["gbuf"], whole["gbuf"][50 * 200:90 * 200])
+
+
+def test_counters_count_the_work(hip):
+    sc = H.scene_bulb(160, 120, seed=1)
+    p = hip.params_from_scene(sc)
+    g = hip.march(p)
+    c = g["counters"]
+    assert c["pixels"] == 160 * 120
+    assert c["hits"] == int((np.isfinite(g["depth"]) & (g["depth"] > 0)).sum())
+    r = O.render(O.params(sc), threads=0)
+    # the oracle's f64 march takes the same steps within a few per cent
+    assert abs(c["steps"] - r["counters"]["steps"]) / r["counters"]["steps"] < 0.05
