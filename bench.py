@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--de", choices=["analytic", "numeric"], default="analytic")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also copy a 512 MiB device buffer once (the HBM counter scale of a profile)")
     return ap.parse_args()
 
 
@@ -82,6 +84,12 @@ def main():
         H.check(st, "mb3d_hip_march")
         return H.counters()
 
+    if a.copy_calibration:
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()

@@ -171,7 +171,7 @@ typedef struct mb3d_hip_out {
 typedef struct mb3d_hip_options {
     int32_t precision;     /* MB3D_HIP_PREC_* */
     int32_t tile_w, tile_h; /* pixel tile of one wave's work item (default 8 x 8) */
-    int32_t waves_per_cu;  /* persistent grid: waves per CU (default 8) */
+    int32_t waves_per_cu;  /* persistent grid: waves per CU (default 16) */
     int32_t it_split;      /* extension: march steps per resident turn before a ray yields (0: none) */
     float guard_band;      /* FP32 stop decisions within (1 + guard_band) x DE stop go to FP64 (default 0.25) */
     int32_t reserved[4];
@@ -240,6 +240,12 @@ int32_t mb3d_hip_march(const mb3d_hip_params *params, const mb3d_hip_post_job *p
 int32_t mb3d_hip_post(const mb3d_hip_params *params, const mb3d_hip_post_job *post,
                       int32_t row0, int32_t rows, uint32_t *gbuf, int32_t on_device,
                       const mb3d_hip_options *options);
+
+/* calc::de_at (src/calc.rs:2695-2703) at `n` points (3 doubles each, host
+ * memory): the f64 DE in world units and the iteration count there.  Either
+ * output may be NULL. */
+int32_t mb3d_hip_de_at(const mb3d_hip_params *params, const double *points, int64_t n, double *de,
+                       int32_t *its);
 
 /* The work counters of the last march/post call on this thread. */
 int32_t mb3d_hip_counters_get(mb3d_hip_counters *out);

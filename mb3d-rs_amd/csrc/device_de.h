@@ -189,11 +189,13 @@ __device__ float de32_numeric(const DevP &P, const double c[3], float ms, int &i
 // The analytic DEs in FP32: Amazing Box |z| / |dr| (option 11,
 // src/iteration.rs:562-568) and the power-8 extension 0.5 r ln r / dr.
 template <int DEM>
-__device__ float de32_analytic(const DevP &P, const double c[3], float ms, int &it_out, Cnt &cnt) {
+__device__ float de32_analytic(const DevP &P, const double c[3], float ms, int &it_out, Cnt &cnt,
+                               float *otrap = nullptr) {
 #pragma clang fp contract(fast)
     const float jx = (float)c[0], jy = (float)c[1], jz = (float)c[2];
     float x = jx, y = jy, z = jz;
     float r = x * x + y * y + z * z;
+    float ot = r;
     float w = 1.0f;
     int it = 0;
     if (DEM == DEM_BOX) {
@@ -202,6 +204,7 @@ __device__ float de32_analytic(const DevP &P, const double c[3], float ms, int &
             w *= boxf(x, y, z, jx, jy, jz, fo, sc, m2, sm);
             it++;
             r = x * x + y * y + z * z;
+            if (otrap) ot = fminf(ot, r);
             if (it >= P.max_it || r > P.f_d_rstop) break;
         }
         cnt.box_32 += it;
@@ -213,10 +216,12 @@ __device__ float de32_analytic(const DevP &P, const double c[3], float ms, int &
             p8f(x, y, z, jx, jy, jz, zm8);
             it++;
             r = x * x + y * y + z * z;
+            if (otrap) ot = fminf(ot, r);
             if (it >= P.max_it || r > P.f_d_rstop) break;
         }
         cnt.p8_32 += it;
     }
+    if (otrap) *otrap = ot;
     it_out = it;
     cnt.de32++;
     float res;

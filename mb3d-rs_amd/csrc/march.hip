@@ -111,6 +111,57 @@ __device__ __forceinline__ long long wave_append(bool want, unsigned long long *
     return (long long)(base + __popcll(m & ((1ull << lane_id()) - 1ull)));
 }
 
+// A wave's share of a global counter, taken `chunk` items at a time, so that
+// the lanes of a persistent wave get their next items without an atomic per
+// turn (a device-wide atomic on one address serialises across the XCDs).
+struct WaveChunk {
+    long long base;  // next item of the wave's chunk (the same in every lane)
+    int left;        // items left in the chunk
+    bool dry;        // the counter passed its end
+};
+
+__device__ __forceinline__ void chunk_init(WaveChunk &q) {
+    q.base = 0;
+    q.left = 0;
+    q.dry = false;
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+// Hands the lanes with `want` the next items of a queue of `total` items
+// (`total` < 0: no end, an allocation).  Call with the whole wave active.
+// Returns the lane's item, or -1 (the queue is exhausted).
+__device__ __forceinline__ long long wave_take(bool want, unsigned long long *ctr, WaveChunk &q, int chunk,
+                                               long long total) {
+    long long item = -1;
+    bool pending = want;
+    const int leader = __ffsll((long long)__ballot(true)) - 1;
+    for (;;) {
+        const unsigned long long m = __ballot(pending);
+        if (!m || q.dry) break;
+        if (q.left == 0) {
+            unsigned long long b = 0;
+            if (lane_id() == leader) b = atomicAdd(ctr, (unsigned long long)chunk);
+            b = __shfl(b, leader, 64);
+            if (total >= 0 && (long long)b >= total) {
+                q.dry = true;
+                break;
+            }
+            q.base = (long long)b;
+            q.left = total >= 0 ? (int)min((long long)chunk, total - (long long)b) : chunk;
+        }
+        const int rank = __popcll(m & lanemask_lt());
+        const int take = min(__popcll(m), q.left);
+        if (pending && rank < take) {
+            item = q.base + rank;
+            pending = false;
+        }
+        q.base += take;
+        q.left -= take;
+    }
+    return item;
+}
+
 __device__ void write_background(const DevP &P, const Bufs &B, int pix, float step_count) {
     const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
     if (B.depth) B.depth[pix] = __builtin_inff();
@@ -164,26 +215,26 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
     int32_t seed = 0;
     const float fguard = P.guard;
     const int tile_n = P.tile_w * P.tile_h;
+    WaveChunk wq, cq;
+    chunk_init(wq);
+    chunk_init(cq);
     for (;;) {
-        // refill: every idle lane takes the next work item (one atomic per wave)
-        const bool need = !have && !exhausted;
-        const unsigned long long m = __ballot(need);
-        if (m) {
-            const int leader = __ffsll((long long)m) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&B.ctr[C_QUEUE], (unsigned long long)__popcll(m));
-            base = __shfl(base, leader, 64);
+        // refill: every idle lane takes the next work item from the wave's chunk
+        bool need = !have && !exhausted;
+        for (int tries = 0; tries < 4 && __any(need); tries++) {
+            const long long q = wave_take(need, &B.ctr[C_QUEUE], wq, 256, P.nslots_q);
             if (need) {
-                const unsigned long long q = base + __popcll(m & ((1ull << lane) - 1ull));
-                if ((long long)q >= P.nslots_q) {
+                if (q < 0) {
                     exhausted = true;
+                    need = false;
                 } else {
-                    const long long tile = (long long)q / tile_n;
-                    const int k = (int)((long long)q - tile * tile_n);
+                    const long long tile = q / tile_n;
+                    const int k = (int)(q - tile * tile_n);
                     const int x = (int)(tile % P.tiles_x) * P.tile_w + k % P.tile_w;
                     const int yr = (int)(tile / P.tiles_x) * P.tile_h + k / P.tile_w;
                     if (x < P.width && yr < P.rows) {
                         have = true;
+                        need = false;
                         px = x;
                         py = P.row0 + yr;
                         pix = yr * P.width + x;
@@ -272,7 +323,7 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
                 }
             }
         }
-        const long long slot = wave_append(handoff, &B.ctr[C_CAND]);
+        const long long slot = wave_take(handoff, &B.ctr[C_CAND], cq, 64, -1);
         if (handoff) {
             Cand &o = B.cand[slot];
             o.c[0] = c[0];
@@ -297,6 +348,8 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
             have = false;
         }
     }
+    // the wave's unused hand-off slots are marked empty
+    if (lane_id() < cq.left) B.cand[cq.base + lane_id()].pix = -1;
     flush_counters(cnt, B.ctr);
 }
 
@@ -374,6 +427,10 @@ __device__ void write_inside(const DevP &P, const Bufs &B, int pix, const Ray64 
 
 enum FinPhase : int { FP_START, FP_RESUME, FP_BACK, FP_STEP, FP_REF, FP_NC, FP_NP, FP_NM };
 
+// N32: the six offset DEs of the normal stencil in FP32, for the analytic
+// DEs (their FP32 error moves a normal by ~0.01 degree; the numeric
+// gradient's does not allow it)
+template <int KIND, int DEM, bool N32>
 __global__ __launch_bounds__(256) void k_finish64(DevP P, Bufs B, int all_pixels) {
     Cnt cnt;
     const long long n_items = all_pixels ? P.npix : (long long)B.ctr[C_CAND];
@@ -390,56 +447,70 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P, Bufs B, int all_pixels
     int iters = 0, q = 0, axis = 0;
     double ct[3] = {0, 0, 0}, nv[3] = {0, 0, 0}, noffset = 0, na = 0;
     float nn = 0.0f;
+    WaveChunk fq, hq;
+    chunk_init(fq);
+    chunk_init(hq);
     for (;;) {
-        const bool need = !have && !exhausted;
-        const long long slot = wave_append(need, &B.ctr[C_FINQ]);
-        if (need) {
-            if (slot >= n_items) {
+        bool need = !have && !exhausted;
+        for (int tries = 0; tries < 4 && __any(need); tries++) {
+            const long long slot = wave_take(need, &B.ctr[C_FINQ], fq, 64, n_items);
+            if (!need) continue;
+            if (slot < 0) {
                 exhausted = true;
+                need = false;
+                continue;
+            }
+            if (!all_pixels && B.cand[slot].pix < 0) continue;  // an empty slot of a wave's chunk
+            have = true;
+            need = false;
+            R.reset(P);
+            guard = 0;
+            if (all_pixels) {
+                pix = (int)slot;
+                const int x = pix % P.width, y = P.row0 + pix / P.width;
+                pixel_ray(P, x, y, R.c, R.v);
+                phase = FP_START;
+                first = P.first_step_random != 0;
+                seed = first ? dev_pixel_seed(y, x) : 0;
+                scount = 0.0f;
+                nsteps = 0;
+                rsf = 1.0f;
+                stepped = false;
             } else {
-                have = true;
-                R.reset(P);
-                guard = 0;
-                if (all_pixels) {
-                    pix = (int)slot;
-                    const int x = pix % P.width, y = P.row0 + pix / P.width;
-                    pixel_ray(P, x, y, R.c, R.v);
+                const Cand &o = B.cand[slot];
+                pix = o.pix;
+                for (int k = 0; k < 3; k++) {
+                    R.c[k] = o.c[k];
+                    R.v[k] = o.v[k];
+                }
+                last_step = o.last_step;
+                last_de = o.last_de;
+                rsf = o.rsf;
+                scount = o.step_count;
+                nsteps = o.nsteps;
+                seed = o.seed;
+                first = (o.flags & CF_FIRST) != 0;
+                stepped = (o.flags & CF_STEPPED) != 0;
+                if (o.flags & CF_START) {
                     phase = FP_START;
-                    first = P.first_step_random != 0;
-                    seed = first ? dev_pixel_seed(y, x) : 0;
-                    scount = 0.0f;
-                    nsteps = 0;
-                    rsf = 1.0f;
-                    stepped = false;
                 } else {
-                    const Cand &o = B.cand[slot];
-                    pix = o.pix;
-                    for (int k = 0; k < 3; k++) {
-                        R.c[k] = o.c[k];
-                        R.v[k] = o.v[k];
-                    }
-                    last_step = o.last_step;
-                    last_de = o.last_de;
-                    rsf = o.rsf;
-                    scount = o.step_count;
-                    nsteps = o.nsteps;
-                    seed = o.seed;
-                    first = (o.flags & CF_FIRST) != 0;
-                    stepped = (o.flags & CF_STEPPED) != 0;
-                    if (o.flags & CF_START) {
-                        phase = FP_START;
-                    } else {
-                        R.mzz = o.mzz;
-                        R.upd(P);
-                        phase = FP_RESUME;
-                    }
+                    R.mzz = o.mzz;
+                    R.upd(P);
+                    phase = FP_RESUME;
                 }
             }
         }
         if (__all(exhausted)) break;
-        if (!have) continue;
-        R.take(de64(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
-        bool done = false;
+        bool hit = false, done = false;
+        if (!have) goto end;
+        if (N32 && (phase == FP_NP || phase == FP_NM)) {
+            int it32 = 0;
+            float ot = 0.0f;
+            R.de = (double)de32_analytic<DEM>(P, R.c, R.ms, it32, cnt, &ot);
+            R.otrap = (double)ot;
+        } else {
+            R.take(de64(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+        }
         switch (phase) {
             case FP_START:
                 if (R.it >= R.max_its || R.de < (double)R.ms) {
@@ -569,11 +640,17 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P, Bufs B, int all_pixels
     write:
         write_surface(P, B, pix, R, nv, nn, de_limited, (uint16_t)iters, scount);
         cnt.hits++;
-        B.hits[atomicAdd(&B.ctr[C_HITS], 1ull)] = pix;
+        hit = true;
         done = true;
     end:
         if (done) have = false;
+        const long long hs = wave_take(hit, &B.ctr[C_HITS], hq, 64, -1);
+        if (hit) {
+            B.hits[hs] = pix;
+            hit = false;
+        }
     }
+    if (lane_id() < hq.left) B.hits[hq.base + lane_id()] = -1;
     flush_counters(cnt, B.ctr);
 }
 
@@ -626,28 +703,34 @@ __global__ __launch_bounds__(256) void k_shadow(DevP P, Bufs B, ShadowJob J) {
     double nvec[3] = {0, 0, 0}, ic[3] = {0, 0, 0};
     double max_lhs = 0, zz2 = 0, zz2mul = 0, dt1 = 0, d = 0, last_de = 0, st = 0;
     float rsf = 2.0f;
+    WaveChunk pq;
+    chunk_init(pq);
     for (;;) {
-        const bool need = !have && !exhausted;
-        const long long slot = wave_append(need, &B.ctr[C_POSTQ]);
-        if (need) {
-            if (slot >= n_items) {
+        bool need = !have && !exhausted;
+        for (int tries = 0; tries < 4 && __any(need); tries++) {
+            const long long slot = wave_take(need, &B.ctr[C_POSTQ], pq, 64, n_items);
+            if (!need) continue;
+            if (slot < 0) {
                 exhausted = true;
-            } else {
-                pix = B.hits[slot];
-                const uint32_t *w = B.gbuf + 5 * (size_t)pix;
-                if ((w[2] >> 16) < 32768u && (w[3] & 0xFFFF) < 32768u) {
-                    have = true;
-                    sh = (w[1] >> 16) & ~clear_mask;
-                    const double n0 = (double)(int16_t)(w[0] & 0xFFFF) / 32767.0;
-                    const double n1 = (double)(int16_t)(w[0] >> 16) / 32767.0;
-                    const double n2 = (double)(int16_t)(w[1] & 0xFFFF) / 32767.0;
-                    for (int k = 0; k < 3; k++) nvec[k] = n0 * P.vg[0][k] + n1 * P.vg[1][k] + n2 * P.vg[2][k];
-                    const int x = pix % P.width;
-                    y = P.row0 + pix / P.width;
-                    zcode = w[2];
-                    ray_from_code(P, R, x, y, zcode);
-                    phase = SH_SURF;
-                }
+                need = false;
+                continue;
+            }
+            pix = B.hits[slot];
+            if (pix < 0) continue;
+            need = false;
+            const uint32_t *w = B.gbuf + 5 * (size_t)pix;
+            if ((w[2] >> 16) < 32768u && (w[3] & 0xFFFF) < 32768u) {
+                have = true;
+                sh = (w[1] >> 16) & ~clear_mask;
+                const double n0 = (double)(int16_t)(w[0] & 0xFFFF) / 32767.0;
+                const double n1 = (double)(int16_t)(w[0] >> 16) / 32767.0;
+                const double n2 = (double)(int16_t)(w[1] & 0xFFFF) / 32767.0;
+                for (int k = 0; k < 3; k++) nvec[k] = n0 * P.vg[0][k] + n1 * P.vg[1][k] + n2 * P.vg[2][k];
+                const int x = pix % P.width;
+                y = P.row0 + pix / P.width;
+                zcode = w[2];
+                ray_from_code(P, R, x, y, zcode);
+                phase = SH_SURF;
             }
         }
         if (__all(exhausted)) break;
@@ -807,28 +890,34 @@ __global__ __launch_bounds__(256) void k_deao(DevP P, Bufs B, AoRays A) {
     double ic[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
     double ms_ao = 0, step_ao = 0, max_dist = 0, md_d10 = 0, dt1 = 0, s_tmp = 0;
     bool b_end = false;
+    WaveChunk aq;
+    chunk_init(aq);
     for (;;) {
-        const bool need = !have && !exhausted;
-        const long long slot = wave_append(need, &B.ctr[C_AOQ]);
-        if (need) {
-            if (slot >= n_items) {
+        bool need = !have && !exhausted;
+        for (int tries = 0; tries < 4 && __any(need); tries++) {
+            const long long slot = wave_take(need, &B.ctr[C_AOQ], aq, 64, n_items);
+            if (!need) continue;
+            if (slot < 0) {
                 exhausted = true;
+                need = false;
+                continue;
+            }
+            pix = B.hits[slot];
+            if (pix < 0) continue;
+            need = false;
+            uint32_t *w = B.gbuf + 5 * (size_t)pix;
+            if ((w[2] >> 16) < 32768u && (w[3] & 0xFFFF) < 32768u) {
+                have = true;
+                zcode = w[2];
+                const float nf[3] = {(float)(int16_t)(w[0] & 0xFFFF), (float)(int16_t)(w[0] >> 16),
+                                     (float)(int16_t)(w[1] & 0xFFFF)};
+                const float nl = fmaxf(sqrtf(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]), 1e-30f);
+                const float nu[3] = {nf[0] / nl, nf[1] / nl, nf[2] / nl};
+                normal_matrix(nu, rw);
+                ray_from_code(P, R, pix % P.width, P.row0 + pix / P.width, zcode);
+                phase = AO_SURF;
             } else {
-                pix = B.hits[slot];
-                uint32_t *w = B.gbuf + 5 * (size_t)pix;
-                if ((w[2] >> 16) < 32768u && (w[3] & 0xFFFF) < 32768u) {
-                    have = true;
-                    zcode = w[2];
-                    const float nf[3] = {(float)(int16_t)(w[0] & 0xFFFF), (float)(int16_t)(w[0] >> 16),
-                                         (float)(int16_t)(w[1] & 0xFFFF)};
-                    const float nl = fmaxf(sqrtf(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]), 1e-30f);
-                    const float nu[3] = {nf[0] / nl, nf[1] / nl, nf[2] / nl};
-                    normal_matrix(nu, rw);
-                    ray_from_code(P, R, pix % P.width, P.row0 + pix / P.width, zcode);
-                    phase = AO_SURF;
-                } else {
-                    w[4] = w[4] & ~0xFFFFu;
-                }
+                w[4] = w[4] & ~0xFFFFu;
             }
         }
         if (__all(exhausted)) break;
@@ -929,6 +1018,17 @@ __global__ __launch_bounds__(256) void k_deao(DevP P, Bufs B, AoRays A) {
         if (done) have = false;
     }
     flush_counters(cnt, B.ctr);
+}
+
+// de_at (src/calc.rs:2695-2703) at a list of points: the f64 DE in world
+// units and the iteration count
+__global__ void k_de_at(DevP P, const double *pts, long long n, double *de, int32_t *its) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const De64 r = de64(P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], P.max_it, 0, P.de_stop);
+        if (de) de[i] = r.de * P.step_width;
+        if (its) its[i] = r.it_result;
+    }
 }
 
 // the surface pixels of a G-buffer marched elsewhere
@@ -1144,6 +1244,10 @@ template <int KIND, int DEM>
 void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b) {
     hipLaunchKernelGGL((k_march32<KIND, DEM>), dim3(grid), dim3(256), 0, s, d, b);
 }
+template <int KIND, int DEM, bool N32>
+void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all) {
+    hipLaunchKernelGGL((k_finish64<KIND, DEM, N32>), dim3(grid), dim3(256), 0, s, d, b, all);
+}
 template <int KIND, int DEM, bool F32>
 void launch_shadow(int grid, hipStream_t s, const DevP &d, const Bufs &b, const ShadowJob &j) {
     hipLaunchKernelGGL((k_shadow<KIND, DEM, F32>), dim3(grid), dim3(256), 0, s, d, b, j);
@@ -1261,17 +1365,21 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
         HIP_TRY(hipMemcpyAsync(b.gbuf, post_gbuf, npix * 20, hipMemcpyHostToDevice, c.stream));
     const int kind = kind_of(p), dem = dem_of(p);
     hipStream_t stream = c.stream;
-    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 8;
+    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
     const int grid_march = std::max(1, c.cus * wpc / 4);
     const int grid_flat = std::max(1, c.cus * 4);
     HIP_TRY(hipEventRecord(c.ev[0], c.stream));
     if (!post_only) {
         if (f64) {
-            hipLaunchKernelGGL(k_finish64, dim3(grid_flat * 2), dim3(256), 0, c.stream, d, b, 1);
+            dispatch(kind, dem, [&](auto K, auto D) {
+                launch_finish<decltype(K)::value, decltype(D)::value, false>(grid_flat * 2, stream, d, b, 1);
+            });
         } else {
             dispatch(kind, dem, [&](auto K, auto D) { launch_march<decltype(K)::value, decltype(D)::value>(grid_march, stream, d, b); });
             HIP_TRY(hipEventRecord(c.ev[3], c.stream));
-            hipLaunchKernelGGL(k_finish64, dim3(grid_flat), dim3(256), 0, c.stream, d, b, 0);
+            dispatch(kind, dem, [&](auto K, auto D) {
+                launch_finish<decltype(K)::value, decltype(D)::value, decltype(D)::value != DEM_NUMERIC>(grid_flat, stream, d, b, 0);
+            });
         }
         HIP_TRY(hipGetLastError());
     } else {
@@ -1410,6 +1518,39 @@ extern "C" int32_t mb3d_hip_post(const mb3d_hip_params *params, const mb3d_hip_p
         return march_locked(params, post, row0, rows, nullptr, options, nullptr, nullptr, nullptr, gbuf, on_device);
     } catch (...) {
         return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_post");
+    }
+}
+
+extern "C" int32_t mb3d_hip_de_at(const mb3d_hip_params *params, const double *points, int64_t n, double *de,
+                                  int32_t *its) {
+    try {
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        Ctx &c = g_ctx;
+        if (c.device < 0) return set_error(MB3D_HIP_ERR_NOT_INIT, "mb3d_hip_init was not called");
+        if (!params || !points || n < 0) return set_error(MB3D_HIP_ERR_INVALID, "bad argument");
+        const std::string why = unsupported_reason(params);
+        if (!why.empty()) return set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
+        if (n == 0) return MB3D_HIP_OK;
+        mb3d_hip_options o;
+        std::memset(&o, 0, sizeof o);
+        DevP d;
+        fill_devp(*params, o, 0, params->height, d);
+        const size_t need = (size_t)n * (24 + 8 + 4) + 1024;
+        if (int e = ensure(&c.d_work, &c.work_bytes, need)) return e;
+        char *w = (char *)c.d_work;
+        double *dp = (double *)w;
+        double *dd = (double *)(w + (size_t)n * 24);
+        int32_t *di = (int32_t *)(w + (size_t)n * 32);
+        HIP_TRY(hipMemcpyAsync(dp, points, (size_t)n * 24, hipMemcpyHostToDevice, c.stream));
+        const int grid = (int)std::min<long long>((n + 255) / 256, (long long)c.cus * 8);
+        hipLaunchKernelGGL(k_de_at, dim3(grid), dim3(256), 0, c.stream, d, dp, (long long)n, dd, di);
+        HIP_TRY(hipGetLastError());
+        if (de) HIP_TRY(hipMemcpyAsync(de, dd, (size_t)n * 8, hipMemcpyDeviceToHost, c.stream));
+        if (its) HIP_TRY(hipMemcpyAsync(its, di, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+        HIP_TRY(hipStreamSynchronize(c.stream));
+        return MB3D_HIP_OK;
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_de_at");
     }
 }
 

@@ -221,6 +221,7 @@ SIGNATURES = {
          C.POINTER(Options)],
     ),
     "mb3d_hip_counters_get": (C.c_int32, [C.POINTER(Counters)]),
+    "mb3d_hip_de_at": (C.c_int32, [C.POINTER(Params), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
 }
 
 _lib = None
@@ -473,7 +474,7 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
-def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=8, guard_band=0.25, it_split=0) -> Options:
+def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=16, guard_band=0.25, it_split=0) -> Options:
     o = Options()
     o.precision = precision
     o.tile_w, o.tile_h = tile
@@ -520,3 +521,52 @@ def counters() -> dict:
     c = Counters()
     load().mb3d_hip_counters_get(C.byref(c))
     return {k: getattr(c, k) for k, _ in Counters._fields_}
+
+
+def scene_deepzoom(width, height, seed, marcher, distance=1e-9, fov_deg=0.5) -> Scene:
+    """The deep zoom on the hybrid chain (Mandelbulb p8 x1, Amazing Box
+    scale 2 x2, 45 iterations): a coarse 64 x 48 march of the seeded hybrid
+    camera finds a surface point (a seeded surface pixel); the camera's start
+    plane sits `distance` scene units in front of it along that pixel's ray,
+    FOV `fov_deg`, the step width matched to the angular pixel footprint
+    there.  `marcher(params) -> f64 depth array` runs the coarse march (the
+    same f64 march on either side of a comparison)."""
+    coarse = scene_hybrid(64, 48, seed=seed)
+    coarse.bin_search_steps = 30
+    p = params_from_scene(coarse)
+    depth = np.asarray(marcher(p), np.float64)
+    hits = np.flatnonzero(np.isfinite(depth) & (depth > 0))
+    if hits.size == 0:
+        raise RuntimeError("the coarse march found no surface")
+    rng = np.random.default_rng(seed + 1000)
+    i = int(hits[rng.integers(hits.size)])
+    x, y = i % 64, i // 64
+    vg = np.array([[p.vgrads[r][k] for k in range(3)] for r in range(3)])
+    ystart = np.array(list(p.ystart))
+    c0 = ystart + vg[0] * x + vg[1] * y
+    cafy = (y / p.height - 0.5) * p.fov_y
+    cafx = float(np.float32((np.float32(p.fovx_off) - np.float32(x + 1)) * np.float32(p.fovx_mul)))
+    w = np.array([-math.sin(cafx), math.sin(cafy), math.cos(cafy) * math.cos(cafx)])
+    w /= np.linalg.norm(w)
+    v = w @ vg
+    surface = c0 + depth[i] * v
+    u = v / np.linalg.norm(v)
+    cam = surface - distance * u
+    s = scene_hybrid(width, height, seed=seed)
+    s.fov_y = fov_deg
+    sw = distance * math.radians(fov_deg) / height
+    s.zoom = 2.1345 / (width * sw)
+    s.mid[:] = list(cam)
+    s.z_start = float(cam[2])
+    s.z_end = s.z_start + 4.0 * distance
+    set_camera(s, look_matrix(u))
+    return s
+
+
+def de_at(p: Params, points):
+    """mb3d_hip_de_at: (DE in world units, iteration counts) at points (n x 3)."""
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    de = np.zeros(len(pts))
+    its = np.zeros(len(pts), np.int32)
+    check(load().mb3d_hip_de_at(C.byref(p), _p(pts), len(pts), _p(de), _p(its)), "mb3d_hip_de_at")
+    return de, its

@@ -53,6 +53,8 @@ def compare(gpu: dict, ref: dict, w: int, h: int, step_width: float, scale: floa
         "normal_deg_max": float(ang.max()) if ang.size else 0.0,
         "normal_deg_p99": q(ang, 99),
         "normal_deg_mean": float(ang.mean()) if ang.size else 0.0,
+        "depth_ok_frac": float((rel <= 1e-4).mean()) if rel.size else 1.0,
+        "normal_ok_frac": float((ang < 0.5).mean()) if ang.size else 1.0,
         "iters_equal_inner": float(it_eq.mean()) if it_eq.size else 1.0,
         "inner": int(inner.sum()),
     }
