@@ -167,6 +167,8 @@ typedef struct mb3d_hip_out {
     int32_t reserved;
 } mb3d_hip_out;
 
+typedef struct mb3d_hip_mlp mb3d_hip_mlp;
+
 /* Launch and precision options of one call (zeros: the defaults). */
 typedef struct mb3d_hip_options {
     int32_t precision;     /* MB3D_HIP_PREC_* */
@@ -174,7 +176,14 @@ typedef struct mb3d_hip_options {
     int32_t waves_per_cu;  /* persistent grid: waves per CU (default 16) */
     int32_t it_split;      /* extension: march steps per resident turn before a ray yields (0: none) */
     float guard_band;      /* FP32 stop decisions within (1 + guard_band) x DE stop go to FP64 (default 0.25) */
-    int32_t reserved[4];
+    /* the learned warm start (extension; NULL: the plain march): each ray
+     * starts at the network's proposal when it is confident (hit probability
+     * above mlp_conf_hit: the predicted hit distance less mlp_margin_rel of
+     * it and mlp_margin_abs scene units; below mlp_conf_miss: the predicted
+     * free distance), verified by the exact DE there and started from its
+     * plane again on an overshoot */
+    const mb3d_hip_mlp *mlp;
+    float mlp_conf_hit, mlp_conf_miss, mlp_margin_rel, mlp_margin_abs;
 } mb3d_hip_options;
 
 /* Work counters of the last call on this thread: the algorithmic work that
@@ -188,6 +197,8 @@ typedef struct mb3d_hip_counters {
     uint64_t its_p8_f32, its_box_f32;  /* formula iterations by kind and precision */
     uint64_t its_p8_f64, its_box_f64;
     uint64_t overflow;       /* pixels whose FP32 values overflowed */
+    uint64_t jumps;          /* warm starts taken */
+    uint64_t rollbacks;      /* warm starts rejected by the exact DE (overshoot) */
     double kernel_ms;        /* device time of the march kernels (HIP events) */
     double post_ms;          /* device time of the post kernels */
     double march_f32_ms;     /* of kernel_ms: the FP32 march */
@@ -246,6 +257,53 @@ int32_t mb3d_hip_post(const mb3d_hip_params *params, const mb3d_hip_post_job *po
  * output may be NULL. */
 int32_t mb3d_hip_de_at(const mb3d_hip_params *params, const double *points, int64_t n, double *de,
                        int32_t *its);
+
+/* ---- batches (no reference counterpart in one call: the reference renders
+ * animation frames one by one, a card each, src/frames.rs:467-520) ----
+ * Frames of any sizes with the same formula and DE mode, marched in one
+ * persistent launch of each pass.  The outputs are device buffers holding
+ * the frames one after another (frame f starts at the sum of the earlier
+ * frames' width x height pixels). */
+int32_t mb3d_hip_march_batch(const mb3d_hip_params *frames, int32_t n, const mb3d_hip_out *out,
+                             const mb3d_hip_options *options);
+
+/* A batch captured once into a HIP graph and replayed: create, launch any
+ * number of times (each launch waits for its last replay), destroy. */
+typedef struct mb3d_hip_plan mb3d_hip_plan;
+int32_t mb3d_hip_plan_create(const mb3d_hip_params *frames, int32_t n, const mb3d_hip_out *out,
+                             const mb3d_hip_options *options, mb3d_hip_plan **plan);
+int32_t mb3d_hip_plan_launch(mb3d_hip_plan *plan, int32_t repeats);
+void mb3d_hip_plan_destroy(mb3d_hip_plan *plan);
+
+/* The autotuner: sweeps the tile shape (8x8 .. 32x8) and the persistent grid
+ * (waves per CU) on graphs of the batch and returns the fastest options in
+ * `best` (its other fields are kept) and their kernel time. */
+int32_t mb3d_hip_autotune(const mb3d_hip_params *frames, int32_t n, const mb3d_hip_out *out,
+                          mb3d_hip_options *best, double *best_ms);
+
+/* ---- the warm-start network (no counterpart in the reference) ----
+ * An MLP proposing each ray's hit distance from its origin and direction
+ * (sin/cos encoding, 3 x 64 ReLU, outputs distance and hit logit), trained
+ * on this backend's own HIP forward and backward kernels from (ray, exact
+ * distance) pairs of the plain march, run as BF16 MFMA inside the march.
+ * Rays: n x 6 floats (origin, unit direction); targets: n x 2 floats
+ * (distance / 4, hit 0/1). */
+#define MB3D_HIP_MLP_PARAMS 13634
+int32_t mb3d_hip_mlp_create(uint64_t seed, mb3d_hip_mlp **net);
+void mb3d_hip_mlp_destroy(mb3d_hip_mlp *net);
+/* copies the f32 parameters out (get) and/or in (set); n = MB3D_HIP_MLP_PARAMS */
+int32_t mb3d_hip_mlp_params(mb3d_hip_mlp *net, float *get, const float *set, int64_t n);
+/* mean loss and its gradient over the batch (one forward and backward) */
+int32_t mb3d_hip_mlp_grad(mb3d_hip_mlp *net, const float *rays, const float *targets, int64_t n, float *loss,
+                          float *grad);
+/* Adam training on the pairs: `steps` minibatches of `batch` (in order,
+ * wrapping), learning rate `lr` with cosine decay; the loss of each step
+ * into loss_curve (may be NULL) */
+int32_t mb3d_hip_mlp_train(mb3d_hip_mlp *net, const float *rays, const float *targets, int64_t n, int32_t steps,
+                           int32_t batch, float lr, float *loss_curve);
+/* the network's outputs (n x 2: distance / 4, hit logit) in f32 or with the
+ * march's BF16 MFMA path */
+int32_t mb3d_hip_mlp_infer(mb3d_hip_mlp *net, const float *rays, int64_t n, int32_t bf16_mfma, float *out);
 
 /* The work counters of the last march/post call on this thread. */
 int32_t mb3d_hip_counters_get(mb3d_hip_counters *out);

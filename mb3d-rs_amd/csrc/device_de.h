@@ -47,6 +47,10 @@ struct DevP {
     double vg[3][3], ystart[3];
     int64_t npix;      // pixels of the rectangle
     int64_t nslots_q;  // work items of the tile queue (tiles x tile_w x tile_h)
+    // a frame of a batch: its first work item in the batch's queue and its
+    // first pixel in the batch's output arrays (0 for a single picture)
+    int64_t q_begin, pix_base;
+    int sync_refill;  // the f64 finish refills a wave when all its lanes are free
 };
 
 // Per-lane work counters, reduced per wave at the end of a kernel.
@@ -613,8 +617,9 @@ struct Refine {
 __device__ __forceinline__ void pixel_ray(const DevP &P, int x, int y, double c[3], double v[3]) {
     const double cafy = ((double)y / (double)P.height - 0.5) * P.fov_y;
     const double cafx = (double)((P.fovx_off - (float)(x + 1)) * P.fovx_mul);
-    const double sx = sin(cafy), cx = cos(cafy);
-    const double sy = sin(cafx), cy = cos(cafx);
+    double sx, cx, sy, cy;
+    sincos(cafy, &sx, &cx);
+    sincos(cafx, &sy, &cy);
     double w0 = -sy, w1 = sx, w2 = cx * cy;
     const double nrm = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
     w0 *= nrm;

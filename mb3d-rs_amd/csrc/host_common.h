@@ -3,11 +3,20 @@
 // check (gpu::unsupported, src/gpu.rs:445-477).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
+#include <mutex>
 #include <string>
 
 #include "../../include/mb3d_hip.h"
 
 namespace mb3d {
+
+// The library's one device context (mb3d_hip_init): its lock, stream (null
+// before init) and compute-unit count.
+std::mutex &lib_mutex();
+hipStream_t lib_stream();
+int lib_cus();
 
 // Records `msg` as this thread's last error and returns `code`.
 int set_error(int code, const char *msg);

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -200,9 +201,25 @@ __device__ int32_t dev_pixel_seed(int64_t y, int64_t x) {
 // ---------------------------------------------------------------------------
 // pass 1: the FP32 march, persistent, lanes refilled one by one
 
-template <int KIND, int DEM>
-__global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
+// the frame of a batch that work item q belongs to
+__device__ __forceinline__ int find_frame(const DevP *__restrict__ tab, int n, long long q) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].q_begin <= q)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// BATCH: the frames of a batch in one launch; `tab` holds each frame's
+// parameters, P0 the batch's queue totals and the shared tile shape
+template <int KIND, int DEM, bool BATCH>
+__global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__restrict__ tab, int nframes) {
     Cnt cnt;
+    int fr = 0;
     const int lane = lane_id();
     bool have = false, exhausted = false;
     int pix = 0, px = 0, py = 0;
@@ -213,8 +230,8 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
     int nsteps = 0;
     bool start = false, first = false;
     int32_t seed = 0;
-    const float fguard = P.guard;
-    const int tile_n = P.tile_w * P.tile_h;
+    const float fguard = P0.guard;
+    const int tile_n = P0.tile_w * P0.tile_h;
     WaveChunk wq, cq;
     chunk_init(wq);
     chunk_init(cq);
@@ -222,35 +239,39 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
         // refill: every idle lane takes the next work item from the wave's chunk
         bool need = !have && !exhausted;
         for (int tries = 0; tries < 4 && __any(need); tries++) {
-            const long long q = wave_take(need, &B.ctr[C_QUEUE], wq, 256, P.nslots_q);
+            const long long q = wave_take(need, &B.ctr[C_QUEUE], wq, 256, P0.nslots_q);
             if (need) {
                 if (q < 0) {
                     exhausted = true;
                     need = false;
                 } else {
-                    const long long tile = q / tile_n;
-                    const int k = (int)(q - tile * tile_n);
-                    const int x = (int)(tile % P.tiles_x) * P.tile_w + k % P.tile_w;
-                    const int yr = (int)(tile / P.tiles_x) * P.tile_h + k / P.tile_w;
-                    if (x < P.width && yr < P.rows) {
+                    if (BATCH) fr = find_frame(tab, nframes, q);
+                    const DevP &F = BATCH ? tab[fr] : P0;
+                    const long long ql = q - F.q_begin;
+                    const long long tile = ql / tile_n;
+                    const int k = (int)(ql - tile * tile_n);
+                    const int x = (int)(tile % F.tiles_x) * F.tile_w + k % F.tile_w;
+                    const int yr = (int)(tile / F.tiles_x) * F.tile_h + k / F.tile_w;
+                    if (x < F.width && yr < F.rows) {
                         have = true;
                         need = false;
                         px = x;
-                        py = P.row0 + yr;
-                        pix = yr * P.width + x;
-                        pixel_ray(P, px, py, c, v);
+                        py = F.row0 + yr;
+                        pix = (int)(F.pix_base + (long long)yr * F.width + x);
+                        pixel_ray(F, px, py, c, v);
                         mzz = 0.0;
-                        ms = P.de_stop;
+                        ms = F.de_stop;
                         rsf = 1.0f;
                         scount = 0.0f;
                         nsteps = 0;
                         start = true;
-                        first = P.first_step_random != 0;
+                        first = F.first_step_random != 0;
                         seed = first ? dev_pixel_seed(py, px) : 0;
                     }
                 }
             }
         }
+        const DevP &P = BATCH ? tab[fr] : P0;
         if (__all(exhausted)) break;
         bool handoff = false, background = false;
         if (have) {
@@ -338,7 +359,7 @@ __global__ __launch_bounds__(256) void k_march32(DevP P, Bufs B) {
             o.rsf = rsf;
             o.step_count = scount;
             o.pix = pix;
-            o.flags = (start ? CF_START : 0) | (first ? CF_FIRST : 0) | (nsteps > 0 ? CF_STEPPED : 0);
+            o.flags = (start ? CF_START : 0) | (first ? CF_FIRST : 0) | (nsteps > 0 ? CF_STEPPED : 0) | (fr << 8);
             o.nsteps = nsteps;
             o.seed = seed;
             have = false;
@@ -430,10 +451,11 @@ enum FinPhase : int { FP_START, FP_RESUME, FP_BACK, FP_STEP, FP_REF, FP_NC, FP_N
 // N32: the six offset DEs of the normal stencil in FP32, for the analytic
 // DEs (their FP32 error moves a normal by ~0.01 degree; the numeric
 // gradient's does not allow it)
-template <int KIND, int DEM, bool N32>
-__global__ __launch_bounds__(256) void k_finish64(DevP P, Bufs B, int all_pixels) {
+template <int KIND, int DEM, bool N32, bool BATCH>
+__global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixels, const DevP *__restrict__ tab) {
     Cnt cnt;
-    const long long n_items = all_pixels ? P.npix : (long long)B.ctr[C_CAND];
+    int fr = 0;
+    const long long n_items = all_pixels ? P0.npix : (long long)B.ctr[C_CAND];
     bool have = false, exhausted = false;
     Ray64 R;
     Refine F;
@@ -452,6 +474,7 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P, Bufs B, int all_pixels
     chunk_init(hq);
     for (;;) {
         bool need = !have && !exhausted;
+        if (P0.sync_refill && __any(have)) need = false;
         for (int tries = 0; tries < 4 && __any(need); tries++) {
             const long long slot = wave_take(need, &B.ctr[C_FINQ], fq, 64, n_items);
             if (!need) continue;
@@ -463,14 +486,16 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P, Bufs B, int all_pixels
             if (!all_pixels && B.cand[slot].pix < 0) continue;  // an empty slot of a wave's chunk
             have = true;
             need = false;
-            R.reset(P);
+            if (!all_pixels && BATCH) fr = B.cand[slot].flags >> 8;
+            const DevP &F = BATCH ? tab[fr] : P0;
+            R.reset(F);
             guard = 0;
             if (all_pixels) {
                 pix = (int)slot;
-                const int x = pix % P.width, y = P.row0 + pix / P.width;
-                pixel_ray(P, x, y, R.c, R.v);
+                const int x = pix % F.width, y = F.row0 + pix / F.width;
+                pixel_ray(F, x, y, R.c, R.v);
                 phase = FP_START;
-                first = P.first_step_random != 0;
+                first = F.first_step_random != 0;
                 seed = first ? dev_pixel_seed(y, x) : 0;
                 scount = 0.0f;
                 nsteps = 0;
@@ -495,12 +520,13 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P, Bufs B, int all_pixels
                     phase = FP_START;
                 } else {
                     R.mzz = o.mzz;
-                    R.upd(P);
+                    R.upd(F);
                     phase = FP_RESUME;
                 }
             }
         }
         if (__all(exhausted)) break;
+        const DevP &P = BATCH ? tab[fr] : P0;
         bool hit = false, done = false;
         if (!have) goto end;
         if (N32 && (phase == FP_NP || phase == FP_NM)) {
@@ -1031,6 +1057,12 @@ __global__ void k_de_at(DevP P, const double *pts, long long n, double *de, int3
     }
 }
 
+// clears the counter words (a kernel, not a memset node: the batch plans
+// are captured into HIP graphs)
+__global__ void k_zero(unsigned long long *p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 // the surface pixels of a G-buffer marched elsewhere
 __global__ void k_collect_hits(DevP P, Bufs B) {
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -1067,6 +1099,10 @@ int set_error(int code, const char *msg) {
     t_error = msg;
     return code;
 }
+
+std::mutex &lib_mutex() { return g_ctx.mu; }
+hipStream_t lib_stream() { return g_ctx.device >= 0 ? g_ctx.stream : nullptr; }
+int lib_cus() { return g_ctx.cus; }
 
 #define HIP_TRY(expr)                                                                             \
     do {                                                                                          \
@@ -1168,6 +1204,10 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
     d.step_width = p.step_width;
     std::memcpy(d.vg, p.vgrads, sizeof d.vg);
     std::memcpy(d.ystart, p.ystart, sizeof d.ystart);
+    {
+        const char *e = std::getenv("MB3D_HIP_FINISH_SYNC");  // debug: wave-synchronous refill of the f64 finish
+        d.sync_refill = e && e[0] == '1';
+    }
     d.tile_w = o.tile_w > 0 ? o.tile_w : 8;
     d.tile_h = o.tile_h > 0 ? o.tile_h : 8;
     d.tiles_x = (p.width + d.tile_w - 1) / d.tile_w;
@@ -1240,13 +1280,13 @@ int dem_of(const mb3d_hip_params &p) {
     return DEM_NUMERIC;
 }
 
-template <int KIND, int DEM>
-void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b) {
-    hipLaunchKernelGGL((k_march32<KIND, DEM>), dim3(grid), dim3(256), 0, s, d, b);
+template <int KIND, int DEM, bool BATCH = false>
+void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b, const DevP *tab = nullptr, int n = 1) {
+    hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH>), dim3(grid), dim3(256), 0, s, d, b, tab, n);
 }
-template <int KIND, int DEM, bool N32>
-void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all) {
-    hipLaunchKernelGGL((k_finish64<KIND, DEM, N32>), dim3(grid), dim3(256), 0, s, d, b, all);
+template <int KIND, int DEM, bool N32, bool BATCH = false>
+void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all, const DevP *tab = nullptr) {
+    hipLaunchKernelGGL((k_finish64<KIND, DEM, N32, BATCH>), dim3(grid), dim3(256), 0, s, d, b, all, tab);
 }
 template <int KIND, int DEM, bool F32>
 void launch_shadow(int grid, hipStream_t s, const DevP &d, const Bufs &b, const ShadowJob &j) {
@@ -1367,7 +1407,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     hipStream_t stream = c.stream;
     const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
     const int grid_march = std::max(1, c.cus * wpc / 4);
-    const int grid_flat = std::max(1, c.cus * 4);
+    int grid_flat = std::max(1, c.cus * 4);
+    if (const char *e = std::getenv("MB3D_HIP_FINISH_GRID")) grid_flat = std::max(1, c.cus * std::atoi(e));  // debug
     HIP_TRY(hipEventRecord(c.ev[0], c.stream));
     if (!post_only) {
         if (f64) {
@@ -1463,6 +1504,218 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     return MB3D_HIP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// batches: many frames (any sizes, the same formula) in one persistent
+// launch of each pass, optionally captured into a HIP graph
+
+struct Plan {
+    int n = 0;
+    int kind = 0, dem = 0;
+    int grid_march = 0, grid_flat = 0;
+    long long npix = 0;
+    DevP P0;
+    std::vector<DevP> tab;
+    DevP *d_tab = nullptr;
+    void *d_work = nullptr;
+    unsigned long long *d_ctr = nullptr;
+    Bufs b;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+};
+
+void plan_free(Plan *pl) {
+    if (!pl) return;
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    if (pl->d_tab) (void)hipFree(pl->d_tab);
+    if (pl->d_work) (void)hipFree(pl->d_work);
+    if (pl->d_ctr) (void)hipFree(pl->d_ctr);
+    if (pl->e0) (void)hipEventDestroy(pl->e0);
+    if (pl->e1) (void)hipEventDestroy(pl->e1);
+    if (pl->e2) (void)hipEventDestroy(pl->e2);
+    delete pl;
+}
+
+// the kernels of one batch run, in stream order; `split` records the event
+// between the two passes (not in a captured graph: an event node there does
+// not time)
+int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, pl.d_ctr, (int)C_WORDS);
+    const DevP *tab = pl.d_tab;
+    const int n = pl.n;
+    dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
+        launch_march<decltype(K)::value, decltype(D)::value, true>(pl.grid_march, stream, pl.P0, pl.b, tab, n);
+    });
+    if (split) HIP_TRY(hipEventRecord(pl.e1, stream));
+    dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
+        launch_finish<decltype(K)::value, decltype(D)::value, decltype(D)::value != DEM_NUMERIC, true>(
+            pl.grid_flat, stream, pl.P0, pl.b, 0, tab);
+    });
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, const mb3d_hip_options *opt_in,
+               bool graph, Plan **result) {
+    Ctx &c = g_ctx;
+    if (c.device < 0) return set_error(MB3D_HIP_ERR_NOT_INIT, "mb3d_hip_init was not called");
+    if (!frames || n <= 0 || !out || !result) return set_error(MB3D_HIP_ERR_INVALID, "bad argument");
+    if (!out->on_device) return set_error(MB3D_HIP_ERR_INVALID, "a batch writes device buffers (out->on_device = 1)");
+    mb3d_hip_options o;
+    std::memset(&o, 0, sizeof o);
+    if (opt_in) o = *opt_in;
+    if (o.tile_w < 0 || o.tile_h < 0 || o.tile_w * o.tile_h > 4096)
+        return set_error(MB3D_HIP_ERR_INVALID, "bad tile shape");
+    const int kind = kind_of(frames[0]), dem = dem_of(frames[0]);
+    for (int f = 0; f < n; f++) {
+        const std::string why = unsupported_reason(&frames[f]);
+        if (!why.empty()) return set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
+        if (kind_of(frames[f]) != kind || dem_of(frames[f]) != dem)
+            return set_error(MB3D_HIP_ERR_INVALID, "the frames of a batch share one formula and DE mode");
+        if (o.precision == MB3D_HIP_PREC_F64 || (o.precision == MB3D_HIP_PREC_AUTO && needs_f64(frames[f])))
+            return set_error(MB3D_HIP_UNSUPPORTED, "a batch runs the FP32 march (a frame needs FP64)");
+    }
+    Plan *pl = new Plan();
+    pl->n = n;
+    pl->kind = kind;
+    pl->dem = dem;
+    pl->tab.resize(n);
+    long long q = 0, pix = 0;
+    for (int f = 0; f < n; f++) {
+        DevP &d = pl->tab[f];
+        fill_devp(frames[f], o, 0, frames[f].height, d);
+        d.q_begin = q;
+        d.pix_base = pix;
+        q += d.nslots_q;
+        pix += d.npix;
+    }
+    if (pix >= (1ll << 31)) {
+        plan_free(pl);
+        return set_error(MB3D_HIP_ERR_INVALID, "a batch holds at most 2^31 pixels");
+    }
+    pl->npix = pix;
+    pl->P0 = pl->tab[0];
+    pl->P0.nslots_q = q;
+    pl->P0.npix = pix;
+    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
+    pl->grid_march = std::max(1, c.cus * wpc / 4);
+    pl->grid_flat = std::max(1, c.cus * 4);
+    auto fail_free = [&](int e) {
+        plan_free(pl);
+        return e;
+    };
+    if (hipMalloc((void **)&pl->d_tab, sizeof(DevP) * n) != hipSuccess ||
+        hipMalloc(&pl->d_work, (size_t)pix * (sizeof(Cand) + sizeof(int32_t)) + 4096) != hipSuccess ||
+        hipMalloc((void **)&pl->d_ctr, C_WORDS * sizeof(unsigned long long)) != hipSuccess)
+        return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipMalloc failed for a batch"));
+    if (hipMemcpy(pl->d_tab, pl->tab.data(), sizeof(DevP) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipMemcpy failed for a batch"));
+    if (hipEventCreate(&pl->e0) != hipSuccess || hipEventCreate(&pl->e1) != hipSuccess ||
+        hipEventCreate(&pl->e2) != hipSuccess)
+        return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipEventCreate failed"));
+    std::memset(&pl->b, 0, sizeof pl->b);
+    pl->b.depth = out->depth;
+    pl->b.depth64 = out->depth64;
+    pl->b.iters = out->iters;
+    pl->b.normals = out->normals;
+    pl->b.gbuf = out->gbuf;
+    pl->b.cand = (Cand *)pl->d_work;
+    pl->b.hits = (int32_t *)((char *)pl->d_work + (size_t)pix * sizeof(Cand));
+    pl->b.ctr = pl->d_ctr;
+    if (graph) {
+        if (hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipStreamBeginCapture failed"));
+        const int e = plan_enqueue(*pl, c.stream, false);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c.stream, &g);
+        pl->graph = g;
+        if (e) return fail_free(e);
+        if (ec != hipSuccess) return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipStreamEndCapture failed"));
+        if (hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0) != hipSuccess)
+            return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipGraphInstantiate failed"));
+    }
+    *result = pl;
+    return MB3D_HIP_OK;
+}
+
+// runs a plan `repeats` times and waits; the counters are the last run's,
+// the times the mean of the runs (the pass split only without a graph)
+int plan_run(Plan &pl, int repeats) {
+    Ctx &c = g_ctx;
+    HIP_TRY(hipEventRecord(pl.e0, c.stream));
+    for (int r = 0; r < repeats; r++) {
+        if (pl.exec) {
+            HIP_TRY(hipGraphLaunch(pl.exec, c.stream));
+        } else if (int e = plan_enqueue(pl, c.stream, repeats == 1)) {
+            return e;
+        }
+    }
+    HIP_TRY(hipEventRecord(pl.e2, c.stream));
+    unsigned long long ctr[C_WORDS];
+    HIP_TRY(hipMemcpyAsync(ctr, pl.d_ctr, sizeof ctr, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    float ms_all = 0, ms_a = -1.0f;
+    if (hipEventElapsedTime(&ms_all, pl.e0, pl.e2) != hipSuccess) ms_all = -1.0f;
+    if (!pl.exec && repeats == 1 && hipEventElapsedTime(&ms_a, pl.e0, pl.e1) != hipSuccess) ms_a = -1.0f;
+    (void)hipGetLastError();
+    mb3d_hip_counters &t = t_counters;
+    std::memset(&t, 0, sizeof t);
+    t.pixels = (uint64_t)pl.npix;
+    t.steps = ctr[C_STATS + 0];
+    t.de_evals_f32 = ctr[C_STATS + 1];
+    t.de_evals_f64 = ctr[C_STATS + 2];
+    t.its_p8_f32 = ctr[C_STATS + 3];
+    t.its_box_f32 = ctr[C_STATS + 4];
+    t.its_p8_f64 = ctr[C_STATS + 5];
+    t.its_box_f64 = ctr[C_STATS + 6];
+    t.hits = ctr[C_STATS + 7];
+    t.overflow = ctr[C_STATS + 8];
+    t.kernel_ms = ms_all > 0.0f ? ms_all / repeats : -1.0;
+    t.march_f32_ms = ms_a;
+    t.finish_f64_ms = ms_a > 0.0f ? t.kernel_ms - ms_a : -1.0;
+    return MB3D_HIP_OK;
+}
+
+// The project's autotuner: every tile shape and persistent grid size of the
+// sweep on a graph of the batch, the fastest kept (min of `runs` replays).
+int autotune_locked(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, mb3d_hip_options *best,
+                    double *best_ms) {
+    static const int tiles[][2] = {{8, 8}, {16, 4}, {16, 8}, {32, 4}, {32, 8}};
+    static const int wpcs[] = {8, 12, 16, 20, 24};
+    mb3d_hip_options o;
+    std::memset(&o, 0, sizeof o);
+    if (best) o = *best;
+    double top = 1e300;
+    mb3d_hip_options win = o;
+    for (const auto &t : tiles) {
+        for (int w : wpcs) {
+            mb3d_hip_options c = o;
+            c.tile_w = t[0];
+            c.tile_h = t[1];
+            c.waves_per_cu = w;
+            Plan *pl = nullptr;
+            if (int e = plan_build(frames, n, out, &c, true, &pl)) return e;
+            double m = 1e300;
+            for (int r = 0; r < 3; r++) {
+                if (int e = plan_run(*pl, 1)) {
+                    plan_free(pl);
+                    return e;
+                }
+                m = std::min(m, t_counters.kernel_ms);
+            }
+            plan_free(pl);
+            if (m < top) {
+                top = m;
+                win = c;
+            }
+        }
+    }
+    if (best) *best = win;
+    if (best_ms) *best_ms = top;
+    return MB3D_HIP_OK;
+}
+
 }  // namespace mb3d
 
 using namespace mb3d;
@@ -1551,6 +1804,66 @@ extern "C" int32_t mb3d_hip_de_at(const mb3d_hip_params *params, const double *p
         return MB3D_HIP_OK;
     } catch (...) {
         return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_de_at");
+    }
+}
+
+struct mb3d_hip_plan {
+    mb3d::Plan *p;
+};
+
+extern "C" int32_t mb3d_hip_march_batch(const mb3d_hip_params *frames, int32_t n, const mb3d_hip_out *out,
+                                        const mb3d_hip_options *options) {
+    try {
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        Plan *pl = nullptr;
+        if (int e = plan_build(frames, n, out, options, false, &pl)) return e;
+        const int e = plan_run(*pl, 1);
+        plan_free(pl);
+        return e;
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_march_batch");
+    }
+}
+
+extern "C" int32_t mb3d_hip_plan_create(const mb3d_hip_params *frames, int32_t n, const mb3d_hip_out *out,
+                                        const mb3d_hip_options *options, mb3d_hip_plan **plan) {
+    try {
+        if (!plan) return set_error(MB3D_HIP_ERR_INVALID, "plan is NULL");
+        *plan = nullptr;
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        Plan *pl = nullptr;
+        if (int e = plan_build(frames, n, out, options, true, &pl)) return e;
+        *plan = new mb3d_hip_plan{pl};
+        return MB3D_HIP_OK;
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_plan_create");
+    }
+}
+
+extern "C" int32_t mb3d_hip_plan_launch(mb3d_hip_plan *plan, int32_t repeats) {
+    try {
+        if (!plan || !plan->p || repeats < 1) return set_error(MB3D_HIP_ERR_INVALID, "bad plan or repeat count");
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        return plan_run(*plan->p, repeats);
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_plan_launch");
+    }
+}
+
+extern "C" void mb3d_hip_plan_destroy(mb3d_hip_plan *plan) {
+    if (!plan) return;
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    plan_free(plan->p);
+    delete plan;
+}
+
+extern "C" int32_t mb3d_hip_autotune(const mb3d_hip_params *frames, int32_t n, const mb3d_hip_out *out,
+                                     mb3d_hip_options *best, double *best_ms) {
+    try {
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        return autotune_locked(frames, n, out, best, best_ms);
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_autotune");
     }
 }
 

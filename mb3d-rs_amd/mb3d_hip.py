@@ -174,7 +174,11 @@ class Options(C.Structure):
         ("waves_per_cu", C.c_int32),
         ("it_split", C.c_int32),
         ("guard_band", C.c_float),
-        ("reserved", C.c_int32 * 4),
+        ("mlp", C.c_void_p),
+        ("mlp_conf_hit", C.c_float),
+        ("mlp_conf_miss", C.c_float),
+        ("mlp_margin_rel", C.c_float),
+        ("mlp_margin_abs", C.c_float),
     ]
 
 
@@ -190,6 +194,8 @@ class Counters(C.Structure):
         ("its_p8_f64", C.c_uint64),
         ("its_box_f64", C.c_uint64),
         ("overflow", C.c_uint64),
+        ("jumps", C.c_uint64),
+        ("rollbacks", C.c_uint64),
         ("kernel_ms", C.c_double),
         ("post_ms", C.c_double),
         ("march_f32_ms", C.c_double),
@@ -222,6 +228,20 @@ SIGNATURES = {
     ),
     "mb3d_hip_counters_get": (C.c_int32, [C.POINTER(Counters)]),
     "mb3d_hip_de_at": (C.c_int32, [C.POINTER(Params), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "mb3d_hip_march_batch": (C.c_int32, [C.POINTER(Params), C.c_int32, C.POINTER(Out), C.POINTER(Options)]),
+    "mb3d_hip_plan_create": (C.c_int32, [C.POINTER(Params), C.c_int32, C.POINTER(Out), C.POINTER(Options),
+                                         C.POINTER(C.c_void_p)]),
+    "mb3d_hip_plan_launch": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "mb3d_hip_plan_destroy": (None, [C.c_void_p]),
+    "mb3d_hip_autotune": (C.c_int32, [C.POINTER(Params), C.c_int32, C.POINTER(Out), C.POINTER(Options),
+                                      C.POINTER(C.c_double)]),
+    "mb3d_hip_mlp_create": (C.c_int32, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "mb3d_hip_mlp_destroy": (None, [C.c_void_p]),
+    "mb3d_hip_mlp_params": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "mb3d_hip_mlp_grad": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "mb3d_hip_mlp_train": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                                       C.c_float, C.c_void_p]),
+    "mb3d_hip_mlp_infer": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
 }
 
 _lib = None
@@ -234,6 +254,13 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError(f"libmb3d_hip.so is not built ({path}): run __graft_entry__.build()")
+    # torch ships its own HIP runtime under another file name; loaded first,
+    # it satisfies the library's libamdhip64.so.7 too, so that a process that
+    # uses both (device buffers from torch) holds one HIP runtime, not two
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(lib, name)
@@ -570,3 +597,149 @@ def de_at(p: Params, points):
     its = np.zeros(len(pts), np.int32)
     check(load().mb3d_hip_de_at(C.byref(p), _p(pts), len(pts), _p(de), _p(its)), "mb3d_hip_de_at")
     return de, its
+
+
+def catmull_rom_weights(t: float):
+    """catmull_rom_weights (src/anim.rs:692-700)."""
+    t2, t3 = t * t, t * t * t
+    return (0.5 * (-t3 + 2.0 * t2 - t), 0.5 * (3.0 * t3 - 5.0 * t2 + 2.0),
+            0.5 * (-3.0 * t3 + 4.0 * t2 + t), 0.5 * (t3 - t2))
+
+
+SIZES = ((320, 240), (640, 480), (1280, 720), (1920, 1080))
+
+
+def animation(n_frames=64, seed=1, keys=6, de_mode=DE_ANALYTIC_P8, sizes=SIZES):
+    """The batched animation: frames of sizes drawn uniformly from `sizes`
+    along a seeded Catmull-Rom camera path around the power-8 Mandelbulb
+    (camera distance 1.8-3.0, looking at the origin).  Returns scenes."""
+    rng = np.random.default_rng(seed)
+    pos = []
+    for _ in range(keys):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        pos.append(d * rng.uniform(1.8, 3.0))
+    scenes = []
+    for f in range(n_frames):
+        u = f / max(n_frames - 1, 1) * (keys - 3)
+        k = min(int(u), keys - 4)
+        w = catmull_rom_weights(u - k)
+        cam = sum(w[i] * pos[k + i] for i in range(4))
+        wd, ht = sizes[int(rng.integers(len(sizes)))]
+        s = scene_bulb(wd, ht, seed=seed, de_mode=de_mode)
+        dist = float(np.linalg.norm(cam))
+        s.mid[:] = [0.0, 0.0, 0.0]
+        # the start plane through the camera point, facing the origin
+        set_camera(s, look_matrix(-cam))
+        s.mid[:] = list(cam)
+        s.z_start = float(cam[2])
+        s.z_end = s.z_start + dist + 30.0
+        s.zoom = 0.8 * 2.5 / dist
+        scenes.append(s)
+    return scenes
+
+
+def params_array(scenes):
+    arr = (Params * len(scenes))()
+    for i, s in enumerate(scenes):
+        arr[i] = params_from_scene(s)
+    return arr
+
+
+class Plan:
+    """A batch of frames captured into a HIP graph (mb3d_hip_plan_*)."""
+
+    def __init__(self, frames, out: Out, opts: Options = None):
+        self.lib = load()
+        self.h = C.c_void_p()
+        self.frames = frames
+        self.out = out
+        check(self.lib.mb3d_hip_plan_create(frames, len(frames), C.byref(out),
+                                            C.byref(opts) if opts is not None else None, C.byref(self.h)),
+              "mb3d_hip_plan_create")
+
+    def launch(self, repeats=1):
+        check(self.lib.mb3d_hip_plan_launch(self.h, repeats), "mb3d_hip_plan_launch")
+        return counters()
+
+    def close(self):
+        if self.h:
+            self.lib.mb3d_hip_plan_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(frames, out: Out, opts: Options = None):
+    o = opts if opts is not None else options()
+    ms = C.c_double()
+    check(load().mb3d_hip_autotune(frames, len(frames), C.byref(out), C.byref(o), C.byref(ms)), "mb3d_hip_autotune")
+    return o, ms.value
+
+
+MLP_PARAMS = 13634
+
+
+def _f32(a, cols):
+    a = np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, cols))
+    return a, a.ctypes.data_as(C.c_void_p)
+
+
+class Mlp:
+    """The warm-start network (mb3d_hip_mlp_*): rays are n x 6 (origin,
+    unit direction), targets n x 2 (hit distance / 4, hit 0/1)."""
+
+    def __init__(self, seed=0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        check(self.lib.mb3d_hip_mlp_create(seed, C.byref(self.h)), "mb3d_hip_mlp_create")
+
+    def params(self, new=None):
+        if new is not None:
+            a = np.ascontiguousarray(np.asarray(new, np.float32))
+            assert a.size == MLP_PARAMS
+            check(self.lib.mb3d_hip_mlp_params(self.h, None, a.ctypes.data_as(C.c_void_p), MLP_PARAMS), "mlp_params")
+        got = np.empty(MLP_PARAMS, np.float32)
+        check(self.lib.mb3d_hip_mlp_params(self.h, got.ctypes.data_as(C.c_void_p), None, MLP_PARAMS), "mlp_params")
+        return got
+
+    def grad(self, rays, targets):
+        r, rp = _f32(rays, 6)
+        t, tp = _f32(targets, 2)
+        assert len(r) == len(t)
+        loss = np.zeros(1, np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        check(self.lib.mb3d_hip_mlp_grad(self.h, rp, tp, len(r), loss.ctypes.data_as(C.c_void_p),
+                                         g.ctypes.data_as(C.c_void_p)), "mlp_grad")
+        return float(loss[0]), g
+
+    def train(self, rays, targets, steps, batch=4096, lr=2e-3):
+        r, rp = _f32(rays, 6)
+        t, tp = _f32(targets, 2)
+        assert len(r) == len(t)
+        curve = np.zeros(steps, np.float32)
+        check(self.lib.mb3d_hip_mlp_train(self.h, rp, tp, len(r), steps, batch, lr,
+                                          curve.ctypes.data_as(C.c_void_p)), "mlp_train")
+        return curve
+
+    def infer(self, rays, mfma=True):
+        r, rp = _f32(rays, 6)
+        out = np.empty((len(r), 2), np.float32)
+        check(self.lib.mb3d_hip_mlp_infer(self.h, rp, len(r), int(bool(mfma)), out.ctypes.data_as(C.c_void_p)),
+              "mlp_infer")
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.mb3d_hip_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
