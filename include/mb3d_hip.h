@@ -252,6 +252,15 @@ int32_t mb3d_hip_post(const mb3d_hip_params *params, const mb3d_hip_post_job *po
                       int32_t row0, int32_t rows, uint32_t *gbuf, int32_t on_device,
                       const mb3d_hip_options *options);
 
+/* The rays of the pixels of rows [row0, row0 + rows), as the warm-start
+ * network takes them (host memory, row-major): `rays` 6 floats per pixel
+ * (start point on the start plane, unit direction), `step_len` the length
+ * of one march unit along the ray (a depth in march units times it is the
+ * distance in scene units).  Either output may be NULL.  The start point
+ * and direction are the march's (calc_vgrads_fov, calc_start_pos,
+ * src/calc.rs:1561-1580 and 1673-1689). */
+int32_t mb3d_hip_rays(const mb3d_hip_params *params, int32_t row0, int32_t rows, float *rays, float *step_len);
+
 /* calc::de_at (src/calc.rs:2695-2703) at `n` points (3 doubles each, host
  * memory): the f64 DE in world units and the iteration count there.  Either
  * output may be NULL. */

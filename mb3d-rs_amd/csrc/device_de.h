@@ -58,6 +58,7 @@ struct Cnt {
     uint32_t steps = 0, de32 = 0, de64 = 0;
     uint32_t p8_32 = 0, box_32 = 0, p8_64 = 0, box_64 = 0;
     uint32_t hits = 0, overflow = 0;
+    uint32_t jumps = 0, rollbacks = 0;  // warm starts kept, warm starts undone
 };
 
 // ---------------------------------------------------------------------------

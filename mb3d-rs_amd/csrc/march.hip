@@ -28,6 +28,7 @@
 #include "../../include/mb3d_hip.h"
 #include "device_de.h"
 #include "host_common.h"
+#include "mlp_device.h"
 
 namespace mb3d {
 
@@ -51,7 +52,8 @@ enum : int {
     C_FINQ = 3,    // next item of the f64 finish
     C_POSTQ = 4,   // next item of the shadow pass
     C_AOQ = 5,     // next item of the DE AO pass
-    C_STATS = 8,   // 9 work counters from here
+    C_INFQ = 6,    // next chunk of the warm start's inference
+    C_STATS = 8,   // 11 work counters from here
     C_WORDS = 24
 };
 
@@ -64,6 +66,11 @@ struct Bufs {
     Cand *cand;
     int32_t *hits;
     unsigned long long *ctr;
+    // the warm start: the network's proposal per work item of the queue
+    // (hit distance / 4, hit logit) and a ready flag per chunk of
+    // LEARN_CHUNK items (zeroed before the launch)
+    float2 *prop;
+    uint32_t *ready;
 };
 
 // DE AO ray set (build_rays, src/deao.rs:75-105), computed on the host
@@ -92,22 +99,35 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 __device__ void flush_counters(const Cnt &c, unsigned long long *ctr) {
-    const unsigned long long v[9] = {c.steps, c.de32, c.de64, c.p8_32, c.box_32, c.p8_64, c.box_64, c.hits, c.overflow};
+    const unsigned long long v[11] = {c.steps, c.de32,  c.de64,     c.p8_32, c.box_32,   c.p8_64,
+                                      c.box_64, c.hits, c.overflow, c.jumps, c.rollbacks};
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
+    for (int k = 0; k < 11; k++) {
         const unsigned long long s = wave_sum(v[k]);
         if (lane_id() == 0 && s) atomicAdd(&ctr[C_STATS + k], s);
     }
+}
+
+// The value of the wave's first active lane, as a wave-uniform (scalar)
+// value: broadcasts an atomic's result from the lane that made it (always
+// the first active lane here), so that the compiler sees the loops whose exit
+// depends on it as uniform.
+__device__ __forceinline__ unsigned long long first_lane_u64(unsigned long long v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
 }
 
 // appends one entry per lane with `want` set: returns the lane's slot
 __device__ __forceinline__ long long wave_append(bool want, unsigned long long *counter) {
     const unsigned long long m = __ballot(want);
     if (!m) return -1;
-    const int leader = __ffsll((long long)m) - 1;
-    unsigned long long base = 0;
-    if (lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader, 64);
+    // the first active lane makes the atomic (first_lane_u64 reads it)
+    const int leader = __ffsll((long long)__ballot(true)) - 1;
+    // every active lane makes the atomic, adding 0 but the first (the
+    // compiler folds them into one): no lane-specific branch
+    const unsigned long long base =
+        first_lane_u64(atomicAdd(counter, lane_id() == leader ? (unsigned long long)__popcll(m) : 0ull));
     if (!want) return -1;
     return (long long)(base + __popcll(m & ((1ull << lane_id()) - 1ull)));
 }
@@ -129,11 +149,17 @@ __device__ __forceinline__ void chunk_init(WaveChunk &q) {
 
 __device__ __forceinline__ unsigned long long lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
+struct NoHook {
+    __device__ void operator()(long long, int) const {}
+};
+
 // Hands the lanes with `want` the next items of a queue of `total` items
-// (`total` < 0: no end, an allocation).  Call with the whole wave active.
-// Returns the lane's item, or -1 (the queue is exhausted).
+// (`total` < 0: no end, an allocation).  Call with the whole wave active;
+// `hook(base, count)` runs, with the whole wave active, for each new chunk
+// the wave takes.  Returns the lane's item, or -1 (the queue is exhausted).
+template <typename Hook = NoHook>
 __device__ __forceinline__ long long wave_take(bool want, unsigned long long *ctr, WaveChunk &q, int chunk,
-                                               long long total) {
+                                               long long total, Hook &&hook = Hook()) {
     long long item = -1;
     bool pending = want;
     const int leader = __ffsll((long long)__ballot(true)) - 1;
@@ -141,15 +167,15 @@ __device__ __forceinline__ long long wave_take(bool want, unsigned long long *ct
         const unsigned long long m = __ballot(pending);
         if (!m || q.dry) break;
         if (q.left == 0) {
-            unsigned long long b = 0;
-            if (lane_id() == leader) b = atomicAdd(ctr, (unsigned long long)chunk);
-            b = __shfl(b, leader, 64);
+            const unsigned long long b =
+                first_lane_u64(atomicAdd(ctr, lane_id() == leader ? (unsigned long long)chunk : 0ull));
             if (total >= 0 && (long long)b >= total) {
                 q.dry = true;
                 break;
             }
             q.base = (long long)b;
             q.left = total >= 0 ? (int)min((long long)chunk, total - (long long)b) : chunk;
+            hook(q.base, q.left);
         }
         const int rank = __popcll(m & lanemask_lt());
         const int take = min(__popcll(m), q.left);
@@ -214,13 +240,81 @@ __device__ __forceinline__ int find_frame(const DevP *__restrict__ tab, int n, l
     return lo;
 }
 
+// the pixel of work item q of the tile queue (false: the item is in a
+// ragged edge tile, outside the picture)
+template <bool BATCH>
+__device__ __forceinline__ bool item_pixel(const DevP &P0, const DevP *__restrict__ tab, int nframes, long long q,
+                                           int &fr, int &x, int &yr) {
+    fr = BATCH ? find_frame(tab, nframes, q) : 0;
+    const DevP &F = BATCH ? tab[fr] : P0;
+    const int tile_n = P0.tile_w * P0.tile_h;
+    const long long ql = q - F.q_begin;
+    const long long tile = ql / tile_n;
+    const int k = (int)(ql - tile * tile_n);
+    x = (int)(tile % F.tiles_x) * F.tile_w + k % F.tile_w;
+    yr = (int)(tile / F.tiles_x) * F.tile_h + k / F.tile_w;
+    return x < F.width && yr < F.rows;
+}
+
+constexpr int LEARN_CHUNK = 256;
+
 // BATCH: the frames of a batch in one launch; `tab` holds each frame's
-// parameters, P0 the batch's queue totals and the shared tile shape
-template <int KIND, int DEM, bool BATCH>
-__global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__restrict__ tab, int nframes) {
+// parameters, P0 the batch's queue totals and the shared tile shape.
+// LEARN: the warm start.  The kernel's waves first run the network (BF16
+// MFMA, all 64 lanes, 4 x 64 rays a chunk) over chunks of the queue, each
+// chunk's proposals published with a ready flag; then they march.  A ray
+// whose chunk is ready starts where the network puts it: confident hits at
+// the predicted distance less a margin, confident misses past the end (the
+// background); a ray whose chunk is not ready yet marches plainly (no wait).
+// A start that is not free -- the DE there below the stop distance (with
+// the hand-off guard) or the iteration limit reached -- is rolled back to
+// the plain march from the start plane.  This is the reference's cone-march
+// jump and check (march_pixel_from, src/calc.rs:2293-2308) with the
+// network's depth in place of the coarse ray's.
+template <int KIND, int DEM, bool BATCH, bool LEARN>
+__global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__restrict__ tab, int nframes,
+                                                 MlpDev M) {
+    __shared__ __attribute__((aligned(16))) char learn_lds[LEARN ? 4 : 1][LEARN ? MLP_LDS_BYTES : 1];
     Cnt cnt;
     int fr = 0;
     const int lane = lane_id();
+    if constexpr (LEARN) {
+        const long long nchunks = (P0.nslots_q + LEARN_CHUNK - 1) / LEARN_CHUNK;
+        char *lds = learn_lds[threadIdx.x >> 6];
+        // No lane-specific branch in this loop: with `if (lane == 0)` around
+        // the atomic and the flag store, the compiler threaded the two
+        // branches into a second loop that the other lanes ran without the
+        // first (which had made the atomic), re-reading a stale chunk index
+        // for ever.  Every lane makes the atomic (adding 0 but the first) and
+        // every lane stores the flag.
+        for (;;) {
+            const unsigned long long k = first_lane_u64(atomicAdd(&B.ctr[C_INFQ], lane == 0 ? 1ull : 0ull));
+            if ((long long)k >= nchunks) break;
+#pragma unroll 1
+            for (int r = 0; r < LEARN_CHUNK / 64; r++) {
+                const long long q = (long long)k * LEARN_CHUNK + r * 64 + lane;
+                float o[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 1.0f};
+                int f = 0, x = 0, yr = 0;
+                const bool in_q = q < P0.nslots_q;
+                if (in_q && item_pixel<BATCH>(P0, tab, nframes, q, f, x, yr)) {
+                    const DevP &F = BATCH ? tab[f] : P0;
+                    double cc[3], vv[3];
+                    pixel_ray(F, x, F.row0 + yr, cc, vv);
+                    const double il = 1.0 / sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+                    for (int j = 0; j < 3; j++) {
+                        o[j] = (float)cc[j];
+                        d[j] = (float)(vv[j] * il);
+                    }
+                }
+                float t = 0.0f, lg = 0.0f;
+                mlp_wave_infer(M, o, d, lds, t, lg);
+                if (in_q) B.prop[q] = make_float2(t, lg);
+            }
+            // the chunk's proposals reach L2 before its flag
+            __threadfence();
+            __hip_atomic_store(&B.ready[k], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
     bool have = false, exhausted = false;
     int pix = 0, px = 0, py = 0;
     double c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
@@ -228,10 +322,9 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
     float last_step = 0, last_de = 0, dtmp = 0;
     float ms = 0, rsf = 1, scount = 0;
     int nsteps = 0;
-    bool start = false, first = false;
+    bool start = false, first = false, jumped = false;
     int32_t seed = 0;
     const float fguard = P0.guard;
-    const int tile_n = P0.tile_w * P0.tile_h;
     WaveChunk wq, cq;
     chunk_init(wq);
     chunk_init(cq);
@@ -241,32 +334,57 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
         for (int tries = 0; tries < 4 && __any(need); tries++) {
             const long long q = wave_take(need, &B.ctr[C_QUEUE], wq, 256, P0.nslots_q);
             if (need) {
+                int x = 0, yr = 0;
                 if (q < 0) {
                     exhausted = true;
                     need = false;
-                } else {
-                    if (BATCH) fr = find_frame(tab, nframes, q);
+                } else if (item_pixel<BATCH>(P0, tab, nframes, q, fr, x, yr)) {
                     const DevP &F = BATCH ? tab[fr] : P0;
-                    const long long ql = q - F.q_begin;
-                    const long long tile = ql / tile_n;
-                    const int k = (int)(ql - tile * tile_n);
-                    const int x = (int)(tile % F.tiles_x) * F.tile_w + k % F.tile_w;
-                    const int yr = (int)(tile / F.tiles_x) * F.tile_h + k / F.tile_w;
-                    if (x < F.width && yr < F.rows) {
-                        have = true;
-                        need = false;
-                        px = x;
-                        py = F.row0 + yr;
-                        pix = (int)(F.pix_base + (long long)yr * F.width + x);
-                        pixel_ray(F, px, py, c, v);
-                        mzz = 0.0;
-                        ms = F.de_stop;
-                        rsf = 1.0f;
-                        scount = 0.0f;
-                        nsteps = 0;
-                        start = true;
-                        first = F.first_step_random != 0;
-                        seed = first ? dev_pixel_seed(py, px) : 0;
+                    have = true;
+                    need = false;
+                    px = x;
+                    py = F.row0 + yr;
+                    pix = (int)(F.pix_base + (long long)yr * F.width + x);
+                    pixel_ray(F, px, py, c, v);
+                    mzz = 0.0;
+                    ms = F.de_stop;
+                    rsf = 1.0f;
+                    scount = 0.0f;
+                    nsteps = 0;
+                    start = true;
+                    jumped = false;
+                    first = F.first_step_random != 0;
+                    seed = first ? dev_pixel_seed(py, px) : 0;
+                    // (agent-scope loads: the flag and the proposals come from L2,
+                    // where the writing wave's release put them)
+                    if (LEARN && __hip_atomic_load(&B.ready[q / LEARN_CHUNK], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)) {
+                        const unsigned long long raw = __hip_atomic_load(
+                            reinterpret_cast<unsigned long long *>(B.prop + q), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+                        const float2 pr = make_float2(__uint_as_float((uint32_t)raw), __uint_as_float((uint32_t)(raw >> 32)));
+                        const float ph = 1.0f / (1.0f + __expf(-pr.y));
+                        double jump = 0.0;
+                        if (ph >= M.conf_hit) {
+                            const double vl = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                            const double tj = 4.0 * (double)pr.x * (1.0 - (double)M.margin_rel) - M.margin_abs;
+                            if (tj > 0.0) jump = tj / vl;
+                        } else if (ph <= M.conf_miss) {
+                            jump = F.zend * 2.0;
+                        }
+                        if (jump > F.zend) {  // past the end: the background
+                            write_background(F, B, pix, 0.0f);
+                            cnt.jumps++;
+                            have = false;
+                            need = true;
+                        } else if (jump > 0.0) {
+                            mzz = jump;
+                            c[0] += v[0] * jump;
+                            c[1] += v[1] * jump;
+                            c[2] += v[2] * jump;
+                            ms = F.de_stop * (1.0f + (float)mzz * F.de_stop_factor);
+                            jumped = true;
+                        }
                     }
                 }
             }
@@ -281,7 +399,23 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
             // finish re-decides everything near the surface); the depth and
             // the position accumulate in f64
             bool top = true;
-            if (!(d32 < __builtin_inff())) {  // single precision overflow: f64 decides
+            if (LEARN && jumped) {
+                // the warm start's check: a free start, clear of the
+                // hand-off guard, is kept; any other goes back to the
+                // start plane and marches plainly
+                jumped = false;
+                if (!(d32 < __builtin_inff()) || it >= P.max_it || !(d32 >= ms * fguard)) {
+                    cnt.rollbacks++;
+                    pixel_ray(P, px, py, c, v);
+                    mzz = 0.0;
+                    ms = P.de_stop;
+                    top = false;
+                } else {
+                    cnt.jumps++;
+                }
+            }
+            if (!top) {
+            } else if (!(d32 < __builtin_inff())) {  // single precision overflow: f64 decides
                 cnt.overflow++;
                 handoff = true;
                 top = false;
@@ -1063,6 +1197,40 @@ __global__ void k_zero(unsigned long long *p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
+// the rays of the rectangle's pixels as the network takes them: start
+// point and unit direction (f32), and the length of one march unit
+__global__ void k_rays(DevP P, float *rays, float *step_len) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < P.npix; i += stride) {
+        const int y = P.row0 + (int)(i / P.width), x = (int)(i % P.width);
+        double c[3], v[3];
+        pixel_ray(P, x, y, c, v);
+        const double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int k = 0; k < 3; k++) {
+            rays[6 * i + k] = (float)c[k];
+            rays[6 * i + 3 + k] = (float)(v[k] / l);
+        }
+        step_len[i] = (float)l;
+    }
+}
+
+// zeroes n words (a kernel, not a memset: it may be captured in a graph)
+__global__ void k_zero32(uint32_t *p, long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
+// the warm start's buffers behind `base` (8-byte aligned): the proposals of
+// `items` work items, then the chunk flags; returns their size in bytes
+size_t learn_bufs(void *base, long long items, Bufs *b) {
+    const long long chunks = (items + LEARN_CHUNK - 1) / LEARN_CHUNK;
+    if (b) {
+        b->prop = (float2 *)base;
+        b->ready = (uint32_t *)((char *)base + (size_t)items * sizeof(float2));
+    }
+    return (size_t)items * sizeof(float2) + (size_t)chunks * sizeof(uint32_t) + 256;
+}
+
 // the surface pixels of a G-buffer marched elsewhere
 __global__ void k_collect_hits(DevP P, Bufs B) {
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -1281,8 +1449,12 @@ int dem_of(const mb3d_hip_params &p) {
 }
 
 template <int KIND, int DEM, bool BATCH = false>
-void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b, const DevP *tab = nullptr, int n = 1) {
-    hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH>), dim3(grid), dim3(256), 0, s, d, b, tab, n);
+void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b, const MlpDev &M, const DevP *tab = nullptr,
+                  int n = 1) {
+    if (M.enabled)
+        hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, true>), dim3(grid), dim3(256), 0, s, d, b, tab, n, M);
+    else
+        hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, false>), dim3(grid), dim3(256), 0, s, d, b, tab, n, M);
 }
 template <int KIND, int DEM, bool N32, bool BATCH = false>
 void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all, const DevP *tab = nullptr) {
@@ -1372,7 +1544,11 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     if (!dev_out) {
         if (int e = ensure(&c.d_out, &c.out_bytes, L.total)) return e;
     }
-    const size_t work_need = npix * sizeof(Cand) + npix * sizeof(int32_t) + 512;
+    MlpDev M;
+    mlp_dev(o.mlp, o, M);
+    if (f64 || post_only) M.enabled = 0;
+    const size_t base_need = (npix * sizeof(Cand) + npix * sizeof(int32_t) + 511) & ~(size_t)255;
+    const size_t work_need = base_need + (M.enabled ? learn_bufs(nullptr, d.nslots_q, nullptr) : 0);
     if (int e = ensure(&c.d_work, &c.work_bytes, work_need)) return e;
     Bufs b;
     std::memset(&b, 0, sizeof b);
@@ -1399,6 +1575,7 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     b.cand = (Cand *)c.d_work;
     b.hits = (int32_t *)((char *)c.d_work + npix * sizeof(Cand));
     b.ctr = c.d_ctr;
+    if (M.enabled) learn_bufs((char *)c.d_work + base_need, d.nslots_q, &b);
 
     HIP_TRY(hipMemsetAsync(c.d_ctr, 0, C_WORDS * sizeof(unsigned long long), c.stream));
     if (post_only && !dev_out)
@@ -1416,7 +1593,12 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
                 launch_finish<decltype(K)::value, decltype(D)::value, false>(grid_flat * 2, stream, d, b, 1);
             });
         } else {
-            dispatch(kind, dem, [&](auto K, auto D) { launch_march<decltype(K)::value, decltype(D)::value>(grid_march, stream, d, b); });
+            if (M.enabled)
+                hipLaunchKernelGGL(k_zero32, dim3(64), dim3(256), 0, stream, b.ready,
+                                   (d.nslots_q + LEARN_CHUNK - 1) / LEARN_CHUNK);
+            dispatch(kind, dem, [&](auto K, auto D) {
+                launch_march<decltype(K)::value, decltype(D)::value>(grid_march, stream, d, b, M);
+            });
             HIP_TRY(hipEventRecord(c.ev[3], c.stream));
             dispatch(kind, dem, [&](auto K, auto D) {
                 launch_finish<decltype(K)::value, decltype(D)::value, decltype(D)::value != DEM_NUMERIC>(grid_flat, stream, d, b, 0);
@@ -1494,6 +1676,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     t.its_box_f64 = ctr[C_STATS + 6];
     t.hits = ctr[C_STATS + 7];
     t.overflow = ctr[C_STATS + 8];
+    t.jumps = ctr[C_STATS + 9];
+    t.rollbacks = ctr[C_STATS + 10];
     t.kernel_ms = ms_march;
     t.post_ms = ms_post;
     t.march_f32_ms = split ? ms_a : 0.0;
@@ -1522,6 +1706,7 @@ struct Plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    MlpDev M;  // the warm start (M.enabled = 0: none)
 };
 
 void plan_free(Plan *pl) {
@@ -1542,10 +1727,13 @@ void plan_free(Plan *pl) {
 // not time)
 int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, pl.d_ctr, (int)C_WORDS);
+    if (pl.M.enabled)
+        hipLaunchKernelGGL(k_zero32, dim3(64), dim3(256), 0, stream, pl.b.ready,
+                           (pl.P0.nslots_q + LEARN_CHUNK - 1) / LEARN_CHUNK);
     const DevP *tab = pl.d_tab;
     const int n = pl.n;
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
-        launch_march<decltype(K)::value, decltype(D)::value, true>(pl.grid_march, stream, pl.P0, pl.b, tab, n);
+        launch_march<decltype(K)::value, decltype(D)::value, true>(pl.grid_march, stream, pl.P0, pl.b, pl.M, tab, n);
     });
     if (split) HIP_TRY(hipEventRecord(pl.e1, stream));
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
@@ -1595,6 +1783,8 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
         return set_error(MB3D_HIP_ERR_INVALID, "a batch holds at most 2^31 pixels");
     }
     pl->npix = pix;
+    mlp_dev(o.mlp, o, pl->M);
+    const size_t base_need = ((size_t)pix * (sizeof(Cand) + sizeof(int32_t)) + 4096 + 255) & ~(size_t)255;
     pl->P0 = pl->tab[0];
     pl->P0.nslots_q = q;
     pl->P0.npix = pix;
@@ -1606,7 +1796,7 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
         return e;
     };
     if (hipMalloc((void **)&pl->d_tab, sizeof(DevP) * n) != hipSuccess ||
-        hipMalloc(&pl->d_work, (size_t)pix * (sizeof(Cand) + sizeof(int32_t)) + 4096) != hipSuccess ||
+        hipMalloc(&pl->d_work, base_need + (pl->M.enabled ? learn_bufs(nullptr, q, nullptr) : 0)) != hipSuccess ||
         hipMalloc((void **)&pl->d_ctr, C_WORDS * sizeof(unsigned long long)) != hipSuccess)
         return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipMalloc failed for a batch"));
     if (hipMemcpy(pl->d_tab, pl->tab.data(), sizeof(DevP) * n, hipMemcpyHostToDevice) != hipSuccess)
@@ -1622,6 +1812,7 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     pl->b.gbuf = out->gbuf;
     pl->b.cand = (Cand *)pl->d_work;
     pl->b.hits = (int32_t *)((char *)pl->d_work + (size_t)pix * sizeof(Cand));
+    if (pl->M.enabled) learn_bufs((char *)pl->d_work + base_need, q, &pl->b);
     pl->b.ctr = pl->d_ctr;
     if (graph) {
         if (hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -1671,6 +1862,8 @@ int plan_run(Plan &pl, int repeats) {
     t.its_box_f64 = ctr[C_STATS + 6];
     t.hits = ctr[C_STATS + 7];
     t.overflow = ctr[C_STATS + 8];
+    t.jumps = ctr[C_STATS + 9];
+    t.rollbacks = ctr[C_STATS + 10];
     t.kernel_ms = ms_all > 0.0f ? ms_all / repeats : -1.0;
     t.march_f32_ms = ms_a;
     t.finish_f64_ms = ms_a > 0.0f ? t.kernel_ms - ms_a : -1.0;
@@ -1804,6 +1997,35 @@ extern "C" int32_t mb3d_hip_de_at(const mb3d_hip_params *params, const double *p
         return MB3D_HIP_OK;
     } catch (...) {
         return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_de_at");
+    }
+}
+
+extern "C" int32_t mb3d_hip_rays(const mb3d_hip_params *params, int32_t row0, int32_t rows, float *rays,
+                                 float *step_len) {
+    try {
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        Ctx &c = g_ctx;
+        if (c.device < 0) return set_error(MB3D_HIP_ERR_NOT_INIT, "mb3d_hip_init was not called");
+        if (!params || row0 < 0 || rows < 0 || row0 + rows > params->height)
+            return set_error(MB3D_HIP_ERR_INVALID, "bad argument");
+        const long long n = (long long)rows * params->width;
+        if (n == 0) return MB3D_HIP_OK;
+        mb3d_hip_options o;
+        std::memset(&o, 0, sizeof o);
+        DevP d;
+        fill_devp(*params, o, row0, rows, d);
+        if (int e = ensure(&c.d_work, &c.work_bytes, (size_t)n * 28 + 1024)) return e;
+        float *dr = (float *)c.d_work;
+        float *dl = dr + (size_t)n * 6;
+        const int grid = (int)std::min<long long>((n + 255) / 256, (long long)c.cus * 8);
+        hipLaunchKernelGGL(k_rays, dim3(grid), dim3(256), 0, c.stream, d, dr, dl);
+        HIP_TRY(hipGetLastError());
+        if (rays) HIP_TRY(hipMemcpyAsync(rays, dr, (size_t)n * 24, hipMemcpyDeviceToHost, c.stream));
+        if (step_len) HIP_TRY(hipMemcpyAsync(step_len, dl, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));
+        HIP_TRY(hipStreamSynchronize(c.stream));
+        return MB3D_HIP_OK;
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_rays");
     }
 }
 

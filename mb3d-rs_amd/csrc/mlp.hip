@@ -480,15 +480,17 @@ extern "C" int32_t mb3d_hip_mlp_infer(mb3d_hip_mlp *h, const float *rays, int64_
     return MB3D_HIP_OK;
 }
 
-// the device form of a network for the march (used by march.hip)
+// the device form of a network for the march
 namespace mb3d {
 void mlp_dev(const mb3d_hip_mlp *h, const mb3d_hip_options &o, MlpDev &M) {
     std::memset(&M, 0, sizeof M);
     if (!h) return;
     M.w = h->packed;
     M.b = h->bias;
-    M.conf_hit = o.mlp_conf_hit > 0.0f ? o.mlp_conf_hit : 0.5f;
-    M.conf_miss = o.mlp_conf_miss > 0.0f ? o.mlp_conf_miss : 0.02f;
+    // 0 selects the default; a hit threshold above 1 or a negative miss
+    // threshold turns that kind of jump off
+    M.conf_hit = o.mlp_conf_hit != 0.0f ? o.mlp_conf_hit : 0.5f;
+    M.conf_miss = o.mlp_conf_miss != 0.0f ? o.mlp_conf_miss : 0.02f;
     M.margin_rel = o.mlp_margin_rel > 0.0f ? o.mlp_margin_rel : 0.08f;
     M.margin_abs = o.mlp_margin_abs;
     M.enabled = 1;

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/mb3d_hip.h"
+
 namespace mb3d {
 
 constexpr int MLP_IN = 6, MLP_FREQ = 6, MLP_K1 = 80, MLP_H = 64, MLP_OUT = 2;
@@ -37,8 +39,13 @@ struct MlpDev {
     int enabled;
 };
 
+// (host) the device form of `net` with the options' thresholds; disabled
+// (enabled = 0) when net is NULL
+void mlp_dev(const mb3d_hip_mlp *net, const mb3d_hip_options &o, MlpDev &M);
+
 typedef short mlp_s4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const mlp_s4 mlp_gs4;  // a weight tile's lane share, global memory
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
@@ -53,9 +60,9 @@ __device__ __forceinline__ void mlp_features(const float o[3], const float d[3],
     int c = 0;
 #pragma unroll
     for (int i = 0; i < MLP_IN; i++) {
-        // sin and cos of pi x, then the double angle for each octave
-        float s, co;
-        sincosf(3.14159265358979f * x[i], &s, &co);
+        // sin and cos of pi x (the hardware's v_sin/v_cos take revolutions;
+        // |x| <= ~1 here), then the double angle for each octave
+        float s = __builtin_amdgcn_sinf(0.5f * x[i]), co = __builtin_amdgcn_cosf(0.5f * x[i]);
 #pragma unroll
         for (int k = 0; k < MLP_FREQ; k++) {
             row[c] = f2bf(s);
@@ -79,7 +86,7 @@ constexpr int MLP_LDS_BYTES = 16 * MLP_K1 * 2 + 16 * MLP_H * 2 + 16 * 2 * 4;
 // (row stride `ld` bf16), K = 16 * kt_n, N = 16 * nt_n; the result (bias,
 // ReLU) into `out` as bf16 (row stride MLP_H), or the 2 outputs as f32.
 template <int KT, int NT, bool LAST>
-__device__ __forceinline__ void mlp_layer(const uint16_t *in, int ld, const uint16_t *__restrict__ w,
+__device__ __forceinline__ void mlp_layer(const uint16_t *in, int ld, const uint16_t *w,
                                           const float *__restrict__ bias, uint16_t *out, float *outf) {
     const int l = __lane_id();
     const int row = l & 15, kq = (l >> 4) * 4;
@@ -90,12 +97,19 @@ __device__ __forceinline__ void mlp_layer(const uint16_t *in, int ld, const uint
         a[kt] = mlp_s4{(short)p[0], (short)p[1], (short)p[2], (short)p[3]};
     }
     __builtin_amdgcn_wave_barrier();
+    // the weight tiles are loaded where they are used: an empty asm makes
+    // the offset opaque, so that the compiler cannot hoist the loads of all
+    // four layers out of the callers' loops and keep ~110 registers of
+    // weights live through the march
+    int woff = 0;
+    asm volatile("" : "+s"(woff));
+    const mlp_gs4 *wt = (const mlp_gs4 *)(w + woff);
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) {
         mlp_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int kt = 0; kt < KT; kt++) {
-            const mlp_s4 b = *reinterpret_cast<const mlp_s4 *>(w + ((kt * NT + nt) * 64 + l) * 4);
+            const mlp_s4 b = wt[(kt * NT + nt) * 64 + l];
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kt], b, acc, 0, 0, 0);
         }
         const int n = nt * 16 + (l & 15);
@@ -127,8 +141,15 @@ __device__ __forceinline__ void mlp_wave_infer(const MlpDev &M, const float o[3]
     const uint16_t *w2 = w1 + 5 * 4 * 256;
     const uint16_t *w3 = w2 + 4 * 4 * 256;
     const uint16_t *w4 = w3 + 4 * 4 * 256;
+#pragma unroll 1
     for (int mt = 0; mt < 4; mt++) {
-        if ((l >> 4) == mt) mlp_features(o, d, in + (l & 15) * MLP_K1);
+        if ((l >> 4) == mt) {
+            // computed per m-tile from opaque copies of the ray: computed once
+            // for all four, the 80 features would stay live in registers
+            float oo[3] = {o[0], o[1], o[2]}, dd[3] = {d[0], d[1], d[2]};
+            asm volatile("" : "+v"(oo[0]), "+v"(oo[1]), "+v"(oo[2]), "+v"(dd[0]), "+v"(dd[1]), "+v"(dd[2]));
+            mlp_features(oo, dd, in + (l & 15) * MLP_K1);
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");

@@ -228,6 +228,7 @@ SIGNATURES = {
     ),
     "mb3d_hip_counters_get": (C.c_int32, [C.POINTER(Counters)]),
     "mb3d_hip_de_at": (C.c_int32, [C.POINTER(Params), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "mb3d_hip_rays": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     "mb3d_hip_march_batch": (C.c_int32, [C.POINTER(Params), C.c_int32, C.POINTER(Out), C.POINTER(Options)]),
     "mb3d_hip_plan_create": (C.c_int32, [C.POINTER(Params), C.c_int32, C.POINTER(Out), C.POINTER(Options),
                                          C.POINTER(C.c_void_p)]),
@@ -501,13 +502,21 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
-def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=16, guard_band=0.25, it_split=0) -> Options:
+def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=16, guard_band=0.25, it_split=0, mlp=None,
+            conf_hit=0.0, conf_miss=0.0, margin_rel=0.0, margin_abs=0.0) -> Options:
+    """Options; `mlp` (an Mlp) turns the warm start on (the thresholds: 0
+    for the library's defaults, conf_hit > 1 or conf_miss < 0 to turn that
+    kind of jump off)."""
     o = Options()
     o.precision = precision
     o.tile_w, o.tile_h = tile
     o.waves_per_cu = waves_per_cu
     o.guard_band = guard_band
     o.it_split = it_split
+    if mlp is not None:
+        o.mlp = mlp.h
+        o.mlp_conf_hit, o.mlp_conf_miss = conf_hit, conf_miss
+        o.mlp_margin_rel, o.mlp_margin_abs = margin_rel, margin_abs
     return o
 
 
@@ -743,3 +752,37 @@ class Mlp:
             self.close()
         except Exception:
             pass
+
+
+def rays(p: Params, row0=0, rows=None):
+    """The pixels' rays as the network takes them (mb3d_hip_rays): n x 6
+    float32 (start point, unit direction) and the march unit's length."""
+    rows = p.height - row0 if rows is None else rows
+    n = rows * p.width
+    r = np.empty((n, 6), np.float32)
+    sl = np.empty(n, np.float32)
+    check(load().mb3d_hip_rays(C.byref(p), row0, rows, _p(r), _p(sl)), "mb3d_hip_rays")
+    return r, sl
+
+
+def training_pairs(p: Params, opts: Options = None):
+    """(ray, exact hit distance) pairs of every pixel of `p` from the plain
+    marcher: rays n x 6, targets n x 2 (distance in scene units / 4, hit)."""
+    g = march(p, opts=opts, outputs=("depth64",))
+    r, sl = rays(p)
+    d = g["depth64"]
+    hit = np.isfinite(d) & (d > 0)
+    t = np.where(hit, d * sl.astype(np.float64), 0.0)
+    return r, np.stack([t / 4.0, hit.astype(np.float64)], 1).astype(np.float32)
+
+
+def train_warm_start(params_list, steps=3000, batch=8192, lr=3e-3, seed=0, net=None):
+    """A network trained on the plain marcher's pairs of every frame of
+    `params_list`; returns (net, loss curve)."""
+    rs, ts = zip(*(training_pairs(p) for p in params_list))
+    rays_all = np.concatenate(rs)
+    tg_all = np.concatenate(ts)
+    perm = np.random.default_rng(seed).permutation(len(rays_all))
+    net = net if net is not None else Mlp(seed)
+    curve = net.train(rays_all[perm], tg_all[perm], steps=steps, batch=batch, lr=lr)
+    return net, curve
