@@ -49,7 +49,8 @@ extern "C" {
 #define MB3D_HIP_DE_ANALYTIC_P8 1 /* extension: running derivative, 0.5 r ln r / dr (one IntPow slot only) */
 
 /* ---- iterate precision of the march ---- */
-#define MB3D_HIP_PREC_AUTO 0 /* FP32 unless the zoom is beyond it (the reference's needs_double rule) */
+#define MB3D_HIP_PREC_AUTO 0 /* FP32 for an analytic DE unless the zoom is beyond it (the reference's
+                              * needs_double rule); FP64 for a numeric DE */
 #define MB3D_HIP_PREC_F32 1  /* FP32 iterate; FP64 ray position, stop decision, bisection, normals */
 #define MB3D_HIP_PREC_F64 2  /* FP64 everywhere */
 

@@ -52,7 +52,6 @@ enum : int {
     C_FINQ = 3,    // next item of the f64 finish
     C_POSTQ = 4,   // next item of the shadow pass
     C_AOQ = 5,     // next item of the DE AO pass
-    C_INFQ = 6,    // next chunk of the warm start's inference
     C_STATS = 8,   // 11 work counters from here
     C_WORDS = 24
 };
@@ -67,10 +66,8 @@ struct Bufs {
     int32_t *hits;
     unsigned long long *ctr;
     // the warm start: the network's proposal per work item of the queue
-    // (hit distance / 4, hit logit) and a ready flag per chunk of
-    // LEARN_CHUNK items (zeroed before the launch)
+    // (hit distance / 4, hit logit)
     float2 *prop;
-    uint32_t *ready;
 };
 
 // DE AO ray set (build_rays, src/deao.rs:75-105), computed on the host
@@ -256,16 +253,50 @@ __device__ __forceinline__ bool item_pixel(const DevP &P0, const DevP *__restric
     return x < F.width && yr < F.rows;
 }
 
-constexpr int LEARN_CHUNK = 256;
+// The warm start's proposals: the network (BF16 MFMA, 64 rays a wave) on
+// the ray of every work item of the queue, written to prop[q] (hit distance
+// / 4, hit logit).  A kernel of its own, ahead of the march in the same
+// stream: a march wave cannot take the proposals of a chunk another wave
+// computed in the same kernel without an agent-scope release and acquire,
+// which on this GPU write back and invalidate an XCD's whole L2 (measured:
+// +0.75 ms a 1080p frame at a fence pair per chunk of 256 rays).
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__restrict__ tab, int nframes, MlpDev M,
+                                                     float2 *prop) {
+    __shared__ __attribute__((aligned(16))) char lds[4][MLP_LDS_BYTES];
+    __shared__ mlp_s4 wl[MLP_TILES * 64];
+    mlp_stage_weights(M, wl);
+    __syncthreads();
+    const int lane = lane_id();
+    const long long groups = (P0.nslots_q + 63) / 64;
+    const long long waves = (long long)gridDim.x * 4;
+    // a uniform loop: every lane of a wave runs the same groups
+    for (long long g = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); g < groups; g += waves) {
+        const long long q = g * 64 + lane;
+        float o[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 1.0f};
+        int f = 0, x = 0, yr = 0;
+        const bool in_q = q < P0.nslots_q;
+        if (in_q && item_pixel<BATCH>(P0, tab, nframes, q, f, x, yr)) {
+            const DevP &F = BATCH ? tab[f] : P0;
+            double cc[3], vv[3];
+            pixel_ray(F, x, F.row0 + yr, cc, vv);
+            const double il = 1.0 / sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+            for (int j = 0; j < 3; j++) {
+                o[j] = (float)cc[j];
+                d[j] = (float)(vv[j] * il);
+            }
+        }
+        float t = 0.0f, lg = 0.0f;
+        mlp_wave_infer(M, wl, o, d, lds[threadIdx.x >> 6], t, lg);
+        if (in_q) prop[q] = make_float2(t, lg);
+    }
+}
 
 // BATCH: the frames of a batch in one launch; `tab` holds each frame's
 // parameters, P0 the batch's queue totals and the shared tile shape.
-// LEARN: the warm start.  The kernel's waves first run the network (BF16
-// MFMA, all 64 lanes, 4 x 64 rays a chunk) over chunks of the queue, each
-// chunk's proposals published with a ready flag; then they march.  A ray
-// whose chunk is ready starts where the network puts it: confident hits at
-// the predicted distance less a margin, confident misses past the end (the
-// background); a ray whose chunk is not ready yet marches plainly (no wait).
+// LEARN: the warm start (B.prop holds k_mlp_propose's proposals).  A ray
+// starts where the network puts it: confident hits at the predicted
+// distance less a margin, confident misses past the end (the background).
 // A start that is not free -- the DE there below the stop distance (with
 // the hand-off guard) or the iteration limit reached -- is rolled back to
 // the plain march from the start plane.  This is the reference's cone-march
@@ -274,47 +305,8 @@ constexpr int LEARN_CHUNK = 256;
 template <int KIND, int DEM, bool BATCH, bool LEARN>
 __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__restrict__ tab, int nframes,
                                                  MlpDev M) {
-    __shared__ __attribute__((aligned(16))) char learn_lds[LEARN ? 4 : 1][LEARN ? MLP_LDS_BYTES : 1];
     Cnt cnt;
     int fr = 0;
-    const int lane = lane_id();
-    if constexpr (LEARN) {
-        const long long nchunks = (P0.nslots_q + LEARN_CHUNK - 1) / LEARN_CHUNK;
-        char *lds = learn_lds[threadIdx.x >> 6];
-        // No lane-specific branch in this loop: with `if (lane == 0)` around
-        // the atomic and the flag store, the compiler threaded the two
-        // branches into a second loop that the other lanes ran without the
-        // first (which had made the atomic), re-reading a stale chunk index
-        // for ever.  Every lane makes the atomic (adding 0 but the first) and
-        // every lane stores the flag.
-        for (;;) {
-            const unsigned long long k = first_lane_u64(atomicAdd(&B.ctr[C_INFQ], lane == 0 ? 1ull : 0ull));
-            if ((long long)k >= nchunks) break;
-#pragma unroll 1
-            for (int r = 0; r < LEARN_CHUNK / 64; r++) {
-                const long long q = (long long)k * LEARN_CHUNK + r * 64 + lane;
-                float o[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 1.0f};
-                int f = 0, x = 0, yr = 0;
-                const bool in_q = q < P0.nslots_q;
-                if (in_q && item_pixel<BATCH>(P0, tab, nframes, q, f, x, yr)) {
-                    const DevP &F = BATCH ? tab[f] : P0;
-                    double cc[3], vv[3];
-                    pixel_ray(F, x, F.row0 + yr, cc, vv);
-                    const double il = 1.0 / sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
-                    for (int j = 0; j < 3; j++) {
-                        o[j] = (float)cc[j];
-                        d[j] = (float)(vv[j] * il);
-                    }
-                }
-                float t = 0.0f, lg = 0.0f;
-                mlp_wave_infer(M, o, d, lds, t, lg);
-                if (in_q) B.prop[q] = make_float2(t, lg);
-            }
-            // the chunk's proposals reach L2 before its flag
-            __threadfence();
-            __hip_atomic_store(&B.ready[k], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
     bool have = false, exhausted = false;
     int pix = 0, px = 0, py = 0;
     double c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
@@ -355,14 +347,8 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
                     jumped = false;
                     first = F.first_step_random != 0;
                     seed = first ? dev_pixel_seed(py, px) : 0;
-                    // (agent-scope loads: the flag and the proposals come from L2,
-                    // where the writing wave's release put them)
-                    if (LEARN && __hip_atomic_load(&B.ready[q / LEARN_CHUNK], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT)) {
-                        const unsigned long long raw = __hip_atomic_load(
-                            reinterpret_cast<unsigned long long *>(B.prop + q), __ATOMIC_RELAXED,
-                            __HIP_MEMORY_SCOPE_AGENT);
-                        const float2 pr = make_float2(__uint_as_float((uint32_t)raw), __uint_as_float((uint32_t)(raw >> 32)));
+                    if constexpr (LEARN) {
+                        const float2 pr = B.prop[q];
                         const float ph = 1.0f / (1.0f + __expf(-pr.y));
                         double jump = 0.0;
                         if (ph >= M.conf_hit) {
@@ -1214,21 +1200,11 @@ __global__ void k_rays(DevP P, float *rays, float *step_len) {
     }
 }
 
-// zeroes n words (a kernel, not a memset: it may be captured in a graph)
-__global__ void k_zero32(uint32_t *p, long long n) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
-}
-
-// the warm start's buffers behind `base` (8-byte aligned): the proposals of
-// `items` work items, then the chunk flags; returns their size in bytes
+// the warm start's proposal buffer behind `base` (8-byte aligned) for
+// `items` work items; returns its size in bytes
 size_t learn_bufs(void *base, long long items, Bufs *b) {
-    const long long chunks = (items + LEARN_CHUNK - 1) / LEARN_CHUNK;
-    if (b) {
-        b->prop = (float2 *)base;
-        b->ready = (uint32_t *)((char *)base + (size_t)items * sizeof(float2));
-    }
-    return (size_t)items * sizeof(float2) + (size_t)chunks * sizeof(uint32_t) + 256;
+    if (b) b->prop = (float2 *)base;
+    return (size_t)items * sizeof(float2) + 256;
 }
 
 // the surface pixels of a G-buffer marched elsewhere
@@ -1310,7 +1286,12 @@ bool needs_f64(const mb3d_hip_params &p) {
     const double precision = (double)p.de_offset / (coord * eps);
     const double step_precision = p.step_width / (coord * eps);
     const bool numeric = !p.is_custom_de;
-    return (numeric && precision < 3.0) || step_precision < 64.0;
+    // beyond the reference's rule: a numeric DE's finite differences at
+    // de_offset keep too few FP32 bits for the stated tolerance (the FP32
+    // march's trajectory drifts; ~0.6 % of a 1080p Mandelbulb's depths
+    // leave 1e-4), so a numeric DE marches in FP64 unless FP32 is asked for
+    (void)precision;
+    return numeric || step_precision < 64.0;
 }
 
 void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, int rows, DevP &d) {
@@ -1446,6 +1427,13 @@ int dem_of(const mb3d_hip_params &p) {
     if (p.de_option == 11) return DEM_BOX;
     if (p.de_option == 32) return DEM_P8;
     return DEM_NUMERIC;
+}
+
+template <bool BATCH>
+void launch_propose(hipStream_t s, const DevP &d, const DevP *tab, int n, const MlpDev &M, float2 *prop) {
+    const long long groups = (d.nslots_q + 63) / 64;
+    const int grid = (int)std::max<long long>(1, std::min<long long>((groups + 3) / 4, (long long)g_ctx.cus * 16));
+    hipLaunchKernelGGL((k_mlp_propose<BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, M, prop);
 }
 
 template <int KIND, int DEM, bool BATCH = false>
@@ -1593,9 +1581,7 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
                 launch_finish<decltype(K)::value, decltype(D)::value, false>(grid_flat * 2, stream, d, b, 1);
             });
         } else {
-            if (M.enabled)
-                hipLaunchKernelGGL(k_zero32, dim3(64), dim3(256), 0, stream, b.ready,
-                                   (d.nslots_q + LEARN_CHUNK - 1) / LEARN_CHUNK);
+            if (M.enabled) launch_propose<false>(stream, d, nullptr, 1, M, b.prop);
             dispatch(kind, dem, [&](auto K, auto D) {
                 launch_march<decltype(K)::value, decltype(D)::value>(grid_march, stream, d, b, M);
             });
@@ -1727,9 +1713,7 @@ void plan_free(Plan *pl) {
 // not time)
 int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, pl.d_ctr, (int)C_WORDS);
-    if (pl.M.enabled)
-        hipLaunchKernelGGL(k_zero32, dim3(64), dim3(256), 0, stream, pl.b.ready,
-                           (pl.P0.nslots_q + LEARN_CHUNK - 1) / LEARN_CHUNK);
+    if (pl.M.enabled) launch_propose<true>(stream, pl.P0, pl.d_tab, pl.n, pl.M, pl.b.prop);
     const DevP *tab = pl.d_tab;
     const int n = pl.n;
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {

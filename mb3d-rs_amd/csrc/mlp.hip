@@ -176,13 +176,16 @@ __global__ void k_mlp_clear_pad(float *p) {
 // the MFMA path of the march, on its own: one wave per 64 rays
 __global__ __launch_bounds__(256) void k_mlp_infer_mfma(MlpDev M, const float *rays, long long n, float *out) {
     __shared__ __attribute__((aligned(16))) char lds[4 * MLP_LDS_BYTES];
+    __shared__ mlp_s4 wl[MLP_TILES * 64];
+    mlp_stage_weights(M, wl);
+    __syncthreads();
     const int wave = threadIdx.x >> 6;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long j = i < n ? i : n - 1;
     const float o[3] = {rays[6 * j], rays[6 * j + 1], rays[6 * j + 2]};
     const float d[3] = {rays[6 * j + 3], rays[6 * j + 4], rays[6 * j + 5]};
     float t, lg;
-    mlp_wave_infer(M, o, d, lds + wave * MLP_LDS_BYTES, t, lg);
+    mlp_wave_infer(M, wl, o, d, lds + wave * MLP_LDS_BYTES, t, lg);
     if (i < n) {
         out[2 * i] = t;
         out[2 * i + 1] = lg;

@@ -39,12 +39,30 @@ def test_bulb_analytic_640x480(hip, seed):
 
 
 def test_bulb_numeric_1080p_band(hip):
+    """The reference's own numeric DE: the automatic precision marches it
+    in FP64, which meets the bar."""
     sc = H.scene_bulb(1920, 1080, seed=2, de_mode=H.DE_REFERENCE)
     p = hip.params_from_scene(sc)
     g = hip.march(p, row0=400, rows=96)
+    assert g["counters"]["de_evals_f32"] == 0
     r = O.render(O.params(sc), row0=400, rows=96, threads=0)
     m = parity.compare(g, r, p.width, 96, p.step_width, 2.0)
     check(m)
+
+
+def test_bulb_numeric_fp32_error_is_bounded(hip):
+    """The numeric DE forced to FP32: its finite differences at de_offset
+    keep too few bits for the bar (the march's trajectory drifts), the
+    error it does have is bounded: mask >= 99.9 %, >= 99 % of the depths
+    within 1e-4 of the scene scale, the mean depth error below 1e-4."""
+    sc = H.scene_bulb(1920, 1080, seed=2, de_mode=H.DE_REFERENCE)
+    p = hip.params_from_scene(sc)
+    g = hip.march(p, row0=400, rows=96, opts=H.options(precision=H.PREC_F32))
+    assert g["counters"]["de_evals_f32"] > 0
+    r = O.render(O.params(sc), row0=400, rows=96, threads=0)
+    m = parity.compare(g, r, p.width, 96, p.step_width, 2.0)
+    assert m["mask_agree"] >= 0.999, m
+    assert m["depth_ok_frac"] >= 0.99 and m["depth_rel_mean"] < 1e-4, m
 
 
 def test_box_1080p_band(hip):
