@@ -1,17 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark of the HIP DE ray marcher on one MI355X.
+"""Benchmark of the HIP DE ray marcher on one MI355X: learned + batched
+against plain.
 
 Workload: the power-8 Mandelbulb Z-buffer at 1920 x 1080 (the small
 configuration's scene -- 60 iterations, bailout r^2 = 16, analytic DE
 0.5 r ln r / dr, camera start plane 2.5 from the origin, FOV 30, seeded yaw
-and pitch, 6 bisection steps, 500-step cap -- rendered at 1080p).  One step
-is one full 1080p frame marched through the C ABI into device buffers (the
-inputs are parameters; the outputs stay in HBM).
+and pitch, 6 bisection steps, 500-step cap -- rendered at 1080p).  Inputs
+are parameters; the outputs stay in device memory (HBM).
 
-Prints one JSON line: Mpixels/s, the roofline figure of the dominant kernel
-(the FP32 march, its algorithmic FP32 FLOPs over its HIP-event time), the CPU
-baseline (the f64 oracle on the host cores) and the accuracy of the frame
-against the oracle.
+- plain: one frame per call of mb3d_hip_march, the plain FP32 march and its
+  f64 finish.
+- plain + batched: the same frames in one HIP-graph plan without the
+  network (what batching alone gives).
+- learned + batched (the value): the warm-start network, trained first
+  (untimed) on the plain marcher's (ray, hit distance) pairs of the same
+  view at 480 x 270, and `--batch` copies of the frame in one HIP-graph
+  plan (one persistent launch of each kernel for all of them); one step is
+  one replay of the plan.
+
+Prints one JSON line: Mpixels/s of both paths, the roofline figure of the
+dominant kernel (its algorithmic FLOPs over its HIP-event time, on the
+library's stream), the CPU baseline (the f64 oracle on the host cores) and
+the accuracy of both paths against the oracle.
 """
 from __future__ import annotations
 
@@ -28,8 +38,8 @@ import numpy as np  # noqa: E402
 
 import mb3d_hip as H  # noqa: E402
 
-PEAK_FP32 = 157.3e12  # MI355X dense FP32 vector, FLOP/s
-PEAK_FP64 = 78.6e12
+PEAK_FP32 = 157.3e12  # MI355X FP32 vector, FLOP/s (published)
+PEAK_FP64 = 78.6e12   # MI355X FP64 vector, FLOP/s (published)
 # algorithmic FLOPs per unit of work (the reference's formulas, counted by op)
 F_P8 = 60.0     # int_pow power 8 (src/formulas.rs:837-858) + the rout update
 F_BOX = 31.0    # amazing_box (src/formulas.rs:707-736) + the rout update
@@ -40,13 +50,14 @@ F_DE = 8.0      # DE tail: sqrt, log, divide, scale, clamp
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10, help="timed replays of the batch (and plain frames x batch)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=4, help="frames per launch of the learned + batched path")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--de", choices=["analytic", "numeric"], default="analytic")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy legs")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also copy a 512 MiB device buffer once (the HBM counter scale of a profile)")
@@ -57,32 +68,29 @@ def flops_f32(c: dict) -> float:
     return c["its_p8_f32"] * F_P8 + c["its_box_f32"] * F_BOX + c["de_evals_f32"] * F_DE + c["steps"] * F_STEP
 
 
+def flops_f64(c: dict) -> float:
+    return c["its_p8_f64"] * F_P8 + c["its_box_f64"] * F_BOX + c["de_evals_f64"] * F_DE
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): every box of this build has one GPU, "
-              "and the marcher shards by pixel bands per GPU (see DESIGN.md)", file=sys.stderr)
+              "and the marcher shards by pixel bands or frames per GPU (see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     import torch
 
     lib = H.load()
     H.check(lib.mb3d_hip_init(0), "mb3d_hip_init")
-    de_mode = H.DE_ANALYTIC_P8 if a.de == "analytic" else H.DE_REFERENCE
-    sc = H.scene_bulb(a.width, a.height, seed=a.seed, de_mode=de_mode)
+    sc = H.scene_bulb(a.width, a.height, seed=a.seed)
     p = H.params_from_scene(sc)
     n = a.width * a.height
     dev = torch.device("cuda:0")
-    depth = torch.empty(n, dtype=torch.float32, device=dev)
-    iters = torch.empty(n, dtype=torch.int16, device=dev)
-    normals = torch.empty(n * 3, dtype=torch.float32, device=dev)
+    depth = torch.empty(n * a.batch, dtype=torch.float32, device=dev)
+    iters = torch.empty(n * a.batch, dtype=torch.int16, device=dev)
+    normals = torch.empty(n * a.batch * 3, dtype=torch.float32, device=dev)
     out = H.Out(depth.data_ptr(), None, iters.data_ptr(), normals.data_ptr(), None, 1, 0)
-    opts = H.options()
-
-    def step():
-        st = lib.mb3d_hip_march(H.C.byref(p), None, 0, a.height, H.C.byref(out), H.C.byref(opts),
-                                H.PROGRESS_FN(0), H.CANCEL_FN(0), None)
-        H.check(st, "mb3d_hip_march")
-        return H.counters()
+    plain_opts = H.options()
 
     if a.copy_calibration:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)
@@ -90,33 +98,91 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+
+    # --- the network (untimed): the plain marcher's pairs of the same view
+    t = time.perf_counter()
+    tw, th = 480, a.height * 480 // a.width
+    net, curve = H.train_warm_start([H.params_from_scene(H.scene_bulb(tw, th, seed=a.seed))],
+                                    steps=a.train_steps, batch=8192, lr=3e-3, seed=0)
+    train_s = time.perf_counter() - t
+    learn_opts = H.options(mlp=net)
+
+    def plain_frame():
+        st = lib.mb3d_hip_march(H.C.byref(p), None, 0, a.height, H.C.byref(out), H.C.byref(plain_opts),
+                                H.PROGRESS_FN(0), H.CANCEL_FN(0), None)
+        H.check(st, "mb3d_hip_march")
+        return H.counters()
+
+    # --- plain: one frame per call, the same number of frames as the batch leg
     for _ in range(a.warmup):
-        step()
+        plain_frame()
     torch.cuda.synchronize()
-    k_ms, f32_ms, cs = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(a.steps * a.batch):
+        cp = plain_frame()
+    torch.cuda.synchronize()
+    plain_wall = time.perf_counter() - t0
+    plain_value = n * a.steps * a.batch / plain_wall / 1e6
+
+    # --- plain + batched: the same plan without the network (what the
+    # batching alone gives)
+    frames = H.params_array([sc] * a.batch)
+    plan = H.Plan(frames, out, plain_opts)
+    for _ in range(a.warmup):
+        plan.launch(1)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        c = step()
-        k_ms.append(c["kernel_ms"])
-        f32_ms.append(c["march_f32_ms"])
-        cs.append(c)
+        plan.launch(1)
+    torch.cuda.synchronize()
+    pb_wall = time.perf_counter() - t0
+    plan.close()
+    plain_batched_value = n * a.batch * a.steps / pb_wall / 1e6
+
+    # --- learned + batched: the frame `batch` times in one graph plan
+    plan = H.Plan(frames, out, learn_opts)
+    for _ in range(a.warmup):
+        plan.launch(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        cl = plan.launch(1)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / a.steps
-    value = n * a.steps / wall / 1e6
-    c = cs[-1]
-    f32_avg = float(np.mean(f32_ms))
-    achieved = flops_f32(c) / (f32_avg * 1e-3)
+    plan.close()
+    value = n * a.batch * a.steps / wall / 1e6
+
+    # --- the dominant kernel: per-kernel HIP-event times of single learned
+    # frames (the events bracket k_march32 and k_finish64 on the library's
+    # stream), averaged
+    ms_m, ms_f, cf = [], [], None
+    for _ in range(max(3, a.steps)):
+        st = lib.mb3d_hip_march(H.C.byref(p), None, 0, a.height, H.C.byref(out), H.C.byref(learn_opts),
+                                H.PROGRESS_FN(0), H.CANCEL_FN(0), None)
+        H.check(st, "mb3d_hip_march")
+        cf = H.counters()
+        ms_m.append(cf["march_f32_ms"])
+        ms_f.append(cf["finish_f64_ms"])
+    t_march, t_finish = float(np.mean(ms_m)), float(np.mean(ms_f))
+    k_march = {"kernel": "k_mlp_propose + k_march32", "bound": "valu-fp32", "kernel_ms": t_march,
+               "flops_per_launch": flops_f32(cf), "peak": PEAK_FP32 / 1e12}
+    k_finish = {"kernel": "k_finish64", "bound": "valu-fp64", "kernel_ms": t_finish,
+                "flops_per_launch": flops_f64(cf), "peak": PEAK_FP64 / 1e12}
+    for k in (k_march, k_finish):
+        k["achieved"] = k["flops_per_launch"] / (k["kernel_ms"] * 1e-3) / 1e12
+        k["frac"] = k["achieved"] / k["peak"]
+    dom = k_finish if t_finish >= t_march else k_march
     roofline = {
-        "bound": "valu-fp32",
-        "kernel": "k_march32",
-        "achieved": achieved / 1e12,
-        "peak": PEAK_FP32 / 1e12,
+        "bound": dom["bound"],
+        "kernel": dom["kernel"],
+        "achieved": dom["achieved"],
+        "peak": dom["peak"],
         "unit": "TFLOP/s",
-        "frac": achieved / PEAK_FP32,
+        "frac": dom["frac"],
         "traffic": None,
-        "kernel_ms": f32_avg,
-        "flops_per_launch": flops_f32(c),
+        "kernel_ms": dom["kernel_ms"],
+        "flops_per_launch": dom["flops_per_launch"],
+        "kernels": [k_march, k_finish],
     }
     result = {
         "metric": "Mpixels/s 1080p Mandelbulb-p8 Z-buffer; learned vs plain at equal depth error",
@@ -125,42 +191,57 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": wall * 1e3 / a.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 iterate, fp64 ray/stop/bisection/normals",
-        "data": "synthetic (seeded camera)",
+        "dtype": "f32/f64",
+        "data": "synthetic (seeded camera, random-initialised network trained on the plain marcher's pairs)",
         "config": {
-            "workload": f"mandelbulb-p8 z-buffer {a.width}x{a.height}, {a.de} DE, 60 its, bailout r2=16, "
-                        f"fov 30, seed {a.seed}, plain FP32 marcher",
-            "path": "plain",
+            "workload": f"mandelbulb-p8 z-buffer {a.width}x{a.height}, analytic DE, 60 its, bailout r2=16, "
+                        f"fov 30, seed {a.seed}; learned warm start + {a.batch} frames per graph launch",
+            "path": "learned+batched",
+            "batch": a.batch,
         },
-        "kernel_ms_per_frame": float(np.mean(k_ms)),
-        "counters": {k: c[k] for k in ("hits", "steps", "de_evals_f32", "de_evals_f64", "its_p8_f32",
-                                        "its_p8_f64", "overflow")},
+        "plain": {"value": plain_value, "unit": "Mpixels/s", "ms_per_frame": plain_wall * 1e3 / (a.steps * a.batch),
+                  "kernel_ms_per_frame": cp["kernel_ms"], "march_f32_ms": cp["march_f32_ms"],
+                  "finish_f64_ms": cp["finish_f64_ms"], "steps": cp["steps"], "its_p8_f32": cp["its_p8_f32"]},
+        "plain_batched": {"value": plain_batched_value, "unit": "Mpixels/s",
+                          "ms_per_frame": pb_wall * 1e3 / (a.steps * a.batch)},
+        "speedup_vs_plain": value / plain_value,
+        "speedup_vs_plain_batched": value / plain_batched_value,
+        "learned": {"ms_per_frame": wall * 1e3 / (a.steps * a.batch), "jumps": cl["jumps"] // a.batch,
+                    "rollbacks": cl["rollbacks"] // a.batch, "steps": cl["steps"] // a.batch,
+                    "its_p8_f32": cl["its_p8_f32"] // a.batch, "train_s": train_s,
+                    "train_loss": [float(curve[:20].mean()), float(curve[-100:].mean())]},
         "roofline": roofline,
     }
     if not a.no_cpu:
-        result["cpu_baseline"], result["accuracy"] = cpu_leg(sc, p, a)
+        result["cpu_baseline"], result["accuracy"] = cpu_leg(sc, p, a, learn_opts)
+    net.close()
     print(json.dumps(result))
 
 
-def cpu_leg(sc, p, a):
+def cpu_leg(sc, p, a, learn_opts):
     """The f64 oracle (a restatement of the reference's CPU marcher) on the
-    host cores, on the same frame, and the frame's accuracy against it."""
+    host cores, on the same frame, and both paths' accuracy against it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as O
     import parity
 
-    threads = a.cpu_threads or os.cpu_count() or 1
+    threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
     po = O.params(sc)
     t = time.perf_counter()
     ref = O.render(po, threads=threads)
     dt = time.perf_counter() - t
-    g = H.march(p)
-    acc = parity.compare(g, ref, p.width, p.height, p.step_width, 2.0)
+    keys = ("mask_agree", "depth_rel_p99", "depth_rel_max", "depth_ok_frac", "normal_ok_frac", "normal_deg_p99",
+            "iters_equal_inner")
+    acc = {}
+    for name, o in (("plain", None), ("learned", learn_opts)):
+        g = H.march(p, opts=o)
+        m = parity.compare(g, ref, p.width, p.height, p.step_width, 2.0)
+        acc[name] = {k: m[k] for k in keys}
     cpu = {
         "value": p.width * p.height / dt / 1e6,
         "unit": "Mpixels/s",
