@@ -1,0 +1,202 @@
+// gpu.cpp -- the reference's `crate::gpu` interface over the C ABI (see
+// gpu.hpp).  Bands of rows go through mb3d_hip_march one after another, so
+// that rows reach the sink, and progress and cancel are asked, between
+// bands (as the wgpu backend's slices, src/gpu.rs:3215-3486).
+#include "gpu.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+namespace mb3d::gpu {
+
+namespace {
+
+constexpr size_t BAND_ROWS = 128;  // rows per call between sink, progress and cancel
+
+std::atomic<bool> &enabled_flag() {
+    static std::atomic<bool> on([] {
+        const char *e = std::getenv("MB3D_GPU");
+        return !(e && std::strcmp(e, "0") == 0);
+    }());
+    return on;
+}
+
+std::atomic<bool> g_aborted{false};
+std::mutex g_status_mu;
+std::string g_status;
+thread_local std::string t_status;
+thread_local bool t_rows_bad = false;
+
+std::string error_text() {
+    const char *e = mb3d_hip_last_error();
+    return e ? std::string(e) : std::string();
+}
+
+struct CallCtx {
+    const Progress *progress;
+    const Cancel *cancel;
+};
+
+// Marches rows [row0, row0 + rows) into `words` (5 per pixel, host memory).
+int march_rows(const mb3d_hip_params &p, const mb3d_hip_post_job *post, size_t row0, size_t rows, uint32_t *words,
+               const mb3d_hip_options *options) {
+    mb3d_hip_out out;
+    std::memset(&out, 0, sizeof out);
+    out.gbuf = words;
+    return mb3d_hip_march(&p, post, (int32_t)row0, (int32_t)rows, &out, options, nullptr, nullptr, nullptr);
+}
+
+// The start of a card calculation: switched on, not stopped, supported,
+// device open.  An empty text means go.
+std::string may_start(const mb3d_hip_params &p) {
+    if (!enabled()) return "the graphics card is switched off";
+    if (aborted()) return "stopped";
+    if (auto why = unsupported(p)) return *why;
+    if (mb3d_hip_init(0) != MB3D_HIP_OK) return error_text();
+    return std::string();
+}
+
+}  // namespace
+
+SiLight unpack(const uint32_t w[5]) {
+    SiLight s;
+    s.normal[0] = (int16_t)(uint16_t)(w[0] & 0xFFFF);
+    s.normal[1] = (int16_t)(uint16_t)(w[0] >> 16);
+    s.normal[2] = (int16_t)(uint16_t)(w[1] & 0xFFFF);
+    s.shadow = (uint16_t)(w[1] >> 16);
+    s.zpos_fine = w[2];
+    s.si_gradient = (uint16_t)(w[3] & 0xFFFF);
+    s.otrap = (uint16_t)(w[3] >> 16);
+    s.amb_shadow = (uint16_t)(w[4] & 0xFFFF);
+    return s;
+}
+
+void pack(const SiLight &s, uint32_t w[5]) {
+    w[0] = (uint32_t)(uint16_t)s.normal[0] | ((uint32_t)(uint16_t)s.normal[1] << 16);
+    w[1] = (uint32_t)(uint16_t)s.normal[2] | ((uint32_t)s.shadow << 16);
+    w[2] = s.zpos_fine;
+    w[3] = (uint32_t)s.si_gradient | ((uint32_t)s.otrap << 16);
+    w[4] = s.amb_shadow;
+}
+
+bool enabled() { return enabled_flag().load(); }
+void set_enabled(bool on) { enabled_flag().store(on); }
+
+std::string last_status() {
+    std::lock_guard<std::mutex> lk(g_status_mu);
+    return g_status;
+}
+
+std::string thread_status() { return t_status; }
+
+void set_status(const std::string &s) {
+    t_status = s;
+    std::lock_guard<std::mutex> lk(g_status_mu);
+    g_status = s;
+}
+
+bool rows_overflowed() { return t_rows_bad; }
+void clear_rows_overflowed() { t_rows_bad = false; }
+void abort_all() { g_aborted.store(true); }
+bool aborted() { return g_aborted.load(); }
+
+std::optional<std::string> unsupported(const mb3d_hip_params &p) {
+    char why[512] = {0};
+    const int st = mb3d_hip_supported(&p, why, sizeof why);
+    if (st == MB3D_HIP_OK) return std::nullopt;
+    if (st < 0) return error_text();
+    return std::string(why);
+}
+
+std::optional<std::vector<SiLight>> march(const mb3d_hip_params &p, const mb3d_hip_post_job *post,
+                                          const Progress &progress, const Cancel &cancel, const Sink &sink,
+                                          const mb3d_hip_options *options) {
+    t_rows_bad = false;
+    const std::string why = may_start(p);
+    if (!why.empty()) {
+        set_status("CPU: " + why);
+        return std::nullopt;
+    }
+    const size_t w = (size_t)p.width, h = (size_t)p.height;
+    std::vector<uint32_t> words(w * h * 5);
+    std::vector<SiLight> gbuf(w * h);
+    for (size_t row0 = 0; row0 < h; row0 += BAND_ROWS) {
+        if (cancel && cancel()) {
+            set_status("CPU: cancelled");
+            return std::nullopt;
+        }
+        const size_t rows = std::min(BAND_ROWS, h - row0);
+        const int st = march_rows(p, post, row0, rows, words.data() + row0 * w * 5, options);
+        if (st == MB3D_HIP_OVERFLOW) {
+            t_rows_bad = true;
+            set_status("CPU: single precision overflowed");
+            return std::nullopt;
+        }
+        if (st != MB3D_HIP_OK) {
+            set_status("CPU: " + error_text());
+            return std::nullopt;
+        }
+        for (size_t i = row0 * w; i < (row0 + rows) * w; i++) gbuf[i] = unpack(&words[i * 5]);
+        if (sink)
+            for (size_t y = row0; y < row0 + rows; y++) sink(y, gbuf.data() + y * w, w);
+        if (progress) progress(row0 + rows, h);
+    }
+    set_status("GPU (MI355X, HIP)");
+    return gbuf;
+}
+
+Status march_bands(const mb3d_hip_params &p, const mb3d_hip_post_job *post, const Bands &bands,
+                   const Cancel &cancel, const Sink &sink, const mb3d_hip_options *options) {
+    t_rows_bad = false;
+    const std::string why = may_start(p);
+    if (!why.empty()) return Status::fail(why);
+    const size_t w = (size_t)p.width, h = (size_t)p.height;
+    std::vector<uint32_t> words;
+    std::vector<SiLight> row(w);
+    for (size_t i = 0;; i++) {
+        if (cancel && cancel()) return Status::fail("cancelled");
+        const auto b = bands(i);
+        if (!b) break;
+        const size_t row0 = b->first, rows = b->second;
+        if (row0 >= h || rows == 0) continue;
+        const size_t n = std::min(rows, h - row0);
+        words.resize(n * w * 5);
+        const int st = march_rows(p, post, row0, n, words.data(), options);
+        if (st == MB3D_HIP_OVERFLOW) {
+            t_rows_bad = true;
+            return Status::fail("single precision overflowed");
+        }
+        if (st != MB3D_HIP_OK) return Status::fail(error_text());
+        for (size_t y = 0; y < n; y++) {
+            for (size_t x = 0; x < w; x++) row[x] = unpack(&words[(y * w + x) * 5]);
+            if (sink) sink(row0 + y, row.data(), w);
+        }
+    }
+    return Status{};
+}
+
+Status post_on_card(const mb3d_hip_params &p, std::vector<SiLight> &gbuf, const mb3d_hip_post_job &post,
+                    const Progress &progress, const Cancel &cancel) {
+    if (!enabled()) return Status::fail("the graphics card is switched off");
+    if (auto why = unsupported(p)) return Status::fail(*why);
+    if (mb3d_hip_init(0) != MB3D_HIP_OK) return Status::fail(error_text());
+    const size_t w = (size_t)p.width, h = (size_t)p.height;
+    if (gbuf.size() < w * h) return Status::fail("the G-buffer is smaller than the picture");
+    std::vector<uint32_t> words;
+    for (size_t row0 = 0; row0 < h; row0 += BAND_ROWS) {
+        if (cancel && cancel()) return Status::fail("cancelled");
+        const size_t rows = std::min(BAND_ROWS, h - row0);
+        words.resize(rows * w * 5);
+        for (size_t i = 0; i < rows * w; i++) pack(gbuf[row0 * w + i], &words[i * 5]);
+        const int st = mb3d_hip_post(&p, &post, (int32_t)row0, (int32_t)rows, words.data(), 0, nullptr);
+        if (st != MB3D_HIP_OK) return Status::fail(error_text());
+        for (size_t i = 0; i < rows * w; i++) gbuf[row0 * w + i] = unpack(&words[i * 5]);
+        if (progress) progress(row0 + rows, h);
+    }
+    return Status{};
+}
+
+}  // namespace mb3d::gpu
