@@ -599,6 +599,61 @@ def scene_deepzoom(width, height, seed, marcher, distance=1e-9, fov_deg=0.5) -> 
     return s
 
 
+def scene_bulb_deepzoom(width, height, seed, marcher, de, distance=1e-9, fov_deg=0.5,
+                        de_mode=DE_ANALYTIC_P8) -> Scene:
+    """A deep zoom on the power-8 Mandelbulb: a coarse 64 x 48 march of the
+    seeded bulb camera picks a surface pixel (seeded); along its ray the
+    first point where the orbit stops escaping is bracketed and bisected to
+    f64 resolution (`de(params, points) -> (de, its)`, its = max_it deciding
+    inside); the camera's start plane sits `distance` in front of it, FOV
+    `fov_deg`, the step width the angular pixel footprint there."""
+    coarse = scene_bulb(64, 48, seed=seed, de_mode=de_mode)
+    p = params_from_scene(coarse)
+    depth = np.asarray(marcher(p), np.float64)
+    hits = np.flatnonzero(np.isfinite(depth) & (depth > 0))
+    rng = np.random.default_rng(seed + 2000)
+    for _ in range(64):
+        i = int(hits[rng.integers(hits.size)])
+        x, y = i % 64, i // 64
+        vg = np.array([[p.vgrads[r][k] for k in range(3)] for r in range(3)])
+        c0 = np.array(list(p.ystart)) + vg[0] * x + vg[1] * y
+        cafy = (y / p.height - 0.5) * p.fov_y
+        cafx = float(np.float32((np.float32(p.fovx_off) - np.float32(x + 1)) * np.float32(p.fovx_mul)))
+        w = np.array([-math.sin(cafx), math.sin(cafy), math.cos(cafy) * math.cos(cafx)])
+        w /= np.linalg.norm(w)
+        v = w @ vg
+        u = v / np.linalg.norm(v)
+        t0 = depth[i] * np.linalg.norm(v)
+        ts = t0 + np.linspace(-0.05, 0.3, 351)
+        _, its = de(p, c0[None, :] + ts[:, None] * u[None, :])
+        inside = np.asarray(its) >= p.max_it
+        k = np.flatnonzero(~inside[:-1] & inside[1:])
+        if k.size:
+            break
+    else:
+        raise RuntimeError("no coarse ray enters the set")
+    lo, hi = ts[k[0]], ts[k[0] + 1]
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        if mid <= lo or mid >= hi:
+            break
+        _, it = de(p, (c0 + mid * u)[None, :])
+        if it[0] >= p.max_it:
+            hi = mid
+        else:
+            lo = mid
+    cam = c0 + lo * u - distance * u
+    s = scene_bulb(width, height, seed=seed, de_mode=de_mode)
+    s.fov_y = fov_deg
+    sw = distance * math.radians(fov_deg) / height
+    s.zoom = 2.1345 / (width * sw)
+    s.mid[:] = [float(c) for c in cam]
+    s.z_start = float(cam[2])  # CalcParams::new offsets the start plane by z_start - mid.z
+    s.z_end = s.z_start + 4.0 * distance
+    set_camera(s, look_matrix(u))
+    return s
+
+
 def de_at(p: Params, points):
     """mb3d_hip_de_at: (DE in world units, iteration counts) at points (n x 3)."""
     pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
