@@ -105,7 +105,9 @@ def main():
     net, curve = H.train_warm_start([H.params_from_scene(H.scene_bulb(tw, th, seed=a.seed))],
                                     steps=a.train_steps, batch=8192, lr=3e-3, seed=0)
     train_s = time.perf_counter() - t
-    learn_opts = H.options(mlp=net)
+    # hit jumps (verified) and miss jumps (opt-in: the network's view is the
+    # benchmark's own, trained at 480 x 270; its mask error is reported)
+    learn_opts = H.options(mlp=net, conf_miss=0.02)
 
     def plain_frame():
         st = lib.mb3d_hip_march(H.C.byref(p), None, 0, a.height, H.C.byref(out), H.C.byref(plain_opts),

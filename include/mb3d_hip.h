@@ -177,12 +177,15 @@ typedef struct mb3d_hip_options {
     int32_t waves_per_cu;  /* persistent grid: waves per CU (default 16) */
     int32_t it_split;      /* extension: march steps per resident turn before a ray yields (0: none) */
     float guard_band;      /* FP32 stop decisions within (1 + guard_band) x DE stop go to FP64 (default 0.25) */
-    /* the learned warm start (extension; NULL: the plain march): each ray
-     * starts at the network's proposal when it is confident (hit probability
-     * above mlp_conf_hit: the predicted hit distance less mlp_margin_rel of
-     * it and mlp_margin_abs scene units; below mlp_conf_miss: the predicted
-     * free distance), verified by the exact DE there and started from its
-     * plane again on an overshoot */
+    /* the learned warm start (extension; NULL: the plain march).  A ray
+     * whose hit probability is at least mlp_conf_hit (0: 0.5; above 1: no
+     * hit jumps) starts at the predicted hit distance less mlp_margin_rel of
+     * it (0: 0.08) and mlp_margin_abs scene units.  The start is kept when
+     * the exact DE there clears the stop distance's hand-off guard and a
+     * tenth of the predicted distance left, with the iteration limit not
+     * reached; otherwise the ray marches from its start plane.  A ray whose
+     * hit probability is at most mlp_conf_miss (0: no miss jumps) is
+     * background without a march -- unverified, so opt-in. */
     const mb3d_hip_mlp *mlp;
     float mlp_conf_hit, mlp_conf_miss, mlp_margin_rel, mlp_margin_abs;
 } mb3d_hip_options;

@@ -315,6 +315,7 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
     float ms = 0, rsf = 1, scount = 0;
     int nsteps = 0;
     bool start = false, first = false, jumped = false;
+    float jrem = 0.0f;  // a jump's predicted distance left to the surface (march units)
     int32_t seed = 0;
     const float fguard = P0.guard;
     WaveChunk wq, cq;
@@ -353,8 +354,12 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
                         double jump = 0.0;
                         if (ph >= M.conf_hit) {
                             const double vl = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                            const double tj = 4.0 * (double)pr.x * (1.0 - (double)M.margin_rel) - M.margin_abs;
-                            if (tj > 0.0) jump = tj / vl;
+                            const double th = 4.0 * (double)pr.x;
+                            const double tj = th * (1.0 - (double)M.margin_rel) - M.margin_abs;
+                            if (tj > 0.0) {
+                                jump = tj / vl;
+                                jrem = (float)((th - tj) / vl);
+                            }
                         } else if (ph <= M.conf_miss) {
                             jump = F.zend * 2.0;
                         }
@@ -387,10 +392,14 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
             bool top = true;
             if (LEARN && jumped) {
                 // the warm start's check: a free start, clear of the
-                // hand-off guard, is kept; any other goes back to the
-                // start plane and marches plainly
+                // hand-off guard and of a tenth of the distance the network
+                // left to the surface, is kept (a DE far under that distance
+                // means a surface nearer than predicted: one in front, or one
+                // the jump passed through); any other goes back to the start
+                // plane and marches plainly
                 jumped = false;
-                if (!(d32 < __builtin_inff()) || it >= P.max_it || !(d32 >= ms * fguard)) {
+                if (!(d32 < __builtin_inff()) || it >= P.max_it || !(d32 >= ms * fguard) ||
+                    !(d32 >= 0.1f * jrem)) {
                     cnt.rollbacks++;
                     pixel_ray(P, px, py, c, v);
                     mzz = 0.0;

@@ -490,10 +490,10 @@ void mlp_dev(const mb3d_hip_mlp *h, const mb3d_hip_options &o, MlpDev &M) {
     if (!h) return;
     M.w = h->packed;
     M.b = h->bias;
-    // 0 selects the default; a hit threshold above 1 or a negative miss
-    // threshold turns that kind of jump off
+    // a hit threshold of 0 selects 0.5, above 1 turns hit jumps off; miss
+    // jumps (unverified) only with a positive threshold
     M.conf_hit = o.mlp_conf_hit != 0.0f ? o.mlp_conf_hit : 0.5f;
-    M.conf_miss = o.mlp_conf_miss != 0.0f ? o.mlp_conf_miss : 0.02f;
+    M.conf_miss = o.mlp_conf_miss > 0.0f ? o.mlp_conf_miss : -1.0f;
     M.margin_rel = o.mlp_margin_rel > 0.0f ? o.mlp_margin_rel : 0.08f;
     M.margin_abs = o.mlp_margin_abs;
     M.enabled = 1;

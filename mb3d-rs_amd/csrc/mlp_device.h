@@ -36,7 +36,7 @@ struct MlpDev {
     const uint16_t *w;  // MLP_PACKED_U16 bf16 values, tile-major
     const float *b;     // MLP_BIAS: b1, b2, b3, b4 (padded to 16)
     float conf_hit;     // sigmoid(logit) above: a jump to the predicted hit
-    float conf_miss;    // below: a jump to the predicted free distance
+    float conf_miss;    // at most: background without a march (< 0: never)
     float margin_rel;   // the jump stops short of the prediction by this share
     float margin_abs;   // ... and this many scene units
     int enabled;

@@ -504,9 +504,9 @@ def _p(a):
 
 def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=16, guard_band=0.25, it_split=0, mlp=None,
             conf_hit=0.0, conf_miss=0.0, margin_rel=0.0, margin_abs=0.0) -> Options:
-    """Options; `mlp` (an Mlp) turns the warm start on (the thresholds: 0
-    for the library's defaults, conf_hit > 1 or conf_miss < 0 to turn that
-    kind of jump off)."""
+    """Options; `mlp` (an Mlp) turns the warm start on: hit jumps with the
+    library's defaults (conf_hit 0: 0.5, above 1: off; margin_rel 0: 0.08),
+    miss jumps (unverified) only with conf_miss > 0."""
     o = Options()
     o.precision = precision
     o.tile_w, o.tile_h = tile

@@ -35,7 +35,7 @@ def test_the_trained_network_saves_steps_within_the_bar(hip, trained):
     sc = H.scene_bulb(640, 360, seed=1)
     p = hip.params_from_scene(sc)
     plain = hip.march(p)
-    learned = hip.march(p, opts=H.options(mlp=trained))
+    learned = hip.march(p, opts=H.options(mlp=trained, conf_miss=0.02))
     cp, cl = plain["counters"], learned["counters"]
     # the gain: most rays start where the network puts them, and the FP32
     # march takes far fewer steps and iterations
@@ -60,8 +60,8 @@ def test_an_untrained_network_gains_nothing_and_breaks_nothing(hip, trained):
     raw = H.Mlp(seed=11)
     try:
         plain = hip.march(p)["counters"]
-        g_raw = hip.march(p, opts=H.options(mlp=raw))
-        g_tr = hip.march(p, opts=H.options(mlp=trained))
+        g_raw = hip.march(p, opts=H.options(mlp=raw, conf_miss=0.02))
+        g_tr = hip.march(p, opts=H.options(mlp=trained, conf_miss=0.02))
     finally:
         raw.close()
     r = O.render(O.params(sc), threads=0)
@@ -78,7 +78,7 @@ def test_jumps_off_is_the_plain_march(hip, trained):
     sc = H.scene_bulb(320, 240, seed=2)
     p = hip.params_from_scene(sc)
     plain = hip.march(p)
-    off = hip.march(p, opts=H.options(mlp=trained, conf_hit=2.0, conf_miss=-1.0))
+    off = hip.march(p, opts=H.options(mlp=trained, conf_hit=2.0))
     assert off["counters"]["jumps"] == 0
     for k in ("depth", "iters", "gbuf"):
         assert np.array_equal(off[k], plain[k]), k
