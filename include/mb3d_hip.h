@@ -273,10 +273,10 @@ int32_t mb3d_hip_de_at(const mb3d_hip_params *params, const double *points, int6
 
 /* ---- batches (no reference counterpart in one call: the reference renders
  * animation frames one by one, a card each, src/frames.rs:467-520) ----
- * Frames of any sizes with the same formula and DE mode, marched in one
- * persistent launch of each pass.  The outputs are device buffers holding
- * the frames one after another (frame f starts at the sum of the earlier
- * frames' width x height pixels). */
+ * Up to 64 frames of any sizes with the same formula and DE mode, marched
+ * in one persistent launch of each pass.  The outputs are device buffers
+ * holding the frames one after another (frame f starts at the sum of the
+ * earlier frames' width x height pixels). */
 int32_t mb3d_hip_march_batch(const mb3d_hip_params *frames, int32_t n, const mb3d_hip_out *out,
                              const mb3d_hip_options *options);
 

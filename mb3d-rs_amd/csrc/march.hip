@@ -224,6 +224,21 @@ __device__ int32_t dev_pixel_seed(int64_t y, int64_t x) {
 // ---------------------------------------------------------------------------
 // pass 1: the FP32 march, persistent, lanes refilled one by one
 
+// A batch's frame table copied to LDS (the dynamic LDS of the launch,
+// nframes x sizeof(DevP)): the march and finish read a frame's parameters in
+// their inner loops, and from global memory every read is a vector load
+// (the frame differs per lane).  All threads of the block call it.
+__device__ __forceinline__ const DevP *stage_frames(const DevP *__restrict__ tab, int nframes) {
+    extern __shared__ __attribute__((aligned(16))) uint2 frames_lds[];
+    const int words = nframes * (int)(sizeof(DevP) / sizeof(uint2));
+    const uint2 *src = reinterpret_cast<const uint2 *>(tab);
+    for (int i = threadIdx.x; i < words; i += blockDim.x) frames_lds[i] = src[i];
+    __syncthreads();
+    return reinterpret_cast<const DevP *>(frames_lds);
+}
+static_assert(sizeof(DevP) % sizeof(uint2) == 0, "DevP in 8-byte words");
+constexpr int MAX_BATCH_FRAMES = 64;  // 64 x 744 B of LDS
+
 // the frame of a batch that work item q belongs to
 __device__ __forceinline__ int find_frame(const DevP *__restrict__ tab, int n, long long q) {
     int lo = 0, hi = n - 1;
@@ -303,8 +318,9 @@ __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__rest
 // jump and check (march_pixel_from, src/calc.rs:2293-2308) with the
 // network's depth in place of the coarse ray's.
 template <int KIND, int DEM, bool BATCH, bool LEARN>
-__global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__restrict__ tab, int nframes,
+__global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__restrict__ gtab, int nframes,
                                                  MlpDev M) {
+    const DevP *__restrict__ tab = BATCH ? stage_frames(gtab, nframes) : gtab;
     Cnt cnt;
     int fr = 0;
     bool have = false, exhausted = false;
@@ -581,7 +597,9 @@ enum FinPhase : int { FP_START, FP_RESUME, FP_BACK, FP_STEP, FP_REF, FP_NC, FP_N
 // DEs (their FP32 error moves a normal by ~0.01 degree; the numeric
 // gradient's does not allow it)
 template <int KIND, int DEM, bool N32, bool BATCH>
-__global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixels, const DevP *__restrict__ tab) {
+__global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixels, const DevP *__restrict__ gtab,
+                                                  int nframes) {
+    const DevP *__restrict__ tab = BATCH ? stage_frames(gtab, nframes) : gtab;
     Cnt cnt;
     int fr = 0;
     const long long n_items = all_pixels ? P0.npix : (long long)B.ctr[C_CAND];
@@ -1448,14 +1466,17 @@ void launch_propose(hipStream_t s, const DevP &d, const DevP *tab, int n, const 
 template <int KIND, int DEM, bool BATCH = false>
 void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b, const MlpDev &M, const DevP *tab = nullptr,
                   int n = 1) {
+    const size_t lds = BATCH ? (size_t)n * sizeof(DevP) : 0;
     if (M.enabled)
-        hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, true>), dim3(grid), dim3(256), 0, s, d, b, tab, n, M);
+        hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, true>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
     else
-        hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, false>), dim3(grid), dim3(256), 0, s, d, b, tab, n, M);
+        hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, false>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
 }
 template <int KIND, int DEM, bool N32, bool BATCH = false>
-void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all, const DevP *tab = nullptr) {
-    hipLaunchKernelGGL((k_finish64<KIND, DEM, N32, BATCH>), dim3(grid), dim3(256), 0, s, d, b, all, tab);
+void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all, const DevP *tab = nullptr,
+                   int n = 1) {
+    const size_t lds = BATCH ? (size_t)n * sizeof(DevP) : 0;
+    hipLaunchKernelGGL((k_finish64<KIND, DEM, N32, BATCH>), dim3(grid), dim3(256), lds, s, d, b, all, tab, n);
 }
 template <int KIND, int DEM, bool F32>
 void launch_shadow(int grid, hipStream_t s, const DevP &d, const Bufs &b, const ShadowJob &j) {
@@ -1731,7 +1752,7 @@ int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
     if (split) HIP_TRY(hipEventRecord(pl.e1, stream));
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
         launch_finish<decltype(K)::value, decltype(D)::value, decltype(D)::value != DEM_NUMERIC, true>(
-            pl.grid_flat, stream, pl.P0, pl.b, 0, tab);
+            pl.grid_flat, stream, pl.P0, pl.b, 0, tab, n);
     });
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1742,6 +1763,7 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     Ctx &c = g_ctx;
     if (c.device < 0) return set_error(MB3D_HIP_ERR_NOT_INIT, "mb3d_hip_init was not called");
     if (!frames || n <= 0 || !out || !result) return set_error(MB3D_HIP_ERR_INVALID, "bad argument");
+    if (n > MAX_BATCH_FRAMES) return set_error(MB3D_HIP_ERR_INVALID, "a batch holds at most 64 frames");
     if (!out->on_device) return set_error(MB3D_HIP_ERR_INVALID, "a batch writes device buffers (out->on_device = 1)");
     mb3d_hip_options o;
     std::memset(&o, 0, sizeof o);
