@@ -175,7 +175,7 @@ typedef struct mb3d_hip_options {
     int32_t precision;     /* MB3D_HIP_PREC_* */
     int32_t tile_w, tile_h; /* pixel tile of one wave's work item (default 8 x 8) */
     int32_t waves_per_cu;  /* persistent grid: waves per CU (default 16) */
-    int32_t it_split;      /* extension: march steps per resident turn before a ray yields (0: none) */
+    int32_t it_split;      /* extension: FP32 march steps after which a ray goes on in the f64 finish (0: never) */
     float guard_band;      /* FP32 stop decisions within (1 + guard_band) x DE stop go to FP64 (default 0.25) */
     /* the learned warm start (extension; NULL: the plain march).  A ray
      * whose hit probability is at least mlp_conf_hit (0: 0.5; above 1: no
@@ -288,9 +288,10 @@ int32_t mb3d_hip_plan_create(const mb3d_hip_params *frames, int32_t n, const mb3
 int32_t mb3d_hip_plan_launch(mb3d_hip_plan *plan, int32_t repeats);
 void mb3d_hip_plan_destroy(mb3d_hip_plan *plan);
 
-/* The autotuner: sweeps the tile shape (8x8 .. 32x8) and the persistent grid
- * (waves per CU) on graphs of the batch and returns the fastest options in
- * `best` (its other fields are kept) and their kernel time. */
+/* The autotuner: sweeps the tile shape (8x8 .. 32x8), the persistent grid
+ * (waves per CU) and the iteration-cap split (it_split 0, 128, 256) on graphs
+ * of the batch and returns the fastest options in `best` (its other fields
+ * are kept) and their kernel time. */
 int32_t mb3d_hip_autotune(const mb3d_hip_params *frames, int32_t n, const mb3d_hip_out *out,
                           mb3d_hip_options *best, double *best_ms);
 

@@ -50,7 +50,7 @@ struct DevP {
     // a frame of a batch: its first work item in the batch's queue and its
     // first pixel in the batch's output arrays (0 for a single picture)
     int64_t q_begin, pix_base;
-    int sync_refill;  // the f64 finish refills a wave when all its lanes are free
+    int it_split;  // march steps after which the FP32 pass hands a ray to the f64 finish (0: none)
 };
 
 // Per-lane work counters, reduced per wave at the end of a kernel.

@@ -453,6 +453,12 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
                 } else if (it < P.min_it || dtmp >= ms * fguard) {
                     if (P.step_cap > 0 && nsteps >= P.step_cap) {
                         background = true;
+                    } else if (P.it_split > 0 && nsteps >= P.it_split) {
+                        // the iteration-cap split: a ray still marching after
+                        // it_split steps goes on in the f64 finish, which
+                        // resumes it from here, so that the long rays of a
+                        // launch do not hold the FP32 pass's tail
+                        handoff = true;
                     } else {
                         last_de = dtmp;
                         float d = fmaxf(0.11f, (dtmp - P.ms_de_sub * ms) * P.z_step_div * rsf);
@@ -621,7 +627,6 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
     chunk_init(hq);
     for (;;) {
         bool need = !have && !exhausted;
-        if (P0.sync_refill && __any(have)) need = false;
         for (int tries = 0; tries < 4 && __any(need); tries++) {
             const long long slot = wave_take(need, &B.ctr[C_FINQ], fq, 64, n_items);
             if (!need) continue;
@@ -1380,10 +1385,7 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
     d.step_width = p.step_width;
     std::memcpy(d.vg, p.vgrads, sizeof d.vg);
     std::memcpy(d.ystart, p.ystart, sizeof d.ystart);
-    {
-        const char *e = std::getenv("MB3D_HIP_FINISH_SYNC");  // debug: wave-synchronous refill of the f64 finish
-        d.sync_refill = e && e[0] == '1';
-    }
+    d.it_split = o.it_split > 0 ? o.it_split : 0;
     d.tile_w = o.tile_w > 0 ? o.tile_w : 8;
     d.tile_h = o.tile_h > 0 ? o.tile_h : 8;
     d.tiles_x = (p.width + d.tile_w - 1) / d.tile_w;
@@ -1603,7 +1605,6 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
     const int grid_march = std::max(1, c.cus * wpc / 4);
     int grid_flat = std::max(1, c.cus * 4);
-    if (const char *e = std::getenv("MB3D_HIP_FINISH_GRID")) grid_flat = std::max(1, c.cus * std::atoi(e));  // debug
     HIP_TRY(hipEventRecord(c.ev[0], c.stream));
     if (!post_only) {
         if (f64) {
@@ -1885,12 +1886,14 @@ int plan_run(Plan &pl, int repeats) {
     return MB3D_HIP_OK;
 }
 
-// The project's autotuner: every tile shape and persistent grid size of the
-// sweep on a graph of the batch, the fastest kept (min of `runs` replays).
+// The project's autotuner: every tile shape, persistent grid size and
+// iteration-cap split of the sweep on a graph of the batch, the fastest kept
+// (min of 3 replays each).
 int autotune_locked(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, mb3d_hip_options *best,
                     double *best_ms) {
     static const int tiles[][2] = {{8, 8}, {16, 4}, {16, 8}, {32, 4}, {32, 8}};
-    static const int wpcs[] = {8, 12, 16, 20, 24};
+    static const int wpcs[] = {12, 16, 20, 24};
+    static const int splits[] = {0, 128, 256};
     mb3d_hip_options o;
     std::memset(&o, 0, sizeof o);
     if (best) o = *best;
@@ -1898,24 +1901,27 @@ int autotune_locked(const mb3d_hip_params *frames, int n, const mb3d_hip_out *ou
     mb3d_hip_options win = o;
     for (const auto &t : tiles) {
         for (int w : wpcs) {
-            mb3d_hip_options c = o;
-            c.tile_w = t[0];
-            c.tile_h = t[1];
-            c.waves_per_cu = w;
-            Plan *pl = nullptr;
-            if (int e = plan_build(frames, n, out, &c, true, &pl)) return e;
-            double m = 1e300;
-            for (int r = 0; r < 3; r++) {
-                if (int e = plan_run(*pl, 1)) {
-                    plan_free(pl);
-                    return e;
+            for (int sp : splits) {
+                mb3d_hip_options c = o;
+                c.tile_w = t[0];
+                c.tile_h = t[1];
+                c.waves_per_cu = w;
+                c.it_split = sp;
+                Plan *pl = nullptr;
+                if (int e = plan_build(frames, n, out, &c, true, &pl)) return e;
+                double m = 1e300;
+                for (int r = 0; r < 3; r++) {
+                    if (int e = plan_run(*pl, 1)) {
+                        plan_free(pl);
+                        return e;
+                    }
+                    m = std::min(m, t_counters.kernel_ms);
                 }
-                m = std::min(m, t_counters.kernel_ms);
-            }
-            plan_free(pl);
-            if (m < top) {
-                top = m;
-                win = c;
+                plan_free(pl);
+                if (m < top) {
+                    top = m;
+                    win = c;
+                }
             }
         }
     }

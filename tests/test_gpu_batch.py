@@ -121,3 +121,58 @@ def test_batch_rejects_mixed_formulas(hip):
     bufs, out = device_out(64 * 48 * 2)
     assert hip.load().mb3d_hip_march_batch(arr, 2, H.C.byref(out), None) == -1
     assert "share" in H.last_error()
+
+
+def test_iteration_cap_split_keeps_the_bar(hip):
+    """it_split hands rays still marching after that many FP32 steps to the
+    f64 finish, which resumes them: the pictures stay within the bar."""
+    sc = H.scene_bulb(320, 240, seed=4)
+    p = hip.params_from_scene(sc)
+    r = O.render(O.params(sc), threads=0)
+    for sp in (8, 64):
+        g = hip.march(p, opts=H.options(it_split=sp))
+        m = parity.compare(g, r, p.width, p.height, p.step_width, 2.0)
+        assert m["mask_agree"] >= 0.999 and m["depth_ok_frac"] >= 0.999, (sp, m)
+        assert m["normal_ok_frac"] >= 0.999, (sp, m)
+
+
+def test_animation_of_64_frames_in_one_plan(hip):
+    """The batched configuration at its full size: 64 frames with sizes from
+    {320x240, 640x480, 1280x720, 1920x1080} along a seeded camera path, one
+    graph-captured plan after the autotuner's choice; a sampled frame is
+    checked against the oracle."""
+    import time
+
+    import torch
+
+    scenes = H.animation(64, seed=11)
+    arr = H.params_array(scenes)
+    n = sum(s.width * s.height for s in scenes)
+    dev = torch.device("cuda:0")
+    depth = torch.zeros(n, dtype=torch.float32, device=dev)
+    d64 = torch.zeros(n, dtype=torch.float64, device=dev)
+    iters = torch.zeros(n, dtype=torch.int16, device=dev)
+    normals = torch.zeros(3 * n, dtype=torch.float32, device=dev)
+    out = H.Out(depth.data_ptr(), d64.data_ptr(), iters.data_ptr(), normals.data_ptr(), 0, 1, 0)
+    best, ms = H.autotune(arr, out)
+    plan = H.Plan(arr, out, best)
+    try:
+        plan.launch(1)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan.launch(3)
+        wall = (time.perf_counter() - t) / 3
+    finally:
+        plan.close()
+    print(f"64 frames, {n / 1e6:.1f} Mpx: {wall * 1e3:.1f} ms a launch, {n / wall / 1e6:.0f} Mpix/s "
+          f"(tile {best.tile_w}x{best.tile_h}, {best.waves_per_cu} waves/CU, split {best.it_split})")
+    off = 0
+    for i, s in enumerate(scenes):
+        m = s.width * s.height
+        if i == 5:
+            r = O.render(O.params(s), threads=0)
+            g = {"depth64": d64[off:off + m].cpu().numpy(), "iters": iters[off:off + m].cpu().numpy().astype(np.uint16),
+                 "normals": normals[3 * off:3 * (off + m)].cpu().numpy().reshape(-1, 3)}
+            mm = parity.compare(g, r, s.width, s.height, arr[i].step_width, 2.0)
+            assert mm["mask_agree"] >= 0.999 and mm["depth_ok_frac"] >= 0.999, mm
+        off += m
