@@ -128,9 +128,12 @@ def main():
     plain_value = n * a.steps * a.batch / plain_wall / 1e6
 
     # --- plain + batched: the same plan without the network (what the
-    # batching alone gives)
+    # batching alone gives); both batched legs take the autotuner's tile,
+    # grid and iteration-cap split for their own options (untimed)
     frames = H.params_array([sc] * a.batch)
-    plan = H.Plan(frames, out, plain_opts)
+    pb_opts, _ = H.autotune(frames, out, H.options())
+    lb_opts, _ = H.autotune(frames, out, H.options(mlp=net, conf_miss=0.02))
+    plan = H.Plan(frames, out, pb_opts)
     for _ in range(a.warmup):
         plan.launch(1)
     torch.cuda.synchronize()
@@ -143,7 +146,7 @@ def main():
     plain_batched_value = n * a.batch * a.steps / pb_wall / 1e6
 
     # --- learned + batched: the frame `batch` times in one graph plan
-    plan = H.Plan(frames, out, learn_opts)
+    plan = H.Plan(frames, out, lb_opts)
     for _ in range(a.warmup):
         plan.launch(1)
     torch.cuda.synchronize()
@@ -161,7 +164,7 @@ def main():
     # library's stream; a captured graph does not time its nodes), averaged
     ms_m, ms_f, cf = [], [], None
     for _ in range(max(3, a.steps)):
-        H.check(lib.mb3d_hip_march_batch(frames, a.batch, H.C.byref(out), H.C.byref(learn_opts)), "march_batch")
+        H.check(lib.mb3d_hip_march_batch(frames, a.batch, H.C.byref(out), H.C.byref(lb_opts)), "march_batch")
         cf = H.counters()
         ms_m.append(cf["march_f32_ms"])
         ms_f.append(cf["finish_f64_ms"])
@@ -204,6 +207,8 @@ def main():
                         f"fov 30, seed {a.seed}; learned warm start + {a.batch} frames per graph launch",
             "path": "learned+batched",
             "batch": a.batch,
+            "autotuned": {"learned": [lb_opts.tile_w, lb_opts.tile_h, lb_opts.waves_per_cu, lb_opts.it_split],
+                          "plain_batched": [pb_opts.tile_w, pb_opts.tile_h, pb_opts.waves_per_cu, pb_opts.it_split]},
         },
         "plain": {"value": plain_value, "unit": "Mpixels/s", "ms_per_frame": plain_wall * 1e3 / (a.steps * a.batch),
                   "kernel_ms_per_frame": cp["kernel_ms"], "march_f32_ms": cp["march_f32_ms"],
