@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MB3D_HIP_ABI_VERSION 1
+#define MB3D_HIP_ABI_VERSION 2
 #define MB3D_HIP_MAX_SLOTS 6
 #define MB3D_HIP_MAX_LIGHTS 6
 
@@ -39,10 +39,18 @@ extern "C" {
 #define MB3D_HIP_ERR_DEVICE (-2)  /* HIP runtime error, no device */
 #define MB3D_HIP_ERR_NOT_INIT (-3)
 
-/* ---- formula kinds of a hybrid slot (src/formulas.rs:28-68) ---- */
+/* ---- formula kinds of a hybrid slot: the built-in formulas
+ *      (`Formula`, src/formulas.rs:28-68) ---- */
 #define MB3D_HIP_FORMULA_NONE 0
-#define MB3D_HIP_FORMULA_INTPOW 1     /* 'Integer Power' (triplex sine bulb); power 8 on the card */
-#define MB3D_HIP_FORMULA_AMAZINGBOX 2 /* 'Amazing Box' (Mandelbox) */
+#define MB3D_HIP_FORMULA_INTPOW 1        /* 'Integer Power' (triplex sine bulb), powers 2..8 */
+#define MB3D_HIP_FORMULA_AMAZINGBOX 2    /* 'Amazing Box' (Mandelbox) */
+#define MB3D_HIP_FORMULA_REALPOWER 3     /* 'Real Power' */
+#define MB3D_HIP_FORMULA_QUATERNION 4    /* 'Quaternion' (4D; DE option 4) */
+#define MB3D_HIP_FORMULA_TRICORN 5       /* 'Tricorn' */
+#define MB3D_HIP_FORMULA_BULBOX 6        /* 'Bulbox' (power-2 bulb / Amazing Box blend) */
+#define MB3D_HIP_FORMULA_FOLDINGINTPOW 7 /* 'Folding Int Pow' */
+#define MB3D_HIP_FORMULA_AEXIONC 8       /* 'Aexion C' */
+#define MB3D_HIP_FORMULA_OPTIONS 8       /* values in mb3d_hip_formula.options */
 
 /* ---- DE modes ---- */
 #define MB3D_HIP_DE_REFERENCE 0   /* the reference's: numeric gradient (option 0) or analytic box (11) */
@@ -51,7 +59,9 @@ extern "C" {
 /* ---- iterate precision of the march ---- */
 #define MB3D_HIP_PREC_AUTO 0 /* FP32 for an analytic DE unless the zoom is beyond it (the reference's
                               * needs_double rule); FP64 for a numeric DE */
-#define MB3D_HIP_PREC_F32 1  /* FP32 iterate; FP64 ray position, stop decision, bisection, normals */
+#define MB3D_HIP_PREC_F32 1  /* FP32 iterate; FP64 ray position, stop decision, bisection, normals.
+                              * Integer Power 8 and Amazing Box (and their hybrids) only: the other
+                              * formulas, other powers and the 4D hybrids march in FP64 */
 #define MB3D_HIP_PREC_F64 2  /* FP64 everywhere */
 
 /* One slot of the alternating hybrid (`FormulaEntry` + `Formula`,
@@ -65,6 +75,16 @@ typedef struct mb3d_hip_formula {
     double scale;       /* AmazingBox: scale */
     double min_r;       /* AmazingBox: Min R */
     double fold;        /* AmazingBox: fold */
+    /* the other kinds: the values of Formula::options() in its order
+     * (src/formulas.rs:308-357), integers and switches as 0/1 doubles:
+     *   Real Power     float power, Z multiplier
+     *   Quaternion     YW multiplier, W add
+     *   Tricorn        Z multiplier, CZ multiplier
+     *   Bulbox         box scale, box Min R, box fold, bulb scaling, R threshold, R threshold 2
+     *   Folding Int Pow  integer power, Z multiplier, R fold
+     *   Aexion C       float power, Z multiplier, rotate C, conditional phi, float power C,
+     *                  Cz multiplier, power C on |Z - C|, mode (0..31) */
+    double options[MB3D_HIP_FORMULA_OPTIONS];
 } mb3d_hip_formula;
 
 /* The subset of the reference's `Scene` (src/scene.rs:398-532) that feeds
@@ -87,7 +107,7 @@ typedef struct mb3d_hip_scene {
     int32_t n_formulas;        /* 1..6 */
     int32_t de_mode;           /* MB3D_HIP_DE_* */
     int32_t step_cap;          /* extension: max march steps per ray (0: the reference's guard of 2e6) */
-    int32_t reserved;
+    int32_t julia;             /* 0/1: Julia mode, the constant is julia_c (Scene::julia) */
     double rstop;              /* bailout radius (not squared); <= 0: the formulas' default */
     double zoom;
     double mid[3];
@@ -98,6 +118,8 @@ typedef struct mb3d_hip_scene {
     double z_step_div;         /* raystep */
     double raystep_limiter;
     double color_mul;
+    double julia_c[4];         /* the Julia constant (x, y, z, w); w is also J4 of the 3D hybrids */
+    double rot_4d[3];          /* XW, YW, ZW rotations of the 4D hybrids (radians) */
     mb3d_hip_formula formulas[MB3D_HIP_MAX_SLOTS];
 } mb3d_hip_scene;
 
@@ -123,6 +145,8 @@ typedef struct mb3d_hip_params {
     int32_t first_step_random, normals_on_de;
     int32_t color_option, dfog_on_it;
     int32_t step_cap;
+    int32_t four_d;          /* 1: the 4D alternating hybrid (DE options 4..6, src/calc.rs:475) */
+    int32_t do_julia;        /* Julia mode: the constant is julia_c */
     int32_t reserved;
     double slot_z_mul[MB3D_HIP_MAX_SLOTS];
     double slot_scale[MB3D_HIP_MAX_SLOTS];
@@ -136,6 +160,11 @@ typedef struct mb3d_hip_params {
     double d_rstop, rstop3d, fov_y, zcorr, zc_mul, zend, step_width;
     double vgrads[3][3];
     double ystart[3];
+    /* the options of the kinds other than Integer Power and Amazing Box
+     * (mb3d_hip_formula.options) */
+    double slot_opt[MB3D_HIP_MAX_SLOTS][MB3D_HIP_FORMULA_OPTIONS];
+    double smatrix4[4][4];   /* build_smatrix4 of rot_4d, f32-rounded (src/math.rs:178-204) */
+    double julia_c[4];       /* `ju` (src/calc.rs:52) */
 } mb3d_hip_params;
 
 /* The post passes that run with the march (`PostJob`, src/calc.rs:187-205;
@@ -198,8 +227,9 @@ typedef struct mb3d_hip_counters {
     uint64_t steps;          /* march steps */
     uint64_t de_evals_f32;   /* DE evaluations with the FP32 iterate */
     uint64_t de_evals_f64;   /* DE evaluations with the FP64 iterate */
-    uint64_t its_p8_f32, its_box_f32;  /* formula iterations by kind and precision */
-    uint64_t its_p8_f64, its_box_f64;
+    uint64_t its_p8_f32, its_box_f32;  /* formula iterations by kind and precision: Integer Power, */
+    uint64_t its_p8_f64, its_box_f64;  /* Amazing Box, */
+    uint64_t its_other_f64;            /* the other formulas (FP64 only) */
     uint64_t overflow;       /* pixels whose FP32 values overflowed */
     uint64_t jumps;          /* warm starts taken */
     uint64_t rollbacks;      /* warm starts rejected by the exact DE (overshoot) */
@@ -229,6 +259,13 @@ void mb3d_hip_shutdown(void);
 /* Scene defaults (Scene::default, src/scene.rs:626-700) with one Integer
  * Power p8 slot (src/formulas.rs:266-306). */
 void mb3d_hip_scene_default(mb3d_hip_scene *scene);
+
+/* Formula::default_for (src/formulas.rs:266-306): the defaults of the
+ * built-in formula `name` ("Integer Power", "Real Power", "Quaternion",
+ * "Tricorn", "Amazing Box", "Bulbox", "Folding Int Pow", "Aexion C", or
+ * the reference's aliases), one counted iteration.  MB3D_HIP_UNSUPPORTED
+ * for other names (the .m3f formulas). */
+int32_t mb3d_hip_formula_default(const char *name, mb3d_hip_formula *formula);
 
 /* CalcParams::new (src/calc.rs:375-898) for the built-in formulas. */
 int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *scene, mb3d_hip_params *params);

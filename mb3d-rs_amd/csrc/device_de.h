@@ -14,7 +14,9 @@
 //    stop near the surface, the bisection, the normals, the iteration count.
 //
 // Formula code: int_pow power 8 (src/formulas.rs:837-858) and amazing_box
-// (src/formulas.rs:707-736).
+// (src/formulas.rs:707-736) in both precisions; every other built-in
+// formula (Formula::iterate, src/formulas.rs:576-858), the 4D hybrids and
+// Julia mode in FP64 (kind K_GEN).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,7 +26,10 @@
 
 namespace mb3d {
 
-enum Kind : int { K_P8 = 0, K_BOX = 1, K_HYB = 2 };
+// K_P8, K_BOX, K_HYB: power-8 Integer Power, Amazing Box and their
+// hybrids, with an FP32 iterate; K_GEN: any built-in formula, the 4D
+// hybrids and Julia mode, FP64 only.
+enum Kind : int { K_P8 = 0, K_BOX = 1, K_HYB = 2, K_GEN = 3 };
 enum DeMode : int { DEM_NUMERIC = 0, DEM_BOX = 1, DEM_P8 = 2 };
 
 // The march parameters as the kernels take them (by value, so that the
@@ -34,9 +39,20 @@ struct DevP {
     int n_slots, end_to, repeat_from, max_it, min_it;
     int de_option, is_custom_de, de_add_steps, dfog_on_it, first_step_random, step_cap;
     int tile_w, tile_h, tiles_x;
-    int slot_kind[MB3D_HIP_MAX_SLOTS], slot_iters[MB3D_HIP_MAX_SLOTS], slot_ade[MB3D_HIP_MAX_SLOTS];
-    double slot_z_mul[MB3D_HIP_MAX_SLOTS], slot_scale[MB3D_HIP_MAX_SLOTS];
-    double slot_min_r[MB3D_HIP_MAX_SLOTS], slot_fold[MB3D_HIP_MAX_SLOTS];
+    int slot_kind[MB3D_HIP_MAX_SLOTS], slot_iters[MB3D_HIP_MAX_SLOTS];
+    // Integer Power, Folding Int Pow: the power; Aexion C: rotate C (1),
+    // conditional phi (2), power C on |Z - C| (4), mode << 8
+    int slot_iarg[MB3D_HIP_MAX_SLOTS];
+    int four_d, do_julia;
+    // the f64 options of each slot, Formula::options order without the
+    // integers (src/formulas.rs:308-357): Integer Power (z_mul); Amazing
+    // Box (scale, min_r, fold); Real Power (power, z_mul); Quaternion
+    // (yw_mul, w_add); Tricorn (z_mul, cz_mul); Bulbox (scale, min_r, fold,
+    // bulb_scale, r_threshold, r_threshold2); Folding Int Pow (z_mul, fold);
+    // Aexion C (power, z_mul, power_c, cz_mul)
+    double opt[MB3D_HIP_MAX_SLOTS][6];
+    double ju[4];    // the Julia constant (and J4)
+    float sm4[12];   // rows of smatrix4's first three columns (f32 values)
     float fz_mul[MB3D_HIP_MAX_SLOTS], fscale[MB3D_HIP_MAX_SLOTS], fmin_r2[MB3D_HIP_MAX_SLOTS];
     float fscale_min_r2[MB3D_HIP_MAX_SLOTS], ffold[MB3D_HIP_MAX_SLOTS];
     float d_de_scale, de_stop, z_step_div, mh04zsd, de_stop_factor, ln_rstop, de_offset, de_offset006;
@@ -56,7 +72,7 @@ struct DevP {
 // Per-lane work counters, reduced per wave at the end of a kernel.
 struct Cnt {
     uint32_t steps = 0, de32 = 0, de64 = 0;
-    uint32_t p8_32 = 0, box_32 = 0, p8_64 = 0, box_64 = 0;
+    uint32_t p8_32 = 0, box_32 = 0, p8_64 = 0, box_64 = 0, oth_64 = 0;
     uint32_t hits = 0, overflow = 0;
     uint32_t jumps = 0, rollbacks = 0;  // warm starts kept, warm starts undone
 };
@@ -266,53 +282,269 @@ __device__ __forceinline__ uint32_t clip15(float s) {
 // The f64 iteration record of one evaluation (Iteration,
 // src/iteration.rs:59-105), the parts the built-in formulas use.
 struct It64 {
-    double v[4], j[3];
+    double v[4], j[4];
     double rout, rold, otrap, deriv1;
     int it_result;
-    uint32_t p8, box;
+    uint32_t p8, box, oth;
 };
 
-// Formula::iterate via call_slot (src/iteration.rs:325-345)
+// int_pow (src/formulas.rs:769-858), powers 2..7 (8: int_pow8_64)
+__device__ __forceinline__ void int_pow_low64(int power, double zmul, double v[4], const double j[4]) {
+    const double x = v[0], y = v[1], z = v[2];
+    const double EPS = 1e-40;
+    const double sx = x * x, sy = y * y, sz = z * z;
+    if (power <= 2) {
+        const double r = sx + sy;
+        v[2] = (sqrt(r) * zmul * z) * 2.0 + j[2];
+        const double a = (r - z * z) / r;
+        v[0] = (sx - sy) * a + j[0];
+        v[1] = (x * y * a) * 2.0 + j[1];
+    } else if (power == 3) {
+        const double r = sx + sy;
+        const double a = 1.0 - 3.0 * sz / (r + EPS);
+        v[0] = (sx - 3.0 * sy) * a * x + j[0];
+        v[2] = j[2] - (sz - 3.0 * r) * z * zmul;
+        v[1] = a * (3.0 * sx - sy) * y + j[1];
+    } else if (power == 4) {
+        const double r = sx + sy;
+        const double a = 1.0 + sz * (sz - 6.0 * r) / (r * r + EPS);
+        v[0] = (sx * (sx - 6.0 * sy) + sy * sy) * a + j[0];
+        v[2] = sqrt(r) * (r - sz) * z * 4.0 * zmul + j[2];
+        v[1] = a * x * (sx - sy) * 4.0 * y + j[1];
+    } else if (power == 5) {
+        const double r = sx + sy;
+        const double a = 1.0 + (sz * sz - sz * r * 2.0) * 5.0 / (r * r + EPS);
+        v[1] = (5.0 * sx * sx - sy * (10.0 * sx - sy)) * a * y + j[1];
+        v[2] = (sz * (sz - 10.0 * r) + 5.0 * r * r) * z * zmul + j[2];
+        v[0] = a * x * (sx * (sx - 10.0 * sy) + 5.0 * sy * sy) + j[0];
+    } else if (power == 6) {
+        const double r = sx + sy, rr = r * r;
+        const double a = 1.0 - sz * (15.0 * rr + sz * (sz - r * 15.0)) / (rr * r + EPS);
+        v[1] = (3.0 * sy * sy + sx * (3.0 * sx - 10.0 * sy)) * a * y * x * 2.0 + j[1];
+        v[2] = (sz * (3.0 * sz - 10.0 * r) + 3.0 * rr) * sqrt(r) * z * zmul * 2.0 + j[2];
+        v[0] = (sy * sy * (15.0 * sx - sy) + sx * sx * (sx - 15.0 * sy)) * a + j[0];
+    } else {
+        const double r = sx + sy, rr = r * r;
+        const double a = 1.0 - 7.0 * (sz * (3.0 * rr + sz * (sz - 5.0 * r))) / (rr * r + EPS);
+        v[1] = (sx * (21.0 * sy * sy + sx * (7.0 * sx - 35.0 * sy)) - sy * sy * sy) * a * y + j[1];
+        v[2] = j[2] - (sz * sz * sz - 7.0 * r * (rr + sz * (3.0 * sz - 5.0 * r))) * z * zmul;
+        v[0] = a * x * (sx * (35.0 * sy * sy + sx * (sx - 21.0 * sy)) - 7.0 * sy * sy * sy) + j[0];
+    }
+}
+
+// HybridIntP8 (src/formulas.rs:837-858)
+__device__ __forceinline__ void int_pow8_64(double zmul, double v[4], const double j[4]) {
+    const double x = v[0], y = v[1], z = v[2];
+    const double xx = x * x, yy = y * y, zz = z * z;
+    const double r = xx + yy;
+    const double rr = r * r;
+    const double zzzz = zz * zz;
+    const double t = (zzzz - 6.0 * r * zz + rr) * (zz - r) * sqrt(r);
+    v[2] = -(t * z * 8.0 * zmul) + j[2];
+    const double num = (rr * 70.0 + zzzz) * zzzz - 28.0 * zz * r * (zzzz + rr);
+    const double a = num / (rr * rr + 1e-40) + 1.0;
+    const double xxxx = xx * xx, yyyy = yy * yy;
+    v[1] = (yyyy * (7.0 * xx - yy) + xxxx * (xx - 7.0 * yy)) * 8.0 * x * y * a + j[1];
+    v[0] = a * (xxxx * xxxx + yyyy * (70.0 * xxxx + yyyy) - 28.0 * (xx * yy * (yyyy + xxxx))) + j[0];
+}
+
+__device__ __forceinline__ void int_pow64(int power, double zmul, double v[4], const double j[4]) {
+    if (power >= 8)
+        int_pow8_64(zmul, v, j);
+    else
+        int_pow_low64(power, zmul, v, j);
+}
+
+// amazing_box (src/formulas.rs:713-736)
+__device__ __forceinline__ void box64(double scale, double min_r, double fold, double v[4], const double j[4],
+                                      bool ade) {
+    const double x = bfold64(v[0], fold), y = bfold64(v[1], fold), z = bfold64(v[2], fold);
+    const double r = z * z + y * y + x * x;
+    const double m = fmax(min_r, 1e-40);
+    const double sqr_min_r = m * m;
+    double mul;
+    if (r < sqr_min_r)
+        mul = scale / sqr_min_r;
+    else if (1.0 < r)
+        mul = scale;
+    else
+        mul = scale / r;
+    if (ade) v[3] *= mul;
+    v[2] = z * mul + j[2];
+    v[1] = y * mul + j[1];
+    v[0] = x * mul + j[0];
+}
+
+// int_pow2_scale (src/formulas.rs:739-752), the bulb of Bulbox
+__device__ __forceinline__ void pow2_scale64(double scaling, double v[4], const double j[4]) {
+    const double s = 1.0 / scaling;
+    const double x = v[0] * s, y = v[1] * s, z = v[2] * s;
+    const double xx = x * x, yy = y * y, r = xx + yy;
+    v[2] = -((sqrt(r) * z) * 2.0) * scaling + j[2];
+    const double a = (r - z * z) / r;
+    v[0] = (xx - yy) * a * scaling + j[0];
+    v[1] = (x * y * a) * 2.0 * scaling + j[1];
+}
+
+// Bulbox, HybridSuperCube2 (src/formulas.rs:596-627)
+__device__ __forceinline__ void bulbox64(const double o[6], double v[4], const double j[4], double rout) {
+    const double rt1 = o[4] * o[4], rt2 = o[5] * o[5];
+    if (!(rout < rt1)) {
+        box64(o[0], o[1], o[2], v, j, false);
+        return;
+    }
+    if (rout < rt2) {
+        pow2_scale64(o[3], v, j);
+        return;
+    }
+    const double m = (rout - rt2) / (rt1 - rt2);
+    const double in0 = v[0], in1 = v[1], in2 = v[2];
+    pow2_scale64(o[3], v, j);
+    const double b0 = v[0], b1 = v[1], b2 = v[2];
+    v[0] = in0;
+    v[1] = in1;
+    v[2] = in2;
+    box64(o[0], o[1], o[2], v, j, false);
+    double r1 = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double m2 = 1.0 - m;
+    const double r2 = r1 * m + m2 * sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+    v[0] = v[0] * m + m2 * b0;
+    v[1] = v[1] * m + m2 * b1;
+    v[2] = v[2] * m + m2 * b2;
+    r1 = r2 / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-40);
+    v[0] *= r1;
+    v[1] *= r1;
+    v[2] *= r1;
+}
+
+// real_power, HybridFloatPow (src/formulas.rs:755-766): angles by atan2,
+// the radius from the stored squared radius
+__device__ __forceinline__ void real_power64(double pw, double zmul, double v[4], const double j[4], double rout) {
+    const double x = v[0], y = v[1], z = v[2];
+    double st, ct, sp, cp;
+    sincos(atan2(y, x) * pw, &st, &ct);
+    sincos(atan2(z, sqrt(x * x + y * y)) * pw, &sp, &cp);
+    const double r = pow(rout, pw * 0.5);
+    v[0] = ct * cp * r + j[0];
+    v[1] = st * cp * r + j[1];
+    v[2] = sp * r * zmul + j[2];
+}
+
+// aexion_c (src/formulas.rs:644-703): a real power bulb step with the
+// angles from the y axis; with "rotate C" the constant is turned by a power
+// of its own angles (or those of Z - C, mode 16)
+__device__ __forceinline__ void aexion64(const double o[6], int flags, double v[4], double j[4]) {
+    const double pw = o[0];
+    const double x = v[0], y = v[1], z = v[2];
+    double st, ct, sp, cp;
+    sincos(atan2(sqrt(x * x + z * z), y) * pw, &st, &ct);
+    sincos(atan2(z, x) * pw, &sp, &cp);
+    const double r1 = exp(log(x * x + y * y + z * z) * pw * 0.5);
+    v[0] = cp * ct * r1 + j[0];
+    v[2] = st * r1 * o[1] + j[2];
+    v[1] = ct * sp * r1 + j[1];
+    if (!(flags & 1)) return;
+    const int mode = flags >> 8;
+    double pd = o[2];
+    if (flags & 4) {
+        const double d0 = v[0] - j[0], d1 = v[1] - j[1], d2 = v[2] - j[2];
+        pd *= sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    }
+    const double r = sqrt(j[0] * j[0] + j[1] * j[1] + j[2] * j[2]);
+    double a = (mode & 16) ? v[1] - j[1] : j[1];
+    double b = (mode & 16) ? v[2] - j[2] : j[2];
+    const double c = (mode & 16) ? v[0] - j[0] : j[0];
+    int tested = 0, other = 1;  // the component "conditional phi" tests: 0 x, 1 y, 2 z
+    if (mode & 8) {
+        const double t = a;
+        a = b;
+        b = t;
+        other = 2;
+    }
+    const double s = sqrt(b * b + c * c);
+    double th = (mode & 1) ? atan2(a, s) : atan2(s, a);
+    double ph;
+    if (mode & 2) {
+        tested = 3 - other;
+        ph = atan2(c, b);
+    } else {
+        ph = atan2(b, c);
+    }
+    if (mode & 4) {
+        const double t = th;
+        th = ph;
+        ph = t;
+        tested = other;
+    }
+    if (flags & 2) {
+        const double t = tested == 0 ? v[0] : (tested == 1 ? v[1] : v[2]);
+        if (!signbit(t)) ph = -ph;
+    }
+    double sa, ca, sb, cb;
+    sincos(ph * pd, &sa, &ca);
+    sincos(th * pd, &sb, &cb);
+    j[0] = ca * cb * r;
+    j[2] = sb * r * o[3];
+    j[1] = cb * sa * r;
+}
+
+// Formula::iterate via call_slot (src/iteration.rs:325-345,
+// src/formulas.rs:576-641).  The power-8 and box kinds keep their two
+// formulas inline; K_GEN selects among all built-ins.
+template <int KIND>
 __device__ __forceinline__ void call_slot64(const DevP &P, int n, It64 &s) {
     double *v = s.v;
-    const double *j = s.j;
-    if (P.slot_kind[n] == MB3D_HIP_FORMULA_INTPOW) {
-        if (P.slot_ade[n]) {  // the analytic power-8 extension: dr <- 8 r^7 dr + 1
-            const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-            s.deriv1 = 8.0 * (r2 * r2 * r2 * sqrt(r2)) * s.deriv1 + 1.0;
+    const double *o = P.opt[n];
+    const int kind = P.slot_kind[n];
+    if (KIND != K_GEN) {
+        if (kind == MB3D_HIP_FORMULA_INTPOW) {
+            if (P.is_custom_de) {  // the analytic power-8 extension: dr <- 8 r^7 dr + 1
+                const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                s.deriv1 = 8.0 * (r2 * r2 * r2 * sqrt(r2)) * s.deriv1 + 1.0;
+            }
+            int_pow8_64(o[0], v, s.j);
+            s.p8++;
+        } else {
+            box64(o[0], o[1], o[2], v, s.j, P.is_custom_de != 0);
+            s.box++;
         }
-        const double x = v[0], y = v[1], z = v[2];
-        const double xx = x * x, yy = y * y, zz = z * z;
-        const double r = xx + yy;
-        const double rr = r * r;
-        const double zzzz = zz * zz;
-        const double t = (zzzz - 6.0 * r * zz + rr) * (zz - r) * sqrt(r);
-        v[2] = -(t * z * 8.0 * P.slot_z_mul[n]) + j[2];
-        const double num = (rr * 70.0 + zzzz) * zzzz - 28.0 * zz * r * (zzzz + rr);
-        const double a = num / (rr * rr + 1e-40) + 1.0;
-        const double xxxx = xx * xx, yyyy = yy * yy;
-        v[1] = (yyyy * (7.0 * xx - yy) + xxxx * (xx - 7.0 * yy)) * 8.0 * x * y * a + j[1];
-        v[0] = a * (xxxx * xxxx + yyyy * (70.0 * xxxx + yyyy) - 28.0 * (xx * yy * (yyyy + xxxx))) + j[0];
-        s.p8++;
-    } else {
-        const double fold = P.slot_fold[n], scale = P.slot_scale[n];
-        const double x = bfold64(v[0], fold), y = bfold64(v[1], fold), z = bfold64(v[2], fold);
-        const double r = z * z + y * y + x * x;
-        const double m = fmax(P.slot_min_r[n], 1e-40);
-        const double sqr_min_r = m * m;
-        double mul;
-        if (r < sqr_min_r)
-            mul = scale / sqr_min_r;
-        else if (1.0 < r)
-            mul = scale;
-        else
-            mul = scale / r;
-        if (P.slot_ade[n]) v[3] *= mul;
-        v[2] = z * mul + j[2];
-        v[1] = y * mul + j[1];
-        v[0] = x * mul + j[0];
-        s.box++;
+        return;
     }
+    switch (kind) {
+        case MB3D_HIP_FORMULA_INTPOW:
+            int_pow64(P.slot_iarg[n], o[0], v, s.j);
+            s.p8++;
+            return;
+        case MB3D_HIP_FORMULA_AMAZINGBOX:
+            box64(o[0], o[1], o[2], v, s.j, P.is_custom_de != 0);
+            s.box++;
+            return;
+        case MB3D_HIP_FORMULA_REALPOWER: real_power64(o[0], o[1], v, s.j, s.rout); break;
+        case MB3D_HIP_FORMULA_QUATERNION: {  // src/formulas.rs:580-586
+            const double x = v[0], y = v[1], z = v[2], w = v[3];
+            v[0] = x * x - y * y - z * z - w * w + s.j[0];
+            v[1] = 2.0 * (y * x + z * w) + s.j[1];
+            v[2] = 2.0 * (z * x + o[0] * y * w) + s.j[2];
+            v[3] = 2.0 * (w * x + y * z) + o[1] + s.j[3];
+            break;
+        }
+        case MB3D_HIP_FORMULA_TRICORN: {  // src/formulas.rs:587-592
+            const double x = v[0], y = v[1], z = v[2];
+            v[2] = z * x * o[0] + s.j[2] * o[1];
+            v[0] = x * x - (y * y + z * z) + s.j[0];
+            v[1] = 2.0 * (y * x) + s.j[1];
+            break;
+        }
+        case MB3D_HIP_FORMULA_BULBOX: bulbox64(o, v, s.j, s.rout); break;
+        case MB3D_HIP_FORMULA_FOLDINGINTPOW:  // src/formulas.rs:629-634
+            v[0] = bfold64(v[0], o[1]);
+            v[1] = bfold64(v[1], o[1]);
+            v[2] = bfold64(v[2], o[1]);
+            int_pow64(P.slot_iarg[n], o[0], v, s.j);
+            break;
+        default: aexion64(o, P.slot_iarg[n], v, s.j); break;
+    }
+    s.oth++;
 }
 
 // calc_smooth_iterations (src/iteration.rs:140-154)
@@ -327,12 +559,14 @@ __device__ __forceinline__ float smooth64(const DevP &P, const It64 &s, int n) {
     return (float)(((double)P.ln_rstop - d) / (e + 1e-100) + (double)s.it_result);
 }
 
-// run_with (src/iteration.rs:268-315), 3D, counted slots; returns the
-// index of the last slot used
+// run_with (src/iteration.rs:268-315), counted slots, 3D or 4D squared
+// radius; returns the index of the last slot used
+template <int KIND>
 __device__ __forceinline__ int run64(const DevP &P, It64 &s, int max_it, double rstop) {
     int n = 0;
     int btmp = P.slot_iters[0];
     s.it_result = 0;
+    const bool four = KIND == K_GEN && P.four_d;
     for (;;) {
         s.rold = s.rout;
         int guard = 0;
@@ -342,22 +576,42 @@ __device__ __forceinline__ int run64(const DevP &P, It64 &s, int max_it, double 
             btmp = P.slot_iters[n];
             if (++guard > 64) return n;
         }
-        call_slot64(P, n, s);
+        call_slot64<KIND>(P, n, s);
         btmp -= 1;
         s.it_result += 1;
-        s.rout = s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2];
+        if (four)
+            s.rout = s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2] + s.v[3] * s.v[3];
+        else
+            s.rout = s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2];
         if (s.rout < s.otrap) s.otrap = s.rout;
         if (s.it_result >= max_it || s.rout > rstop) return n;
     }
 }
 
-__device__ __forceinline__ void start64(It64 &s, double cx, double cy, double cz) {
-    s.j[0] = cx;
-    s.j[1] = cy;
-    s.j[2] = cz;
+// The start of an evaluation at c: hybrid_3d / hybrid_3d_de
+// (src/iteration.rs:521-552), the Julia seed as the constant in Julia
+// mode, J4 kept in j[3]; the 4D hybrids rotate c into 4D first (start_4d,
+// rotate_4dex: src/iteration.rs:417-428, src/math.rs:209-215).
+template <int KIND>
+__device__ __forceinline__ void start64(const DevP &P, It64 &s, double cx, double cy, double cz) {
+    if (KIND == K_GEN && P.four_d) {
+        for (int i = 0; i < 4; i++) {
+            s.v[i] = cx * (double)P.sm4[3 * i] + cy * (double)P.sm4[3 * i + 1] + cz * (double)P.sm4[3 * i + 2];
+            s.j[i] = P.do_julia ? P.ju[i] : s.v[i];
+        }
+        s.rout = s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2] + s.v[3] * s.v[3];
+        s.otrap = s.rout;
+        return;
+    }
+    const bool julia = KIND == K_GEN && P.do_julia;
+    s.j[0] = julia ? P.ju[0] : cx;
+    s.j[1] = julia ? P.ju[1] : cy;
+    s.j[2] = julia ? P.ju[2] : cz;
+    s.j[3] = P.ju[3];
     s.v[0] = cx;
     s.v[1] = cy;
     s.v[2] = cz;
+    s.v[3] = 0.0;
     s.rout = cx * cx + cy * cy + cz * cz;
     s.otrap = s.rout;
 }
@@ -367,25 +621,25 @@ struct De64 {
     double de, rout, otrap;
     int it_result;
     float smooth_it;
-    uint32_t p8, box;
+    uint32_t p8, box, oth;
 };
 
 // calc_de_part without dIFS (src/calc.rs:1495-1559): hybrid_3d_de
 // (src/iteration.rs:539-585) for the analytic options -- option 32 being the
-// textbook 0.5 r ln r / dr of the power-8 extension -- or hybrid_3d
-// (src/iteration.rs:521-537) four times for the numeric gradient.  One
-// out-of-line copy for all kernels; arguments and result in registers.
+// textbook 0.5 r ln r / dr of the power-8 extension -- or the alternating
+// hybrid (3D or 4D) four times for the numeric gradient.  Arguments and
+// result in registers.
+template <int KIND>
 __device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double cz, int max_it, int calc_sit,
                                   float ms) {
     De64 o;
     It64 s;
-    s.p8 = s.box = 0;
+    s.p8 = s.box = s.oth = 0;
     s.deriv1 = 1.0;
-    s.v[3] = 0.0;
     s.rold = 0.0;
     double result;
     o.smooth_it = 0.0f;
-    start64(s, cx, cy, cz);
+    start64<KIND>(P, s, cx, cy, cz);
     if (P.is_custom_de) {
         if ((P.de_option & 0x38) == 32)
             s.deriv1 = 1.0;
@@ -393,7 +647,7 @@ __device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double
             s.v[3] = s.rout;
         else
             s.v[3] = 1.0;
-        const int n = run64(P, s, max_it, P.d_rstop);
+        const int n = run64<KIND>(P, s, max_it, P.d_rstop);
         if ((P.de_option & 0x38) == 32)
             result = sqrt(s.rout) * 0.25 * log(s.rout) / s.deriv1;
         else
@@ -404,7 +658,7 @@ __device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double
         o.rout = s.rout;
         o.otrap = s.otrap;
     } else {
-        const int n = run64(P, s, max_it, P.d_rstop);
+        const int n = run64<KIND>(P, s, max_it, P.d_rstop);
         if (calc_sit) o.smooth_it = smooth64(P, s, n);
         o.it_result = s.it_result;
         o.rout = s.rout;
@@ -417,9 +671,8 @@ __device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double
             const double off = (double)P.de_offset;
             double g[3];
             for (int k = 0; k < 3; k++) {
-                start64(s, k == 0 ? cx + off : cx, k == 1 ? cy + off : cy, k == 2 ? cz + off : cz);
-                s.v[3] = 0.0;
-                run64(P, s, it0, P.rstop3d);
+                start64<KIND>(P, s, k == 0 ? cx + off : cx, k == 1 ? cy + off : cy, k == 2 ? cz + off : cz);
+                run64<KIND>(P, s, it0, P.rstop3d);
                 g[k] = (buf_rout - s.rout) * (buf_rout - s.rout);
             }
             result = buf_rout * log(buf_rout) * (double)P.d_de_scale /
@@ -432,6 +685,7 @@ __device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double
     o.de = result;
     o.p8 = s.p8;
     o.box = s.box;
+    o.oth = s.oth;
     return o;
 }
 
@@ -481,6 +735,7 @@ struct Ray64 {
         cnt.de64++;
         cnt.p8_64 += r.p8;
         cnt.box_64 += r.box;
+        cnt.oth_64 += r.oth;
     }
     // calc_zpos_and_rough, smooth_normals = 0 (src/calc.rs:2194-2205)
     __device__ __forceinline__ uint32_t zpos_fine(const DevP &P) const {

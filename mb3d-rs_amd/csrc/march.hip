@@ -52,7 +52,7 @@ enum : int {
     C_FINQ = 3,    // next item of the f64 finish
     C_POSTQ = 4,   // next item of the shadow pass
     C_AOQ = 5,     // next item of the DE AO pass
-    C_STATS = 8,   // 11 work counters from here
+    C_STATS = 8,   // 12 work counters from here
     C_WORDS = 24
 };
 
@@ -96,10 +96,10 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 __device__ void flush_counters(const Cnt &c, unsigned long long *ctr) {
-    const unsigned long long v[11] = {c.steps, c.de32,  c.de64,     c.p8_32, c.box_32,   c.p8_64,
-                                      c.box_64, c.hits, c.overflow, c.jumps, c.rollbacks};
+    const unsigned long long v[12] = {c.steps,  c.de32, c.de64,     c.p8_32, c.box_32,    c.p8_64,
+                                      c.box_64, c.hits, c.overflow, c.jumps, c.rollbacks, c.oth_64};
 #pragma unroll
-    for (int k = 0; k < 11; k++) {
+    for (int k = 0; k < 12; k++) {
         const unsigned long long s = wave_sum(v[k]);
         if (lane_id() == 0 && s) atomicAdd(&ctr[C_STATS + k], s);
     }
@@ -237,7 +237,8 @@ __device__ __forceinline__ const DevP *stage_frames(const DevP *__restrict__ tab
     return reinterpret_cast<const DevP *>(frames_lds);
 }
 static_assert(sizeof(DevP) % sizeof(uint2) == 0, "DevP in 8-byte words");
-constexpr int MAX_BATCH_FRAMES = 64;  // 64 x 744 B of LDS
+constexpr int MAX_BATCH_FRAMES = 64;
+static_assert(MAX_BATCH_FRAMES * sizeof(DevP) <= 65536, "a batch's frame table fits one block's LDS");
 
 // the frame of a batch that work item q belongs to
 __device__ __forceinline__ int find_frame(const DevP *__restrict__ tab, int n, long long q) {
@@ -687,7 +688,7 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
             R.de = (double)de32_analytic<DEM>(P, R.c, R.ms, it32, cnt, &ot);
             R.otrap = (double)ot;
         } else {
-            R.take(de64(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+            R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
         }
         switch (phase) {
             case FP_START:
@@ -840,18 +841,20 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
 // FP32 overflow), so that no DE site is duplicated per state.
 template <int KIND, int DEM, bool F32>
 __device__ __forceinline__ void post_de(const DevP &P, Ray64 &R, bool want64, Cnt &cnt) {
-    if (F32 && !want64) {
-        int it = 0;
-        const float d32 = de32<KIND, DEM>(P, R.c, R.ms, it, cnt);
-        if (d32 < __builtin_inff() && it < R.max_it && d32 > R.ms * P.guard) {
-            R.de = (double)d32;
-            R.it = it;
-            R.max_its = R.max_it;
-            return;
+    if constexpr (F32) {
+        if (!want64) {
+            int it = 0;
+            const float d32 = de32<KIND, DEM>(P, R.c, R.ms, it, cnt);
+            if (d32 < __builtin_inff() && it < R.max_it && d32 > R.ms * P.guard) {
+                R.de = (double)d32;
+                R.it = it;
+                R.max_its = R.max_it;
+                return;
+            }
+            if (!(d32 < __builtin_inff())) cnt.overflow++;
         }
-        if (!(d32 < __builtin_inff())) cnt.overflow++;
     }
-    R.take(de64(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+    R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
 }
 
 // hs_surface_start / surface_point: the start of the f64 surface point of a
@@ -1200,10 +1203,11 @@ __global__ __launch_bounds__(256) void k_deao(DevP P, Bufs B, AoRays A) {
 
 // de_at (src/calc.rs:2695-2703) at a list of points: the f64 DE in world
 // units and the iteration count
+template <int KIND>
 __global__ void k_de_at(DevP P, const double *pts, long long n, double *de, int32_t *its) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const De64 r = de64(P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], P.max_it, 0, P.de_stop);
+        const De64 r = de64<KIND>(P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], P.max_it, 0, P.de_stop);
         if (de) de[i] = r.de * P.step_width;
         if (its) its[i] = r.it_result;
     }
@@ -1290,19 +1294,24 @@ int lib_cus() { return g_ctx.cus; }
         }                                                                                         \
     } while (0)
 
+int kind_of(const mb3d_hip_params &p);
+
 std::string unsupported_reason(const mb3d_hip_params *p) {
     if (!p) return "no parameters";
     if (p->abi_version != MB3D_HIP_ABI_VERSION) return "parameter ABI version mismatch";
     if (p->n_slots < 1 || p->n_slots > MB3D_HIP_MAX_SLOTS) return "bad hybrid slot count";
     for (int i = 0; i < p->n_slots; i++) {
-        if (p->slot_kind[i] == MB3D_HIP_FORMULA_INTPOW && p->slot_power[i] != 8 && p->slot_iters[i] != 0)
-            return "Integer Power other than power 8 (not on this backend)";
-        if (p->slot_kind[i] != MB3D_HIP_FORMULA_INTPOW && p->slot_kind[i] != MB3D_HIP_FORMULA_AMAZINGBOX)
+        if (p->slot_kind[i] < MB3D_HIP_FORMULA_INTPOW || p->slot_kind[i] > MB3D_HIP_FORMULA_AEXIONC)
             return "formula not on this backend";
         if (p->slot_uncounted[i] && p->slot_iters[i] != 0) return "uncounted hybrid slots (not on this backend)";
+        if ((p->slot_ade[i] != 0) != (p->is_custom_de != 0))
+            return "per-slot analytic DE flags that differ (a DE combination; not on this backend)";
     }
-    if (!(p->de_option == 0 || p->de_option == 11 || p->de_option == 32))
+    if (!(p->de_option == 0 || p->de_option == 4 || p->de_option == 11 || p->de_option == 32))
         return "this DE option (not on this backend)";
+    if ((p->four_d != 0) != (p->de_option >= 4 && p->de_option <= 6)) return "the 4D mode does not fit the DE option";
+    if (p->de_option == 32 && kind_of(*p) == K_GEN)
+        return "the analytic power-8 DE with another power or in Julia mode (not on this backend)";
     if (p->sm_normals != 0) return "smooth normals (not on this backend yet)";
     if (p->color_option != 0) return "colour option other than 0 (not on this backend yet)";
     if (!p->normals_on_de) return "normals on the smooth iteration (not on this backend yet)";
@@ -1312,6 +1321,7 @@ std::string unsupported_reason(const mb3d_hip_params *p) {
 // needs_double (src/gpu.rs:569-576, 1019-1040): the FP32 iterate cannot
 // resolve the numeric gradient offset or the pixel step
 bool needs_f64(const mb3d_hip_params &p) {
+    if (kind_of(p) == K_GEN) return true;  // no FP32 iterate for these formulas
     const double coord = std::max({std::fabs(p.ystart[0]), std::fabs(p.ystart[1]), std::fabs(p.ystart[2])}) +
                          p.step_width * (p.width + p.height) + 2.0;
     const double eps = 1.1920928955078125e-07;
@@ -1343,14 +1353,44 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
     d.dfog_on_it = p.dfog_on_it;
     d.first_step_random = p.first_step_random;
     d.step_cap = p.step_cap;
+    d.four_d = p.four_d;
+    d.do_julia = p.do_julia;
+    for (int k = 0; k < 4; k++) d.ju[k] = p.julia_c[k];
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 3; k++) d.sm4[3 * i + k] = (float)p.smatrix4[i][k];
     for (int i = 0; i < MB3D_HIP_MAX_SLOTS; i++) {
         d.slot_kind[i] = p.slot_kind[i];
         d.slot_iters[i] = std::max(p.slot_iters[i], 0);
-        d.slot_ade[i] = p.slot_ade[i];
-        d.slot_z_mul[i] = p.slot_z_mul[i];
-        d.slot_scale[i] = p.slot_scale[i];
-        d.slot_min_r[i] = p.slot_min_r[i];
-        d.slot_fold[i] = p.slot_fold[i];
+        // the f64 options (DevP::opt's order) and the integer argument
+        double *o = d.opt[i];
+        const double *so = p.slot_opt[i];
+        switch (p.slot_kind[i]) {
+            case MB3D_HIP_FORMULA_INTPOW:
+                o[0] = p.slot_z_mul[i];
+                d.slot_iarg[i] = p.slot_power[i];
+                break;
+            case MB3D_HIP_FORMULA_AMAZINGBOX:
+                o[0] = p.slot_scale[i];
+                o[1] = p.slot_min_r[i];
+                o[2] = p.slot_fold[i];
+                break;
+            case MB3D_HIP_FORMULA_FOLDINGINTPOW:
+                o[0] = so[1];
+                o[1] = so[2];
+                d.slot_iarg[i] = p.slot_power[i];
+                break;
+            case MB3D_HIP_FORMULA_AEXIONC:
+                o[0] = so[0];
+                o[1] = so[1];
+                o[2] = so[4];
+                o[3] = so[5];
+                d.slot_iarg[i] = (so[2] != 0.0 ? 1 : 0) | (so[3] != 0.0 ? 2 : 0) | (so[6] != 0.0 ? 4 : 0) |
+                                 ((int)so[7] << 8);
+                break;
+            default:  // Real Power, Quaternion, Tricorn, Bulbox: options() in order
+                for (int k = 0; k < 6; k++) o[k] = so[k];
+                break;
+        }
         d.fz_mul[i] = (float)p.slot_z_mul[i];
         d.fscale[i] = (float)p.slot_scale[i];
         const double m = std::fmax(p.slot_min_r[i], 1e-40);
@@ -1443,10 +1483,13 @@ int ensure(void **buf, size_t *have, size_t need) {
 
 int kind_of(const mb3d_hip_params &p) {
     bool p8 = false, box = false;
+    if (p.four_d || p.do_julia) return K_GEN;
     for (int i = 0; i < p.n_slots; i++) {
         if (p.slot_iters[i] == 0) continue;
-        if (p.slot_kind[i] == MB3D_HIP_FORMULA_INTPOW) p8 = true;
-        if (p.slot_kind[i] == MB3D_HIP_FORMULA_AMAZINGBOX) box = true;
+        const int k = p.slot_kind[i];
+        if (k == MB3D_HIP_FORMULA_INTPOW && p.slot_power[i] == 8) p8 = true;
+        else if (k == MB3D_HIP_FORMULA_AMAZINGBOX) box = true;
+        else return K_GEN;
     }
     if (p8 && box) return K_HYB;
     return box ? K_BOX : K_P8;
@@ -1469,7 +1512,8 @@ template <int KIND, int DEM, bool BATCH = false>
 void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b, const MlpDev &M, const DevP *tab = nullptr,
                   int n = 1) {
     const size_t lds = BATCH ? (size_t)n * sizeof(DevP) : 0;
-    if (M.enabled)
+    if constexpr (KIND == K_GEN) return;  // FP64 only: the hosts never ask for it
+    else if (M.enabled)
         hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, true>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
     else
         hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, false>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
@@ -1478,15 +1522,23 @@ template <int KIND, int DEM, bool N32, bool BATCH = false>
 void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all, const DevP *tab = nullptr,
                    int n = 1) {
     const size_t lds = BATCH ? (size_t)n * sizeof(DevP) : 0;
-    hipLaunchKernelGGL((k_finish64<KIND, DEM, N32, BATCH>), dim3(grid), dim3(256), lds, s, d, b, all, tab, n);
+    if constexpr (KIND == K_GEN && (N32 || BATCH)) return;  // FP64 only, never batched
+    else hipLaunchKernelGGL((k_finish64<KIND, DEM, N32, BATCH>), dim3(grid), dim3(256), lds, s, d, b, all, tab, n);
 }
 template <int KIND, int DEM, bool F32>
 void launch_shadow(int grid, hipStream_t s, const DevP &d, const Bufs &b, const ShadowJob &j) {
-    hipLaunchKernelGGL((k_shadow<KIND, DEM, F32>), dim3(grid), dim3(256), 0, s, d, b, j);
+    if constexpr (KIND == K_GEN && F32) return;
+    else hipLaunchKernelGGL((k_shadow<KIND, DEM, F32>), dim3(grid), dim3(256), 0, s, d, b, j);
 }
 template <int KIND, int DEM, bool F32>
 void launch_deao(int grid, hipStream_t s, const DevP &d, const Bufs &b, const AoRays &r) {
-    hipLaunchKernelGGL((k_deao<KIND, DEM, F32>), dim3(grid), dim3(256), 0, s, d, b, r);
+    if constexpr (KIND == K_GEN && F32) return;
+    else hipLaunchKernelGGL((k_deao<KIND, DEM, F32>), dim3(grid), dim3(256), 0, s, d, b, r);
+}
+
+template <int KIND>
+void launch_de_at(int grid, hipStream_t s, const DevP &d, const double *pts, long long n, double *de, int32_t *its) {
+    hipLaunchKernelGGL((k_de_at<KIND>), dim3(grid), dim3(256), 0, s, d, pts, n, de, its);
 }
 
 // dispatch over the (formula kind, DE mode) pairs that exist
@@ -1496,6 +1548,8 @@ void dispatch(int kind, int dem, F &&f) {
     else if (kind == K_P8 && dem == DEM_P8) f(std::integral_constant<int, K_P8>(), std::integral_constant<int, DEM_P8>());
     else if (kind == K_BOX && dem == DEM_BOX) f(std::integral_constant<int, K_BOX>(), std::integral_constant<int, DEM_BOX>());
     else if (kind == K_BOX && dem == DEM_NUMERIC) f(std::integral_constant<int, K_BOX>(), std::integral_constant<int, DEM_NUMERIC>());
+    else if (kind == K_GEN && dem == DEM_BOX) f(std::integral_constant<int, K_GEN>(), std::integral_constant<int, DEM_BOX>());
+    else if (kind == K_GEN) f(std::integral_constant<int, K_GEN>(), std::integral_constant<int, DEM_NUMERIC>());
     else f(std::integral_constant<int, K_HYB>(), std::integral_constant<int, DEM_NUMERIC>());
 }
 
@@ -1550,7 +1604,9 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     if (o.tile_w < 0 || o.tile_h < 0 || o.tile_w * o.tile_h > 4096)
         return set_error(MB3D_HIP_ERR_INVALID, "bad tile shape");
     if (cancel && cancel(user)) return set_error(MB3D_HIP_CANCELLED, "cancelled");
-    const bool f64 = o.precision == MB3D_HIP_PREC_F64 || (o.precision == MB3D_HIP_PREC_AUTO && needs_f64(p));
+    // (the K_GEN formulas have no FP32 iterate: FP64 whatever was asked)
+    const bool f64 = o.precision == MB3D_HIP_PREC_F64 || (o.precision == MB3D_HIP_PREC_AUTO && needs_f64(p)) ||
+                     kind_of(p) == K_GEN;
     DevP d;
     fill_devp(p, o, row0, rows, d);
     const size_t npix = (size_t)d.npix;
@@ -1695,6 +1751,7 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     t.overflow = ctr[C_STATS + 8];
     t.jumps = ctr[C_STATS + 9];
     t.rollbacks = ctr[C_STATS + 10];
+    t.its_other_f64 = ctr[C_STATS + 11];
     t.kernel_ms = ms_march;
     t.post_ms = ms_post;
     t.march_f32_ms = split ? ms_a : 0.0;
@@ -1777,7 +1834,8 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
         if (!why.empty()) return set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
         if (kind_of(frames[f]) != kind || dem_of(frames[f]) != dem)
             return set_error(MB3D_HIP_ERR_INVALID, "the frames of a batch share one formula and DE mode");
-        if (o.precision == MB3D_HIP_PREC_F64 || (o.precision == MB3D_HIP_PREC_AUTO && needs_f64(frames[f])))
+        if (o.precision == MB3D_HIP_PREC_F64 || (o.precision == MB3D_HIP_PREC_AUTO && needs_f64(frames[f])) ||
+            kind_of(frames[f]) == K_GEN)
             return set_error(MB3D_HIP_UNSUPPORTED, "a batch runs the FP32 march (a frame needs FP64)");
     }
     Plan *pl = new Plan();
@@ -1880,6 +1938,7 @@ int plan_run(Plan &pl, int repeats) {
     t.overflow = ctr[C_STATS + 8];
     t.jumps = ctr[C_STATS + 9];
     t.rollbacks = ctr[C_STATS + 10];
+    t.its_other_f64 = ctr[C_STATS + 11];
     t.kernel_ms = ms_all > 0.0f ? ms_all / repeats : -1.0;
     t.march_f32_ms = ms_a;
     t.finish_f64_ms = ms_a > 0.0f ? t.kernel_ms - ms_a : -1.0;
@@ -2010,7 +2069,10 @@ extern "C" int32_t mb3d_hip_de_at(const mb3d_hip_params *params, const double *p
         int32_t *di = (int32_t *)(w + (size_t)n * 32);
         HIP_TRY(hipMemcpyAsync(dp, points, (size_t)n * 24, hipMemcpyHostToDevice, c.stream));
         const int grid = (int)std::min<long long>((n + 255) / 256, (long long)c.cus * 8);
-        hipLaunchKernelGGL(k_de_at, dim3(grid), dim3(256), 0, c.stream, d, dp, (long long)n, dd, di);
+        hipStream_t stream = c.stream;
+        dispatch(kind_of(*params), dem_of(*params), [&](auto K, auto) {
+            launch_de_at<decltype(K)::value>(grid, stream, d, dp, (long long)n, dd, di);
+        });
         HIP_TRY(hipGetLastError());
         if (de) HIP_TRY(hipMemcpyAsync(de, dd, (size_t)n * 8, hipMemcpyDeviceToHost, c.stream));
         if (its) HIP_TRY(hipMemcpyAsync(its, di, (size_t)n * 4, hipMemcpyDeviceToHost, c.stream));

@@ -3,11 +3,14 @@
 // formulas), and the backend's capability check.
 //
 //   mb3d_hip_scene_default      Scene::default          src/scene.rs:626-700
+//   mb3d_hip_formula_default    Formula::default_for    src/formulas.rs:266-306
 //   mb3d_hip_params_from_scene  CalcParams::new         src/calc.rs:375-898
 //   mb3d_hip_supported          gpu::unsupported        src/gpu.rs:445-477
 //   mb3d_hip_pixel_seed         pixel_seed              src/calc.rs:1125-1140
 #include <algorithm>
+#include <cctype>
 #include <cmath>
+#include <initializer_list>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -32,19 +35,121 @@ struct FormulaInfo {
 };
 
 FormulaInfo info_of(const mb3d_hip_formula &f) {
-    if (f.kind == MB3D_HIP_FORMULA_AMAZINGBOX) return {11, 0.2, 1024.0, 2.0, f.scale, true};
-    double sp = (double)std::min(std::max(f.power, 2), 8);
-    return {0, 1.0, 16.0, sp, f.z_mul, false};
+    const double *o = f.options;
+    // a real power of 0 counts as 1e-40 for the smooth iterations
+    const double real_sip = o[0] == 0.0 ? 1e-40 : o[0];
+    switch (f.kind) {
+        case MB3D_HIP_FORMULA_AMAZINGBOX: return {11, 0.2, 1024.0, 2.0, f.scale, true};
+        case MB3D_HIP_FORMULA_REALPOWER: return {0, 1.0, 16.0, real_sip, o[0], true};
+        case MB3D_HIP_FORMULA_QUATERNION: return {4, 1.0, 16.0, 2.0, o[0], true};
+        case MB3D_HIP_FORMULA_TRICORN: return {0, 1.0, 16.0, 2.0, o[0], true};
+        case MB3D_HIP_FORMULA_BULBOX: return {0, 0.2, 1024.0, 2.0, o[0], true};
+        case MB3D_HIP_FORMULA_FOLDINGINTPOW: return {0, 0.5, 1024.0, 2.0, o[1], false};
+        case MB3D_HIP_FORMULA_AEXIONC: return {0, 1.0, 16.0, real_sip, o[0], true};
+        default: return {0, 1.0, 16.0, (double)std::min(std::max(f.power, 2), 8), f.z_mul, false};
+    }
 }
 
-// combine_de_options (src/calc.rs:318-372) for DE options 0 and 11
+// combine_de_options (src/calc.rs:318-372): the DE option of the hybrid so
+// far (`acc`, -1 at first) and that of the next formula
 int combine_de(int acc, int de) {
-    if (acc == -1 || acc == 21 || acc == 22) return de;
-    if (acc == 11) {
-        if (de == 2 || de == 5) return 2;
-        return de != 6 ? 0 : 11;
+    auto in = [de](std::initializer_list<int> l) { return std::find(l.begin(), l.end(), de) != l.end(); };
+    switch (acc) {
+        case -1:
+        case 21:
+        case 22: return de;
+        case 2: return in({2, 5, 6, 11}) ? 2 : 0;
+        case 4: return in({5, 6}) ? 4 : 0;
+        case 5: return de == 4 ? 4 : in({2, 11}) ? 2 : (de != 6 ? 0 : 5);
+        case 6: return in({2, 4, 5, 11}) ? de : 0;
+        case 11: return in({2, 5}) ? 2 : (de != 6 ? 0 : 11);
+        default: return acc;
     }
-    return acc;
+}
+
+// An integer option as Formula::set_option stores it: rounded half away
+// from zero, clamped (NaN casts to 0 and is clamped likewise),
+// src/formulas.rs:383-404
+int int_option(double v, int lo, int hi) {
+    const double r = std::round(v);
+    if (!(r >= lo)) return lo;
+    return r > hi ? hi : (int)r;
+}
+
+// The options of the formula kinds that keep them in `options`, as the
+// formula holds them: Folding Int Pow's power an integer, Aexion C's
+// switches 0/1 and its mode 0..31.  Returns the integer power (or 0).
+int slot_options(const mb3d_hip_formula &f, double out[MB3D_HIP_FORMULA_OPTIONS]) {
+    std::copy(f.options, f.options + MB3D_HIP_FORMULA_OPTIONS, out);
+    if (f.kind == MB3D_HIP_FORMULA_FOLDINGINTPOW) {
+        const int pw = int_option(f.options[0], 2, 8);
+        out[0] = pw;
+        return pw;
+    }
+    if (f.kind == MB3D_HIP_FORMULA_AEXIONC) {
+        for (int k : {2, 3, 6}) out[k] = std::round(f.options[k]) != 0.0 ? 1.0 : 0.0;
+        out[7] = int_option(f.options[7], 0, 31);
+    }
+    return 0;
+}
+
+// build_smatrix4 (src/math.rs:178-204): the XW, YW and ZW plane rotations
+// applied in that order (m3 * m2 * m1), each entry rounded to f32
+void smatrix4(const double a[3], double out[4][4]) {
+    double m[3][4][4];
+    for (int r = 0; r < 3; r++) {
+        for (int i = 0; i < 4; i++)
+            for (int k = 0; k < 4; k++) m[r][i][k] = i == k ? 1.0 : 0.0;
+        const double sn = std::sin(a[r]), cs = std::cos(a[r]);
+        m[r][r][r] = cs;
+        m[r][3][3] = cs;
+        m[r][r][3] = -sn;
+        m[r][3][r] = sn;
+    }
+    double acc[4][4], t[4][4];
+    std::memcpy(acc, m[0], sizeof acc);
+    for (int r = 1; r < 3; r++) {
+        for (int i = 0; i < 4; i++)
+            for (int k = 0; k < 4; k++) {
+                double sum = 0.0;
+                for (int q = 0; q < 4; q++) sum += m[r][i][q] * acc[q][k];
+                t[i][k] = sum;
+            }
+        std::memcpy(acc, t, sizeof acc);
+    }
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 4; k++) out[i][k] = (double)(float)acc[i][k];
+}
+
+// Formula::default_for's names and aliases (src/formulas.rs:266-306),
+// compared lower-case with '_' and '-' read as spaces
+int kind_of_name(const char *name) {
+    std::string n;
+    for (const char *c = name; *c; c++) {
+        const char ch = (char)std::tolower((unsigned char)*c);
+        n += (ch == '_' || ch == '-') ? ' ' : ch;
+    }
+    const size_t b = n.find_first_not_of(" \t\r\n"), e = n.find_last_not_of(" \t\r\n");
+    n = b == std::string::npos ? std::string() : n.substr(b, e - b + 1);
+    static const struct {
+        const char *name;
+        int kind;
+    } table[] = {
+        {"integer power", MB3D_HIP_FORMULA_INTPOW}, {"intpow", MB3D_HIP_FORMULA_INTPOW},
+        {"mandelbulb", MB3D_HIP_FORMULA_INTPOW}, {"bulb", MB3D_HIP_FORMULA_INTPOW},
+        {"real power", MB3D_HIP_FORMULA_REALPOWER}, {"realpower", MB3D_HIP_FORMULA_REALPOWER},
+        {"floatpow", MB3D_HIP_FORMULA_REALPOWER}, {"quaternion", MB3D_HIP_FORMULA_QUATERNION},
+        {"quat", MB3D_HIP_FORMULA_QUATERNION}, {"tricorn", MB3D_HIP_FORMULA_TRICORN},
+        {"amazing box", MB3D_HIP_FORMULA_AMAZINGBOX}, {"amazingbox", MB3D_HIP_FORMULA_AMAZINGBOX},
+        {"mandelbox", MB3D_HIP_FORMULA_AMAZINGBOX}, {"box", MB3D_HIP_FORMULA_AMAZINGBOX},
+        {"bulbox", MB3D_HIP_FORMULA_BULBOX}, {"folding int pow", MB3D_HIP_FORMULA_FOLDINGINTPOW},
+        {"foldingintpow", MB3D_HIP_FORMULA_FOLDINGINTPOW}, {"foldintpow", MB3D_HIP_FORMULA_FOLDINGINTPOW},
+        {"aexion c", MB3D_HIP_FORMULA_AEXIONC}, {"aexionc", MB3D_HIP_FORMULA_AEXIONC},
+        {"aexion", MB3D_HIP_FORMULA_AEXIONC},
+    };
+    for (const auto &t : table)
+        if (n == t.name) return t.kind;
+    return MB3D_HIP_FORMULA_NONE;
 }
 
 }  // namespace
@@ -77,6 +182,59 @@ extern "C" void mb3d_hip_scene_default(mb3d_hip_scene *s) {
     s->formulas[0].z_mul = -1.0;
 }
 
+extern "C" int32_t mb3d_hip_formula_default(const char *name, mb3d_hip_formula *f) {
+    using mb3d::set_error;
+    if (!name || !f) return set_error(MB3D_HIP_ERR_INVALID, "NULL argument");
+    const int kind = kind_of_name(name);
+    if (kind == MB3D_HIP_FORMULA_NONE) {
+        const std::string why = std::string("not a built-in formula: ") + name;
+        return set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
+    }
+    std::memset(f, 0, sizeof *f);
+    f->kind = kind;
+    f->iterations = 1;
+    double *o = f->options;
+    switch (kind) {
+        case MB3D_HIP_FORMULA_INTPOW:
+            f->power = 8;
+            f->z_mul = -1.0;
+            break;
+        case MB3D_HIP_FORMULA_AMAZINGBOX:
+            f->scale = 2.0;
+            f->min_r = 0.5;
+            f->fold = 1.0;
+            break;
+        case MB3D_HIP_FORMULA_REALPOWER:
+            o[0] = 8.0;
+            o[1] = -1.0;
+            break;
+        case MB3D_HIP_FORMULA_QUATERNION:
+            o[0] = -1.0;
+            o[1] = 0.0;
+            break;
+        case MB3D_HIP_FORMULA_TRICORN:
+            o[0] = -2.0;
+            o[1] = 1.0;
+            break;
+        case MB3D_HIP_FORMULA_BULBOX: {
+            const double d[6] = {2.0, 0.5, 1.0, 1.0, 2.0, 2.0};
+            std::copy(d, d + 6, o);
+            break;
+        }
+        case MB3D_HIP_FORMULA_FOLDINGINTPOW:
+            o[0] = 2.0;
+            o[1] = -1.0;
+            o[2] = 2.0;
+            break;
+        case MB3D_HIP_FORMULA_AEXIONC: {
+            const double d[8] = {8.0, 1.0, 1.0, 1.0, 8.0, 1.0, 0.0, 0.0};
+            std::copy(d, d + 8, o);
+            break;
+        }
+    }
+    return MB3D_HIP_OK;
+}
+
 extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_params *out) {
     using mb3d::set_error;
     if (!sc || !out) return set_error(MB3D_HIP_ERR_INVALID, "NULL argument");
@@ -87,7 +245,7 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
     bool any_its = false;
     for (int i = 0; i < nf; i++) {
         const int k = sc->formulas[i].kind;
-        if (k != MB3D_HIP_FORMULA_INTPOW && k != MB3D_HIP_FORMULA_AMAZINGBOX)
+        if (k < MB3D_HIP_FORMULA_INTPOW || k > MB3D_HIP_FORMULA_AEXIONC)
             return set_error(MB3D_HIP_ERR_INVALID, "unknown formula kind");
         any_its |= sc->formulas[i].iterations != 0;
     }
@@ -116,6 +274,7 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
         p.slot_scale[i] = f.scale;
         p.slot_min_r[i] = f.min_r;
         p.slot_fold[i] = f.fold;
+        if (const int pw = slot_options(f, p.slot_opt[i])) p.slot_power[i] = pw;
     }
     p.end_to = last;
     int rf = std::min(sc->repeat_from, last);
@@ -137,6 +296,10 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
     p.de_option = de_option;
     p.is_custom_de = custom;
     for (int i = 0; i < p.n_slots; i++) p.slot_ade[i] = custom;
+    p.four_d = de_option >= 4 && de_option <= 6;  // HybridMode::Alt4D (src/calc.rs:475)
+    p.do_julia = sc->julia != 0;
+    std::copy(sc->julia_c, sc->julia_c + 4, p.julia_c);
+    smatrix4(sc->rot_4d, p.smatrix4);
 
     // DE scale, colour offset and smooth-iteration logs (src/calc.rs:534-600)
     const int max_it = sc->iterations;

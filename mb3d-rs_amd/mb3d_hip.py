@@ -20,12 +20,21 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmb3d_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_SLOTS = 6
 MAX_LIGHTS = 6
 
 OK, CANCELLED, UNSUPPORTED, OVERFLOW = 0, 1, 2, 3
 FORMULA_NONE, FORMULA_INTPOW, FORMULA_AMAZINGBOX = 0, 1, 2
+FORMULA_REALPOWER, FORMULA_QUATERNION, FORMULA_TRICORN = 3, 4, 5
+FORMULA_BULBOX, FORMULA_FOLDINGINTPOW, FORMULA_AEXIONC = 6, 7, 8
+FORMULA_OPTIONS = 8
+# Formula::all_names (src/formulas.rs:249-262) and the kinds they map to
+FORMULA_NAMES = {
+    "Integer Power": FORMULA_INTPOW, "Real Power": FORMULA_REALPOWER, "Quaternion": FORMULA_QUATERNION,
+    "Tricorn": FORMULA_TRICORN, "Amazing Box": FORMULA_AMAZINGBOX, "Bulbox": FORMULA_BULBOX,
+    "Folding Int Pow": FORMULA_FOLDINGINTPOW, "Aexion C": FORMULA_AEXIONC,
+}
 DE_REFERENCE, DE_ANALYTIC_P8 = 0, 1
 PREC_AUTO, PREC_F32, PREC_F64 = 0, 1, 2
 
@@ -40,6 +49,7 @@ class Formula(C.Structure):
         ("scale", C.c_double),
         ("min_r", C.c_double),
         ("fold", C.c_double),
+        ("options", C.c_double * FORMULA_OPTIONS),
     ]
 
 
@@ -62,7 +72,7 @@ class Scene(C.Structure):
         ("n_formulas", C.c_int32),
         ("de_mode", C.c_int32),
         ("step_cap", C.c_int32),
-        ("reserved", C.c_int32),
+        ("julia", C.c_int32),
         ("rstop", C.c_double),
         ("zoom", C.c_double),
         ("mid", C.c_double * 3),
@@ -74,6 +84,8 @@ class Scene(C.Structure):
         ("z_step_div", C.c_double),
         ("raystep_limiter", C.c_double),
         ("color_mul", C.c_double),
+        ("julia_c", C.c_double * 4),
+        ("rot_4d", C.c_double * 3),
         ("formulas", Formula * MAX_SLOTS),
     ]
 
@@ -106,6 +118,8 @@ class Params(C.Structure):
         ("color_option", C.c_int32),
         ("dfog_on_it", C.c_int32),
         ("step_cap", C.c_int32),
+        ("four_d", C.c_int32),
+        ("do_julia", C.c_int32),
         ("reserved", C.c_int32),
         ("slot_z_mul", S6d),
         ("slot_scale", S6d),
@@ -137,6 +151,9 @@ class Params(C.Structure):
         ("step_width", C.c_double),
         ("vgrads", (C.c_double * 3) * 3),
         ("ystart", C.c_double * 3),
+        ("slot_opt", (C.c_double * FORMULA_OPTIONS) * MAX_SLOTS),
+        ("smatrix4", (C.c_double * 4) * 4),
+        ("julia_c", C.c_double * 4),
     ]
 
 
@@ -193,6 +210,7 @@ class Counters(C.Structure):
         ("its_box_f32", C.c_uint64),
         ("its_p8_f64", C.c_uint64),
         ("its_box_f64", C.c_uint64),
+        ("its_other_f64", C.c_uint64),
         ("overflow", C.c_uint64),
         ("jumps", C.c_uint64),
         ("rollbacks", C.c_uint64),
@@ -213,6 +231,7 @@ SIGNATURES = {
     "mb3d_hip_init": (C.c_int32, [C.c_int32]),
     "mb3d_hip_shutdown": (None, []),
     "mb3d_hip_scene_default": (None, [C.POINTER(Scene)]),
+    "mb3d_hip_formula_default": (C.c_int32, [C.c_char_p, C.POINTER(Formula)]),
     "mb3d_hip_params_from_scene": (C.c_int32, [C.POINTER(Scene), C.POINTER(Params)]),
     "mb3d_hip_supported": (C.c_int32, [C.POINTER(Params), C.c_char_p, C.c_size_t]),
     "mb3d_hip_pixel_seed": (C.c_int32, [C.c_int64, C.c_int64]),
@@ -351,6 +370,62 @@ def amazing_box(scale=2.0, min_r=0.5, fold=1.0, iterations=1) -> Formula:
     return f
 
 
+def formula(name: str, iterations=1, **options) -> Formula:
+    """A built-in formula with its defaults (Formula::default_for,
+    src/formulas.rs:266-306, through mb3d_hip_formula_default); keyword
+    options override them by position name: for the kinds other than
+    Integer Power and Amazing Box, ``options`` index names from OPTION_NAMES."""
+    f = Formula()
+    check(load().mb3d_hip_formula_default(name.encode(), C.byref(f)), f"formula {name!r}")
+    f.iterations = iterations
+    names = OPTION_NAMES.get(f.kind, ())
+    for k, v in options.items():
+        if k in names:
+            f.options[names.index(k)] = v
+        elif not names and k in ("power", "z_mul", "scale", "min_r", "fold"):
+            setattr(f, k, v)
+        else:
+            raise KeyError(f"{name} has no option {k!r}")
+    return f
+
+
+# the option names of Formula::options (src/formulas.rs:308-357) for the
+# kinds that keep their values in Formula.options
+OPTION_NAMES = {
+    FORMULA_REALPOWER: ("power", "z_mul"),
+    FORMULA_QUATERNION: ("yw_mul", "w_add"),
+    FORMULA_TRICORN: ("z_mul", "cz_mul"),
+    FORMULA_BULBOX: ("scale", "min_r", "fold", "bulb_scale", "r_threshold", "r_threshold2"),
+    FORMULA_FOLDINGINTPOW: ("power", "z_mul", "fold"),
+    FORMULA_AEXIONC: ("power", "z_mul", "rot_c", "cond_phi", "power_c", "cz_mul", "powc_dist", "mode"),
+}
+
+
+def scene_preset(name: str, width=96, height=72) -> Scene:
+    """Scene::preset (src/scene.rs:811-838): one built-in formula with its
+    camera framing (z_start, zoom, iterations, pitch and yaw); the
+    Quaternion preset is a Julia set at (-0.2, 0.6, 0.2, 0)."""
+    s = scene_default()
+    s.width, s.height = width, height
+    f = formula(name)
+    frames = {
+        FORMULA_AMAZINGBOX: (-14.0, 0.18, 20, (0.35, -0.6)),
+        FORMULA_BULBOX: (-14.0, 0.18, 20, (0.35, -0.6)),
+        FORMULA_QUATERNION: (-3.0, 0.8, 12, (0.4, 0.3)),
+        FORMULA_FOLDINGINTPOW: (-20.0, 0.15, 20, (0.5, 0.4)),
+        FORMULA_TRICORN: (-3.0, 0.6, 12, (0.6, 0.2)),
+        FORMULA_AEXIONC: (-3.0, 0.65, 10, (0.45, 0.3)),
+    }
+    zs, zoom, its, rot = frames.get(f.kind, (-2.5, 0.8, 12, (-0.55, 0.45)))
+    if f.kind == FORMULA_QUATERNION:
+        s.julia = 1
+        s.julia_c[:] = [-0.2, 0.6, 0.2, 0.0]
+    s.z_start, s.zoom, s.iterations = zs, zoom, its
+    set_camera(s, vgrads_from_angles(rot[0], rot[1], 0.0))
+    set_formulas(s, [f])
+    return s
+
+
 def set_formulas(s: Scene, fs: list) -> None:
     for i in range(MAX_SLOTS):
         s.formulas[i] = fs[i] if i < len(fs) else Formula()
@@ -480,6 +555,46 @@ def post_job(lights=(), deao_quality=None, shadow_max_len_mul=1.0, params: Param
         j.deao_quality = deao_quality
         j.deao_max_len = 1.0
     return j
+
+
+def struct_to_dict(st) -> dict:
+    """The fields of a ctypes struct of this ABI as plain Python values
+    (nested arrays as lists), for storing outside the process."""
+    def val(v):
+        if isinstance(v, C.Array):
+            return [val(x) for x in v]
+        if isinstance(v, C.Structure):
+            return struct_to_dict(v)
+        return v
+    return {name: val(getattr(st, name)) for name, _ in st._fields_}
+
+
+def struct_from_dict(cls, d: dict):
+    """A ctypes struct of type `cls` from struct_to_dict's output; fields the
+    dict lacks stay zero, names it has that `cls` lacks are an error."""
+    st = cls()
+    names = {n for n, _ in cls._fields_}
+    for k, v in d.items():
+        if k not in names:
+            raise KeyError(f"{cls.__name__} has no field {k!r}")
+        cur = getattr(st, k)
+        if isinstance(cur, C.Array):
+            _fill_array(cur, v)
+        elif isinstance(cur, C.Structure):
+            setattr(st, k, struct_from_dict(type(cur), v))
+        else:
+            setattr(st, k, v)
+    return st
+
+
+def _fill_array(arr, vals):
+    for i, v in enumerate(vals):
+        if isinstance(arr[i], C.Array):
+            _fill_array(arr[i], v)
+        elif isinstance(arr[i], C.Structure):
+            arr[i] = struct_from_dict(type(arr[i]), v)
+        else:
+            arr[i] = v
 
 
 def params_from_scene(s: Scene) -> Params:

@@ -1,12 +1,13 @@
 // mb3d_oracle.cpp -- TEST INFRASTRUCTURE (the f64 CPU oracle), not product code.
 //
 // A restatement of mb3d-rs's CPU ray marcher for the built-in formulas
-// 'Integer Power' and 'Amazing Box' and their alternating hybrids, in the
-// reference's operation order and with its f32 roundings kept (so that the
-// results are those of the reference's own CPU path):
+// and their alternating 3D and 4D hybrids, in the reference's operation
+// order and with its f32 roundings kept (so that the results are those of
+// the reference's own CPU path):
 //   parameter set-up   CalcParams::new          src/calc.rs:375-898
-//   iteration          Iteration::run_with ...  src/iteration.rs:140-154,268-315,521-585
-//   formulas           int_pow, amazing_box     src/formulas.rs:707-736,769-858
+//   iteration          Iteration::run_with ...  src/iteration.rs:140-154,268-315,521-585,757-772
+//   formulas           Formula::iterate         src/formulas.rs:576-858
+//   4D rotation        build_smatrix4, rotate_4dex  src/math.rs:178-215
 //   distance estimate  calc_de_part             src/calc.rs:1495-1559
 //   march              march_pixel_from, march_from  src/calc.rs:2231-2322,2533-2657
 //   bisection          bin_search, bin_search_it     src/calc.rs:1706-1792
@@ -109,6 +110,146 @@ inline void amazing_box(double scale, double min_r, double fold, double v[4], co
     v[0] = x * mul + j[0];
 }
 
+// int_pow2_scale (src/formulas.rs:739-752): HybridItIntPow2scale of Bulbox
+inline void int_pow2_scale(double scaling, double v[4], const double j[4]) {
+    double s = 1.0 / scaling;
+    double x = v[0] * s, y = v[1] * s, z = v[2] * s;
+    double xx = x * x, yy = y * y, r = xx + yy;
+    v[2] = -((std::sqrt(r) * z) * 2.0) * scaling + j[2];
+    double a = (r - z * z) / r;
+    v[0] = (xx - yy) * a * scaling + j[0];
+    v[1] = (x * y * a) * 2.0 * scaling + j[1];
+}
+
+// real_power (src/formulas.rs:755-766): HybridFloatPow, radius from the
+// stored squared radius of the previous counted iteration
+inline void real_power(double pw, double z_mul, double v[4], const double j[4], double rout) {
+    double x = v[0], y = v[1], z = v[2];
+    double theta = std::atan2(y, x) * pw;
+    double sin_t = std::sin(theta), cos_t = std::cos(theta);  // f64::sin_cos
+    double phi = std::atan2(z, std::sqrt(x * x + y * y)) * pw;
+    double sin_p = std::sin(phi), cos_p = std::cos(phi);
+    double r = std::pow(rout, pw * 0.5);
+    v[0] = cos_t * cos_p * r + j[0];
+    v[1] = sin_t * cos_p * r + j[1];
+    v[2] = sin_p * r * z_mul + j[2];
+}
+
+// Quaternion (src/formulas.rs:580-586)
+inline void quaternion(double yw_mul, double w_add, double v[4], const double j[4]) {
+    double x = v[0], y = v[1], z = v[2], w = v[3];
+    v[0] = x * x - y * y - z * z - w * w + j[0];
+    v[1] = 2.0 * (y * x + z * w) + j[1];
+    v[2] = 2.0 * (z * x + yw_mul * y * w) + j[2];
+    v[3] = 2.0 * (w * x + y * z) + w_add + j[3];
+}
+
+// Tricorn (src/formulas.rs:587-592)
+inline void tricorn(double z_mul, double cz_mul, double v[4], const double j[4]) {
+    double x = v[0], y = v[1], z = v[2];
+    v[2] = z * x * z_mul + j[2] * cz_mul;
+    v[0] = x * x - (y * y + z * z) + j[0];
+    v[1] = 2.0 * (y * x) + j[1];
+}
+
+// Bulbox, HybridSuperCube2 (src/formulas.rs:596-627): the power-2 bulb
+// inside R threshold 2, the box outside R threshold, blended between
+inline void bulbox(const double o[8], double v[4], const double j[4], double rout) {
+    double scale = o[0], min_r = o[1], fold = o[2], bulb_scale = o[3];
+    double rt1 = o[4] * o[4];
+    double rt2 = o[5] * o[5];
+    if (rout < rt1) {
+        if (rout < rt2) {
+            int_pow2_scale(bulb_scale, v, j);
+        } else {
+            double m = (rout - rt2) / (rt1 - rt2);
+            double in[3] = {v[0], v[1], v[2]};
+            int_pow2_scale(bulb_scale, v, j);
+            double out[3] = {v[0], v[1], v[2]};
+            v[0] = in[0];
+            v[1] = in[1];
+            v[2] = in[2];
+            amazing_box(scale, min_r, fold, v, j, false);
+            double r1 = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            double m2 = 1.0 - m;
+            double r2 = r1 * m + m2 * std::sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+            v[0] = v[0] * m + m2 * out[0];
+            v[1] = v[1] * m + m2 * out[1];
+            v[2] = v[2] * m + m2 * out[2];
+            r1 = r2 / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-40);
+            v[0] *= r1;
+            v[1] *= r1;
+            v[2] *= r1;
+        }
+    } else {
+        amazing_box(scale, min_r, fold, v, j, false);
+    }
+}
+
+// AexionC (src/formulas.rs:644-703): a real power bulb step with the angles
+// from the y axis, then (rotate C) the constant turned by a power of its
+// own angles.  o: power, z_mul, rotate C, conditional phi, power C, cz_mul,
+// power C on |Z - C|, mode.
+inline void aexion_c(const double o[8], double v[4], double j[4]) {
+    double power = o[0], z_mul = o[1], power_c = o[4], cz_mul = o[5];
+    bool rot_c = o[2] != 0.0, cond_phi = o[3] != 0.0, powc_dist = o[6] != 0.0;
+    int mode = (int)o[7];
+    double x = v[0], y = v[1], z = v[2];
+    double r1 = x * x + y * y + z * z;
+    double th = std::atan2(std::sqrt(x * x + z * z), y) * power;
+    double ph = std::atan2(z, x) * power;
+    r1 = std::exp(std::log(r1) * power * 0.5);
+    double st = std::sin(th), ct = std::cos(th);
+    double sp = std::sin(ph), cp = std::cos(ph);
+    v[0] = cp * ct * r1 + j[0];
+    v[2] = st * r1 * z_mul + j[2];
+    v[1] = ct * sp * r1 + j[1];
+    if (!rot_c) return;
+    double pd = power_c;
+    if (powc_dist) {
+        double d0 = v[0] - j[0], d1 = v[1] - j[1], d2 = v[2] - j[2];
+        pd *= std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    }
+    double r = std::sqrt(j[0] * j[0] + j[1] * j[1] + j[2] * j[2]);
+    double a, b, c;
+    if (mode & 16) {
+        a = v[1] - j[1];
+        b = v[2] - j[2];
+        c = v[0] - j[0];
+    } else {
+        a = j[1];
+        b = j[2];
+        c = j[0];
+    }
+    int ofs = 0, ecx = 8;  // the component "conditional phi" tests: 0 x, 8 y, 16 z
+    if (mode & 8) {
+        std::swap(a, b);
+        ecx = 16;
+    }
+    double s = std::sqrt(b * b + c * c);
+    double t1 = (mode & 1) ? std::atan2(a, s) : std::atan2(s, a);
+    double p1;
+    if (mode & 2) {
+        ofs = 24 - ecx;
+        p1 = std::atan2(c, b);
+    } else {
+        p1 = std::atan2(b, c);
+    }
+    if (mode & 4) {
+        std::swap(t1, p1);
+        ofs = ecx;
+    }
+    if (cond_phi) {
+        double t = ofs == 0 ? v[0] : (ofs == 8 ? v[1] : v[2]);
+        if (!std::signbit(t)) p1 = -p1;
+    }
+    double sa = std::sin(p1 * pd), ca = std::cos(p1 * pd);
+    double sb = std::sin(t1 * pd), cb = std::cos(t1 * pd);
+    j[0] = ca * cb * r;
+    j[2] = sb * r * cz_mul;
+    j[1] = cb * sa * r;
+}
+
 // int_pow (src/formulas.rs:769-858)
 void int_pow(int power, double zmul, double v[4], const double j[4]) {
     double x = v[0], y = v[1], z = v[2];
@@ -182,7 +323,9 @@ struct Iter {
     double deriv1 = 1.0;
     int start_from = 0, end_to = 0, repeat_from = 0;
     float fhln[6] = {0, 0, 0, 0, 0, 0};
-    uint64_t its_p8 = 0, its_box = 0;
+    bool four_d = false, do_julia = false;
+    double ju[4] = {0, 0, 0, 0};
+    uint64_t its_p8 = 0, its_box = 0, its_other = 0;
 };
 
 struct Marcher {
@@ -206,6 +349,10 @@ struct Marcher {
         it.end_to = p.end_to;
         it.repeat_from = p.repeat_from;
         for (int k = 0; k < 6; k++) it.fhln[k] = p.fhln[k];
+        it.four_d = p.four_d != 0;
+        it.do_julia = p.do_julia != 0;
+        for (int k = 0; k < 4; k++) it.ju[k] = p.julia_c[k];
+        for (int k = 0; k < 4; k++) it.j[k] = p.julia_c[k];  // j starts as ju (src/calc.rs:1058)
         ms_de_stop = p.de_stop;
         max_its_result = p.max_it;
     }
@@ -227,6 +374,21 @@ struct Marcher {
         } else if (kind == MB3D_HIP_FORMULA_AMAZINGBOX) {
             amazing_box(p.slot_scale[n], p.slot_min_r[n], p.slot_fold[n], it.v, it.j, p.slot_ade[n] != 0);
             it.its_box++;
+        } else {
+            const double *o = p.slot_opt[n];
+            switch (kind) {
+                case MB3D_HIP_FORMULA_REALPOWER: real_power(o[0], o[1], it.v, it.j, it.rout); break;
+                case MB3D_HIP_FORMULA_QUATERNION: quaternion(o[0], o[1], it.v, it.j); break;
+                case MB3D_HIP_FORMULA_TRICORN: tricorn(o[0], o[1], it.v, it.j); break;
+                case MB3D_HIP_FORMULA_BULBOX: bulbox(o, it.v, it.j, it.rout); break;
+                case MB3D_HIP_FORMULA_FOLDINGINTPOW:  // src/formulas.rs:629-634
+                    for (int k = 0; k < 3; k++) it.v[k] = box_fold(it.v[k], o[2]);
+                    int_pow(p.slot_power[n], o[1], it.v, it.j);
+                    break;
+                case MB3D_HIP_FORMULA_AEXIONC: aexion_c(o, it.v, it.j); break;
+                default: break;
+            }
+            it.its_other++;
         }
     }
 
@@ -268,20 +430,52 @@ struct Marcher {
                 continue;
             }
             it.it_result += 1;
-            double x = it.v[0], y = it.v[1], z = it.v[2];
-            it.rout = x * x + y * y + z * z;
+            double x = it.v[0], y = it.v[1], z = it.v[2], w = it.v[3];
+            it.rout = it.four_d ? x * x + y * y + z * z + w * w : x * x + y * y + z * z;
             if (it.rout < it.otrap) it.otrap = it.rout;
             // NaN compares false: keeps iterating
             if (it.it_result >= it.max_it || it.rout > it.rstop) return n;
         }
     }
 
-    // Iteration::hybrid_3d (src/iteration.rs:521-537)
-    void hybrid_3d() {
-        const double *c = it.c;
+    // the constant of a 3D hybrid: the point, or the Julia seed; j[3] (J4)
+    // is kept (src/iteration.rs:523-527)
+    void start_j3() {
+        const double *c = it.do_julia ? it.ju : it.c;
         it.j[0] = c[0];
         it.j[1] = c[1];
         it.j[2] = c[2];
+    }
+
+    // Iteration::hybrid_4d (src/iteration.rs:757-772) with start_4d
+    // (src/iteration.rs:417-428): the point rotated into 4D
+    void hybrid_4d() {
+        const double *c = it.c;
+        double v4[4];
+        for (int i = 0; i < 4; i++)  // rotate_4dex (src/math.rs:209-215)
+            v4[i] = c[0] * p.smatrix4[i][0] + c[1] * p.smatrix4[i][1] + c[2] * p.smatrix4[i][2];
+        for (int k = 0; k < 4; k++) {
+            it.v[k] = v4[k];
+            it.j[k] = it.do_julia ? it.ju[k] : v4[k];
+        }
+        it.rout = v4[0] * v4[0] + v4[1] * v4[1] + v4[2] * v4[2] + v4[3] * v4[3];
+        it.otrap = it.rout;
+        int n = run();
+        if (it.calc_sit) calc_smooth_iterations(n);
+    }
+
+    // mand_function (src/iteration.rs:859-866) for the alternating hybrids
+    void mand_function() {
+        if (it.four_d)
+            hybrid_4d();
+        else
+            hybrid_3d();
+    }
+
+    // Iteration::hybrid_3d (src/iteration.rs:521-537)
+    void hybrid_3d() {
+        const double *c = it.c;
+        start_j3();
         it.v[0] = c[0];
         it.v[1] = c[1];
         it.v[2] = c[2];
@@ -296,9 +490,7 @@ struct Marcher {
     // the analytic power-8 extension with the textbook 0.5 r ln r / dr.
     double hybrid_3d_de() {
         const double *c = it.c;
-        it.j[0] = c[0];
-        it.j[1] = c[1];
-        it.j[2] = c[2];
+        start_j3();
         it.v[0] = c[0];
         it.v[1] = c[1];
         it.v[2] = c[2];
@@ -330,7 +522,7 @@ struct Marcher {
         if (p.is_custom_de) {
             result = hybrid_3d_de() * (double)p.d_de_scale;
         } else {
-            hybrid_3d();
+            mand_function();
             if (it.rout < 1e-200) {
                 result = 0.0;
             } else {
@@ -345,7 +537,7 @@ struct Marcher {
                 for (int k = 0; k < 3; k++) {
                     double bufd = it.c[k];
                     it.c[k] = bufd + off;
-                    hybrid_3d();
+                    mand_function();
                     g[k] = (buf_rout - it.rout) * (buf_rout - it.rout);
                     it.c[k] = bufd;
                 }
@@ -1006,23 +1198,101 @@ void run_post(const mb3d_hip_params &p, const mb3d_hip_post_job &job, int row0, 
 namespace {
 
 int formula_de_option(const mb3d_hip_formula &f) {  // Formula::de_option (src/formulas.rs:492-499)
+    if (f.kind == MB3D_HIP_FORMULA_QUATERNION) return 4;
     return f.kind == MB3D_HIP_FORMULA_AMAZINGBOX ? 11 : 0;
 }
 double formula_de_scale(const mb3d_hip_formula &f) {  // de_scale (src/formulas.rs:502-509)
-    return f.kind == MB3D_HIP_FORMULA_AMAZINGBOX ? 0.2 : 1.0;
+    if (f.kind == MB3D_HIP_FORMULA_AMAZINGBOX || f.kind == MB3D_HIP_FORMULA_BULBOX) return 0.2;
+    return f.kind == MB3D_HIP_FORMULA_FOLDINGINTPOW ? 0.5 : 1.0;
 }
 double formula_default_rstop(const mb3d_hip_formula &f) {  // default_rstop (src/formulas.rs:520-528)
-    return f.kind == MB3D_HIP_FORMULA_AMAZINGBOX ? 1024.0 : 16.0;
+    const int k = f.kind;
+    return (k == MB3D_HIP_FORMULA_AMAZINGBOX || k == MB3D_HIP_FORMULA_BULBOX || k == MB3D_HIP_FORMULA_FOLDINGINTPOW)
+               ? 1024.0
+               : 16.0;
 }
 double formula_si_pow(const mb3d_hip_formula &f) {  // si_pow (src/formulas.rs:531-545)
     if (f.kind == MB3D_HIP_FORMULA_INTPOW) return (double)std::min(std::max(f.power, 2), 8);
+    if (f.kind == MB3D_HIP_FORMULA_REALPOWER || f.kind == MB3D_HIP_FORMULA_AEXIONC)
+        return f.options[0] == 0.0 ? 1e-40 : f.options[0];
     return 2.0;
 }
-double formula_first_option(const mb3d_hip_formula &f) {  // first_option (src/formulas.rs:548-560)
-    return f.kind == MB3D_HIP_FORMULA_INTPOW ? f.z_mul : f.scale;
+double formula_first_option(const mb3d_hip_formula &f) {  // first_option (src/formulas.rs:547-558)
+    switch (f.kind) {
+        case MB3D_HIP_FORMULA_INTPOW: return f.z_mul;
+        case MB3D_HIP_FORMULA_AMAZINGBOX: return f.scale;
+        case MB3D_HIP_FORMULA_FOLDINGINTPOW: return f.options[1];  // the Z multiplier
+        default: return f.options[0];
+    }
 }
-bool formula_first_option_double(const mb3d_hip_formula &f) {  // first_option_type
-    return f.kind != MB3D_HIP_FORMULA_INTPOW;
+bool formula_first_option_double(const mb3d_hip_formula &f) {  // first_option_type (src/formulas.rs:560-568)
+    return f.kind != MB3D_HIP_FORMULA_INTPOW && f.kind != MB3D_HIP_FORMULA_FOLDINGINTPOW;
+}
+
+// Rust's f64::round (half away from zero) as set_option rounds integer
+// options (src/formulas.rs:383-393)
+int round_clamp(double v, int lo, int hi) {
+    double r = std::round(v);
+    if (!(r >= (double)lo)) return lo;  // (NaN: Rust's cast gives 0, which the clamp raises to lo)
+    if (r > (double)hi) return hi;
+    return (int)r;
+}
+
+// The options of the kinds other than Integer Power and Amazing Box as the
+// formula holds them: Folding Int Pow's power an integer 2..8, Aexion C's
+// switches 0/1 and its mode an integer 0..31 (set_option,
+// src/formulas.rs:377-404).  `power` gets Folding Int Pow's power.
+void normalized_options(const mb3d_hip_formula &f, double o[8], int32_t &power) {
+    for (int k = 0; k < MB3D_HIP_FORMULA_OPTIONS; k++) o[k] = f.options[k];
+    if (f.kind == MB3D_HIP_FORMULA_FOLDINGINTPOW) {
+        power = round_clamp(f.options[0], 2, 8);
+        o[0] = (double)power;
+    } else if (f.kind == MB3D_HIP_FORMULA_AEXIONC) {
+        for (int k : {2, 3, 6}) o[k] = std::round(f.options[k]) != 0.0 ? 1.0 : 0.0;
+        o[7] = (double)round_clamp(f.options[7], 0, 31);
+    }
+}
+
+// build_smatrix4 (src/math.rs:178-204): XW, then YW, then ZW plane
+// rotations, kept in f32
+void build_smatrix4(double xw, double yw, double zw, double r[4][4]) {
+    auto ini = [](double m[4][4]) {
+        for (int i = 0; i < 4; i++)
+            for (int k = 0; k < 4; k++) m[i][k] = i == k ? 1.0 : 0.0;
+    };
+    auto mul = [](const double a[4][4], const double b[4][4], double o[4][4]) {
+        double t[4][4];
+        for (int i = 0; i < 4; i++)
+            for (int k = 0; k < 4; k++) {
+                double acc = 0.0;  // Iterator::sum adds in order (the sign of an all-zero sum aside)
+                for (int q = 0; q < 4; q++) acc += a[i][q] * b[q][k];
+                t[i][k] = acc;
+            }
+        std::memcpy(o, t, sizeof t);
+    };
+    double sx = std::sin(xw), cx = std::cos(xw);
+    double sy = std::sin(yw), cy = std::cos(yw);
+    double sz = std::sin(zw), cz = std::cos(zw);
+    double m[4][4], m2[4][4], m3[4][4];
+    ini(m);
+    m[0][0] = cx;
+    m[3][3] = cx;
+    m[0][3] = -sx;
+    m[3][0] = sx;
+    ini(m2);
+    m2[1][1] = cy;
+    m2[3][3] = cy;
+    m2[1][3] = -sy;
+    m2[3][1] = sy;
+    mul(m2, m, m2);
+    ini(m3);
+    m3[2][2] = cz;
+    m3[3][3] = cz;
+    m3[2][3] = -sz;
+    m3[3][2] = sz;
+    mul(m3, m2, r);
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 4; k++) r[i][k] = (double)(float)r[i][k];
 }
 
 }  // namespace
@@ -1034,7 +1304,7 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
     if (nf < 1 || nf > MB3D_HIP_MAX_SLOTS) return fail("1..6 formulas");
     for (int i = 0; i < nf; i++) {
         int k = sc->formulas[i].kind;
-        if (k != MB3D_HIP_FORMULA_INTPOW && k != MB3D_HIP_FORMULA_AMAZINGBOX) return fail("unknown formula kind");
+        if (k < MB3D_HIP_FORMULA_INTPOW || k > MB3D_HIP_FORMULA_AEXIONC) return fail("unknown formula kind");
     }
     bool any = false;
     for (int i = 0; i < nf; i++) any = any || sc->formulas[i].iterations != 0;
@@ -1060,6 +1330,7 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
         p.slot_scale[i] = f.scale;
         p.slot_min_r[i] = f.min_r;
         p.slot_fold[i] = f.fold;
+        normalized_options(f, p.slot_opt[i], p.slot_power[i]);
     }
     int end_to = last;
     int repeat_from = std::min(sc->repeat_from, end_to);
@@ -1072,9 +1343,17 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
         if (de < 0 || de == de_option) continue;
         if (de_option == -1 || de_option == 21 || de_option == 22)
             de_option = de;
+        else if (de_option == 2)
+            de_option = (de == 2 || de == 5 || de == 6 || de == 11) ? 2 : 0;
+        else if (de_option == 4)
+            de_option = (de == 5 || de == 6) ? 4 : 0;
+        else if (de_option == 5)
+            de_option = de == 4 ? 4 : ((de == 2 || de == 11) ? 2 : (de != 6 ? 0 : 5));
+        else if (de_option == 6)
+            de_option = (de == 2 || de == 4 || de == 5 || de == 11) ? de : 0;
         else if (de_option == 11)
             de_option = (de == 2 || de == 5) ? 2 : (de != 6 ? 0 : 11);
-        // (de_option 0 keeps 0)
+        // (other options keep theirs)
     }
     if (sc->de_mode == MB3D_HIP_DE_ANALYTIC_P8) {
         if (ns != 1 || sc->formulas[0].kind != MB3D_HIP_FORMULA_INTPOW) return fail("analytic p8 DE needs one Integer Power slot");
@@ -1086,6 +1365,10 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
                         de_option == 32;
     p.de_option = de_option;
     p.is_custom_de = is_custom_de;
+    p.four_d = de_option >= 4 && de_option <= 6;  // HybridMode::Alt4D (src/calc.rs:475)
+    p.do_julia = sc->julia != 0;
+    for (int k = 0; k < 4; k++) p.julia_c[k] = sc->julia_c[k];
+    build_smatrix4(sc->rot_4d[0], sc->rot_4d[1], sc->rot_4d[2], p.smatrix4);
     for (int i = 0; i < ns; i++) {
         p.slot_ade[i] = is_custom_de;
         int de = formula_de_option(sc->formulas[i]);
@@ -1280,11 +1563,23 @@ extern "C" int32_t orc_pixel_seed(int64_t y, int64_t x) {
 }
 
 extern "C" void orc_iterate(const mb3d_hip_formula *f, double v[4], double j[4], double rout, int32_t ade) {
-    (void)rout;
-    if (f->kind == MB3D_HIP_FORMULA_INTPOW)
-        int_pow(f->power, f->z_mul, v, j);
-    else if (f->kind == MB3D_HIP_FORMULA_AMAZINGBOX)
-        amazing_box(f->scale, f->min_r, f->fold, v, j, ade != 0);
+    double o[8];
+    int32_t power = f->power;
+    normalized_options(*f, o, power);
+    switch (f->kind) {
+        case MB3D_HIP_FORMULA_INTPOW: int_pow(f->power, f->z_mul, v, j); break;
+        case MB3D_HIP_FORMULA_AMAZINGBOX: amazing_box(f->scale, f->min_r, f->fold, v, j, ade != 0); break;
+        case MB3D_HIP_FORMULA_REALPOWER: real_power(o[0], o[1], v, j, rout); break;
+        case MB3D_HIP_FORMULA_QUATERNION: quaternion(o[0], o[1], v, j); break;
+        case MB3D_HIP_FORMULA_TRICORN: tricorn(o[0], o[1], v, j); break;
+        case MB3D_HIP_FORMULA_BULBOX: bulbox(o, v, j, rout); break;
+        case MB3D_HIP_FORMULA_FOLDINGINTPOW:
+            for (int k = 0; k < 3; k++) v[k] = box_fold(v[k], o[2]);
+            int_pow(power, o[1], v, j);
+            break;
+        case MB3D_HIP_FORMULA_AEXIONC: aexion_c(o, v, j); break;
+        default: break;
+    }
 }
 
 extern "C" double orc_de_at(const mb3d_hip_params *p, const double pos[3], int32_t *its) {
@@ -1334,7 +1629,7 @@ extern "C" int32_t orc_render(const mb3d_hip_params *pp, const mb3d_hip_post_job
     if (row0 < 0 || rows < 0 || row0 + rows > p.height) return fail("rows out of the picture");
     int w = p.width;
     std::vector<Px> px((size_t)rows * w);
-    std::atomic<uint64_t> c_steps{0}, c_de{0}, c_p8{0}, c_box{0}, c_hits{0};
+    std::atomic<uint64_t> c_steps{0}, c_de{0}, c_p8{0}, c_box{0}, c_oth{0}, c_hits{0};
     for_rows_interleaved(rows, nthreads(threads), [&](int r) {
         Marcher m(p);
         int y = row0 + r;
@@ -1351,6 +1646,7 @@ extern "C" int32_t orc_render(const mb3d_hip_params *pp, const mb3d_hip_post_job
         c_de += m.de_evals;
         c_p8 += m.it.its_p8;
         c_box += m.it.its_box;
+        c_oth += m.it.its_other;
     });
     if (post) run_post(p, *post, row0, rows, px, threads);
     for (size_t i = 0; i < px.size(); i++) {
@@ -1369,6 +1665,7 @@ extern "C" int32_t orc_render(const mb3d_hip_params *pp, const mb3d_hip_post_job
         cnt->de_evals = c_de;
         cnt->its_p8 = c_p8;
         cnt->its_box = c_box;
+        cnt->its_other = c_oth;
     }
     return 0;
 }

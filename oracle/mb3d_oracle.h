@@ -2,8 +2,8 @@
  *
  * The f64 CPU oracle: a restatement of mb3d-rs's CPU ray marcher
  * (src/calc.rs, src/iteration.rs, src/formulas.rs, src/math.rs,
- * src/deao.rs) for the built-in Integer Power and Amazing Box formulas and
- * their alternating hybrids.  Only the tests, the smoke check and the CPU
+ * src/deao.rs) for the built-in formulas and their alternating 3D and 4D
+ * hybrids, with Julia mode.  Only the tests, the smoke check and the CPU
  * baseline leg of the benchmark load it, always as the checker.
  *
  * It shares the plain data layouts of include/mb3d_hip.h (scene,
@@ -24,7 +24,7 @@ extern "C" {
 /* Work counters of an oracle run (same meaning as mb3d_hip_counters). */
 typedef struct orc_counters {
     uint64_t pixels, hits, steps, de_evals;
-    uint64_t its_p8, its_box;
+    uint64_t its_p8, its_box, its_other;  /* Integer Power, Amazing Box, the other formulas */
 } orc_counters;
 
 /* Oracle output per pixel of a rectangle; any pointer may be NULL. */
@@ -44,7 +44,9 @@ int32_t orc_params_from_scene(const mb3d_hip_scene *scene, mb3d_hip_params *para
 /* pixel_seed (src/calc.rs:1125-1140). */
 int32_t orc_pixel_seed(int64_t y, int64_t x);
 
-/* One formula iteration (Formula::iterate, src/formulas.rs:576-641). */
+/* One formula iteration (Formula::iterate, src/formulas.rs:576-641);
+ * `rout` is the stored squared radius (Real Power, Bulbox), `j` the
+ * constant (Aexion C turns it). */
 void orc_iterate(const mb3d_hip_formula *f, double v[4], double j[4], double rout, int32_t ade);
 
 /* de_at (src/calc.rs:2695-2703): the DE at a world point in world units,

@@ -21,7 +21,7 @@ import mb3d_hip as H  # noqa: E402  (struct layouts of include/mb3d_hip.h)
 
 
 class OrcCounters(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in ("pixels", "hits", "steps", "de_evals", "its_p8", "its_box")]
+    _fields_ = [(n, C.c_uint64) for n in ("pixels", "hits", "steps", "de_evals", "its_p8", "its_box", "its_other")]
 
 
 class OrcOut(C.Structure):
@@ -136,10 +136,12 @@ def march_ray(p, origin, direction, zz=0.0):
     return t, its.value, steps.value
 
 
-def iterate(f, v, j, ade=False):
+def iterate(f, v, j, ade=False, rout=0.0):
+    """One Formula::iterate step; `rout` is the stored squared radius that
+    Real Power and Bulbox read.  Returns the new (v, j)."""
     v = np.array(v, np.float64)
     j = np.array(j, np.float64)
-    load().orc_iterate(C.byref(f), _ptr(v), _ptr(j), 0.0, int(ade))
+    load().orc_iterate(C.byref(f), _ptr(v), _ptr(j), float(rout), int(ade))
     return v, j
 
 

@@ -4,7 +4,7 @@ workload rendered by the oracle (the f64 C++ restatement of the reference's
 CPU marcher), after its own tests pass (tests/test_oracle.py).
 
 Each fixture is one .npz: the march parameters and post job as the C ABI
-takes them (raw struct bytes), and the oracle's outputs -- depth (f64 march
+takes them (their fields by name, as JSON), and the oracle's outputs -- depth (f64 march
 units, +inf background), iterations, normals, the 5-word G-buffer records.
 tests/test_golden.py checks that the oracle still reproduces them bit for
 bit and that the HIP backend meets the parity bar against them.
@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 for sub in ("mb3d-rs_amd", "oracle"):
     sys.path.insert(0, os.path.join(ROOT, sub))
 
-import ctypes as C  # noqa: E402
+import json  # noqa: E402
 
 import numpy as np  # noqa: E402
 
@@ -46,8 +46,8 @@ def main():
         r = O.render(p, job, threads=0)
         np.savez_compressed(
             os.path.join(HERE, name + ".npz"),
-            params=np.frombuffer(bytes(p), np.uint8),
-            post=np.frombuffer(bytes(job), np.uint8) if job is not None else np.zeros(0, np.uint8),
+            params_json=np.array(json.dumps(H.struct_to_dict(p))),
+            post_json=np.array(json.dumps(H.struct_to_dict(job) if job is not None else None)),
             scale=np.float64(scale),
             depth=r["depth"], iters=r["iters"], normals=r["normals"], gbuf=r["gbuf"])
         hit = np.isfinite(r["depth"]) & (r["depth"] > 0)

@@ -3,6 +3,7 @@ tests/golden/make_golden.py from the oracle): the oracle must still
 reproduce them bit for bit (CPU), and the HIP backend must meet the parity
 bar against them (GPU) -- the same pictures, without running the oracle."""
 import glob
+import json
 import os
 
 import numpy as np
@@ -18,8 +19,9 @@ NAMES = [os.path.basename(f)[:-4] for f in FIXTURES]
 
 def load(name):
     z = np.load(os.path.join(HERE, "golden", name + ".npz"))
-    p = H.Params.from_buffer_copy(z["params"].tobytes())
-    job = H.PostJob.from_buffer_copy(z["post"].tobytes()) if z["post"].size else None
+    p = H.struct_from_dict(H.Params, json.loads(str(z["params_json"])))
+    post = json.loads(str(z["post_json"]))
+    job = H.struct_from_dict(H.PostJob, post) if post else None
     return p, job, float(z["scale"]), {k: z[k] for k in ("depth", "iters", "normals", "gbuf")}
 
 

@@ -5,6 +5,7 @@ calls with the documented status and a last-error text."""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -37,9 +38,29 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_struct_sizes():
     assert H.load().mb3d_hip_abi_version() == H.ABI_VERSION
-    # the sizes the header's layout gives on x86-64
-    assert C.sizeof(H.Formula) == 48
     assert C.sizeof(H.Params) % 8 == 0 and C.sizeof(H.Scene) % 8 == 0
+
+
+def test_ctypes_layouts_match_the_header(tmp_path):
+    """Every struct of the mirror against the C compiler's layout of
+    include/mb3d_hip.h: size and the offset of every field."""
+    structs = {"mb3d_hip_formula": H.Formula, "mb3d_hip_scene": H.Scene, "mb3d_hip_params": H.Params,
+               "mb3d_hip_post_job": H.PostJob, "mb3d_hip_out": H.Out, "mb3d_hip_options": H.Options,
+               "mb3d_hip_counters": H.Counters}
+    lines, want = [], []
+    for cname, cls in structs.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(C.sizeof(cls))
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
+            want.append(getattr(cls, fname).offset)
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "mb3d_hip.h"\nint main(void) {\n'
+                   + "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == want
 
 
 def test_scene_default_matches_python_mirror():
@@ -57,12 +78,70 @@ SCENES = {
 }
 
 
+def _rotated_quaternion():
+    s = H.scene_preset("Quaternion", 320, 240)
+    s.rot_4d[:] = [0.3, -0.7, 1.1]
+    return s
+
+
+def _mixed_hybrid():
+    s = H.scene_preset("Real Power", 320, 240)
+    H.set_formulas(s, [H.formula("Quaternion", 2), H.formula("Tricorn"), H.formula("Folding Int Pow", power=4.6),
+                       H.formula("Aexion C", mode=21.4, rot_c=0.6)])
+    s.repeat_from = 1
+    return s
+
+
+for _name in H.FORMULA_NAMES:
+    SCENES["preset-" + _name] = (lambda n: lambda: H.scene_preset(n, 320, 240))(_name)
+SCENES["quaternion-rot4d"] = _rotated_quaternion
+SCENES["mixed-hybrid"] = _mixed_hybrid
+
+
 @pytest.mark.parametrize("name", sorted(SCENES))
 def test_params_match_the_oracle_bit_for_bit(name):
     sc = SCENES[name]()
     a = H.params_from_scene(sc)
     b = O.params(sc)
     assert bytes(a) == bytes(b)
+
+
+def test_formula_defaults_follow_default_for():
+    """Formula::default_for (src/formulas.rs:266-306), aliases included."""
+    want = {
+        "Real Power": (H.FORMULA_REALPOWER, [8.0, -1.0]),
+        "quat": (H.FORMULA_QUATERNION, [-1.0, 0.0]),
+        "Tricorn": (H.FORMULA_TRICORN, [-2.0, 1.0]),
+        "bulbox": (H.FORMULA_BULBOX, [2.0, 0.5, 1.0, 1.0, 2.0, 2.0]),
+        "folding_int-pow": (H.FORMULA_FOLDINGINTPOW, [2.0, -1.0, 2.0]),
+        " Aexion ": (H.FORMULA_AEXIONC, [8.0, 1.0, 1.0, 1.0, 8.0, 1.0, 0.0, 0.0]),
+    }
+    for name, (kind, opts) in want.items():
+        f = H.formula(name)
+        assert f.kind == kind and f.iterations == 1
+        assert list(f.options)[: len(opts)] == opts and not any(list(f.options)[len(opts):])
+    f = H.formula("Mandelbulb")
+    assert (f.kind, f.power, f.z_mul) == (H.FORMULA_INTPOW, 8, -1.0)
+    f = H.formula("mandelbox")
+    assert (f.kind, f.scale, f.min_r, f.fold) == (H.FORMULA_AMAZINGBOX, 2.0, 0.5, 1.0)
+    g = H.Formula()
+    assert H.load().mb3d_hip_formula_default(b"Menger3", C.byref(g)) == H.UNSUPPORTED
+
+
+def test_params_of_the_new_formulas():
+    """DE option, 4D mode, DE scale and bailout per formula (src/formulas.rs:
+    492-528, src/calc.rs:475); the integer options as set_option keeps them."""
+    p = H.params_from_scene(H.scene_preset("Quaternion"))
+    assert p.de_option == 4 and p.four_d and p.do_julia and p.d_rstop == 256.0
+    assert [list(r) for r in p.smatrix4] == [[1.0 if i == k else 0.0 for k in range(4)] for i in range(4)]
+    p = H.params_from_scene(H.scene_preset("Bulbox"))
+    assert p.de_option == 0 and not p.four_d and p.d_rstop == 1024.0 ** 2
+    de_offset = np.float32(min(np.float32(p.de_stop) * np.float32(0.1), np.float32(0.004)))
+    assert p.d_de_scale == np.float32(np.float32(0.2) * de_offset)
+    p = H.params_from_scene(_mixed_hybrid())
+    assert p.de_option == 0 and not p.four_d  # 4 combined with 0 is the 3D numeric DE
+    assert p.slot_power[2] == 5 and p.slot_opt[2][0] == 5.0
+    assert list(p.slot_opt[3]) == [8.0, 1.0, 1.0, 1.0, 8.0, 1.0, 0.0, 21.0]
 
 
 def test_params_of_the_hybrid_chain():
@@ -105,12 +184,16 @@ def test_invalid_scenes_are_rejected():
 
 def test_unsupported_scenes_name_the_reason():
     lib = H.load()
-    s = H.scene_default()
-    s.formulas[0].power = 5
+    s = H.scene_bulb(64, 48)  # the analytic power-8 DE in Julia mode
+    s.julia = 1
     p = H.params_from_scene(s)
     buf = C.create_string_buffer(200)
     assert lib.mb3d_hip_supported(C.byref(p), buf, 200) == H.UNSUPPORTED
-    assert b"power" in buf.value
+    assert b"Julia" in buf.value
+    s = H.scene_default()
+    s.formulas[0].power = 5  # other powers march in FP64
+    p = H.params_from_scene(s)
+    assert lib.mb3d_hip_supported(C.byref(p), buf, 200) == H.OK
     s = H.scene_default()
     s.smooth_normals = 3
     p = H.params_from_scene(s)

@@ -25,17 +25,7 @@ import oracle as O
 
 def preset(name, w=96, h=72):
     """Scene::preset (src/scene.rs:811-838) at the test size."""
-    s = H.scene_default()
-    s.width, s.height = w, h
-    if name == "Integer Power":
-        s.z_start, s.zoom, s.iterations = -2.5, 0.8, 12
-        H.set_camera(s, H.vgrads_from_angles(-0.55, 0.45, 0.0))
-        H.set_formulas(s, [H.intpow(8, 1)])
-    else:
-        s.z_start, s.zoom, s.iterations = -14.0, 0.18, 20
-        H.set_camera(s, H.vgrads_from_angles(0.35, -0.6, 0.0))
-        H.set_formulas(s, [H.amazing_box(2.0, 0.5, 1.0, 1)])
-    return s
+    return H.scene_preset(name, w, h)
 
 
 def triplex(n, v):
@@ -110,7 +100,7 @@ def test_box_analytic_distance_estimate_is_sane():
     assert 0.5 < d < 6.0
 
 
-@pytest.mark.parametrize("name", ["Integer Power", "Amazing Box"])
+@pytest.mark.parametrize("name", sorted(H.FORMULA_NAMES))
 def test_presets_hit_the_object(name):
     r = O.render(O.params(preset(name)), threads=4)
     obj = np.isfinite(r["depth"])
