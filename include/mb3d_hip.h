@@ -108,6 +108,10 @@ typedef struct mb3d_hip_scene {
     int32_t de_mode;           /* MB3D_HIP_DE_* */
     int32_t step_cap;          /* extension: max march steps per ray (0: the reference's guard of 2e6) */
     int32_t julia;             /* 0/1: Julia mode, the constant is julia_c (Scene::julia) */
+    int32_t interpolation;     /* 0/1: the interpolation hybrid of the first two formulas
+                                  (Scene::interpolation, src/scene.rs:510) */
+    float interpolation_w[2];  /* its weights (normalised to a sum of 1 by the set-up) */
+    int32_t reserved;
     double rstop;              /* bailout radius (not squared); <= 0: the formulas' default */
     double zoom;
     double mid[3];
@@ -145,9 +149,9 @@ typedef struct mb3d_hip_params {
     int32_t first_step_random, normals_on_de;
     int32_t color_option, dfog_on_it;
     int32_t step_cap;
-    int32_t four_d;          /* 1: the 4D alternating hybrid (DE options 4..6, src/calc.rs:475) */
+    int32_t four_d;          /* 1: a 4D hybrid (DE options 4..6, src/calc.rs:475) */
     int32_t do_julia;        /* Julia mode: the constant is julia_c */
-    int32_t reserved;
+    int32_t ipol;            /* 1: the interpolation hybrid of slots 0 and 1 (HybridMode::Ipol3D/4D) */
     double slot_z_mul[MB3D_HIP_MAX_SLOTS];
     double slot_scale[MB3D_HIP_MAX_SLOTS];
     double slot_min_r[MB3D_HIP_MAX_SLOTS];
@@ -157,6 +161,7 @@ typedef struct mb3d_hip_params {
     float mcts_m, ms_de_sub, d_col_plus, col_var_de_stop_mul;
     float fovx_off, fovx_mul, pl_optic_z, mct_color_mul;
     float fhln[MB3D_HIP_MAX_SLOTS];
+    float ipol_w[2];         /* the interpolation weights, normalised (CalcParams::ipol, src/calc.rs:25) */
     double d_rstop, rstop3d, fov_y, zcorr, zc_mul, zend, step_width;
     double vgrads[3][3];
     double ystart[3];

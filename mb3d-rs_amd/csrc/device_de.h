@@ -43,7 +43,8 @@ struct DevP {
     // Integer Power, Folding Int Pow: the power; Aexion C: rotate C (1),
     // conditional phi (2), power C on |Z - C| (4), mode << 8
     int slot_iarg[MB3D_HIP_MAX_SLOTS];
-    int four_d, do_julia;
+    int four_d, do_julia, ipol;
+    float ipol_w[2];  // the interpolation hybrid's weights
     // the f64 options of each slot, Formula::options order without the
     // integers (src/formulas.rs:308-357): Integer Power (z_mul); Amazing
     // Box (scale, min_r, fold); Real Power (power, z_mul); Quaternion
@@ -559,10 +560,54 @@ __device__ __forceinline__ float smooth64(const DevP &P, const It64 &s, int n) {
     return (float)(((double)P.ln_rstop - d) / (e + 1e-100) + (double)s.it_result);
 }
 
+// run_ipol (src/iteration.rs:347-401): the interpolation hybrid of slots 0
+// and 1 -- both applied to the same vector, the results and their lengths
+// blended by the weights; with an analytic DE the derivative in w (3D)
+// blends too.  Returns the slot the smooth iterations use (0).
+__device__ __forceinline__ int run_ipol64(const DevP &P, It64 &s, int max_it, double rstop) {
+    const double s1 = (double)P.ipol_w[0], s2 = (double)P.ipol_w[1];
+    const bool four = P.four_d != 0, de = P.is_custom_de != 0;
+    s.it_result = 0;
+    for (;;) {
+        s.rold = s.rout;
+        const double y0 = s.v[0], y1 = s.v[1], y2 = s.v[2], y3 = s.v[3];
+        call_slot64<K_GEN>(P, 0, s);
+        const double a0 = s.v[0], a1 = s.v[1], a2 = s.v[2], a3 = s.v[3];
+        s.v[0] = y0;
+        s.v[1] = y1;
+        s.v[2] = y2;
+        s.v[3] = y3;
+        call_slot64<K_GEN>(P, 1, s);
+        const double x = s.v[0], y = s.v[1], z = s.v[2], w = s.v[3];
+        double la, lb;
+        if (four) {
+            la = sqrt(a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3);
+            lb = sqrt(x * x + y * y + z * z + w * w);
+        } else {
+            la = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+            lb = sqrt(x * x + y * y + z * z);
+        }
+        const double len = la * s1 + lb * s2;
+        double v0 = a0 * s1 + x * s2, v1 = a1 * s1 + y * s2, v2 = a2 * s1 + z * s2, v3 = a3 * s1 + w * s2;
+        if (de && !four) v3 = fabs(a3) * s1 + fabs(w) * s2;
+        const double l2 = four ? v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3 : v0 * v0 + v1 * v1 + v2 * v2;
+        const double f = len / sqrt(l2 + 1e-40);
+        s.v[0] = v0 * f;
+        s.v[1] = v1 * f;
+        s.v[2] = v2 * f;
+        s.v[3] = four ? v3 * f : v3;
+        s.it_result += 1;
+        s.rout = len * len;
+        if (s.rout < s.otrap) s.otrap = s.rout;
+        if (s.it_result >= max_it || s.rout > rstop) return 0;
+    }
+}
+
 // run_with (src/iteration.rs:268-315), counted slots, 3D or 4D squared
 // radius; returns the index of the last slot used
 template <int KIND>
 __device__ __forceinline__ int run64(const DevP &P, It64 &s, int max_it, double rstop) {
+    if (KIND == K_GEN && P.ipol) return run_ipol64(P, s, max_it, rstop);
     int n = 0;
     int btmp = P.slot_iters[0];
     s.it_result = 0;

@@ -1355,6 +1355,9 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
     d.step_cap = p.step_cap;
     d.four_d = p.four_d;
     d.do_julia = p.do_julia;
+    d.ipol = p.ipol;
+    d.ipol_w[0] = p.ipol_w[0];
+    d.ipol_w[1] = p.ipol_w[1];
     for (int k = 0; k < 4; k++) d.ju[k] = p.julia_c[k];
     for (int i = 0; i < 4; i++)
         for (int k = 0; k < 3; k++) d.sm4[3 * i + k] = (float)p.smatrix4[i][k];
@@ -1483,7 +1486,7 @@ int ensure(void **buf, size_t *have, size_t need) {
 
 int kind_of(const mb3d_hip_params &p) {
     bool p8 = false, box = false;
-    if (p.four_d || p.do_julia) return K_GEN;
+    if (p.four_d || p.do_julia || p.ipol) return K_GEN;
     for (int i = 0; i < p.n_slots; i++) {
         if (p.slot_iters[i] == 0) continue;
         const int k = p.slot_kind[i];

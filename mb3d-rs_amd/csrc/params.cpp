@@ -249,7 +249,24 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
             return set_error(MB3D_HIP_ERR_INVALID, "unknown formula kind");
         any_its |= sc->formulas[i].iterations != 0;
     }
-    if (!any_its) return set_error(MB3D_HIP_ERR_INVALID, "no formula with iterations > 0");
+    // the interpolation hybrid of the first two formulas, its weights
+    // normalised in f32 (src/calc.rs:378-397)
+    const bool ipol = sc->interpolation != 0;
+    float ipw[2] = {1.0f, 0.0f};
+    if (ipol) {
+        if (nf < 2) return set_error(MB3D_HIP_ERR_INVALID, "an interpolation hybrid needs two formulas");
+        const float sum = sc->interpolation_w[0] + sc->interpolation_w[1];
+        if (sum < 1e-10f) {
+            ipw[0] = 1.0f;
+            ipw[1] = 0.0f;
+        } else {
+            const float inv = 1.0f / sum;
+            ipw[0] = sc->interpolation_w[0] * inv;
+            ipw[1] = sc->interpolation_w[1] * inv;
+        }
+    } else if (!any_its) {
+        return set_error(MB3D_HIP_ERR_INVALID, "no formula with iterations > 0");
+    }
 
     mb3d_hip_params p;
     std::memset(&p, 0, sizeof p);
@@ -259,16 +276,17 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
     const double W = sc->width, Hh = sc->height;
 
     // the hybrid ends at the last slot with iterations (CheckHybridOptions)
-    int last = nf - 1;
-    while (last > 0 && sc->formulas[last].iterations == 0) --last;
+    // (an interpolation hybrid: two slots of one iteration each)
+    int last = ipol ? 1 : nf - 1;
+    while (!ipol && last > 0 && sc->formulas[last].iterations == 0) --last;
     p.n_slots = last + 1;
     FormulaInfo fi[MB3D_HIP_MAX_SLOTS];
     for (int i = 0; i < p.n_slots; i++) {
         const mb3d_hip_formula &f = sc->formulas[i];
         fi[i] = info_of(f);
         p.slot_kind[i] = f.kind;
-        p.slot_iters[i] = f.iterations < 0 ? -f.iterations : f.iterations;
-        p.slot_uncounted[i] = f.iterations < 0 || fi[i].de_option < 0 || fi[i].de_option > 20;
+        p.slot_iters[i] = ipol ? 1 : (f.iterations < 0 ? -f.iterations : f.iterations);
+        p.slot_uncounted[i] = (!ipol && f.iterations < 0) || fi[i].de_option < 0 || fi[i].de_option > 20;
         p.slot_power[i] = f.power;
         p.slot_z_mul[i] = f.z_mul;
         p.slot_scale[i] = f.scale;
@@ -284,8 +302,24 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
     int de_option = -1;
     for (int i = 0; i <= last; i++)
         if (p.slot_iters[i] != 0 && fi[i].de_option != de_option) de_option = combine_de(de_option, fi[i].de_option);
+    if (ipol) {
+        // ipol_de_option (src/calc.rs:255-307): only the formulas whose raw
+        // weight is positive (as the sign of its f32 bits) count
+        de_option = -1;
+        for (int x = 0; x < 2; x++) {
+            int32_t bits;
+            std::memcpy(&bits, &sc->interpolation_w[x], sizeof bits);
+            const int de = fi[x].de_option;
+            if (bits <= 0 || de < 0 || de > 20 || de == de_option) continue;
+            const bool difs_mix = (de_option >= 0 && de_option < 20 && de > 19) || (de_option > 19 && de < 20);
+            if (difs_mix) return set_error(MB3D_HIP_ERR_INVALID, "formula option is not valid");
+            de_option = combine_de(de_option, de);
+        }
+        if (de_option > 19) return set_error(MB3D_HIP_ERR_INVALID, "dIFS formulas do not work in an interpolation hybrid");
+        if (de_option < 0) return set_error(MB3D_HIP_ERR_INVALID, "formula option is not valid");
+    }
     if (sc->de_mode == MB3D_HIP_DE_ANALYTIC_P8) {
-        if (p.n_slots != 1 || sc->formulas[0].kind != MB3D_HIP_FORMULA_INTPOW)
+        if (ipol || p.n_slots != 1 || sc->formulas[0].kind != MB3D_HIP_FORMULA_INTPOW)
             return set_error(MB3D_HIP_ERR_INVALID, "analytic p8 DE needs one Integer Power slot");
         de_option = 32;  // extension: running derivative
     } else if (sc->de_mode != MB3D_HIP_DE_REFERENCE) {
@@ -296,7 +330,10 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
     p.de_option = de_option;
     p.is_custom_de = custom;
     for (int i = 0; i < p.n_slots; i++) p.slot_ade[i] = custom;
-    p.four_d = de_option >= 4 && de_option <= 6;  // HybridMode::Alt4D (src/calc.rs:475)
+    p.four_d = de_option >= 4 && de_option <= 6;  // HybridMode::Alt4D / Ipol4D (src/calc.rs:475-484)
+    p.ipol = ipol;
+    p.ipol_w[0] = ipw[0];
+    p.ipol_w[1] = ipw[1];
     p.do_julia = sc->julia != 0;
     std::copy(sc->julia_c, sc->julia_c + 4, p.julia_c);
     smatrix4(sc->rot_4d, p.smatrix4);
@@ -307,25 +344,28 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
     double cnt_all = 0, scale_sum = 0, abs_first = 0, cnt_ade = 0;
     for (int i = 0; i < p.n_slots; i++) {
         if (p.slot_iters[i] == 0) continue;
-        const double its = p.slot_iters[i];
-        if (!p.slot_uncounted[i]) {
+        // an interpolation hybrid weighs each formula by its weight and counts it once
+        const double its = ipol ? (double)ipw[i] : (double)p.slot_iters[i];
+        const double cnt = ipol ? 1.0 : its;
+        if (ipol || !p.slot_uncounted[i]) {
             const double z1 = fi[i].si_pow > 1.0 ? fi[i].si_pow : fi[i].first_option;
             p.fhln[i] = (float)(1.0 / std::log(std::fmax(2.0, std::fabs(z1))));
         }
         const int j = fi[i].de_option;
-        if (custom && !p.slot_uncounted[i] && (j == 5 || j == 11)) {
-            cnt_all += its;
-            cnt_ade += its;
+        if (custom && (ipol || !p.slot_uncounted[i]) && (j == 5 || j == 11)) {
+            cnt_all += cnt;
+            cnt_ade += cnt;
             double x2 = fi[i].first_option;
             if (x2 == 0.0) x2 = 1e-40;
             scale_sum += x2 < 0.0 ? 0.65 * its : (x2 * 1.2 + 1.0) / x2 * its;
             abs_first += std::fabs(x2) * its;
-        } else if (j >= 0) {
-            cnt_all += its;
+        } else if (j >= 0 || ipol) {
+            cnt_all += cnt;
             scale_sum += (custom ? 1.0 : fi[i].de_scale) * its;
         }
     }
-    const double de_scale0 = (cnt_all > 0.0 && de_option != 20) ? scale_sum / cnt_all : 1.0;
+    double de_scale0 = 1.0;
+    if (cnt_all > 0.0 && de_option != 20) de_scale0 = ipol ? scale_sum : scale_sum / cnt_all;
     float col_plus = 0.0f;
     if (cnt_all > 0.0 && cnt_ade > 0.0) col_plus = (float)(std::pow(std::fabs(abs_first / cnt_ade), 0.25) * 40.0 - 49.0);
 
@@ -334,8 +374,8 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
     {
         double num = 0, den = 0;
         for (int i = 0; i < p.n_slots; i++) {
-            if (!(p.slot_iters[i] > 0 && !p.slot_uncounted[i])) continue;
-            const double its = p.slot_iters[i];
+            if (!ipol && !(p.slot_iters[i] > 0 && !p.slot_uncounted[i])) continue;
+            const double its = ipol ? (double)ipw[i] : (double)p.slot_iters[i];
             const int de = fi[i].de_option;
             if (de == 0 || de == 4) {
                 num += its * 0.6;

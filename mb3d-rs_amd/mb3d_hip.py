@@ -73,6 +73,9 @@ class Scene(C.Structure):
         ("de_mode", C.c_int32),
         ("step_cap", C.c_int32),
         ("julia", C.c_int32),
+        ("interpolation", C.c_int32),
+        ("interpolation_w", C.c_float * 2),
+        ("reserved", C.c_int32),
         ("rstop", C.c_double),
         ("zoom", C.c_double),
         ("mid", C.c_double * 3),
@@ -120,7 +123,7 @@ class Params(C.Structure):
         ("step_cap", C.c_int32),
         ("four_d", C.c_int32),
         ("do_julia", C.c_int32),
-        ("reserved", C.c_int32),
+        ("ipol", C.c_int32),
         ("slot_z_mul", S6d),
         ("slot_scale", S6d),
         ("slot_min_r", S6d),
@@ -142,6 +145,7 @@ class Params(C.Structure):
         ("pl_optic_z", C.c_float),
         ("mct_color_mul", C.c_float),
         ("fhln", C.c_float * MAX_SLOTS),
+        ("ipol_w", C.c_float * 2),
         ("d_rstop", C.c_double),
         ("rstop3d", C.c_double),
         ("fov_y", C.c_double),

@@ -323,8 +323,9 @@ struct Iter {
     double deriv1 = 1.0;
     int start_from = 0, end_to = 0, repeat_from = 0;
     float fhln[6] = {0, 0, 0, 0, 0, 0};
-    bool four_d = false, do_julia = false;
+    bool four_d = false, do_julia = false, ipol = false;
     double ju[4] = {0, 0, 0, 0};
+    float ipw[2] = {1.0f, 0.0f};
     uint64_t its_p8 = 0, its_box = 0, its_other = 0;
 };
 
@@ -351,6 +352,9 @@ struct Marcher {
         for (int k = 0; k < 6; k++) it.fhln[k] = p.fhln[k];
         it.four_d = p.four_d != 0;
         it.do_julia = p.do_julia != 0;
+        it.ipol = p.ipol != 0;
+        it.ipw[0] = p.ipol_w[0];
+        it.ipw[1] = p.ipol_w[1];
         for (int k = 0; k < 4; k++) it.ju[k] = p.julia_c[k];
         for (int k = 0; k < 4; k++) it.j[k] = p.julia_c[k];  // j starts as ju (src/calc.rs:1058)
         ms_de_stop = p.de_stop;
@@ -449,7 +453,7 @@ struct Marcher {
 
     // Iteration::hybrid_4d (src/iteration.rs:757-772) with start_4d
     // (src/iteration.rs:417-428): the point rotated into 4D
-    void hybrid_4d() {
+    void start_4d() {
         const double *c = it.c;
         double v4[4];
         for (int i = 0; i < 4; i++)  // rotate_4dex (src/math.rs:209-215)
@@ -460,13 +464,95 @@ struct Marcher {
         }
         it.rout = v4[0] * v4[0] + v4[1] * v4[1] + v4[2] * v4[2] + v4[3] * v4[3];
         it.otrap = it.rout;
+    }
+
+    void hybrid_4d() {
+        start_4d();
         int n = run();
         if (it.calc_sit) calc_smooth_iterations(n);
     }
 
-    // mand_function (src/iteration.rs:859-866) for the alternating hybrids
+    // Iteration::run_ipol (src/iteration.rs:347-401): both formulas on the
+    // same vector, their results and lengths blended by the weights; `de`
+    // blends the running derivative of the analytic DE in w (3D)
+    void run_ipol(bool de) {
+        const double s1 = (double)it.ipw[0], s2 = (double)it.ipw[1];
+        const bool four = it.four_d;
+        it.it_result = 0;
+        for (;;) {
+            it.rold = it.rout;
+            double y1[4] = {it.v[0], it.v[1], it.v[2], it.v[3]};
+            call_slot(0);
+            double x1[4] = {it.v[0], it.v[1], it.v[2], it.v[3]};
+            for (int k = 0; k < 4; k++) it.v[k] = y1[k];
+            call_slot(1);
+            double x = it.v[0], y = it.v[1], z = it.v[2], w = it.v[3];
+            double xx, yy;
+            if (four) {
+                xx = std::sqrt(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2] + x1[3] * x1[3]);
+                yy = std::sqrt(x * x + y * y + z * z + w * w);
+            } else {
+                xx = std::sqrt(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2]);
+                yy = std::sqrt(x * x + y * y + z * z);
+            }
+            xx = xx * s1 + yy * s2;
+            double v[4] = {x1[0] * s1 + x * s2, x1[1] * s1 + y * s2, x1[2] * s1 + z * s2, x1[3] * s1 + w * s2};
+            if (de && !four) v[3] = std::fabs(x1[3]) * s1 + std::fabs(w) * s2;
+            double l2 = four ? v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]
+                             : v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            yy = xx / std::sqrt(l2 + 1e-40);
+            v[0] *= yy;
+            v[1] *= yy;
+            v[2] *= yy;
+            if (four) v[3] *= yy;
+            for (int k = 0; k < 4; k++) it.v[k] = v[k];
+            it.it_result += 1;
+            it.rout = xx * xx;
+            if (it.rout < it.otrap) it.otrap = it.rout;
+            if (it.it_result >= it.max_it || it.rout > it.rstop) break;
+        }
+    }
+
+    // ipol_start_3d (src/iteration.rs:403-415)
+    void ipol_start_3d() {
+        const double *c = it.c;
+        start_j3();
+        it.v[0] = c[0];
+        it.v[1] = c[1];
+        it.v[2] = c[2];
+        it.rout = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+        it.otrap = it.rout;
+    }
+
+    // doInterpolHybridPas / Pas4D (src/iteration.rs:431-438,457-464)
+    void ipol_plain() {
+        if (it.four_d) {
+            start_4d();
+        } else {
+            ipol_start_3d();
+            it.v[3] = 0.0;
+        }
+        run_ipol(false);
+        if (it.calc_sit) calc_smooth_iterations(0);
+    }
+
+    // doInterpolHybridPasDE, 3D (src/iteration.rs:441-455)
+    double ipol_3d_de() {
+        ipol_start_3d();
+        it.v[3] = (it.de_option & 0x18) == 16 ? it.rout : 1.0;
+        run_ipol(true);
+        double w = it.v[3];
+        double r = (it.de_option & 7) == 4 ? std::fabs(it.v[2]) * std::log(std::fabs(it.v[2])) / w
+                                           : std::sqrt(it.rout) / std::fabs(w);
+        if (it.calc_sit) calc_smooth_iterations(0);
+        return r;
+    }
+
+    // mand_function (src/iteration.rs:859-866)
     void mand_function() {
-        if (it.four_d)
+        if (it.ipol)
+            ipol_plain();
+        else if (it.four_d)
             hybrid_4d();
         else
             hybrid_3d();
@@ -520,7 +606,7 @@ struct Marcher {
         de_evals++;
         double result;
         if (p.is_custom_de) {
-            result = hybrid_3d_de() * (double)p.d_de_scale;
+            result = (it.ipol ? ipol_3d_de() : hybrid_3d_de()) * (double)p.d_de_scale;
         } else {
             mand_function();
             if (it.rout < 1e-200) {
@@ -1306,9 +1392,27 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
         int k = sc->formulas[i].kind;
         if (k < MB3D_HIP_FORMULA_INTPOW || k > MB3D_HIP_FORMULA_AEXIONC) return fail("unknown formula kind");
     }
+    // interpolation hybrid: the first two formulas, weights normalised to a
+    // sum of 1 in f32 (src/calc.rs:378-397)
+    const bool ipol = sc->interpolation != 0;
+    float ipw[2] = {1.0f, 0.0f};
+    if (ipol) {
+        if (nf < 2) return fail("an interpolation hybrid needs two formulas");
+        float x1 = sc->interpolation_w[0], x2 = sc->interpolation_w[1];
+        float y1 = x1 + x2;
+        if (y1 < 1e-10f) {
+            y1 = 1.0f;
+            x1 = 1.0f;
+            x2 = 0.0f;
+        } else {
+            y1 = 1.0f / y1;
+        }
+        ipw[0] = x1 * y1;
+        ipw[1] = x2 * y1;
+    }
     bool any = false;
     for (int i = 0; i < nf; i++) any = any || sc->formulas[i].iterations != 0;
-    if (!any) return fail("no formula with iterations > 0");
+    if (!any && !ipol) return fail("no formula with iterations > 0");
     mb3d_hip_params p;
     std::memset(&p, 0, sizeof p);
     p.abi_version = MB3D_HIP_ABI_VERSION;
@@ -1316,15 +1420,15 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
     p.width = width;
     p.height = height;
     // slots; the last with iterations ends the hybrid (CheckHybridOptions)
-    int last = nf - 1;
-    while (last > 0 && sc->formulas[last].iterations == 0) last--;
+    int last = ipol ? 1 : nf - 1;  // an interpolation hybrid's two slots count one iteration each
+    while (!ipol && last > 0 && sc->formulas[last].iterations == 0) last--;
     int ns = last + 1;
     p.n_slots = ns;
     for (int i = 0; i < ns; i++) {
         const mb3d_hip_formula &f = sc->formulas[i];
         p.slot_kind[i] = f.kind;
-        p.slot_iters[i] = std::abs(f.iterations);
-        p.slot_uncounted[i] = f.iterations < 0;
+        p.slot_iters[i] = ipol ? 1 : std::abs(f.iterations);
+        p.slot_uncounted[i] = !ipol && f.iterations < 0;
         p.slot_power[i] = f.power;
         p.slot_z_mul[i] = f.z_mul;
         p.slot_scale[i] = f.scale;
@@ -1355,7 +1459,39 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
             de_option = (de == 2 || de == 5) ? 2 : (de != 6 ? 0 : 11);
         // (other options keep theirs)
     }
+    if (ipol) {
+        // ipol_de_option (src/calc.rs:255-307): the formulas whose raw weight
+        // is positive (as the bits of the f32)
+        de_option = -1;
+        for (int x = 0; x < 2; x++) {
+            int32_t bits;
+            std::memcpy(&bits, &sc->interpolation_w[x], 4);
+            if (bits <= 0) continue;
+            int de = formula_de_option(sc->formulas[x]);
+            if (de < 0 || de > 20 || de == de_option) continue;
+            if ((de_option >= 0 && de_option < 20 && de > 19) || (de_option > 19 && de >= 0 && de < 20)) {
+                de_option = -1;
+                break;
+            }
+            if (de_option == -1)
+                de_option = de;
+            else if (de_option == 2)
+                de_option = (de == 2 || de == 5 || de == 6 || de == 11) ? 2 : 0;
+            else if (de_option == 4)
+                de_option = (de == 5 || de == 6) ? 4 : 0;
+            else if (de_option == 5)
+                de_option = de == 4 ? 4 : ((de == 2 || de == 11) ? 2 : (de != 6 ? 0 : 5));
+            else if (de_option == 6)
+                de_option = (de == 2 || de == 4 || de == 5 || de == 11) ? de : 0;
+            else if (de_option == 11)
+                de_option = (de == 2 || de == 5) ? 2 : (de != 6 ? 0 : 11);
+        }
+        if (de_option > 19) de_option = 20;
+        if (de_option >= 20) return fail("dIFS formulas do not work in an interpolation hybrid");
+        if (de_option < 0) return fail("formula option is not valid");
+    }
     if (sc->de_mode == MB3D_HIP_DE_ANALYTIC_P8) {
+        if (ipol) return fail("analytic p8 DE needs one Integer Power slot");
         if (ns != 1 || sc->formulas[0].kind != MB3D_HIP_FORMULA_INTPOW) return fail("analytic p8 DE needs one Integer Power slot");
         de_option = 32;
     } else if (sc->de_mode != MB3D_HIP_DE_REFERENCE) {
@@ -1365,7 +1501,10 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
                         de_option == 32;
     p.de_option = de_option;
     p.is_custom_de = is_custom_de;
-    p.four_d = de_option >= 4 && de_option <= 6;  // HybridMode::Alt4D (src/calc.rs:475)
+    p.four_d = de_option >= 4 && de_option <= 6;  // HybridMode::Alt4D / Ipol4D (src/calc.rs:475-484)
+    p.ipol = ipol;
+    p.ipol_w[0] = ipw[0];
+    p.ipol_w[1] = ipw[1];
     p.do_julia = sc->julia != 0;
     for (int k = 0; k < 4; k++) p.julia_c[k] = sc->julia_c[k];
     build_smatrix4(sc->rot_4d[0], sc->rot_4d[1], sc->rot_4d[2], p.smatrix4);
@@ -1382,12 +1521,13 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
     for (int x = 0; x < ns; x++) {
         if (p.slot_iters[x] == 0) continue;
         const mb3d_hip_formula &f = sc->formulas[x];
-        if (!p.slot_uncounted[x]) {
+        if (ipol || !p.slot_uncounted[x]) {
             double sip = formula_si_pow(f);
             double z1 = sip > 1.0 ? sip : formula_first_option(f);
             p.fhln[x] = (float)(1.0 / std::log(std::fmax(2.0, std::fabs(z1))));
         }
-        double z1 = (double)p.slot_iters[x], cnt = (double)p.slot_iters[x];
+        // interpolation hybrid: weighted by the weights, counted once
+        double z1 = ipol ? (double)ipw[x] : (double)p.slot_iters[x], cnt = ipol ? 1.0 : (double)p.slot_iters[x];
         int j = formula_de_option(f);
         if (is_custom_de && !p.slot_uncounted[x] && (j == 5 || j == 11)) {
             n += cnt;
@@ -1406,7 +1546,7 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
     }
     float d_col_plus = 0.0f;
     double d_de_scale = 1.0;
-    if (n > 0.0 && de_option != 20) d_de_scale = x1 / n;
+    if (n > 0.0 && de_option != 20) d_de_scale = ipol ? x1 : x1 / n;
     if (n > 0.0 && ii > 0.0) d_col_plus = (float)(std::pow(std::fabs(y1 / ii), 0.25) * 40.0 - 49.0);
     // mctColVarDEstopMul (src/calc.rs:610-670), one part
     float col_var = 0.6f;
@@ -1414,7 +1554,7 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
         double yy1 = 0, zz1 = 0, zz2 = 0;
         int ia = 0, itmp = 0;
         for (int x = 0; x < ns; x++) {
-            bool counted = p.slot_iters[x] > 0 && !p.slot_uncounted[x];
+            bool counted = ipol || (p.slot_iters[x] > 0 && !p.slot_uncounted[x]);
             if (!counted) {
                 if (p.slot_iters[x] > 0) {
                     if (x > end_to)
@@ -1425,7 +1565,7 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
                 continue;
             }
             const mb3d_hip_formula &f = sc->formulas[x];
-            double x2 = (double)p.slot_iters[x];
+            double x2 = ipol ? (double)ipw[x] : (double)p.slot_iters[x];
             int de = formula_de_option(f);
             if (de == 0 || de == 4) {
                 yy1 += x2 * 0.6;

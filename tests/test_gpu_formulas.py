@@ -166,3 +166,30 @@ def test_batches_reject_the_general_formulas(hip):
     st = H.load().mb3d_hip_march_batch(arr, 1, H.C.byref(out), None)
     assert st == H.UNSUPPORTED and "FP32" in H.last_error()
     assert p.width == 64
+
+
+def ipol_scene(f0, f1, w, preset, size=(480, 360)):
+    s = H.scene_preset(preset, *size)
+    H.set_formulas(s, [f0, f1])
+    s.interpolation = 1
+    s.interpolation_w[:] = w
+    return s
+
+
+@pytest.mark.parametrize("case", ["real-tricorn", "p8-box", "box-box-analytic", "quaternions-4d"])
+def test_interpolation_hybrids(hip, case):
+    """The interpolation hybrids (run_ipol, src/iteration.rs:347-401): 3D
+    numeric, 3D with the box's analytic DE, 4D numeric."""
+    sc, scale = {
+        "real-tricorn": (ipol_scene(H.formula("Real Power", power=5.5), H.formula("Tricorn"), [0.5, 0.5],
+                                    "Integer Power"), 2.0),
+        "p8-box": (ipol_scene(H.formula("Integer Power"), H.formula("Amazing Box"), [0.7, 0.3], "Integer Power"),
+                   2.0),
+        "box-box-analytic": (ipol_scene(H.formula("Amazing Box"), H.formula("Amazing Box", scale=-1.5), [2.0, 1.0],
+                                        "Amazing Box"), 4.0),
+        "quaternions-4d": (ipol_scene(H.formula("Quaternion"), H.formula("Quaternion", yw_mul=0.5), [1.0, 3.0],
+                                      "Quaternion"), 2.0),
+    }[case]
+    p, g, r, m = run(hip, sc, scale)
+    assert p.ipol and np.isfinite(r["depth"]).mean() > 0.01
+    check(m)

@@ -208,3 +208,26 @@ def test_march_before_init_fails_without_touching_a_device():
     o = H.Out()
     st = lib.mb3d_hip_march(C.byref(p), None, 0, 48, C.byref(o), None, H.PROGRESS_FN(0), H.CANCEL_FN(0), None)
     assert st == -3 and "init" in H.last_error()
+
+
+def _ipol_scene(f0, f1, w, four=False):
+    s = H.scene_preset("Quaternion" if four else "Integer Power", 320, 240)
+    H.set_formulas(s, [f0, f1])
+    s.interpolation = 1
+    s.interpolation_w[:] = w
+    return s
+
+
+@pytest.mark.parametrize("case", ["p8-box", "box-box-analytic", "quaternions-4d", "negative-weight"])
+def test_interpolation_params_match_the_oracle(case):
+    sc = {
+        "p8-box": lambda: _ipol_scene(H.formula("Integer Power"), H.formula("Amazing Box"), [0.7, 0.3]),
+        "box-box-analytic": lambda: _ipol_scene(H.formula("Amazing Box"), H.formula("Amazing Box", scale=-1.5),
+                                                [2.0, 1.0]),
+        "quaternions-4d": lambda: _ipol_scene(H.formula("Quaternion"), H.formula("Quaternion", yw_mul=0.5),
+                                              [1.0, 3.0], four=True),
+        "negative-weight": lambda: _ipol_scene(H.formula("Integer Power"), H.formula("Amazing Box"), [-0.5, 1.0]),
+    }[case]()
+    a = H.params_from_scene(sc)
+    b = O.params(sc)
+    assert a.ipol and bytes(a) == bytes(b)

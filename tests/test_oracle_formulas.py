@@ -15,6 +15,7 @@ Closed forms and reductions:
   threshold = Amazing Box; between, the length blends linearly
   Folding Int Pow with a fold beyond the point = Integer Power
   build_smatrix4: orthonormal, single-plane rotations by hand
+  interpolation hybrids: F with itself = F; (F0, F1, w0, w1) = (F1, F0, w1, w0)
 """
 import cmath
 import math
@@ -239,3 +240,52 @@ def test_julia_mode_uses_the_seed():
                 want = it
                 break
         assert its == want, (x, y)
+
+
+def ipol_scene(f0, f1, w, name="Integer Power", size=(96, 72)):
+    s = H.scene_preset(name, *size)
+    H.set_formulas(s, [f0, f1])
+    s.interpolation = 1
+    s.interpolation_w[:] = w
+    return s
+
+
+def test_interpolation_weights_are_normalised():
+    """GetMCTparasFromHeader's normalisation (src/calc.rs:378-397) and the
+    interpolation DE scale: the weighted sum of the formulas' scales."""
+    p = O.params(ipol_scene(H.formula("Integer Power"), H.formula("Bulbox"), [3.0, 1.0]))
+    assert p.ipol and list(p.ipol_w) == [np.float32(0.75), np.float32(0.25)]
+    de_offset = np.float32(min(np.float32(p.de_stop) * np.float32(0.1), np.float32(0.004)))
+    assert p.d_de_scale == np.float32(np.float32(0.75 * 1.0 + 0.25 * 0.2) * de_offset)
+    # no positive weight: no formula gives the DE option (an error, as the reference's)
+    with pytest.raises(RuntimeError, match="not valid"):
+        O.params(ipol_scene(H.formula("Integer Power"), H.formula("Bulbox"), [0.0, 0.0]))
+    # a weight <= 0 leaves its formula's DE option out: the box alone is analytic
+    p = O.params(ipol_scene(H.formula("Integer Power"), H.formula("Amazing Box"), [-1.0, 1.0]))
+    assert p.de_option == 11 and p.is_custom_de
+    p = O.params(ipol_scene(H.formula("Quaternion"), H.formula("Quaternion", yw_mul=0.5), [1.0, 1.0]))
+    assert p.de_option == 4 and p.four_d
+
+
+def test_interpolation_of_a_formula_with_itself_is_that_formula():
+    """Blending two equal results gives the result: the picture of
+    (F, F) at any weights is that of F alone, up to the last bit of the
+    blended radius."""
+    f = H.formula("Integer Power", power=5)
+    a = O.render(O.params(ipol_scene(f, f, [0.3, 0.7])), threads=4)
+    s = H.scene_preset("Integer Power", 96, 72)
+    H.set_formulas(s, [f])
+    b = O.render(O.params(s), threads=4)
+    both = np.isfinite(a["depth"]) & np.isfinite(b["depth"])
+    assert (np.isfinite(a["depth"]) == np.isfinite(b["depth"])).mean() > 0.998
+    assert np.abs(a["depth"][both] - b["depth"][both]).max() < 1e-3
+    assert (a["iters"] == b["iters"]).mean() > 0.998
+
+
+def test_interpolation_is_symmetric():
+    """(F0, F1, w0, w1) = (F1, F0, w1, w0): the blend is a sum of two
+    products, whatever the order."""
+    f0, f1 = H.formula("Integer Power", power=3), H.formula("Tricorn")
+    a = O.render(O.params(ipol_scene(f0, f1, [0.6, 0.4])), threads=4)
+    b = O.render(O.params(ipol_scene(f1, f0, [0.4, 0.6])), threads=4)
+    assert np.array_equal(a["depth"], b["depth"]) and np.array_equal(a["iters"], b["iters"])
