@@ -111,6 +111,14 @@ typedef struct mb3d_hip_scene {
     int32_t interpolation;     /* 0/1: the interpolation hybrid of the first two formulas
                                   (Scene::interpolation, src/scene.rs:510) */
     float interpolation_w[2];  /* its weights (normalised to a sum of 1 by the set-up) */
+    /* the DE combination of two hybrid parts (Scene::decomb, DeCombParams,
+     * src/scene.rs:2178-2200): 0 none, 1 min, 2 max, 3 max with the second
+     * part inverted, 4 smooth min (linear), 5 smooth min */
+    int32_t decomb;
+    int32_t decomb_end1, decomb_start2, decomb_end2, decomb_repeat2; /* slot indices from 0 */
+    int32_t decomb_iterations2; /* the second part's maximum iterations */
+    float decomb_smooth;        /* the smooth minima's distance (scene units) */
+    int32_t part2_placed;       /* 0/1: the second part placed by part2_* (Scene::part2_place) */
     int32_t reserved;
     double rstop;              /* bailout radius (not squared); <= 0: the formulas' default */
     double zoom;
@@ -124,6 +132,9 @@ typedef struct mb3d_hip_scene {
     double color_mul;
     double julia_c[4];         /* the Julia constant (x, y, z, w); w is also J4 of the 3D hybrids */
     double rot_4d[3];          /* XW, YW, ZW rotations of the 4D hybrids (radians) */
+    double part2_offset[3];    /* the second part's middle (Placement, src/scene.rs:49-56) */
+    double part2_rotation[3];  /* its rotation about x, y, z (degrees) */
+    double part2_scale;        /* its size */
     mb3d_hip_formula formulas[MB3D_HIP_MAX_SLOTS];
 } mb3d_hip_scene;
 
@@ -152,6 +163,14 @@ typedef struct mb3d_hip_params {
     int32_t four_d;          /* 1: a 4D hybrid (DE options 4..6, src/calc.rs:475) */
     int32_t do_julia;        /* Julia mode: the constant is julia_c */
     int32_t ipol;            /* 1: the interpolation hybrid of slots 0 and 1 (HybridMode::Ipol3D/4D) */
+    /* the DE combination (DeComb, src/calc.rs:224-236): its kind (0 none)
+     * and the second part (HybridPart): slots start2..end2 repeating from
+     * repeat2, its iteration limit, DE option, analytic flag, 4D mode */
+    int32_t decomb;
+    int32_t start2, end2, repeat2, max_it2;
+    int32_t de_option2, is_custom_de2, four_d2;
+    int32_t placed;          /* the second part at place_* (Place, src/calc.rs:238-253) */
+    int32_t reserved;
     double slot_z_mul[MB3D_HIP_MAX_SLOTS];
     double slot_scale[MB3D_HIP_MAX_SLOTS];
     double slot_min_r[MB3D_HIP_MAX_SLOTS];
@@ -162,6 +181,8 @@ typedef struct mb3d_hip_params {
     float fovx_off, fovx_mul, pl_optic_z, mct_color_mul;
     float fhln[MB3D_HIP_MAX_SLOTS];
     float ipol_w[2];         /* the interpolation weights, normalised (CalcParams::ipol, src/calc.rs:25) */
+    float d_de_scale2;       /* the second part's DE scale */
+    float reserved2;
     double d_rstop, rstop3d, fov_y, zcorr, zc_mul, zend, step_width;
     double vgrads[3][3];
     double ystart[3];
@@ -170,6 +191,8 @@ typedef struct mb3d_hip_params {
     double slot_opt[MB3D_HIP_MAX_SLOTS][MB3D_HIP_FORMULA_OPTIONS];
     double smatrix4[4][4];   /* build_smatrix4 of rot_4d, f32-rounded (src/math.rs:178-204) */
     double julia_c[4];       /* `ju` (src/calc.rs:52) */
+    double decomb_smooth;    /* the smooth minima's distance in steps */
+    double place_offset[3], place_m[3][3], place_scale; /* m (p - offset); distances times scale */
 } mb3d_hip_params;
 
 /* The post passes that run with the march (`PostJob`, src/calc.rs:187-205;

@@ -33,27 +33,23 @@ enum Kind : int { K_P8 = 0, K_BOX = 1, K_HYB = 2, K_GEN = 3 };
 enum DeMode : int { DEM_NUMERIC = 0, DEM_BOX = 1, DEM_P8 = 2 };
 
 // The march parameters as the kernels take them (by value, so that the
-// uniform ones live in scalar registers).
+// uniform ones live in scalar registers).  The fields up to `opt2` are the
+// core every kind reads; a batch stages only those in LDS (FrameTab), so the
+// fields from `opt2` on belong to the general kind (K_GEN) alone, which
+// never runs batched.
 struct DevP {
     int width, height, row0, rows;
     int n_slots, end_to, repeat_from, max_it, min_it;
     int de_option, is_custom_de, de_add_steps, dfog_on_it, first_step_random, step_cap;
     int tile_w, tile_h, tiles_x;
     int slot_kind[MB3D_HIP_MAX_SLOTS], slot_iters[MB3D_HIP_MAX_SLOTS];
-    // Integer Power, Folding Int Pow: the power; Aexion C: rotate C (1),
-    // conditional phi (2), power C on |Z - C| (4), mode << 8
-    int slot_iarg[MB3D_HIP_MAX_SLOTS];
-    int four_d, do_julia, ipol;
-    float ipol_w[2];  // the interpolation hybrid's weights
-    // the f64 options of each slot, Formula::options order without the
-    // integers (src/formulas.rs:308-357): Integer Power (z_mul); Amazing
-    // Box (scale, min_r, fold); Real Power (power, z_mul); Quaternion
-    // (yw_mul, w_add); Tricorn (z_mul, cz_mul); Bulbox (scale, min_r, fold,
-    // bulb_scale, r_threshold, r_threshold2); Folding Int Pow (z_mul, fold);
-    // Aexion C (power, z_mul, power_c, cz_mul)
-    double opt[MB3D_HIP_MAX_SLOTS][6];
-    double ju[4];    // the Julia constant (and J4)
-    float sm4[12];   // rows of smatrix4's first three columns (f32 values)
+    // the first three f64 options of each slot, Formula::options order
+    // without the integers (src/formulas.rs:308-357), the rest in opt2:
+    // Integer Power (z_mul); Amazing Box (scale, min_r, fold); Real Power
+    // (power, z_mul); Quaternion (yw_mul, w_add); Tricorn (z_mul, cz_mul);
+    // Bulbox (scale, min_r, fold | bulb_scale, r_threshold, r_threshold2);
+    // Folding Int Pow (z_mul, fold); Aexion C (power, z_mul, power_c | cz_mul)
+    double opt[MB3D_HIP_MAX_SLOTS][3];
     float fz_mul[MB3D_HIP_MAX_SLOTS], fscale[MB3D_HIP_MAX_SLOTS], fmin_r2[MB3D_HIP_MAX_SLOTS];
     float fscale_min_r2[MB3D_HIP_MAX_SLOTS], ffold[MB3D_HIP_MAX_SLOTS];
     float d_de_scale, de_stop, z_step_div, mh04zsd, de_stop_factor, ln_rstop, de_offset, de_offset006;
@@ -68,6 +64,26 @@ struct DevP {
     // first pixel in the batch's output arrays (0 for a single picture)
     int64_t q_begin, pix_base;
     int it_split;  // march steps after which the FP32 pass hands a ray to the f64 finish (0: none)
+
+    // ---- the general kind (K_GEN) only ----
+    double opt2[MB3D_HIP_MAX_SLOTS][3];
+    // Integer Power, Folding Int Pow: the power; Aexion C: rotate C (1),
+    // conditional phi (2), power C on |Z - C| (4), mode << 8
+    int slot_iarg[MB3D_HIP_MAX_SLOTS];
+    int slot_ade[MB3D_HIP_MAX_SLOTS];  // the analytic-DE variant (each part's own flag)
+    int slot_unc[MB3D_HIP_MAX_SLOTS];  // iterated but not counted (nHybrid < 0)
+    int four_d, do_julia, ipol;
+    float ipol_w[2];  // the interpolation hybrid's weights
+    double ju[4];     // the Julia constant (and J4)
+    float sm4[12];    // rows of smatrix4's first three columns (f32 values)
+    // the DE combination (0: none; 1 min, 2 max, 3 max of the inverted
+    // second part, 4 and 5 smooth minima) and its second hybrid part
+    int decomb, start2, end2, repeat2, max_it2, de_option2, custom2, four_d2, placed;
+    float d_de_scale2;
+    double smooth;  // the smooth combinations' distance (steps)
+    // the second part's space (Place, src/calc.rs:238-253): m (p - offset),
+    // its distances times place_scale
+    double place_off[3], place_m[3][3], place_scale;
 };
 
 // Per-lane work counters, reduced per wave at the end of a kernel.
@@ -388,19 +404,21 @@ __device__ __forceinline__ void pow2_scale64(double scaling, double v[4], const 
 }
 
 // Bulbox, HybridSuperCube2 (src/formulas.rs:596-627)
-__device__ __forceinline__ void bulbox64(const double o[6], double v[4], const double j[4], double rout) {
-    const double rt1 = o[4] * o[4], rt2 = o[5] * o[5];
+// o: box scale, Min R, fold; o2: bulb scaling, R threshold, R threshold 2
+__device__ __forceinline__ void bulbox64(const double o[3], const double o2[3], double v[4], const double j[4],
+                                         double rout) {
+    const double rt1 = o2[1] * o2[1], rt2 = o2[2] * o2[2];
     if (!(rout < rt1)) {
         box64(o[0], o[1], o[2], v, j, false);
         return;
     }
     if (rout < rt2) {
-        pow2_scale64(o[3], v, j);
+        pow2_scale64(o2[0], v, j);
         return;
     }
     const double m = (rout - rt2) / (rt1 - rt2);
     const double in0 = v[0], in1 = v[1], in2 = v[2];
-    pow2_scale64(o[3], v, j);
+    pow2_scale64(o2[0], v, j);
     const double b0 = v[0], b1 = v[1], b2 = v[2];
     v[0] = in0;
     v[1] = in1;
@@ -434,7 +452,8 @@ __device__ __forceinline__ void real_power64(double pw, double zmul, double v[4]
 // aexion_c (src/formulas.rs:644-703): a real power bulb step with the
 // angles from the y axis; with "rotate C" the constant is turned by a power
 // of its own angles (or those of Z - C, mode 16)
-__device__ __forceinline__ void aexion64(const double o[6], int flags, double v[4], double j[4]) {
+// o: power, Z multiplier, power C; cz_mul: the Cz multiplier
+__device__ __forceinline__ void aexion64(const double o[3], double cz_mul, int flags, double v[4], double j[4]) {
     const double pw = o[0];
     const double x = v[0], y = v[1], z = v[2];
     double st, ct, sp, cp;
@@ -485,7 +504,7 @@ __device__ __forceinline__ void aexion64(const double o[6], int flags, double v[
     sincos(ph * pd, &sa, &ca);
     sincos(th * pd, &sb, &cb);
     j[0] = ca * cb * r;
-    j[2] = sb * r * o[3];
+    j[2] = sb * r * cz_mul;
     j[1] = cb * sa * r;
 }
 
@@ -517,7 +536,7 @@ __device__ __forceinline__ void call_slot64(const DevP &P, int n, It64 &s) {
             s.p8++;
             return;
         case MB3D_HIP_FORMULA_AMAZINGBOX:
-            box64(o[0], o[1], o[2], v, s.j, P.is_custom_de != 0);
+            box64(o[0], o[1], o[2], v, s.j, P.slot_ade[n] != 0);
             s.box++;
             return;
         case MB3D_HIP_FORMULA_REALPOWER: real_power64(o[0], o[1], v, s.j, s.rout); break;
@@ -536,14 +555,14 @@ __device__ __forceinline__ void call_slot64(const DevP &P, int n, It64 &s) {
             v[1] = 2.0 * (y * x) + s.j[1];
             break;
         }
-        case MB3D_HIP_FORMULA_BULBOX: bulbox64(o, v, s.j, s.rout); break;
+        case MB3D_HIP_FORMULA_BULBOX: bulbox64(o, P.opt2[n], v, s.j, s.rout); break;
         case MB3D_HIP_FORMULA_FOLDINGINTPOW:  // src/formulas.rs:629-634
             v[0] = bfold64(v[0], o[1]);
             v[1] = bfold64(v[1], o[1]);
             v[2] = bfold64(v[2], o[1]);
             int_pow64(P.slot_iarg[n], o[0], v, s.j);
             break;
-        default: aexion64(o, P.slot_iarg[n], v, s.j); break;
+        default: aexion64(o, P.opt2[n][0], P.slot_iarg[n], v, s.j); break;
     }
     s.oth++;
 }
@@ -603,28 +622,45 @@ __device__ __forceinline__ int run_ipol64(const DevP &P, It64 &s, int max_it, do
     }
 }
 
+// One hybrid part (HybridPart, src/calc.rs:137-150; set_part,
+// src/calc.rs:1470-1490): its slots, its DE option and scale, 3D or 4D.
+// A picture without a DE combination has one part.
+struct Part64 {
+    int start, end, repeat, de_option, custom, four;
+    double dds;
+};
+
+template <int KIND>
+__device__ __forceinline__ Part64 part1_of(const DevP &P) {
+    return {0, P.end_to, P.repeat_from, P.de_option, P.is_custom_de, KIND == K_GEN && P.four_d, (double)P.d_de_scale};
+}
+
 // run_with (src/iteration.rs:268-315), counted slots, 3D or 4D squared
 // radius; returns the index of the last slot used
 template <int KIND>
-__device__ __forceinline__ int run64(const DevP &P, It64 &s, int max_it, double rstop) {
+__device__ __forceinline__ int run64(const DevP &P, const Part64 &pt, It64 &s, int max_it, double rstop) {
     if (KIND == K_GEN && P.ipol) return run_ipol64(P, s, max_it, rstop);
-    int n = 0;
-    int btmp = P.slot_iters[0];
+    int n = pt.start;
+    int btmp = P.slot_iters[n];
     s.it_result = 0;
-    const bool four = KIND == K_GEN && P.four_d;
+    uint32_t uncounted_guard = 0;
     for (;;) {
         s.rold = s.rout;
         int guard = 0;
         while (btmp <= 0) {
             n += 1;
-            if (n > P.end_to) n = P.repeat_from;
+            if (n > pt.end) n = pt.repeat;
             btmp = P.slot_iters[n];
             if (++guard > 64) return n;
         }
         call_slot64<KIND>(P, n, s);
         btmp -= 1;
+        if (KIND == K_GEN && P.slot_unc[n]) {  // a slot that does not count (src/iteration.rs:294-300)
+            if (++uncounted_guard > 100000) return n;
+            continue;
+        }
         s.it_result += 1;
-        if (four)
+        if (KIND == K_GEN && pt.four)
             s.rout = s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2] + s.v[3] * s.v[3];
         else
             s.rout = s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2];
@@ -638,8 +674,8 @@ __device__ __forceinline__ int run64(const DevP &P, It64 &s, int max_it, double 
 // mode, J4 kept in j[3]; the 4D hybrids rotate c into 4D first (start_4d,
 // rotate_4dex: src/iteration.rs:417-428, src/math.rs:209-215).
 template <int KIND>
-__device__ __forceinline__ void start64(const DevP &P, It64 &s, double cx, double cy, double cz) {
-    if (KIND == K_GEN && P.four_d) {
+__device__ __forceinline__ void start64(const DevP &P, bool four, It64 &s, double cx, double cy, double cz) {
+    if (KIND == K_GEN && four) {
         for (int i = 0; i < 4; i++) {
             s.v[i] = cx * (double)P.sm4[3 * i] + cy * (double)P.sm4[3 * i + 1] + cz * (double)P.sm4[3 * i + 2];
             s.j[i] = P.do_julia ? P.ju[i] : s.v[i];
@@ -652,7 +688,7 @@ __device__ __forceinline__ void start64(const DevP &P, It64 &s, double cx, doubl
     s.j[0] = julia ? P.ju[0] : cx;
     s.j[1] = julia ? P.ju[1] : cy;
     s.j[2] = julia ? P.ju[2] : cz;
-    s.j[3] = P.ju[3];
+    s.j[3] = KIND == K_GEN ? P.ju[3] : 0.0;
     s.v[0] = cx;
     s.v[1] = cy;
     s.v[2] = cz;
@@ -665,18 +701,20 @@ __device__ __forceinline__ void start64(const DevP &P, It64 &s, double cx, doubl
 struct De64 {
     double de, rout, otrap;
     int it_result;
+    int max_its;       // max_its_result: the ruling part's iteration limit
+    int max_it_after;  // the iteration limit the evaluation leaves (set_part)
     float smooth_it;
     uint32_t p8, box, oth;
 };
 
-// calc_de_part without dIFS (src/calc.rs:1495-1559): hybrid_3d_de
-// (src/iteration.rs:539-585) for the analytic options -- option 32 being the
-// textbook 0.5 r ln r / dr of the power-8 extension -- or the alternating
-// hybrid (3D or 4D) four times for the numeric gradient.  Arguments and
-// result in registers.
+// calc_de_part without dIFS (src/calc.rs:1495-1559) for one part:
+// hybrid_3d_de (src/iteration.rs:539-585) for the analytic options --
+// option 32 being the textbook 0.5 r ln r / dr of the power-8 extension --
+// or the alternating hybrid (3D or 4D) four times for the numeric gradient.
+// Arguments and result in registers.
 template <int KIND>
-__device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double cz, int max_it, int calc_sit,
-                                  float ms) {
+__device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, double cx, double cy, double cz, int max_it,
+                                       int calc_sit, float ms) {
     De64 o;
     It64 s;
     s.p8 = s.box = s.oth = 0;
@@ -684,26 +722,26 @@ __device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double
     s.rold = 0.0;
     double result;
     o.smooth_it = 0.0f;
-    start64<KIND>(P, s, cx, cy, cz);
-    if (P.is_custom_de) {
-        if ((P.de_option & 0x38) == 32)
+    start64<KIND>(P, pt.four, s, cx, cy, cz);
+    if (pt.custom) {
+        if ((pt.de_option & 0x38) == 32)
             s.deriv1 = 1.0;
-        else if ((P.de_option & 0x38) == 16)
+        else if ((pt.de_option & 0x38) == 16)
             s.v[3] = s.rout;
         else
             s.v[3] = 1.0;
-        const int n = run64<KIND>(P, s, max_it, P.d_rstop);
-        if ((P.de_option & 0x38) == 32)
+        const int n = run64<KIND>(P, pt, s, max_it, P.d_rstop);
+        if ((pt.de_option & 0x38) == 32)
             result = sqrt(s.rout) * 0.25 * log(s.rout) / s.deriv1;
         else
             result = sqrt(s.rout) / fabs(s.v[3]);
         if (calc_sit) o.smooth_it = smooth64(P, s, n);
-        result = result * (double)P.d_de_scale;
+        result = result * pt.dds;
         o.it_result = s.it_result;
         o.rout = s.rout;
         o.otrap = s.otrap;
     } else {
-        const int n = run64<KIND>(P, s, max_it, P.d_rstop);
+        const int n = run64<KIND>(P, pt, s, max_it, P.d_rstop);
         if (calc_sit) o.smooth_it = smooth64(P, s, n);
         o.it_result = s.it_result;
         o.rout = s.rout;
@@ -716,22 +754,97 @@ __device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double
             const double off = (double)P.de_offset;
             double g[3];
             for (int k = 0; k < 3; k++) {
-                start64<KIND>(P, s, k == 0 ? cx + off : cx, k == 1 ? cy + off : cy, k == 2 ? cz + off : cz);
-                run64<KIND>(P, s, it0, P.rstop3d);
+                start64<KIND>(P, pt.four, s, k == 0 ? cx + off : cx, k == 1 ? cy + off : cy, k == 2 ? cz + off : cz);
+                run64<KIND>(P, pt, s, it0, P.rstop3d);
                 g[k] = (buf_rout - s.rout) * (buf_rout - s.rout);
             }
-            result = buf_rout * log(buf_rout) * (double)P.d_de_scale /
-                     (sqrt(g[0] + g[1] + g[2]) + (double)P.de_offset006);
+            result = buf_rout * log(buf_rout) * pt.dds / (sqrt(g[0] + g[1] + g[2]) + (double)P.de_offset006);
             o.otrap = s.otrap;  // the last pass's, as the reference leaves it
         }
     }
     const double mn = (double)ms * 0.25;
     if (result < mn || isnan(result)) result = mn;
     o.de = result;
+    o.max_its = max_it;
+    o.max_it_after = max_it;
     o.p8 = s.p8;
     o.box = s.box;
     o.oth = s.oth;
     return o;
+}
+
+// calc_de_fractal with a DE combination (src/calc.rs:1352-1456): the first
+// part at c, the second at its own place (and inverted for kind 3), then the
+// minimum, maximum or a smooth minimum of the two; the iteration count,
+// trap and smooth iterations of the ruling part (blended for the smooth
+// minima).  Afterwards the iteration limit is the first part's.
+__device__ __noinline__ De64 de64_comb(const DevP &P, double cx, double cy, double cz, int max_it, int calc_sit,
+                                       float ms) {
+    const De64 a = de64_part<K_GEN>(P, part1_of<K_GEN>(P), cx, cy, cz, max_it, calc_sit, ms);
+    double qx = cx, qy = cy, qz = cz;
+    if (P.placed) {  // Place::apply: m (p - offset)
+        const double d0 = cx - P.place_off[0], d1 = cy - P.place_off[1], d2 = cz - P.place_off[2];
+        qx = P.place_m[0][0] * d0 + P.place_m[0][1] * d1 + P.place_m[0][2] * d2;
+        qy = P.place_m[1][0] * d0 + P.place_m[1][1] * d1 + P.place_m[1][2] * d2;
+        qz = P.place_m[2][0] * d0 + P.place_m[2][1] * d1 + P.place_m[2][2] * d2;
+    }
+    const Part64 p2 = {P.start2, P.end2, P.repeat2, P.de_option2, P.custom2, P.four_d2, (double)P.d_de_scale2};
+    De64 o = de64_part<K_GEN>(P, p2, qx, qy, qz, P.max_it2, calc_sit, ms);
+    double rst = o.de;
+    const double msd = (double)ms;
+    if (P.decomb == 3) {  // the second part inverted (calc_de_part's `invert`)
+        rst = msd * 4.0 - rst * 3.0;
+        if (rst >= msd) o.it_result -= 1;
+    }
+    if (P.placed) rst *= P.place_scale;
+    double result = a.de;
+    bool first_rules = false;
+    if (P.decomb == 1) {
+        if (rst >= result)
+            first_rules = true;
+        else
+            result = rst;
+    } else if (P.decomb <= 3) {
+        if (rst > result)
+            result = rst;
+        else
+            first_rules = true;
+    } else {
+        if (!(rst < result)) {
+            o.it_result = a.it_result;
+            o.max_its = a.max_its;
+        }
+        const double sm = P.smooth;
+        if (P.decomb == 4) {
+            result = fmin(rst - fmax(sm - result, 0.0), result - fmax(sm - rst, 0.0));
+        } else {
+            const double rd1 = fmax(sm - result - msd, 0.0), d2 = fmax(sm - rst - msd, 0.0);
+            result = fmin(rst - rd1 * (sm - d2) / sm, result - d2 * (sm - rd1) / sm);
+        }
+        const double r2 = fabs(rst), w = fabs(result);
+        const double f = 1.0 / (w + r2 + 1e-10);
+        o.smooth_it = (float)(((double)o.smooth_it * w + (double)a.smooth_it * r2) * f);
+        o.otrap = (o.otrap * w + a.otrap * r2) * f;
+    }
+    if (first_rules) {
+        o.it_result = a.it_result;
+        o.smooth_it = a.smooth_it;
+        o.otrap = a.otrap;
+        o.max_its = a.max_its;
+    }
+    o.de = result;
+    o.max_it_after = P.max_it;
+    o.p8 += a.p8;
+    o.box += a.box;
+    o.oth += a.oth;
+    return o;
+}
+
+template <int KIND>
+__device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double cz, int max_it, int calc_sit,
+                                  float ms) {
+    if (KIND == K_GEN && P.decomb) return de64_comb(P, cx, cy, cz, max_it, calc_sit, ms);
+    return de64_part<KIND>(P, part1_of<KIND>(P), cx, cy, cz, max_it, calc_sit, ms);
 }
 
 // The reference's Marcher (src/calc.rs:1143-1170) for one ray in f64: the
@@ -776,7 +889,8 @@ struct Ray64 {
         rout = r.rout;
         otrap = r.otrap;
         if (calc_sit) smooth_it = r.smooth_it;
-        max_its = max_it;
+        max_its = r.max_its;
+        max_it = r.max_it_after;
         cnt.de64++;
         cnt.p8_64 += r.p8;
         cnt.box_64 += r.box;

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdlib>
 #include <cmath>
 #include <cstdio>
@@ -224,24 +225,45 @@ __device__ int32_t dev_pixel_seed(int64_t y, int64_t x) {
 // ---------------------------------------------------------------------------
 // pass 1: the FP32 march, persistent, lanes refilled one by one
 
-// A batch's frame table copied to LDS (the dynamic LDS of the launch,
-// nframes x sizeof(DevP)): the march and finish read a frame's parameters in
-// their inner loops, and from global memory every read is a vector load
-// (the frame differs per lane).  All threads of the block call it.
-__device__ __forceinline__ const DevP *stage_frames(const DevP *__restrict__ tab, int nframes) {
+// A batch's frame table: frame f's parameters at base + f * STRIDE bytes.
+// In global memory the records are whole DevPs; the march and finish
+// kernels stage their cores (the fields before DevP::opt2, all that the
+// batched kinds read) in LDS, DEVP_CORE bytes apart.
+constexpr int DEVP_CORE = (int)offsetof(DevP, opt2);
+static_assert(DEVP_CORE % 8 == 0, "the core of DevP in 8-byte words");
+template <int STRIDE>
+struct Frames {
+    const char *base;
+    __device__ __forceinline__ const DevP &operator[](int f) const {
+        return *reinterpret_cast<const DevP *>(base + (size_t)f * STRIDE);
+    }
+};
+using LdsFrames = Frames<DEVP_CORE>;
+using GlobalFrames = Frames<(int)sizeof(DevP)>;
+
+// The cores of a batch's frames copied to LDS (the dynamic LDS of the
+// launch, nframes x DEVP_CORE bytes): the march and finish read a frame's
+// parameters in their inner loops, and from global memory every read is a
+// vector load (the frame differs per lane).  All threads of the block call
+// it.
+__device__ __forceinline__ LdsFrames stage_frames(const DevP *__restrict__ tab, int nframes) {
     extern __shared__ __attribute__((aligned(16))) uint2 frames_lds[];
-    const int words = nframes * (int)(sizeof(DevP) / sizeof(uint2));
+    constexpr int core_words = DEVP_CORE / (int)sizeof(uint2), full_words = (int)(sizeof(DevP) / sizeof(uint2));
     const uint2 *src = reinterpret_cast<const uint2 *>(tab);
-    for (int i = threadIdx.x; i < words; i += blockDim.x) frames_lds[i] = src[i];
+    for (int i = threadIdx.x; i < nframes * core_words; i += blockDim.x) {
+        const int f = i / core_words, w = i - f * core_words;
+        frames_lds[i] = src[f * full_words + w];
+    }
     __syncthreads();
-    return reinterpret_cast<const DevP *>(frames_lds);
+    return LdsFrames{reinterpret_cast<const char *>(frames_lds)};
 }
 static_assert(sizeof(DevP) % sizeof(uint2) == 0, "DevP in 8-byte words");
 constexpr int MAX_BATCH_FRAMES = 64;
-static_assert(MAX_BATCH_FRAMES * sizeof(DevP) <= 65536, "a batch's frame table fits one block's LDS");
+static_assert(MAX_BATCH_FRAMES * DEVP_CORE <= 65536, "a batch's frame table fits one block's LDS");
 
 // the frame of a batch that work item q belongs to
-__device__ __forceinline__ int find_frame(const DevP *__restrict__ tab, int n, long long q) {
+template <class TAB>
+__device__ __forceinline__ int find_frame(const TAB &tab, int n, long long q) {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -255,9 +277,9 @@ __device__ __forceinline__ int find_frame(const DevP *__restrict__ tab, int n, l
 
 // the pixel of work item q of the tile queue (false: the item is in a
 // ragged edge tile, outside the picture)
-template <bool BATCH>
-__device__ __forceinline__ bool item_pixel(const DevP &P0, const DevP *__restrict__ tab, int nframes, long long q,
-                                           int &fr, int &x, int &yr) {
+template <bool BATCH, class TAB>
+__device__ __forceinline__ bool item_pixel(const DevP &P0, const TAB &tab, int nframes, long long q, int &fr, int &x,
+                                           int &yr) {
     fr = BATCH ? find_frame(tab, nframes, q) : 0;
     const DevP &F = BATCH ? tab[fr] : P0;
     const int tile_n = P0.tile_w * P0.tile_h;
@@ -277,8 +299,9 @@ __device__ __forceinline__ bool item_pixel(const DevP &P0, const DevP *__restric
 // which on this GPU write back and invalidate an XCD's whole L2 (measured:
 // +0.75 ms a 1080p frame at a fence pair per chunk of 256 rays).
 template <bool BATCH>
-__global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__restrict__ tab, int nframes, MlpDev M,
+__global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__restrict__ gtab, int nframes, MlpDev M,
                                                      float2 *prop) {
+    const GlobalFrames tab{reinterpret_cast<const char *>(gtab)};
     __shared__ __attribute__((aligned(16))) char lds[4][MLP_LDS_BYTES];
     __shared__ mlp_s4 wl[MLP_TILES * 64];
     mlp_stage_weights(M, wl);
@@ -321,7 +344,7 @@ __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__rest
 template <int KIND, int DEM, bool BATCH, bool LEARN>
 __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__restrict__ gtab, int nframes,
                                                  MlpDev M) {
-    const DevP *__restrict__ tab = BATCH ? stage_frames(gtab, nframes) : gtab;
+    const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
     Cnt cnt;
     int fr = 0;
     bool have = false, exhausted = false;
@@ -606,7 +629,7 @@ enum FinPhase : int { FP_START, FP_RESUME, FP_BACK, FP_STEP, FP_REF, FP_NC, FP_N
 template <int KIND, int DEM, bool N32, bool BATCH>
 __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixels, const DevP *__restrict__ gtab,
                                                   int nframes) {
-    const DevP *__restrict__ tab = BATCH ? stage_frames(gtab, nframes) : gtab;
+    const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
     Cnt cnt;
     int fr = 0;
     const long long n_items = all_pixels ? P0.npix : (long long)B.ctr[C_CAND];
@@ -1303,9 +1326,19 @@ std::string unsupported_reason(const mb3d_hip_params *p) {
     for (int i = 0; i < p->n_slots; i++) {
         if (p->slot_kind[i] < MB3D_HIP_FORMULA_INTPOW || p->slot_kind[i] > MB3D_HIP_FORMULA_AEXIONC)
             return "formula not on this backend";
-        if (p->slot_uncounted[i] && p->slot_iters[i] != 0) return "uncounted hybrid slots (not on this backend)";
-        if ((p->slot_ade[i] != 0) != (p->is_custom_de != 0))
-            return "per-slot analytic DE flags that differ (a DE combination; not on this backend)";
+        const bool second = p->decomb && i > p->end_to;
+        if ((p->slot_ade[i] != 0) != ((second ? p->is_custom_de2 : p->is_custom_de) != 0))
+            return "a slot's analytic DE flag differs from its part's";
+    }
+    if (p->decomb) {
+        if (p->decomb < 1 || p->decomb > 5) return "this DE combination (not on this backend)";
+        if (p->start2 <= p->end_to || p->end2 < p->start2 || p->end2 >= p->n_slots || p->repeat2 < p->start2 ||
+            p->repeat2 > p->end2)
+            return "the DE combination's second part does not fit the hybrid";
+        if (!(p->de_option2 == 0 || p->de_option2 == 4 || p->de_option2 == 11)) return "this DE option of the second part";
+        if ((p->four_d2 != 0) != (p->de_option2 == 4)) return "the second part's 4D mode does not fit its DE option";
+        if (p->ipol) return "a DE combination of an interpolation hybrid";
+        if (p->decomb >= 4 && !(p->decomb_smooth > 0.0)) return "the smooth combination's distance";
     }
     if (!(p->de_option == 0 || p->de_option == 4 || p->de_option == 11 || p->de_option == 32))
         return "this DE option (not on this backend)";
@@ -1358,14 +1391,30 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
     d.ipol = p.ipol;
     d.ipol_w[0] = p.ipol_w[0];
     d.ipol_w[1] = p.ipol_w[1];
+    d.decomb = p.decomb;
+    d.start2 = p.start2;
+    d.end2 = p.end2;
+    d.repeat2 = p.repeat2;
+    d.max_it2 = p.max_it2;
+    d.de_option2 = p.de_option2;
+    d.custom2 = p.is_custom_de2;
+    d.four_d2 = p.four_d2;
+    d.d_de_scale2 = p.d_de_scale2;
+    d.smooth = p.decomb_smooth;
+    d.placed = p.placed;
+    for (int k = 0; k < 3; k++) {
+        d.place_off[k] = p.place_offset[k];
+        for (int q = 0; q < 3; q++) d.place_m[k][q] = p.place_m[k][q];
+    }
+    d.place_scale = p.place_scale;
     for (int k = 0; k < 4; k++) d.ju[k] = p.julia_c[k];
     for (int i = 0; i < 4; i++)
         for (int k = 0; k < 3; k++) d.sm4[3 * i + k] = (float)p.smatrix4[i][k];
     for (int i = 0; i < MB3D_HIP_MAX_SLOTS; i++) {
         d.slot_kind[i] = p.slot_kind[i];
         d.slot_iters[i] = std::max(p.slot_iters[i], 0);
-        // the f64 options (DevP::opt's order) and the integer argument
-        double *o = d.opt[i];
+        // the f64 options (DevP::opt, then opt2) and the integer argument
+        double o[6] = {0, 0, 0, 0, 0, 0};
         const double *so = p.slot_opt[i];
         switch (p.slot_kind[i]) {
             case MB3D_HIP_FORMULA_INTPOW:
@@ -1394,6 +1443,12 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
                 for (int k = 0; k < 6; k++) o[k] = so[k];
                 break;
         }
+        for (int k = 0; k < 3; k++) {
+            d.opt[i][k] = o[k];
+            d.opt2[i][k] = o[3 + k];
+        }
+        d.slot_ade[i] = p.slot_ade[i];
+        d.slot_unc[i] = p.slot_uncounted[i] != 0;
         d.fz_mul[i] = (float)p.slot_z_mul[i];
         d.fscale[i] = (float)p.slot_scale[i];
         const double m = std::fmax(p.slot_min_r[i], 1e-40);
@@ -1486,9 +1541,10 @@ int ensure(void **buf, size_t *have, size_t need) {
 
 int kind_of(const mb3d_hip_params &p) {
     bool p8 = false, box = false;
-    if (p.four_d || p.do_julia || p.ipol) return K_GEN;
+    if (p.four_d || p.do_julia || p.ipol || p.decomb) return K_GEN;
     for (int i = 0; i < p.n_slots; i++) {
         if (p.slot_iters[i] == 0) continue;
+        if (p.slot_uncounted[i]) return K_GEN;
         const int k = p.slot_kind[i];
         if (k == MB3D_HIP_FORMULA_INTPOW && p.slot_power[i] == 8) p8 = true;
         else if (k == MB3D_HIP_FORMULA_AMAZINGBOX) box = true;
@@ -1514,7 +1570,7 @@ void launch_propose(hipStream_t s, const DevP &d, const DevP *tab, int n, const 
 template <int KIND, int DEM, bool BATCH = false>
 void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b, const MlpDev &M, const DevP *tab = nullptr,
                   int n = 1) {
-    const size_t lds = BATCH ? (size_t)n * sizeof(DevP) : 0;
+    const size_t lds = BATCH ? (size_t)n * DEVP_CORE : 0;
     if constexpr (KIND == K_GEN) return;  // FP64 only: the hosts never ask for it
     else if (M.enabled)
         hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, true>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
@@ -1524,7 +1580,7 @@ void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b, const M
 template <int KIND, int DEM, bool N32, bool BATCH = false>
 void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all, const DevP *tab = nullptr,
                    int n = 1) {
-    const size_t lds = BATCH ? (size_t)n * sizeof(DevP) : 0;
+    const size_t lds = BATCH ? (size_t)n * DEVP_CORE : 0;
     if constexpr (KIND == K_GEN && (N32 || BATCH)) return;  // FP64 only, never batched
     else hipLaunchKernelGGL((k_finish64<KIND, DEM, N32, BATCH>), dim3(grid), dim3(256), lds, s, d, b, all, tab, n);
 }

@@ -176,6 +176,14 @@ extern "C" void mb3d_hip_scene_default(mb3d_hip_scene *s) {
     s->z_step_div = 0.5;
     s->raystep_limiter = 1.0;
     s->color_mul = 1.0;
+    // DeCombParams::new (src/scene.rs:2196-2200) for when a combination is switched on
+    s->decomb_end1 = 0;
+    s->decomb_start2 = 1;
+    s->decomb_end2 = 5;
+    s->decomb_repeat2 = 1;
+    s->decomb_iterations2 = s->iterations;
+    s->decomb_smooth = 0.5f;
+    s->part2_scale = 1.0;
     s->formulas[0].kind = MB3D_HIP_FORMULA_INTPOW;
     s->formulas[0].iterations = 1;
     s->formulas[0].power = 8;
@@ -294,14 +302,42 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
         p.slot_fold[i] = f.fold;
         if (const int pw = slot_options(f, p.slot_opt[i])) p.slot_power[i] = pw;
     }
-    p.end_to = last;
-    int rf = std::min(sc->repeat_from, last);
+    // a DE combination splits the hybrid in two parts, when both have slots
+    // with iterations (src/calc.rs:421-431)
+    const int dck = ipol ? 0 : sc->decomb;
+    const bool dc = dck != 0 && sc->decomb_end1 < last && sc->decomb_start2 <= last;
+    if (dc && (dck < 1 || dck > 6)) return set_error(MB3D_HIP_ERR_INVALID, "unknown DE combination");
+    const int end1 = dc ? std::min(sc->decomb_end1, last) : last;
+    p.end_to = end1;
+    int rf = std::min(sc->repeat_from, end1);
     while (rf > 0 && p.slot_iters[rf] <= 0) --rf;
     p.repeat_from = rf;
 
-    int de_option = -1;
-    for (int i = 0; i <= last; i++)
-        if (p.slot_iters[i] != 0 && fi[i].de_option != de_option) de_option = combine_de(de_option, fi[i].de_option);
+    auto combined = [&](int from, int to) {
+        int de = -1;
+        for (int i = from; i <= to; i++)
+            if (p.slot_iters[i] != 0 && fi[i].de_option >= 0 && fi[i].de_option != de) de = combine_de(de, fi[i].de_option);
+        return de;
+    };
+    int de_option = combined(0, end1);
+    // the second part: its slots, DE option and repeat slot (src/calc.rs:487-520)
+    int de2 = -1;
+    if (dc) {
+        p.decomb = dck;
+        p.start2 = std::min(sc->decomb_start2, last);
+        p.end2 = std::min(std::max(sc->decomb_end2, p.start2), last);
+        de2 = combined(p.start2, p.end2);
+        if (de2 < 0)
+            return set_error(MB3D_HIP_ERR_INVALID, "DE combination: the second formula part has no valid DE option");
+        if (dck == 6) return set_error(MB3D_HIP_ERR_INVALID, "DE combination 'mix': the second hybrid part must be dIFS");
+        int r2 = std::min(std::max(sc->decomb_repeat2, p.start2), p.end2);
+        while (r2 > p.start2 && p.slot_iters[r2] <= 0) --r2;
+        p.repeat2 = r2;
+        p.max_it2 = sc->decomb_iterations2;
+        p.de_option2 = de2;
+        p.is_custom_de2 = de2 == 2 || de2 == 5 || de2 == 6 || de2 == 11 || de2 == 20;
+        p.four_d2 = de2 >= 4 && de2 <= 6;
+    }
     if (ipol) {
         // ipol_de_option (src/calc.rs:255-307): only the formulas whose raw
         // weight is positive (as the sign of its f32 bits) count
@@ -329,7 +365,7 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
                         de_option == 32;
     p.de_option = de_option;
     p.is_custom_de = custom;
-    for (int i = 0; i < p.n_slots; i++) p.slot_ade[i] = custom;
+    for (int i = 0; i < p.n_slots; i++) p.slot_ade[i] = dc && i > end1 ? p.is_custom_de2 : custom;
     p.four_d = de_option >= 4 && de_option <= 6;  // HybridMode::Alt4D / Ipol4D (src/calc.rs:475-484)
     p.ipol = ipol;
     p.ipol_w[0] = ipw[0];
@@ -341,9 +377,15 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
     // DE scale, colour offset and smooth-iteration logs (src/calc.rs:534-600)
     const int max_it = sc->iterations;
     for (float &h : p.fhln) h = (float)(1.0 / std::log(2.0));
-    double cnt_all = 0, scale_sum = 0, abs_first = 0, cnt_ade = 0;
+    // the sums of each part: the second part's slots are those after the
+    // first part's last
+    double sums[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     for (int i = 0; i < p.n_slots; i++) {
         if (p.slot_iters[i] == 0) continue;
+        const bool second = dc && i > end1;
+        double &cnt_all = sums[second][0], &scale_sum = sums[second][1], &abs_first = sums[second][2],
+               &cnt_ade = sums[second][3];
+        const bool custom_i = second ? p.is_custom_de2 != 0 : custom;
         // an interpolation hybrid weighs each formula by its weight and counts it once
         const double its = ipol ? (double)ipw[i] : (double)p.slot_iters[i];
         const double cnt = ipol ? 1.0 : its;
@@ -352,7 +394,7 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
             p.fhln[i] = (float)(1.0 / std::log(std::fmax(2.0, std::fabs(z1))));
         }
         const int j = fi[i].de_option;
-        if (custom && (ipol || !p.slot_uncounted[i]) && (j == 5 || j == 11)) {
+        if (custom_i && (ipol || !p.slot_uncounted[i]) && (j == 5 || j == 11)) {
             cnt_all += cnt;
             cnt_ade += cnt;
             double x2 = fi[i].first_option;
@@ -361,20 +403,32 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
             abs_first += std::fabs(x2) * its;
         } else if (j >= 0 || ipol) {
             cnt_all += cnt;
-            scale_sum += (custom ? 1.0 : fi[i].de_scale) * its;
+            scale_sum += (custom_i ? 1.0 : fi[i].de_scale) * its;
         }
     }
-    double de_scale0 = 1.0;
-    if (cnt_all > 0.0 && de_option != 20) de_scale0 = ipol ? scale_sum : scale_sum / cnt_all;
+    double de_scale0 = 1.0, de_scale2 = 1.0;
+    if (sums[0][0] > 0.0 && de_option != 20) de_scale0 = ipol ? sums[0][1] : sums[0][1] / sums[0][0];
+    if (dc && sums[1][0] > 0.0 && de2 != 20) de_scale2 = sums[1][1] / sums[1][0];
+    // (with a DE combination the colour offset is the second part's)
+    const double *cs = sums[dc ? 1 : 0];
     float col_plus = 0.0f;
-    if (cnt_all > 0.0 && cnt_ade > 0.0) col_plus = (float)(std::pow(std::fabs(abs_first / cnt_ade), 0.25) * 40.0 - 49.0);
+    if (cs[0] > 0.0 && cs[3] > 0.0) col_plus = (float)(std::pow(std::fabs(cs[2] / cs[3]), 0.25) * 40.0 - 49.0);
 
-    // mctColVarDEstopMul (src/calc.rs:600-670) for one hybrid part
+    // mctColVarDEstopMul (src/calc.rs:594-650): per part; a DE combination
+    // takes the part with more counted iterations
     float col_var = 0.6f;
     {
-        double num = 0, den = 0;
+        double nums[2] = {0, 0}, dens[2] = {0, 0};
+        int uncounted[2] = {0, 0};
+        bool part2_counted = false;
         for (int i = 0; i < p.n_slots; i++) {
-            if (!ipol && !(p.slot_iters[i] > 0 && !p.slot_uncounted[i])) continue;
+            const int part = dc && i > end1 ? 1 : 0;
+            if (!ipol && !(p.slot_iters[i] > 0 && !p.slot_uncounted[i])) {
+                if (p.slot_iters[i] > 0) uncounted[part]++;
+                continue;
+            }
+            part2_counted |= part == 1;
+            double &num = nums[part], &den = dens[part];
             const double its = ipol ? (double)ipw[i] : (double)p.slot_iters[i];
             const int de = fi[i].de_option;
             if (de == 0 || de == 4) {
@@ -393,7 +447,16 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
                 }
             }
         }
-        if (den > 1e-3) col_var = (float)(num / den);
+        if (!part2_counted) {
+            if (dens[0] > 1e-3) col_var = (float)(nums[0] / dens[0]);
+        } else {
+            col_var = dens[0] > 0.0 ? (float)(nums[0] / dens[0]) : 0.6f;
+            const double z1 = dens[1], z2 = dens[0];
+            if ((z1 >= z2 && z1 > 1e-3) || (uncounted[1] > uncounted[0] && std::fabs(z1 - z2) < 1e-3 && z1 > 1e-3))
+                col_var = (float)(nums[1] / dens[1]);
+            else if (z2 < 1e-3)
+                col_var = 0.6f;
+        }
     }
 
     // bailout (Scene::effective_rstop, src/scene.rs:799-809)
@@ -440,6 +503,36 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
 
     float de_offset = std::fmin(de_stop * 0.1f, 0.004f);
     p.d_de_scale = custom ? (float)((double)(float)de_scale0 / sw) : (float)de_scale0 * de_offset;
+    if (dc) {
+        p.d_de_scale2 = p.is_custom_de2 ? (float)((double)(float)de_scale2 / sw) : (float)de_scale2 * de_offset;
+        p.decomb_smooth = (double)std::fmax(sc->decomb_smooth, 1e-30f) / sw;
+        // the second part placed as an object of its own (Place, src/calc.rs:748-752)
+        const double *o = sc->part2_offset, *r = sc->part2_rotation;
+        const bool moved = o[0] != 0.0 || o[1] != 0.0 || o[2] != 0.0 || r[0] != 0.0 || r[1] != 0.0 || r[2] != 0.0 ||
+                           sc->part2_scale != 1.0;
+        if (sc->part2_placed && moved) {
+            double a[3], sn[3], cs[3];
+            for (int k = 0; k < 3; k++) {
+                a[k] = r[k] * (kPi / 180.0);
+                sn[k] = std::sin(a[k]);
+                cs[k] = std::cos(a[k]);
+            }
+            // build_rot_matrix (src/math.rs:466-475), rows scaled to 1 / scale
+            const double m[3][3] = {{cs[1] * cs[2], -cs[1] * sn[2], sn[1]},
+                                    {sn[0] * sn[1] * cs[2] + cs[0] * sn[2], cs[0] * cs[2] - sn[0] * sn[1] * sn[2],
+                                     -sn[0] * cs[1]},
+                                    {sn[0] * sn[2] - cs[0] * sn[1] * cs[2], cs[0] * sn[1] * sn[2] + sn[0] * cs[2],
+                                     cs[0] * cs[1]}};
+            const double len = 1.0 / std::fmax(sc->part2_scale, 1e-300);
+            for (int i = 0; i < 3; i++) {
+                const double d = len / std::sqrt(m[i][0] * m[i][0] + m[i][1] * m[i][1] + m[i][2] * m[i][2]);
+                for (int k = 0; k < 3; k++) p.place_m[i][k] = m[i][k] * d;
+                p.place_offset[i] = o[i];
+            }
+            p.place_scale = sc->part2_scale;
+            p.placed = 1;
+        }
+    }
     p.de_offset006 = de_offset * 0.06f;
     p.de_offset = (float)((double)de_offset * sw);
 
@@ -454,8 +547,10 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
     p.ln_rstop = (float)std::log(std::log(rstop));
     p.d_rstop = (double)(float)(rstop * rstop);
     p.rstop3d = (double)(float)((p.d_rstop * p.d_rstop) * 64.0);
-    p.mcts_m = 32767.0f / (float)std::max(max_it + 1, 1);
-    p.d_col_plus = col_plus + (float)max_it * 0.1f;
+    // the iterations of both parts of a DE combination (iMaxitF2)
+    const int its_all = !dc ? max_it : (dck > 5 ? max_it + sc->decomb_iterations2 : std::max(max_it, sc->decomb_iterations2));
+    p.mcts_m = 32767.0f / (float)std::max(its_all + 1, 1);
+    p.d_col_plus = col_plus + (float)its_all * 0.1f;
     p.col_var_de_stop_mul = col_var;
 
     p.sm_normals = std::min(std::max(sc->smooth_normals, 0), 8);

@@ -75,6 +75,14 @@ class Scene(C.Structure):
         ("julia", C.c_int32),
         ("interpolation", C.c_int32),
         ("interpolation_w", C.c_float * 2),
+        ("decomb", C.c_int32),
+        ("decomb_end1", C.c_int32),
+        ("decomb_start2", C.c_int32),
+        ("decomb_end2", C.c_int32),
+        ("decomb_repeat2", C.c_int32),
+        ("decomb_iterations2", C.c_int32),
+        ("decomb_smooth", C.c_float),
+        ("part2_placed", C.c_int32),
         ("reserved", C.c_int32),
         ("rstop", C.c_double),
         ("zoom", C.c_double),
@@ -89,6 +97,9 @@ class Scene(C.Structure):
         ("color_mul", C.c_double),
         ("julia_c", C.c_double * 4),
         ("rot_4d", C.c_double * 3),
+        ("part2_offset", C.c_double * 3),
+        ("part2_rotation", C.c_double * 3),
+        ("part2_scale", C.c_double),
         ("formulas", Formula * MAX_SLOTS),
     ]
 
@@ -124,6 +135,16 @@ class Params(C.Structure):
         ("four_d", C.c_int32),
         ("do_julia", C.c_int32),
         ("ipol", C.c_int32),
+        ("decomb", C.c_int32),
+        ("start2", C.c_int32),
+        ("end2", C.c_int32),
+        ("repeat2", C.c_int32),
+        ("max_it2", C.c_int32),
+        ("de_option2", C.c_int32),
+        ("is_custom_de2", C.c_int32),
+        ("four_d2", C.c_int32),
+        ("placed", C.c_int32),
+        ("reserved", C.c_int32),
         ("slot_z_mul", S6d),
         ("slot_scale", S6d),
         ("slot_min_r", S6d),
@@ -146,6 +167,8 @@ class Params(C.Structure):
         ("mct_color_mul", C.c_float),
         ("fhln", C.c_float * MAX_SLOTS),
         ("ipol_w", C.c_float * 2),
+        ("d_de_scale2", C.c_float),
+        ("reserved2", C.c_float),
         ("d_rstop", C.c_double),
         ("rstop3d", C.c_double),
         ("fov_y", C.c_double),
@@ -158,6 +181,10 @@ class Params(C.Structure):
         ("slot_opt", (C.c_double * FORMULA_OPTIONS) * MAX_SLOTS),
         ("smatrix4", (C.c_double * 4) * 4),
         ("julia_c", C.c_double * 4),
+        ("decomb_smooth", C.c_double),
+        ("place_offset", C.c_double * 3),
+        ("place_m", (C.c_double * 3) * 3),
+        ("place_scale", C.c_double),
     ]
 
 
@@ -358,6 +385,11 @@ def scene_default() -> Scene:
     s.z_step_div = 0.5
     s.raystep_limiter = 1.0
     s.color_mul = 1.0
+    # DeCombParams::new (src/scene.rs:2196-2200), used when a combination is on
+    s.decomb_end1, s.decomb_start2, s.decomb_end2, s.decomb_repeat2 = 0, 1, 5, 1
+    s.decomb_iterations2 = s.iterations
+    s.decomb_smooth = 0.5
+    s.part2_scale = 1.0
     set_formulas(s, [intpow(8, 1)])
     return s
 

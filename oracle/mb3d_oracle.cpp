@@ -337,6 +337,14 @@ struct Marcher {
     double vfov[3] = {0, 0, 1};
     double cafy = 0;
     int max_its_result;
+    // the hybrid parts (HybridPart, src/calc.rs:137-150): the picture's one
+    // part, or the two of a DE combination
+    struct PartO {
+        int start, end, repeat, max_it, de_option;
+        bool custom, four;
+        float dds;
+    };
+    PartO part1{}, part2{}, part{};
     int32_t seed = 0;
     uint64_t de_evals = 0;
 
@@ -359,6 +367,10 @@ struct Marcher {
         for (int k = 0; k < 4; k++) it.j[k] = p.julia_c[k];  // j starts as ju (src/calc.rs:1058)
         ms_de_stop = p.de_stop;
         max_its_result = p.max_it;
+        part1 = {0, p.end_to, p.repeat_from, p.max_it, p.de_option, p.is_custom_de != 0, p.four_d != 0, p.d_de_scale};
+        part2 = {p.start2, p.end2, p.repeat2, p.max_it2, p.de_option2, p.is_custom_de2 != 0, p.four_d2 != 0,
+                 p.d_de_scale2};
+        part = part1;
     }
 
     // Formula::iterate through call_slot (src/iteration.rs:325-345,
@@ -602,11 +614,12 @@ struct Marcher {
     }
 
     // Marcher::calc_de_part for one part without dIFS (src/calc.rs:1495-1559)
-    double calc_de() {
-        de_evals++;
+    // Marcher::calc_de_part without dIFS (src/calc.rs:1495-1559) for the
+    // current part; `invert`: the part as "inside" (kind 3's second part)
+    double calc_de_part(bool invert) {
         double result;
-        if (p.is_custom_de) {
-            result = (it.ipol ? ipol_3d_de() : hybrid_3d_de()) * (double)p.d_de_scale;
+        if (part.custom) {
+            result = (it.ipol ? ipol_3d_de() : hybrid_3d_de()) * (double)part.dds;
         } else {
             mand_function();
             if (it.rout < 1e-200) {
@@ -627,7 +640,7 @@ struct Marcher {
                     g[k] = (buf_rout - it.rout) * (buf_rout - it.rout);
                     it.c[k] = bufd;
                 }
-                result = buf_rout * std::log(buf_rout) * (double)p.d_de_scale /
+                result = buf_rout * std::log(buf_rout) * (double)part.dds /
                          (std::sqrt(g[0] + g[1] + g[2]) + (double)p.de_offset006);
                 it.rout = buf_rout;
                 it.it_result = it.max_it;
@@ -639,6 +652,87 @@ struct Marcher {
         max_its_result = it.max_it;
         double mn = (double)ms_de_stop * 0.25;
         if (result < mn || std::isnan(result)) result = mn;
+        if (invert) {
+            result = (double)ms_de_stop * 4.0 - result * 3.0;
+            if (result >= (double)ms_de_stop) it.it_result -= 1;
+        }
+        return result;
+    }
+
+    // set_part (src/calc.rs:1470-1490)
+    void set_part(const PartO &pt) {
+        part = pt;
+        it.start_from = pt.start;
+        it.end_to = pt.end;
+        it.repeat_from = pt.repeat;
+        it.max_it = pt.max_it;
+        it.de_option = pt.de_option;
+        it.four_d = pt.four;
+    }
+
+    // calc_de_fractal (src/calc.rs:1352-1456) without the mix (kind 6) and
+    // inside rendering
+    double calc_de() {
+        de_evals++;
+        if (!p.decomb) return calc_de_part(false);
+        double result = calc_de_part(false);
+        const int b_it = it.it_result, b_max = max_its_result;
+        const double b_otrap = it.otrap;
+        const float b_sit = it.smooth_it;
+        set_part(part2);
+        double c0[3] = {it.c[0], it.c[1], it.c[2]};
+        if (p.placed) {  // Place::apply: rotate_vector(p - offset, m)
+            double d[3] = {c0[0] - p.place_offset[0], c0[1] - p.place_offset[1], c0[2] - p.place_offset[2]};
+            for (int k = 0; k < 3; k++) it.c[k] = dot3(p.place_m[k], d);
+        }
+        double rst = calc_de_part(p.decomb == 3);
+        if (p.placed) {
+            rst *= p.place_scale;
+            for (int k = 0; k < 3; k++) it.c[k] = c0[k];
+        }
+        const double ms = (double)ms_de_stop;
+        bool restore1 = false;
+        switch (p.decomb) {
+            case 1:
+                if (rst >= result)
+                    restore1 = true;
+                else
+                    result = rst;
+                break;
+            case 2:
+            case 3:
+                if (rst > result)
+                    result = rst;
+                else
+                    restore1 = true;
+                break;
+            default: {  // 4, 5: the smooth minima
+                if (!(rst < result)) {
+                    it.it_result = b_it;
+                    max_its_result = b_max;
+                }
+                const double sm = p.decomb_smooth;
+                if (p.decomb == 4) {
+                    result = std::fmin(rst - std::fmax(sm - result, 0.0), result - std::fmax(sm - rst, 0.0));
+                } else {
+                    const double rd1 = std::fmax(sm - result - ms, 0.0);
+                    const double d2 = std::fmax(sm - rst - ms, 0.0);
+                    result = std::fmin(rst - rd1 * (sm - d2) / sm, result - d2 * (sm - rd1) / sm);
+                }
+                const double r2 = std::fabs(rst), w = std::fabs(result);
+                const double f = 1.0 / (w + r2 + 1e-10);
+                it.smooth_it = (float)(((double)it.smooth_it * w + (double)b_sit * r2) * f);
+                it.otrap = (it.otrap * w + b_otrap * r2) * f;
+                break;
+            }
+        }
+        if (restore1) {
+            it.it_result = b_it;
+            it.smooth_it = b_sit;
+            it.otrap = b_otrap;
+            max_its_result = b_max;
+        }
+        set_part(part1);
         return result;
     }
 
@@ -1436,29 +1530,50 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
         p.slot_fold[i] = f.fold;
         normalized_options(f, p.slot_opt[i], p.slot_power[i]);
     }
-    int end_to = last;
+    // the DE combination's two parts (src/calc.rs:421-431): only when both
+    // have slots with iterations
+    const int dck = ipol ? 0 : sc->decomb;
+    const bool dc = dck != 0 && sc->decomb_end1 < last && sc->decomb_start2 <= last;
+    if (dc && (dck < 1 || dck > 6)) return fail("unknown DE combination");
+    int end_to = dc ? std::min(sc->decomb_end1, last) : last;
     int repeat_from = std::min(sc->repeat_from, end_to);
     while (repeat_from > 0 && p.slot_iters[repeat_from] <= 0) repeat_from--;
     // combine_de_options (src/calc.rs:318-372)
-    int de_option = -1;
-    for (int i = 0; i <= end_to; i++) {
-        if (p.slot_iters[i] == 0) continue;
-        int de = formula_de_option(sc->formulas[i]);
-        if (de < 0 || de == de_option) continue;
-        if (de_option == -1 || de_option == 21 || de_option == 22)
-            de_option = de;
-        else if (de_option == 2)
-            de_option = (de == 2 || de == 5 || de == 6 || de == 11) ? 2 : 0;
-        else if (de_option == 4)
-            de_option = (de == 5 || de == 6) ? 4 : 0;
-        else if (de_option == 5)
-            de_option = de == 4 ? 4 : ((de == 2 || de == 11) ? 2 : (de != 6 ? 0 : 5));
-        else if (de_option == 6)
-            de_option = (de == 2 || de == 4 || de == 5 || de == 11) ? de : 0;
-        else if (de_option == 11)
-            de_option = (de == 2 || de == 5) ? 2 : (de != 6 ? 0 : 11);
-        // (other options keep theirs)
+    auto combine = [&](int start, int end) {
+        int de_option = -1;
+        for (int i = start; i <= end; i++) {
+            if (p.slot_iters[i] == 0) continue;
+            int de = formula_de_option(sc->formulas[i]);
+            if (de < 0 || de == de_option) continue;
+            if (de_option == -1 || de_option == 21 || de_option == 22)
+                de_option = de;
+            else if (de_option == 2)
+                de_option = (de == 2 || de == 5 || de == 6 || de == 11) ? 2 : 0;
+            else if (de_option == 4)
+                de_option = (de == 5 || de == 6) ? 4 : 0;
+            else if (de_option == 5)
+                de_option = de == 4 ? 4 : ((de == 2 || de == 11) ? 2 : (de != 6 ? 0 : 5));
+            else if (de_option == 6)
+                de_option = (de == 2 || de == 4 || de == 5 || de == 11) ? de : 0;
+            else if (de_option == 11)
+                de_option = (de == 2 || de == 5) ? 2 : (de != 6 ? 0 : 11);
+            // (other options keep theirs)
+        }
+        return de_option;
+    };
+    int de_option = combine(0, end_to);
+    // the second part (src/calc.rs:487-520)
+    int start2 = 0, end2 = 0, rep2 = 0, de2 = -1;
+    if (dc) {
+        start2 = std::min(sc->decomb_start2, last);
+        end2 = std::min(std::max(sc->decomb_end2, start2), last);
+        de2 = combine(start2, end2);
+        if (de2 < 0) return fail("DE combination: the second formula part has no valid DE option");
+        if (dck == 6 && de2 != 20) return fail("DE combination 'mix': the second hybrid part must be dIFS");
+        rep2 = std::min(std::max(sc->decomb_repeat2, start2), end2);
+        while (rep2 > start2 && p.slot_iters[rep2] <= 0) rep2--;
     }
+    const bool is_custom_de2 = de2 == 2 || de2 == 5 || de2 == 6 || de2 == 11 || de2 == 20;
     if (ipol) {
         // ipol_de_option (src/calc.rs:255-307): the formulas whose raw weight
         // is positive (as the bits of the f32)
@@ -1508,8 +1623,18 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
     p.do_julia = sc->julia != 0;
     for (int k = 0; k < 4; k++) p.julia_c[k] = sc->julia_c[k];
     build_smatrix4(sc->rot_4d[0], sc->rot_4d[1], sc->rot_4d[2], p.smatrix4);
+    if (dc) {
+        p.decomb = dck;
+        p.start2 = start2;
+        p.end2 = end2;
+        p.repeat2 = rep2;
+        p.max_it2 = sc->decomb_iterations2;
+        p.de_option2 = de2;
+        p.is_custom_de2 = is_custom_de2;
+        p.four_d2 = de2 >= 4 && de2 <= 6;
+    }
     for (int i = 0; i < ns; i++) {
-        p.slot_ade[i] = is_custom_de;
+        p.slot_ade[i] = dc && i > end_to ? is_custom_de2 : is_custom_de;
         int de = formula_de_option(sc->formulas[i]);
         p.slot_uncounted[i] = p.slot_uncounted[i] || de < 0 || de > 20;
     }
@@ -1518,9 +1643,18 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
     float fhln_default = (float)(1.0 / std::log(2.0));
     for (int k = 0; k < 6; k++) p.fhln[k] = fhln_default;
     double n = 0, x1 = 0, y1 = 0, ii = 0;
+    bool have_part1 = false;
+    double part1_scale = 1.0;  // the first part's, once the second part's slots begin
     for (int x = 0; x < ns; x++) {
         if (p.slot_iters[x] == 0) continue;
         const mb3d_hip_formula &f = sc->formulas[x];
+        const bool second = dc && x > end_to;
+        if (second && !have_part1) {
+            have_part1 = true;
+            part1_scale = n > 0.0 && de_option != 20 ? x1 / n : 1.0;
+            n = x1 = y1 = ii = 0.0;
+        }
+        const bool custom = second ? is_custom_de2 : is_custom_de;
         if (ipol || !p.slot_uncounted[x]) {
             double sip = formula_si_pow(f);
             double z1 = sip > 1.0 ? sip : formula_first_option(f);
@@ -1529,7 +1663,7 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
         // interpolation hybrid: weighted by the weights, counted once
         double z1 = ipol ? (double)ipw[x] : (double)p.slot_iters[x], cnt = ipol ? 1.0 : (double)p.slot_iters[x];
         int j = formula_de_option(f);
-        if (is_custom_de && !p.slot_uncounted[x] && (j == 5 || j == 11)) {
+        if (custom && (ipol || !p.slot_uncounted[x]) && (j == 5 || j == 11)) {
             n += cnt;
             ii += cnt;
             double x2 = formula_first_option(f);
@@ -1539,20 +1673,27 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
             else
                 x1 += (x2 * 1.2 + 1.0) / x2 * z1;
             y1 += std::fabs(x2) * z1;
-        } else if (j >= 0) {
+        } else if (j >= 0 || ipol) {
             n += cnt;
-            x1 += (is_custom_de ? 1.0 : formula_de_scale(f)) * z1;  // ade_scale is 1 for built-ins
+            x1 += (custom ? 1.0 : formula_de_scale(f)) * z1;  // ade_scale is 1 for built-ins
         }
     }
     float d_col_plus = 0.0f;
-    double d_de_scale = 1.0;
-    if (n > 0.0 && de_option != 20) d_de_scale = ipol ? x1 : x1 / n;
+    double d_de_scale = 1.0, d_de_scale2 = 1.0;
+    if (have_part1) {
+        d_de_scale = part1_scale;
+        if (n > 0.0 && de2 != 20) d_de_scale2 = x1 / n;
+    } else if (n > 0.0 && de_option != 20) {
+        d_de_scale = ipol ? x1 : x1 / n;
+    }
     if (n > 0.0 && ii > 0.0) d_col_plus = (float)(std::pow(std::fabs(y1 / ii), 0.25) * 40.0 - 49.0);
-    // mctColVarDEstopMul (src/calc.rs:610-670), one part
+    // mctColVarDEstopMul (src/calc.rs:594-650); with a DE combination the
+    // value of the part with more counted iterations
     float col_var = 0.6f;
     {
         double yy1 = 0, zz1 = 0, zz2 = 0;
         int ia = 0, itmp = 0;
+        bool second = false;
         for (int x = 0; x < ns; x++) {
             bool counted = ipol || (p.slot_iters[x] > 0 && !p.slot_uncounted[x]);
             if (!counted) {
@@ -1563,6 +1704,14 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
                         itmp++;
                 }
                 continue;
+            }
+            if (x > end_to && !second) {
+                second = true;
+                col_var = zz1 > 0.0 ? (float)(yy1 / zz1) : 0.6f;
+                zz2 = zz1;
+                zz1 = 0.0;
+                yy1 = 0.0;
+                if (!dc || de2 == 20) break;
             }
             const mb3d_hip_formula &f = sc->formulas[x];
             double x2 = ipol ? (double)ipw[x] : (double)p.slot_iters[x];
@@ -1633,6 +1782,29 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
         dds = (float)((double)(float)d_de_scale / step_width);
     else
         dds = (float)d_de_scale * de_offset;
+    if (dc) {
+        p.d_de_scale2 = is_custom_de2 ? (float)((double)(float)d_de_scale2 / step_width) : (float)d_de_scale2 * de_offset;
+        p.decomb_smooth = (double)std::fmax(sc->decomb_smooth, 1e-30f) / step_width;
+        // the second part as an object of its own (src/calc.rs:748-752)
+        const bool moved = sc->part2_offset[0] != 0.0 || sc->part2_offset[1] != 0.0 || sc->part2_offset[2] != 0.0 ||
+                           sc->part2_rotation[0] != 0.0 || sc->part2_rotation[1] != 0.0 ||
+                           sc->part2_rotation[2] != 0.0 || sc->part2_scale != 1.0;
+        if (sc->part2_placed && moved && dck != 6) {
+            const double d2r = 3.141592653589793 / 180.0;  // f64::to_radians
+            const double xa = sc->part2_rotation[0] * d2r, ya = sc->part2_rotation[1] * d2r,
+                         za = sc->part2_rotation[2] * d2r;
+            // build_rot_matrix (src/math.rs:466-475)
+            const double sx = std::sin(xa), cx = std::cos(xa), sy = std::sin(ya), cy = std::cos(ya);
+            const double sz = std::sin(za), cz = std::cos(za);
+            const double rm[3][3] = {{cy * cz, -cy * sz, sy},
+                                     {sx * sy * cz + cx * sz, cx * cz - sx * sy * sz, -sx * cy},
+                                     {sx * sz - cx * sy * cz, cx * sy * sz + sx * cz, cx * cy}};
+            normalise_matrix_to(1.0 / std::fmax(sc->part2_scale, 1e-300), rm, p.place_m);
+            for (int k = 0; k < 3; k++) p.place_offset[k] = sc->part2_offset[k];
+            p.place_scale = sc->part2_scale;
+            p.placed = 1;
+        }
+    }
     float de_offset006 = de_offset * 0.06f;
     de_offset = (float)((double)de_offset * step_width);
     float z_step_div = z_step_div0, ms_de_sub = 0.0f;
@@ -1645,8 +1817,11 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
     float ln_rstop = (float)std::log(std::log(rstop));
     double d_rstop = (double)(float)(rstop * rstop);
     double rstop3d = (double)(float)((d_rstop * d_rstop) * 64.0);
-    float mcts_m = 32767.0f / (float)std::max(max_it + 1, 1);
-    d_col_plus = d_col_plus + (float)max_it * 0.1f;
+    // the iterations of both parts of a DE combination (iMaxitF2)
+    int its_all = max_it;
+    if (dc) its_all = dck > 5 ? max_it + sc->decomb_iterations2 : std::max(max_it, sc->decomb_iterations2);
+    float mcts_m = 32767.0f / (float)std::max(its_all + 1, 1);
+    d_col_plus = d_col_plus + (float)its_all * 0.1f;
 
     p.d_de_scale = dds;
     p.sm_normals = std::min(std::max(sc->smooth_normals, 0), 8);

@@ -35,9 +35,12 @@ def compare(gpu: dict, ref: dict, w: int, h: int, step_width: float, scale: floa
     rel = np.abs(g64[both] - ref["depth"][both]) * step_width / scale
     ng = np.asarray(gpu["normals"], np.float64)[both]
     no = np.asarray(ref["normals"], np.float64)[both]
-    cosang = np.clip(np.sum(ng * no, axis=1) / np.maximum(
-        np.linalg.norm(ng, axis=1) * np.linalg.norm(no, axis=1), 1e-300), -1.0, 1.0)
+    lg, lo = np.linalg.norm(ng, axis=1), np.linalg.norm(no, axis=1)
+    cosang = np.clip(np.sum(ng * no, axis=1) / np.maximum(lg * lo, 1e-300), -1.0, 1.0)
     ang = np.degrees(np.arccos(cosang))
+    # a flat DE (a zero gradient, e.g. inside both parts of a smooth DE
+    # combination) gives the zero normal on both sides: that agrees
+    ang[(lg == 0) & (lo == 0)] = 0.0
     inner = both & ~sil
     it_eq = (gpu["iters"][inner] == ref["iters"][inner])
     q = lambda a, p: float(np.percentile(a, p)) if a.size else 0.0

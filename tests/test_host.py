@@ -231,3 +231,35 @@ def test_interpolation_params_match_the_oracle(case):
     a = H.params_from_scene(sc)
     b = O.params(sc)
     assert a.ipol and bytes(a) == bytes(b)
+
+
+def _decomb_scene(kind, place=False, f2=None, its2=20):
+    s = H.scene_preset("Integer Power", 320, 240)
+    s.z_start, s.zoom = -4.0, 0.45
+    H.set_formulas(s, [H.formula("Integer Power"), H.formula("Real Power", 2), f2 or H.formula("Amazing Box", scale=-1.5),
+                       H.formula("Tricorn", -1)])
+    s.decomb, s.decomb_end1, s.decomb_start2, s.decomb_end2, s.decomb_repeat2 = kind, 1, 2, 3, 3
+    s.decomb_iterations2, s.decomb_smooth = its2, 0.3
+    if place:
+        s.part2_placed = 1
+        s.part2_offset[:], s.part2_rotation[:], s.part2_scale = (1.6, 0.3, 0.0), (10.0, 20.0, 30.0), 0.35
+    return s
+
+
+@pytest.mark.parametrize("case", ["min", "max-placed", "inverted", "smooth-linear", "smooth", "4d-second",
+                                  "uncounted-second"])
+def test_decomb_params_match_the_oracle(case):
+    sc = {
+        "min": lambda: _decomb_scene(1),
+        "max-placed": lambda: _decomb_scene(2, place=True),
+        "inverted": lambda: _decomb_scene(3),
+        "smooth-linear": lambda: _decomb_scene(4),
+        "smooth": lambda: _decomb_scene(5, place=True, its2=40),
+        "4d-second": lambda: _decomb_scene(1, f2=H.formula("Quaternion", 3)),
+        "uncounted-second": lambda: _decomb_scene(1, f2=H.formula("Bulbox", -2)),
+    }[case]()
+    a = H.params_from_scene(sc)
+    b = O.params(sc)
+    assert a.decomb and bytes(a) == bytes(b)
+    buf = C.create_string_buffer(200)
+    assert H.load().mb3d_hip_supported(C.byref(a), buf, 200) == H.OK, buf.value
