@@ -43,7 +43,11 @@ struct Cand {
     float rsf, step_count;
     int32_t pix, flags, nsteps, seed;
 };
-enum : int32_t { CF_START = 1, CF_FIRST = 2, CF_STEPPED = 4 };
+// (a record of the finish's refined surface points reuses the slot: c, mzz,
+// step_count, pix; rsf = the DE stop then (bin_search_it moves the point
+// without updating it); nsteps = the iteration count there; CF_DELIM = the
+// stop was the DE's, not the iteration limit's; the frame in flags >> 8)
+enum : int32_t { CF_START = 1, CF_FIRST = 2, CF_STEPPED = 4, CF_DELIM = 8 };
 
 // counter words of Bufs::ctr
 enum : int {
@@ -53,6 +57,7 @@ enum : int {
     C_FINQ = 3,    // next item of the f64 finish
     C_POSTQ = 4,   // next item of the shadow pass
     C_AOQ = 5,     // next item of the DE AO pass
+    C_SURF = 6,    // refined surface list length (the normals pass's items)
     C_STATS = 8,   // 12 work counters from here
     C_WORDS = 24
 };
@@ -65,6 +70,7 @@ struct Bufs {
     uint32_t *gbuf;
     Cand *cand;
     int32_t *hits;
+    int32_t *surf;  // hand-off slots of the refined surface points (-1: a chunk's empty tail)
     unsigned long long *ctr;
     // the warm start: the network's proposal per work item of the queue
     // (hit distance / 4, hit logit)
@@ -621,12 +627,13 @@ __device__ void write_inside(const DevP &P, const Bufs &B, int pix, const Ray64 
     }
 }
 
-enum FinPhase : int { FP_START, FP_RESUME, FP_BACK, FP_STEP, FP_REF, FP_NC, FP_NP, FP_NM };
+enum FinPhase : int { FP_START, FP_RESUME, FP_BACK, FP_STEP, FP_REF };
 
-// N32: the six offset DEs of the normal stencil in FP32, for the analytic
-// DEs (their FP32 error moves a normal by ~0.01 degree; the numeric
-// gradient's does not allow it)
-template <int KIND, int DEM, bool N32, bool BATCH>
+// The finish ends at the refined surface point: its record goes to the
+// surface list, and k_normals computes the normal, colour and G-buffer
+// words.  So that every phase here makes the same f64 DE: one DE site per
+// turn.
+template <int KIND, int DEM, bool BATCH>
 __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixels, const DevP *__restrict__ gtab,
                                                   int nframes) {
     const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
@@ -637,18 +644,16 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
     Ray64 R;
     Refine F;
     int pix = 0, phase = FP_START;
+    long long cslot = 0;  // the ray's hand-off slot (its pixel with all_pixels)
     double last_step = 0, last_de = 0, dtmp = 0, back_dt1 = 0;
     float rsf = 1.0f, scount = 0.0f;
     int nsteps = 0;
     uint32_t guard = 0;
     bool first = false, stepped = false, de_limited = false;
     int32_t seed = 0;
-    int iters = 0, q = 0, axis = 0;
-    double ct[3] = {0, 0, 0}, nv[3] = {0, 0, 0}, noffset = 0, na = 0;
-    float nn = 0.0f;
-    WaveChunk fq, hq;
+    WaveChunk fq, sq;
     chunk_init(fq);
-    chunk_init(hq);
+    chunk_init(sq);
     for (;;) {
         bool need = !have && !exhausted;
         for (int tries = 0; tries < 4 && __any(need); tries++) {
@@ -662,6 +667,7 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
             if (!all_pixels && B.cand[slot].pix < 0) continue;  // an empty slot of a wave's chunk
             have = true;
             need = false;
+            cslot = slot;
             if (!all_pixels && BATCH) fr = B.cand[slot].flags >> 8;
             const DevP &F = BATCH ? tab[fr] : P0;
             R.reset(F);
@@ -703,16 +709,9 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
         }
         if (__all(exhausted)) break;
         const DevP &P = BATCH ? tab[fr] : P0;
-        bool hit = false, done = false;
+        bool surf = false, done = false;
         if (!have) goto end;
-        if (N32 && (phase == FP_NP || phase == FP_NM)) {
-            int it32 = 0;
-            float ot = 0.0f;
-            R.de = (double)de32_analytic<DEM>(P, R.c, R.ms, it32, cnt, &ot);
-            R.otrap = (double)ot;
-        } else {
-            R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
-        }
+        R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
         switch (phase) {
             case FP_START:
                 if (R.it >= R.max_its || R.de < (double)R.ms) {
@@ -737,24 +736,9 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
             case FP_STEP:
                 dtmp = R.de;
                 goto clamp;
-            case FP_REF:
+            default:  // FP_REF
                 if (F.after(R, P)) goto end;
                 goto refined;
-            case FP_NC:
-                nn = R.smooth_it;
-                R.calc_sit = false;
-                q = 0;
-                goto nplus;
-            case FP_NP:
-                na = R.de;
-                for (int k = 0; k < 3; k++) R.c[k] += P.vg[axis][k] * (-2.0 * noffset);
-                phase = FP_NM;
-                goto end;
-            default:  // FP_NM
-                nv[axis] = (na - R.de) * 0.5;
-                if (++q < 3) goto nplus;
-                for (int k = 0; k < 3; k++) R.c[k] = ct[k];
-                goto write;
         }
     clamp:
         // after a step of march_from (src/calc.rs:2626-2638)
@@ -826,31 +810,93 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
                 goto end;
             }
         }
-    refined:
-        // calculate_normals, smooth_normals 0 (src/calc.rs:1813-1858)
-        iters = max(R.it, 0);
-        R.calc_sit = true;
-        noffset = (double)fminf(P.de_stop, 1.0f) * (1.0 + R.mzz * (double)P.de_stop_factor) * 0.15;
-        for (int k = 0; k < 3; k++) ct[k] = R.c[k];
-        phase = FP_NC;
-        goto end;
-    nplus:
-        axis = q == 0 ? 2 : q - 1;  // the order vg2, vg0, vg1
-        for (int k = 0; k < 3; k++) R.c[k] = ct[k] + P.vg[axis][k] * noffset;
-        phase = FP_NP;
-        goto end;
-    write:
-        write_surface(P, B, pix, R, nv, nn, de_limited, (uint16_t)iters, scount);
-        cnt.hits++;
-        hit = true;
+    refined: {
+        // the refined surface point (surface_found, src/calc.rs:2471-2481):
+        // its record for k_normals, in the ray's hand-off slot
+        Cand &o = B.cand[cslot];
+        for (int k = 0; k < 3; k++) o.c[k] = R.c[k];
+        o.mzz = R.mzz;
+        o.rsf = R.ms;
+        o.step_count = scount;
+        o.pix = pix;
+        o.nsteps = max(R.it, 0);
+        o.flags = (de_limited ? CF_DELIM : 0) | (fr << 8);
+        surf = true;
         done = true;
+    }
     end:
         if (done) have = false;
-        const long long hs = wave_take(hit, &B.ctr[C_HITS], hq, 64, -1);
-        if (hit) {
-            B.hits[hs] = pix;
-            hit = false;
+        const long long ss = wave_take(surf, &B.ctr[C_SURF], sq, 64, -1);
+        if (surf) B.surf[ss] = (int32_t)cslot;
+    }
+    if (lane_id() < sq.left) B.surf[sq.base + lane_id()] = -1;
+    flush_counters(cnt, B.ctr);
+}
+
+// The normals pass over the refined surface points: calculate_normals with
+// smooth_normals 0 (src/calc.rs:1813-1858) -- the centre DE with the smooth
+// iterations, then the DEs at +-noffset along vg2, vg0, vg1 -- and the
+// surface's G-buffer words (write_surface).  Every lane makes the same seven
+// DEs, so the waves stay converged.  N32: the six offset DEs in FP32, for
+// the analytic DEs (their FP32 error moves a normal by ~0.01 degree; the
+// numeric gradient's does not allow it).
+template <int KIND, int DEM, bool N32, bool BATCH>
+__global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__restrict__ gtab, int nframes) {
+    const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
+    Cnt cnt;
+    WaveChunk hq;
+    chunk_init(hq);
+    const long long n = (long long)B.ctr[C_SURF];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    // whole waves step together, so that the hit list's appends are uniform
+    for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+        const long long i = base + lane_id();
+        const int slot = i < n ? B.surf[i] : -1;
+        bool hit = false;
+        int pix = 0;
+        if (slot >= 0) {
+            const Cand &o = B.cand[slot];
+            const int fr = BATCH ? o.flags >> 8 : 0;
+            const DevP &P = BATCH ? tab[fr] : P0;
+            Ray64 R;
+            R.reset(P);
+            double ct[3], nv[3];
+            for (int k = 0; k < 3; k++) ct[k] = R.c[k] = o.c[k];
+            R.mzz = o.mzz;
+            R.ms = o.rsf;
+            R.calc_sit = true;
+            const double noffset = (double)fminf(P.de_stop, 1.0f) * (1.0 + R.mzz * (double)P.de_stop_factor) * 0.15;
+            R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+            const float nn = R.smooth_it;
+            R.calc_sit = false;
+            for (int q = 0; q < 3; q++) {
+                const int axis = q == 0 ? 2 : q - 1;  // the order vg2, vg0, vg1
+                double de[2];
+                for (int side = 0; side < 2; side++) {
+                    for (int k = 0; k < 3; k++) R.c[k] = ct[k] + P.vg[axis][k] * noffset;
+                    if (side) {
+                        for (int k = 0; k < 3; k++) R.c[k] += P.vg[axis][k] * (-2.0 * noffset);
+                    }
+                    if (N32) {
+                        int it32 = 0;
+                        float ot = 0.0f;
+                        R.de = (double)de32_analytic<DEM>(P, R.c, R.ms, it32, cnt, &ot);
+                        R.otrap = (double)ot;
+                    } else {
+                        R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+                    }
+                    de[side] = R.de;
+                }
+                nv[axis] = (de[0] - de[1]) * 0.5;
+            }
+            for (int k = 0; k < 3; k++) R.c[k] = ct[k];
+            write_surface(P, B, o.pix, R, nv, nn, (o.flags & CF_DELIM) != 0, (uint16_t)o.nsteps, o.step_count);
+            cnt.hits++;
+            hit = true;
+            pix = o.pix;
         }
+        const long long hs = wave_take(hit, &B.ctr[C_HITS], hq, 64, -1);
+        if (hit) B.hits[hs] = pix;
     }
     if (lane_id() < hq.left) B.hits[hq.base + lane_id()] = -1;
     flush_counters(cnt, B.ctr);
@@ -1267,6 +1313,14 @@ size_t learn_bufs(void *base, long long items, Bufs *b) {
 }
 
 // the surface pixels of a G-buffer marched elsewhere
+// The lists the kernels append to in chunks of 64 (hand-offs, refined
+// surface points, hits) hold the items and up to 63 empty slots per wave
+// that appended: the capacity for `items` with the largest grid used (the
+// march's waves_per_cu, the f64 finish's 32 waves per CU).
+size_t list_capacity(long long items, int cus, int wpc) {
+    return (size_t)items + 64 * (size_t)cus * (size_t)std::max(wpc, 32);
+}
+
 __global__ void k_collect_hits(DevP P, Bufs B) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < P.npix; i += stride) {
@@ -1577,12 +1631,26 @@ void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b, const M
     else
         hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, false>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
 }
-template <int KIND, int DEM, bool N32, bool BATCH = false>
+template <int KIND, int DEM, bool BATCH = false>
 void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all, const DevP *tab = nullptr,
                    int n = 1) {
     const size_t lds = BATCH ? (size_t)n * DEVP_CORE : 0;
-    if constexpr (KIND == K_GEN && (N32 || BATCH)) return;  // FP64 only, never batched
-    else hipLaunchKernelGGL((k_finish64<KIND, DEM, N32, BATCH>), dim3(grid), dim3(256), lds, s, d, b, all, tab, n);
+    if constexpr (KIND == K_GEN && BATCH) return;  // FP64 only, never batched
+    else hipLaunchKernelGGL((k_finish64<KIND, DEM, BATCH>), dim3(grid), dim3(256), lds, s, d, b, all, tab, n);
+}
+// the normals pass after the finish (N32: FP32 offset DEs, analytic DEs only)
+template <int KIND, int DEM, bool BATCH = false>
+void launch_normals(int grid, hipStream_t s, const DevP &d, const Bufs &b, bool f32, const DevP *tab = nullptr,
+                    int n = 1) {
+    const size_t lds = BATCH ? (size_t)n * DEVP_CORE : 0;
+    if constexpr (KIND == K_GEN && BATCH) return;
+    else if constexpr (KIND != K_GEN && DEM != DEM_NUMERIC) {
+        if (f32) {
+            hipLaunchKernelGGL((k_normals<KIND, DEM, true, BATCH>), dim3(grid), dim3(256), lds, s, d, b, tab, n);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_normals<KIND, DEM, false, BATCH>), dim3(grid), dim3(256), lds, s, d, b, tab, n);
 }
 template <int KIND, int DEM, bool F32>
 void launch_shadow(int grid, hipStream_t s, const DevP &d, const Bufs &b, const ShadowJob &j) {
@@ -1682,7 +1750,9 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     MlpDev M;
     mlp_dev(o.mlp, o, M);
     if (f64 || post_only) M.enabled = 0;
-    const size_t base_need = (npix * sizeof(Cand) + npix * sizeof(int32_t) + 511) & ~(size_t)255;
+    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
+    const size_t cap = list_capacity((long long)npix, c.cus, wpc);
+    const size_t base_need = (cap * sizeof(Cand) + 2 * cap * sizeof(int32_t) + 511) & ~(size_t)255;
     const size_t work_need = base_need + (M.enabled ? learn_bufs(nullptr, d.nslots_q, nullptr) : 0);
     if (int e = ensure(&c.d_work, &c.work_bytes, work_need)) return e;
     Bufs b;
@@ -1708,7 +1778,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
         b.gbuf = (out && out->gbuf) || want_shadow || want_ao ? (uint32_t *)(ob + L.gbuf) : nullptr;
     }
     b.cand = (Cand *)c.d_work;
-    b.hits = (int32_t *)((char *)c.d_work + npix * sizeof(Cand));
+    b.hits = (int32_t *)((char *)c.d_work + cap * sizeof(Cand));
+    b.surf = b.hits + cap;
     b.ctr = c.d_ctr;
     if (M.enabled) learn_bufs((char *)c.d_work + base_need, d.nslots_q, &b);
 
@@ -1717,14 +1788,14 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
         HIP_TRY(hipMemcpyAsync(b.gbuf, post_gbuf, npix * 20, hipMemcpyHostToDevice, c.stream));
     const int kind = kind_of(p), dem = dem_of(p);
     hipStream_t stream = c.stream;
-    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
     const int grid_march = std::max(1, c.cus * wpc / 4);
     int grid_flat = std::max(1, c.cus * 4);
     HIP_TRY(hipEventRecord(c.ev[0], c.stream));
     if (!post_only) {
         if (f64) {
             dispatch(kind, dem, [&](auto K, auto D) {
-                launch_finish<decltype(K)::value, decltype(D)::value, false>(grid_flat * 2, stream, d, b, 1);
+                launch_finish<decltype(K)::value, decltype(D)::value>(grid_flat * 2, stream, d, b, 1);
+                launch_normals<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, false);
             });
         } else {
             if (M.enabled) launch_propose<false>(stream, d, nullptr, 1, M, b.prop);
@@ -1733,7 +1804,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
             });
             HIP_TRY(hipEventRecord(c.ev[3], c.stream));
             dispatch(kind, dem, [&](auto K, auto D) {
-                launch_finish<decltype(K)::value, decltype(D)::value, decltype(D)::value != DEM_NUMERIC>(grid_flat, stream, d, b, 0);
+                launch_finish<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, 0);
+                launch_normals<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, true);
             });
         }
         HIP_TRY(hipGetLastError());
@@ -1868,8 +1940,8 @@ int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
     });
     if (split) HIP_TRY(hipEventRecord(pl.e1, stream));
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
-        launch_finish<decltype(K)::value, decltype(D)::value, decltype(D)::value != DEM_NUMERIC, true>(
-            pl.grid_flat, stream, pl.P0, pl.b, 0, tab, n);
+        launch_finish<decltype(K)::value, decltype(D)::value, true>(pl.grid_flat, stream, pl.P0, pl.b, 0, tab, n);
+        launch_normals<decltype(K)::value, decltype(D)::value, true>(pl.grid_flat, stream, pl.P0, pl.b, true, tab, n);
     });
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1917,11 +1989,12 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     }
     pl->npix = pix;
     mlp_dev(o.mlp, o, pl->M);
-    const size_t base_need = ((size_t)pix * (sizeof(Cand) + sizeof(int32_t)) + 4096 + 255) & ~(size_t)255;
+    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
+    const size_t cap = list_capacity(pix, c.cus, wpc);
+    const size_t base_need = (cap * (sizeof(Cand) + 2 * sizeof(int32_t)) + 4096 + 255) & ~(size_t)255;
     pl->P0 = pl->tab[0];
     pl->P0.nslots_q = q;
     pl->P0.npix = pix;
-    const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
     pl->grid_march = std::max(1, c.cus * wpc / 4);
     pl->grid_flat = std::max(1, c.cus * 4);
     auto fail_free = [&](int e) {
@@ -1944,7 +2017,8 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     pl->b.normals = out->normals;
     pl->b.gbuf = out->gbuf;
     pl->b.cand = (Cand *)pl->d_work;
-    pl->b.hits = (int32_t *)((char *)pl->d_work + (size_t)pix * sizeof(Cand));
+    pl->b.hits = (int32_t *)((char *)pl->d_work + cap * sizeof(Cand));
+    pl->b.surf = pl->b.hits + cap;
     if (pl->M.enabled) learn_bufs((char *)pl->d_work + base_need, q, &pl->b);
     pl->b.ctr = pl->d_ctr;
     if (graph) {
