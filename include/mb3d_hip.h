@@ -258,6 +258,7 @@ typedef struct mb3d_hip_counters {
     uint64_t its_p8_f32, its_box_f32;  /* formula iterations by kind and precision: Integer Power, */
     uint64_t its_p8_f64, its_box_f64;  /* Amazing Box, */
     uint64_t its_other_f64;            /* the other formulas (FP64 only) */
+    uint64_t its_normals_f32;          /* of its_p8_f32 + its_box_f32: the normals pass's FP32 stencil */
     uint64_t overflow;       /* pixels whose FP32 values overflowed */
     uint64_t jumps;          /* warm starts taken */
     uint64_t rollbacks;      /* warm starts rejected by the exact DE (overshoot) */

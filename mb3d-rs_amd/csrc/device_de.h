@@ -90,6 +90,7 @@ struct DevP {
 struct Cnt {
     uint32_t steps = 0, de32 = 0, de64 = 0;
     uint32_t p8_32 = 0, box_32 = 0, p8_64 = 0, box_64 = 0, oth_64 = 0;
+    uint32_t nrm_32 = 0;  // of p8_32 and box_32: the normals pass's FP32 stencil
     uint32_t hits = 0, overflow = 0;
     uint32_t jumps = 0, rollbacks = 0;  // warm starts kept, warm starts undone
 };

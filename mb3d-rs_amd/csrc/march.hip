@@ -43,11 +43,20 @@ struct Cand {
     float rsf, step_count;
     int32_t pix, flags, nsteps, seed;
 };
-// (a record of the finish's refined surface points reuses the slot: c, mzz,
-// step_count, pix; rsf = the DE stop then (bin_search_it moves the point
-// without updating it); nsteps = the iteration count there; CF_DELIM = the
-// stop was the DE's, not the iteration limit's; the frame in flags >> 8)
-enum : int32_t { CF_START = 1, CF_FIRST = 2, CF_STEPPED = 4, CF_DELIM = 8 };
+enum : int32_t { CF_START = 1, CF_FIRST = 2, CF_STEPPED = 4 };
+
+// A refined surface point, from the finish to the normals pass, written and
+// read in list order (coalesced): the point, its depth and DE stop (the DE
+// stop of its last update: bin_search_it moves the point without updating
+// it), the step count, the pixel, and the iteration count with the flags.
+struct SurfRec {
+    double c[3];
+    double mzz;
+    float ms, step_count;
+    int32_t pix;
+    int32_t info;  // iteration count | SR_DELIM | frame << 20
+};
+enum : int32_t { SR_ITERS = 0xFFFF, SR_DELIM = 1 << 16 };  // SR_DELIM: the stop was the DE's, not the limit's
 
 // counter words of Bufs::ctr
 enum : int {
@@ -58,7 +67,7 @@ enum : int {
     C_POSTQ = 4,   // next item of the shadow pass
     C_AOQ = 5,     // next item of the DE AO pass
     C_SURF = 6,    // refined surface list length (the normals pass's items)
-    C_STATS = 8,   // 12 work counters from here
+    C_STATS = 8,   // 13 work counters from here
     C_WORDS = 24
 };
 
@@ -70,7 +79,7 @@ struct Bufs {
     uint32_t *gbuf;
     Cand *cand;
     int32_t *hits;
-    int32_t *surf;  // hand-off slots of the refined surface points (-1: a chunk's empty tail)
+    SurfRec *surf;  // the refined surface points (pix -1: a chunk's empty tail)
     unsigned long long *ctr;
     // the warm start: the network's proposal per work item of the queue
     // (hit distance / 4, hit logit)
@@ -103,10 +112,10 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 __device__ void flush_counters(const Cnt &c, unsigned long long *ctr) {
-    const unsigned long long v[12] = {c.steps,  c.de32, c.de64,     c.p8_32, c.box_32,    c.p8_64,
-                                      c.box_64, c.hits, c.overflow, c.jumps, c.rollbacks, c.oth_64};
+    const unsigned long long v[13] = {c.steps,  c.de32, c.de64,     c.p8_32, c.box_32,    c.p8_64, c.box_64,
+                                      c.hits,   c.overflow, c.jumps, c.rollbacks, c.oth_64, c.nrm_32};
 #pragma unroll
-    for (int k = 0; k < 12; k++) {
+    for (int k = 0; k < 13; k++) {
         const unsigned long long s = wave_sum(v[k]);
         if (lane_id() == 0 && s) atomicAdd(&ctr[C_STATS + k], s);
     }
@@ -644,7 +653,6 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
     Ray64 R;
     Refine F;
     int pix = 0, phase = FP_START;
-    long long cslot = 0;  // the ray's hand-off slot (its pixel with all_pixels)
     double last_step = 0, last_de = 0, dtmp = 0, back_dt1 = 0;
     float rsf = 1.0f, scount = 0.0f;
     int nsteps = 0;
@@ -667,7 +675,6 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
             if (!all_pixels && B.cand[slot].pix < 0) continue;  // an empty slot of a wave's chunk
             have = true;
             need = false;
-            cslot = slot;
             if (!all_pixels && BATCH) fr = B.cand[slot].flags >> 8;
             const DevP &F = BATCH ? tab[fr] : P0;
             R.reset(F);
@@ -810,26 +817,25 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
                 goto end;
             }
         }
-    refined: {
+    refined:
         // the refined surface point (surface_found, src/calc.rs:2471-2481):
-        // its record for k_normals, in the ray's hand-off slot
-        Cand &o = B.cand[cslot];
-        for (int k = 0; k < 3; k++) o.c[k] = R.c[k];
-        o.mzz = R.mzz;
-        o.rsf = R.ms;
-        o.step_count = scount;
-        o.pix = pix;
-        o.nsteps = max(R.it, 0);
-        o.flags = (de_limited ? CF_DELIM : 0) | (fr << 8);
+        // its normal and colour follow in k_normals
         surf = true;
         done = true;
-    }
     end:
         if (done) have = false;
         const long long ss = wave_take(surf, &B.ctr[C_SURF], sq, 64, -1);
-        if (surf) B.surf[ss] = (int32_t)cslot;
+        if (surf) {
+            SurfRec &o = B.surf[ss];
+            for (int k = 0; k < 3; k++) o.c[k] = R.c[k];
+            o.mzz = R.mzz;
+            o.ms = R.ms;
+            o.step_count = scount;
+            o.pix = pix;
+            o.info = min(max(R.it, 0), (int)SR_ITERS) | (de_limited ? SR_DELIM : 0) | (fr << 20);
+        }
     }
-    if (lane_id() < sq.left) B.surf[sq.base + lane_id()] = -1;
+    if (lane_id() < sq.left) B.surf[sq.base + lane_id()].pix = -1;
     flush_counters(cnt, B.ctr);
 }
 
@@ -851,19 +857,18 @@ __global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__
     // whole waves step together, so that the hit list's appends are uniform
     for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
         const long long i = base + lane_id();
-        const int slot = i < n ? B.surf[i] : -1;
+        const SurfRec o = i < n ? B.surf[i] : SurfRec{{0, 0, 0}, 0, 0, 0, -1, 0};
         bool hit = false;
         int pix = 0;
-        if (slot >= 0) {
-            const Cand &o = B.cand[slot];
-            const int fr = BATCH ? o.flags >> 8 : 0;
+        if (o.pix >= 0) {
+            const int fr = BATCH ? o.info >> 20 : 0;
             const DevP &P = BATCH ? tab[fr] : P0;
             Ray64 R;
             R.reset(P);
             double ct[3], nv[3];
             for (int k = 0; k < 3; k++) ct[k] = R.c[k] = o.c[k];
             R.mzz = o.mzz;
-            R.ms = o.rsf;
+            R.ms = o.ms;
             R.calc_sit = true;
             const double noffset = (double)fminf(P.de_stop, 1.0f) * (1.0 + R.mzz * (double)P.de_stop_factor) * 0.15;
             R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
@@ -882,6 +887,7 @@ __global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__
                         float ot = 0.0f;
                         R.de = (double)de32_analytic<DEM>(P, R.c, R.ms, it32, cnt, &ot);
                         R.otrap = (double)ot;
+                        cnt.nrm_32 += it32;
                     } else {
                         R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
                     }
@@ -890,7 +896,7 @@ __global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__
                 nv[axis] = (de[0] - de[1]) * 0.5;
             }
             for (int k = 0; k < 3; k++) R.c[k] = ct[k];
-            write_surface(P, B, o.pix, R, nv, nn, (o.flags & CF_DELIM) != 0, (uint16_t)o.nsteps, o.step_count);
+            write_surface(P, B, o.pix, R, nv, nn, (o.info & SR_DELIM) != 0, (uint16_t)(o.info & SR_ITERS), o.step_count);
             cnt.hits++;
             hit = true;
             pix = o.pix;
@@ -1752,7 +1758,7 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     if (f64 || post_only) M.enabled = 0;
     const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
     const size_t cap = list_capacity((long long)npix, c.cus, wpc);
-    const size_t base_need = (cap * sizeof(Cand) + 2 * cap * sizeof(int32_t) + 511) & ~(size_t)255;
+    const size_t base_need = (cap * (sizeof(Cand) + sizeof(SurfRec) + sizeof(int32_t)) + 511) & ~(size_t)255;
     const size_t work_need = base_need + (M.enabled ? learn_bufs(nullptr, d.nslots_q, nullptr) : 0);
     if (int e = ensure(&c.d_work, &c.work_bytes, work_need)) return e;
     Bufs b;
@@ -1778,8 +1784,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
         b.gbuf = (out && out->gbuf) || want_shadow || want_ao ? (uint32_t *)(ob + L.gbuf) : nullptr;
     }
     b.cand = (Cand *)c.d_work;
-    b.hits = (int32_t *)((char *)c.d_work + cap * sizeof(Cand));
-    b.surf = b.hits + cap;
+    b.surf = (SurfRec *)((char *)c.d_work + cap * sizeof(Cand));
+    b.hits = (int32_t *)(b.surf + cap);
     b.ctr = c.d_ctr;
     if (M.enabled) learn_bufs((char *)c.d_work + base_need, d.nslots_q, &b);
 
@@ -1883,6 +1889,7 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     t.jumps = ctr[C_STATS + 9];
     t.rollbacks = ctr[C_STATS + 10];
     t.its_other_f64 = ctr[C_STATS + 11];
+    t.its_normals_f32 = ctr[C_STATS + 12];
     t.kernel_ms = ms_march;
     t.post_ms = ms_post;
     t.march_f32_ms = split ? ms_a : 0.0;
@@ -1991,7 +1998,7 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     mlp_dev(o.mlp, o, pl->M);
     const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
     const size_t cap = list_capacity(pix, c.cus, wpc);
-    const size_t base_need = (cap * (sizeof(Cand) + 2 * sizeof(int32_t)) + 4096 + 255) & ~(size_t)255;
+    const size_t base_need = (cap * (sizeof(Cand) + sizeof(SurfRec) + sizeof(int32_t)) + 4096 + 255) & ~(size_t)255;
     pl->P0 = pl->tab[0];
     pl->P0.nslots_q = q;
     pl->P0.npix = pix;
@@ -2017,8 +2024,8 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     pl->b.normals = out->normals;
     pl->b.gbuf = out->gbuf;
     pl->b.cand = (Cand *)pl->d_work;
-    pl->b.hits = (int32_t *)((char *)pl->d_work + cap * sizeof(Cand));
-    pl->b.surf = pl->b.hits + cap;
+    pl->b.surf = (SurfRec *)((char *)pl->d_work + cap * sizeof(Cand));
+    pl->b.hits = (int32_t *)(pl->b.surf + cap);
     if (pl->M.enabled) learn_bufs((char *)pl->d_work + base_need, q, &pl->b);
     pl->b.ctr = pl->d_ctr;
     if (graph) {
@@ -2072,6 +2079,7 @@ int plan_run(Plan &pl, int repeats) {
     t.jumps = ctr[C_STATS + 9];
     t.rollbacks = ctr[C_STATS + 10];
     t.its_other_f64 = ctr[C_STATS + 11];
+    t.its_normals_f32 = ctr[C_STATS + 12];
     t.kernel_ms = ms_all > 0.0f ? ms_all / repeats : -1.0;
     t.march_f32_ms = ms_a;
     t.finish_f64_ms = ms_a > 0.0f ? t.kernel_ms - ms_a : -1.0;

@@ -242,6 +242,7 @@ class Counters(C.Structure):
         ("its_p8_f64", C.c_uint64),
         ("its_box_f64", C.c_uint64),
         ("its_other_f64", C.c_uint64),
+        ("its_normals_f32", C.c_uint64),
         ("overflow", C.c_uint64),
         ("jumps", C.c_uint64),
         ("rollbacks", C.c_uint64),
