@@ -405,26 +405,26 @@ __device__ __forceinline__ void pow2_scale64(double scaling, double v[4], const 
 }
 
 // Bulbox, HybridSuperCube2 (src/formulas.rs:596-627)
-// o: box scale, Min R, fold; o2: bulb scaling, R threshold, R threshold 2
-__device__ __forceinline__ void bulbox64(const double o[3], const double o2[3], double v[4], const double j[4],
-                                         double rout) {
-    const double rt1 = o2[1] * o2[1], rt2 = o2[2] * o2[2];
+// box scale, Min R, fold; bulb scaling, R threshold, R threshold 2
+__device__ __forceinline__ void bulbox64(double scale, double min_r, double fold, double bulb, double rth,
+                                         double rth2, double v[4], const double j[4], double rout) {
+    const double rt1 = rth * rth, rt2 = rth2 * rth2;
     if (!(rout < rt1)) {
-        box64(o[0], o[1], o[2], v, j, false);
+        box64(scale, min_r, fold, v, j, false);
         return;
     }
     if (rout < rt2) {
-        pow2_scale64(o2[0], v, j);
+        pow2_scale64(bulb, v, j);
         return;
     }
     const double m = (rout - rt2) / (rt1 - rt2);
     const double in0 = v[0], in1 = v[1], in2 = v[2];
-    pow2_scale64(o2[0], v, j);
+    pow2_scale64(bulb, v, j);
     const double b0 = v[0], b1 = v[1], b2 = v[2];
     v[0] = in0;
     v[1] = in1;
     v[2] = in2;
-    box64(o[0], o[1], o[2], v, j, false);
+    box64(scale, min_r, fold, v, j, false);
     double r1 = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     const double m2 = 1.0 - m;
     const double r2 = r1 * m + m2 * sqrt(b0 * b0 + b1 * b1 + b2 * b2);
@@ -453,20 +453,20 @@ __device__ __forceinline__ void real_power64(double pw, double zmul, double v[4]
 // aexion_c (src/formulas.rs:644-703): a real power bulb step with the
 // angles from the y axis; with "rotate C" the constant is turned by a power
 // of its own angles (or those of Z - C, mode 16)
-// o: power, Z multiplier, power C; cz_mul: the Cz multiplier
-__device__ __forceinline__ void aexion64(const double o[3], double cz_mul, int flags, double v[4], double j[4]) {
-    const double pw = o[0];
+// power, Z multiplier, power C, Cz multiplier, the switches and mode
+__device__ __forceinline__ void aexion64(double pw, double z_mul, double power_c, double cz_mul, int flags,
+                                         double v[4], double j[4]) {
     const double x = v[0], y = v[1], z = v[2];
     double st, ct, sp, cp;
     sincos(atan2(sqrt(x * x + z * z), y) * pw, &st, &ct);
     sincos(atan2(z, x) * pw, &sp, &cp);
     const double r1 = exp(log(x * x + y * y + z * z) * pw * 0.5);
     v[0] = cp * ct * r1 + j[0];
-    v[2] = st * r1 * o[1] + j[2];
+    v[2] = st * r1 * z_mul + j[2];
     v[1] = ct * sp * r1 + j[1];
     if (!(flags & 1)) return;
     const int mode = flags >> 8;
-    double pd = o[2];
+    double pd = power_c;
     if (flags & 4) {
         const double d0 = v[0] - j[0], d1 = v[1] - j[1], d2 = v[2] - j[2];
         pd *= sqrt(d0 * d0 + d1 * d1 + d2 * d2);
@@ -515,7 +515,9 @@ __device__ __forceinline__ void aexion64(const double o[3], double cz_mul, int f
 template <int KIND>
 __device__ __forceinline__ void call_slot64(const DevP &P, int n, It64 &s) {
     double *v = s.v;
-    const double *o = P.opt[n];
+    // the options as values (a pointer into the kernel's parameter block
+    // would make the compiler copy the block to scratch memory)
+    const double o[3] = {P.opt[n][0], P.opt[n][1], P.opt[n][2]};
     const int kind = P.slot_kind[n];
     if (KIND != K_GEN) {
         if (kind == MB3D_HIP_FORMULA_INTPOW) {
@@ -556,14 +558,16 @@ __device__ __forceinline__ void call_slot64(const DevP &P, int n, It64 &s) {
             v[1] = 2.0 * (y * x) + s.j[1];
             break;
         }
-        case MB3D_HIP_FORMULA_BULBOX: bulbox64(o, P.opt2[n], v, s.j, s.rout); break;
+        case MB3D_HIP_FORMULA_BULBOX:
+            bulbox64(o[0], o[1], o[2], P.opt2[n][0], P.opt2[n][1], P.opt2[n][2], v, s.j, s.rout);
+            break;
         case MB3D_HIP_FORMULA_FOLDINGINTPOW:  // src/formulas.rs:629-634
             v[0] = bfold64(v[0], o[1]);
             v[1] = bfold64(v[1], o[1]);
             v[2] = bfold64(v[2], o[1]);
             int_pow64(P.slot_iarg[n], o[0], v, s.j);
             break;
-        default: aexion64(o, P.opt2[n][0], P.slot_iarg[n], v, s.j); break;
+        default: aexion64(o[0], o[1], o[2], P.opt2[n][0], P.slot_iarg[n], v, s.j); break;
     }
     s.oth++;
 }
@@ -713,7 +717,10 @@ struct De64 {
 // option 32 being the textbook 0.5 r ln r / dr of the power-8 extension --
 // or the alternating hybrid (3D or 4D) four times for the numeric gradient.
 // Arguments and result in registers.
-template <int KIND>
+// DEM: the DE mode when the kernel knows it (DEM_P8 and DEM_BOX are
+// analytic: the numeric gradient's code is left out), DEM_NUMERIC when the
+// part's own flag decides.
+template <int KIND, int DEM = DEM_NUMERIC>
 __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, double cx, double cy, double cz, int max_it,
                                        int calc_sit, float ms) {
     De64 o;
@@ -724,7 +731,7 @@ __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, doubl
     double result;
     o.smooth_it = 0.0f;
     start64<KIND>(P, pt.four, s, cx, cy, cz);
-    if (pt.custom) {
+    if (DEM != DEM_NUMERIC || pt.custom) {
         if ((pt.de_option & 0x38) == 32)
             s.deriv1 = 1.0;
         else if ((pt.de_option & 0x38) == 16)
@@ -779,7 +786,7 @@ __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, doubl
 // minimum, maximum or a smooth minimum of the two; the iteration count,
 // trap and smooth iterations of the ruling part (blended for the smooth
 // minima).  Afterwards the iteration limit is the first part's.
-__device__ __noinline__ De64 de64_comb(const DevP &P, double cx, double cy, double cz, int max_it, int calc_sit,
+__device__ __forceinline__ De64 de64_comb(const DevP &P, double cx, double cy, double cz, int max_it, int calc_sit,
                                        float ms) {
     const De64 a = de64_part<K_GEN>(P, part1_of<K_GEN>(P), cx, cy, cz, max_it, calc_sit, ms);
     double qx = cx, qy = cy, qz = cz;
@@ -841,11 +848,11 @@ __device__ __noinline__ De64 de64_comb(const DevP &P, double cx, double cy, doub
     return o;
 }
 
-template <int KIND>
+template <int KIND, int DEM = DEM_NUMERIC>
 __device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double cz, int max_it, int calc_sit,
                                   float ms) {
     if (KIND == K_GEN && P.decomb) return de64_comb(P, cx, cy, cz, max_it, calc_sit, ms);
-    return de64_part<KIND>(P, part1_of<KIND>(P), cx, cy, cz, max_it, calc_sit, ms);
+    return de64_part<KIND, KIND == K_GEN ? DEM_NUMERIC : DEM>(P, part1_of<KIND>(P), cx, cy, cz, max_it, calc_sit, ms);
 }
 
 // The reference's Marcher (src/calc.rs:1143-1170) for one ray in f64: the

@@ -718,7 +718,7 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
         const DevP &P = BATCH ? tab[fr] : P0;
         bool surf = false, done = false;
         if (!have) goto end;
-        R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+        R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
         switch (phase) {
             case FP_START:
                 if (R.it >= R.max_its || R.de < (double)R.ms) {
@@ -871,7 +871,7 @@ __global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__
             R.ms = o.ms;
             R.calc_sit = true;
             const double noffset = (double)fminf(P.de_stop, 1.0f) * (1.0 + R.mzz * (double)P.de_stop_factor) * 0.15;
-            R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+            R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
             const float nn = R.smooth_it;
             R.calc_sit = false;
             for (int q = 0; q < 3; q++) {
@@ -889,7 +889,7 @@ __global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__
                         R.otrap = (double)ot;
                         cnt.nrm_32 += it32;
                     } else {
-                        R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+                        R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
                     }
                     de[side] = R.de;
                 }
@@ -929,7 +929,7 @@ __device__ __forceinline__ void post_de(const DevP &P, Ray64 &R, bool want64, Cn
             if (!(d32 < __builtin_inff())) cnt.overflow++;
         }
     }
-    R.take(de64<KIND>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+    R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
 }
 
 // hs_surface_start / surface_point: the start of the f64 surface point of a
@@ -1282,7 +1282,7 @@ template <int KIND>
 __global__ void k_de_at(DevP P, const double *pts, long long n, double *de, int32_t *its) {
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const De64 r = de64<KIND>(P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], P.max_it, 0, P.de_stop);
+        const De64 r = de64<KIND, DEM_NUMERIC>(P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], P.max_it, 0, P.de_stop);
         if (de) de[i] = r.de * P.step_width;
         if (its) its[i] = r.it_result;
     }
