@@ -1,0 +1,1303 @@
+// march_kernels.h -- the device code of the DE ray marcher for gfx950: the
+// kernels every pass runs, as templates over the formula kind and DE mode.
+// march.hip instantiates the FP32 kinds' kernels and holds the C ABI;
+// march_gen.hip instantiates the general kind's (K_GEN), so that the two
+// compile in parallel.
+//
+// Passes of one call (all on one stream, no host round trip in between):
+//  0. k_mlp_propose  the warm-start network's proposal per work item (opt-in)
+//  1. k_march32  persistent, wave64 lanes refilled from a tile queue: the
+//                FP32-iterate march (march_pixel_from + march_from,
+//                src/calc.rs:2231-2322,2533-2657) up to the guard band
+//                around the DE stop; background rays finish here, rays near
+//                the surface are handed to pass 2 through a compacted list.
+//  2. k_finish64 the reference's f64 marcher from the hand-off state: the
+//                stop decision and bisection (src/calc.rs:1706-1792), up to
+//                the refined surface point, listed for pass 3.
+//  3. k_normals  per refined surface point: normals (1813-1858), colour and
+//                depth (2471-2517, 2194-2205), the G-buffer words.
+//  4. k_shadow   per surface pixel: hs_surface_start + the march towards each
+//                light (src/calc.rs:2741-2899, without the row shortcut).
+//  5. k_deao     per surface pixel: surface_point + DE AO rays
+//                (src/calc.rs:2157-2192, src/deao.rs:268-418).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/mb3d_hip.h"
+#include "device_de.h"
+#include "mlp_device.h"
+
+namespace mb3d {
+
+// ---------------------------------------------------------------------------
+// device buffers of one call
+
+// State of a ray handed from the FP32 march to the FP64 finish.
+struct Cand {
+    double c[3], v[3];
+    double mzz, last_step, last_de;
+    float rsf, step_count;
+    int32_t pix, flags, nsteps, seed;
+};
+enum : int32_t { CF_START = 1, CF_FIRST = 2, CF_STEPPED = 4 };
+
+// A refined surface point, from the finish to the normals pass, written and
+// read in list order (coalesced): the point, its depth and DE stop (the DE
+// stop of its last update: bin_search_it moves the point without updating
+// it), the step count, the pixel, and the iteration count with the flags.
+struct SurfRec {
+    double c[3];
+    double mzz;
+    float ms, step_count;
+    int32_t pix;
+    int32_t info;  // iteration count | SR_DELIM | frame << 20
+};
+enum : int32_t { SR_ITERS = 0xFFFF, SR_DELIM = 1 << 16 };  // SR_DELIM: the stop was the DE's, not the limit's
+
+// counter words of Bufs::ctr
+enum : int {
+    C_QUEUE = 0,   // next work item of the march queue
+    C_CAND = 1,    // hand-off list length
+    C_HITS = 2,    // surface pixel list length
+    C_FINQ = 3,    // next item of the f64 finish
+    C_POSTQ = 4,   // next item of the shadow pass
+    C_AOQ = 5,     // next item of the DE AO pass
+    C_SURF = 6,    // refined surface list length (the normals pass's items)
+    C_STATS = 8,   // 13 work counters from here
+    C_WORDS = 24
+};
+
+struct Bufs {
+    float *depth;
+    double *depth64;
+    uint16_t *iters;
+    float *normals;
+    uint32_t *gbuf;
+    Cand *cand;
+    int32_t *hits;
+    SurfRec *surf;  // the refined surface points (pix -1: a chunk's empty tail)
+    unsigned long long *ctr;
+    // the warm start: the network's proposal per work item of the queue
+    // (hit distance / 4, hit logit)
+    float2 *prop;
+};
+
+// DE AO ray set (build_rays, src/deao.rs:75-105), computed on the host
+struct AoRays {
+    float dir[33][3];
+    int count;
+    float d_step_mul, d_min_a_dif, correction_weight, max_len;
+};
+
+struct ShadowJob {
+    int n;
+    int bit[MB3D_HIP_MAX_LIGHTS];
+    double vec[MB3D_HIP_MAX_LIGHTS][3];
+    float max_len_mul;
+};
+
+// ---------------------------------------------------------------------------
+// wave helpers
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline void flush_counters(const Cnt &c, unsigned long long *ctr) {
+    const unsigned long long v[13] = {c.steps,  c.de32, c.de64,     c.p8_32, c.box_32,    c.p8_64, c.box_64,
+                                      c.hits,   c.overflow, c.jumps, c.rollbacks, c.oth_64, c.nrm_32};
+#pragma unroll
+    for (int k = 0; k < 13; k++) {
+        const unsigned long long s = wave_sum(v[k]);
+        if (lane_id() == 0 && s) atomicAdd(&ctr[C_STATS + k], s);
+    }
+}
+
+// The value of the wave's first active lane, as a wave-uniform (scalar)
+// value: broadcasts an atomic's result from the lane that made it (always
+// the first active lane here), so that the compiler sees the loops whose exit
+// depends on it as uniform.
+__device__ __forceinline__ unsigned long long first_lane_u64(unsigned long long v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// appends one entry per lane with `want` set: returns the lane's slot
+__device__ __forceinline__ long long wave_append(bool want, unsigned long long *counter) {
+    const unsigned long long m = __ballot(want);
+    if (!m) return -1;
+    // the first active lane makes the atomic (first_lane_u64 reads it)
+    const int leader = __ffsll((long long)__ballot(true)) - 1;
+    // every active lane makes the atomic, adding 0 but the first (the
+    // compiler folds them into one): no lane-specific branch
+    const unsigned long long base =
+        first_lane_u64(atomicAdd(counter, lane_id() == leader ? (unsigned long long)__popcll(m) : 0ull));
+    if (!want) return -1;
+    return (long long)(base + __popcll(m & ((1ull << lane_id()) - 1ull)));
+}
+
+// A wave's share of a global counter, taken `chunk` items at a time, so that
+// the lanes of a persistent wave get their next items without an atomic per
+// turn (a device-wide atomic on one address serialises across the XCDs).
+struct WaveChunk {
+    long long base;  // next item of the wave's chunk (the same in every lane)
+    int left;        // items left in the chunk
+    bool dry;        // the counter passed its end
+};
+
+__device__ __forceinline__ void chunk_init(WaveChunk &q) {
+    q.base = 0;
+    q.left = 0;
+    q.dry = false;
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+struct NoHook {
+    __device__ void operator()(long long, int) const {}
+};
+
+// Hands the lanes with `want` the next items of a queue of `total` items
+// (`total` < 0: no end, an allocation).  Call with the whole wave active;
+// `hook(base, count)` runs, with the whole wave active, for each new chunk
+// the wave takes.  Returns the lane's item, or -1 (the queue is exhausted).
+template <typename Hook = NoHook>
+__device__ __forceinline__ long long wave_take(bool want, unsigned long long *ctr, WaveChunk &q, int chunk,
+                                               long long total, Hook &&hook = Hook()) {
+    long long item = -1;
+    bool pending = want;
+    const int leader = __ffsll((long long)__ballot(true)) - 1;
+    for (;;) {
+        const unsigned long long m = __ballot(pending);
+        if (!m || q.dry) break;
+        if (q.left == 0) {
+            const unsigned long long b =
+                first_lane_u64(atomicAdd(ctr, lane_id() == leader ? (unsigned long long)chunk : 0ull));
+            if (total >= 0 && (long long)b >= total) {
+                q.dry = true;
+                break;
+            }
+            q.base = (long long)b;
+            q.left = total >= 0 ? (int)min((long long)chunk, total - (long long)b) : chunk;
+            hook(q.base, q.left);
+        }
+        const int rank = __popcll(m & lanemask_lt());
+        const int take = min(__popcll(m), q.left);
+        if (pending && rank < take) {
+            item = q.base + rank;
+            pending = false;
+        }
+        q.base += take;
+        q.left -= take;
+    }
+    return item;
+}
+
+__device__ inline void write_background(const DevP &P, const Bufs &B, int pix, float step_count) {
+    const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
+    if (B.depth) B.depth[pix] = __builtin_inff();
+    if (B.depth64) B.depth64[pix] = __builtin_inf();
+    if (B.iters) B.iters[pix] = 0;
+    if (B.normals) {
+        B.normals[3 * pix] = 0.0f;
+        B.normals[3 * pix + 1] = 0.0f;
+        B.normals[3 * pix + 2] = 0.0f;
+    }
+    if (B.gbuf) {
+        uint32_t *g = B.gbuf + 5 * (size_t)pix;
+        g[0] = 0;
+        g[1] = sh << 16;
+        g[2] = 32768u << 16;
+        g[3] = 0;
+        g[4] = 5000;
+    }
+}
+
+// pixel_seed (src/calc.rs:1125-1140)
+__device__ inline int32_t dev_pixel_seed(int64_t y, int64_t x) {
+    uint32_t s = (uint32_t)(int32_t)(0x24563487LL + (y + 1) * 0x324594A1LL);
+    uint32_t mul = 214013u, add = 2531011u, am = 1u, aa = 0u;
+    for (uint64_t n = x > 0 ? (uint64_t)x : 0; n; n >>= 1) {
+        if (n & 1) {
+            aa = aa * mul + add;
+            am *= mul;
+        }
+        add = add * mul + add;
+        mul *= mul;
+    }
+    return (int32_t)(am * s + aa);
+}
+
+// ---------------------------------------------------------------------------
+// pass 1: the FP32 march, persistent, lanes refilled one by one
+
+// A batch's frame table: frame f's parameters at base + f * STRIDE bytes.
+// In global memory the records are whole DevPs; the march and finish
+// kernels stage their cores (the fields before DevP::opt2, all that the
+// batched kinds read) in LDS, DEVP_CORE bytes apart.
+constexpr int DEVP_CORE = (int)offsetof(DevP, opt2);
+static_assert(DEVP_CORE % 8 == 0, "the core of DevP in 8-byte words");
+template <int STRIDE>
+struct Frames {
+    const char *base;
+    __device__ __forceinline__ const DevP &operator[](int f) const {
+        return *reinterpret_cast<const DevP *>(base + (size_t)f * STRIDE);
+    }
+};
+using LdsFrames = Frames<DEVP_CORE>;
+using GlobalFrames = Frames<(int)sizeof(DevP)>;
+
+// The cores of a batch's frames copied to LDS (the dynamic LDS of the
+// launch, nframes x DEVP_CORE bytes): the march and finish read a frame's
+// parameters in their inner loops, and from global memory every read is a
+// vector load (the frame differs per lane).  All threads of the block call
+// it.
+__device__ __forceinline__ LdsFrames stage_frames(const DevP *__restrict__ tab, int nframes) {
+    extern __shared__ __attribute__((aligned(16))) uint2 frames_lds[];
+    constexpr int core_words = DEVP_CORE / (int)sizeof(uint2), full_words = (int)(sizeof(DevP) / sizeof(uint2));
+    const uint2 *src = reinterpret_cast<const uint2 *>(tab);
+    for (int i = threadIdx.x; i < nframes * core_words; i += blockDim.x) {
+        const int f = i / core_words, w = i - f * core_words;
+        frames_lds[i] = src[f * full_words + w];
+    }
+    __syncthreads();
+    return LdsFrames{reinterpret_cast<const char *>(frames_lds)};
+}
+static_assert(sizeof(DevP) % sizeof(uint2) == 0, "DevP in 8-byte words");
+constexpr int MAX_BATCH_FRAMES = 64;
+static_assert(MAX_BATCH_FRAMES * DEVP_CORE <= 65536, "a batch's frame table fits one block's LDS");
+
+// the frame of a batch that work item q belongs to
+template <class TAB>
+__device__ __forceinline__ int find_frame(const TAB &tab, int n, long long q) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].q_begin <= q)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// the pixel of work item q of the tile queue (false: the item is in a
+// ragged edge tile, outside the picture)
+template <bool BATCH, class TAB>
+__device__ __forceinline__ bool item_pixel(const DevP &P0, const TAB &tab, int nframes, long long q, int &fr, int &x,
+                                           int &yr) {
+    fr = BATCH ? find_frame(tab, nframes, q) : 0;
+    const DevP &F = BATCH ? tab[fr] : P0;
+    const int tile_n = P0.tile_w * P0.tile_h;
+    const long long ql = q - F.q_begin;
+    const long long tile = ql / tile_n;
+    const int k = (int)(ql - tile * tile_n);
+    x = (int)(tile % F.tiles_x) * F.tile_w + k % F.tile_w;
+    yr = (int)(tile / F.tiles_x) * F.tile_h + k / F.tile_w;
+    return x < F.width && yr < F.rows;
+}
+
+// The warm start's proposals: the network (BF16 MFMA, 64 rays a wave) on
+// the ray of every work item of the queue, written to prop[q] (hit distance
+// / 4, hit logit).  A kernel of its own, ahead of the march in the same
+// stream: a march wave cannot take the proposals of a chunk another wave
+// computed in the same kernel without an agent-scope release and acquire,
+// which on this GPU write back and invalidate an XCD's whole L2 (measured:
+// +0.75 ms a 1080p frame at a fence pair per chunk of 256 rays).
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__restrict__ gtab, int nframes, MlpDev M,
+                                                     float2 *prop) {
+    const GlobalFrames tab{reinterpret_cast<const char *>(gtab)};
+    __shared__ __attribute__((aligned(16))) char lds[4][MLP_LDS_BYTES];
+    __shared__ mlp_s4 wl[MLP_TILES * 64];
+    mlp_stage_weights(M, wl);
+    __syncthreads();
+    const int lane = lane_id();
+    const long long groups = (P0.nslots_q + 63) / 64;
+    const long long waves = (long long)gridDim.x * 4;
+    // a uniform loop: every lane of a wave runs the same groups
+    for (long long g = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); g < groups; g += waves) {
+        const long long q = g * 64 + lane;
+        float o[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 1.0f};
+        int f = 0, x = 0, yr = 0;
+        const bool in_q = q < P0.nslots_q;
+        if (in_q && item_pixel<BATCH>(P0, tab, nframes, q, f, x, yr)) {
+            const DevP &F = BATCH ? tab[f] : P0;
+            double cc[3], vv[3];
+            pixel_ray(F, x, F.row0 + yr, cc, vv);
+            const double il = 1.0 / sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+            for (int j = 0; j < 3; j++) {
+                o[j] = (float)cc[j];
+                d[j] = (float)(vv[j] * il);
+            }
+        }
+        float t = 0.0f, lg = 0.0f;
+        mlp_wave_infer(M, wl, o, d, lds[threadIdx.x >> 6], t, lg);
+        if (in_q) prop[q] = make_float2(t, lg);
+    }
+}
+
+// BATCH: the frames of a batch in one launch; `tab` holds each frame's
+// parameters, P0 the batch's queue totals and the shared tile shape.
+// LEARN: the warm start (B.prop holds k_mlp_propose's proposals).  A ray
+// starts where the network puts it: confident hits at the predicted
+// distance less a margin, confident misses past the end (the background).
+// A start that is not free -- the DE there below the stop distance (with
+// the hand-off guard) or the iteration limit reached -- is rolled back to
+// the plain march from the start plane.  This is the reference's cone-march
+// jump and check (march_pixel_from, src/calc.rs:2293-2308) with the
+// network's depth in place of the coarse ray's.
+template <int KIND, int DEM, bool BATCH, bool LEARN>
+__global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__restrict__ gtab, int nframes,
+                                                 MlpDev M) {
+    const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
+    Cnt cnt;
+    int fr = 0;
+    bool have = false, exhausted = false;
+    int pix = 0, px = 0, py = 0;
+    double c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    double mzz = 0;
+    float last_step = 0, last_de = 0, dtmp = 0;
+    float ms = 0, rsf = 1, scount = 0;
+    int nsteps = 0;
+    bool start = false, first = false, jumped = false;
+    float jrem = 0.0f;  // a jump's predicted distance left to the surface (march units)
+    int32_t seed = 0;
+    const float fguard = P0.guard;
+    WaveChunk wq, cq;
+    chunk_init(wq);
+    chunk_init(cq);
+    for (;;) {
+        // refill: every idle lane takes the next work item from the wave's chunk
+        bool need = !have && !exhausted;
+        for (int tries = 0; tries < 4 && __any(need); tries++) {
+            const long long q = wave_take(need, &B.ctr[C_QUEUE], wq, 256, P0.nslots_q);
+            if (need) {
+                int x = 0, yr = 0;
+                if (q < 0) {
+                    exhausted = true;
+                    need = false;
+                } else if (item_pixel<BATCH>(P0, tab, nframes, q, fr, x, yr)) {
+                    const DevP &F = BATCH ? tab[fr] : P0;
+                    have = true;
+                    need = false;
+                    px = x;
+                    py = F.row0 + yr;
+                    pix = (int)(F.pix_base + (long long)yr * F.width + x);
+                    pixel_ray(F, px, py, c, v);
+                    mzz = 0.0;
+                    ms = F.de_stop;
+                    rsf = 1.0f;
+                    scount = 0.0f;
+                    nsteps = 0;
+                    start = true;
+                    jumped = false;
+                    first = F.first_step_random != 0;
+                    seed = first ? dev_pixel_seed(py, px) : 0;
+                    if constexpr (LEARN) {
+                        const float2 pr = B.prop[q];
+                        const float ph = 1.0f / (1.0f + __expf(-pr.y));
+                        double jump = 0.0;
+                        if (ph >= M.conf_hit) {
+                            const double vl = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                            const double th = 4.0 * (double)pr.x;
+                            const double tj = th * (1.0 - (double)M.margin_rel) - M.margin_abs;
+                            if (tj > 0.0) {
+                                jump = tj / vl;
+                                jrem = (float)((th - tj) / vl);
+                            }
+                        } else if (ph <= M.conf_miss) {
+                            jump = F.zend * 2.0;
+                        }
+                        if (jump > F.zend) {  // past the end: the background
+                            write_background(F, B, pix, 0.0f);
+                            cnt.jumps++;
+                            have = false;
+                            need = true;
+                        } else if (jump > 0.0) {
+                            mzz = jump;
+                            c[0] += v[0] * jump;
+                            c[1] += v[1] * jump;
+                            c[2] += v[2] * jump;
+                            ms = F.de_stop * (1.0f + (float)mzz * F.de_stop_factor);
+                            jumped = true;
+                        }
+                    }
+                }
+            }
+        }
+        const DevP &P = BATCH ? tab[fr] : P0;
+        if (__all(exhausted)) break;
+        bool handoff = false, background = false;
+        if (have) {
+            int it = 0;
+            const float d32 = de32<KIND, DEM>(P, c, ms, it, cnt);
+            // the step bookkeeping of the FP32 march runs in f32 (the f64
+            // finish re-decides everything near the surface); the depth and
+            // the position accumulate in f64
+            bool top = true;
+            if (LEARN && jumped) {
+                // the warm start's check: a free start, clear of the
+                // hand-off guard and of a tenth of the distance the network
+                // left to the surface, is kept (a DE far under that distance
+                // means a surface nearer than predicted: one in front, or one
+                // the jump passed through); any other goes back to the start
+                // plane and marches plainly
+                jumped = false;
+                if (!(d32 < __builtin_inff()) || it >= P.max_it || !(d32 >= ms * fguard) ||
+                    !(d32 >= 0.1f * jrem)) {
+                    cnt.rollbacks++;
+                    pixel_ray(P, px, py, c, v);
+                    mzz = 0.0;
+                    ms = P.de_stop;
+                    top = false;
+                } else {
+                    cnt.jumps++;
+                }
+            }
+            if (!top) {
+            } else if (!(d32 < __builtin_inff())) {  // single precision overflow: f64 decides
+                cnt.overflow++;
+                handoff = true;
+                top = false;
+            } else if (start) {
+                if (it >= P.max_it || !(d32 >= ms * fguard)) {
+                    handoff = true;
+                    top = false;
+                } else {
+                    dtmp = d32;
+                    last_step = dtmp * P.z_step_div;
+                }
+            } else {
+                dtmp = fminf(d32, last_de + last_step);
+                if (last_de > dtmp + 1e-30f) {
+                    const float t = last_step * __builtin_amdgcn_rcpf(last_de - dtmp);
+                    rsf = t < 1.0f ? fmaxf(0.5f, t) : 1.0f;
+                } else {
+                    rsf = 1.0f;
+                }
+            }
+            if (top) {
+                if (it >= P.max_it) {
+                    handoff = true;  // the half step back (src/calc.rs:2545-2552) is f64 work
+                } else if (it < P.min_it || dtmp >= ms * fguard) {
+                    if (P.step_cap > 0 && nsteps >= P.step_cap) {
+                        background = true;
+                    } else if (P.it_split > 0 && nsteps >= P.it_split) {
+                        // the iteration-cap split: a ray still marching after
+                        // it_split steps goes on in the f64 finish, which
+                        // resumes it from here, so that the long rays of a
+                        // launch do not hold the FP32 pass's tail
+                        handoff = true;
+                    } else {
+                        last_de = dtmp;
+                        float d = fmaxf(0.11f, (dtmp - P.ms_de_sub * ms) * P.z_step_div * rsf);
+                        const float s1 = fmaxf(ms, 0.4f) * P.mh04zsd;
+                        const bool count = P.dfog_on_it == 0 || it == P.dfog_on_it;
+                        if (s1 < d) {
+                            if (count) scount += s1 * __builtin_amdgcn_rcpf(d);
+                            d = s1;
+                        } else if (count) {
+                            scount += 1.0f;
+                        }
+                        if (first) {
+                            first = false;
+                            seed = (int32_t)((uint32_t)seed * 214013u + 2531011u);
+                            d = (float)((double)d * ((double)(seed & 0x7FFFFFFF) * (1.0 / (double)0x7FFFFFFF)));
+                        }
+                        const double dd = (double)d;
+                        mzz += dd;
+                        if (mzz > P.zend) {
+                            background = true;
+                        } else {
+                            nsteps++;
+                            cnt.steps++;
+                            last_step = d;
+                            c[0] += v[0] * dd;
+                            c[1] += v[1] * dd;
+                            c[2] += v[2] * dd;
+                            ms = P.de_stop * (1.0f + (float)mzz * P.de_stop_factor);
+                            start = false;
+                        }
+                    }
+                } else {
+                    handoff = true;
+                }
+            }
+        }
+        const long long slot = wave_take(handoff, &B.ctr[C_CAND], cq, 64, -1);
+        if (handoff) {
+            Cand &o = B.cand[slot];
+            o.c[0] = c[0];
+            o.c[1] = c[1];
+            o.c[2] = c[2];
+            o.v[0] = v[0];
+            o.v[1] = v[1];
+            o.v[2] = v[2];
+            o.mzz = mzz;
+            o.last_step = last_step;
+            o.last_de = last_de;
+            o.rsf = rsf;
+            o.step_count = scount;
+            o.pix = pix;
+            o.flags = (start ? CF_START : 0) | (first ? CF_FIRST : 0) | (nsteps > 0 ? CF_STEPPED : 0) | (fr << 8);
+            o.nsteps = nsteps;
+            o.seed = seed;
+            have = false;
+        }
+        if (background) {
+            write_background(P, B, pix, scount);
+            have = false;
+        }
+    }
+    // the wave's unused hand-off slots are marked empty
+    if (lane_id() < cq.left) B.cand[cq.base + lane_id()].pix = -1;
+    flush_counters(cnt, B.ctr);
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: the f64 finish.  Each lane is a resumable state machine with one
+// DE site (the reference's WGSL keeps the same shape, one evaluate() call in
+// a loop of states, src/gpu_march.wgsl:1903): lanes in different states --
+// march, bisection, normals -- share the one f64 DE of every turn, and a lane
+// that finishes takes the next ray from the list at once.
+
+// store_normal (src/calc.rs:1794-1804)
+__device__ inline void store_normal(const double n[3], int16_t out[3]) {
+    const double d = 32767.0 / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] + 1e-100);
+    for (int k = 0; k < 3; k++) {
+        const double v = rte(n[k] * d);
+        out[k] = isnan(v) ? 0 : (int16_t)fmin(fmax(v, -32767.0), 32767.0);
+    }
+    if (out[0] == 0 && out[1] == 0) out[2] = out[2] > 0 ? 32767 : -32767;
+}
+
+// the surface values of surface_found (src/calc.rs:2471-2517) into the outputs
+__device__ inline void write_surface(const DevP &P, const Bufs &B, int pix, const Ray64 &R, const double n[3], float nn,
+                              bool de_limited, uint16_t iters, float step_count) {
+    int16_t nq[3];
+    store_normal(n, nq);
+    float g;
+    if (de_limited) {
+        const float lg = (float)log((double)fmaxf(P.de_stop, R.ms) * P.step_width);
+        g = 32767.0f - (nn + P.d_col_plus + P.col_var * lg) * P.mcts_m;
+    } else {
+        g = 32767.0f - nn * P.mcts_m;
+    }
+    const uint32_t si_gradient = clip15(g);
+    const uint32_t otrap = clip15((float)(R.otrap * 4096.0));  // do_color, option 0
+    const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
+    const double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (B.depth) B.depth[pix] = (float)R.mzz;
+    if (B.depth64) B.depth64[pix] = R.mzz;
+    if (B.iters) B.iters[pix] = iters;
+    if (B.normals) {
+        for (int k = 0; k < 3; k++) B.normals[3 * pix + k] = l > 0 ? (float)(n[k] / l) : 0.0f;
+    }
+    if (B.gbuf) {
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        w[0] = (uint32_t)(uint16_t)nq[0] | ((uint32_t)(uint16_t)nq[1] << 16);
+        w[1] = (uint32_t)(uint16_t)nq[2] | (sh << 16);
+        w[2] = R.zpos_fine(P);
+        w[3] = si_gradient | (otrap << 16);
+        w[4] = 5000;
+    }
+}
+
+// a ray that starts inside the set (src/calc.rs:2277-2293)
+__device__ inline void write_inside(const DevP &P, const Bufs &B, int pix, const Ray64 &R) {
+    const double t = fmin(fmax(R.rout / P.d_rstop, 0.0), 1.0);
+    const uint32_t sg = 32768u + (uint32_t)rte(32767.0 * t);
+    const uint32_t otrap = clip15((float)(R.otrap * 4096.0));
+    if (B.depth) B.depth[pix] = 0.0f;
+    if (B.depth64) B.depth64[pix] = 0.0;
+    if (B.iters) B.iters[pix] = (uint16_t)max(R.it, 0);
+    if (B.normals) {
+        B.normals[3 * pix] = 0.0f;
+        B.normals[3 * pix + 1] = 0.0f;
+        B.normals[3 * pix + 2] = -1.0f;
+    }
+    if (B.gbuf) {
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        w[0] = 0;
+        w[1] = (uint32_t)(uint16_t)(int16_t)-32767;
+        w[2] = 0x7FFF0000u;
+        w[3] = sg | (otrap << 16);
+        w[4] = 5000;
+    }
+}
+
+enum FinPhase : int { FP_START, FP_RESUME, FP_BACK, FP_STEP, FP_REF };
+
+// The finish ends at the refined surface point: its record goes to the
+// surface list, and k_normals computes the normal, colour and G-buffer
+// words.  So that every phase here makes the same f64 DE: one DE site per
+// turn.
+template <int KIND, int DEM, bool BATCH>
+__global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixels, const DevP *__restrict__ gtab,
+                                                  int nframes) {
+    const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
+    Cnt cnt;
+    int fr = 0;
+    const long long n_items = all_pixels ? P0.npix : (long long)B.ctr[C_CAND];
+    bool have = false, exhausted = false;
+    Ray64 R;
+    Refine F;
+    int pix = 0, phase = FP_START;
+    double last_step = 0, last_de = 0, dtmp = 0, back_dt1 = 0;
+    float rsf = 1.0f, scount = 0.0f;
+    int nsteps = 0;
+    uint32_t guard = 0;
+    bool first = false, stepped = false, de_limited = false;
+    int32_t seed = 0;
+    WaveChunk fq, sq;
+    chunk_init(fq);
+    chunk_init(sq);
+    for (;;) {
+        bool need = !have && !exhausted;
+        for (int tries = 0; tries < 4 && __any(need); tries++) {
+            const long long slot = wave_take(need, &B.ctr[C_FINQ], fq, 64, n_items);
+            if (!need) continue;
+            if (slot < 0) {
+                exhausted = true;
+                need = false;
+                continue;
+            }
+            if (!all_pixels && B.cand[slot].pix < 0) continue;  // an empty slot of a wave's chunk
+            have = true;
+            need = false;
+            if (!all_pixels && BATCH) fr = B.cand[slot].flags >> 8;
+            const DevP &F = BATCH ? tab[fr] : P0;
+            R.reset(F);
+            guard = 0;
+            if (all_pixels) {
+                pix = (int)slot;
+                const int x = pix % F.width, y = F.row0 + pix / F.width;
+                pixel_ray(F, x, y, R.c, R.v);
+                phase = FP_START;
+                first = F.first_step_random != 0;
+                seed = first ? dev_pixel_seed(y, x) : 0;
+                scount = 0.0f;
+                nsteps = 0;
+                rsf = 1.0f;
+                stepped = false;
+            } else {
+                const Cand &o = B.cand[slot];
+                pix = o.pix;
+                for (int k = 0; k < 3; k++) {
+                    R.c[k] = o.c[k];
+                    R.v[k] = o.v[k];
+                }
+                last_step = o.last_step;
+                last_de = o.last_de;
+                rsf = o.rsf;
+                scount = o.step_count;
+                nsteps = o.nsteps;
+                seed = o.seed;
+                first = (o.flags & CF_FIRST) != 0;
+                stepped = (o.flags & CF_STEPPED) != 0;
+                if (o.flags & CF_START) {
+                    phase = FP_START;
+                } else {
+                    R.mzz = o.mzz;
+                    R.upd(F);
+                    phase = FP_RESUME;
+                }
+            }
+        }
+        if (__all(exhausted)) break;
+        const DevP &P = BATCH ? tab[fr] : P0;
+        bool surf = false, done = false;
+        if (!have) goto end;
+        R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+        switch (phase) {
+            case FP_START:
+                if (R.it >= R.max_its || R.de < (double)R.ms) {
+                    write_inside(P, B, pix, R);
+                    done = true;
+                    goto end;
+                }
+                dtmp = R.de;
+                last_step = dtmp * (double)P.z_step_div;
+                goto top;
+            case FP_RESUME:
+                dtmp = R.de;
+                if (!stepped) {
+                    last_step = dtmp * (double)P.z_step_div;
+                    goto top;
+                }
+                goto clamp;
+            case FP_BACK:
+                dtmp = R.de;
+                last_step = -back_dt1;
+                goto cond;
+            case FP_STEP:
+                dtmp = R.de;
+                goto clamp;
+            default:  // FP_REF
+                if (F.after(R, P)) goto end;
+                goto refined;
+        }
+    clamp:
+        // after a step of march_from (src/calc.rs:2626-2638)
+        if (dtmp > last_de + last_step) dtmp = last_de + last_step;
+        if (last_de > dtmp + 1e-30) {
+            const double t = last_step / (last_de - dtmp);
+            rsf = t < 1.0 ? (float)fmax(0.5, t) : 1.0f;
+        } else {
+            rsf = 1.0f;
+        }
+    top:
+        // the head of march_from's loop (src/calc.rs:2540-2552)
+        if (++guard > 2000000u) {
+            write_background(P, B, pix, scount);
+            done = true;
+            goto end;
+        }
+        if (R.it >= R.max_its) {
+            back_dt1 = -0.5 * last_step;
+            R.move(back_dt1);
+            R.upd(P);
+            phase = FP_BACK;
+            goto end;
+        }
+    cond:
+        if (R.it < P.min_it || (R.it < R.max_its && dtmp >= (double)R.ms)) {
+            // the next step (src/calc.rs:2553-2620)
+            if (P.step_cap > 0 && nsteps >= P.step_cap) {
+                write_background(P, B, pix, scount);
+                done = true;
+                goto end;
+            }
+            last_de = dtmp;
+            double d = fmax(0.11, (dtmp - (double)P.ms_de_sub * (double)R.ms) * (double)P.z_step_div * (double)rsf);
+            const double s1 = (double)(fmaxf(R.ms, 0.4f) * P.mh04zsd);
+            const bool count = P.dfog_on_it == 0 || R.it == P.dfog_on_it;
+            if (s1 < d) {
+                if (count) scount += (float)(s1 / d);
+                d = s1;
+            } else if (count) {
+                scount += 1.0f;
+            }
+            if (first) {
+                first = false;
+                seed = (int32_t)((uint32_t)seed * 214013u + 2531011u);
+                d *= (double)(seed & 0x7FFFFFFF) * (1.0 / (double)0x7FFFFFFF);
+            }
+            R.mzz += d;
+            if (R.mzz > P.zend) {
+                write_background(P, B, pix, scount);
+                done = true;
+                goto end;
+            }
+            nsteps++;
+            cnt.steps++;
+            last_step = d;
+            for (int k = 0; k < 3; k++) R.c[k] += R.v[k] * d;
+            R.upd(P);
+            phase = FP_STEP;
+            goto end;
+        }
+        // surface found (src/calc.rs:2471-2481)
+        de_limited = R.it < R.max_its || dtmp < (double)R.ms;
+        if (P.de_add_steps != 0) {
+            const bool more = de_limited ? F.start_bs(R, P, dtmp, last_step, P.de_add_steps)
+                                         : F.start_bsi(R, P.de_add_steps);
+            if (more) {
+                phase = FP_REF;
+                goto end;
+            }
+        }
+    refined:
+        // the refined surface point (surface_found, src/calc.rs:2471-2481):
+        // its normal and colour follow in k_normals
+        surf = true;
+        done = true;
+    end:
+        if (done) have = false;
+        const long long ss = wave_take(surf, &B.ctr[C_SURF], sq, 64, -1);
+        if (surf) {
+            SurfRec &o = B.surf[ss];
+            for (int k = 0; k < 3; k++) o.c[k] = R.c[k];
+            o.mzz = R.mzz;
+            o.ms = R.ms;
+            o.step_count = scount;
+            o.pix = pix;
+            o.info = min(max(R.it, 0), (int)SR_ITERS) | (de_limited ? SR_DELIM : 0) | (fr << 20);
+        }
+    }
+    if (lane_id() < sq.left) B.surf[sq.base + lane_id()].pix = -1;
+    flush_counters(cnt, B.ctr);
+}
+
+// The normals pass over the refined surface points: calculate_normals with
+// smooth_normals 0 (src/calc.rs:1813-1858) -- the centre DE with the smooth
+// iterations, then the DEs at +-noffset along vg2, vg0, vg1 -- and the
+// surface's G-buffer words (write_surface).  Every lane makes the same seven
+// DEs, so the waves stay converged.  N32: the six offset DEs in FP32, for
+// the analytic DEs when the stencil's offset spans enough FP32 ulps of the
+// coordinates (the host's fp32_stencil_ok; at the 1080p Mandelbulb their
+// error moves a normal by ~0.01 degree); the numeric gradient's DE does not
+// allow it.
+template <int KIND, int DEM, bool N32, bool BATCH>
+__global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__restrict__ gtab, int nframes) {
+    const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
+    Cnt cnt;
+    WaveChunk hq;
+    chunk_init(hq);
+    const long long n = (long long)B.ctr[C_SURF];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    // whole waves step together, so that the hit list's appends are uniform
+    for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+        const long long i = base + lane_id();
+        const SurfRec o = i < n ? B.surf[i] : SurfRec{{0, 0, 0}, 0, 0, 0, -1, 0};
+        bool hit = false;
+        int pix = 0;
+        if (o.pix >= 0) {
+            const int fr = BATCH ? o.info >> 20 : 0;
+            const DevP &P = BATCH ? tab[fr] : P0;
+            Ray64 R;
+            R.reset(P);
+            double ct[3], nv[3];
+            for (int k = 0; k < 3; k++) ct[k] = R.c[k] = o.c[k];
+            R.mzz = o.mzz;
+            R.ms = o.ms;
+            R.calc_sit = true;
+            const double noffset = (double)fminf(P.de_stop, 1.0f) * (1.0 + R.mzz * (double)P.de_stop_factor) * 0.15;
+            R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+            const float nn = R.smooth_it;
+            R.calc_sit = false;
+            for (int q = 0; q < 3; q++) {
+                const int axis = q == 0 ? 2 : q - 1;  // the order vg2, vg0, vg1
+                double de[2];
+                for (int side = 0; side < 2; side++) {
+                    for (int k = 0; k < 3; k++) R.c[k] = ct[k] + P.vg[axis][k] * noffset;
+                    if (side) {
+                        for (int k = 0; k < 3; k++) R.c[k] += P.vg[axis][k] * (-2.0 * noffset);
+                    }
+                    if (N32) {
+                        int it32 = 0;
+                        float ot = 0.0f;
+                        R.de = (double)de32_analytic<DEM>(P, R.c, R.ms, it32, cnt, &ot);
+                        R.otrap = (double)ot;
+                        cnt.nrm_32 += it32;
+                    } else {
+                        R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+                    }
+                    de[side] = R.de;
+                }
+                nv[axis] = (de[0] - de[1]) * 0.5;
+            }
+            for (int k = 0; k < 3; k++) R.c[k] = ct[k];
+            write_surface(P, B, o.pix, R, nv, nn, (o.info & SR_DELIM) != 0, (uint16_t)(o.info & SR_ITERS), o.step_count);
+            cnt.hits++;
+            hit = true;
+            pix = o.pix;
+        }
+        const long long hs = wave_take(hit, &B.ctr[C_HITS], hq, 64, -1);
+        if (hit) B.hits[hs] = pix;
+    }
+    if (lane_id() < hq.left) B.hits[hq.base + lane_id()] = -1;
+    flush_counters(cnt, B.ctr);
+}
+
+// ---------------------------------------------------------------------------
+// the post passes: hard shadows and DE AO, same state-machine shape
+
+// The DE of a post ray: the bulk precision, and f64 where the decision is
+// close (within the guard band of the DE stop, at the iteration limit, or an
+// FP32 overflow), so that no DE site is duplicated per state.
+template <int KIND, int DEM, bool F32>
+__device__ __forceinline__ void post_de(const DevP &P, Ray64 &R, bool want64, Cnt &cnt) {
+    if constexpr (F32) {
+        if (!want64) {
+            int it = 0;
+            const float d32 = de32<KIND, DEM>(P, R.c, R.ms, it, cnt);
+            if (d32 < __builtin_inff() && it < R.max_it && d32 > R.ms * P.guard) {
+                R.de = (double)d32;
+                R.it = it;
+                R.max_its = R.max_it;
+                return;
+            }
+            if (!(d32 < __builtin_inff())) cnt.overflow++;
+        }
+    }
+    R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+}
+
+// hs_surface_start / surface_point: the start of the f64 surface point of a
+// G-buffer pixel from its depth code (src/calc.rs:2741-2753, 2163-2171)
+__device__ __forceinline__ void ray_from_code(const DevP &P, Ray64 &R, int x, int y, uint32_t zpos_fine) {
+    R.reset(P);
+    pixel_ray(P, x, y, R.c, R.v);
+    const double zf = (double)(zpos_fine >> 8);
+    R.mzz = depth_of_code(P, zf, 8388351.5);
+    for (int k = 0; k < 3; k++) R.c[k] += R.v[k] * R.mzz;
+    R.upd(P);
+}
+
+enum ShPhase : int { SH_SURF, SH_REF, SH_D0, SH_DS };
+
+template <int KIND, int DEM, bool F32>
+__global__ __launch_bounds__(256) void k_shadow(DevP P, Bufs B, ShadowJob J) {
+    Cnt cnt;
+    const long long n_items = (long long)B.ctr[C_HITS];
+    uint32_t clear_mask = 0;
+    for (int l = 0; l < J.n; l++) clear_mask |= (uint32_t)J.bit[l];
+    bool have = false, exhausted = false;
+    Ray64 R;
+    Refine F;
+    int pix = 0, phase = SH_SURF, l = 0, guard = 0, y = 0;
+    uint32_t sh = 0, zcode = 0;
+    double nvec[3] = {0, 0, 0}, ic[3] = {0, 0, 0};
+    double max_lhs = 0, zz2 = 0, zz2mul = 0, dt1 = 0, d = 0, last_de = 0, st = 0;
+    float rsf = 2.0f;
+    WaveChunk pq;
+    chunk_init(pq);
+    for (;;) {
+        bool need = !have && !exhausted;
+        for (int tries = 0; tries < 4 && __any(need); tries++) {
+            const long long slot = wave_take(need, &B.ctr[C_POSTQ], pq, 64, n_items);
+            if (!need) continue;
+            if (slot < 0) {
+                exhausted = true;
+                need = false;
+                continue;
+            }
+            pix = B.hits[slot];
+            if (pix < 0) continue;
+            need = false;
+            const uint32_t *w = B.gbuf + 5 * (size_t)pix;
+            if ((w[2] >> 16) < 32768u && (w[3] & 0xFFFF) < 32768u) {
+                have = true;
+                sh = (w[1] >> 16) & ~clear_mask;
+                const double n0 = (double)(int16_t)(w[0] & 0xFFFF) / 32767.0;
+                const double n1 = (double)(int16_t)(w[0] >> 16) / 32767.0;
+                const double n2 = (double)(int16_t)(w[1] & 0xFFFF) / 32767.0;
+                for (int k = 0; k < 3; k++) nvec[k] = n0 * P.vg[0][k] + n1 * P.vg[1][k] + n2 * P.vg[2][k];
+                const int x = pix % P.width;
+                y = P.row0 + pix / P.width;
+                zcode = w[2];
+                ray_from_code(P, R, x, y, zcode);
+                phase = SH_SURF;
+            }
+        }
+        if (__all(exhausted)) break;
+        if (!have) continue;
+        post_de<KIND, DEM, F32>(P, R, phase == SH_SURF || phase == SH_REF, cnt);
+        bool done = false;
+        switch (phase) {
+            case SH_SURF: {
+                const bool de_limited = R.it < R.max_its || R.de < (double)R.ms;
+                const double zf = (double)(zcode >> 8);
+                const bool more = de_limited ? F.start_bs(R, P, R.de, depth_of_code(P, zf, 8388351.9) - R.mzz,
+                                                          P.de_add_steps)
+                                             : F.start_bsi(R, P.de_add_steps);
+                if (more) {
+                    phase = SH_REF;
+                    goto end;
+                }
+                goto start;
+            }
+            case SH_REF:
+                if (F.after(R, P)) goto end;
+                goto start;
+            case SH_D0:
+                d = R.de;
+                goto head;
+            default:  // SH_DS
+                d = R.de;
+                if (R.it >= R.max_its || d <= (double)R.ms) goto endl;
+                if (d > last_de + st) d = last_de + st;
+                if (last_de > d + 1e-30) {
+                    const float t = (float)(st / (last_de - d));
+                    rsf = t < 1.0f ? fmaxf(t, 0.5f) : 1.0f;
+                } else {
+                    rsf = 1.0f;
+                }
+                if (dt1 < 0.0) goto endl;
+                goto head;
+        }
+    start: {
+        // hard_shadow_row from the surface point (src/calc.rs:2793-2800)
+        R.mzz = fmax(R.mzz - 0.1, 0.0);
+        for (int k = 0; k < 3; k++) R.c[k] += R.v[k] * -0.1;
+        R.upd(P);
+        for (int k = 0; k < 3; k++) ic[k] = R.c[k];
+        // hs_max_len (src/calc.rs:2767-2775)
+        const float mz = (float)fmin(R.mzz, P.zend * 0.4);
+        max_lhs = (double)((float)(P.width + y) * 0.6f *
+                           (1.0f + 0.5f * mz * (float)fmax(P.fov_y, 0.0) / (float)P.height) * J.max_len_mul);
+        l = 0;
+    }
+    light:
+        if (l >= J.n) goto write;
+        {
+            const double *lv = J.vec[l];
+            for (int k = 0; k < 3; k++) R.c[k] = ic[k];
+            zz2 = R.mzz;
+            const double ll = lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2];
+            const double vv = R.v[0] * R.v[0] + R.v[1] * R.v[1] + R.v[2] * R.v[2];
+            zz2mul = -((lv[0] * R.v[0] + lv[1] * R.v[1] + lv[2] * R.v[2]) / fmax(sqrt(ll * vv), 1e-300));
+            if (nvec[0] * lv[0] + nvec[1] * lv[1] + nvec[2] * lv[2] > 0.0) {
+                sh |= (uint32_t)J.bit[l];
+                l++;
+                goto light;
+            }
+            dt1 = max_lhs;
+            if (!(dt1 > 0.0)) goto endl;
+            R.ms = (float)((double)P.de_stop * (1.0 + fabs(zz2) * (double)P.de_stop_factor));
+            rsf = 2.0f;
+            guard = 0;
+            phase = SH_D0;
+            goto end;
+        }
+    head:
+        if (++guard > 1000000) goto endl;
+        {
+            const double *lv = J.vec[l];
+            last_de = d;
+            const float a = fmaxf(0.11f, (float)((d - (double)P.ms_de_sub * (double)R.ms) * (double)P.z_step_div *
+                                                 (double)rsf));
+            const float b = fmaxf(R.ms, 0.4f) * P.mh04zsd;
+            st = (double)fminf(a, b);
+            dt1 -= st;
+            for (int k = 0; k < 3; k++) R.c[k] += lv[k] * -st;
+            zz2 += st * zz2mul;
+            R.ms = (float)((double)P.de_stop * (1.0 + fabs(zz2) * (double)P.de_stop_factor));
+            cnt.steps++;
+            phase = SH_DS;
+            goto end;
+        }
+    endl:
+        if (dt1 > 0.0) sh |= (uint32_t)J.bit[l];
+        l++;
+        goto light;
+    write: {
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        w[1] = (w[1] & 0xFFFFu) | (sh << 16);
+        done = true;
+    }
+    end:
+        if (done) have = false;
+    }
+    flush_counters(cnt, B.ctr);
+}
+
+// normal_matrix (src/deao.rs:42-58)
+__device__ inline void normal_matrix(const float n[3], float mo[3][3]) {
+    const double a = acos(fmin(fmax(-(double)n[2], -1.0), 1.0)) * 0.5;
+    double sa = sin(a);
+    const double ca = cos(a);
+    const double nn = sqrt((double)(n[1] * n[1] + n[0] * n[0]));
+    double q0 = 0, q1 = 0, q2 = 0;
+    if (!(nn < 1e-25)) {
+        sa /= nn;
+        q0 = -(double)n[1] * sa;
+        q1 = (double)n[0] * sa;
+    }
+    const double q3 = ca;
+    mo[0][0] = (float)(1.0 - 2.0 * (q1 * q1 + q2 * q2));
+    mo[0][1] = (float)(2.0 * (q0 * q1 + q2 * q3));
+    mo[0][2] = (float)(2.0 * (q0 * q2 - q1 * q3));
+    mo[1][0] = (float)(2.0 * (q0 * q1 - q2 * q3));
+    mo[1][1] = (float)(1.0 - 2.0 * (q0 * q0 + q2 * q2));
+    mo[1][2] = (float)(2.0 * (q2 * q1 + q0 * q3));
+    mo[2][0] = (float)(2.0 * (q0 * q2 + q1 * q3));
+    mo[2][1] = (float)(2.0 * (q1 * q2 - q0 * q3));
+    mo[2][2] = (float)(1.0 - 2.0 * (q0 * q0 + q1 * q1));
+}
+
+enum AoPhase : int { AO_SURF, AO_REF, AO_D };
+
+template <int KIND, int DEM, bool F32>
+__global__ __launch_bounds__(256) void k_deao(DevP P, Bufs B, AoRays A) {
+    __shared__ float s_min_ra[256 * 33];
+    float *min_ra = s_min_ra + threadIdx.x;  // min_ra[k * 256]
+    Cnt cnt;
+    const long long n_items = (long long)B.ctr[C_HITS];
+    const double sw = P.step_width;
+    const double s_max_d =
+        (double)A.max_len * 0.5 * sqrt((double)P.height * (double)P.height + (double)P.width * (double)P.width);
+    const double de_stop = (double)P.de_stop;
+    const double dsm = (double)A.d_step_mul;
+    const int rc = A.count;
+    const double de_mul = sqrt((double)rc * 0.5);
+    const double abr = 1.2 / asin(fmin(fmax(1.0 / de_mul, -1.0), 1.0));
+    // the camera rows scaled to the step width (normalise_matrix_to)
+    double rot_v[3][3];
+    for (int i = 0; i < 3; i++) {
+        const double dd = sw / sqrt(P.vg[i][0] * P.vg[i][0] + P.vg[i][1] * P.vg[i][1] + P.vg[i][2] * P.vg[i][2]);
+        for (int j = 0; j < 3; j++) rot_v[i][j] = P.vg[i][j] * dd;
+    }
+    bool have = false, exhausted = false;
+    Ray64 R;
+    Refine F;
+    int pix = 0, phase = AO_SURF, k = 0, guard = 0;
+    uint32_t zcode = 0;
+    float rw[3][3];
+    double ic[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+    double ms_ao = 0, step_ao = 0, max_dist = 0, md_d10 = 0, dt1 = 0, s_tmp = 0;
+    bool b_end = false;
+    WaveChunk aq;
+    chunk_init(aq);
+    for (;;) {
+        bool need = !have && !exhausted;
+        for (int tries = 0; tries < 4 && __any(need); tries++) {
+            const long long slot = wave_take(need, &B.ctr[C_AOQ], aq, 64, n_items);
+            if (!need) continue;
+            if (slot < 0) {
+                exhausted = true;
+                need = false;
+                continue;
+            }
+            pix = B.hits[slot];
+            if (pix < 0) continue;
+            need = false;
+            uint32_t *w = B.gbuf + 5 * (size_t)pix;
+            if ((w[2] >> 16) < 32768u && (w[3] & 0xFFFF) < 32768u) {
+                have = true;
+                zcode = w[2];
+                const float nf[3] = {(float)(int16_t)(w[0] & 0xFFFF), (float)(int16_t)(w[0] >> 16),
+                                     (float)(int16_t)(w[1] & 0xFFFF)};
+                const float nl = fmaxf(sqrtf(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]), 1e-30f);
+                const float nu[3] = {nf[0] / nl, nf[1] / nl, nf[2] / nl};
+                normal_matrix(nu, rw);
+                ray_from_code(P, R, pix % P.width, P.row0 + pix / P.width, zcode);
+                phase = AO_SURF;
+            } else {
+                w[4] = w[4] & ~0xFFFFu;
+            }
+        }
+        if (__all(exhausted)) break;
+        if (!have) continue;
+        post_de<KIND, DEM, F32>(P, R, phase != AO_D, cnt);
+        bool done = false;
+        switch (phase) {
+            case AO_SURF: {
+                // surface_point (src/calc.rs:2172-2188): 5 steps to |DE - stop| <= 0.004
+                const double dd = fmin(R.de, (double)R.ms * 2.0);
+                const double zf = (double)(zcode >> 8);
+                bool more;
+                if (R.it < R.max_its || dd < (double)R.ms)
+                    more = F.start_sp(R, P, dd, depth_of_code(P, zf, 8388351.9) - R.mzz, 5, 0.004);
+                else
+                    more = F.start_bsi(R, 5);
+                if (more) {
+                    phase = AO_REF;
+                    goto end;
+                }
+                goto refined;
+            }
+            case AO_REF:
+                if (F.after(R, P)) goto end;
+                goto refined;
+            default: {  // AO_D
+                const double dt2 = R.de;
+                s_tmp = fmin(s_tmp, (dt2 - ms_ao + dt1 * md_d10) / dt1);
+                if (s_tmp < 0.02) goto ray_end;
+                dt1 += fmax(dt2, dt1 * dsm);
+                if (b_end || ++guard >= 100000) goto ray_end;
+                goto ray_step;
+            }
+        }
+    refined:
+        if (F.kind == RK_BSI)
+            R.ms = (float)((double)P.de_stop * (1.0 + fabs(R.mzz) * (double)P.de_stop_factor));
+        // deao_row after surface_point (src/deao.rs:303-345)
+        ms_ao = fmin((double)R.ms, 1e6);
+        for (int q = 0; q < 3; q++) ic[q] = R.c[q];
+        step_ao = ms_ao / de_stop;
+        max_dist = s_max_d * sqrt(step_ao);
+        ms_ao = P.de_stop_factor == 0.0f ? de_stop / (dsm * dsm) : ms_ao / (dsm * dsm);
+        md_d10 = 0.1 / (max_dist * de_mul);
+        k = 0;
+    ray:
+        if (k >= rc) goto write;
+        {
+            const float *dr = A.dir[k];
+            const float v0 = rw[0][0] * dr[0] + rw[0][1] * dr[1] + rw[0][2] * dr[2];
+            const float v1 = rw[1][0] * dr[0] + rw[1][1] * dr[1] + rw[1][2] * dr[2];
+            const float v2 = rw[2][0] * dr[0] + rw[2][1] * dr[1] + rw[2][2] * dr[2];
+            for (int q = 0; q < 3; q++)
+                sv[q] = (double)v0 * rot_v[0][q] + (double)v1 * rot_v[1][q] + (double)v2 * rot_v[2][q];
+            dt1 = step_ao * dsm;
+            s_tmp = 1.0;
+            b_end = false;
+            guard = 0;
+        }
+    ray_step:
+        if (dt1 > max_dist) {
+            dt1 = max_dist;
+            b_end = true;
+        }
+        for (int q = 0; q < 3; q++) R.c[q] = ic[q] + sv[q] * dt1;
+        cnt.steps++;
+        phase = AO_D;
+        goto end;
+    ray_end:
+        min_ra[k * 256] = (float)(fmax(s_tmp, 0.0) * de_mul);
+        k++;
+        goto ray;
+    write: {
+        double amount = 0.0;
+        for (int a = 0; a < rc; a++) {
+            float s_add = 0.0f;
+            const float ma = min_ra[a * 256];
+            if (ma < 1.0f) {
+                const float max_add = 1.0f - ma;
+                for (int b = 0; b < rc; b++) {
+                    if (b == a) continue;
+                    const float dt = A.dir[a][0] * A.dir[b][0] + A.dir[a][1] * A.dir[b][1] + A.dir[a][2] * A.dir[b][2];
+                    if (dt > A.d_min_a_dif) {
+                        const float ac = acosf(fminf(fmaxf(dt, -1.0f), 1.0f));
+                        const float ov = min_ra[b * 256] - (float)((double)ac * abr) + 1.0f;
+                        if (ov > 0.0f) s_add += fminf(max_add, ov) * A.correction_weight;
+                    }
+                }
+            }
+            amount += (double)fminf(s_add + ma, 1.0f);
+        }
+        const double av = rte(16383.0 * (1.0 - amount / (double)rc));
+        uint32_t *w = B.gbuf + 5 * (size_t)pix;
+        w[4] = (w[4] & ~0xFFFFu) | (uint32_t)fmax(av, 0.0);
+        done = true;
+    }
+    end:
+        if (done) have = false;
+    }
+    flush_counters(cnt, B.ctr);
+}
+
+// de_at (src/calc.rs:2695-2703) at a list of points: the f64 DE in world
+// units and the iteration count
+template <int KIND>
+__global__ void k_de_at(DevP P, const double *pts, long long n, double *de, int32_t *its) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const De64 r = de64<KIND, DEM_NUMERIC>(P, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], P.max_it, 0, P.de_stop);
+        if (de) de[i] = r.de * P.step_width;
+        if (its) its[i] = r.it_result;
+    }
+}
+
+// The general kind's launches (march_gen.hip); `dem` is DEM_NUMERIC or
+// DEM_BOX, `f32` is always false for it (no FP32 iterate).
+void launch_gen_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all, int dem);
+void launch_gen_normals(int grid, hipStream_t s, const DevP &d, const Bufs &b, int dem);
+void launch_gen_shadow(int grid, hipStream_t s, const DevP &d, const Bufs &b, const ShadowJob &j, int dem);
+void launch_gen_deao(int grid, hipStream_t s, const DevP &d, const Bufs &b, const AoRays &r, int dem);
+void launch_gen_de_at(int grid, hipStream_t s, const DevP &d, const double *pts, long long n, double *de,
+                      int32_t *its);
+
+}  // namespace mb3d

@@ -144,3 +144,38 @@ def test_counters_count_the_work(hip):
     r = O.render(O.params(sc), threads=0)
     # the oracle's f64 march takes the same steps within a few per cent
     assert abs(c["steps"] - r["counters"]["steps"]) / r["counters"]["steps"] < 0.05
+
+
+def test_every_pixel_a_surface_1080p(hip):
+    """A 1080p view filled by the Amazing Box, every pixel a surface: the
+    hand-off, surface and hit lists fill up (with the chunk slack per wave)
+    and the picture still meets the bar; the same frame four times in one
+    batch plan gives the same depths.
+
+    Normals: at this zoom the box's fold creases cover the view, and a
+    normal stencil that straddles a crease turns with the smallest move of
+    the surface point.  The oracle against itself, with the view moved by
+    1e-8 of the scene along the view axis, keeps 99.87 % of the normals
+    within 0.5 degrees (99.85 % at 1e-7, 99.77 % at 1e-6), so the share
+    bar here is 99.7 %; the 99th percentile stays under 0.5 degrees."""
+    sc = H.scene_preset("Amazing Box", 1920, 1080)
+    sc.z_start, sc.zoom = -9.0, 3.0
+    p, g, r, m = run(hip, sc, 4.0)
+    hit = np.isfinite(r["depth"]) & (r["depth"] > 0)
+    assert hit.mean() > 0.999 and g["counters"]["hits"] == int(hit.sum()), g["counters"]
+    assert m["mask_agree"] >= 0.999, m
+    assert m["depth_ok_frac"] >= 0.999 and m["depth_rel_p99"] <= 1e-4, m
+    assert m["normal_ok_frac"] >= 0.997 and m["normal_deg_p99"] < 0.5, m
+    assert m["iters_equal_inner"] >= 0.995, m
+    import torch
+
+    n = p.width * p.height
+    depth = torch.empty(4 * n, dtype=torch.float32, device="cuda")
+    out = H.Out(depth.data_ptr(), None, None, None, None, 1, 0)
+    frames = H.params_array([sc] * 4)
+    H.check(H.load().mb3d_hip_march_batch(frames, 4, H.C.byref(out), None), "march_batch")
+    torch.cuda.synchronize()
+    d = depth.cpu().numpy().reshape(4, n)
+    single = np.asarray(g["depth"], np.float32)
+    for f in range(4):
+        assert np.array_equal(d[f], single)
