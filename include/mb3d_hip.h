@@ -247,6 +247,110 @@ typedef struct mb3d_hip_options {
     float mlp_conf_hit, mlp_conf_miss, mlp_margin_rel, mlp_margin_abs;
 } mb3d_hip_options;
 
+/* ---- painting: the G-buffer lit and coloured (src/lighting.rs; the
+ *      reference paints on the CPU only, src/gpu.rs:6) ---- */
+#define MB3D_HIP_PAINT_U8 0  /* 8 bit per channel: paint (src/render.rs:2499) */
+#define MB3D_HIP_PAINT_U16 1 /* 16 bit: paint16 (src/render.rs:2512) */
+#define MB3D_HIP_PAINT_F32 2 /* float, 1 = white, not clipped above white: paint_f32 (src/render.rs:2518) */
+
+/* One light of the lighting record (`Light`, src/lighting.rs:95-123). */
+typedef struct mb3d_hip_light {
+    int32_t on;
+    int32_t spec_func;          /* specular exponent 2 << spec_func (0..7) */
+    int32_t diff_func;          /* diffuse function 0..3 */
+    int32_t relative_to_object; /* angles relative to the object (rotated with the view) */
+    int32_t hs_enabled;         /* hard shadows enabled for this light */
+    int32_t positional;         /* `position` instead of the angles */
+    int32_t visible;            /* visible light source: 0 none, 2, 4 (old style), 6, 8.. (glow) */
+    int32_t map;                /* light map number: 0 (light maps are not supported) */
+    uint8_t color[3];
+    uint8_t reserved;
+    float amplitude;
+    double x_angle, y_angle;    /* radians */
+    double position[3];         /* scene coordinates of a positional light */
+} mb3d_hip_light;
+
+/* A surface colour of the palette (`PaletteColor`, src/lighting.rs:149-154). */
+typedef struct mb3d_hip_palette_color {
+    uint16_t position;
+    uint8_t diffuse[3];
+    uint8_t specular[3];
+} mb3d_hip_palette_color;
+
+/* The lighting record (`Lighting`, src/lighting.rs:159-288) with the
+ * scene's shadow state applied (effective_lighting, src/render.rs:2275-2283:
+ * hs_calced, hs_soft, hs_set_cos).  Defaults: mb3d_hip_lighting_default.
+ * Light maps, background pictures, the diffuse colour map and the
+ * background as ambient light are not supported (set-up refuses them). */
+typedef struct mb3d_hip_lighting {
+    mb3d_hip_light lights[MB3D_HIP_MAX_LIGHTS];
+    mb3d_hip_palette_color palette[10];
+    uint16_t interior_pos[4];   /* the 4 interior colours: positions, */
+    uint8_t interior_col[4][3]; /* colours */
+    uint8_t interior_spec[4];   /* and specular amounts */
+    uint8_t amb_top[3], amb_bottom[3];
+    uint8_t depth_col[3], depth_col2[3];   /* depth colour at the top / bottom of the picture */
+    uint8_t dyn_fog_col[3], dyn_fog_col2[3];
+    uint8_t palette_alpha[10];  /* used when palette_alpha_set (else the brightest specular component) */
+    uint8_t fine_col_adj[2];    /* used when fine_col_adj_on: 0..60, 30 neutral */
+    uint8_t reserved_b[2];
+    float fog_offset, fog_far_offset, depth_fog, diffuse, dyn_fog, specular, ambient;
+    float color_start, color_end, amb_shadow, ind_light, gamma, roughness, var_col_z;
+    float interior_start, interior_end, diffuse_shadowing;
+    int32_t color_cycling, color_on_otrap, far_fog, depth_func, fine_col_adj_on;
+    int32_t hs_calced;          /* bit i: light i's hard shadow is in the G-buffer */
+    int32_t hs_soft, hs_set_cos, internal_gamma2, no_col_ipol, amb_rel_obj;
+    int32_t dfog_options, ex_mode, palette_alpha_set;
+    int32_t vol_light;          /* -1: none; else the light whose volumetric light the shadow words hold
+                                   (Scene::vol_light, src/scene.rs:514) */
+    int32_t has_amb_frame;      /* amb_frame / amb_fov (Lighting::amb_frame) */
+    int32_t bg_image, diff_map, bg_ambient; /* unsupported features: 0 */
+    int32_t reserved;
+    double amb_frame[3][3];
+    double amb_fov;
+} mb3d_hip_lighting;
+
+/* One derived light (`LightVal`, src/lighting.rs:582-603). */
+typedef struct mb3d_hip_light_val {
+    int32_t idx, on, sub_amb_sh, hs_calced;
+    int32_t hs_mask;            /* 0x400 << idx, or 0xFFFF for the soft shadow */
+    int32_t pow_func, diff_func, positional, visible;
+    float ln[3];                /* view-space direction, or position relative to the scene middle */
+    float col[3];
+    float lmax_l, pos_z, pos_lp;
+} mb3d_hip_light_val;
+
+/* The painter's derived light values (`LightVals`, src/lighting.rs:605-698)
+ * for the supported features, lights in paint order (SortLights).  Filled
+ * by mb3d_hip_paint_setup, or by the caller from its own LightVals. */
+typedef struct mb3d_hip_light_vals {
+    int32_t n_lights;
+    int32_t col_cycling, col_on_otrap, far_fog, depth_func, gamma_h, sqr, no_col_ipol;
+    int32_t amb_rel_obj, has_amb_frame, dfog_options, ex_mode, vol_light, has_ground;
+    int32_t col_pos[10];
+    int32_t icol_pos[4];
+    mb3d_hip_light_val lights[MB3D_HIP_MAX_LIGHTS];
+    float amb_col[3], amb_col2[3], depth_col[3], depth_col2[3], dyn_fog_col[3], dyn_fog_col2[3];
+    float s_depth, s_shad, s_shad_gr, s_shad_zmul, s_dyn_fog_mul, s_amb_shad, s_diff, s_spec;
+    float s_ind_light_reflect, s_col_zmul, s_c_start, s_c_mul, s_ci_start, s_ci_mul, s_roughness_factor;
+    float s_diffuse_shadowing, s_gamma;
+    float col_dif[10][3], col_spe[10][3], col_spe_a[10], s_c_div[10];
+    float col_int[4][3], s_ic_div[4], col_int_spec[4];
+    float amb_frame[3][3], amb_fov;
+    float ground_dif[3], ground_spe[3]; /* the ground plane's colours (LightVals::set_ground) */
+} mb3d_hip_light_vals;
+
+/* The painter's camera (`PaintCamera`, src/lighting.rs:726-745). */
+typedef struct mb3d_hip_paint_camera {
+    int32_t width, height;
+    int32_t planar;             /* optic: 0 common, 1 rectilinear, 2 panorama, 3 fisheye */
+    int32_t reserved;
+    float fov, aspect, x_off, pl_optic_z;
+    double zcorr, zc_mul, step_width, zz_stmit_dif;
+    float m[3][3];              /* unit view matrix (rows: picture x, y, view direction) */
+    float start_pos[3], x_add[3], y_add[3];
+} mb3d_hip_paint_camera;
+
 /* Work counters of the last call on this thread: the algorithmic work that
  * the roofline figures are computed from (each lane counts its own). */
 typedef struct mb3d_hip_counters {
@@ -387,6 +491,34 @@ int32_t mb3d_hip_mlp_infer(mb3d_hip_mlp *net, const float *rays, int64_t n, int3
 
 /* The work counters of the last march/post call on this thread. */
 int32_t mb3d_hip_counters_get(mb3d_hip_counters *out);
+
+/* ---- painting ---- */
+
+/* Lighting::default (src/lighting.rs:290-392): MB3D's start-up light
+ * preset; vol_light -1. */
+void mb3d_hip_lighting_default(mb3d_hip_lighting *lighting);
+
+/* The painter's camera and light values of a scene (paint_camera and
+ * light_vals, src/render.rs:2340-2380 and 2285-2319, with LightVals::new,
+ * set_vol_light, rotate_object_lights and place_lights,
+ * src/lighting.rs:963-1320), for the common optic without stereo.
+ * MB3D_HIP_UNSUPPORTED (reason in mb3d_hip_last_error) for light maps,
+ * background pictures, the diffuse map or the background as ambient. */
+int32_t mb3d_hip_paint_setup(const mb3d_hip_scene *scene, const mb3d_hip_params *params,
+                             const mb3d_hip_lighting *lighting, mb3d_hip_paint_camera *camera,
+                             mb3d_hip_light_vals *light_vals);
+
+/* The colours of the pixels of rectangle (x0, y0, w, h) of the picture
+ * from their G-buffer records (5 words per pixel, the card record of
+ * src/gpu.rs:3452-3463, row-major, w per row): LightVals::pixel_color_s
+ * (src/lighting.rs:1739-1760, shade 1800-2150) for every pixel, as
+ * paint_core's loop (src/render.rs:2718-2736).  `rgb` holds 3 channels per
+ * pixel in `format` (MB3D_HIP_PAINT_*).  on_device: both buffers are
+ * device memory of the library's device (the G-buffer as the march left
+ * it), else host memory. */
+int32_t mb3d_hip_paint(const mb3d_hip_paint_camera *camera, const mb3d_hip_light_vals *light_vals,
+                       const uint32_t *gbuf, int32_t x0, int32_t y0, int32_t w, int32_t h,
+                       int32_t format, void *rgb, int32_t on_device);
 
 #ifdef __cplusplus
 }

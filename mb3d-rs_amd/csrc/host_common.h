@@ -17,6 +17,7 @@ namespace mb3d {
 std::mutex &lib_mutex();
 hipStream_t lib_stream();
 int lib_cus();
+int lib_device();  // -1 before init
 
 // Records `msg` as this thread's last error and returns `code`.
 int set_error(int code, const char *msg);

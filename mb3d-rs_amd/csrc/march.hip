@@ -15,6 +15,7 @@
 
 #include "../../include/mb3d_hip.h"
 #include "host_common.h"
+#include "paint_common.h"
 #include "march_kernels.h"
 
 namespace mb3d {
@@ -97,6 +98,7 @@ int set_error(int code, const char *msg) {
 std::mutex &lib_mutex() { return g_ctx.mu; }
 hipStream_t lib_stream() { return g_ctx.device >= 0 ? g_ctx.stream : nullptr; }
 int lib_cus() { return g_ctx.cus; }
+int lib_device() { return g_ctx.device; }
 
 #define HIP_TRY(expr)                                                                             \
     do {                                                                                          \
@@ -142,8 +144,6 @@ std::string unsupported_reason(const mb3d_hip_params *p) {
     return "";
 }
 
-// needs_double (src/gpu.rs:569-576, 1019-1040): the FP32 iterate cannot
-// resolve the numeric gradient offset or the pixel step
 // The normals' stencil in FP32 (the analytic DEs): when its offset (0.15 of
 // the DE stop, src/calc.rs:1817-1818) spans at least 128 FP32 ulps of the
 // scene's coordinates, so that the rounding of the stencil's points moves a
@@ -156,6 +156,8 @@ bool fp32_stencil_ok(const mb3d_hip_params &p) {
     return noffset >= 128.0 * coord * 1.1920928955078125e-07;
 }
 
+// needs_double (src/gpu.rs:569-576, 1019-1040): the FP32 iterate cannot
+// resolve the numeric gradient offset or the pixel step
 bool needs_f64(const mb3d_hip_params &p) {
     if (kind_of(p) == K_GEN) return true;  // no FP32 iterate for these formulas
     const double coord = std::max({std::fabs(p.ystart[0]), std::fabs(p.ystart[1]), std::fabs(p.ystart[2])}) +
@@ -340,6 +342,11 @@ int ensure(void **buf, size_t *have, size_t need) {
     HIP_TRY(hipMalloc(buf, n));
     *have = n;
     return 0;
+}
+
+void *lib_work(size_t bytes, int *status) {
+    *status = ensure(&g_ctx.d_work, &g_ctx.work_bytes, bytes);
+    return *status ? nullptr : g_ctx.d_work;
 }
 
 int kind_of(const mb3d_hip_params &p) {

@@ -25,6 +25,7 @@ MAX_SLOTS = 6
 MAX_LIGHTS = 6
 
 OK, CANCELLED, UNSUPPORTED, OVERFLOW = 0, 1, 2, 3
+ERR_INVALID, ERR_DEVICE, ERR_NOT_INIT = -1, -2, -3
 FORMULA_NONE, FORMULA_INTPOW, FORMULA_AMAZINGBOX = 0, 1, 2
 FORMULA_REALPOWER, FORMULA_QUATERNION, FORMULA_TRICORN = 3, 4, 5
 FORMULA_BULBOX, FORMULA_FOLDINGINTPOW, FORMULA_AEXIONC = 6, 7, 8
@@ -253,6 +254,122 @@ class Counters(C.Structure):
     ]
 
 
+PAINT_U8, PAINT_U16, PAINT_F32 = 0, 1, 2
+
+
+class Light(C.Structure):
+    _fields_ = [
+        ("on", C.c_int32),
+        ("spec_func", C.c_int32),
+        ("diff_func", C.c_int32),
+        ("relative_to_object", C.c_int32),
+        ("hs_enabled", C.c_int32),
+        ("positional", C.c_int32),
+        ("visible", C.c_int32),
+        ("map", C.c_int32),
+        ("color", C.c_uint8 * 3),
+        ("reserved", C.c_uint8),
+        ("amplitude", C.c_float),
+        ("x_angle", C.c_double),
+        ("y_angle", C.c_double),
+        ("position", C.c_double * 3),
+    ]
+
+
+class PaletteColor(C.Structure):
+    _fields_ = [("position", C.c_uint16), ("diffuse", C.c_uint8 * 3), ("specular", C.c_uint8 * 3)]
+
+
+_LIGHTING_FLOATS = ("fog_offset", "fog_far_offset", "depth_fog", "diffuse", "dyn_fog", "specular", "ambient",
+                    "color_start", "color_end", "amb_shadow", "ind_light", "gamma", "roughness", "var_col_z",
+                    "interior_start", "interior_end", "diffuse_shadowing")
+_LIGHTING_INTS = ("color_cycling", "color_on_otrap", "far_fog", "depth_func", "fine_col_adj_on", "hs_calced",
+                  "hs_soft", "hs_set_cos", "internal_gamma2", "no_col_ipol", "amb_rel_obj", "dfog_options",
+                  "ex_mode", "palette_alpha_set", "vol_light", "has_amb_frame", "bg_image", "diff_map",
+                  "bg_ambient", "reserved")
+
+
+class Lighting(C.Structure):
+    _fields_ = [
+        ("lights", Light * MAX_LIGHTS),
+        ("palette", PaletteColor * 10),
+        ("interior_pos", C.c_uint16 * 4),
+        ("interior_col", (C.c_uint8 * 3) * 4),
+        ("interior_spec", C.c_uint8 * 4),
+        ("amb_top", C.c_uint8 * 3),
+        ("amb_bottom", C.c_uint8 * 3),
+        ("depth_col", C.c_uint8 * 3),
+        ("depth_col2", C.c_uint8 * 3),
+        ("dyn_fog_col", C.c_uint8 * 3),
+        ("dyn_fog_col2", C.c_uint8 * 3),
+        ("palette_alpha", C.c_uint8 * 10),
+        ("fine_col_adj", C.c_uint8 * 2),
+        ("reserved_b", C.c_uint8 * 2),
+    ] + [(n, C.c_float) for n in _LIGHTING_FLOATS] + [(n, C.c_int32) for n in _LIGHTING_INTS] + [
+        ("amb_frame", (C.c_double * 3) * 3),
+        ("amb_fov", C.c_double),
+    ]
+
+
+class LightVal(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("idx", "on", "sub_amb_sh", "hs_calced", "hs_mask", "pow_func",
+                                         "diff_func", "positional", "visible")] + [
+        ("ln", C.c_float * 3),
+        ("col", C.c_float * 3),
+        ("lmax_l", C.c_float),
+        ("pos_z", C.c_float),
+        ("pos_lp", C.c_float),
+    ]
+
+
+class LightVals(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("n_lights", "col_cycling", "col_on_otrap", "far_fog", "depth_func",
+                                         "gamma_h", "sqr", "no_col_ipol", "amb_rel_obj", "has_amb_frame",
+                                         "dfog_options", "ex_mode", "vol_light", "has_ground")] + [
+        ("col_pos", C.c_int32 * 10),
+        ("icol_pos", C.c_int32 * 4),
+        ("lights", LightVal * MAX_LIGHTS),
+    ] + [(n, C.c_float * 3) for n in ("amb_col", "amb_col2", "depth_col", "depth_col2", "dyn_fog_col",
+                                      "dyn_fog_col2")] + [
+        (n, C.c_float) for n in ("s_depth", "s_shad", "s_shad_gr", "s_shad_zmul", "s_dyn_fog_mul", "s_amb_shad",
+                                 "s_diff", "s_spec", "s_ind_light_reflect", "s_col_zmul", "s_c_start", "s_c_mul",
+                                 "s_ci_start", "s_ci_mul", "s_roughness_factor", "s_diffuse_shadowing", "s_gamma")
+    ] + [
+        ("col_dif", (C.c_float * 3) * 10),
+        ("col_spe", (C.c_float * 3) * 10),
+        ("col_spe_a", C.c_float * 10),
+        ("s_c_div", C.c_float * 10),
+        ("col_int", (C.c_float * 3) * 4),
+        ("s_ic_div", C.c_float * 4),
+        ("col_int_spec", C.c_float * 4),
+        ("amb_frame", (C.c_float * 3) * 3),
+        ("amb_fov", C.c_float),
+        ("ground_dif", C.c_float * 3),
+        ("ground_spe", C.c_float * 3),
+    ]
+
+
+class PaintCamera(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("planar", C.c_int32),
+        ("reserved", C.c_int32),
+        ("fov", C.c_float),
+        ("aspect", C.c_float),
+        ("x_off", C.c_float),
+        ("pl_optic_z", C.c_float),
+        ("zcorr", C.c_double),
+        ("zc_mul", C.c_double),
+        ("step_width", C.c_double),
+        ("zz_stmit_dif", C.c_double),
+        ("m", (C.c_float * 3) * 3),
+        ("start_pos", C.c_float * 3),
+        ("x_add", C.c_float * 3),
+        ("y_add", C.c_float * 3),
+    ]
+
+
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int64, C.c_int64)
 CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
@@ -294,6 +411,11 @@ SIGNATURES = {
     "mb3d_hip_mlp_train": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
                                        C.c_float, C.c_void_p]),
     "mb3d_hip_mlp_infer": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
+    "mb3d_hip_lighting_default": (None, [C.POINTER(Lighting)]),
+    "mb3d_hip_paint_setup": (C.c_int32, [C.POINTER(Scene), C.POINTER(Params), C.POINTER(Lighting),
+                                         C.POINTER(PaintCamera), C.POINTER(LightVals)]),
+    "mb3d_hip_paint": (C.c_int32, [C.POINTER(PaintCamera), C.POINTER(LightVals), C.c_void_p, C.c_int32, C.c_int32,
+                                   C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32]),
 }
 
 _lib = None
@@ -703,6 +825,43 @@ def post(p: Params, job: PostJob, gbuf, row0=0, rows=None, opts: Options = None)
                            C.byref(opts) if opts is not None else None)
     check(st, "mb3d_hip_post")
     return g
+
+
+# ---------------------------------------------------------------------------
+# painting
+
+
+def lighting_default() -> Lighting:
+    """Lighting::default (src/lighting.rs:290-392)."""
+    lt = Lighting()
+    load().mb3d_hip_lighting_default(C.byref(lt))
+    return lt
+
+
+def paint_setup(s: Scene, p: Params, lt: Lighting = None):
+    """(PaintCamera, LightVals) of the scene: paint_camera + light_vals
+    (src/render.rs:2340-2380, 2285-2319)."""
+    lt = lighting_default() if lt is None else lt
+    cam, lv = PaintCamera(), LightVals()
+    check(load().mb3d_hip_paint_setup(C.byref(s), C.byref(p), C.byref(lt), C.byref(cam), C.byref(lv)),
+          "paint_setup")
+    return cam, lv
+
+
+PAINT_DTYPES = {PAINT_U8: np.uint8, PAINT_U16: np.uint16, PAINT_F32: np.float32}
+
+
+def paint(cam: PaintCamera, lv: LightVals, gbuf, x0=0, y0=0, w=None, h=None, fmt=PAINT_U8):
+    """mb3d_hip_paint on a host G-buffer (n x 5 u32, row-major): the
+    picture as an (h, w, 3) array of the format's type."""
+    w = cam.width if w is None else w
+    h = cam.height if h is None else h
+    g = np.ascontiguousarray(gbuf, dtype=np.uint32).reshape(-1, 5)
+    if g.shape[0] != w * h:
+        raise ValueError(f"the G-buffer holds {g.shape[0]} records, the rectangle {w * h}")
+    out = np.zeros((h, w, 3), PAINT_DTYPES[fmt])
+    check(load().mb3d_hip_paint(C.byref(cam), C.byref(lv), _p(g), x0, y0, w, h, fmt, _p(out), 0), "mb3d_hip_paint")
+    return out
 
 
 def counters() -> dict:

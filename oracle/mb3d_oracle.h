@@ -3,7 +3,7 @@
  * The f64 CPU oracle: a restatement of mb3d-rs's CPU ray marcher
  * (src/calc.rs, src/iteration.rs, src/formulas.rs, src/math.rs,
  * src/deao.rs) for the built-in formulas and their alternating 3D and 4D
- * hybrids, with Julia mode.  Only the tests, the smoke check and the CPU
+ * hybrids, with Julia mode, and of its painter (src/lighting.rs).  Only the tests, the smoke check and the CPU
  * baseline leg of the benchmark load it, always as the checker.
  *
  * It shares the plain data layouts of include/mb3d_hip.h (scene,
@@ -72,6 +72,28 @@ int32_t orc_post(const mb3d_hip_params *p, const mb3d_hip_post_job *post, int32_
  * depth reached (+inf: background), its iteration count and steps. */
 double orc_march_ray(const mb3d_hip_params *p, const double origin[3], const double dir[3],
                      double zz, int32_t *its, int32_t *steps);
+
+/* ---- the painter (paint_oracle.cpp) ---- */
+
+/* Lighting::default (src/lighting.rs:290-392). */
+void orc_lighting_default(mb3d_hip_lighting *lighting);
+
+/* paint_camera and light_vals (src/render.rs:2340-2380, 2285-2319) for the
+ * common optic without stereo. */
+int32_t orc_paint_setup(const mb3d_hip_scene *scene, const mb3d_hip_params *params,
+                        const mb3d_hip_lighting *lighting, mb3d_hip_paint_camera *camera,
+                        mb3d_hip_light_vals *light_vals);
+
+/* GetCosTabVal (src/lighting.rs:501-541), and the 8 x 128 tables of cos_tabs. */
+float orc_cos_tab_val(int32_t tnr, float dotp, float rough, int32_t sqr);
+void orc_cos_tables(float *out);
+
+/* pixel_color_s (src/lighting.rs:1739-1760) for every record of the
+ * rectangle (x0, y0, w, h): 3 channels a pixel in `format`
+ * (MB3D_HIP_PAINT_*), rows interleaved over threads (<= 0: all cores). */
+int32_t orc_paint(const mb3d_hip_paint_camera *camera, const mb3d_hip_light_vals *light_vals,
+                  const uint32_t *gbuf, int32_t x0, int32_t y0, int32_t w, int32_t h, int32_t format,
+                  void *rgb, int32_t threads);
 
 #ifdef __cplusplus
 }

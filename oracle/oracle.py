@@ -51,6 +51,18 @@ def load():
     lib.orc_last_error.restype = C.c_char_p
     lib.orc_params_from_scene.argtypes = [C.POINTER(H.Scene), C.POINTER(H.Params)]
     lib.orc_params_from_scene.restype = C.c_int32
+    lib.orc_lighting_default.argtypes = [C.POINTER(H.Lighting)]
+    lib.orc_lighting_default.restype = None
+    lib.orc_paint_setup.argtypes = [C.POINTER(H.Scene), C.POINTER(H.Params), C.POINTER(H.Lighting),
+                                    C.POINTER(H.PaintCamera), C.POINTER(H.LightVals)]
+    lib.orc_paint_setup.restype = C.c_int32
+    lib.orc_cos_tab_val.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_int32]
+    lib.orc_cos_tab_val.restype = C.c_float
+    lib.orc_cos_tables.argtypes = [C.c_void_p]
+    lib.orc_cos_tables.restype = None
+    lib.orc_paint.argtypes = [C.POINTER(H.PaintCamera), C.POINTER(H.LightVals), C.c_void_p, C.c_int32, C.c_int32,
+                              C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32]
+    lib.orc_paint.restype = C.c_int32
     lib.orc_pixel_seed.argtypes = [C.c_int64, C.c_int64]
     lib.orc_pixel_seed.restype = C.c_int32
     lib.orc_iterate.argtypes = [C.POINTER(H.Formula), C.c_void_p, C.c_void_p, C.c_double, C.c_int32]
@@ -147,3 +159,46 @@ def iterate(f, v, j, ade=False, rout=0.0):
 
 def pixel_seed(y, x):
     return load().orc_pixel_seed(y, x)
+
+
+# ---------------------------------------------------------------------------
+# the painter
+
+
+def lighting_default():
+    """Lighting::default (src/lighting.rs:290-392), the oracle's restatement."""
+    lt = H.Lighting()
+    load().orc_lighting_default(C.byref(lt))
+    return lt
+
+
+def paint_setup(s, p, lt=None):
+    """(PaintCamera, LightVals): paint_camera + light_vals restated."""
+    lt = lighting_default() if lt is None else lt
+    cam, lv = H.PaintCamera(), H.LightVals()
+    st = load().orc_paint_setup(C.byref(s), C.byref(p), C.byref(lt), C.byref(cam), C.byref(lv))
+    if st != 0:
+        raise RuntimeError(f"orc_paint_setup failed ({st})")
+    return cam, lv
+
+
+def cos_tab_val(tnr, dotp, rough=0.0, sqr=False):
+    return float(load().orc_cos_tab_val(tnr, dotp, rough, int(sqr)))
+
+
+def cos_tables():
+    t = np.zeros((8, 128), np.float32)
+    load().orc_cos_tables(t.ctypes.data_as(C.c_void_p))
+    return t
+
+
+def paint(cam, lv, gbuf, x0=0, y0=0, w=None, h=None, fmt=0, threads=0):
+    """pixel_color_s over the rectangle: an (h, w, 3) array."""
+    w = cam.width if w is None else w
+    h = cam.height if h is None else h
+    g = np.ascontiguousarray(gbuf, dtype=np.uint32).reshape(-1, 5)
+    assert g.shape[0] == w * h
+    out = np.zeros((h, w, 3), {0: np.uint8, 1: np.uint16, 2: np.float32}[fmt])
+    load().orc_paint(C.byref(cam), C.byref(lv), g.ctypes.data_as(C.c_void_p), x0, y0, w, h, fmt,
+                     out.ctypes.data_as(C.c_void_p), threads)
+    return out

@@ -46,7 +46,10 @@ def test_ctypes_layouts_match_the_header(tmp_path):
     include/mb3d_hip.h: size and the offset of every field."""
     structs = {"mb3d_hip_formula": H.Formula, "mb3d_hip_scene": H.Scene, "mb3d_hip_params": H.Params,
                "mb3d_hip_post_job": H.PostJob, "mb3d_hip_out": H.Out, "mb3d_hip_options": H.Options,
-               "mb3d_hip_counters": H.Counters}
+               "mb3d_hip_counters": H.Counters, "mb3d_hip_light": H.Light,
+               "mb3d_hip_palette_color": H.PaletteColor, "mb3d_hip_lighting": H.Lighting,
+               "mb3d_hip_light_val": H.LightVal, "mb3d_hip_light_vals": H.LightVals,
+               "mb3d_hip_paint_camera": H.PaintCamera}
     lines, want = [], []
     for cname, cls in structs.items():
         lines.append(f'printf("%zu\\n", sizeof({cname}));')
