@@ -22,7 +22,9 @@ Prints one JSON line: Mpixels/s of the three paths, the roofline figure of
 the learned + batched path's dominant kernel (its algorithmic FLOPs per
 launch over its HIP-event time on the library's stream), the CPU baseline
 (the f64 oracle on the host cores) and the accuracy of the plain and the
-learned march against the oracle.
+learned march against the oracle.  Beside the metric: the painter's kernel
+on the frame's G-buffer (device time, bytes, the oracle's painter on the
+host cores).
 """
 from __future__ import annotations
 
@@ -41,6 +43,7 @@ import mb3d_hip as H  # noqa: E402
 
 PEAK_FP32 = 157.3e12  # MI355X FP32 vector, FLOP/s (published)
 PEAK_FP64 = 78.6e12   # MI355X FP64 vector, FLOP/s (published)
+PEAK_HBM = 8.0e12     # MI355X HBM3E, bytes/s (published)
 # algorithmic FLOPs per unit of work (the reference's formulas, counted by op)
 F_P8 = 60.0     # int_pow power 8 (src/formulas.rs:837-858) + the rout update
 F_BOX = 31.0    # amazing_box (src/formulas.rs:707-736) + the rout update
@@ -196,6 +199,28 @@ def main():
         "flops_per_launch": dom["flops_per_launch"],
         "kernels": [k_march, k_finish],
     }
+    # --- the painter (beside the metric): the frame's G-buffer, marched once
+    # into device memory, lit and coloured on the device by k_paint (the
+    # reference's default lighting); its HIP-event time per launch
+    gbuf = torch.empty(n * 5, dtype=torch.int32, device=dev)
+    rgb = torch.empty(n * 3, dtype=torch.uint8, device=dev)
+    out_g = H.Out(None, None, None, None, gbuf.data_ptr(), 1, 0)
+    H.check(lib.mb3d_hip_march(H.C.byref(p), None, 0, a.height, H.C.byref(out_g), H.C.byref(plain_opts),
+                               H.PROGRESS_FN(0), H.CANCEL_FN(0), None), "mb3d_hip_march")
+    cam, lv = H.paint_setup(sc, p, H.lighting_default())
+    pms = []
+    for i in range(a.warmup + max(5, a.steps)):
+        H.check(lib.mb3d_hip_paint(H.C.byref(cam), H.C.byref(lv), H.C.c_void_p(gbuf.data_ptr()), 0, 0, a.width,
+                                   a.height, H.PAINT_U8, H.C.c_void_p(rgb.data_ptr()), 1), "mb3d_hip_paint")
+        if i >= a.warmup:
+            pms.append(H.counters()["paint_ms"])
+    t_paint = float(np.mean(pms))
+    paint_bytes = n * (20 + 3)  # one 20-byte record read, 3 bytes written per pixel
+    paint = {"kernel": "k_paint<u8>", "kernel_ms": t_paint, "mpix_s": n / (t_paint * 1e-3) / 1e6,
+             "bytes_per_launch": paint_bytes, "achieved_gbs": paint_bytes / (t_paint * 1e-3) / 1e9,
+             "peak_gbs": PEAK_HBM / 1e9, "frac_hbm": paint_bytes / (t_paint * 1e-3) / PEAK_HBM}
+    paint_gbuf = gbuf.cpu().numpy().view(np.uint32).reshape(n, 5)
+
     result = {
         "metric": "Mpixels/s 1080p Mandelbulb-p8 Z-buffer; learned vs plain at equal depth error",
         "value": value,
@@ -229,9 +254,11 @@ def main():
                     "its_p8_f32": cl["its_p8_f32"] // a.batch, "train_s": train_s,
                     "train_loss": [float(curve[:20].mean()), float(curve[-100:].mean())]},
         "roofline": roofline,
+        "paint": paint,
     }
     if not a.no_cpu:
         result["cpu_baseline"], result["accuracy"] = cpu_leg(sc, p, a, learn_opts)
+        paint["cpu"] = paint_cpu_leg(cam, lv, paint_gbuf, result["cpu_baseline"]["cores"])
     net.close()
     print(json.dumps(result))
 
@@ -265,6 +292,20 @@ def cpu_leg(sc, p, a, learn_opts):
                   f"{threads} threads, rows interleaved as src/render.rs:757-803",
     }
     return cpu, acc
+
+
+def paint_cpu_leg(cam, lv, g, threads):
+    """The oracle's painter (a restatement of the reference's
+    pixel_color_s) on the host cores, on the same G-buffer."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    O.paint(cam, lv, g, threads=threads)  # the diffuse tables, once
+    t = time.perf_counter()
+    O.paint(cam, lv, g, threads=threads)
+    dt = time.perf_counter() - t
+    return {"value": g.shape[0] / dt / 1e6, "unit": "Mpixels/s", "cores": threads, "kind": "port",
+            "sample": f"one full {cam.width}x{cam.height} G-buffer, {dt:.3f} s"}
 
 
 if __name__ == "__main__":

@@ -370,6 +370,7 @@ typedef struct mb3d_hip_counters {
     double post_ms;          /* device time of the post kernels */
     double march_f32_ms;     /* of kernel_ms: the FP32 march */
     double finish_f64_ms;    /* of kernel_ms: the FP64 finish */
+    double paint_ms;         /* device time of the last mb3d_hip_paint's kernel (HIP events) */
 } mb3d_hip_counters;
 
 typedef void (*mb3d_hip_progress_fn)(void *user, int64_t done, int64_t total);
@@ -489,7 +490,8 @@ int32_t mb3d_hip_mlp_train(mb3d_hip_mlp *net, const float *rays, const float *ta
  * march's BF16 MFMA path */
 int32_t mb3d_hip_mlp_infer(mb3d_hip_mlp *net, const float *rays, int64_t n, int32_t bf16_mfma, float *out);
 
-/* The work counters of the last march/post call on this thread. */
+/* The work counters of the last march/post call on this thread (and the
+ * last paint's kernel time). */
 int32_t mb3d_hip_counters_get(mb3d_hip_counters *out);
 
 /* ---- painting ---- */

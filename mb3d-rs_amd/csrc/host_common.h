@@ -19,6 +19,9 @@ hipStream_t lib_stream();
 int lib_cus();
 int lib_device();  // -1 before init
 
+// This thread's work counters (mb3d_hip_counters_get).
+mb3d_hip_counters &thread_counters();
+
 // Records `msg` as this thread's last error and returns `code`.
 int set_error(int code, const char *msg);
 

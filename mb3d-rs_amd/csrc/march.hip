@@ -99,6 +99,7 @@ std::mutex &lib_mutex() { return g_ctx.mu; }
 hipStream_t lib_stream() { return g_ctx.device >= 0 ? g_ctx.stream : nullptr; }
 int lib_cus() { return g_ctx.cus; }
 int lib_device() { return g_ctx.device; }
+mb3d_hip_counters &thread_counters() { return t_counters; }
 
 #define HIP_TRY(expr)                                                                             \
     do {                                                                                          \
@@ -640,7 +641,9 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     const bool split = !post_only && !f64;
     if (split) HIP_TRY(hipEventElapsedTime(&ms_a, c.ev[0], c.ev[3]));
     mb3d_hip_counters &t = t_counters;
+    const double paint_ms = t.paint_ms;  // the last paint's, kept
     std::memset(&t, 0, sizeof t);
+    t.paint_ms = paint_ms;
     t.pixels = npix;
     t.steps = ctr[C_STATS + 0];
     t.de_evals_f32 = ctr[C_STATS + 1];
@@ -833,7 +836,9 @@ int plan_run(Plan &pl, int repeats) {
     if (!pl.exec && repeats == 1 && hipEventElapsedTime(&ms_a, pl.e0, pl.e1) != hipSuccess) ms_a = -1.0f;
     (void)hipGetLastError();
     mb3d_hip_counters &t = t_counters;
+    const double paint_ms = t.paint_ms;  // the last paint's, kept
     std::memset(&t, 0, sizeof t);
+    t.paint_ms = paint_ms;
     t.pixels = (uint64_t)pl.npix;
     t.steps = ctr[C_STATS + 0];
     t.de_evals_f32 = ctr[C_STATS + 1];

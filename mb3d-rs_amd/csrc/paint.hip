@@ -630,6 +630,7 @@ __global__ void __launch_bounds__(kPaintBlock) k_paint(mb3d_hip_paint_camera cam
 }
 
 int g_tab_device = -1;  // the device whose g_cos_tab holds the tables
+hipEvent_t g_ev[2] = {nullptr, nullptr};  // the kernel's time, created with the tables
 
 #define PAINT_TRY(expr)                                                                  \
     do {                                                                                 \
@@ -658,6 +659,11 @@ int paint_locked(const mb3d_hip_paint_camera *cam, const mb3d_hip_light_vals *lv
         float tab[8][128];
         paint_cos_tables(tab);
         PAINT_TRY(hipMemcpyToSymbol(HIP_SYMBOL(g_cos_tab), tab, sizeof tab, 0, hipMemcpyHostToDevice));
+        for (auto &e : g_ev) {  // events of this device
+            if (e) (void)hipEventDestroy(e);
+            e = nullptr;
+            PAINT_TRY(hipEventCreate(&e));
+        }
         g_tab_device = lib_device();
     }
     const int bpp = fmt == MB3D_HIP_PAINT_U8 ? 3 : fmt == MB3D_HIP_PAINT_U16 ? 6 : 12;
@@ -672,6 +678,7 @@ int paint_locked(const mb3d_hip_paint_camera *cam, const mb3d_hip_light_vals *lv
         d_rgb = reinterpret_cast<unsigned char *>(work + ((gb + 255) & ~(size_t)255));
         PAINT_TRY(hipMemcpyAsync(const_cast<uint32_t *>(d_g), gbuf, gb, hipMemcpyHostToDevice, st));
     }
+    PAINT_TRY(hipEventRecord(g_ev[0], st));
     const long long blocks = (npix + kPaintBlock - 1) / kPaintBlock;
     const int grid = (int)std::min<long long>(blocks, (long long)std::max(lib_cus(), 1) * 8);
     if (fmt == MB3D_HIP_PAINT_U8)
@@ -684,8 +691,12 @@ int paint_locked(const mb3d_hip_paint_camera *cam, const mb3d_hip_light_vals *lv
         hipLaunchKernelGGL(k_paint<MB3D_HIP_PAINT_F32>, dim3(grid), dim3(kPaintBlock), 0, st, *cam, *lv, d_g, x0, y0,
                            w, npix, d_rgb);
     PAINT_TRY(hipGetLastError());
+    PAINT_TRY(hipEventRecord(g_ev[1], st));
     if (!on_device) PAINT_TRY(hipMemcpyAsync(rgb, d_rgb, (size_t)npix * bpp, hipMemcpyDeviceToHost, st));
     PAINT_TRY(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    PAINT_TRY(hipEventElapsedTime(&ms, g_ev[0], g_ev[1]));
+    thread_counters().paint_ms = ms;
     return MB3D_HIP_OK;
 }
 

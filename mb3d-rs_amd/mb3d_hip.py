@@ -251,6 +251,7 @@ class Counters(C.Structure):
         ("post_ms", C.c_double),
         ("march_f32_ms", C.c_double),
         ("finish_f64_ms", C.c_double),
+        ("paint_ms", C.c_double),
     ]
 
 
