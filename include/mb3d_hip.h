@@ -494,6 +494,19 @@ int32_t mb3d_hip_mlp_infer(mb3d_hip_mlp *net, const float *rays, int64_t n, int3
  * last paint's kernel time). */
 int32_t mb3d_hip_counters_get(mb3d_hip_counters *out);
 
+/* ---- the G-buffer's file records ---- */
+
+/* gbuffer::pack / unpack (src/gbuffer.rs:60-98): n card records (5 u32
+ * each, as mb3d_hip_out.gbuf) to and from n 18-byte little-endian records
+ * -- normal[3] i16, zpos_fine u32, shadow, amb_shadow, si_gradient, otrap
+ * u16 -- the pixel block of an .m3i (write_m3i, src/m3p.rs:1846-1853).
+ * The card record's overflow flag (bit 16 of the last word) is not kept.
+ * on_device: both buffers are device memory of the library's device, else
+ * host memory. */
+#define MB3D_HIP_REC_BYTES 18
+int32_t mb3d_hip_gbuf_pack(const uint32_t *gbuf, int64_t n, uint8_t *records, int32_t on_device);
+int32_t mb3d_hip_gbuf_unpack(const uint8_t *records, int64_t n, uint32_t *gbuf, int32_t on_device);
+
 /* ---- painting ---- */
 
 /* Lighting::default (src/lighting.rs:290-392): MB3D's start-up light

@@ -412,6 +412,8 @@ SIGNATURES = {
     "mb3d_hip_mlp_train": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
                                        C.c_float, C.c_void_p]),
     "mb3d_hip_mlp_infer": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
+    "mb3d_hip_gbuf_pack": (C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
+    "mb3d_hip_gbuf_unpack": (C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
     "mb3d_hip_lighting_default": (None, [C.POINTER(Lighting)]),
     "mb3d_hip_paint_setup": (C.c_int32, [C.POINTER(Scene), C.POINTER(Params), C.POINTER(Lighting),
                                          C.POINTER(PaintCamera), C.POINTER(LightVals)]),
@@ -826,6 +828,26 @@ def post(p: Params, job: PostJob, gbuf, row0=0, rows=None, opts: Options = None)
                            C.byref(opts) if opts is not None else None)
     check(st, "mb3d_hip_post")
     return g
+
+
+REC_BYTES = 18
+
+
+def gbuf_pack(gbuf):
+    """mb3d_hip_gbuf_pack on host records: (n, 5) u32 -> n * 18 bytes."""
+    g = np.ascontiguousarray(gbuf, dtype=np.uint32).reshape(-1, 5)
+    out = np.zeros(g.shape[0] * REC_BYTES, np.uint8)
+    check(load().mb3d_hip_gbuf_pack(_p(g), g.shape[0], _p(out), 0), "mb3d_hip_gbuf_pack")
+    return out
+
+
+def gbuf_unpack(records):
+    """mb3d_hip_gbuf_unpack on host bytes: n * 18 bytes -> (n, 5) u32."""
+    r = np.ascontiguousarray(records, dtype=np.uint8).reshape(-1)
+    n = r.size // REC_BYTES
+    out = np.zeros((n, 5), np.uint32)
+    check(load().mb3d_hip_gbuf_unpack(_p(r), n, _p(out), 0), "mb3d_hip_gbuf_unpack")
+    return out
 
 
 # ---------------------------------------------------------------------------

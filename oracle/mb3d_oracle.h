@@ -84,6 +84,11 @@ int32_t orc_paint_setup(const mb3d_hip_scene *scene, const mb3d_hip_params *para
                         const mb3d_hip_lighting *lighting, mb3d_hip_paint_camera *camera,
                         mb3d_hip_light_vals *light_vals);
 
+/* gbuffer::pack / unpack (src/gbuffer.rs:60-98) between card records (5
+ * words) and 18-byte file records. */
+void orc_gbuf_pack(const uint32_t *gbuf, int64_t n, uint8_t *out);
+void orc_gbuf_unpack(const uint8_t *in, int64_t n, uint32_t *gbuf);
+
 /* GetCosTabVal (src/lighting.rs:501-541), and the 8 x 128 tables of cos_tabs. */
 float orc_cos_tab_val(int32_t tnr, float dotp, float rough, int32_t sqr);
 void orc_cos_tables(float *out);

@@ -56,6 +56,10 @@ def load():
     lib.orc_paint_setup.argtypes = [C.POINTER(H.Scene), C.POINTER(H.Params), C.POINTER(H.Lighting),
                                     C.POINTER(H.PaintCamera), C.POINTER(H.LightVals)]
     lib.orc_paint_setup.restype = C.c_int32
+    lib.orc_gbuf_pack.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+    lib.orc_gbuf_pack.restype = None
+    lib.orc_gbuf_unpack.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+    lib.orc_gbuf_unpack.restype = None
     lib.orc_cos_tab_val.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_int32]
     lib.orc_cos_tab_val.restype = C.c_float
     lib.orc_cos_tables.argtypes = [C.c_void_p]
@@ -201,4 +205,21 @@ def paint(cam, lv, gbuf, x0=0, y0=0, w=None, h=None, fmt=0, threads=0):
     out = np.zeros((h, w, 3), {0: np.uint8, 1: np.uint16, 2: np.float32}[fmt])
     load().orc_paint(C.byref(cam), C.byref(lv), g.ctypes.data_as(C.c_void_p), x0, y0, w, h, fmt,
                      out.ctypes.data_as(C.c_void_p), threads)
+    return out
+
+
+def gbuf_pack(gbuf):
+    """gbuffer::pack of card records: (n, 5) u32 -> n * 18 bytes."""
+    g = np.ascontiguousarray(gbuf, dtype=np.uint32).reshape(-1, 5)
+    out = np.zeros(g.shape[0] * 18, np.uint8)
+    load().orc_gbuf_pack(g.ctypes.data_as(C.c_void_p), g.shape[0], out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def gbuf_unpack(records):
+    """gbuffer::unpack into card records: n * 18 bytes -> (n, 5) u32."""
+    r = np.ascontiguousarray(records, dtype=np.uint8).reshape(-1)
+    n = r.size // 18
+    out = np.zeros((n, 5), np.uint32)
+    load().orc_gbuf_unpack(r.ctypes.data_as(C.c_void_p), n, out.ctypes.data_as(C.c_void_p))
     return out

@@ -19,6 +19,7 @@
 //                          calc_colors_alpha, palette_place, calc_colors_inside
 //                                                       src/lighting.rs:1560-1705
 //                          pos_light_shape              src/lighting.rs:899-941
+//   orc_gbuf_pack/unpack   gbuffer::pack / unpack       src/gbuffer.rs:60-98
 //
 // f32 where the reference computes in f32 (SVec = [f32; 3]), f64 where it
 // computes in f64; glibc's sinf/cosf/asinf as the reference's f32 methods
@@ -977,6 +978,40 @@ int32_t orc_paint_setup(const mb3d_hip_scene *sc, const mb3d_hip_params *p, cons
     *cam_out = cam;
     *lv_out = L;
     return 0;
+}
+
+// gbuffer::pack (src/gbuffer.rs:64-80) of the card records read as
+// march_card does (src/gpu.rs:3452-3463), and gbuffer::unpack
+// (src/gbuffer.rs:83-98) written back as card records
+void orc_gbuf_pack(const uint32_t *gbuf, int64_t n, uint8_t *out) {
+    for (int64_t i = 0; i < n; i++) {
+        const SiLight s = from_words(gbuf + 5 * i);
+        uint8_t *d = out + 18 * i;
+        auto le16 = [&](int at, uint16_t v) {
+            d[at] = (uint8_t)(v & 0xFF);
+            d[at + 1] = (uint8_t)(v >> 8);
+        };
+        for (int k = 0; k < 3; k++) le16(2 * k, (uint16_t)s.normal[k]);
+        for (int k = 0; k < 4; k++) d[6 + k] = (uint8_t)(s.zpos_fine >> (8 * k));
+        le16(10, s.shadow);
+        le16(12, s.amb_shadow);
+        le16(14, s.si_gradient);
+        le16(16, s.otrap);
+    }
+}
+
+void orc_gbuf_unpack(const uint8_t *in, int64_t n, uint32_t *gbuf) {
+    for (int64_t i = 0; i < n; i++) {
+        const uint8_t *b = in + 18 * i;
+        auto h = [&](int k) { return (uint32_t)b[k] | ((uint32_t)b[k + 1] << 8); };
+        const uint32_t zf = (uint32_t)b[6] | ((uint32_t)b[7] << 8) | ((uint32_t)b[8] << 16) | ((uint32_t)b[9] << 24);
+        uint32_t *u = gbuf + 5 * i;
+        u[0] = h(0) | (h(2) << 16);
+        u[1] = h(4) | (h(10) << 16);
+        u[2] = zf;
+        u[3] = h(14) | (h(16) << 16);
+        u[4] = h(12);
+    }
 }
 
 float orc_cos_tab_val(int32_t tnr, float dotp, float rough, int32_t sqr) { return cos_tab_val(tnr, dotp, rough, sqr != 0); }

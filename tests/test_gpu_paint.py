@@ -176,3 +176,32 @@ def test_bad_arguments_are_refused(hip):
     with pytest.raises(RuntimeError, match="diffuse"):
         H.paint(cam, bad, g)
     assert H.paint(cam, lv, np.zeros((0, 5), np.uint32), w=0, h=0).shape == (0, 0, 3)
+
+
+def test_gbuf_records_match_the_oracle(hip):
+    """mb3d_hip_gbuf_pack / unpack against the oracle's gbuffer::pack /
+    unpack on random card records (with the overflow flag set at random),
+    from host memory and on the device (also from an odd byte address)."""
+    import torch
+
+    n = 100_003
+    g = random_records(n, seed=9)
+    g[::7, 4] |= 0x10000
+    want = O.gbuf_pack(g)
+    got = H.gbuf_pack(g)
+    assert np.array_equal(got, want)
+    back = H.gbuf_unpack(got)
+    assert np.array_equal(back, O.gbuf_unpack(want))
+    assert np.array_equal(back, g & np.array([0xFFFFFFFF] * 4 + [0xFFFF], np.uint32))
+    lib = H.load()
+    dg = torch.from_numpy(g.view(np.int32).copy()).cuda()
+    raw = torch.zeros(n * 18 + 1, dtype=torch.uint8, device="cuda")
+    for off in (0, 1):
+        H.check(lib.mb3d_hip_gbuf_pack(H.C.c_void_p(dg.data_ptr()), n, H.C.c_void_p(raw.data_ptr() + off), 1), "pack")
+        torch.cuda.synchronize()
+        assert np.array_equal(raw.cpu().numpy()[off:off + n * 18], want), off
+        dback = torch.zeros(n * 5, dtype=torch.int32, device="cuda")
+        H.check(lib.mb3d_hip_gbuf_unpack(H.C.c_void_p(raw.data_ptr() + off), n, H.C.c_void_p(dback.data_ptr()), 1),
+                "unpack")
+        torch.cuda.synchronize()
+        assert np.array_equal(dback.cpu().numpy().view(np.uint32).reshape(n, 5), back), off

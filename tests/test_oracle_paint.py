@@ -414,3 +414,40 @@ def test_lighting_defaults_agree():
     assert bytes(H.lighting_default()) == bytes(O.lighting_default())
     assert H.lighting_default().vol_light == -1
     assert C.sizeof(H.Lighting) == C.sizeof(O.lighting_default())
+
+
+# ---- the G-buffer's file records (gbuffer::pack / unpack)
+
+
+def test_m3i_round_trip_record():
+    """src/m3p.rs:1671-1692: 24 x 16 default records, record 5 with normal
+    (1, -2, 3) and gradient 1234: its first 6 bytes are 1 0 254 255 3 0,
+    and the gradient comes back."""
+    g = np.zeros((24 * 16, 5), np.uint32)
+    g[:, 2] = 32768 << 16  # SiLight::default: background
+    g[5, 0] = 1 | ((-2 & 0xFFFF) << 16)
+    g[5, 1] = 3
+    g[5, 3] = 1234
+    d = O.gbuf_pack(g)
+    assert d.size == 24 * 16 * 18
+    assert list(d[5 * 18:5 * 18 + 6]) == [1, 0, 254, 255, 3, 0]
+    back = O.gbuf_unpack(d)
+    assert back[5, 3] & 0xFFFF == 1234
+    assert np.array_equal(back, g)
+
+
+def test_pack_is_the_little_endian_field_order():
+    """One record by hand: normal, zpos_fine, shadow, amb_shadow,
+    si_gradient, otrap, little endian (gbuffer::pack)."""
+    g = np.array([[0x2222_1111, 0xABCD_3333, 0x4433_2211, 0x7788_5566, 0x0001_0FED]], np.uint32)
+    d = O.gbuf_pack(g)
+    want = [0x11, 0x11, 0x22, 0x22, 0x33, 0x33, 0x11, 0x22, 0x33, 0x44, 0xCD, 0xAB, 0xED, 0x0F, 0x66, 0x55, 0x88, 0x77]
+    assert list(d) == want
+    # the card's overflow flag (bit 16 of the last word) is not a file field
+    assert O.gbuf_unpack(d)[0, 4] == 0x0FED
+
+
+def test_pack_round_trips_a_marched_gbuffer():
+    sc, p, r = _bulb(48, 36)
+    g = r["gbuf"]
+    assert np.array_equal(O.gbuf_unpack(O.gbuf_pack(g)), g & np.array([0xFFFFFFFF] * 4 + [0xFFFF], np.uint32))
