@@ -494,6 +494,46 @@ int32_t mb3d_hip_mlp_infer(mb3d_hip_mlp *net, const float *rays, int64_t n, int3
  * last paint's kernel time). */
 int32_t mb3d_hip_counters_get(mb3d_hip_counters *out);
 
+/* ---- secondary rays (the reference's reflection batches: gpu::reflect_batch,
+ *      src/gpu.rs:2521-2737, on the CPU SrTracer::march_ray,
+ *      src/reflect.rs:1464-1674) ---- */
+
+/* A ray from a surface point (ReflectTask, src/gpu.rs:2426-2441). */
+typedef struct mb3d_hip_ray_task {
+    double origin[3];   /* the point the ray leaves */
+    float dir[3];       /* its direction, one step width long */
+    float zz;           /* the view depth there (steps): the DE stop grows with it */
+    uint32_t pixel;     /* its pixel (y * width + x): the pixel's view ray projects the ray on the depth */
+    int32_t inside;     /* 1: it starts inside the set (no march: open air) */
+    float max_len;      /* the longest march (steps); 0: to the far end (zend) */
+    int32_t reserved;
+} mb3d_hip_ray_task;
+
+#define MB3D_HIP_RAY_HIT 1          /* the ray met a surface */
+#define MB3D_HIP_RAY_DE_LIMITED 2   /* the DE stopped it (not the iteration limit) */
+
+/* What a ray met (ReflectHit, src/gpu.rs:2447-2457; RayHit,
+ * src/reflect.rs:324-336). */
+typedef struct mb3d_hip_ray_hit {
+    uint32_t gbuf[5];   /* the hit's card record: normal, depth code of the ray length, step count, colour,
+                           AO 5000 (none); open air: background with the step count */
+    int32_t flags;      /* MB3D_HIP_RAY_* */
+    double pos[3];      /* where the ray ended */
+    double zz;          /* the view depth there */
+    double zz2;         /* the ray length after the surface search */
+    double zz2_paint;   /* the ray length where the march stopped */
+    float step_count;
+    int32_t iters;      /* the iteration count at the hit */
+} mb3d_hip_ray_hit;
+
+/* Marches `n` rays (the march of a reflection batch, without its shadow and
+ * occlusion pipeline).  depth_sqrt: the hit's depth code from
+ * sqrt((zz2 + zz)^2 - zz), as the painter's far fog or internal gamma 2 ask
+ * (src/reflect.rs:1634-1638), else from zz2.  on_device: `tasks` and `hits`
+ * are device memory, else host memory. */
+int32_t mb3d_hip_march_rays(const mb3d_hip_params *params, const mb3d_hip_ray_task *tasks, int64_t n,
+                            int32_t depth_sqrt, mb3d_hip_ray_hit *hits, int32_t on_device);
+
 /* ---- the G-buffer's file records ---- */
 
 /* gbuffer::pack / unpack (src/gbuffer.rs:60-98): n card records (5 u32

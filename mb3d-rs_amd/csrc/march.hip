@@ -440,6 +440,15 @@ void launch_de_at(int grid, hipStream_t s, const DevP &d, const double *pts, lon
         hipLaunchKernelGGL((k_de_at<KIND>), dim3(grid), dim3(256), 0, s, d, pts, n, de, its);
 }
 
+template <int KIND, int DEM>
+void launch_rays(int grid, hipStream_t s, const DevP &d, const mb3d_hip_ray_task *tasks, long long n, int depth_sqrt,
+                 mb3d_hip_ray_hit *hits, unsigned long long *ctr) {
+    if constexpr (KIND == K_GEN)
+        launch_gen_rays(grid, s, d, tasks, n, depth_sqrt, hits, ctr, DEM);
+    else
+        hipLaunchKernelGGL((k_rays<KIND, DEM>), dim3(grid), dim3(256), 0, s, d, tasks, n, depth_sqrt, hits, ctr);
+}
+
 // dispatch over the (formula kind, DE mode) pairs that exist
 template <class F>
 void dispatch(int kind, int dem, F &&f) {
@@ -994,6 +1003,69 @@ extern "C" int32_t mb3d_hip_de_at(const mb3d_hip_params *params, const double *p
         return MB3D_HIP_OK;
     } catch (...) {
         return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_de_at");
+    }
+}
+
+extern "C" int32_t mb3d_hip_march_rays(const mb3d_hip_params *params, const mb3d_hip_ray_task *tasks, int64_t n,
+                                       int32_t depth_sqrt, mb3d_hip_ray_hit *hits, int32_t on_device) {
+    try {
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        Ctx &c = g_ctx;
+        if (c.device < 0) return set_error(MB3D_HIP_ERR_NOT_INIT, "mb3d_hip_init was not called");
+        if (!params || !tasks || !hits || n < 0) return set_error(MB3D_HIP_ERR_INVALID, "bad argument");
+        const std::string why = unsupported_reason(params);
+        if (!why.empty()) return set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
+        if (n == 0) return MB3D_HIP_OK;
+        const uint32_t npix = (uint32_t)params->width * (uint32_t)params->height;
+        if (!on_device)
+            for (int64_t i = 0; i < n; i++)
+                if (tasks[i].pixel >= npix) return set_error(MB3D_HIP_ERR_INVALID, "a ray's pixel is outside the picture");
+        mb3d_hip_options o;
+        std::memset(&o, 0, sizeof o);
+        DevP d;
+        fill_devp(*params, o, 0, params->height, d);
+        const size_t tb = (size_t)n * sizeof(mb3d_hip_ray_task), hb = (size_t)n * sizeof(mb3d_hip_ray_hit);
+        const mb3d_hip_ray_task *dt = tasks;
+        mb3d_hip_ray_hit *dh = hits;
+        if (!on_device) {
+            const size_t off = (tb + 255) & ~(size_t)255;
+            if (int e = ensure(&c.d_work, &c.work_bytes, off + hb + 256)) return e;
+            dt = (const mb3d_hip_ray_task *)c.d_work;
+            dh = (mb3d_hip_ray_hit *)((char *)c.d_work + off);
+            HIP_TRY(hipMemcpyAsync((void *)dt, tasks, tb, hipMemcpyHostToDevice, c.stream));
+        }
+        hipStream_t stream = c.stream;
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, c.d_ctr, (int)C_WORDS);
+        HIP_TRY(hipEventRecord(c.ev[0], stream));
+        const int grid = (int)std::min<long long>((n + 255) / 256, (long long)c.cus * 8);
+        unsigned long long *ctr = c.d_ctr;
+        dispatch(kind_of(*params), dem_of(*params), [&](auto K, auto D) {
+            launch_rays<decltype(K)::value, decltype(D)::value>(grid, stream, d, dt, (long long)n, depth_sqrt, dh, ctr);
+        });
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c.ev[1], stream));
+        if (!on_device) HIP_TRY(hipMemcpyAsync(hits, dh, hb, hipMemcpyDeviceToHost, stream));
+        unsigned long long cw[C_WORDS];
+        HIP_TRY(hipMemcpyAsync(cw, c.d_ctr, sizeof cw, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+        mb3d_hip_counters &t = t_counters;
+        const double paint_ms = t.paint_ms;
+        std::memset(&t, 0, sizeof t);
+        t.paint_ms = paint_ms;
+        t.pixels = (uint64_t)n;
+        t.steps = cw[C_STATS + 0];
+        t.de_evals_f64 = cw[C_STATS + 2];
+        t.its_p8_f64 = cw[C_STATS + 5];
+        t.its_box_f64 = cw[C_STATS + 6];
+        t.hits = cw[C_STATS + 7];
+        t.its_other_f64 = cw[C_STATS + 11];
+        t.kernel_ms = ms;
+        t.finish_f64_ms = ms;
+        return MB3D_HIP_OK;
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_march_rays");
     }
 }
 

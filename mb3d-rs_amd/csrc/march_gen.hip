@@ -47,4 +47,12 @@ void launch_gen_de_at(int grid, hipStream_t s, const DevP &d, const double *pts,
     hipLaunchKernelGGL((k_de_at<K_GEN>), dim3(grid), dim3(256), 0, s, d, pts, n, de, its);
 }
 
+void launch_gen_rays(int grid, hipStream_t s, const DevP &d, const mb3d_hip_ray_task *tasks, long long n,
+                     int depth_sqrt, mb3d_hip_ray_hit *hits, unsigned long long *ctr, int dem) {
+    with_dem(dem, [&](auto D) {
+        hipLaunchKernelGGL((k_rays<K_GEN, decltype(D)::value>), dim3(grid), dim3(256), 0, s, d, tasks, n, depth_sqrt,
+                           hits, ctr);
+    });
+}
+
 }  // namespace mb3d

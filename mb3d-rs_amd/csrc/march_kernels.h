@@ -912,6 +912,238 @@ __global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__
 }
 
 // ---------------------------------------------------------------------------
+// secondary rays: SrTracer::march_ray (src/reflect.rs:1464-1674), the march
+// of a reflected ray from a surface point, as the reference's card batches
+// them (gpu::reflect_batch's march pipeline, src/gpu.rs:2521-2737).  One
+// task per lane, one f64 DE site per turn: the first DE, the march, the
+// bisection on the ray length (or bin_search_it when the iteration limit
+// stopped it), then the normal's seven DEs.
+
+enum RayPhase : int { RP_FIRST, RP_MARCH, RP_BIS, RP_BSI, RP_NRM };
+enum : int32_t { RH_HIT = 1, RH_DE_LIMITED = 2 };
+
+// the hit's card record (the surface part of march_ray)
+__device__ inline void ray_hit_record(const DevP &P, const Ray64 &R, const double n[3], float nn, bool de_limited,
+                                      double zz2, float last_zz, bool depth_sqrt, float step_count,
+                                      mb3d_hip_ray_hit &h) {
+    int16_t nq[3];
+    store_normal(n, nq);
+    float g;
+    if (de_limited) {
+        const float lg = (float)log((double)R.ms * P.step_width);
+        g = 32767.0f - (nn + P.d_col_plus + P.col_var * lg) * P.mcts_m;
+    } else {
+        g = 32767.0f - nn * P.mcts_m;
+    }
+    const uint32_t si_gradient = clip15(g);
+    const uint32_t otrap = clip15((float)(R.otrap * 4096.0));  // do_color, option 0
+    double ds = zz2;
+    if (depth_sqrt) {
+        const double a = zz2 + (double)last_zz;
+        ds = sqrt(fmax(a * a - (double)last_zz + 1e-30, 0.0));
+    }
+    const long long v = (long long)rte(P.zc_mul * (sqrt(fmax(ds, 0.0) * P.zcorr + 1.0) - 1.0));
+    const uint32_t z = (uint32_t)min(max(8388352ll - v, 0ll), 8388352ll);
+    const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
+    h.gbuf[0] = (uint32_t)(uint16_t)nq[0] | ((uint32_t)(uint16_t)nq[1] << 16);
+    h.gbuf[1] = (uint32_t)(uint16_t)nq[2] | (sh << 16);
+    h.gbuf[2] = z << 8;
+    h.gbuf[3] = si_gradient | (otrap << 16);
+    h.gbuf[4] = 5000;
+}
+
+template <int KIND, int DEM>
+__global__ __launch_bounds__(256) void k_rays(DevP P, const mb3d_hip_ray_task *__restrict__ tasks, long long n,
+                                              int depth_sqrt, mb3d_hip_ray_hit *__restrict__ hits,
+                                              unsigned long long *__restrict__ ctr) {
+    Cnt cnt;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+        const mb3d_hip_ray_task T = tasks[t];
+        mb3d_hip_ray_hit h;
+        const double dir[3] = {(double)T.dir[0], (double)T.dir[1], (double)T.dir[2]};
+        const int px = (int)(T.pixel % (uint32_t)P.width), py = (int)(T.pixel / (uint32_t)P.width);
+        double cs[3], vf[3];
+        pixel_ray(P, px, py, cs, vf);
+        const double vv = vf[0] * vf[0] + vf[1] * vf[1] + vf[2] * vf[2];
+        const double dd = dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2];
+        const double zzplus = (vf[0] * dir[0] + vf[1] * dir[1] + vf[2] * dir[2]) / sqrt(vv * dd + 1e-100);
+        double zz = (double)T.zz;
+        const float last_zz = (float)fabs(zz);
+        double zz2 = fmin((double)P.de_stop * 0.25, 1.0);
+        zz += zz2 * zzplus;
+        Ray64 R;
+        R.reset(P);
+        for (int k = 0; k < 3; k++) R.c[k] = T.origin[k] + dir[k] * zz2;
+        auto ms_at = [&](double z) { return (float)((double)P.de_stop * (1.0 + fabs(z) * (double)P.de_stop_factor)); };
+        R.ms = ms_at(zz);
+        const double max_l = T.max_len > 0.0f ? (double)T.max_len : P.zend;
+        bool open_air = true, de_limited = true;
+        float step_count = 0.0f, rsfmul = 1.0f;
+        double dtmp = 0.0, rlast_de = 0.0, rlast_sw = 0.0, zz2_paint = 0.0, bis_ds = 0.0, bsi_before = 0.0;
+        double ct[3] = {0, 0, 0}, nv[3] = {0, 0, 0}, de_a = 0.0, noffset = 0.0;
+        float nn = 0.0f;
+        int bis_left = 0, nq = 0;
+        uint32_t guard = 0;
+        Refine F;
+        int phase = RP_FIRST;
+        if (T.inside) {
+            zz2 = P.zend;
+        } else {
+            open_air = false;
+            for (;;) {
+                R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+                bool refine = false;
+                if (phase == RP_FIRST) {
+                    dtmp = R.de;
+                    rlast_sw = dtmp * (double)P.z_step_div;
+                    if (!(dtmp > 1e-10)) refine = true;
+                } else if (phase == RP_MARCH) {
+                    dtmp = R.de;
+                    if (R.it >= R.max_its) {
+                        de_limited = false;
+                        refine = true;
+                    } else if (dtmp < (double)R.ms && R.it >= P.min_it) {
+                        refine = true;
+                    } else {
+                        if (dtmp > rlast_de + rlast_sw) dtmp = rlast_de + rlast_sw;
+                        if (rlast_de > dtmp + 1e-10) {
+                            const double d = rlast_sw / (rlast_de - dtmp);
+                            rsfmul = d < 1.0 ? (float)fmax(d, 0.5) : 1.0f;
+                        } else {
+                            rsfmul = 1.0f;
+                        }
+                        if (++guard > 1000000u) {
+                            open_air = true;
+                            break;
+                        }
+                    }
+                } else if (phase == RP_BIS) {
+                    bis_ds = R.de < (double)R.ms ? fabs(bis_ds) * -0.55 : fabs(bis_ds) * 0.55;
+                    goto bis_head;
+                } else if (phase == RP_BSI) {
+                    if (F.after(R, P)) continue;
+                    zz2 = R.mzz;
+                    zz += (zz2 - bsi_before) * zzplus;
+                    goto normals;
+                } else {  // RP_NRM: the centre (smooth iterations), then vg2, vg0, vg1 both ways
+                    if (nq == 0) {
+                        nn = R.smooth_it;
+                        h.iters = R.it;
+                        R.calc_sit = false;
+                    } else {
+                        const int axis = ((nq - 1) >> 1) == 0 ? 2 : ((nq - 1) >> 1) - 1;
+                        if ((nq & 1) == 1) {
+                            de_a = R.de;
+                        } else {
+                            nv[axis] = (de_a - R.de) * 0.5;
+                        }
+                    }
+                    if (++nq == 7) {
+                        for (int k = 0; k < 3; k++) R.c[k] = ct[k];
+                        ray_hit_record(P, R, nv, nn, de_limited, zz2, last_zz, depth_sqrt != 0, step_count, h);
+                        cnt.hits++;
+                        break;
+                    }
+                    {
+                        // the point off the centre, then the other side from it (add_weight
+                        // by -2 noffset, as the reference steps)
+                        const int axis = ((nq - 1) >> 1) == 0 ? 2 : ((nq - 1) >> 1) - 1;
+                        if (nq & 1) {
+                            for (int k = 0; k < 3; k++) R.c[k] = ct[k] + P.vg[axis][k] * noffset;
+                        } else {
+                            for (int k = 0; k < 3; k++) R.c[k] += P.vg[axis][k] * (-2.0 * noffset);
+                        }
+                    }
+                    continue;
+                }
+                if (refine) {
+                    zz2_paint = zz2;
+                    if (P.de_add_steps != 0) {
+                        if (de_limited) {
+                            bis_left = P.de_add_steps;
+                            bis_ds = rlast_sw * -0.5;
+                            goto bis_head;
+                        }
+                        bsi_before = zz2;
+                        for (int k = 0; k < 3; k++) R.v[k] = dir[k];
+                        R.mzz = zz2;
+                        F.start_bsi(R, P.de_add_steps);
+                        phase = RP_BSI;
+                        continue;
+                    }
+                    goto normals;
+                }
+                {
+                    // one step of the march (src/reflect.rs:1486-1530)
+                    rlast_de = dtmp;
+                    double dstep = fmax((dtmp - (double)P.ms_de_sub * (double)R.ms) * (double)P.z_step_div * (double)rsfmul,
+                                        0.11);
+                    const double s = (double)(fmaxf(R.ms, 0.4f) * P.mh04zsd);
+                    const bool count = P.dfog_on_it == 0 || R.it == P.dfog_on_it;
+                    if (s < dstep) {
+                        if (count) step_count += (float)(s / dstep);
+                        dstep = s;
+                    } else if (count) {
+                        step_count += 1.0f;
+                    }
+                    rlast_sw = dstep;
+                    zz2 += dstep;
+                    cnt.steps++;
+                    if (zz2 > max_l) {
+                        open_air = true;
+                        break;
+                    }
+                    for (int k = 0; k < 3; k++) R.c[k] += dir[k] * dstep;
+                    zz += dstep * zzplus;
+                    R.ms = ms_at(zz);
+                    phase = RP_MARCH;
+                    continue;
+                }
+            bis_head:
+                // the bisection on the ray length (src/reflect.rs:1555-1571)
+                if (fabs(R.de - (double)R.ms) > 0.001) {
+                    zz2 += bis_ds;
+                    zz += bis_ds * zzplus;
+                    for (int k = 0; k < 3; k++) R.c[k] += dir[k] * bis_ds;
+                    R.ms = ms_at(zz);
+                    if (--bis_left > 0) {
+                        phase = RP_BIS;
+                        continue;
+                    }
+                }
+            normals:
+                // calculate_normals at the point (src/calc.rs:1813-1858)
+                R.mzz = fabs(zz);
+                noffset = (double)fminf(P.de_stop, 1.0f) * (1.0 + R.mzz * (double)P.de_stop_factor) * 0.15;
+                for (int k = 0; k < 3; k++) ct[k] = R.c[k];
+                R.calc_sit = true;
+                nq = 0;
+                phase = RP_NRM;
+            }
+        }
+        if (open_air) {
+            const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
+            h.gbuf[0] = 0;
+            h.gbuf[1] = sh << 16;
+            h.gbuf[2] = 32768u << 16;
+            h.gbuf[3] = 0;
+            h.gbuf[4] = 0;
+            zz2_paint = zz2;
+        }
+        h.flags = (open_air ? 0 : RH_HIT) | (de_limited ? RH_DE_LIMITED : 0);
+        for (int k = 0; k < 3; k++) h.pos[k] = R.c[k];
+        h.zz = zz;
+        h.zz2 = zz2;
+        h.zz2_paint = zz2_paint;
+        h.step_count = step_count;
+        if (open_air) h.iters = R.it;
+        hits[t] = h;
+    }
+    flush_counters(cnt, ctr);
+}
+
+// ---------------------------------------------------------------------------
 // the post passes: hard shadows and DE AO, same state-machine shape
 
 // The DE of a post ray: the bulk precision, and f64 where the decision is
@@ -1299,5 +1531,7 @@ void launch_gen_shadow(int grid, hipStream_t s, const DevP &d, const Bufs &b, co
 void launch_gen_deao(int grid, hipStream_t s, const DevP &d, const Bufs &b, const AoRays &r, int dem);
 void launch_gen_de_at(int grid, hipStream_t s, const DevP &d, const double *pts, long long n, double *de,
                       int32_t *its);
+void launch_gen_rays(int grid, hipStream_t s, const DevP &d, const mb3d_hip_ray_task *tasks, long long n,
+                     int depth_sqrt, mb3d_hip_ray_hit *hits, unsigned long long *ctr, int dem);
 
 }  // namespace mb3d

@@ -255,6 +255,41 @@ class Counters(C.Structure):
     ]
 
 
+RAY_HIT, RAY_DE_LIMITED = 1, 2
+
+
+class RayTask(C.Structure):
+    _fields_ = [
+        ("origin", C.c_double * 3),
+        ("dir", C.c_float * 3),
+        ("zz", C.c_float),
+        ("pixel", C.c_uint32),
+        ("inside", C.c_int32),
+        ("max_len", C.c_float),
+        ("reserved", C.c_int32),
+    ]
+
+
+class RayHit(C.Structure):
+    _fields_ = [
+        ("gbuf", C.c_uint32 * 5),
+        ("flags", C.c_int32),
+        ("pos", C.c_double * 3),
+        ("zz", C.c_double),
+        ("zz2", C.c_double),
+        ("zz2_paint", C.c_double),
+        ("step_count", C.c_float),
+        ("iters", C.c_int32),
+    ]
+
+
+# numpy views of the two records
+RAY_TASK_DTYPE = np.dtype([("origin", "<f8", 3), ("dir", "<f4", 3), ("zz", "<f4"), ("pixel", "<u4"),
+                           ("inside", "<i4"), ("max_len", "<f4"), ("reserved", "<i4")])
+RAY_HIT_DTYPE = np.dtype([("gbuf", "<u4", 5), ("flags", "<i4"), ("pos", "<f8", 3), ("zz", "<f8"), ("zz2", "<f8"),
+                          ("zz2_paint", "<f8"), ("step_count", "<f4"), ("iters", "<i4")])
+
+
 PAINT_U8, PAINT_U16, PAINT_F32 = 0, 1, 2
 
 
@@ -412,6 +447,8 @@ SIGNATURES = {
     "mb3d_hip_mlp_train": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
                                        C.c_float, C.c_void_p]),
     "mb3d_hip_mlp_infer": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
+    "mb3d_hip_march_rays": (C.c_int32, [C.POINTER(Params), C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
+                                        C.c_int32]),
     "mb3d_hip_gbuf_pack": (C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
     "mb3d_hip_gbuf_unpack": (C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
     "mb3d_hip_lighting_default": (None, [C.POINTER(Lighting)]),
@@ -828,6 +865,47 @@ def post(p: Params, job: PostJob, gbuf, row0=0, rows=None, opts: Options = None)
                            C.byref(opts) if opts is not None else None)
     check(st, "mb3d_hip_post")
     return g
+
+
+def march_rays(p: Params, tasks, depth_sqrt=False):
+    """mb3d_hip_march_rays on host records (a RAY_TASK_DTYPE array): the
+    hits as a RAY_HIT_DTYPE array."""
+    t = np.ascontiguousarray(tasks, dtype=RAY_TASK_DTYPE)
+    h = np.zeros(t.shape[0], RAY_HIT_DTYPE)
+    check(load().mb3d_hip_march_rays(C.byref(p), _p(t), t.shape[0], int(depth_sqrt), _p(h), 0), "mb3d_hip_march_rays")
+    return h
+
+
+def reflection_tasks(p: Params, g: dict, off_steps=2.0):
+    """First-bounce reflection rays of a marched picture (march()'s dict
+    with depth64 and normals): from each surface pixel, `off_steps` step
+    widths off the surface along its normal, the view ray mirrored about
+    the normal, one step width long."""
+    rays_, step = rays(p)
+    return reflection_tasks_from(p, rays_[:, :3], rays_[:, 3:] * step[:, None], g, off_steps)
+
+
+def reflection_tasks_from(p: Params, starts, vfovs, g: dict, off_steps=2.0):
+    """reflection_tasks with the pixels' start points and view vectors (one
+    march unit long) given."""
+    n = p.width * p.height
+    d = np.asarray(g["depth64"], np.float64)
+    surf = np.isfinite(d) & (d > 0)
+    idx = np.nonzero(surf)[0]
+    o = np.asarray(starts, np.float64).reshape(n, 3)[idx]
+    v = np.asarray(vfovs, np.float64).reshape(n, 3)[idx]
+    vg = np.array([list(r) for r in p.vgrads], np.float64)
+    vg /= np.linalg.norm(vg, axis=1, keepdims=True)
+    nw = np.asarray(g["normals"], np.float64).reshape(n, 3)[idx] @ vg
+    nw /= np.maximum(np.linalg.norm(nw, axis=1, keepdims=True), 1e-300)
+    pos = o + v * d[idx][:, None] + nw * (off_steps * p.step_width)
+    r = v - 2.0 * np.sum(v * nw, axis=1, keepdims=True) * nw
+    t = np.zeros(len(idx), RAY_TASK_DTYPE)
+    t["origin"] = pos
+    t["dir"] = r.astype(np.float32)
+    t["zz"] = d[idx].astype(np.float32)
+    t["pixel"] = idx.astype(np.uint32)
+    return t
 
 
 REC_BYTES = 18

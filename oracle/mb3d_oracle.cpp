@@ -347,6 +347,7 @@ struct Marcher {
     PartO part1{}, part2{}, part{};
     int32_t seed = 0;
     uint64_t de_evals = 0;
+    int nrm_it = 0;  // the iteration count of calculate_normals' centre DE
 
     explicit Marcher(const mb3d_hip_params &pp) : p(pp) {
         // CalcParams::new_iteration (src/calc.rs:1054-1087)
@@ -843,6 +844,7 @@ struct Marcher {
         double ct1[3] = {it.c[0], it.c[1], it.c[2]};
         calc_de();
         float nn = it.smooth_it;
+        nrm_it = it.it_result;
         it.calc_sit = false;
         const int order[3][2] = {{2, 2}, {0, 0}, {1, 1}};
         for (auto &ka : order) {
@@ -1934,6 +1936,161 @@ extern "C" double orc_march_ray(const mb3d_hip_params *p, const double origin[3]
     if (its) *its = si.iters;
     if (steps) *steps = si.steps;
     return si.depth;
+}
+
+// SrTracer::march_ray (src/reflect.rs:1464-1674) for one task, with the
+// pass's options at the card's: no volumetric light, no ground plane, no
+// hard shadows or occlusion at the hit (amb_shadow 5000), outside
+// rendering.  The view ray of the task's pixel (setup_ray) gives zzplus.
+static void march_ray_task(Marcher &m, const mb3d_hip_ray_task &t, bool depth_sqrt, mb3d_hip_ray_hit &h) {
+    const mb3d_hip_params &p = m.p;
+    const double sw = p.step_width;
+    const int px = (int)(t.pixel % (uint32_t)p.width), py = (int)(t.pixel / (uint32_t)p.width);
+    m.setup_ray(px, py);  // m.vfov: the pixel's view vector
+    const double new_vec[3] = {(double)t.dir[0], (double)t.dir[1], (double)t.dir[2]};
+    const double max_l = t.max_len > 0.0f ? (double)t.max_len : p.zend;
+    double zz = (double)t.zz;
+    for (int k = 0; k < 3; k++) m.it.c[k] = t.origin[k];
+    m.it.calc_sit = false;
+    m.it.max_it = p.max_it;
+    const double zzplus = dot3(m.vfov, new_vec) / std::sqrt(dot3(m.vfov, m.vfov) * dot3(new_vec, new_vec) + 1e-100);
+    const float last_zz = (float)std::fabs(zz);
+    double zz2 = std::fmin((double)p.de_stop * 0.25, 1.0);
+    zz += zz2 * zzplus;
+    add_weight(m.it.c, new_vec, zz2);
+    auto upd = [&](double z) { m.ms_de_stop = (float)((double)p.de_stop * (1.0 + std::fabs(z) * (double)p.de_stop_factor)); };
+    upd(zz);
+    bool open_air = true, de_limited = true;
+    float step_count = 0.0f;
+    double rlast_sw = 0.0;
+    if (t.inside) {
+        zz2 = p.zend;
+    } else {
+        open_air = false;
+        float rsfmul = 1.0f;
+        double dtmp = m.calc_de();
+        rlast_sw = dtmp * (double)p.z_step_div;
+        uint32_t guard = 0;
+        if (dtmp > 1e-10) {
+            for (;;) {
+                const double rlast_de = dtmp;
+                double dstep = std::fmax((dtmp - (double)p.ms_de_sub * (double)m.ms_de_stop) * (double)p.z_step_div * (double)rsfmul,
+                                         0.11);
+                const double s = (double)(std::fmax(m.ms_de_stop, 0.4f) * p.mh04zsd);
+                const bool count = p.dfog_on_it == 0 || m.it.it_result == p.dfog_on_it;
+                if (s < dstep) {
+                    if (count) step_count += (float)(s / dstep);
+                    dstep = s;
+                } else if (count) {
+                    step_count += 1.0f;
+                }
+                rlast_sw = dstep;
+                zz2 += dstep;
+                if (zz2 > max_l) {
+                    open_air = true;
+                    break;
+                }
+                add_weight(m.it.c, new_vec, dstep);
+                zz += dstep * zzplus;
+                upd(zz);
+                dtmp = m.calc_de();
+                if (m.it.it_result >= m.max_its_result) {
+                    de_limited = false;
+                    break;
+                }
+                if (dtmp < (double)m.ms_de_stop && m.it.it_result >= p.min_it) break;
+                if (dtmp > rlast_de + rlast_sw) dtmp = rlast_de + rlast_sw;
+                if (rlast_de > dtmp + 1e-10) {
+                    const double d = rlast_sw / (rlast_de - dtmp);
+                    rsfmul = d < 1.0 ? (float)std::fmax(d, 0.5) : 1.0f;
+                } else {
+                    rsfmul = 1.0f;
+                }
+                if (++guard > 1000000u) {
+                    open_air = true;
+                    break;
+                }
+            }
+        }
+    }
+    const double zz2_paint = zz2;
+    std::memset(&h, 0, sizeof h);
+    const uint16_t shadow = (uint16_t)round_even(std::fmin(std::fmax((double)step_count, 0.0), 1023.0));
+    if (!open_air) {
+        if (p.de_add_steps != 0) {
+            if (de_limited) {
+                int itmp = p.de_add_steps;
+                double ds = rlast_sw * -0.5;
+                double d = m.calc_de();
+                while (std::fabs(d - (double)m.ms_de_stop) > 0.001) {
+                    zz2 += ds;
+                    zz += ds * zzplus;
+                    add_weight(m.it.c, new_vec, ds);
+                    upd(zz);
+                    itmp -= 1;
+                    if (itmp <= 0) break;
+                    d = m.calc_de();
+                    ds = d < (double)m.ms_de_stop ? std::fabs(ds) * -0.55 : std::fabs(ds) * 0.55;
+                }
+            } else {
+                const double before = zz2;
+                double vf[3] = {m.vfov[0], m.vfov[1], m.vfov[2]};
+                for (int k = 0; k < 3; k++) m.vfov[k] = new_vec[k];
+                m.mzz = zz2;
+                m.bin_search_it();
+                zz2 = m.mzz;
+                for (int k = 0; k < 3; k++) m.vfov[k] = vf[k];
+                zz += (zz2 - before) * zzplus;
+            }
+        }
+        m.mzz = std::fabs(zz);
+        double n[3];
+        const float nn = m.calculate_normals(n);
+        h.iters = m.nrm_it;
+        Px si;
+        Marcher::store_normal(si.normal, n);
+        float v;
+        if (de_limited)
+            v = 32767.0f - (nn + p.d_col_plus + p.col_var_de_stop_mul * (float)std::log((double)m.ms_de_stop * sw)) * p.mcts_m;
+        else
+            v = 32767.0f - nn * p.mcts_m;
+        si.si_gradient = clip15(v);
+        do_color(m, si);
+        double ds = zz2;
+        if (depth_sqrt) {
+            const double a = zz2 + (double)last_zz;
+            ds = std::sqrt(std::fmax(a * a - (double)last_zz + 1e-30, 0.0));
+        }
+        const int64_t vz = (int64_t)round_even(p.zc_mul * (std::sqrt(std::fmax(ds, 0.0) * p.zcorr + 1.0) - 1.0));
+        const int64_t z = std::min<int64_t>(std::max<int64_t>(8388352 - vz, 0), 8388352);
+        si.zpos_fine = (uint32_t)z << 8;
+        si.shadow = shadow;
+        si.amb_shadow = 5000;
+        to_words(si, h.gbuf);
+    } else {
+        h.gbuf[1] = (uint32_t)shadow << 16;
+        h.gbuf[2] = 32768u << 16;
+    }
+    h.flags = (open_air ? 0 : MB3D_HIP_RAY_HIT) | (de_limited ? MB3D_HIP_RAY_DE_LIMITED : 0);
+    for (int k = 0; k < 3; k++) h.pos[k] = m.it.c[k];
+    h.zz = zz;
+    h.zz2 = zz2;
+    h.zz2_paint = zz2_paint;
+    h.step_count = step_count;
+    if (open_air) h.iters = m.it.it_result;
+}
+
+extern "C" int32_t orc_march_rays(const mb3d_hip_params *pp, const mb3d_hip_ray_task *tasks, int64_t n,
+                                  int32_t depth_sqrt, mb3d_hip_ray_hit *hits, int32_t threads) {
+    if (int e = check_params(pp)) return e;
+    const int64_t chunk = 64;
+    const int64_t nchunks = (n + chunk - 1) / chunk;
+    for_rows_interleaved((int)nchunks, nthreads(threads), [&](int c) {
+        Marcher m(*pp);
+        for (int64_t i = (int64_t)c * chunk; i < std::min(n, (int64_t)(c + 1) * chunk); i++)
+            march_ray_task(m, tasks[i], depth_sqrt != 0, hits[i]);
+    });
+    return 0;
 }
 
 extern "C" int32_t orc_render(const mb3d_hip_params *pp, const mb3d_hip_post_job *post, int32_t row0, int32_t rows,
