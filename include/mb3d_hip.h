@@ -506,8 +506,26 @@ typedef struct mb3d_hip_ray_task {
     uint32_t pixel;     /* its pixel (y * width + x): the pixel's view ray projects the ray on the depth */
     int32_t inside;     /* 1: it starts inside the set (no march: open air) */
     float max_len;      /* the longest march (steps); 0: to the far end (zend) */
-    int32_t reserved;
+    int32_t ao_q;       /* the AO quality at the hit, 0..3 (deao_1pos; the bounces take lower ones) */
 } mb3d_hip_ray_task;
+
+/* The shadow and occlusion pass at the rays' hits (ReflectOpts,
+ * src/gpu.rs:2459-2481; calc_hs and deao_1pos, src/reflect.rs:805-1090). */
+typedef struct mb3d_hip_ray_opts {
+    int32_t n_lights;                        /* the lights with hard shadows (0: none) */
+    int32_t light_idx[MB3D_HIP_MAX_LIGHTS];  /* header light index: the hit's shadow bit 0x400 << idx */
+    int32_t light_positional[MB3D_HIP_MAX_LIGHTS];
+    int32_t light_visible[MB3D_HIP_MAX_LIGHTS];
+    float light_lmax[MB3D_HIP_MAX_LIGHTS];   /* lmax_l: how far a positional light reaches */
+    double light_vec[MB3D_HIP_MAX_LIGHTS][3]; /* HSvec of a global light (world frame, -step_width long
+                                                towards the light), the scene position of a positional one */
+    float soft_radius;  /* > 0: one soft shadow of the last light, of this radius */
+    float hs_mul;       /* the shadow rays' length multiplier */
+    int32_t ao;         /* 1: DE ambient occlusion at the hit, of each task's ao_q */
+    float ao_max_len;
+    int32_t ao_first_random; /* 0 (a random first AO step is not supported) */
+    int32_t reserved;
+} mb3d_hip_ray_opts;
 
 #define MB3D_HIP_RAY_HIT 1          /* the ray met a surface */
 #define MB3D_HIP_RAY_DE_LIMITED 2   /* the DE stopped it (not the iteration limit) */
@@ -526,13 +544,17 @@ typedef struct mb3d_hip_ray_hit {
     int32_t iters;      /* the iteration count at the hit */
 } mb3d_hip_ray_hit;
 
-/* Marches `n` rays (the march of a reflection batch, without its shadow and
- * occlusion pipeline).  depth_sqrt: the hit's depth code from
+/* Marches `n` rays, then runs the shadow and occlusion pass of `opts` at
+ * their hits (NULL: none; AO 5000 and no shadow bits, as the reference
+ * without them).  depth_sqrt: the hit's depth code from
  * sqrt((zz2 + zz)^2 - zz), as the painter's far fog or internal gamma 2 ask
- * (src/reflect.rs:1634-1638), else from zz2.  on_device: `tasks` and `hits`
- * are device memory, else host memory. */
+ * (src/reflect.rs:1634-1638), else from zz2.  The AO is returned as the card
+ * returns it: the caller scales it by the bounce (amb * 8 / (8 + bounce),
+ * src/reflect.rs:2270-2275).  on_device: `tasks` and `hits` are device
+ * memory, else host memory. */
 int32_t mb3d_hip_march_rays(const mb3d_hip_params *params, const mb3d_hip_ray_task *tasks, int64_t n,
-                            int32_t depth_sqrt, mb3d_hip_ray_hit *hits, int32_t on_device);
+                            int32_t depth_sqrt, const mb3d_hip_ray_opts *opts, mb3d_hip_ray_hit *hits,
+                            int32_t on_device);
 
 /* ---- the G-buffer's file records ---- */
 

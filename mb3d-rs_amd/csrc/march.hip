@@ -449,6 +449,82 @@ void launch_rays(int grid, hipStream_t s, const DevP &d, const mb3d_hip_ray_task
         hipLaunchKernelGGL((k_rays<KIND, DEM>), dim3(grid), dim3(256), 0, s, d, tasks, n, depth_sqrt, hits, ctr);
 }
 
+template <int KIND, int DEM>
+void launch_ray_post(int grid, hipStream_t s, const DevP &d, const mb3d_hip_ray_task *tasks, long long n,
+                     const RayPost &j, mb3d_hip_ray_hit *hits, unsigned long long *ctr) {
+    if constexpr (KIND == K_GEN)
+        launch_gen_ray_post(grid, s, d, tasks, n, j, hits, ctr, DEM);
+    else
+        hipLaunchKernelGGL((k_ray_post<KIND, DEM>), dim3(grid), dim3(256), 0, s, d, tasks, n, j, hits, ctr);
+}
+
+// The hit pass's tables from the options: the lights, and per AO quality
+// 0..3 deao_1pos's ray set, step factor, overlap weight and the overlap term
+// acos(d_k . d_k2) abr of each pair near enough (src/reflect.rs:954-1090).
+int ray_post_of(const mb3d_hip_ray_opts &o, RayPost &j) {
+    std::memset(&j, 0, sizeof j);
+    if (o.n_lights < 0 || o.n_lights > MB3D_HIP_MAX_LIGHTS) return set_error(MB3D_HIP_ERR_INVALID, "bad light count");
+    if (o.soft_radius > 0.0f && o.n_lights < 1) return set_error(MB3D_HIP_ERR_INVALID, "a soft shadow needs a light");
+    if (o.ao && o.ao_first_random)
+        return set_error(MB3D_HIP_UNSUPPORTED, "a random first AO step at ray hits (not on this backend)");
+    j.n_lights = o.n_lights;
+    j.soft = o.soft_radius > 0.0f;
+    j.ao = o.ao != 0;
+    j.soft_radius = o.soft_radius;
+    j.hs_mul = o.hs_mul;
+    j.ao_max_len = o.ao_max_len;
+    for (int i = 0; i < o.n_lights; i++) {
+        if (o.light_idx[i] < 0 || o.light_idx[i] >= MB3D_HIP_MAX_LIGHTS)
+            return set_error(MB3D_HIP_ERR_INVALID, "bad light index");
+        j.bit[i] = o.light_idx[i];
+        j.positional[i] = o.light_positional[i] != 0;
+        j.visible[i] = o.light_visible[i];
+        j.lmax[i] = o.light_lmax[i];
+        for (int k = 0; k < 3; k++) j.vec[i][k] = o.light_vec[i][k];
+    }
+    const double tau = 6.283185307179586;
+    auto ray_dir = [](double ya, double za, float out[3]) {
+        out[0] = (float)(std::sin(ya) * std::cos(za));
+        out[1] = (float)(-std::sin(ya) * std::sin(za));
+        out[2] = (float)(-std::cos(ya));
+    };
+    int nd = 0, pen = 0;
+    for (int q = 0; q < 4; q++) {
+        j.first[q] = nd;
+        float dmin;
+        if (q == 0) {
+            const double abr = 60.0 * (3.141592653589793 / 180.0);
+            for (int i = 0; i < 3; i++) ray_dir(0.5 * abr, (double)i * tau / 3.0, j.dirs[nd++]);
+            j.d_step_mul[q] = 1.8;
+            dmin = -1.0f;
+            j.corr_w[q] = 0.3f;
+        } else {
+            const double abr = 1.5707963267948966 / ((double)q + 0.9);
+            ray_dir(0.0, 0.0, j.dirs[nd++]);
+            for (int iy = 1; iy <= q; iy++) {
+                const int n = (int)std::nearbyint(std::sin(0.25) * tau / abr);
+                for (int ix = 0; ix < n; ix++) ray_dir(abr * (double)iy, (double)ix * tau / (double)n, j.dirs[nd++]);
+            }
+            j.d_step_mul[q] = 1.0 + std::sin(abr);
+            dmin = (float)std::cos(abr * 1.2);
+            j.corr_w[q] = q == 1 ? 0.2f : 0.1666f;
+        }
+        const int cnt = nd - j.first[q];
+        j.count[q] = cnt;
+        j.de_mul[q] = std::sqrt((double)cnt * 0.5);
+        const double abr2 = 1.2 / std::asin(std::fmin(std::fmax(1.0 / j.de_mul[q], -1.0), 1.0));
+        j.pen_off[q] = pen;
+        for (int a = 0; a < cnt; a++)
+            for (int b = 0; b < cnt; b++) {
+                const float *da = j.dirs[j.first[q] + a], *db = j.dirs[j.first[q] + b];
+                const float dt = da[0] * db[0] + da[1] * db[1] + da[2] * db[2];
+                const float ac = std::acos(std::fmin(std::fmax(dt, -1.0f), 1.0f));  // f32 acos, as the reference
+                j.pen[pen++] = dt > dmin ? (float)((double)ac * abr2) : INFINITY;
+            }
+    }
+    return 0;
+}
+
 // dispatch over the (formula kind, DE mode) pairs that exist
 template <class F>
 void dispatch(int kind, int dem, F &&f) {
@@ -1007,7 +1083,8 @@ extern "C" int32_t mb3d_hip_de_at(const mb3d_hip_params *params, const double *p
 }
 
 extern "C" int32_t mb3d_hip_march_rays(const mb3d_hip_params *params, const mb3d_hip_ray_task *tasks, int64_t n,
-                                       int32_t depth_sqrt, mb3d_hip_ray_hit *hits, int32_t on_device) {
+                                       int32_t depth_sqrt, const mb3d_hip_ray_opts *opts, mb3d_hip_ray_hit *hits,
+                                       int32_t on_device) {
     try {
         std::lock_guard<std::mutex> lk(g_ctx.mu);
         Ctx &c = g_ctx;
@@ -1015,6 +1092,10 @@ extern "C" int32_t mb3d_hip_march_rays(const mb3d_hip_params *params, const mb3d
         if (!params || !tasks || !hits || n < 0) return set_error(MB3D_HIP_ERR_INVALID, "bad argument");
         const std::string why = unsupported_reason(params);
         if (!why.empty()) return set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
+        RayPost post;
+        const bool with_post = opts && (opts->n_lights > 0 || opts->ao);
+        if (opts)
+            if (int e = ray_post_of(*opts, post)) return e;
         if (n == 0) return MB3D_HIP_OK;
         const uint32_t npix = (uint32_t)params->width * (uint32_t)params->height;
         if (!on_device)
@@ -1041,6 +1122,9 @@ extern "C" int32_t mb3d_hip_march_rays(const mb3d_hip_params *params, const mb3d
         unsigned long long *ctr = c.d_ctr;
         dispatch(kind_of(*params), dem_of(*params), [&](auto K, auto D) {
             launch_rays<decltype(K)::value, decltype(D)::value>(grid, stream, d, dt, (long long)n, depth_sqrt, dh, ctr);
+            if (with_post)
+                launch_ray_post<decltype(K)::value, decltype(D)::value>(grid, stream, d, dt, (long long)n, post, dh,
+                                                                        ctr);
         });
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(c.ev[1], stream));

@@ -55,4 +55,12 @@ void launch_gen_rays(int grid, hipStream_t s, const DevP &d, const mb3d_hip_ray_
     });
 }
 
+void launch_gen_ray_post(int grid, hipStream_t s, const DevP &d, const mb3d_hip_ray_task *tasks, long long n,
+                         const RayPost &j, mb3d_hip_ray_hit *hits, unsigned long long *ctr, int dem) {
+    with_dem(dem, [&](auto D) {
+        hipLaunchKernelGGL((k_ray_post<K_GEN, decltype(D)::value>), dim3(grid), dim3(256), 0, s, d, tasks, n, j, hits,
+                           ctr);
+    });
+}
+
 }  // namespace mb3d

@@ -1143,6 +1143,267 @@ __global__ __launch_bounds__(256) void k_rays(DevP P, const mb3d_hip_ray_task *_
     flush_counters(cnt, ctr);
 }
 
+// The shadow and occlusion pass at the rays' hits (calc_hs and deao_1pos,
+// src/reflect.rs:805-1090; the reference's card runs it as a pipeline of its
+// own after the march, src/gpu.rs:2562-2575).  The AO ray sets of the four
+// qualities and the pairwise overlap term acos(d_k . d_k2) abr are built on
+// the host: they depend on the directions only.
+struct RayPost {
+    int n_lights, soft, ao, reserved;
+    int bit[MB3D_HIP_MAX_LIGHTS], positional[MB3D_HIP_MAX_LIGHTS], visible[MB3D_HIP_MAX_LIGHTS];
+    float lmax[MB3D_HIP_MAX_LIGHTS];
+    double vec[MB3D_HIP_MAX_LIGHTS][3];
+    float soft_radius, hs_mul, ao_max_len, pad;
+    int count[4], first[4], pen_off[4];
+    float dirs[26][3];
+    double d_step_mul[4], de_mul[4];
+    float corr_w[4];
+    float pen[236];  // per quality, count x count: acos(d_k . d_k2) abr, +inf where the pair is not near enough
+};
+constexpr int kRayAoMax = 13;
+
+__device__ inline void normal_matrix(const float n[3], float mo[3][3]);  // (with the DE AO pass below)
+
+enum RayPostPhase : int { RQ_HS_FIRST, RQ_HS, RQ_AO };
+
+template <int KIND, int DEM>
+__global__ __launch_bounds__(256) void k_ray_post(DevP P, const mb3d_hip_ray_task *__restrict__ tasks, long long n,
+                                                  RayPost J, mb3d_hip_ray_hit *__restrict__ hits,
+                                                  unsigned long long *__restrict__ ctr) {
+    __shared__ float s_min_ra[kRayAoMax][256];
+    float *min_ra = &s_min_ra[0][threadIdx.x];  // min_ra[k * 256]
+    Cnt cnt;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+        mb3d_hip_ray_hit h = hits[t];
+        if (!(h.flags & RH_HIT)) continue;
+        const mb3d_hip_ray_task T = tasks[t];
+        const int px = (int)(T.pixel % (uint32_t)P.width), py = (int)(T.pixel / (uint32_t)P.width);
+        double cs[3], vf[3];
+        pixel_ray(P, px, py, cs, vf);
+        const double sw = P.step_width;
+        const double ic[3] = {h.pos[0], h.pos[1], h.pos[2]};
+        const double mzz = fabs(h.zz);
+        const float zzf = (float)mzz;
+        const int16_t nq[3] = {(int16_t)(h.gbuf[0] & 0xFFFFu), (int16_t)(h.gbuf[0] >> 16), (int16_t)(h.gbuf[1] & 0xFFFFu)};
+        uint32_t shadow = h.gbuf[1] >> 16;
+        uint32_t amb = h.gbuf[4] & 0xFFFFu;
+        Ray64 R;
+        R.reset(P);
+        for (int k = 0; k < 3; k++) R.c[k] = ic[k];
+        auto ms_at = [&](double z) { return (float)((double)P.de_stop * (1.0 + fabs(z) * (double)P.de_stop_factor)); };
+        // calc_hs's values
+        const double max_lhs = (double)((float)(P.width + P.height) * 0.6f *
+                                        (1.0f + 0.5f * fminf(zzf, (float)P.zend * 0.4f) * fmaxf((float)P.fov_y, 0.0f) /
+                                                    (float)P.height) *
+                                        J.hs_mul);
+        double nvec[3];
+        {
+            const double nn[3] = {(double)nq[0] / 32767.0, (double)nq[1] / 32767.0, (double)nq[2] / 32767.0};
+            for (int k = 0; k < 3; k++) nvec[k] = nn[0] * P.vg[0][k] + nn[1] * P.vg[1][k] + nn[2] * P.vg[2][k];
+        }
+        int li = J.soft ? J.n_lights - 1 : 0;
+        const int li_end = J.n_lights;
+        double hsvec[3] = {0, 0, 0}, dt1 = 0.0, dmaxl = 0.0, zz2 = 0.0, zz2mul = 0.0, d = 0.0, last_de = 0.0, st = 0.0;
+        float rsfd = 1.0f, zrsoft = 1.0f, zrs_mul = 0.0f;
+        int guard = 0;
+        int phase = -1;
+        if (J.n_lights > 0 && !J.soft) {
+            uint32_t mask = 0;
+            for (int i = 0; i < J.n_lights; i++) mask |= 0x400u << J.bit[i];
+            shadow &= ~mask;
+        }
+        // AO state
+        int q = 0, nd = 0, k_ao = 0, gao = 0;
+        double ms_ao = 0.0, step_ao = 0.0, max_dist = 0.0, md_d10 = 0.0, adt = 0.0, as = 1.0, sv[3] = {0, 0, 0};
+        bool aend = false;
+        float rw[3][3];
+        for (;;) {
+            if (phase < 0) {
+                // the next light's march, or the occlusion when the lights are done
+                bool started = false;
+                while (!started && li < li_end && J.n_lights > 0) {
+                    const int i = li++;
+                    if (J.soft) shadow |= 0xFC00u;  // lit, unless the march says otherwise
+                    for (int k = 0; k < 3; k++) R.c[k] = ic[k];
+                    zz2 = mzz;
+                    R.ms = ms_at((double)zzf);
+                    double len = max_lhs;
+                    for (int k = 0; k < 3; k++) hsvec[k] = J.vec[i][k];
+                    bool reach = true;
+                    if (J.positional[i]) {
+                        double v[3];
+                        for (int k = 0; k < 3; k++) v[k] = J.vec[i][k] - R.c[k];
+                        const double dd2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                        if (dd2 > (double)(J.lmax[i] * J.hs_mul)) {
+                            reach = false;
+                        } else {
+                            if (dd2 < (len * sw) * (len * sw)) len = sqrt(dd2) / sw;
+                            const double l = sqrt(dd2);
+                            for (int k = 0; k < 3; k++) hsvec[k] = v[k] * (-sw / l);
+                        }
+                    }
+                    const uint32_t bit = 0x400u << J.bit[i];
+                    if (!reach) {
+                        if (!J.soft) shadow |= bit;
+                        continue;
+                    }
+                    zz2mul = -(hsvec[0] * vf[0] + hsvec[1] * vf[1] + hsvec[2] * vf[2]) /
+                             sqrt((hsvec[0] * hsvec[0] + hsvec[1] * hsvec[1] + hsvec[2] * hsvec[2]) *
+                                  (vf[0] * vf[0] + vf[1] * vf[1] + vf[2] * vf[2]));
+                    if (J.soft) {
+                        const float sr = fmaxf(J.soft_radius, 0.001f);
+                        zrs_mul = J.positional[i] ? ((J.visible[i] & 6) == 2 ? 70.0f / sr : 40.0f / sr) : 80.0f / sr;
+                    }
+                    if (!(len > 0.0)) continue;
+                    if (nvec[0] * hsvec[0] + nvec[1] * hsvec[1] + nvec[2] * hsvec[2] > 0.0) {
+                        if (J.soft)
+                            shadow &= 0x3FFu;
+                        else
+                            shadow |= bit;
+                        continue;
+                    }
+                    dt1 = dmaxl = len;
+                    rsfd = 1.0f;
+                    zrsoft = 1.0f;
+                    guard = 0;
+                    phase = RQ_HS_FIRST;
+                    started = true;
+                }
+                if (!started) {
+                    for (int k = 0; k < 3; k++) R.c[k] = ic[k];
+                    if (!J.ao) break;
+                    // deao_1pos's set-up
+                    q = min(max(T.ao_q, 0), 3);
+                    nd = J.count[q];
+                    R.calc_sit = false;
+                    ms_ao = (double)fminf((float)((double)P.de_stop * (1.0 + mzz * (double)P.de_stop_factor)), 1e6f);
+                    const double s_max_d = (double)J.ao_max_len * 0.5 *
+                                           sqrt((double)P.height * (double)P.height + (double)P.width * (double)P.width);
+                    step_ao = ms_ao / (double)P.de_stop;
+                    max_dist = s_max_d * sqrt(step_ao);
+                    float nf[3] = {(float)nq[0], (float)nq[1], (float)nq[2]};
+                    const float l = fmaxf(sqrtf(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]), 1e-30f);
+                    for (int k = 0; k < 3; k++) nf[k] = nf[k] / l;
+                    normal_matrix(nf, rw);
+                    const double dsm = J.d_step_mul[q];
+                    if (P.de_stop_factor != 0.0f)
+                        ms_ao /= dsm * dsm;
+                    else
+                        ms_ao = (double)P.de_stop / (dsm * dsm);
+                    md_d10 = 0.1 / (max_dist * J.de_mul[q]);
+                    k_ao = 0;
+                    phase = RQ_AO;
+                    goto ao_ray;
+                }
+            }
+            if (phase == RQ_AO && k_ao >= nd) break;
+            R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+            if (phase == RQ_HS_FIRST) {
+                d = R.de;
+                phase = RQ_HS;
+                goto hs_step;
+            } else if (phase == RQ_HS) {
+                d = R.de;
+                if (J.soft) {
+                    const float r = (float)((dmaxl - dt1) / max_lhs);
+                    const float r8 = (r * r) * (r * r);
+                    zrsoft = fminf(zrsoft, (float)((d - (double)R.ms) * (double)zrs_mul / (dmaxl - dt1 + 0.11)) + r8 * r8);
+                }
+                if (R.it >= R.max_its || d < (double)R.ms) goto hs_done;
+                if (d > last_de + st) d = last_de + st;
+                if (last_de > d + 1e-30) {
+                    const float tt = (float)(st / (last_de - d));
+                    rsfd = tt < 1.0f ? fmaxf(tt, 0.5f) : 1.0f;
+                } else {
+                    rsfd = 1.0f;
+                }
+                if (dt1 < 0.0 || guard > 1000000) goto hs_done;
+                goto hs_step;
+            } else {
+                // an AO probe's DE (deao_1pos's ray loop)
+                const double d2 = R.de;
+                as = fmin(as, (d2 - ms_ao + adt * md_d10) / adt);
+                if (as < 0.02) goto ao_next;
+                adt += fmax(d2, adt * J.d_step_mul[q]);
+                if (aend || gao > 100000) goto ao_next;
+                goto ao_probe;
+            }
+        hs_step:
+            guard += 1;
+            last_de = d;
+            st = (double)fminf((float)(d * (double)P.z_step_div * (double)rsfd), fmaxf(R.ms, 0.4f) * P.mh04zsd);
+            dt1 -= st;
+            for (int k = 0; k < 3; k++) R.c[k] += hsvec[k] * (-st);
+            zz2 += st * zz2mul;
+            R.ms = ms_at(zz2);
+            cnt.steps++;
+            continue;
+        hs_done:
+            if (J.soft) {
+                shadow = (shadow & 0x3FFu) | ((uint32_t)rtef(fminf(fmaxf(zrsoft, 0.0f), 1.0f) * 63.4f) << 10);
+                li = li_end;
+            } else if (dt1 > 0.0) {
+                shadow |= 0x400u << J.bit[li - 1];
+            }
+            phase = -1;
+            continue;
+        ao_next:
+            min_ra[k_ao * 256] = (float)(fmax(as, 0.0) * J.de_mul[q]);
+            k_ao++;
+        ao_ray:
+            if (k_ao >= nd) continue;
+            {
+                const float *dr = J.dirs[J.first[q] + k_ao];
+                const float v[3] = {rw[0][0] * dr[0] + rw[0][1] * dr[1] + rw[0][2] * dr[2],
+                                    rw[1][0] * dr[0] + rw[1][1] * dr[1] + rw[1][2] * dr[2],
+                                    rw[2][0] * dr[0] + rw[2][1] * dr[1] + rw[2][2] * dr[2]};
+                // rotate_vector_reverse by vgrads rows normalised to the step width
+                for (int k = 0; k < 3; k++) sv[k] = 0.0;
+                for (int r = 0; r < 3; r++) {
+                    const double vv = sqrt(P.vg[r][0] * P.vg[r][0] + P.vg[r][1] * P.vg[r][1] + P.vg[r][2] * P.vg[r][2]);
+                    const double sc = sw / vv;
+                    for (int k = 0; k < 3; k++) sv[k] += (double)v[r] * (P.vg[r][k] * sc);
+                }
+                adt = step_ao * J.d_step_mul[q];
+                as = 1.0;
+                aend = false;
+                gao = 0;
+            }
+        ao_probe:
+            gao += 1;
+            if (adt > max_dist) {
+                adt = max_dist;
+                aend = true;
+            }
+            for (int k = 0; k < 3; k++) R.c[k] = ic[k] + sv[k] * adt;
+        }
+        if (J.ao) {
+            // the rays' overlap correction and the AO word
+            double amount = 0.0;
+            const float *pen = J.pen + J.pen_off[q];
+            for (int k = 0; k < nd; k++) {
+                const float mk = min_ra[k * 256];
+                float s_add = 0.0f;
+                if (mk < 1.0f) {
+                    const float max_add = 1.0f - mk;
+                    for (int k2 = 0; k2 < nd; k2++) {
+                        if (k2 == k) continue;
+                        const float overlap = min_ra[k2 * 256] - pen[k * nd + k2] + 1.0f;
+                        if (overlap > 0.0f) s_add += fminf(max_add, overlap) * J.corr_w[q];
+                    }
+                }
+                amount += (double)fminf(s_add + mk, 1.0f);
+            }
+            amb = (uint32_t)fmax(rte(16383.0 * (1.0 - amount / (double)nd)), 0.0);
+        }
+        h.gbuf[1] = (h.gbuf[1] & 0xFFFFu) | ((shadow & 0xFFFFu) << 16);
+        h.gbuf[4] = amb;
+        hits[t] = h;
+    }
+    flush_counters(cnt, ctr);
+}
+
 // ---------------------------------------------------------------------------
 // the post passes: hard shadows and DE AO, same state-machine shape
 
@@ -1533,5 +1794,7 @@ void launch_gen_de_at(int grid, hipStream_t s, const DevP &d, const double *pts,
                       int32_t *its);
 void launch_gen_rays(int grid, hipStream_t s, const DevP &d, const mb3d_hip_ray_task *tasks, long long n,
                      int depth_sqrt, mb3d_hip_ray_hit *hits, unsigned long long *ctr, int dem);
+void launch_gen_ray_post(int grid, hipStream_t s, const DevP &d, const mb3d_hip_ray_task *tasks, long long n,
+                         const RayPost &j, mb3d_hip_ray_hit *hits, unsigned long long *ctr, int dem);
 
 }  // namespace mb3d

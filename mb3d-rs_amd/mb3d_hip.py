@@ -266,6 +266,23 @@ class RayTask(C.Structure):
         ("pixel", C.c_uint32),
         ("inside", C.c_int32),
         ("max_len", C.c_float),
+        ("ao_q", C.c_int32),
+    ]
+
+
+class RayOpts(C.Structure):
+    _fields_ = [
+        ("n_lights", C.c_int32),
+        ("light_idx", C.c_int32 * MAX_LIGHTS),
+        ("light_positional", C.c_int32 * MAX_LIGHTS),
+        ("light_visible", C.c_int32 * MAX_LIGHTS),
+        ("light_lmax", C.c_float * MAX_LIGHTS),
+        ("light_vec", (C.c_double * 3) * MAX_LIGHTS),
+        ("soft_radius", C.c_float),
+        ("hs_mul", C.c_float),
+        ("ao", C.c_int32),
+        ("ao_max_len", C.c_float),
+        ("ao_first_random", C.c_int32),
         ("reserved", C.c_int32),
     ]
 
@@ -285,7 +302,7 @@ class RayHit(C.Structure):
 
 # numpy views of the two records
 RAY_TASK_DTYPE = np.dtype([("origin", "<f8", 3), ("dir", "<f4", 3), ("zz", "<f4"), ("pixel", "<u4"),
-                           ("inside", "<i4"), ("max_len", "<f4"), ("reserved", "<i4")])
+                           ("inside", "<i4"), ("max_len", "<f4"), ("ao_q", "<i4")])
 RAY_HIT_DTYPE = np.dtype([("gbuf", "<u4", 5), ("flags", "<i4"), ("pos", "<f8", 3), ("zz", "<f8"), ("zz2", "<f8"),
                           ("zz2_paint", "<f8"), ("step_count", "<f4"), ("iters", "<i4")])
 
@@ -447,8 +464,8 @@ SIGNATURES = {
     "mb3d_hip_mlp_train": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
                                        C.c_float, C.c_void_p]),
     "mb3d_hip_mlp_infer": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
-    "mb3d_hip_march_rays": (C.c_int32, [C.POINTER(Params), C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
-                                        C.c_int32]),
+    "mb3d_hip_march_rays": (C.c_int32, [C.POINTER(Params), C.c_void_p, C.c_int64, C.c_int32, C.POINTER(RayOpts),
+                                        C.c_void_p, C.c_int32]),
     "mb3d_hip_gbuf_pack": (C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
     "mb3d_hip_gbuf_unpack": (C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
     "mb3d_hip_lighting_default": (None, [C.POINTER(Lighting)]),
@@ -867,13 +884,38 @@ def post(p: Params, job: PostJob, gbuf, row0=0, rows=None, opts: Options = None)
     return g
 
 
-def march_rays(p: Params, tasks, depth_sqrt=False):
+def march_rays(p: Params, tasks, depth_sqrt=False, opts: "RayOpts" = None):
     """mb3d_hip_march_rays on host records (a RAY_TASK_DTYPE array): the
-    hits as a RAY_HIT_DTYPE array."""
+    hits as a RAY_HIT_DTYPE array; `opts` the hits' shadow and AO pass."""
     t = np.ascontiguousarray(tasks, dtype=RAY_TASK_DTYPE)
     h = np.zeros(t.shape[0], RAY_HIT_DTYPE)
-    check(load().mb3d_hip_march_rays(C.byref(p), _p(t), t.shape[0], int(depth_sqrt), _p(h), 0), "mb3d_hip_march_rays")
+    check(load().mb3d_hip_march_rays(C.byref(p), _p(t), t.shape[0], int(depth_sqrt),
+                                     C.byref(opts) if opts is not None else None, _p(h), 0), "mb3d_hip_march_rays")
     return h
+
+
+def ray_opts(p: Params, lights=(), positional=(), soft_radius=0.0, hs_mul=1.0, ao_max_len=None) -> "RayOpts":
+    """RayOpts: hard shadows of global lights (world directions towards
+    them) and positional ones ((scene position, lmax_l)), header indices in
+    order; one soft shadow of the last with soft_radius > 0; DE AO at the
+    hits with ao_max_len."""
+    o = RayOpts()
+    i = 0
+    for d in lights:
+        v = np.asarray(d, np.float64)
+        v = -p.step_width * v / np.linalg.norm(v)
+        o.light_idx[i], o.light_positional[i] = i, 0
+        o.light_vec[i][:] = list(v)
+        i += 1
+    for pos, lmax in positional:
+        o.light_idx[i], o.light_positional[i], o.light_lmax[i] = i, 1, lmax
+        o.light_vec[i][:] = list(pos)
+        i += 1
+    o.n_lights = i
+    o.soft_radius, o.hs_mul = soft_radius, hs_mul
+    if ao_max_len is not None:
+        o.ao, o.ao_max_len = 1, ao_max_len
+    return o
 
 
 def reflection_tasks(p: Params, g: dict, off_steps=2.0):

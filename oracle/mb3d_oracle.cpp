@@ -2080,15 +2080,254 @@ static void march_ray_task(Marcher &m, const mb3d_hip_ray_task &t, bool depth_sq
     if (open_air) h.iters = m.it.it_result;
 }
 
+// calc_hs (src/reflect.rs:805-952) at a ray's hit: m at the hit (it.c, mzz
+// = |zz|, vfov the pixel's view vector), the hit's shadow word in and out
+static uint16_t ray_hard_shadows(Marcher &m, const mb3d_hip_ray_opts &o, uint16_t shadow, const int16_t nq[3]) {
+    const mb3d_hip_params &p = m.p;
+    const double sw = p.step_width;
+    const float zz = (float)std::fabs(m.mzz);
+    double ic[3] = {m.it.c[0], m.it.c[1], m.it.c[2]};
+    const double max_lhs = (double)((float)(p.width + p.height) * 0.6f *
+                                    (1.0f + 0.5f * std::fmin(zz, (float)p.zend * 0.4f) * std::fmax((float)p.fov_y, 0.0f) /
+                                                (float)p.height) *
+                                    o.hs_mul);
+    const double nn[3] = {(double)nq[0] / 32767.0, (double)nq[1] / 32767.0, (double)nq[2] / 32767.0};
+    double nvec[3];
+    rot_rev(nn, p.vgrads, nvec);
+    auto upd = [&](double z) { m.ms_de_stop = (float)((double)p.de_stop * (1.0 + std::fabs(z) * (double)p.de_stop_factor)); };
+    // the march towards the light (the closure `march`): the soft shadow's
+    // value, dt1 the length left
+    auto march = [&](const double hsvec[3], double &dt1, double &zz2, double zz2mul, bool soft, float zrs_mul) -> float {
+        float rsfd = 1.0f, zrsoft = 1.0f;
+        const double dmaxl = dt1;
+        double d = m.calc_de();
+        int guard = 0;
+        for (;;) {
+            guard += 1;
+            const double last_de = d;
+            const double st = (double)std::fmin((float)(d * (double)p.z_step_div * (double)rsfd),
+                                                std::fmax(m.ms_de_stop, 0.4f) * p.mh04zsd);
+            dt1 -= st;
+            add_weight(m.it.c, hsvec, -st);
+            zz2 += st * zz2mul;
+            upd(zz2);
+            d = m.calc_de();
+            if (soft) {
+                const float r = (float)((dmaxl - dt1) / max_lhs);
+                const float r8 = (r * r) * (r * r);
+                zrsoft = std::fmin(zrsoft, (float)((d - (double)m.ms_de_stop) * (double)zrs_mul / (dmaxl - dt1 + 0.11)) + r8 * r8);
+            }
+            if (m.it.it_result >= m.max_its_result || d < (double)m.ms_de_stop) break;
+            if (d > last_de + st) d = last_de + st;
+            if (last_de > d + 1e-30) {
+                const float t = (float)(st / (last_de - d));
+                rsfd = t < 1.0f ? std::fmax(t, 0.5f) : 1.0f;
+            } else {
+                rsfd = 1.0f;
+            }
+            if (dt1 < 0.0 || guard > 1000000) break;
+        }
+        return zrsoft;
+    };
+    auto light_dir = [&](int i, double &len, double hsvec[3]) -> bool {  // false: out of the light's reach
+        for (int k = 0; k < 3; k++) hsvec[k] = o.light_vec[i][k];
+        if (!o.light_positional[i]) return true;
+        double v[3];
+        for (int k = 0; k < 3; k++) v[k] = o.light_vec[i][k] - m.it.c[k];
+        const double d = dot3(v, v);
+        if (d > (double)(o.light_lmax[i] * o.hs_mul)) return false;
+        if (d < (len * sw) * (len * sw)) len = std::sqrt(d) / sw;
+        const double l = std::sqrt(d);  // norm_to(-sw, v)
+        for (int k = 0; k < 3; k++) hsvec[k] = v[k] * (-sw / l);
+        return true;
+    };
+    auto zz2mul_of = [&](const double hsvec[3]) {
+        return -dot3(hsvec, m.vfov) / std::sqrt(dot3(hsvec, hsvec) * dot3(m.vfov, m.vfov));
+    };
+    if (o.soft_radius > 0.0f) {
+        const int i = o.n_lights - 1;
+        const float sr = std::fmax(o.soft_radius, 0.001f);
+        const float zrs_mul = o.light_positional[i] ? ((o.light_visible[i] & 6) == 2 ? 70.0f / sr : 40.0f / sr) : 80.0f / sr;
+        shadow |= 0xFC00;
+        double zz2 = m.mzz;
+        upd((double)zz);
+        double dmaxl = max_lhs;
+        double hsvec[3];
+        if (!light_dir(i, dmaxl, hsvec)) return shadow;
+        const double zz2mul = zz2mul_of(hsvec);
+        if (dmaxl > 0.0) {
+            if (dot3(nvec, hsvec) > 0.0) return shadow & 0x3FF;
+            double dt1 = dmaxl;
+            const float zr = march(hsvec, dt1, zz2, zz2mul, true, zrs_mul);
+            shadow = (uint16_t)((shadow & 0x3FF) | ((uint16_t)round_even((double)(std::fmin(std::fmax(zr, 0.0f), 1.0f) * 63.4f)) << 10));
+            for (int k = 0; k < 3; k++) m.it.c[k] = ic[k];
+        }
+        return shadow;
+    }
+    uint16_t mask = 0;
+    for (int i = 0; i < o.n_lights; i++) mask |= (uint16_t)(0x400 << o.light_idx[i]);
+    shadow &= (uint16_t)~mask;
+    for (int i = 0; i < o.n_lights; i++) {
+        const uint16_t bit = (uint16_t)(0x400 << o.light_idx[i]);
+        for (int k = 0; k < 3; k++) m.it.c[k] = ic[k];
+        double zz2 = m.mzz;
+        upd((double)zz);
+        double dt1 = max_lhs;
+        double hsvec[3];
+        if (!light_dir(i, dt1, hsvec)) {
+            shadow |= bit;
+            continue;
+        }
+        const double zz2mul = zz2mul_of(hsvec);
+        if (dt1 > 0.0) {
+            if (dot3(nvec, hsvec) > 0.0) {
+                shadow |= bit;
+                continue;
+            }
+            march(hsvec, dt1, zz2, zz2mul, false, 0.0f);
+            if (dt1 > 0.0) shadow |= bit;
+        }
+    }
+    for (int k = 0; k < 3; k++) m.it.c[k] = ic[k];
+    return shadow;
+}
+
+// deao_1pos (src/reflect.rs:954-1090) at a ray's hit, outside rendering and
+// without a random first step: the AO word
+static uint16_t ray_deao(Marcher &m, int quality, float max_len, const int16_t nq[3]) {
+    const mb3d_hip_params &p = m.p;
+    const double sw = p.step_width;
+    m.it.calc_sit = false;
+    double rot_v[3][3];
+    for (int i = 0; i < 3; i++) {  // normalise_matrix_to(sw, vgrads)
+        const double d = sw / std::sqrt(dot3(p.vgrads[i], p.vgrads[i]));
+        for (int j = 0; j < 3; j++) rot_v[i][j] = p.vgrads[i][j] * d;
+    }
+    const double s_max_d = (double)max_len * 0.5 * std::sqrt((double)p.height * (double)p.height + (double)p.width * (double)p.width);
+    double ms_de_stop = (double)std::fmin((float)((double)p.de_stop * (1.0 + std::fabs(m.mzz) * (double)p.de_stop_factor)), 1e6f);
+    double ic[3] = {m.it.c[0], m.it.c[1], m.it.c[2]};
+    const double step_ao = ms_de_stop / (double)p.de_stop;
+    const double max_dist = s_max_d * std::sqrt(step_ao);
+    auto ray_dir = [](double ya, double za, float o[3]) {
+        const double sy = std::sin(ya), cy = std::cos(ya), sz = std::sin(za), cz = std::cos(za);
+        o[0] = (float)(sy * cz);
+        o[1] = (float)(-sy * sz);
+        o[2] = (float)(-cy);
+    };
+    const double tau = 6.283185307179586;
+    float dirs[16][3];
+    int nd = 0;
+    double d_step_mul;
+    float d_min_a_dif, corr_w;
+    if (quality == 0) {
+        const double abr = 60.0 * (3.141592653589793 / 180.0);
+        for (int i = 0; i < 3; i++) ray_dir(0.5 * abr, (double)i * tau / 3.0, dirs[nd++]);
+        d_step_mul = 1.8;
+        d_min_a_dif = -1.0f;
+        corr_w = 0.3f;
+    } else {
+        const double abr = 1.5707963267948966 / ((double)quality + 0.9);
+        ray_dir(0.0, 0.0, dirs[nd++]);
+        for (int iy = 1; iy <= quality; iy++) {
+            const int n = (int)round_even(std::sin(0.25) * tau / abr);
+            for (int ix = 0; ix < n; ix++) ray_dir(abr * (double)iy, (double)ix * tau / (double)n, dirs[nd++]);
+        }
+        d_step_mul = 1.0 + std::sin(abr);
+        d_min_a_dif = (float)std::cos(abr * 1.2);
+        corr_w = quality == 1 ? 0.2f : 0.1666f;
+    }
+    float n[3] = {(float)nq[0], (float)nq[1], (float)nq[2]};
+    const float l = std::fmax(std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]), 1e-30f);
+    for (int k = 0; k < 3; k++) n[k] = n[k] / l;
+    float rot_w[3][3];
+    normal_matrix(n, rot_w);
+    if (p.de_stop_factor != 0.0f)
+        ms_de_stop /= d_step_mul * d_step_mul;
+    else
+        ms_de_stop = (double)p.de_stop / (d_step_mul * d_step_mul);
+    const int ray_count = nd;
+    const double de_mul = std::sqrt((double)ray_count * 0.5);
+    const double abr = 1.2 / std::asin(std::fmin(std::fmax(1.0 / de_mul, -1.0), 1.0));
+    const double md_d10 = 0.1 / (max_dist * de_mul);
+    float min_ra[16];
+    for (int k = 0; k < ray_count; k++) {
+        const float *d = dirs[k];
+        const float v[3] = {rot_w[0][0] * d[0] + rot_w[0][1] * d[1] + rot_w[0][2] * d[2],
+                            rot_w[1][0] * d[0] + rot_w[1][1] * d[1] + rot_w[1][2] * d[2],
+                            rot_w[2][0] * d[0] + rot_w[2][1] * d[1] + rot_w[2][2] * d[2]};
+        const double vd[3] = {(double)v[0], (double)v[1], (double)v[2]};
+        double sv[3];
+        rot_rev(vd, rot_v, sv);
+        double dt1 = step_ao * d_step_mul;
+        double s = 1.0;
+        bool end = false;
+        int guard = 0;
+        for (;;) {
+            guard += 1;
+            if (dt1 > max_dist) {
+                dt1 = max_dist;
+                end = true;
+            }
+            for (int q = 0; q < 3; q++) m.it.c[q] = ic[q];
+            add_weight(m.it.c, sv, dt1);
+            const double d2 = m.calc_de();
+            s = std::fmin(s, (d2 - ms_de_stop + dt1 * md_d10) / dt1);
+            if (s < 0.02) break;
+            dt1 += std::fmax(d2, dt1 * d_step_mul);
+            if (end || guard > 100000) break;
+        }
+        min_ra[k] = (float)(std::fmax(s, 0.0) * de_mul);
+    }
+    double amount = 0.0;
+    for (int k = 0; k < ray_count; k++) {
+        float s_add = 0.0f;
+        if (min_ra[k] < 1.0f) {
+            const float max_add = 1.0f - min_ra[k];
+            for (int k2 = 0; k2 < ray_count; k2++) {
+                if (k2 == k) continue;
+                const float dt = dirs[k][0] * dirs[k2][0] + dirs[k][1] * dirs[k2][1] + dirs[k][2] * dirs[k2][2];
+                if (dt > d_min_a_dif) {
+                    // acos of the f32 dot (f32::acos), widened
+                    const float ac = std::acos(std::fmin(std::fmax(dt, -1.0f), 1.0f));
+                    const float overlap = min_ra[k2] - (float)((double)ac * abr) + 1.0f;
+                    if (overlap > 0.0f) s_add += std::fmin(max_add, overlap) * corr_w;
+                }
+            }
+        }
+        amount += (double)std::fmin(s_add + min_ra[k], 1.0f);
+    }
+    for (int q = 0; q < 3; q++) m.it.c[q] = ic[q];
+    return (uint16_t)std::fmax(round_even(16383.0 * (1.0 - amount / (double)ray_count)), 0.0);
+}
+
 extern "C" int32_t orc_march_rays(const mb3d_hip_params *pp, const mb3d_hip_ray_task *tasks, int64_t n,
-                                  int32_t depth_sqrt, mb3d_hip_ray_hit *hits, int32_t threads) {
+                                  int32_t depth_sqrt, const mb3d_hip_ray_opts *opts, mb3d_hip_ray_hit *hits,
+                                  int32_t threads) {
     if (int e = check_params(pp)) return e;
     const int64_t chunk = 64;
     const int64_t nchunks = (n + chunk - 1) / chunk;
     for_rows_interleaved((int)nchunks, nthreads(threads), [&](int c) {
         Marcher m(*pp);
-        for (int64_t i = (int64_t)c * chunk; i < std::min(n, (int64_t)(c + 1) * chunk); i++)
-            march_ray_task(m, tasks[i], depth_sqrt != 0, hits[i]);
+        for (int64_t i = (int64_t)c * chunk; i < std::min(n, (int64_t)(c + 1) * chunk); i++) {
+            mb3d_hip_ray_hit &h = hits[i];
+            march_ray_task(m, tasks[i], depth_sqrt != 0, h);
+            if (!opts || !(h.flags & MB3D_HIP_RAY_HIT)) continue;
+            // the hit's pass (src/reflect.rs:1647-1670): m at the hit, its DE stop kept
+            const float saved = m.ms_de_stop;
+            for (int k = 0; k < 3; k++) m.it.c[k] = h.pos[k];
+            Px si;
+            from_words(h.gbuf, si);
+            if (opts->n_lights > 0) {
+                m.mzz = std::fabs(h.zz);
+                si.shadow = ray_hard_shadows(m, *opts, si.shadow, si.normal);
+            }
+            if (opts->ao) {
+                m.mzz = std::fabs(h.zz);
+                si.amb_shadow = ray_deao(m, tasks[i].ao_q, opts->ao_max_len, si.normal);
+            }
+            m.ms_de_stop = saved;
+            to_words(si, h.gbuf);
+        }
     });
     return 0;
 }

@@ -74,10 +74,11 @@ double orc_march_ray(const mb3d_hip_params *p, const double origin[3], const dou
                      double zz, int32_t *its, int32_t *steps);
 
 /* SrTracer::march_ray (src/reflect.rs:1464-1674) for each task: the card's
- * reflection batch march (no volumetric light, ground, shadows or occlusion
- * at the hit).  Tasks in chunks of 64 over threads (<= 0: all cores). */
+ * reflection batch (no volumetric light or ground), with calc_hs and
+ * deao_1pos at the hits when `opts` asks (src/reflect.rs:805-1090).  Tasks
+ * in chunks of 64 over threads (<= 0: all cores). */
 int32_t orc_march_rays(const mb3d_hip_params *p, const mb3d_hip_ray_task *tasks, int64_t n, int32_t depth_sqrt,
-                       mb3d_hip_ray_hit *hits, int32_t threads);
+                       const mb3d_hip_ray_opts *opts, mb3d_hip_ray_hit *hits, int32_t threads);
 
 /* ---- the painter (paint_oracle.cpp) ---- */
 

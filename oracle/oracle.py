@@ -56,7 +56,8 @@ def load():
     lib.orc_paint_setup.argtypes = [C.POINTER(H.Scene), C.POINTER(H.Params), C.POINTER(H.Lighting),
                                     C.POINTER(H.PaintCamera), C.POINTER(H.LightVals)]
     lib.orc_paint_setup.restype = C.c_int32
-    lib.orc_march_rays.argtypes = [C.POINTER(H.Params), C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int32]
+    lib.orc_march_rays.argtypes = [C.POINTER(H.Params), C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p,
+                                   C.c_int32]
     lib.orc_march_rays.restype = C.c_int32
     lib.orc_gbuf_pack.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
     lib.orc_gbuf_pack.restype = None
@@ -227,12 +228,13 @@ def gbuf_unpack(records):
     return out
 
 
-def march_rays(p, tasks, depth_sqrt=False, threads=0):
-    """SrTracer::march_ray for each task (H.RAY_TASK_DTYPE records)."""
+def march_rays(p, tasks, depth_sqrt=False, opts=None, threads=0):
+    """SrTracer::march_ray for each task (H.RAY_TASK_DTYPE records), with the
+    hits' shadows and occlusion of `opts` (an H.RayOpts)."""
     t = np.ascontiguousarray(tasks, dtype=H.RAY_TASK_DTYPE)
     h = np.zeros(t.shape[0], H.RAY_HIT_DTYPE)
     st = load().orc_march_rays(C.byref(p), t.ctypes.data_as(C.c_void_p), t.shape[0], int(depth_sqrt),
-                               h.ctypes.data_as(C.c_void_p), threads)
+                               C.byref(opts) if opts is not None else None, h.ctypes.data_as(C.c_void_p), threads)
     if st != 0:
         raise RuntimeError(f"orc_march_rays failed ({st})")
     return h

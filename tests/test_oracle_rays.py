@@ -11,6 +11,12 @@ src/reflect.rs:1464-1674) pinned on the CPU.
 - With depth 0 at the start the square-root depth code (the painter's far
   fog) is the plain one.
 - The result does not depend on the thread count.
+- The hits' shadow and occlusion pass (calc_hs and deao_1pos,
+  src/reflect.rs:805-1090), on view rays that reproduce the primary hits:
+  the hard-shadow bit of a light equals the primary shadow pass's for the
+  same light at >= 97 % of the pixels, the AO follows the primary DE AO
+  (correlation >= 0.9; the two differ by the primary's neighbour
+  correction), and the soft shadow is darker where the hard one is set.
 """
 import numpy as np
 import pytest
@@ -141,3 +147,35 @@ def test_rays_do_not_depend_on_the_thread_count():
     a = O.march_rays(p, t, threads=1)
     b = O.march_rays(p, t, threads=7)
     assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("preset", ["Integer Power", "Amazing Box"])
+def test_the_hit_pass_follows_the_primary_shadows_and_occlusion(preset):
+    sc = H.scene_preset(preset, 96, 72)
+    p = H.params_from_scene(sc)
+    job = H.post_job(lights=[(0.6, -0.5, -0.6)], deao_quality=1, params=p)
+    r = O.render(p, job, threads=0)
+    st, vf = pixel_rays(p)
+    t = view_tasks(p, st, vf)
+    t["ao_q"] = 2  # 7 rays, as the primary quality 1
+    o = H.RayOpts()
+    o.n_lights, o.light_idx[0], o.hs_mul = 1, 0, job.shadow_max_len_mul
+    o.light_vec[0][:] = list(job.light_vec[0])
+    o.ao, o.ao_max_len = 1, job.deao_max_len
+    hh = O.march_rays(p, t, opts=o)
+    surf = np.isfinite(r["depth"]) & (r["depth"] > 0)
+    both = surf & ((hh["flags"] & H.RAY_HIT) != 0)
+    a = (hh["gbuf"][both, 1] >> 16) & 0x400
+    b = (r["gbuf"][both, 1] >> 16) & 0x400
+    assert (a == b).mean() >= 0.97 and 0.05 < (a != 0).mean() < 0.95
+    ao_r, ao_p = hh["gbuf"][both, 4] & 0xFFFF, r["gbuf"][both, 4] & 0xFFFF
+    assert np.corrcoef(ao_r, ao_p)[0, 1] >= 0.9
+    # the step counts and depths of the march are kept
+    plain = O.march_rays(p, t)
+    keep = np.array([0xFFFFFFFF, 0x0000FFFF, 0xFFFFFFFF, 0xFFFFFFFF, 0], np.uint32)
+    assert np.array_equal(hh["gbuf"] & keep, plain["gbuf"] & keep)
+    # one soft shadow of the light: its value (0..63) is lower where the hard bit is set
+    o.soft_radius = 2.0
+    hs = O.march_rays(p, t, opts=o)
+    soft = (hs["gbuf"][both, 1] >> 26).astype(np.float64)
+    assert soft[a != 0].mean() + 10 < soft[a == 0].mean()
