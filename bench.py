@@ -24,7 +24,8 @@ launch over its HIP-event time on the library's stream), the CPU baseline
 (the f64 oracle on the host cores) and the accuracy of the plain and the
 learned march against the oracle.  Beside the metric: the painter's kernel
 on the frame's G-buffer (device time, bytes, the oracle's painter on the
-host cores).
+host cores), and the frame's first-bounce reflection rays through the
+secondary-ray kernels (device time, the oracle on a sample).
 """
 from __future__ import annotations
 
@@ -221,6 +222,28 @@ def main():
              "peak_gbs": PEAK_HBM / 1e9, "frac_hbm": paint_bytes / (t_paint * 1e-3) / PEAK_HBM}
     paint_gbuf = gbuf.cpu().numpy().view(np.uint32).reshape(n, 5)
 
+    # --- secondary rays (beside the metric): the frame's first-bounce
+    # reflection rays from every surface pixel, marched on the device with
+    # one hard shadow and quality-2 DE AO at their hits (tasks and hits in
+    # device memory); the ray kernels' HIP-event time per batch
+    g_host = H.march(p, outputs=("depth64", "normals"))
+    tasks = H.reflection_tasks(p, g_host)
+    tasks["ao_q"] = 2
+    ray_opts = H.ray_opts(p, lights=[(0.6, -0.5, -0.6)], ao_max_len=1.0)
+    d_tasks = torch.from_numpy(tasks.view(np.uint8).copy()).to(dev)
+    d_hits = torch.empty(len(tasks) * H.RAY_HIT_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    rms = []
+    for i in range(a.warmup + max(3, a.steps // 2)):
+        H.check(lib.mb3d_hip_march_rays(H.C.byref(p), H.C.c_void_p(d_tasks.data_ptr()), len(tasks), 0,
+                                        H.C.byref(ray_opts), H.C.c_void_p(d_hits.data_ptr()), 1), "mb3d_hip_march_rays")
+        if i >= a.warmup:
+            rms.append(H.counters()["kernel_ms"])
+    t_rays = float(np.mean(rms))
+    hits_np = d_hits.cpu().numpy().view(H.RAY_HIT_DTYPE)
+    rays = {"kernel": "k_rays + k_ray_post", "rays": len(tasks),
+            "hits": int(((hits_np["flags"] & H.RAY_HIT) != 0).sum()), "kernel_ms": t_rays,
+            "mrays_s": len(tasks) / (t_rays * 1e-3) / 1e6, "hit_pass": "1 hard shadow, DE AO quality 2 (7 rays)"}
+
     result = {
         "metric": "Mpixels/s 1080p Mandelbulb-p8 Z-buffer; learned vs plain at equal depth error",
         "value": value,
@@ -255,10 +278,12 @@ def main():
                     "train_loss": [float(curve[:20].mean()), float(curve[-100:].mean())]},
         "roofline": roofline,
         "paint": paint,
+        "rays": rays,
     }
     if not a.no_cpu:
         result["cpu_baseline"], result["accuracy"] = cpu_leg(sc, p, a, learn_opts)
         paint["cpu"] = paint_cpu_leg(cam, lv, paint_gbuf, result["cpu_baseline"]["cores"])
+        rays["cpu"] = rays_cpu_leg(p, tasks, ray_opts, result["cpu_baseline"]["cores"])
     net.close()
     print(json.dumps(result))
 
@@ -306,6 +331,23 @@ def paint_cpu_leg(cam, lv, g, threads):
     dt = time.perf_counter() - t
     return {"value": g.shape[0] / dt / 1e6, "unit": "Mpixels/s", "cores": threads, "kind": "port",
             "sample": f"one full {cam.width}x{cam.height} G-buffer, {dt:.3f} s"}
+
+
+def rays_cpu_leg(p, tasks, opts, threads, sample=100_000):
+    """The oracle's secondary-ray march and hit pass (a restatement of the
+    reference's SrTracer::march_ray, calc_hs and deao_1pos) on the host
+    cores, on every k-th task of the same batch (about `sample` of them, so
+    that the sample covers the whole picture)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    k = max(1, len(tasks) // sample)
+    t_s = np.ascontiguousarray(tasks[::k])
+    t = time.perf_counter()
+    O.march_rays(p, t_s, opts=opts, threads=threads)
+    dt = time.perf_counter() - t
+    return {"value": len(t_s) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"every {k}th ray of the batch, {len(t_s)} rays, {dt:.2f} s"}
 
 
 if __name__ == "__main__":
