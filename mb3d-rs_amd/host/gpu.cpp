@@ -6,9 +6,12 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
+#include <thread>
 
 namespace mb3d::gpu {
 
@@ -197,6 +200,214 @@ Status post_on_card(const mb3d_hip_params &p, std::vector<SiLight> &gbuf, const 
         if (progress) progress(row0 + rows, h);
     }
     return Status{};
+}
+
+namespace {
+
+// Which rows are where (render::Board, src/render.rs:1380-1470, one card):
+// delivered rows, rows a second worker backs up, and the card's bands.
+class Board {
+  public:
+    explicit Board(size_t h) : delivered_(h), backup_(h) {
+        for (size_t y = 0; y < h; y++) delivered_[y] = backup_[y] = false;
+    }
+    void took(size_t r0, size_t n) {
+        std::lock_guard<std::mutex> lk(mu_);
+        card_.push_back({r0, n});
+    }
+    // the first result of row y counts
+    bool deliver(size_t y) { return !delivered_[y].exchange(true); }
+    bool is_delivered(size_t y) const { return delivered_[y].load(); }
+    // up to `max` rows in a row the other side took and has not delivered,
+    // nobody else backing them up (the card steals from the CPU's rows, a
+    // CPU thread from the card's bands)
+    std::optional<std::pair<size_t, size_t>> steal(bool for_card, size_t max, const std::vector<size_t> &cpu_taken) {
+        std::vector<std::pair<size_t, size_t>> from;
+        if (for_card) {
+            for (size_t y : cpu_taken) from.push_back({y, 1});
+        } else {
+            std::lock_guard<std::mutex> lk(mu_);
+            from = card_;
+        }
+        auto open = [&](size_t y) { return !delivered_[y].load() && !backup_[y].load(); };
+        for (auto [r0, n] : from) {
+            size_t y = r0;
+            while (y < r0 + n && !open(y)) y++;
+            if (y >= r0 + n) continue;
+            size_t m = 0;
+            while (y + m < r0 + n && m < std::max<size_t>(max, 1) && open(y + m) && !backup_[y + m].exchange(true)) m++;
+            if (m > 0) {
+                if (for_card) took(y, m);
+                return std::make_pair(y, m);
+            }
+        }
+        return std::nullopt;
+    }
+    // every row of the card's bands is delivered (by it or by others)
+    bool card_done() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (auto [r0, n] : card_)
+            for (size_t y = r0; y < r0 + n; y++)
+                if (!delivered_[y].load()) return false;
+        return true;
+    }
+    std::vector<std::pair<size_t, size_t>> card_bands() {
+        std::lock_guard<std::mutex> lk(mu_);
+        return card_;
+    }
+
+  private:
+    std::vector<std::atomic<bool>> delivered_, backup_;
+    std::mutex mu_;
+    std::vector<std::pair<size_t, size_t>> card_;
+};
+
+}  // namespace
+
+Mixed march_mixed(const mb3d_hip_params &p, const mb3d_hip_post_job *post, size_t threads, const CpuRow &cpu_row,
+                  const Sink &sink, const Cancel &cancel, const Progress &progress, const Done &done,
+                  const std::vector<SiLight> *have, size_t band_rows, const mb3d_hip_options *options) {
+    const size_t w = (size_t)std::max(p.width, 1), h = (size_t)std::max(p.height, 0);
+    Mixed res;
+    res.gbuf.assign(w * h, SiLight{});
+    auto is_done = [&](size_t y) { return done && done(y); };
+    if (have)
+        for (size_t y = 0; y < h; y++)
+            if (is_done(y) && have->size() >= (y + 1) * w) std::copy_n(have->data() + y * w, w, res.gbuf.data() + y * w);
+    Board board(h);
+    std::mutex free_mu, put_mu;
+    size_t top = 0, bottom = h;  // the free rows [top, bottom)
+    std::vector<size_t> cpu_taken;
+    std::atomic<size_t> done_rows{0}, card_rows{0}, cpu_rows{0};
+    for (size_t y = 0; y < h; y++)
+        if (is_done(y)) {
+            board.deliver(y);
+            done_rows++;
+        }
+    auto stopped = [&] { return cancel && cancel(); };
+    auto left = [&] {
+        std::lock_guard<std::mutex> lk(free_mu);
+        while (top < bottom && is_done(top)) top++;
+        while (bottom > top && is_done(bottom - 1)) bottom--;
+        return top < bottom;
+    };
+    auto put = [&](size_t y, const SiLight *row) {
+        std::copy_n(row, w, res.gbuf.data() + y * w);
+        std::lock_guard<std::mutex> lk(put_mu);
+        if (sink) sink(y, row, w);
+        const size_t d = ++done_rows;
+        if (progress) progress(d, h);
+    };
+    // the card: bands from the top, then rows backed up from the CPU's
+    std::mutex pending_mu;
+    std::vector<size_t> pending;  // the card's delivered rows, until its verdict
+    Status card_status;
+    bool card_joined = false;
+    const std::string why = may_start(p);
+    std::thread card;
+    if (why.empty()) {
+        card_joined = true;
+        card = std::thread([&] {
+            Bands bands = [&](size_t) -> std::optional<std::pair<size_t, size_t>> {
+                {
+                    std::lock_guard<std::mutex> lk(free_mu);
+                    while (top < bottom && is_done(top)) top++;
+                    if (top < bottom) {
+                        size_t n = 0;
+                        while (top + n < bottom && n < band_rows && !is_done(top + n)) n++;
+                        const size_t r0 = top;
+                        top += n;
+                        board.took(r0, n);
+                        return std::make_pair(r0, n);
+                    }
+                }
+                std::vector<size_t> taken;
+                {
+                    std::lock_guard<std::mutex> lk(free_mu);
+                    taken = cpu_taken;
+                }
+                return board.steal(true, band_rows, taken);
+            };
+            Cancel stop = [&] { return stopped() || (!left() && board.card_done()); };
+            Sink card_sink = [&](size_t y, const SiLight *row, size_t) {
+                if (!board.deliver(y)) return;
+                {
+                    std::lock_guard<std::mutex> lk(pending_mu);
+                    pending.push_back(y);
+                }
+                card_rows++;
+                put(y, row);
+            };
+            card_status = march_bands(p, post, bands, stop, card_sink, options);
+        });
+    }
+    // the CPU's threads: rows from the bottom, then rows backed up from the card's bands
+    auto worker = [&] {
+        std::vector<SiLight> row(w);
+        for (;;) {
+            if (stopped()) return;
+            std::optional<size_t> y;
+            {
+                std::lock_guard<std::mutex> lk(free_mu);
+                while (bottom > top && is_done(bottom - 1)) bottom--;
+                if (bottom > top) {
+                    y = --bottom;
+                    cpu_taken.push_back(*y);
+                }
+            }
+            if (!y) {
+                auto r = board.steal(false, 1, {});
+                if (!r) {
+                    // nothing free and nothing to back up: wait for the card's open rows
+                    if (!card_joined || board.card_done()) return;
+                    std::this_thread::sleep_for(std::chrono::milliseconds(1));
+                    continue;
+                }
+                y = r->first;
+            }
+            cpu_row(*y, row.data());
+            if (board.deliver(*y)) {
+                cpu_rows++;
+                put(*y, row.data());
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (size_t i = 0; i < std::max<size_t>(threads, 1); i++) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+    if (card.joinable()) card.join();
+    // a card that overflowed: its rows are redone here (the first result did
+    // not count after all)
+    if (card_joined && !card_status.ok && card_status.error != "cancelled" && !stopped()) {
+        std::vector<size_t> redo;
+        {
+            std::lock_guard<std::mutex> lk(pending_mu);
+            redo = pending;
+        }
+        std::vector<SiLight> row(w);
+        for (size_t y : redo) {
+            cpu_row(y, row.data());
+            put(y, row.data());
+            cpu_rows++;
+        }
+        card_rows = 0;
+    }
+    // rows a stopped card took and never delivered, with the CPU's threads gone
+    for (size_t y = 0; y < h && !stopped(); y++) {
+        if (board.is_delivered(y)) continue;
+        std::vector<SiLight> row(w);
+        cpu_row(y, row.data());
+        if (board.deliver(y)) {
+            cpu_rows++;
+            put(y, row.data());
+        }
+    }
+    res.card_rows = card_rows.load();
+    res.cpu_rows = cpu_rows.load();
+    if (stopped()) res.status = Status::fail("cancelled");
+    set_status(card_joined && res.card_rows ? "GPU (MI355X, HIP): the march shared with the CPU"
+                                             : "CPU: " + (why.empty() ? card_status.error : why));
+    return res;
 }
 
 }  // namespace mb3d::gpu

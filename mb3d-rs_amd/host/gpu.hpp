@@ -88,4 +88,34 @@ Status march_bands(const mb3d_hip_params &p, const mb3d_hip_post_job *post, cons
 Status post_on_card(const mb3d_hip_params &p, std::vector<SiLight> &gbuf, const mb3d_hip_post_job &post,
                     const Progress &progress, const Cancel &cancel);
 
+// ---- the march shared between the card and the CPU's threads
+// (render::march_mixed and its Board, src/render.rs:1380-1470, 1500-1700;
+// the renderer's Auto mode) ----
+
+// The caller's CPU marcher: row y of the picture (width pixels) into `row`.
+using CpuRow = std::function<void(size_t y, SiLight *row)>;
+// A checkpoint holds row y already (its pixels are in `have`).
+using Done = std::function<bool(size_t y)>;
+
+struct Mixed {
+    std::vector<SiLight> gbuf;
+    size_t card_rows = 0;  // rows the card delivered (0: it never joined, or its rows were redone)
+    size_t cpu_rows = 0;   // rows the CPU's threads delivered
+    Status status;         // fail("cancelled") when stopped
+};
+
+// The picture's rows marched by `threads` CPU threads (cpu_row), from the
+// bottom, and by the card, in bands of `band_rows` from the top (with the
+// post passes of `post` on its bands), until they meet.  Rows that `done`
+// names are the checkpoint's, taken from `have` and not marched.  When no
+// row is free, an idle worker backs up rows the other side took and has not
+// delivered; the first result of a row counts.  Every row reaches `sink`
+// as it is delivered, once -- but for a card whose bands overflow single
+// precision: its rows are then redone by the CPU marcher and reach `sink`
+// again.  A card that cannot start leaves all rows to the CPU threads.
+Mixed march_mixed(const mb3d_hip_params &p, const mb3d_hip_post_job *post, size_t threads, const CpuRow &cpu_row,
+                  const Sink &sink, const Cancel &cancel, const Progress &progress, const Done &done = {},
+                  const std::vector<SiLight> *have = nullptr, size_t band_rows = 16,
+                  const mb3d_hip_options *options = nullptr);
+
 }  // namespace mb3d::gpu

@@ -10,12 +10,15 @@
 //       sink, gpu::march_bands over two bands, mb3d_hip_march directly --
 //       and writes each G-buffer (5 words per pixel) to OUTDIR, plus the
 //       post passes of gpu::post_on_card on the plain march's G-buffer
+#include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iterator>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../mb3d-rs_amd/host/gpu.hpp"
@@ -163,9 +166,59 @@ static int gpu_checks(const std::string &params_path, const std::string &post_pa
     return 0;
 }
 
+// gpu::march_mixed: the CPU marcher is the picture the test computed
+// beforehand (CPU_GBUF, 5 words a pixel), one row per call after `delay_us`;
+// rows y % 7 == 3 are a checkpoint's (a marker record).  Writes the result,
+// each row's sink count, and the rows the card and the CPU delivered.
+static int mixed_checks(const std::string &params_path, const std::string &cpu_path, const std::string &outdir,
+                        int threads, int delay_us, int stop_after) {
+    const auto pb = read_file(params_path);
+    mb3d_hip_params p;
+    if (pb.size() != sizeof p) return fail("params size");
+    std::memcpy(&p, pb.data(), sizeof p);
+    const size_t w = (size_t)p.width, h = (size_t)p.height;
+    const auto cb = read_file(cpu_path);
+    if (cb.size() != w * h * 20) return fail("the CPU picture's size");
+    std::vector<uint32_t> cpu_words(w * h * 5);
+    std::memcpy(cpu_words.data(), cb.data(), cb.size());
+    g::SiLight marker;
+    marker.zpos_fine = 0x5A5A5A5Au;
+    marker.si_gradient = 0x1234;
+    std::vector<g::SiLight> have(w * h, marker);
+    std::vector<int> sinks(h, 0);
+    std::atomic<size_t> cpu_calls{0};
+    g::CpuRow cpu_row = [&](size_t y, g::SiLight *row) {
+        if (delay_us > 0) std::this_thread::sleep_for(std::chrono::microseconds(delay_us));
+        for (size_t x = 0; x < w; x++) row[x] = g::unpack(&cpu_words[(y * w + x) * 5]);
+        cpu_calls++;
+    };
+    std::atomic<size_t> delivered{0};
+    const auto res = g::march_mixed(
+        p, nullptr, (size_t)threads, cpu_row,
+        [&](size_t y, const g::SiLight *, size_t n) {
+            if (n != w) std::abort();
+            sinks[y]++;
+            delivered++;
+        },
+        [&] { return stop_after > 0 && delivered.load() >= (size_t)stop_after; }, nullptr,
+        [](size_t y) { return y % 7 == 3; }, &have, 8);
+    write_words(outdir + "/mixed.bin", res.gbuf);
+    {
+        std::ofstream f(outdir + "/sinks.bin", std::ios::binary);
+        f.write(reinterpret_cast<const char *>(sinks.data()), (std::streamsize)(sinks.size() * sizeof(int)));
+    }
+    std::printf("card_rows %zu cpu_rows %zu cpu_calls %zu status %s %s\n", res.card_rows, res.cpu_rows,
+                cpu_calls.load(), res.status.ok ? "ok" : res.status.error.c_str(), g::last_status().c_str());
+    std::printf("ok\n");
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc >= 2 && std::string(argv[1]) == "cpu") return cpu_checks();
     if (argc >= 5 && std::string(argv[1]) == "gpu") return gpu_checks(argv[2], argv[3], argv[4]);
-    std::fprintf(stderr, "usage: gpu_api_check cpu | gpu PARAMS POST|- OUTDIR\n");
+    if (argc >= 8 && std::string(argv[1]) == "mixed")
+        return mixed_checks(argv[2], argv[3], argv[4], std::atoi(argv[5]), std::atoi(argv[6]), std::atoi(argv[7]));
+    std::fprintf(stderr, "usage: gpu_api_check cpu | gpu PARAMS POST|- OUTDIR | mixed PARAMS CPU_GBUF OUTDIR THREADS "
+                         "DELAY_US STOP_AFTER\n");
     return 2;
 }

@@ -43,3 +43,45 @@ def test_march_bands_and_post_on_card_agree_with_the_c_abi(hip, tmp_path):
     assert (((post[surf, 1] >> 16) & 0xFC00) == ((direct[surf, 1] >> 16) & 0xFC00)).mean() >= 0.995
     ref = hip.march(p, post=job)["gbuf"]
     assert np.array_equal(ref, direct)
+
+
+def _mixed(hip, tmp_path, threads, delay_us, stop_after=0):
+    import oracle as O
+
+    sc = H.scene_preset("Integer Power", 120, 90)
+    p = hip.params_from_scene(sc)
+    (tmp_path / "p.bin").write_bytes(bytes(p))
+    cpu = O.render(O.params(sc), None, threads=0)["gbuf"]
+    cpu.astype(np.uint32).tofile(tmp_path / "cpu.bin")
+    r = subprocess.run([CHECK, "mixed", str(tmp_path / "p.bin"), str(tmp_path / "cpu.bin"), str(tmp_path),
+                        str(threads), str(delay_us), str(stop_after)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
+    info = r.stdout.split()
+    card_rows, cpu_rows = int(info[1]), int(info[3])
+    got = np.fromfile(tmp_path / "mixed.bin", np.uint32).reshape(p.height, p.width, 5)
+    sinks = np.fromfile(tmp_path / "sinks.bin", np.int32)
+    card = hip.march(p)["gbuf"].reshape(p.height, p.width, 5)
+    return p, got, sinks, card, cpu.reshape(p.height, p.width, 5), card_rows, cpu_rows, r.stdout
+
+
+@pytest.mark.gpu
+def test_march_mixed_shares_the_rows(hip, tmp_path):
+    """gpu::march_mixed (render::march_mixed, src/render.rs:1500-1700): four
+    CPU threads (the oracle's rows, 2 ms each) and the card split the
+    picture; every row but the checkpoint's is delivered once, by one side,
+    and is that side's row."""
+    p, got, sinks, card, cpu, card_rows, cpu_rows, out = _mixed(hip, tmp_path, 4, 2000)
+    ck = np.arange(p.height) % 7 == 3
+    assert np.all(sinks[ck] == 0) and np.all(sinks[~ck] == 1), out
+    assert np.all(got[ck, :, 2] == 0x5A5A5A5A)  # the checkpoint's rows are kept
+    from_card = np.all(got == card, axis=(1, 2)) & ~ck
+    from_cpu = np.all(got == cpu, axis=(1, 2)) & ~ck
+    assert np.all(from_card | from_cpu | ck), out
+    assert card_rows > 0 and cpu_rows > 0 and card_rows + cpu_rows == int((~ck).sum()), out
+
+
+@pytest.mark.gpu
+def test_march_mixed_stops_when_asked(hip, tmp_path):
+    p, got, sinks, card, cpu, card_rows, cpu_rows, out = _mixed(hip, tmp_path, 2, 3000, stop_after=20)
+    assert "cancelled" in out and sinks.sum() < p.height - 12, out
+    assert np.all(sinks <= 1)
