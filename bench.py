@@ -244,6 +244,32 @@ def main():
             "hits": int(((hits_np["flags"] & H.RAY_HIT) != 0).sum()), "kernel_ms": t_rays,
             "mrays_s": len(tasks) / (t_rays * 1e-3) / 1e6, "hit_pass": "1 hard shadow, DE AO quality 2 (7 rays)"}
 
+    # --- the Monte Carlo renderer's marches (beside the metric): every pixel's
+    # primary ray with the inside probe, then a shadow ray towards one light
+    # from each surface it met (tasks and results in device memory); the k_mc
+    # batches' HIP-event time
+    def mc_time(t, reps):
+        d_t = torch.from_numpy(t.view(np.uint8).copy()).to(dev)
+        d_r = torch.empty(len(t) * H.MC_RES_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        ms = []
+        for i in range(a.warmup + reps):
+            H.check(lib.mb3d_hip_mc_batch(H.C.byref(p), H.C.c_void_p(d_t.data_ptr()), len(t), 0,
+                                          H.C.c_void_p(d_r.data_ptr()), 1), "mb3d_hip_mc_batch")
+            if i >= a.warmup:
+                ms.append(H.counters()["kernel_ms"])
+        return float(np.mean(ms)), d_r.cpu().numpy().view(H.MC_RES_DTYPE)
+
+    mc_light = (0.6, -0.5, -0.6)
+    mc_prim = H.mc_primary_tasks(p, seed=a.seed)
+    t_mc0, mc_res = mc_time(mc_prim, max(3, a.steps // 2))
+    mc_shad, _ = H.mc_shadow_tasks(p, mc_res, mc_prim["dir"], mc_light)
+    t_mc1, mc_sres = mc_time(mc_shad, max(3, a.steps // 2))
+    mc = {"kernel": "k_mc",
+          "primary": {"rays": len(mc_prim), "surfaces": int((mc_res["kind"] == 1).sum()), "kernel_ms": t_mc0,
+                      "mrays_s": len(mc_prim) / (t_mc0 * 1e-3) / 1e6},
+          "shadow": {"rays": len(mc_shad), "blocked": int((mc_sres["len"] > 0.01).sum()), "kernel_ms": t_mc1,
+                     "mrays_s": len(mc_shad) / (t_mc1 * 1e-3) / 1e6, "length": 300.0}}
+
     result = {
         "metric": "Mpixels/s 1080p Mandelbulb-p8 Z-buffer; learned vs plain at equal depth error",
         "value": value,
@@ -279,11 +305,13 @@ def main():
         "roofline": roofline,
         "paint": paint,
         "rays": rays,
+        "mc": mc,
     }
     if not a.no_cpu:
         result["cpu_baseline"], result["accuracy"] = cpu_leg(sc, p, a, learn_opts)
         paint["cpu"] = paint_cpu_leg(cam, lv, paint_gbuf, result["cpu_baseline"]["cores"])
         rays["cpu"] = rays_cpu_leg(p, tasks, ray_opts, result["cpu_baseline"]["cores"])
+        mc["cpu"] = mc_cpu_leg(p, mc_prim, mc_shad, result["cpu_baseline"]["cores"])
     net.close()
     print(json.dumps(result))
 
@@ -348,6 +376,24 @@ def rays_cpu_leg(p, tasks, opts, threads, sample=100_000):
     dt = time.perf_counter() - t
     return {"value": len(t_s) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"every {k}th ray of the batch, {len(t_s)} rays, {dt:.2f} s"}
+
+
+def mc_cpu_leg(p, prim, shad, threads, sample=100_000):
+    """The oracle's Monte Carlo marches (a restatement of the reference's
+    McMarch::run) on the host cores, on every k-th task of each batch."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    out = {"unit": "Mrays/s", "cores": threads, "kind": "port"}
+    for name, t in (("primary", prim), ("shadow", shad)):
+        k = max(1, len(t) // sample)
+        t_s = np.ascontiguousarray(t[::k])
+        t0 = time.perf_counter()
+        O.mc_batch(p, t_s, threads=threads)
+        dt = time.perf_counter() - t0
+        out[name] = len(t_s) / dt / 1e6
+        out[name + "_sample"] = f"every {k}th task, {len(t_s)} tasks, {dt:.2f} s"
+    return out
 
 
 if __name__ == "__main__":

@@ -556,6 +556,62 @@ int32_t mb3d_hip_march_rays(const mb3d_hip_params *params, const mb3d_hip_ray_ta
                             int32_t depth_sqrt, const mb3d_hip_ray_opts *opts, mb3d_hip_ray_hit *hits,
                             int32_t on_device);
 
+/* ---- the Monte Carlo renderer's marches (gpu::mc_batch,
+ *      src/gpu.rs:3013-3190; on the CPU McMarch::run, src/mc.rs:478-977) ---- */
+
+#define MB3D_HIP_MC_RAY 0      /* a ray of CalcRay, from pos along dir, at most len */
+#define MB3D_HIP_MC_SHADOW 1   /* a shadow ray of CalcHSMC towards a light, len long */
+#define MB3D_HIP_MC_THROUGH 2  /* the way through a transparent object, len left */
+
+#define MB3D_HIP_MC_INSIDE 1     /* task and result: inside rendering (the DE from inside the set) */
+#define MB3D_HIP_MC_PROBE 2      /* task: a primary ray, which first looks whether it starts inside */
+#define MB3D_HIP_MC_TRANS 4      /* task: a shadow ray finds, colours and returns the surface it hits */
+#define MB3D_HIP_MC_NO_GROUND 8  /* task: the ground plane is left out (there is none on this backend) */
+#define MB3D_HIP_MC_DE_LIMITED 2 /* result: the DE stopped the ray (not the iteration limit) */
+
+/* A march of the tracer (McReq, src/mc.rs:410-438). */
+typedef struct mb3d_hip_mc_task {
+    double pos[3];      /* the start */
+    double dir[3];      /* the direction, one step width long */
+    double zz;          /* the view depth at the start (steps): the DE stop grows with it */
+    double zz2;         /* the ray's length so far (a shadow ray: the view depth along it) */
+    double zzplus;      /* the view depth per step */
+    double len;         /* the longest march, the shadow ray's length or the length left */
+    double vfov[3];     /* the sample's view vector (the volumetric light's, which is not on this backend) */
+    int32_t kind;       /* MB3D_HIP_MC_RAY, _SHADOW or _THROUGH */
+    int32_t seed;       /* the random seed (GetRand) */
+    uint32_t flags;     /* MB3D_HIP_MC_INSIDE | _PROBE | _TRANS | _NO_GROUND */
+    float ms_de_stop;   /* the DE stop of a THROUGH task */
+} mb3d_hip_mc_task;
+
+/* Where a march ended (McRes, src/mc.rs:440-468). */
+typedef struct mb3d_hip_mc_res {
+    double pos[3];      /* where it ended */
+    double zz;          /* the view depth there (RAY) */
+    double zz2;         /* the ray length where the march stopped (RAY), the depth along it (SHADOW) */
+    double len;         /* the length left (SHADOW, THROUGH) */
+    double normals[3];  /* the surface's unit normal */
+    float dyn_fog;      /* the dynamic fog's step count (RAY) */
+    float ms_de_stop;   /* the DE stop where it ended (SHADOW) */
+    int32_t kind;       /* 0 open air (or a shadow ray without a surface), 1 a surface, 2 a primary ray
+                           that starts inside the set */
+    int32_t seed;       /* the random seed after the march */
+    uint16_t si_grad, otrap;  /* the surface's gradient and orbit trap colour */
+    uint32_t flags;     /* MB3D_HIP_MC_INSIDE | MB3D_HIP_MC_DE_LIMITED (dIFS formulas are not on this backend:
+                           no dIFS flags) */
+} mb3d_hip_mc_res;
+
+/* Marches `n` tasks of the Monte Carlo renderer.  secant: the secant search
+ * refines a ray's surface (else the bisection).  The card's own batch
+ * returns f32 positions and depths unless the scene needs FP64; this one
+ * returns every result in f64.  A primary ray that starts inside the set
+ * ends there (kind 2): the turn-around of inside-and-outside rendering
+ * (src/mc.rs:692-697) is off, as this backend's scenes have no such
+ * option.  on_device: `tasks` and `res` are device memory, else host
+ * memory. */
+int32_t mb3d_hip_mc_batch(const mb3d_hip_params *params, const mb3d_hip_mc_task *tasks, int64_t n, int32_t secant,
+                          mb3d_hip_mc_res *res, int32_t on_device);
+
 /* ---- the G-buffer's file records ---- */
 
 /* gbuffer::pack / unpack (src/gbuffer.rs:60-98): n card records (5 u32

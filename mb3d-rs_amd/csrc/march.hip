@@ -17,6 +17,7 @@
 #include "host_common.h"
 #include "paint_common.h"
 #include "march_kernels.h"
+#include "mc_kernel.h"
 
 namespace mb3d {
 
@@ -456,6 +457,15 @@ void launch_ray_post(int grid, hipStream_t s, const DevP &d, const mb3d_hip_ray_
         launch_gen_ray_post(grid, s, d, tasks, n, j, hits, ctr, DEM);
     else
         hipLaunchKernelGGL((k_ray_post<KIND, DEM>), dim3(grid), dim3(256), 0, s, d, tasks, n, j, hits, ctr);
+}
+
+template <int KIND, int DEM>
+void launch_mc(int grid, hipStream_t s, const DevP &d, const mb3d_hip_mc_task *tasks, long long n, int secant,
+               mb3d_hip_mc_res *res, unsigned long long *ctr) {
+    if constexpr (KIND == K_GEN)
+        launch_gen_mc(grid, s, d, tasks, n, secant, res, ctr, DEM);
+    else
+        hipLaunchKernelGGL((k_mc<KIND, DEM>), dim3(grid), dim3(256), 0, s, d, tasks, n, secant, res, ctr);
 }
 
 // The hit pass's tables from the options: the lights, and per AO quality
@@ -1150,6 +1160,69 @@ extern "C" int32_t mb3d_hip_march_rays(const mb3d_hip_params *params, const mb3d
         return MB3D_HIP_OK;
     } catch (...) {
         return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_march_rays");
+    }
+}
+
+extern "C" int32_t mb3d_hip_mc_batch(const mb3d_hip_params *params, const mb3d_hip_mc_task *tasks, int64_t n,
+                                     int32_t secant, mb3d_hip_mc_res *res, int32_t on_device) {
+    try {
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        Ctx &c = g_ctx;
+        if (c.device < 0) return set_error(MB3D_HIP_ERR_NOT_INIT, "mb3d_hip_init was not called");
+        if (!params || !tasks || !res || n < 0) return set_error(MB3D_HIP_ERR_INVALID, "bad argument");
+        const std::string why = unsupported_reason(params);
+        if (!why.empty()) return set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
+        if (n == 0) return MB3D_HIP_OK;
+        if (!on_device)
+            for (int64_t i = 0; i < n; i++)
+                if (tasks[i].kind < MB3D_HIP_MC_RAY || tasks[i].kind > MB3D_HIP_MC_THROUGH)
+                    return set_error(MB3D_HIP_ERR_INVALID, "a task's kind is not 0..2");
+        mb3d_hip_options o;
+        std::memset(&o, 0, sizeof o);
+        DevP d;
+        fill_devp(*params, o, 0, params->height, d);
+        const size_t tb = (size_t)n * sizeof(mb3d_hip_mc_task), rb = (size_t)n * sizeof(mb3d_hip_mc_res);
+        const mb3d_hip_mc_task *dt = tasks;
+        mb3d_hip_mc_res *dr = res;
+        if (!on_device) {
+            const size_t off = (tb + 255) & ~(size_t)255;
+            if (int e = ensure(&c.d_work, &c.work_bytes, off + rb + 256)) return e;
+            dt = (const mb3d_hip_mc_task *)c.d_work;
+            dr = (mb3d_hip_mc_res *)((char *)c.d_work + off);
+            HIP_TRY(hipMemcpyAsync((void *)dt, tasks, tb, hipMemcpyHostToDevice, c.stream));
+        }
+        hipStream_t stream = c.stream;
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, c.d_ctr, (int)C_WORDS);
+        HIP_TRY(hipEventRecord(c.ev[0], stream));
+        const int grid = (int)std::min<long long>((n + 255) / 256, (long long)c.cus * 8);
+        unsigned long long *ctr = c.d_ctr;
+        dispatch(kind_of(*params), dem_of(*params), [&](auto K, auto D) {
+            launch_mc<decltype(K)::value, decltype(D)::value>(grid, stream, d, dt, (long long)n, secant != 0, dr, ctr);
+        });
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c.ev[1], stream));
+        if (!on_device) HIP_TRY(hipMemcpyAsync(res, dr, rb, hipMemcpyDeviceToHost, stream));
+        unsigned long long cw[C_WORDS];
+        HIP_TRY(hipMemcpyAsync(cw, c.d_ctr, sizeof cw, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c.ev[0], c.ev[1]));
+        mb3d_hip_counters &t = t_counters;
+        const double paint_ms = t.paint_ms;
+        std::memset(&t, 0, sizeof t);
+        t.paint_ms = paint_ms;
+        t.pixels = (uint64_t)n;
+        t.steps = cw[C_STATS + 0];
+        t.de_evals_f64 = cw[C_STATS + 2];
+        t.its_p8_f64 = cw[C_STATS + 5];
+        t.its_box_f64 = cw[C_STATS + 6];
+        t.hits = cw[C_STATS + 7];
+        t.its_other_f64 = cw[C_STATS + 11];
+        t.kernel_ms = ms;
+        t.finish_f64_ms = ms;
+        return MB3D_HIP_OK;
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_mc_batch");
     }
 }
 

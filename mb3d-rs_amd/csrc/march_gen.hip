@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "march_kernels.h"
+#include "mc_kernel.h"
 
 namespace mb3d {
 
@@ -59,6 +60,14 @@ void launch_gen_ray_post(int grid, hipStream_t s, const DevP &d, const mb3d_hip_
                          const RayPost &j, mb3d_hip_ray_hit *hits, unsigned long long *ctr, int dem) {
     with_dem(dem, [&](auto D) {
         hipLaunchKernelGGL((k_ray_post<K_GEN, decltype(D)::value>), dim3(grid), dim3(256), 0, s, d, tasks, n, j, hits,
+                           ctr);
+    });
+}
+
+void launch_gen_mc(int grid, hipStream_t s, const DevP &d, const mb3d_hip_mc_task *tasks, long long n, int secant,
+                   mb3d_hip_mc_res *res, unsigned long long *ctr, int dem) {
+    with_dem(dem, [&](auto D) {
+        hipLaunchKernelGGL((k_mc<K_GEN, decltype(D)::value>), dim3(grid), dim3(256), 0, s, d, tasks, n, secant, res,
                            ctr);
     });
 }

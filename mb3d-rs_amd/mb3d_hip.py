@@ -306,6 +306,17 @@ RAY_TASK_DTYPE = np.dtype([("origin", "<f8", 3), ("dir", "<f4", 3), ("zz", "<f4"
 RAY_HIT_DTYPE = np.dtype([("gbuf", "<u4", 5), ("flags", "<i4"), ("pos", "<f8", 3), ("zz", "<f8"), ("zz2", "<f8"),
                           ("zz2_paint", "<f8"), ("step_count", "<f4"), ("iters", "<i4")])
 
+# the Monte Carlo renderer's marches (McReq / McRes, src/mc.rs:410-468)
+MC_RAY, MC_SHADOW, MC_THROUGH = 0, 1, 2
+MC_INSIDE, MC_PROBE, MC_TRANS, MC_NO_GROUND = 1, 2, 4, 8
+MC_DE_LIMITED = 2
+MC_TASK_DTYPE = np.dtype([("pos", "<f8", 3), ("dir", "<f8", 3), ("zz", "<f8"), ("zz2", "<f8"), ("zzplus", "<f8"),
+                          ("len", "<f8"), ("vfov", "<f8", 3), ("kind", "<i4"), ("seed", "<i4"), ("flags", "<u4"),
+                          ("ms_de_stop", "<f4")])
+MC_RES_DTYPE = np.dtype([("pos", "<f8", 3), ("zz", "<f8"), ("zz2", "<f8"), ("len", "<f8"), ("normals", "<f8", 3),
+                         ("dyn_fog", "<f4"), ("ms_de_stop", "<f4"), ("kind", "<i4"), ("seed", "<i4"),
+                         ("si_grad", "<u2"), ("otrap", "<u2"), ("flags", "<u4")])
+
 
 PAINT_U8, PAINT_U16, PAINT_F32 = 0, 1, 2
 
@@ -466,6 +477,7 @@ SIGNATURES = {
     "mb3d_hip_mlp_infer": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p]),
     "mb3d_hip_march_rays": (C.c_int32, [C.POINTER(Params), C.c_void_p, C.c_int64, C.c_int32, C.POINTER(RayOpts),
                                         C.c_void_p, C.c_int32]),
+    "mb3d_hip_mc_batch": (C.c_int32, [C.POINTER(Params), C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int32]),
     "mb3d_hip_gbuf_pack": (C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
     "mb3d_hip_gbuf_unpack": (C.c_int32, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
     "mb3d_hip_lighting_default": (None, [C.POINTER(Lighting)]),
@@ -948,6 +960,57 @@ def reflection_tasks_from(p: Params, starts, vfovs, g: dict, off_steps=2.0):
     t["zz"] = d[idx].astype(np.float32)
     t["pixel"] = idx.astype(np.uint32)
     return t
+
+
+def mc_primary_tasks(p: Params, seed=1):
+    """The Monte Carlo renderer's primary rays (CalcRay's first march with
+    the inside probe, src/mc.rs:1915-1940) of every pixel: from its start
+    plane along its view ray (one march unit long), depth 0, zzplus 1, to
+    the far end; seeds from a seeded generator."""
+    r, step = rays(p)
+    n = r.shape[0]
+    t = np.zeros(n, MC_TASK_DTYPE)
+    t["pos"] = r[:, :3]
+    t["dir"] = r[:, 3:].astype(np.float64) * step[:, None]
+    t["vfov"] = t["dir"]
+    t["kind"] = MC_RAY
+    t["zzplus"] = 1.0
+    t["len"] = p.zend
+    t["flags"] = MC_PROBE
+    t["seed"] = np.random.default_rng(seed).integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32)
+    return t
+
+
+def mc_shadow_tasks(p: Params, res, vfov, light, length=300.0, trans=False):
+    """CalcHSMC's shadow rays (src/mc.rs:1513-1566) from the surfaces of a
+    batch's results towards a global light (a world direction towards it):
+    one step width long, `length` steps at most; the view depth per step
+    zz2mul from each result's view vector (`vfov`, one per result)."""
+    res = np.asarray(res, MC_RES_DTYPE)
+    surf = np.flatnonzero(res["kind"] == 1)
+    v = np.asarray(light, np.float64)
+    v = v * (p.step_width / np.linalg.norm(v))
+    vf = np.asarray(vfov, np.float64)[surf]
+    t = np.zeros(len(surf), MC_TASK_DTYPE)
+    t["pos"] = res["pos"][surf]
+    t["dir"] = v
+    t["vfov"] = vf
+    t["kind"] = MC_SHADOW
+    t["zz"] = t["zz2"] = res["zz"][surf]
+    t["zzplus"] = vf @ v / np.sqrt(np.sum(vf * vf, axis=1) * (v @ v))
+    t["len"] = length
+    t["seed"] = res["seed"][surf]
+    t["flags"] = MC_TRANS if trans else 0
+    return t, surf
+
+
+def mc_batch(p: Params, tasks, secant=False):
+    """mb3d_hip_mc_batch on host records (an MC_TASK_DTYPE array): the
+    results as an MC_RES_DTYPE array."""
+    t = np.ascontiguousarray(tasks, dtype=MC_TASK_DTYPE)
+    r = np.zeros(t.shape[0], MC_RES_DTYPE)
+    check(load().mb3d_hip_mc_batch(C.byref(p), _p(t), t.shape[0], int(secant), _p(r), 0), "mb3d_hip_mc_batch")
+    return r
 
 
 REC_BYTES = 18

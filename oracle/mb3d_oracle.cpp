@@ -348,6 +348,7 @@ struct Marcher {
     int32_t seed = 0;
     uint64_t de_evals = 0;
     int nrm_it = 0;  // the iteration count of calculate_normals' centre DE
+    bool calc_inside = false;  // the DE from inside the set (the Monte Carlo marches' inside flag)
 
     explicit Marcher(const mb3d_hip_params &pp) : p(pp) {
         // CalcParams::new_iteration (src/calc.rs:1054-1087)
@@ -673,9 +674,10 @@ struct Marcher {
 
     // calc_de_fractal (src/calc.rs:1352-1456) without the mix (kind 6) and
     // inside rendering
+    // Marcher::calc_de (src/calc.rs:1333-1455), ending in invert_inside
     double calc_de() {
         de_evals++;
-        if (!p.decomb) return calc_de_part(false);
+        if (!p.decomb) return invert_inside(calc_de_part(false));
         double result = calc_de_part(false);
         const int b_it = it.it_result, b_max = max_its_result;
         const double b_otrap = it.otrap;
@@ -734,7 +736,17 @@ struct Marcher {
             max_its_result = b_max;
         }
         set_part(part1);
-        return result;
+        return invert_inside(result);
+    }
+
+    // invert_inside (src/calc.rs:1457-1472): inside rendering, the distance
+    // to the surface from inside the set (no dIFS parts on this backend)
+    double invert_inside(double r) {
+        if (!calc_inside) return r;
+        const double ms = (double)ms_de_stop;
+        const double v = ms * 4.0 - r * 3.0;
+        if (v >= ms) it.it_result -= 1;
+        return v;
     }
 
     // calc_vgrads_fov (src/calc.rs:1561-1575), Common optic
@@ -784,12 +796,12 @@ struct Marcher {
         }
     }
 
-    // bin_search_it (src/calc.rs:1737-1792)
-    void bin_search_it() {
+    // bin_search_it (src/calc.rs:1737-1792); `steps` turns (< 0: the scene's)
+    void bin_search_it(int steps = -1) {
         float yp = (float)it.max_it - 0.99f;
         int saved_max = it.max_it;
         it.max_it += 1;
-        int itmp = p.de_add_steps;
+        int itmp = steps < 0 ? p.de_add_steps : steps;
         it.calc_sit = true;
         double dt1 = 0.0, dmul = 1.0;
         bool first = true;
@@ -2328,6 +2340,347 @@ extern "C" int32_t orc_march_rays(const mb3d_hip_params *pp, const mb3d_hip_ray_
             m.ms_de_stop = saved;
             to_words(si, h.gbuf);
         }
+    });
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// the Monte Carlo renderer's marches: McMarch::run (src/mc.rs:488-977) for
+// this backend's scenes -- outside rendering without the inside-and-outside
+// turn-around, no volumetric light, no ground plane, no dIFS parts, colour
+// option 0, normals on the DE.
+
+// GetRand (rand_of, src/mc.rs:470-476)
+static double mc_rand(int32_t &seed) {
+    seed = (int32_t)((uint32_t)seed * 0x343FDu + 0x269EC3u);
+    return (double)(seed & 0x7FFFFFFF) * (1.0 / 2147483647.0);
+}
+
+// not_zero (src/mc.rs:53-63)
+static double mc_not_zero(double d) {
+    if (std::fabs(d) < 1e-40) return d < 0.0 ? -1e-40 : 1e-40;
+    return d;
+}
+
+// update_de_stop of McMarch (src/mc.rs:504-508): the DE stop at view depth z
+static void mc_de_stop(Marcher &m, double z) {
+    m.ms_de_stop = (float)((double)m.p.de_stop * (1.0 + std::fabs(z) * (double)m.p.de_stop_factor));
+}
+
+// McMarch::normals with normals on the DE (src/mc.rs:531-590): the one-sided
+// gradient on the world axes z, x, y, its side from bit 7 of the seed, then
+// random offsets until it faces the ray (four tries); the smoothed
+// iteration count of the centre is returned
+static float mc_normals(Marcher &m, const double raydir[3], int32_t &seed, double out[3]) {
+    const mb3d_hip_params &p = m.p;
+    float sca = (seed & 128) == 0 ? 0.15f : -0.15f;
+    m.it.calc_sit = true;
+    const double no = m.calc_de();
+    const float nn = m.it.smooth_it;
+    m.it.calc_sit = false;
+    int i = 4;
+    double n[3] = {0, 0, 0};
+    bool found = false;
+    for (;;) {
+        if (i < 4) sca = (float)((mc_rand(seed) - 0.5) * 0.5);
+        const double off = (double)std::fmin(p.de_stop, 1.0f) * (1.0 + m.mzz * (double)p.de_stop_factor) * (double)sca *
+                           p.step_width;
+        for (int k : {2, 0, 1}) {
+            const double t = m.it.c[k];
+            m.it.c[k] = t + off;
+            n[k] = (m.calc_de() - no) * (double)sca;
+            m.it.c[k] = t;
+        }
+        const double l = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        if (l > 1e-100 && dot3(n, raydir) < 0.0) {
+            const double d = 1.0 / std::sqrt(l);
+            for (int k = 0; k < 3; k++) out[k] = n[k] * d;
+            found = true;
+            break;
+        }
+        i -= 1;
+        if (i == 0) break;
+    }
+    if (!found) {  // norm_to(-1, raydir)
+        const double d = -1.0 / std::sqrt(dot3(raydir, raydir) + 1e-100);
+        for (int k = 0; k < 3; k++) out[k] = raydir[k] * d;
+    }
+    return nn;
+}
+
+// McMarch::bin_search_it (src/mc.rs:592-643): ten turns on the ray from
+// depth zz, through the Marcher's bin_search_it with that depth and ray
+static void mc_bin_search_it(Marcher &m, double &zz, const double dir[3]) {
+    double vf[3] = {m.vfov[0], m.vfov[1], m.vfov[2]};
+    for (int k = 0; k < 3; k++) m.vfov[k] = dir[k];
+    const double mzz = m.mzz;
+    m.mzz = zz;
+    m.bin_search_it(10);
+    zz = m.mzz;
+    m.mzz = mzz;
+    for (int k = 0; k < 3; k++) m.vfov[k] = vf[k];
+}
+
+// surface_colour (src/mc.rs:645-662) with colour option 0: gradient, orbit trap
+static void mc_colour(Marcher &m, float nn, bool de_limited, mb3d_hip_mc_res &o) {
+    const mb3d_hip_params &p = m.p;
+    float v;
+    if (de_limited)
+        v = 32767.0f - (nn + p.d_col_plus + p.col_var_de_stop_mul * (float)std::log((double)m.ms_de_stop * p.step_width)) *
+                           p.mcts_m;
+    else
+        v = 32767.0f - nn * p.mcts_m;
+    o.si_grad = clip15(v);
+    o.otrap = clip15((float)(m.it.otrap * 4096.0));
+}
+
+// McMarch::ray (src/mc.rs:664-866): CalcRay's march
+static void mc_ray(Marcher &m, const mb3d_hip_mc_task &r, bool secant, bool inside, mb3d_hip_mc_res &o) {
+    const mb3d_hip_params &p = m.p;
+    int32_t seed = r.seed;
+    const double *dir = r.dir;
+    double zz = r.zz, zz2 = r.zz2;
+    const double max_l = r.len, zzplus = r.zzplus;
+    mc_de_stop(m, zz);
+    double dtmp = m.calc_de();
+    if ((r.flags & MB3D_HIP_MC_PROBE) && (m.it.it_result >= m.max_its_result || dtmp < (double)m.ms_de_stop)) {
+        // the start is inside the set (src/mc.rs:691-727; no inside-and-outside turn-around)
+        o.otrap = clip15((float)(m.it.otrap * 4096.0));
+        m.mzz = zz;
+        if (inside) {
+            m.it.calc_sit = true;
+            m.calc_de();
+            o.si_grad = (uint16_t)(clip15(m.it.smooth_it * p.mcts_m) | 32768u);
+        } else {
+            const double t = std::fmin(std::fmax(m.it.rout / p.d_rstop, 0.0), 1.0);
+            o.si_grad = (uint16_t)(32768u + (uint32_t)round_even(32767.0 * t));
+        }
+        o.kind = 2;
+        for (int k = 0; k < 3; k++) o.pos[k] = m.it.c[k];
+        o.flags = inside ? MB3D_HIP_MC_INSIDE : 0;
+        o.seed = seed;
+        return;
+    }
+    float dyn_fog = 0.0f, rsfmul = 1.0f;
+    bool open_air = false, de_limited = true, first_step = true;
+    double rlast_sw = dtmp * (double)p.z_step_div, rlast_de = 0.0;
+    uint32_t guard = 0;
+    for (;;) {
+        rlast_de = dtmp;
+        double dstep = std::fmax((dtmp - (double)p.ms_de_sub * (double)m.ms_de_stop) * (double)p.z_step_div * (double)rsfmul, 0.11);
+        const double d1 = (double)(std::fmax(m.ms_de_stop, 0.4f) * p.mh04zsd);
+        const bool count = p.dfog_on_it == 0 || m.it.it_result == p.dfog_on_it;
+        if (d1 < dstep) {
+            if (count) dyn_fog += (float)(d1 / dstep);
+            dstep = d1;
+        } else if (count) {
+            dyn_fog += 1.0f;
+        }
+        if (first_step) {
+            first_step = false;
+            dstep *= mc_rand(seed);
+        }
+        rlast_sw = dstep;
+        zz2 += dstep;
+        if (zz2 > max_l) {
+            open_air = true;
+            break;
+        }
+        add_weight(m.it.c, dir, dstep);
+        zz += dstep * zzplus;
+        mc_de_stop(m, zz);
+        dtmp = m.calc_de();
+        if (m.it.it_result >= m.max_its_result) {
+            de_limited = false;
+            break;
+        }
+        if (!(dtmp >= (double)m.ms_de_stop) && m.it.it_result >= p.min_it) break;
+        if (dtmp > rlast_de + rlast_sw) dtmp = rlast_de + rlast_sw;
+        if (rlast_de > dtmp + 1e-30) {
+            const double d = rlast_sw / (rlast_de - dtmp);
+            rsfmul = d < 1.0 ? (float)std::fmax(d, 0.5) : 1.0f;
+        } else {
+            rsfmul = 1.0f;
+        }
+        if (++guard > 1000000u) {
+            open_air = true;
+            break;
+        }
+    }
+    if (inside) dyn_fog *= 200.0f * p.de_stop / (float)p.width;
+    o.dyn_fog = dyn_fog;
+    o.zz2 = zz2;
+    o.flags = inside ? MB3D_HIP_MC_INSIDE : 0;
+    if (open_air) {
+        for (int k = 0; k < 3; k++) o.pos[k] = m.it.c[k];
+        o.zz = zz;
+        o.seed = seed;
+        return;
+    }
+    if (de_limited) {
+        if (secant) {  // src/mc.rs:806-830
+            int itmp = 4;
+            while (itmp > 0 && std::fabs(dtmp - (double)m.ms_de_stop) > 0.001) {
+                rlast_de = mc_not_zero(rlast_de - dtmp);
+                const double ms = (double)m.ms_de_stop;
+                const double f = rlast_sw * (dtmp - ms) / rlast_de;
+                if (dtmp < ms)
+                    rlast_sw = (f >= 0.0 || f < std::fabs(rlast_sw) * -0.94) ? std::fabs(rlast_sw) * -0.5 : f;
+                else
+                    rlast_sw = (f <= 0.0 || f > std::fabs(rlast_sw) * 0.94) ? std::fabs(rlast_sw) * 0.5 : f;
+                rlast_de = dtmp;
+                zz += rlast_sw * zzplus;
+                add_weight(m.it.c, dir, rlast_sw);
+                mc_de_stop(m, zz);
+                itmp -= 1;
+                if (itmp <= 0) break;
+                dtmp = m.calc_de();
+            }
+        } else {  // src/mc.rs:831-846
+            int itmp = 10;
+            double ds = rlast_sw * -0.5;
+            while (std::fabs(dtmp - (double)m.ms_de_stop) > 0.001) {
+                zz += ds * zzplus;
+                add_weight(m.it.c, dir, ds);
+                mc_de_stop(m, zz);
+                itmp -= 1;
+                if (itmp <= 0) break;
+                dtmp = m.calc_de();
+                ds = dtmp < (double)m.ms_de_stop ? std::fabs(ds) * -0.55 : std::fabs(ds) * 0.55;
+            }
+        }
+    } else {
+        double ds = zz;
+        mc_bin_search_it(m, ds, dir);
+        zz += (ds - zz) * zzplus;
+    }
+    m.mzz = std::fabs(zz);
+    double n[3];
+    const float nn = mc_normals(m, dir, seed, n);
+    for (int k = 0; k < 3; k++) o.pos[k] = m.it.c[k];
+    mc_colour(m, nn, de_limited, o);
+    o.kind = 1;
+    o.zz = zz;
+    if (de_limited) o.flags |= MB3D_HIP_MC_DE_LIMITED;
+    for (int k = 0; k < 3; k++) o.normals[k] = n[k];
+    o.seed = seed;
+}
+
+// McMarch::shadow (src/mc.rs:868-956): CalcHSMC's ray towards a light
+static void mc_shadow(Marcher &m, const mb3d_hip_mc_task &r, mb3d_hip_mc_res &o) {
+    const mb3d_hip_params &p = m.p;
+    int32_t seed = r.seed;
+    const double *v = r.dir;
+    double dt1 = r.len, zz2 = r.zz2;
+    const double zz2mul = r.zzplus;
+    mc_de_stop(m, r.zz);
+    float rsfd = 2.0f;
+    double dtmp = m.calc_de(), rlast_de = 0.0, rlast_sw = 0.0;
+    for (;;) {
+        rlast_de = dtmp;
+        dtmp = std::fmin(dtmp * (double)p.z_step_div * (double)rsfd, (double)(std::fmax(m.ms_de_stop, 0.4f) * p.mh04zsd));
+        rlast_sw = dtmp;
+        dt1 -= dtmp;
+        add_weight(m.it.c, v, dtmp);
+        zz2 += dtmp * zz2mul;
+        mc_de_stop(m, zz2);
+        dtmp = m.calc_de();
+        if (m.it.it_result >= m.max_its_result || !(dtmp >= (double)m.ms_de_stop) || !(dtmp < 1e30)) break;
+        if (dtmp > rlast_de + rlast_sw) dtmp = rlast_de + rlast_sw;
+        if (rlast_de > dtmp + 1e-30) {
+            const double s = rlast_sw / (rlast_de - dtmp);
+            rsfd = s < 1.0 ? (float)std::fmax(s, 0.5) : 1.0f;
+        } else {
+            rsfd = 1.0f;
+        }
+        if (!(dt1 >= 0.0)) break;
+    }
+    o.len = dt1;
+    o.seed = seed;
+    if (dt1 <= 0.01 || !(r.flags & MB3D_HIP_MC_TRANS)) return;
+    // a transparent object between the surface and the light
+    if (m.it.it_result < m.max_its_result) {
+        int itmp = 3;
+        while (itmp > 0 && std::fabs(dtmp - (double)m.ms_de_stop) > 0.01) {
+            rlast_de = mc_not_zero(rlast_de - dtmp);
+            const double ms = (double)m.ms_de_stop;
+            const double f = rlast_sw * (dtmp - ms) / rlast_de;
+            if (dtmp < ms)
+                rlast_sw = (f >= 0.0 || f < std::fabs(rlast_sw) * -0.94) ? std::fabs(rlast_sw) * -0.5 : f;
+            else
+                rlast_sw = (f <= 0.0 || f > std::fabs(rlast_sw) * 0.94) ? std::fabs(rlast_sw) * 0.5 : f;
+            rlast_de = dtmp;
+            zz2 += rlast_sw * zz2mul;
+            add_weight(m.it.c, v, rlast_sw);
+            mc_de_stop(m, zz2);
+            dtmp = m.calc_de();
+            itmp -= 1;
+        }
+    } else {
+        double d = zz2;
+        mc_bin_search_it(m, d, v);
+        zz2 += (d - zz2) * zz2mul;
+    }
+    m.mzz = std::fabs(zz2);
+    double n[3];
+    const float nn = mc_normals(m, v, seed, n);
+    for (int k = 0; k < 3; k++) o.pos[k] = m.it.c[k];
+    // the iteration count after the normals, as in MB3D's CalcHSMC
+    mc_colour(m, nn, m.it.it_result < m.max_its_result, o);
+    o.kind = 1;
+    o.zz2 = zz2;
+    for (int k = 0; k < 3; k++) o.normals[k] = n[k];
+    o.ms_de_stop = m.ms_de_stop;
+    o.seed = seed;
+}
+
+// McMarch::through (src/mc.rs:958-977): from the far end back towards the
+// start, in steps of at least one, until the DE stop or the start
+static void mc_through(Marcher &m, const mb3d_hip_mc_task &r, mb3d_hip_mc_res &o) {
+    int32_t seed = r.seed;
+    const double *v = r.dir;
+    double dt1 = r.len;
+    m.ms_de_stop = r.ms_de_stop;
+    add_weight(m.it.c, v, dt1);
+    double d = mc_rand(seed) * 8.0 * std::fmax((double)m.ms_de_stop, 1.0);
+    for (;;) {
+        d = -1.0 - std::fabs(d);
+        dt1 += d;
+        add_weight(m.it.c, v, d);
+        d = m.calc_de();
+        if (d > (double)m.ms_de_stop || !(dt1 > 0.0)) break;
+    }
+    o.len = dt1;
+    o.seed = seed;
+}
+
+// McMarch::run (src/mc.rs:488-502) for one task, on a fresh marcher
+static void mc_run(const mb3d_hip_params &p, const mb3d_hip_mc_task &r, bool secant, mb3d_hip_mc_res &o) {
+    Marcher m(p);
+    std::memset(&o, 0, sizeof o);
+    for (int k = 0; k < 3; k++) m.it.c[k] = r.pos[k];
+    const bool inside = (r.flags & MB3D_HIP_MC_INSIDE) != 0;
+    m.calc_inside = inside;
+    m.it.calc_sit = false;
+    if (r.kind == MB3D_HIP_MC_RAY)
+        mc_ray(m, r, secant, inside, o);
+    else if (r.kind == MB3D_HIP_MC_SHADOW)
+        mc_shadow(m, r, o);
+    else
+        mc_through(m, r, o);
+}
+
+extern "C" int32_t orc_mc_batch(const mb3d_hip_params *pp, const mb3d_hip_mc_task *tasks, int64_t n, int32_t secant,
+                                mb3d_hip_mc_res *res, int32_t threads) {
+    if (int e = check_params(pp)) return e;
+    if (n < 0 || (n > 0 && (!tasks || !res))) return fail("bad argument");
+    for (int64_t i = 0; i < n; i++)
+        if (tasks[i].kind < MB3D_HIP_MC_RAY || tasks[i].kind > MB3D_HIP_MC_THROUGH) return fail("task kind not 0..2");
+    const int64_t chunk = 64;
+    const int64_t nchunks = (n + chunk - 1) / chunk;
+    for_rows_interleaved((int)nchunks, nthreads(threads), [&](int c) {
+        for (int64_t i = (int64_t)c * chunk; i < std::min(n, (int64_t)(c + 1) * chunk); i++)
+            mc_run(*pp, tasks[i], secant != 0, res[i]);
     });
     return 0;
 }

@@ -59,6 +59,8 @@ def load():
     lib.orc_march_rays.argtypes = [C.POINTER(H.Params), C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p,
                                    C.c_int32]
     lib.orc_march_rays.restype = C.c_int32
+    lib.orc_mc_batch.argtypes = [C.POINTER(H.Params), C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int32]
+    lib.orc_mc_batch.restype = C.c_int32
     lib.orc_gbuf_pack.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
     lib.orc_gbuf_pack.restype = None
     lib.orc_gbuf_unpack.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
@@ -238,3 +240,15 @@ def march_rays(p, tasks, depth_sqrt=False, opts=None, threads=0):
     if st != 0:
         raise RuntimeError(f"orc_march_rays failed ({st})")
     return h
+
+
+def mc_batch(p, tasks, secant=False, threads=0):
+    """McMarch::run for each task (H.MC_TASK_DTYPE records): H.MC_RES_DTYPE
+    results."""
+    t = np.ascontiguousarray(tasks, dtype=H.MC_TASK_DTYPE)
+    r = np.zeros(t.shape[0], H.MC_RES_DTYPE)
+    st = load().orc_mc_batch(C.byref(p), t.ctypes.data_as(C.c_void_p), t.shape[0], int(secant),
+                             r.ctypes.data_as(C.c_void_p), threads)
+    if st != 0:
+        raise RuntimeError(f"orc_mc_batch failed ({st}): {load().orc_last_error().decode()}")
+    return r

@@ -49,7 +49,10 @@ def test_ctypes_layouts_match_the_header(tmp_path):
                "mb3d_hip_counters": H.Counters, "mb3d_hip_light": H.Light,
                "mb3d_hip_palette_color": H.PaletteColor, "mb3d_hip_lighting": H.Lighting,
                "mb3d_hip_light_val": H.LightVal, "mb3d_hip_light_vals": H.LightVals,
-               "mb3d_hip_paint_camera": H.PaintCamera}
+               "mb3d_hip_paint_camera": H.PaintCamera, "mb3d_hip_ray_opts": H.RayOpts}
+    # the task and result records as numpy record types
+    dtypes = {"mb3d_hip_ray_task": H.RAY_TASK_DTYPE, "mb3d_hip_ray_hit": H.RAY_HIT_DTYPE,
+              "mb3d_hip_mc_task": H.MC_TASK_DTYPE, "mb3d_hip_mc_res": H.MC_RES_DTYPE}
     lines, want = [], []
     for cname, cls in structs.items():
         lines.append(f'printf("%zu\\n", sizeof({cname}));')
@@ -57,6 +60,12 @@ def test_ctypes_layouts_match_the_header(tmp_path):
         for fname, _ in cls._fields_:
             lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
             want.append(getattr(cls, fname).offset)
+    for cname, dt in dtypes.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(dt.itemsize)
+        for fname in dt.names:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
+            want.append(dt.fields[fname][1])
     src = tmp_path / "layout.c"
     src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "mb3d_hip.h"\nint main(void) {\n'
                    + "\n".join(lines) + "\nreturn 0;\n}\n")
