@@ -361,7 +361,7 @@ def paint_cpu_leg(cam, lv, g, threads):
             "sample": f"one full {cam.width}x{cam.height} G-buffer, {dt:.3f} s"}
 
 
-def rays_cpu_leg(p, tasks, opts, threads, sample=100_000):
+def rays_cpu_leg(p, tasks, opts, threads, sample=1_000_000):
     """The oracle's secondary-ray march and hit pass (a restatement of the
     reference's SrTracer::march_ray, calc_hs and deao_1pos) on the host
     cores, on every k-th task of the same batch (about `sample` of them, so
@@ -378,7 +378,7 @@ def rays_cpu_leg(p, tasks, opts, threads, sample=100_000):
             "sample": f"every {k}th ray of the batch, {len(t_s)} rays, {dt:.2f} s"}
 
 
-def mc_cpu_leg(p, prim, shad, threads, sample=100_000):
+def mc_cpu_leg(p, prim, shad, threads, sample=4_000_000):
     """The oracle's Monte Carlo marches (a restatement of the reference's
     McMarch::run) on the host cores, on every k-th task of each batch."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
