@@ -15,7 +15,15 @@
 //   surface            surface_found, calc_zpos_and_rough  src/calc.rs:2194-2205,2471-2517
 //   hard shadows       hard_shadow_row, per pixel (no row shortcut)  src/calc.rs:2741-2899
 //   DE AO              deao_row, surface_point   src/deao.rs:268-418, src/calc.rs:2157-2192
+//   secondary rays     SrTracer::march_ray, calc_hs, deao_1pos  src/reflect.rs:805-1090,1464-1674
+//   Monte Carlo        McMarch::run (ray, shadow, through)      src/mc.rs:488-977
 // Built with -ffp-contract=off: no fused multiply-adds, like the reference.
+//
+// Parity unpinned: the DE AO restatements (deao_row and the hit pass's
+// deao_1pos).  Only the reference's sanity test (some pixels occluded) and
+// their agreement with each other check them; no closed form exists with
+// the formulas this path carries.  Every other function here is checked
+// against known answers, closed forms or brute force in tests/.
 #include "mb3d_oracle.h"
 
 #include <algorithm>
@@ -1920,6 +1928,22 @@ extern "C" double orc_de_at(const mb3d_hip_params *p, const double pos[3], int32
     double d = m.calc_de();
     if (its) *its = m.it.it_result;
     return d * p->step_width;
+}
+
+extern "C" int32_t orc_de_many(const mb3d_hip_params *p, const double *pos, int64_t n, double *de, int32_t *its,
+                               int32_t threads) {
+    if (int e = check_params(p)) return e;
+    const int64_t chunk = 4096;
+    const int64_t nchunks = (n + chunk - 1) / chunk;
+    for_rows_interleaved((int)nchunks, nthreads(threads), [&](int c) {
+        Marcher m(*p);
+        for (int64_t i = (int64_t)c * chunk; i < std::min(n, (int64_t)(c + 1) * chunk); i++) {
+            for (int k = 0; k < 3; k++) m.it.c[k] = pos[3 * i + k];
+            de[i] = m.calc_de();
+            its[i] = m.it.it_result;
+        }
+    });
+    return 0;
 }
 
 extern "C" void orc_pixel_ray(const mb3d_hip_params *p, int32_t x, int32_t y, double c[3], double v[3]) {

@@ -80,6 +80,10 @@ double orc_march_ray(const mb3d_hip_params *p, const double origin[3], const dou
 int32_t orc_march_rays(const mb3d_hip_params *p, const mb3d_hip_ray_task *tasks, int64_t n, int32_t depth_sqrt,
                        const mb3d_hip_ray_opts *opts, mb3d_hip_ray_hit *hits, int32_t threads);
 
+/* the DE (in steps) and iteration count at n points (3 f64 each), as
+ * orc_de_at without its step-width scale */
+int32_t orc_de_many(const mb3d_hip_params *p, const double *pos, int64_t n, double *de, int32_t *its, int32_t threads);
+
 /* the Monte Carlo renderer's marches (McMarch::run, src/mc.rs:488-977), as
  * mb3d_hip_mc_batch */
 int32_t orc_mc_batch(const mb3d_hip_params *p, const mb3d_hip_mc_task *tasks, int64_t n, int32_t secant,

@@ -59,6 +59,8 @@ def load():
     lib.orc_march_rays.argtypes = [C.POINTER(H.Params), C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p,
                                    C.c_int32]
     lib.orc_march_rays.restype = C.c_int32
+    lib.orc_de_many.argtypes = [C.POINTER(H.Params), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32]
+    lib.orc_de_many.restype = C.c_int32
     lib.orc_mc_batch.argtypes = [C.POINTER(H.Params), C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int32]
     lib.orc_mc_batch.restype = C.c_int32
     lib.orc_gbuf_pack.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
@@ -140,6 +142,17 @@ def de_at(p, pos):
     its = C.c_int32()
     d = load().orc_de_at(C.byref(p), _ptr(a), C.byref(its))
     return d, its.value
+
+
+def de_many(p, pos, threads=0):
+    """The DE in steps and the iteration count at each of the points (n x 3)."""
+    a = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    de = np.zeros(a.shape[0])
+    its = np.zeros(a.shape[0], np.int32)
+    st = load().orc_de_many(C.byref(p), _ptr(a), a.shape[0], _ptr(de), _ptr(its), threads)
+    if st != 0:
+        raise RuntimeError(f"orc_de_many failed ({st})")
+    return de, its
 
 
 def pixel_ray(p, x, y):

@@ -226,6 +226,48 @@ def test_shadows_mark_pixels():
     assert (sb[surf] != 0).mean() < (shadowed[surf] != 0).mean()
 
 
+@pytest.mark.parametrize("preset,light", [("Integer Power", (0.6, -0.5, -0.6)), ("Amazing Box", (-0.3, 0.8, -0.5)),
+                                          ("Integer Power", (0.0, 0.9, 0.4))])
+def test_hard_shadows_against_brute_force(preset, light):
+    """hard_shadow_row's bit against a brute-force walk towards the light:
+    from the surface point 0.1 step back along the view ray, the DE sampled
+    every 0.05 step over the pass's reach (hs_max_len, src/calc.rs:2767-2775);
+    blocked where it reaches the DE stop of its depth or the iteration limit,
+    shadowed also where the surface faces away.  The pass never shadows a
+    pixel the walk finds lit; it may miss an occluder thinner than its step."""
+    from test_oracle_mc import view_tasks, world_normals
+
+    sc = H.scene_preset(preset, 48, 36)
+    sc.first_step_random = 0
+    p = O.params(sc)
+    job = H.post_job(lights=[light], params=p)
+    r = O.render(p, job, threads=0)
+    t = view_tasks(p)
+    surf = np.flatnonzero(np.isfinite(r["depth"]) & (r["depth"] > 0))
+    bit = ((r["gbuf"][surf, 1] >> 16) & 0x400) != 0
+    lvec = np.array(job.light_vec[0][:])
+    vf = t["dir"][surf]
+    mzz = np.maximum(r["depth"][surf] - 0.1, 0.0)
+    start = t["pos"][surf] + vf * (r["depth"][surf] - 0.1)[:, None]
+    zz2mul = -(vf @ lvec) / np.sqrt((vf * vf).sum(1) * (lvec @ lvec))
+    y = surf // p.width
+    f32 = np.float32
+    mz = np.minimum(mzz, p.zend * 0.4).astype(f32)
+    reach = ((p.width + y).astype(f32) * f32(0.6) * (f32(1) + f32(0.5) * mz * f32(max(p.fov_y, 0)) / f32(p.height))
+             * f32(job.shadow_max_len_mul)).astype(np.float64)
+    away = world_normals(p, r["normals"])[surf] @ lvec > 0
+    blocked = np.zeros(len(surf), bool)
+    for i in range(len(surf)):
+        s = np.arange(0.05, reach[i] + 1e-9, 0.05)
+        de, its = O.de_many(p, start[i] + np.outer(s, -lvec))
+        ms = p.de_stop * (1.0 + np.abs(mzz[i] + s * zz2mul[i]) * p.de_stop_factor)
+        blocked[i] = np.any((de <= ms) | (its >= p.max_it))
+    brute = blocked | away
+    assert 0.05 < brute.mean() < 0.95
+    assert not np.any(bit & ~brute)
+    assert (bit == brute).mean() >= 0.99, ((bit == brute).mean(), bit.mean(), brute.mean())
+
+
 def test_deao_occludes():
     sc = preset("Integer Power")
     sc.first_step_random = 0
