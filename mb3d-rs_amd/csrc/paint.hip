@@ -567,8 +567,11 @@ __device__ void finish_colour(const mb3d_hip_light_vals &lv, const float o[3], u
     }
 }
 
+// Four waves per SIMD (128 VGPRs, a few spills): 64 us against 72 us at the
+// compiler's three (163 VGPRs) on a 1080p G-buffer; five and six spill more
+// (70 and 68 us)
 template <int FMT>
-__global__ void __launch_bounds__(kPaintBlock) k_paint(mb3d_hip_paint_camera cam, mb3d_hip_light_vals lv_arg,
+__global__ void __launch_bounds__(kPaintBlock, 4) k_paint(mb3d_hip_paint_camera cam, mb3d_hip_light_vals lv_arg,
                                                        const uint32_t *__restrict__ gbuf, int x0, int y0, int w,
                                                        long long npix, unsigned char *__restrict__ rgb) {
     constexpr int B = FMT == MB3D_HIP_PAINT_U8 ? 3 : FMT == MB3D_HIP_PAINT_U16 ? 6 : 12;  // bytes a pixel
