@@ -410,4 +410,78 @@ Mixed march_mixed(const mb3d_hip_params &p, const mb3d_hip_post_job *post, size_
     return res;
 }
 
+McShared mc_batch_shared(const mb3d_hip_params &p, const std::vector<mb3d_hip_mc_task> &tasks, bool secant,
+                         size_t threads, const CpuTasks &cpu, size_t chunk, size_t cpu_chunk, const Cancel &cancel) {
+    const size_t n = tasks.size();
+    McShared out;
+    out.res.assign(n, mb3d_hip_mc_res{});
+    out.side.assign(n, 0);
+    std::mutex mu;
+    size_t front = 0, back = n;  // the free tasks [front, back)
+    std::atomic<size_t> card_n{0}, cpu_n{0};
+    std::atomic<bool> stop{false};
+    auto stopped = [&] {
+        if (!stop.load() && cancel && cancel()) stop = true;
+        return stop.load();
+    };
+    // up to `max` free tasks [a, b) from the front (the card) or the back (the CPU)
+    auto claim = [&](bool from_front, size_t max, size_t &a, size_t &b) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (front >= back) return false;
+        const size_t m = std::min(std::max<size_t>(max, 1), back - front);
+        if (from_front) {
+            a = front;
+            b = front = front + m;
+        } else {
+            b = back;
+            a = back = back - m;
+        }
+        return true;
+    };
+    auto on_cpu = [&](size_t a, size_t b) {
+        cpu(tasks.data() + a, b - a, out.res.data() + a);
+        std::fill(out.side.begin() + a, out.side.begin() + b, (uint8_t)2);
+        cpu_n += b - a;
+    };
+    std::vector<std::thread> pool;
+    if (cpu)
+        for (size_t i = 0; i < threads; i++)
+            pool.emplace_back([&] {
+                size_t a = 0, b = 0;
+                while (!stopped() && claim(false, cpu_chunk, a, b)) on_cpu(a, b);
+            });
+    Status card_status;
+    const std::string why = may_start(p);
+    if (!why.empty()) {
+        card_status = Status::fail(why);
+    } else {
+        size_t a = 0, b = 0;
+        while (!stopped() && claim(true, chunk, a, b)) {
+            if (mb3d_hip_mc_batch(&p, tasks.data() + a, (int64_t)(b - a), secant ? 1 : 0, out.res.data() + a, 0) !=
+                MB3D_HIP_OK) {
+                card_status = Status::fail(error_text());
+                if (cpu) on_cpu(a, b);  // the failed chunk goes to the CPU marcher
+                break;
+            }
+            std::fill(out.side.begin() + a, out.side.begin() + b, (uint8_t)1);
+            card_n += b - a;
+        }
+    }
+    // a card that did not start or failed: the calling thread joins the CPU's side
+    if (!card_status.ok && cpu) {
+        size_t a = 0, b = 0;
+        while (!stopped() && claim(false, cpu_chunk, a, b)) on_cpu(a, b);
+    }
+    for (auto &t : pool) t.join();
+    out.card = card_n;
+    out.cpu = cpu_n;
+    if (stop.load() && out.card + out.cpu < n)
+        out.status = Status::fail("cancelled");
+    else if (out.card + out.cpu < n)
+        out.status = Status::fail(card_status.ok ? std::string("no CPU marcher") : card_status.error);
+    else
+        out.status = card_status;
+    return out;
+}
+
 }  // namespace mb3d::gpu

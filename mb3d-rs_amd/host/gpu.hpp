@@ -118,4 +118,27 @@ Mixed march_mixed(const mb3d_hip_params &p, const mb3d_hip_post_job *post, size_
                   const std::vector<SiLight> *have = nullptr, size_t band_rows = 16,
                   const mb3d_hip_options *options = nullptr);
 
+// ---- the Monte Carlo renderer's batches shared with the CPU's threads
+// (gpu::mc_batch with RaySharing, src/gpu.rs:3013-3190: the card starts the
+// tasks from the front, the CPU takes them from the end) ----
+
+// The caller's CPU marcher of n Monte Carlo tasks (McMarch::run) into `res`.
+using CpuTasks = std::function<void(const mb3d_hip_mc_task *tasks, size_t n, mb3d_hip_mc_res *res)>;
+
+struct McShared {
+    std::vector<mb3d_hip_mc_res> res;
+    std::vector<uint8_t> side;  // per task: 1 the card marched it, 2 the CPU, 0 nobody (cancelled)
+    size_t card = 0, cpu = 0;   // tasks each side marched
+    Status status;              // fail("cancelled") when stopped, or the card's error
+};
+
+// The batch marched by the card in chunks of `chunk` tasks from the front
+// and by `threads` CPU threads (cpu) in chunks of `cpu_chunk` from the end,
+// until they meet: every task once, by one side.  A card that fails or
+// cannot start leaves the rest to the CPU threads (status then names its
+// error, and every task is still marched).
+McShared mc_batch_shared(const mb3d_hip_params &p, const std::vector<mb3d_hip_mc_task> &tasks, bool secant,
+                         size_t threads, const CpuTasks &cpu, size_t chunk = 65536, size_t cpu_chunk = 256,
+                         const Cancel &cancel = {});
+
 }  // namespace mb3d::gpu

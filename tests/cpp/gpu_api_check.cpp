@@ -213,9 +213,49 @@ static int mixed_checks(const std::string &params_path, const std::string &cpu_p
     return 0;
 }
 
+// gpu::mc_batch_shared: the CPU marcher is the oracle's results the test
+// computed beforehand (CPU_RES, one mb3d_hip_mc_res a task), copied after
+// `delay_us` per chunk.  Writes the results and each task's side.
+static int mc_share_checks(const std::string &params_path, const std::string &tasks_path, const std::string &cpu_path,
+                           const std::string &outdir, int threads, int delay_us, int chunk) {
+    const auto pb = read_file(params_path);
+    mb3d_hip_params p;
+    if (pb.size() != sizeof p) return fail("params size");
+    std::memcpy(&p, pb.data(), sizeof p);
+    const auto tb = read_file(tasks_path);
+    const auto cb = read_file(cpu_path);
+    const size_t n = tb.size() / sizeof(mb3d_hip_mc_task);
+    if (tb.size() != n * sizeof(mb3d_hip_mc_task) || cb.size() != n * sizeof(mb3d_hip_mc_res))
+        return fail("tasks or results size");
+    std::vector<mb3d_hip_mc_task> tasks(n);
+    std::memcpy(tasks.data(), tb.data(), tb.size());
+    const auto *want = reinterpret_cast<const mb3d_hip_mc_res *>(cb.data());
+    std::atomic<size_t> calls{0};
+    g::CpuTasks cpu = [&](const mb3d_hip_mc_task *t, size_t m, mb3d_hip_mc_res *r) {
+        if (delay_us > 0) std::this_thread::sleep_for(std::chrono::microseconds(delay_us));
+        const size_t i0 = (size_t)(t - tasks.data());
+        std::memcpy(r, want + i0, m * sizeof *r);
+        calls++;
+    };
+    const auto res = g::mc_batch_shared(p, tasks, false, (size_t)threads, cpu, (size_t)chunk, 64);
+    {
+        std::ofstream f(outdir + "/mc_res.bin", std::ios::binary);
+        f.write(reinterpret_cast<const char *>(res.res.data()), (std::streamsize)(n * sizeof(mb3d_hip_mc_res)));
+        std::ofstream s(outdir + "/mc_side.bin", std::ios::binary);
+        s.write(reinterpret_cast<const char *>(res.side.data()), (std::streamsize)n);
+    }
+    std::printf("card %zu cpu %zu calls %zu status %s\n", res.card, res.cpu, calls.load(),
+                res.status.ok ? "ok" : res.status.error.c_str());
+    std::printf("ok\n");
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc >= 2 && std::string(argv[1]) == "cpu") return cpu_checks();
     if (argc >= 5 && std::string(argv[1]) == "gpu") return gpu_checks(argv[2], argv[3], argv[4]);
+    if (argc >= 9 && std::string(argv[1]) == "mcshare")
+        return mc_share_checks(argv[2], argv[3], argv[4], argv[5], std::atoi(argv[6]), std::atoi(argv[7]),
+                               std::atoi(argv[8]));
     if (argc >= 8 && std::string(argv[1]) == "mixed")
         return mixed_checks(argv[2], argv[3], argv[4], std::atoi(argv[5]), std::atoi(argv[6]), std::atoi(argv[7]));
     std::fprintf(stderr, "usage: gpu_api_check cpu | gpu PARAMS POST|- OUTDIR | mixed PARAMS CPU_GBUF OUTDIR THREADS "

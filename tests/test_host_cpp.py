@@ -85,3 +85,62 @@ def test_march_mixed_stops_when_asked(hip, tmp_path):
     p, got, sinks, card, cpu, card_rows, cpu_rows, out = _mixed(hip, tmp_path, 2, 3000, stop_after=20)
     assert "cancelled" in out and sinks.sum() < p.height - 12, out
     assert np.all(sinks <= 1)
+
+
+def _mc_share(tmp_path, p, threads, delay_us, chunk, env=None):
+    import oracle as O
+
+    tasks = H.mc_primary_tasks(p, seed=3) if env is None else _cpu_tasks(p)
+    (tmp_path / "p.bin").write_bytes(bytes(p))
+    tasks.tofile(tmp_path / "tasks.bin")
+    want = O.mc_batch(p, tasks)
+    want.tofile(tmp_path / "want.bin")
+    r = subprocess.run([CHECK, "mcshare", str(tmp_path / "p.bin"), str(tmp_path / "tasks.bin"),
+                        str(tmp_path / "want.bin"), str(tmp_path), str(threads), str(delay_us), str(chunk)],
+                       capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
+    info = r.stdout.split()
+    got = np.fromfile(tmp_path / "mc_res.bin", H.MC_RES_DTYPE)
+    side = np.fromfile(tmp_path / "mc_side.bin", np.uint8)
+    return tasks, want, got, side, int(info[1]), int(info[3]), r.stdout
+
+
+def _cpu_tasks(p):
+    """View rays of the picture from its start plane, built on the host
+    (no device call)."""
+    from test_oracle_mc import view_tasks
+
+    t = view_tasks(p)
+    t["flags"] = H.MC_PROBE
+    return t
+
+
+def test_mc_batch_shared_without_the_card(tmp_path):
+    """gpu::mc_batch_shared with the card switched off (MB3D_GPU=0): the
+    CPU threads march every task once, from the end."""
+    import os
+
+    p = H.params_from_scene(H.scene_preset("Integer Power", 40, 30))
+    env = dict(os.environ, MB3D_GPU="0")
+    tasks, want, got, side, card, cpu, out = _mc_share(tmp_path, p, 3, 0, 256, env=env)
+    assert card == 0 and cpu == len(tasks) and np.all(side == 2), out
+    assert "switched off" in out
+    assert got.tobytes() == want.tobytes()
+
+
+@pytest.mark.gpu
+def test_mc_batch_shared_with_the_card(hip, tmp_path):
+    """gpu::mc_batch_shared (gpu::mc_batch with RaySharing,
+    src/gpu.rs:3013-3190): the card from the front in chunks of 1024, four
+    CPU threads (the oracle's results, 3 ms a chunk of 64) from the end;
+    every task once, by one side, equal to that side's result."""
+    p = hip.params_from_scene(H.scene_bulb(160, 120, seed=1))
+    tasks, want, got, side, card, cpu, out = _mc_share(tmp_path, p, 4, 3000, 1024)
+    assert card > 0 and cpu > 0 and card + cpu == len(tasks), out
+    assert np.all((side == 1) | (side == 2)) and (side == 1).sum() == card, out
+    # the card's tasks are a prefix, the CPU's a suffix
+    first_cpu = int(np.argmax(side == 2))
+    assert np.all(side[:first_cpu] == 1) and np.all(side[first_cpu:] == 2), out
+    dev = hip.mc_batch(p, tasks)
+    assert got[side == 1].tobytes() == dev[side == 1].tobytes()
+    assert got[side == 2].tobytes() == want[side == 2].tobytes()
