@@ -222,6 +222,18 @@ def main():
              "peak_gbs": PEAK_HBM / 1e9, "frac_hbm": paint_bytes / (t_paint * 1e-3) / PEAK_HBM}
     paint_gbuf = gbuf.cpu().numpy().view(np.uint32).reshape(n, 5)
 
+    # --- the post passes (beside the metric): the frame marched with one hard
+    # shadow and quality-1 DE AO (k_shadow, k_deao); their HIP-event time
+    post = H.post_job(lights=[(0.6, -0.5, -0.6)], deao_quality=1, params=p)
+    post_ms = []
+    for i in range(a.warmup + max(3, a.steps // 2)):
+        H.check(lib.mb3d_hip_march(H.C.byref(p), H.C.byref(post), 0, a.height, H.C.byref(out_g), H.C.byref(plain_opts),
+                                   H.PROGRESS_FN(0), H.CANCEL_FN(0), None), "mb3d_hip_march")
+        if i >= a.warmup:
+            post_ms.append(H.counters()["post_ms"])
+    post_leg = {"kernels": "k_shadow + k_deao", "post_ms": float(np.mean(post_ms)),
+                "mpix_s": n / (float(np.mean(post_ms)) * 1e-3) / 1e6, "job": "1 hard shadow, DE AO quality 1 (8 rays)"}
+
     # --- secondary rays (beside the metric): the frame's first-bounce
     # reflection rays from every surface pixel, marched on the device with
     # one hard shadow and quality-2 DE AO at their hits (tasks and hits in
@@ -304,6 +316,7 @@ def main():
                     "train_loss": [float(curve[:20].mean()), float(curve[-100:].mean())]},
         "roofline": roofline,
         "paint": paint,
+        "post": post_leg,
         "rays": rays,
         "mc": mc,
     }
