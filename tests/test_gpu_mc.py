@@ -175,3 +175,25 @@ def test_the_frame_batches_of_the_benchmark(hip):
     sg, sw = run_both(hip, p, shad)
     compare(sg, sw, "bench shadow")
     assert 0.02 < (sw["len"] > 0.01).mean() < 0.98
+
+
+def test_a_task_of_no_kind_in_device_memory(hip):
+    """Device tasks are not checked on the host: a kind out of 0..2 returns
+    kind -1 with its seed and no march; the other tasks are unaffected."""
+    import torch
+
+    p = hip.params_from_scene(H.scene_preset("Integer Power", 48, 36))
+    t = view_tasks(p)
+    host = hip.mc_batch(p, t)
+    bad = t.copy()
+    bad["kind"][::5] = 7
+    dt = torch.from_numpy(bad.view(np.uint8).copy()).cuda()
+    dr = torch.zeros(len(t) * H.MC_RES_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    H.check(H.load().mb3d_hip_mc_batch(C.byref(p), C.c_void_p(dt.data_ptr()), len(t), 0,
+                                       C.c_void_p(dr.data_ptr()), 1), "mc_batch")
+    torch.cuda.synchronize()
+    dev = dr.cpu().numpy().view(H.MC_RES_DTYPE)
+    assert np.all(dev["kind"][::5] == -1) and np.array_equal(dev["seed"][::5], t["seed"][::5])
+    keep = np.ones(len(t), bool)
+    keep[::5] = False
+    assert dev[keep].tobytes() == host[keep].tobytes()
