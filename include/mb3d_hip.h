@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MB3D_HIP_ABI_VERSION 2
+#define MB3D_HIP_ABI_VERSION 3
 #define MB3D_HIP_MAX_SLOTS 6
 #define MB3D_HIP_MAX_LIGHTS 6
 
@@ -230,7 +230,7 @@ typedef struct mb3d_hip_mlp mb3d_hip_mlp;
 /* Launch and precision options of one call (zeros: the defaults). */
 typedef struct mb3d_hip_options {
     int32_t precision;     /* MB3D_HIP_PREC_* */
-    int32_t tile_w, tile_h; /* pixel tile of one wave's work item (default 8 x 8) */
+    int32_t tile_w, tile_h; /* pixel tile of the work queue (default 8 x 8; powers of two up to 64) */
     int32_t waves_per_cu;  /* persistent grid: waves per CU (default 16) */
     int32_t it_split;      /* extension: FP32 march steps after which a ray goes on in the f64 finish (0: never) */
     float guard_band;      /* FP32 stop decisions within (1 + guard_band) x DE stop go to FP64 (default 0.25) */
@@ -245,6 +245,17 @@ typedef struct mb3d_hip_options {
      * background without a march -- unverified, so opt-in. */
     const mb3d_hip_mlp *mlp;
     float mlp_conf_hit, mlp_conf_miss, mlp_margin_rel, mlp_margin_abs;
+    /* extension: idle lanes a wave of the FP32 march gathers before it
+     * refills them with new work items (0: the default, 1: every turn) */
+    int32_t refill_min;
+    /* extension, with the warm start: a verified miss decision.  Each tile
+     * of the queue marches one coarse cone ray (the cone march,
+     * src/cone.rs:1-31, src/calc.rs:2357-2435) whose reach covers the
+     * tile's rays; its depth is a safe start for the tile's rays, and a
+     * cone that passes the end of the view makes the tile background
+     * (0: off) */
+    int32_t mlp_cone;
+    int32_t reserved_o[2];
 } mb3d_hip_options;
 
 /* ---- painting: the G-buffer lit and coloured (src/lighting.rs; the
@@ -460,9 +471,10 @@ int32_t mb3d_hip_plan_launch(mb3d_hip_plan *plan, int32_t repeats);
 void mb3d_hip_plan_destroy(mb3d_hip_plan *plan);
 
 /* The autotuner: sweeps the tile shape (8x8 .. 32x8), the persistent grid
- * (waves per CU) and the iteration-cap split (it_split 0, 128, 256) on graphs
- * of the batch and returns the fastest options in `best` (its other fields
- * are kept) and their kernel time. */
+ * (waves per CU), the iteration-cap split (it_split 0, 128, 256) and the
+ * refill threshold (refill_min 8, 16) on graphs of the batch and returns
+ * the fastest options in `best` (its other fields are kept) and their
+ * kernel time. */
 int32_t mb3d_hip_autotune(const mb3d_hip_params *frames, int32_t n, const mb3d_hip_out *out,
                           mb3d_hip_options *best, double *best_ms);
 

@@ -64,6 +64,16 @@ struct DevP {
     // first pixel in the batch's output arrays (0 for a single picture)
     int64_t q_begin, pix_base;
     int it_split;  // march steps after which the FP32 pass hands a ray to the f64 finish (0: none)
+    // the frame's entries in the ray tables (RayTab): `width` columns, then
+    // `rows` rows
+    int rt_base;
+    // log2 of the tile width and of the tile's item count (tile sides are
+    // powers of two)
+    int tw_shift, tn_shift;
+    // idle lanes a wave of the FP32 march gathers before it refills them
+    // (1: every turn that has an idle lane)
+    int refill_min;
+    int pad_core;
 
     // ---- the general kind (K_GEN) only ----
     double opt2[MB3D_HIP_MAX_SLOTS][3];
@@ -1050,6 +1060,39 @@ __device__ __forceinline__ void pixel_ray(const DevP &P, int x, int y, double c[
     w2 *= nrm;
     for (int k = 0; k < 3; k++) v[k] = w0 * P.vg[0][k] + w1 * P.vg[1][k] + w2 * P.vg[2][k];
     const double fx = x, fy = y;
+    for (int k = 0; k < 3; k++) c[k] = P.ystart[k] + P.vg[0][k] * fx + P.vg[1][k] * fy;
+}
+
+// The ray tables of a frame: sincos of calc_vgrads_fov's two angles, which
+// depend on the column (cafx) and on the row (cafy) alone.  Entry rt_base +
+// x holds (sin, cos) of column x's angle, entry rt_base + width + yr those
+// of row row0 + yr.  Filled by k_ray_tab with the same sincos calls as
+// pixel_ray, so that pixel_ray_tab's rays are pixel_ray's bit for bit.
+__device__ __forceinline__ void ray_tab_entry(const DevP &P, int i, double2 &out) {
+    double a;
+    if (i < P.width) {
+        a = (double)((P.fovx_off - (float)(i + 1)) * P.fovx_mul);
+    } else {
+        const int y = P.row0 + (i - P.width);
+        a = ((double)y / (double)P.height - 0.5) * P.fov_y;
+    }
+    sincos(a, &out.x, &out.y);
+}
+
+// pixel_ray with the frame's ray tables (`tab`): the view vector from the
+// column's and the row's sincos, normalised and rotated as in pixel_ray
+__device__ __forceinline__ void pixel_ray_tab(const DevP &P, const double2 *__restrict__ tab, int x, int yr, double c[3],
+                                              double v[3]) {
+    const double2 col = tab[P.rt_base + x];
+    const double2 row = tab[P.rt_base + P.width + yr];
+    const double sx = row.x, cx = row.y, sy = col.x, cy = col.y;
+    double w0 = -sy, w1 = sx, w2 = cx * cy;
+    const double nrm = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+    w0 *= nrm;
+    w1 *= nrm;
+    w2 *= nrm;
+    for (int k = 0; k < 3; k++) v[k] = w0 * P.vg[0][k] + w1 * P.vg[1][k] + w2 * P.vg[2][k];
+    const double fx = x, fy = P.row0 + yr;
     for (int k = 0; k < 3; k++) c[k] = P.ystart[k] + P.vg[0][k] * fx + P.vg[1][k] * fy;
 }
 

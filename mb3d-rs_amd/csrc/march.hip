@@ -176,6 +176,20 @@ bool needs_f64(const mb3d_hip_params &p) {
     return numeric || step_precision < 64.0;
 }
 
+// the default of mb3d_hip_options::refill_min: a wave refills once a
+// quarter of its lanes are idle (the 1080p bulb's 48-frame plan: 2733
+// Mpix/s against 2372 when every turn with an idle lane refills)
+constexpr int kRefillMin = 16;
+
+// tile sides: powers of two from 1 to 64, at most 4096 items a tile
+bool tile_ok(const mb3d_hip_options &o) {
+    auto pow2 = [](int v) { return v == 0 || (v > 0 && v <= 64 && (v & (v - 1)) == 0); };
+    return pow2(o.tile_w) && pow2(o.tile_h) && o.tile_w * o.tile_h <= 4096;
+}
+
+// entries of a frame's ray tables (k_ray_tab): its columns, then its rows
+long long ray_tab_entries(const DevP &d) { return (long long)d.width + d.rows; }
+
 void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, int rows, DevP &d) {
     std::memset(&d, 0, sizeof d);
     d.width = p.width;
@@ -293,6 +307,10 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
     d.it_split = o.it_split > 0 ? o.it_split : 0;
     d.tile_w = o.tile_w > 0 ? o.tile_w : 8;
     d.tile_h = o.tile_h > 0 ? o.tile_h : 8;
+    d.tw_shift = __builtin_ctz((unsigned)d.tile_w);
+    d.tn_shift = d.tw_shift + __builtin_ctz((unsigned)d.tile_h);
+    d.refill_min = o.refill_min > 0 ? std::min(o.refill_min, 64) : kRefillMin;
+    d.rt_base = 0;
     d.tiles_x = (p.width + d.tile_w - 1) / d.tile_w;
     const long long tiles_y = (rows + d.tile_h - 1) / d.tile_h;
     d.npix = (long long)p.width * rows;
@@ -373,10 +391,16 @@ int dem_of(const mb3d_hip_params &p) {
 }
 
 template <bool BATCH>
-void launch_propose(hipStream_t s, const DevP &d, const DevP *tab, int n, const MlpDev &M, float2 *prop) {
+void launch_propose(hipStream_t s, const DevP &d, const DevP *tab, int n, const MlpDev &M, const double2 *rtab,
+                    float2 *prop) {
     const long long groups = (d.nslots_q + 63) / 64;
     const int grid = (int)std::max<long long>(1, std::min<long long>((groups + 3) / 4, (long long)g_ctx.cus * 16));
-    hipLaunchKernelGGL((k_mlp_propose<BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, M, prop);
+    hipLaunchKernelGGL((k_mlp_propose<BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, M, rtab, prop);
+}
+
+// the ray tables of one frame (tab NULL: `d`) or of a batch's n frames
+void launch_ray_tab(hipStream_t s, const DevP &d, const DevP *tab, int n, double2 *rtab) {
+    hipLaunchKernelGGL(k_ray_tab, dim3(8, tab ? n : 1), dim3(256), 0, s, tab, d, rtab);
 }
 
 template <int KIND, int DEM, bool BATCH = false>
@@ -595,8 +619,7 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     mb3d_hip_options o;
     std::memset(&o, 0, sizeof o);
     if (opt_in) o = *opt_in;
-    if (o.tile_w < 0 || o.tile_h < 0 || o.tile_w * o.tile_h > 4096)
-        return set_error(MB3D_HIP_ERR_INVALID, "bad tile shape");
+    if (!tile_ok(o)) return set_error(MB3D_HIP_ERR_INVALID, "bad tile shape (sides are powers of two up to 64)");
     if (cancel && cancel(user)) return set_error(MB3D_HIP_CANCELLED, "cancelled");
     // (the K_GEN formulas have no FP32 iterate: FP64 whatever was asked)
     const bool f64 = o.precision == MB3D_HIP_PREC_F64 || (o.precision == MB3D_HIP_PREC_AUTO && needs_f64(p)) ||
@@ -620,7 +643,9 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     const int wpc = o.waves_per_cu > 0 ? o.waves_per_cu : 16;
     const size_t cap = list_capacity((long long)npix, c.cus, wpc);
     const size_t base_need = (cap * (sizeof(Cand) + sizeof(SurfRec) + sizeof(int32_t)) + 511) & ~(size_t)255;
-    const size_t work_need = base_need + (M.enabled ? learn_bufs(nullptr, d.nslots_q, nullptr) : 0);
+    const size_t learn_need = M.enabled ? learn_bufs(nullptr, d.nslots_q, nullptr) : 0;
+    const size_t rtab_bytes = ((size_t)ray_tab_entries(d) * sizeof(double2) + 255) & ~(size_t)255;
+    const size_t work_need = base_need + learn_need + rtab_bytes;
     if (int e = ensure(&c.d_work, &c.work_bytes, work_need)) return e;
     Bufs b;
     std::memset(&b, 0, sizeof b);
@@ -649,6 +674,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     b.hits = (int32_t *)(b.surf + cap);
     b.ctr = c.d_ctr;
     if (M.enabled) learn_bufs((char *)c.d_work + base_need, d.nslots_q, &b);
+    double2 *rtab = (double2 *)((char *)c.d_work + base_need + learn_need);
+    b.rtab = rtab;
 
     HIP_TRY(hipMemsetAsync(c.d_ctr, 0, C_WORDS * sizeof(unsigned long long), c.stream));
     if (post_only && !dev_out)
@@ -665,7 +692,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
                 launch_normals<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, false);
             });
         } else {
-            if (M.enabled) launch_propose<false>(stream, d, nullptr, 1, M, b.prop);
+            launch_ray_tab(stream, d, nullptr, 1, rtab);
+            if (M.enabled) launch_propose<false>(stream, d, nullptr, 1, M, rtab, b.prop);
             dispatch(kind, dem, [&](auto K, auto D) {
                 launch_march<decltype(K)::value, decltype(D)::value>(grid_march, stream, d, b, M);
             });
@@ -803,7 +831,8 @@ void plan_free(Plan *pl) {
 // not time)
 int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, pl.d_ctr, (int)C_WORDS);
-    if (pl.M.enabled) launch_propose<true>(stream, pl.P0, pl.d_tab, pl.n, pl.M, pl.b.prop);
+    launch_ray_tab(stream, pl.P0, pl.d_tab, pl.n, const_cast<double2 *>(pl.b.rtab));
+    if (pl.M.enabled) launch_propose<true>(stream, pl.P0, pl.d_tab, pl.n, pl.M, pl.b.rtab, pl.b.prop);
     const DevP *tab = pl.d_tab;
     const int n = pl.n;
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
@@ -829,8 +858,7 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     mb3d_hip_options o;
     std::memset(&o, 0, sizeof o);
     if (opt_in) o = *opt_in;
-    if (o.tile_w < 0 || o.tile_h < 0 || o.tile_w * o.tile_h > 4096)
-        return set_error(MB3D_HIP_ERR_INVALID, "bad tile shape");
+    if (!tile_ok(o)) return set_error(MB3D_HIP_ERR_INVALID, "bad tile shape (sides are powers of two up to 64)");
     const int kind = kind_of(frames[0]), dem = dem_of(frames[0]);
     for (int f = 0; f < n; f++) {
         const std::string why = unsupported_reason(&frames[f]);
@@ -847,14 +875,16 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     for (int f = 0; f < n; f++) pl->n32 = pl->n32 && fp32_stencil_ok(frames[f]);
     pl->dem = dem;
     pl->tab.resize(n);
-    long long q = 0, pix = 0;
+    long long q = 0, pix = 0, rt = 0;
     for (int f = 0; f < n; f++) {
         DevP &d = pl->tab[f];
         fill_devp(frames[f], o, 0, frames[f].height, d);
         d.q_begin = q;
         d.pix_base = pix;
+        d.rt_base = (int)rt;
         q += d.nslots_q;
         pix += d.npix;
+        rt += ray_tab_entries(d);
     }
     if (pix >= (1ll << 31)) {
         plan_free(pl);
@@ -875,7 +905,8 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
         return e;
     };
     if (hipMalloc((void **)&pl->d_tab, sizeof(DevP) * n) != hipSuccess ||
-        hipMalloc(&pl->d_work, base_need + (pl->M.enabled ? learn_bufs(nullptr, q, nullptr) : 0)) != hipSuccess ||
+        hipMalloc(&pl->d_work, base_need + (pl->M.enabled ? learn_bufs(nullptr, q, nullptr) : 0) +
+                                   (size_t)rt * sizeof(double2)) != hipSuccess ||
         hipMalloc((void **)&pl->d_ctr, C_WORDS * sizeof(unsigned long long)) != hipSuccess)
         return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipMalloc failed for a batch"));
     if (hipMemcpy(pl->d_tab, pl->tab.data(), sizeof(DevP) * n, hipMemcpyHostToDevice) != hipSuccess)
@@ -893,6 +924,7 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     pl->b.surf = (SurfRec *)((char *)pl->d_work + cap * sizeof(Cand));
     pl->b.hits = (int32_t *)(pl->b.surf + cap);
     if (pl->M.enabled) learn_bufs((char *)pl->d_work + base_need, q, &pl->b);
+    pl->b.rtab = (const double2 *)((char *)pl->d_work + base_need + (pl->M.enabled ? learn_bufs(nullptr, q, nullptr) : 0));
     pl->b.ctr = pl->d_ctr;
     if (graph) {
         if (hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -954,14 +986,15 @@ int plan_run(Plan &pl, int repeats) {
     return MB3D_HIP_OK;
 }
 
-// The project's autotuner: every tile shape, persistent grid size and
-// iteration-cap split of the sweep on a graph of the batch, the fastest kept
-// (min of 3 replays each).
+// The project's autotuner: every tile shape, persistent grid size,
+// iteration-cap split and refill threshold of the sweep on a graph of the
+// batch, the fastest kept (min of 3 replays each).
 int autotune_locked(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, mb3d_hip_options *best,
                     double *best_ms) {
     static const int tiles[][2] = {{8, 8}, {16, 4}, {16, 8}, {32, 4}, {32, 8}};
     static const int wpcs[] = {12, 16, 20, 24};
     static const int splits[] = {0, 128, 256};
+    static const int refills[] = {8, 16};
     mb3d_hip_options o;
     std::memset(&o, 0, sizeof o);
     if (best) o = *best;
@@ -969,12 +1002,14 @@ int autotune_locked(const mb3d_hip_params *frames, int n, const mb3d_hip_out *ou
     mb3d_hip_options win = o;
     for (const auto &t : tiles) {
         for (int w : wpcs) {
-            for (int sp : splits) {
+            for (int sp : splits)
+            for (int rf : refills) {
                 mb3d_hip_options c = o;
                 c.tile_w = t[0];
                 c.tile_h = t[1];
                 c.waves_per_cu = w;
                 c.it_split = sp;
+                c.refill_min = rf;
                 Plan *pl = nullptr;
                 if (int e = plan_build(frames, n, out, &c, true, &pl)) return e;
                 double m = 1e300;

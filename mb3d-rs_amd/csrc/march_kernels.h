@@ -85,7 +85,22 @@ struct Bufs {
     // the warm start: the network's proposal per work item of the queue
     // (hit distance / 4, hit logit)
     float2 *prop;
+    // the frames' ray tables (k_ray_tab), read by the FP32 march and the
+    // warm start's proposals
+    const double2 *rtab;
 };
+
+// Fills the ray tables of the frames `tab[0..n)` (one frame: `tab` = &P):
+// block row blockIdx.y takes frame blockIdx.y's width + rows entries.
+static __global__ void k_ray_tab(const DevP *__restrict__ tab, DevP one, double2 *rtab) {
+    const DevP &P = tab ? tab[blockIdx.y] : one;
+    const int n = P.width + P.rows;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double2 e;
+        ray_tab_entry(P, i, e);
+        rtab[P.rt_base + i] = e;
+    }
+}
 
 // DE AO ray set (build_rays, src/deao.rs:75-105), computed on the host
 struct AoRays {
@@ -291,19 +306,52 @@ __device__ __forceinline__ int find_frame(const TAB &tab, int n, long long q) {
     return lo;
 }
 
-// the pixel of work item q of the tile queue (false: the item is in a
-// ragged edge tile, outside the picture)
-template <bool BATCH, class TAB>
-__device__ __forceinline__ bool item_pixel(const DevP &P0, const TAB &tab, int nframes, long long q, int &fr, int &x,
-                                           int &yr) {
-    fr = BATCH ? find_frame(tab, nframes, q) : 0;
-    const DevP &F = BATCH ? tab[fr] : P0;
-    const int tile_n = P0.tile_w * P0.tile_h;
-    const long long ql = q - F.q_begin;
-    const long long tile = ql / tile_n;
-    const int k = (int)(ql - tile * tile_n);
-    x = (int)(tile % F.tiles_x) * F.tile_w + k % F.tile_w;
-    yr = (int)(tile / F.tiles_x) * F.tile_h + k / F.tile_w;
+// The frame of a batch that a run of work items starts in, and the item
+// range of that frame: a wave looks it up once per chunk it takes (a chunk
+// of 256 items spans at most two frames of the usual sizes), and an item
+// outside the range falls back to the search.
+struct FrameHint {
+    int fr;
+    long long lo, hi;  // [lo, hi): the items of frame fr
+};
+
+template <class TAB>
+__device__ __forceinline__ FrameHint frame_hint(const TAB &tab, int n, long long q) {
+    const int f = find_frame(tab, n, q);
+    return {f, tab[f].q_begin, f + 1 < n ? tab[f + 1].q_begin : (long long)1 << 62};
+}
+
+template <class TAB>
+__device__ __forceinline__ int frame_of(const TAB &tab, int n, const FrameHint &h, long long q) {
+    return q >= h.lo && q < h.hi ? h.fr : find_frame(tab, n, q);
+}
+
+// n / d and its remainder for 0 <= n < 2^31, 0 < d, without an integer
+// divide: the float quotient, corrected to the exact one
+__device__ __forceinline__ int div_rem(int n, int d, int &r) {
+    int k = (int)((float)n * __builtin_amdgcn_rcpf((float)d));
+    r = n - k * d;
+    while (r < 0) {
+        k--;
+        r += d;
+    }
+    while (r >= d) {
+        k++;
+        r -= d;
+    }
+    return k;
+}
+
+// the pixel of item ql (counted from its frame's first item) of frame F's
+// tile queue (false: the item is in a ragged edge tile, outside the
+// picture); the tile sides are powers of two
+__device__ __forceinline__ bool item_pixel(const DevP &F, int ql, int &x, int &yr) {
+    const int tile = ql >> F.tn_shift;
+    const int k = ql & ((1 << F.tn_shift) - 1);
+    int tx;
+    const int ty = div_rem(tile, F.tiles_x, tx);
+    x = (tx << F.tw_shift) + (k & (F.tile_w - 1));
+    yr = ty * F.tile_h + (k >> F.tw_shift);
     return x < F.width && yr < F.rows;
 }
 
@@ -316,10 +364,10 @@ __device__ __forceinline__ bool item_pixel(const DevP &P0, const TAB &tab, int n
 // +0.75 ms a 1080p frame at a fence pair per chunk of 256 rays).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__restrict__ gtab, int nframes, MlpDev M,
-                                                     float2 *prop) {
+                                                     const double2 *__restrict__ rtab, float2 *prop) {
     const GlobalFrames tab{reinterpret_cast<const char *>(gtab)};
     __shared__ __attribute__((aligned(16))) char lds[4][MLP_LDS_BYTES];
-    __shared__ mlp_s4 wl[MLP_TILES * 64];
+    __shared__ __attribute__((aligned(16))) mlp_s4 wl[MLP_TILES * 64];
     mlp_stage_weights(M, wl);
     __syncthreads();
     const int lane = lane_id();
@@ -331,10 +379,14 @@ __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__rest
         float o[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 1.0f};
         int f = 0, x = 0, yr = 0;
         const bool in_q = q < P0.nslots_q;
-        if (in_q && item_pixel<BATCH>(P0, tab, nframes, q, f, x, yr)) {
-            const DevP &F = BATCH ? tab[f] : P0;
+        if (BATCH) {
+            const FrameHint h = frame_hint(tab, nframes, g * 64);  // wave-uniform
+            f = in_q ? frame_of(tab, nframes, h, q) : 0;
+        }
+        const DevP &F = BATCH ? tab[f] : P0;
+        if (in_q && item_pixel(F, (int)(q - F.q_begin), x, yr)) {
             double cc[3], vv[3];
-            pixel_ray(F, x, F.row0 + yr, cc, vv);
+            pixel_ray_tab(F, rtab, x, yr, cc, vv);
             const double il = 1.0 / sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
             for (int j = 0; j < 3; j++) {
                 o[j] = (float)cc[j];
@@ -377,24 +429,35 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
     WaveChunk wq, cq;
     chunk_init(wq);
     chunk_init(cq);
+    FrameHint hint = {0, 0, 0};
+    auto on_chunk = [&](long long base, int) {
+        if (BATCH) hint = frame_hint(tab, nframes, base);
+    };
     for (;;) {
-        // refill: every idle lane takes the next work item from the wave's chunk
+        // refill: the idle lanes take the next work items from the wave's
+        // chunk, once at least refill_min of them are idle (or every lane
+        // that still has work to get is)
         bool need = !have && !exhausted;
-        for (int tries = 0; tries < 4 && __any(need); tries++) {
-            const long long q = wave_take(need, &B.ctr[C_QUEUE], wq, 256, P0.nslots_q);
+        const int idle = __popcll(__ballot(need));
+        const bool refill = idle > 0 && (idle >= P0.refill_min || idle == __popcll(__ballot(!exhausted)));
+        for (int tries = 0; refill && tries < 4 && __any(need); tries++) {
+            const long long q = wave_take(need, &B.ctr[C_QUEUE], wq, 256, P0.nslots_q, on_chunk);
             if (need) {
                 int x = 0, yr = 0;
                 if (q < 0) {
                     exhausted = true;
                     need = false;
-                } else if (item_pixel<BATCH>(P0, tab, nframes, q, fr, x, yr)) {
-                    const DevP &F = BATCH ? tab[fr] : P0;
+                    continue;
+                }
+                fr = BATCH ? frame_of(tab, nframes, hint, q) : 0;
+                const DevP &F = BATCH ? tab[fr] : P0;
+                if (item_pixel(F, (int)(q - F.q_begin), x, yr)) {
                     have = true;
                     need = false;
                     px = x;
-                    py = F.row0 + yr;
+                    py = yr;  // the row within the frame's rectangle
                     pix = (int)(F.pix_base + (long long)yr * F.width + x);
-                    pixel_ray(F, px, py, c, v);
+                    pixel_ray_tab(F, B.rtab, px, py, c, v);
                     mzz = 0.0;
                     ms = F.de_stop;
                     rsf = 1.0f;
@@ -403,7 +466,7 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
                     start = true;
                     jumped = false;
                     first = F.first_step_random != 0;
-                    seed = first ? dev_pixel_seed(py, px) : 0;
+                    seed = first ? dev_pixel_seed(F.row0 + py, px) : 0;
                     if constexpr (LEARN) {
                         const float2 pr = B.prop[q];
                         const float ph = 1.0f / (1.0f + __expf(-pr.y));
@@ -457,7 +520,7 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
                 if (!(d32 < __builtin_inff()) || it >= P.max_it || !(d32 >= ms * fguard) ||
                     !(d32 >= 0.1f * jrem)) {
                     cnt.rollbacks++;
-                    pixel_ray(P, px, py, c, v);
+                    pixel_ray_tab(P, B.rtab, px, py, c, v);
                     mzz = 0.0;
                     ms = P.de_stop;
                     top = false;

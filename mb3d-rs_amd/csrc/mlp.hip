@@ -176,7 +176,7 @@ __global__ void k_mlp_clear_pad(float *p) {
 // the MFMA path of the march, on its own: one wave per 64 rays
 __global__ __launch_bounds__(256) void k_mlp_infer_mfma(MlpDev M, const float *rays, long long n, float *out) {
     __shared__ __attribute__((aligned(16))) char lds[4 * MLP_LDS_BYTES];
-    __shared__ mlp_s4 wl[MLP_TILES * 64];
+    __shared__ __attribute__((aligned(16))) mlp_s4 wl[MLP_TILES * 64];
     mlp_stage_weights(M, wl);
     __syncthreads();
     const int wave = threadIdx.x >> 6;
@@ -200,17 +200,29 @@ uint16_t host_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 
-// tile (kt, nt) of W (K x N, row-major), lane l, element i = W[16 kt + 4 (l / 16) + i][16 nt + l % 16]
-void pack_layer(const float *W, int K, int N, int KT, int NT, uint16_t *dst) {
+// The MFMA packing of W (K x N, row-major) as the A operand of
+// out^T = W^T in^T (mlp_device.h): per tile (kt, nt), lane l = 16 q + m,
+// `per` values of W[k][16 nt + m], the inputs k of the lane's K slots
+// (`slot(kt, q, i)`), zero beyond K or N.
+template <class Slot>
+void pack_tiles(const float *W, int K, int N, int KT, int NT, int per, Slot slot, uint16_t *dst) {
     for (int kt = 0; kt < KT; kt++)
         for (int nt = 0; nt < NT; nt++)
             for (int l = 0; l < 64; l++)
-                for (int i = 0; i < 4; i++) {
-                    const int k = 16 * kt + 4 * (l / 16) + i, c = 16 * nt + l % 16;
+                for (int i = 0; i < per; i++) {
+                    const int k = slot(kt, l / 16, i), c = 16 * nt + l % 16;
                     const float w = (k < K && c < N) ? W[k * N + c] : 0.0f;
-                    dst[((kt * NT + nt) * 64 + l) * 4 + i] = host_bf16(w);
+                    dst[((kt * NT + nt) * 64 + l) * per + i] = host_bf16(w);
                 }
 }
+
+// the first layer's K = 32 slots: features 32 kt + 8 q + i (read from LDS
+// in that order); its K = 16 tile: features 64 + 4 q + i
+int slot_l1a(int kt, int q, int i) { return 32 * kt + 8 * q + i; }
+int slot_l1b(int, int q, int i) { return 64 + 4 * q + i; }
+// the K = 32 slots of a layer fed by the previous layer's D tiles 2 kt
+// (elements 0..3) and 2 kt + 1 (elements 4..7)
+int slot_chain(int kt, int q, int i) { return 32 * kt + (i < 4 ? 4 * q + i : 16 + 4 * q + (i - 4)); }
 
 int repack(mb3d_hip_mlp *net, hipStream_t s) {
     std::vector<float> p(MLP_PARAMS);
@@ -219,13 +231,11 @@ int repack(mb3d_hip_mlp *net, hipStream_t s) {
         return set_error(MB3D_HIP_ERR_DEVICE, "copying the network's weights failed");
     std::vector<uint16_t> w(MLP_PACKED_U16);
     uint16_t *q = w.data();
-    pack_layer(p.data() + OFF_W1, MLP_K1, MLP_H, 5, 4, q);
-    q += 5 * 4 * 256;
-    pack_layer(p.data() + OFF_W2, MLP_H, MLP_H, 4, 4, q);
-    q += 4 * 4 * 256;
-    pack_layer(p.data() + OFF_W3, MLP_H, MLP_H, 4, 4, q);
-    q += 4 * 4 * 256;
-    pack_layer(p.data() + OFF_W4, MLP_H, MLP_OUT, 4, 1, q);
+    pack_tiles(p.data() + OFF_W1, MLP_K1, MLP_H, 2, 4, 8, slot_l1a, q + MLP_OFF_L1A);
+    pack_tiles(p.data() + OFF_W1, MLP_K1, MLP_H, 1, 4, 4, slot_l1b, q + MLP_OFF_L1B);
+    pack_tiles(p.data() + OFF_W2, MLP_H, MLP_H, 2, 4, 8, slot_chain, q + MLP_OFF_L2);
+    pack_tiles(p.data() + OFF_W3, MLP_H, MLP_H, 2, 4, 8, slot_chain, q + MLP_OFF_L3);
+    pack_tiles(p.data() + OFF_W4, MLP_H, MLP_OUT, 2, 1, 8, slot_chain, q + MLP_OFF_L4);
     std::vector<float> b(MLP_BIAS, 0.0f);
     std::memcpy(b.data(), p.data() + OFF_B1, MLP_H * 4);
     std::memcpy(b.data() + MLP_H, p.data() + OFF_B2, MLP_H * 4);

@@ -20,7 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmb3d_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_SLOTS = 6
 MAX_LIGHTS = 6
 
@@ -228,6 +228,9 @@ class Options(C.Structure):
         ("mlp_conf_miss", C.c_float),
         ("mlp_margin_rel", C.c_float),
         ("mlp_margin_abs", C.c_float),
+        ("refill_min", C.c_int32),
+        ("mlp_cone", C.c_int32),
+        ("reserved_o", C.c_int32 * 2),
     ]
 
 
@@ -846,16 +849,21 @@ def _p(a):
 
 
 def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=16, guard_band=0.25, it_split=0, mlp=None,
-            conf_hit=0.0, conf_miss=0.0, margin_rel=0.0, margin_abs=0.0) -> Options:
+            conf_hit=0.0, conf_miss=0.0, margin_rel=0.0, margin_abs=0.0, refill_min=0, cone=False) -> Options:
     """Options; `mlp` (an Mlp) turns the warm start on: hit jumps with the
     library's defaults (conf_hit 0: 0.5, above 1: off; margin_rel 0: 0.08),
-    miss jumps (unverified) only with conf_miss > 0."""
+    miss jumps (unverified) only with conf_miss > 0; `cone` adds the cone
+    ray per tile (a verified safe start, and background for tiles it finds
+    empty).  `refill_min`: idle lanes a march wave gathers before refilling
+    (0: the library's default)."""
     o = Options()
     o.precision = precision
     o.tile_w, o.tile_h = tile
     o.waves_per_cu = waves_per_cu
     o.guard_band = guard_band
     o.it_split = it_split
+    o.refill_min = refill_min
+    o.mlp_cone = 1 if cone else 0
     if mlp is not None:
         o.mlp = mlp.h
         o.mlp_conf_hit, o.mlp_conf_miss = conf_hit, conf_miss
