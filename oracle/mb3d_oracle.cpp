@@ -19,11 +19,15 @@
 //   Monte Carlo        McMarch::run (ray, shadow, through)      src/mc.rs:488-977
 // Built with -ffp-contract=off: no fused multiply-adds, like the reference.
 //
-// Parity unpinned: the DE AO restatements (deao_row and the hit pass's
-// deao_1pos).  Only the reference's sanity test (some pixels occluded) and
-// their agreement with each other check them; no closed form exists with
-// the formulas this path carries.  Every other function here is checked
-// against known answers, closed forms or brute force in tests/.
+// Every function here is checked against the reference's known answers,
+// closed forms or brute force in tests/.  The DE AO (deao_row) per pixel:
+// its surface point against the march's depth, its ray directions against
+// a Rodrigues rotation of the ray set onto the normal, its openness against
+// a fine DE walk along each ray (test_deao_against_brute_force); the hit
+// pass's deao_1pos against deao_row on view rays (test_oracle_rays.py).  The
+// DE's size against the brute-force distance to the set, analytic and
+// numeric (test_oracle.py).  The AO walk's step rule (dt += max(DE, dt dsm))
+// is pinned only through the openness it reaches: a doubled step passes.
 #include "mb3d_oracle.h"
 
 #include <algorithm>
@@ -1265,9 +1269,20 @@ bool surface_point(Marcher &m, int x, int y, const Px &si, int steps, double thr
     return true;
 }
 
+// What deao_pixel computed for one pixel, for the tests that check it
+// against brute force (orc_deao_probe).
+struct DeaoProbe {
+    double ic[3];        // the rebuilt surface point
+    double dirs[33][3];  // each ray's step in world units (step_ao-scaled directions)
+    double min_ra[33];   // each ray's openness (times de_mul)
+    double step_ao, max_dist, ms_ao, md_d10, de_mul, dsm;
+    int count;
+};
+
 // deao_row for one pixel, no dither, no random first step
 // (src/deao.rs:268-418); the marcher's parameters have de_add_steps = 5
-void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_hip_post_job &job) {
+void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_hip_post_job &job,
+                DeaoProbe *probe = nullptr) {
     const mb3d_hip_params &p = m.p;
     double sw = p.step_width;
     double rot_v[3][3];
@@ -1320,6 +1335,20 @@ void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_h
             if (b_end) break;
         }
         min_ra[k] = (float)(std::fmax(s_tmp, 0.0) * de_mul);
+        if (probe) {
+            for (int i = 0; i < 3; i++) probe->dirs[k][i] = sv[i];
+            probe->min_ra[k] = min_ra[k];
+        }
+    }
+    if (probe) {
+        for (int i = 0; i < 3; i++) probe->ic[i] = ic[i];
+        probe->step_ao = step_ao;
+        probe->max_dist = max_dist;
+        probe->ms_ao = ms_de_stop;
+        probe->md_d10 = md_d10;
+        probe->de_mul = de_mul;
+        probe->dsm = dsm;
+        probe->count = ray_count;
     }
     double amount = 0.0;
     for (int k = 0; k < ray_count; k++) {
@@ -1953,6 +1982,33 @@ extern "C" void orc_pixel_ray(const mb3d_hip_params *p, int32_t x, int32_t y, do
         c[k] = m.it.c[k];
         v[k] = m.vfov[k];
     }
+}
+
+// One pixel's DE AO (deao_pixel) on its card record `rec` (5 words), with
+// what it computed: out = the surface point (3), the ray steps in world
+// units (33 x 3), the openness per ray (33), then step_ao, max_dist, the
+// DE stop of the rays, md_d10, de_mul and d_step_mul; returns the ray
+// count (0: no surface), the amb word in *amb.
+extern "C" int32_t orc_deao_probe(const mb3d_hip_params *pp, const mb3d_hip_post_job *job, int32_t x, int32_t y,
+                                  const uint32_t rec[5], double *out, int32_t *amb) {
+    if (check_params(pp) || !job || !job->deao) return -1;
+    mb3d_hip_params pc = *pp;
+    pc.de_add_steps = 5;  // DeaoRows::new (src/deao.rs:254-260)
+    Rays base = build_rays(job->deao_quality);
+    Marcher m(pc);
+    Px si;
+    from_words(rec, si);
+    DeaoProbe pr;
+    std::memset(&pr, 0, sizeof pr);
+    deao_pixel(m, x, y, si, base, *job, &pr);
+    *amb = si.amb_shadow;
+    for (int i = 0; i < 3; i++) out[i] = pr.ic[i];
+    for (int k = 0; k < 33; k++)
+        for (int i = 0; i < 3; i++) out[3 + 3 * k + i] = pr.dirs[k][i];
+    for (int k = 0; k < 33; k++) out[102 + k] = pr.min_ra[k];
+    const double info[6] = {pr.step_ao, pr.max_dist, pr.ms_ao, pr.md_d10, pr.de_mul, pr.dsm};
+    for (int i = 0; i < 6; i++) out[135 + i] = info[i];
+    return pr.count;
 }
 
 extern "C" double orc_march_ray(const mb3d_hip_params *p, const double origin[3], const double dir[3], double zz,

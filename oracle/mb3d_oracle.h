@@ -67,6 +67,15 @@ int32_t orc_render(const mb3d_hip_params *p, const mb3d_hip_post_job *post, int3
 int32_t orc_post(const mb3d_hip_params *p, const mb3d_hip_post_job *post, int32_t row0,
                  int32_t rows, uint32_t *gbuf, int32_t threads);
 
+/* One pixel's DE AO (deao_row, src/deao.rs:268-418) on its card record,
+ * with what it computed, for the tests: out (141 doubles) = the rebuilt
+ * surface point, each ray's world step (33 x 3), each ray's openness times
+ * de_mul (33), then step_ao, max_dist, the rays' DE stop, md_d10, de_mul,
+ * d_step_mul.  Returns the ray count (0: not a surface pixel); *amb the
+ * amb_shadow word. */
+int32_t orc_deao_probe(const mb3d_hip_params *p, const mb3d_hip_post_job *job, int32_t x, int32_t y,
+                       const uint32_t rec[5], double *out, int32_t *amb);
+
 /* The march of one ray from `origin` with direction `dir` (one step long)
  * at depth `zz`, as Marcher::march_ray_from (src/calc.rs:2519-2531): the
  * depth reached (+inf: background), its iteration count and steps. */

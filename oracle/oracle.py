@@ -155,6 +155,22 @@ def de_many(p, pos, threads=0):
     return de, its
 
 
+def deao_probe(p, job, x, y, rec):
+    """One pixel's DE AO on its card record (5 words) with what it computed:
+    (ray count, amb word, dict of the surface point, each ray's world step,
+    each ray's openness times de_mul, and step_ao, max_dist, ms_ao, md_d10,
+    de_mul, dsm)."""
+    out = np.zeros(141)
+    amb = C.c_int32()
+    r = np.ascontiguousarray(rec, np.uint32)
+    n = load().orc_deao_probe(C.byref(p), C.byref(job), x, y, _ptr(r), _ptr(out), C.byref(amb))
+    if n < 0:
+        raise RuntimeError("orc_deao_probe failed")
+    info = dict(zip(("step_ao", "max_dist", "ms_ao", "md_d10", "de_mul", "dsm"), out[135:141]))
+    info.update(point=out[:3], dirs=out[3:102].reshape(33, 3)[:n], min_ra=out[102:135][:n])
+    return n, amb.value, info
+
+
 def pixel_ray(p, x, y):
     c = np.zeros(3)
     v = np.zeros(3)

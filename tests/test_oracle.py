@@ -278,3 +278,178 @@ def test_deao_occludes():
         amb = r["gbuf"][:, 4] & 0xFFFF
         assert (amb[surf] > 2000).any()
         assert (amb[surf] < 16383).all()
+
+
+def _sphere_dirs(n):
+    """n directions spread evenly over the sphere (a Fibonacci lattice)."""
+    i = np.arange(n) + 0.5
+    phi = np.arccos(1.0 - 2.0 * i / n)
+    th = np.pi * (1.0 + 5.0 ** 0.5) * i
+    return np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
+
+
+def _inside(p, pts):
+    return O.de_many(p, pts)[1] >= p.max_it
+
+
+def _distance_to_set(p, q, dirs, reach, samples=160, halvings=16):
+    """An upper bound of the distance from q to the set (the points whose
+    orbit does not escape within max_it iterations, decided by the
+    iteration count alone): along each direction the first sample inside,
+    bisected; the nearest of the directions."""
+    ts = np.linspace(0.0, reach, samples + 1)[1:]
+    ins = _inside(p, (q[None, None, :] + dirs[:, None, :] * ts[None, :, None]).reshape(-1, 3)).reshape(len(dirs), -1)
+    hit = ins.any(1)
+    if not hit.any():
+        return np.inf
+    k = np.flatnonzero(hit)
+    j = np.argmax(ins[k], 1)
+    lo = np.where(j > 0, ts[np.maximum(j - 1, 0)], 0.0)
+    hi = ts[j]
+    for _ in range(halvings):
+        m = 0.5 * (lo + hi)
+        inn = _inside(p, q[None, :] + dirs[k] * m[:, None])
+        hi = np.where(inn, m, hi)
+        lo = np.where(inn, lo, m)
+    return float(hi.min())
+
+
+def _de_over_distance(de_mode, n_points=24, seed=3):
+    """DE / brute-force distance at points outside the bulb: each point a
+    random distance (0.02 to 0.25) beyond the set's boundary along a random
+    ray from the origin (bisected on the iteration count)."""
+    p = O.params(H.scene_bulb(96, 72, seed=2, de_mode=de_mode))
+    rng = np.random.default_rng(seed)
+    dirs = _sphere_dirs(300)
+    ratios = []
+    while len(ratios) < n_points:
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        lo, hi = 0.0, 2.0
+        for _ in range(40):
+            m = 0.5 * (lo + hi)
+            if _inside(p, (u * m)[None, :])[0]:
+                lo = m
+            else:
+                hi = m
+        q = u * (hi + rng.uniform(0.02, 0.25))
+        d, its = O.de_at(p, q)
+        dist = _distance_to_set(p, q, dirs, reach=1.2)
+        if its >= p.max_it or not np.isfinite(dist) or dist < 1e-3:
+            continue  # on a filament of the set: no distance to measure
+        ratios.append(d / dist)
+    return np.array(ratios)
+
+
+def test_analytic_p8_de_is_a_distance_bound_of_the_right_size():
+    """The analytic power-8 DE (0.5 r ln r / dr, src/iteration.rs:562-568,
+    in world units) against the distance to the set found by brute force
+    (the first non-escaping point along 300 directions, src/calc.rs:1515-1545
+    marches to where that distance vanishes): a lower bound for nearly all
+    points, and within a factor of four.  Twice the constant puts the upper
+    tail above the distance, half of it the median under a quarter."""
+    r = _de_over_distance(H.DE_ANALYTIC_P8)
+    assert np.percentile(r, 90) <= 1.0, np.sort(r)
+    assert 0.25 <= np.median(r) <= 0.65, np.sort(r)
+
+
+def test_numeric_de_has_the_size_of_the_distance():
+    """The reference's numeric gradient DE (option 0, src/calc.rs:1515-1545:
+    rout ln rout over the finite-difference gradient, twice the analytic
+    bound) against the same brute-force distance: the median near the
+    distance itself.  A factor of two either way leaves the window."""
+    r = _de_over_distance(H.DE_REFERENCE)
+    assert 0.6 <= np.median(r) <= 1.3, np.sort(r)
+    assert np.percentile(r, 90) <= 1.6, np.sort(r)
+
+
+def _rodrigues(a, b):
+    """The rotation taking unit vector a to unit vector b about the axis
+    a x b (the smallest such rotation)."""
+    k = np.cross(a, b)
+    s, c = np.linalg.norm(k), float(np.dot(a, b))
+    if s < 1e-12:
+        return np.eye(3) if c > 0 else np.diag([1.0, -1.0, -1.0])
+    k /= s
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + s * K + (1.0 - c) * K @ K
+
+
+def _ao_set(quality):
+    """The DE AO ray set of a quality in the local frame of a surface whose
+    normal is (0, 0, -1) (the view looks along +z): the straight ray, then
+    rings at polar angles y abr (abr = (pi / 2) / (quality + 0.9)), each
+    with round(sin(y abr) 2 pi / abr) rays evenly spaced in azimuth from the
+    +x side (src/deao.rs:75-105); quality 0: three rays at 30 degrees."""
+    out = []
+    if quality == 0:
+        rings, abr = [(np.pi / 6.0, 3)], None
+    else:
+        out.append((0.0, 0.0))
+        abr = (np.pi / 2.0) / (quality + 0.9)
+        rings = [(y * abr, int(round(np.sin(y * abr) * 2 * np.pi / abr))) for y in range(1, quality + 1)]
+    for polar, n in rings:
+        out += [(polar, 2 * np.pi * x / n) for x in range(n)]
+    return np.array([[np.sin(a) * np.cos(z), -np.sin(a) * np.sin(z), -np.cos(a)] for a, z in out])
+
+
+@pytest.mark.parametrize("quality", [1, 2])
+def test_deao_against_brute_force(quality):
+    """deao_row (src/deao.rs:268-418) per pixel, on a sample of surface
+    pixels, against what it must compute:
+      - the rebuilt surface point (surface_point, src/calc.rs:2157-2192)
+        lies on the pixel's ray at the marched depth;
+      - each ray's world direction is the ray set (rings around the straight
+        ray) turned by the smallest rotation that takes the view axis's
+        reverse to the pixel's normal, then into the world frame by the
+        view's axes (Rodrigues, independent of the quaternion of
+        normal_matrix);
+      - each ray's openness min over its samples of (DE - stop + t md) / t
+        (capped at 1) is never below the same minimum over a fine walk of
+        the DE along the ray (0.05 steps), and close to it for most rays
+        (the pass's samples are sparser: median gap 0.006-0.009 here; a
+        dropped DE stop makes it 0.06);
+      - a pixel with every ray open has amb 0.
+    A swapped axis of the AO frame, a sign of its quaternion or a dropped DE
+    stop fails here; the walk's step rule is pinned only through the
+    openness it reaches (a doubled step passes)."""
+    sc = preset("Integer Power", 64, 48)
+    sc.first_step_random = 0
+    p = O.params(sc)
+    job = H.post_job(deao_quality=quality)
+    r = O.render(p, job, threads=0)
+    vg = np.array([[p.vgrads[i][k] for k in range(3)] for i in range(3)])
+    axes = vg / np.linalg.norm(vg, axis=1, keepdims=True)
+    base = _ao_set(quality)
+    surf = np.flatnonzero(np.isfinite(r["depth"]) & (r["depth"] > 0))
+    rng = np.random.default_rng(quality)
+    gaps = []
+    for i in rng.choice(surf, size=min(30, surf.size), replace=False):
+        x, y = int(i % p.width), int(i // p.width)
+        rec = r["gbuf"][i]
+        n, amb, pr = O.deao_probe(p, job, x, y, rec)
+        assert n == len(base) and amb == (rec[4] & 0xFFFF)
+        c, v = O.pixel_ray(p, x, y)
+        assert np.linalg.norm(pr["point"] - (c + r["depth"][i] * v)) < 0.05 * p.step_width
+        nq = np.array([np.int16(rec[0] & 0xFFFF), np.int16(rec[0] >> 16), np.int16(rec[1] & 0xFFFF)], float)
+        nv = nq / np.linalg.norm(nq)
+        rot = _rodrigues(np.array([0.0, 0.0, -1.0]), nv)
+        want = (base @ rot.T) @ axes  # view frame, then world
+        got = pr["dirs"] / np.linalg.norm(pr["dirs"], axis=1, keepdims=True)
+        assert np.abs(got - want).max() < 1e-5, (x, y)
+        # one step width long (the view's axes normalised to it, src/deao.rs:286)
+        assert np.allclose(np.linalg.norm(pr["dirs"], axis=1), p.step_width, rtol=1e-6)
+        # the openness of each ray against a fine walk of the DE along it
+        for k in range(n):
+            t = np.arange(pr["step_ao"] * pr["dsm"], pr["max_dist"] + 1e-9, 0.05)
+            de, _ = O.de_many(p, pr["point"][None, :] + np.outer(t, got[k]) * p.step_width)
+            walk = min(1.0, float(np.min((de - pr["ms_ao"] + t * pr["md_d10"]) / t)))
+            walk = max(walk, 0.0)
+            s = pr["min_ra"][k] / pr["de_mul"]
+            assert s >= walk - 0.02, (x, y, k, s, walk)
+            gaps.append(s - walk)
+        if np.all(pr["min_ra"] / pr["de_mul"] >= 1.0 - 1e-6):
+            assert amb == 0
+    gaps = np.abs(np.array(gaps))
+    assert np.median(gaps) <= 0.025 and (gaps < 0.02).mean() >= 0.5, (np.median(gaps), (gaps < 0.02).mean())
+    assert gaps.max() <= 0.3, gaps.max()
