@@ -1,31 +1,40 @@
 #!/usr/bin/env python3
-"""Benchmark of the HIP DE ray marcher on one MI355X: learned + batched
-against plain.
+"""Benchmark of the HIP DE ray marcher on one MI355X: the learned warm
+start plus batching against the plain FP32 marcher.
 
 Workload: the power-8 Mandelbulb Z-buffer at 1920 x 1080 (the small
 configuration's scene -- 60 iterations, bailout r^2 = 16, analytic DE
 0.5 r ln r / dr, camera start plane 2.5 from the origin, FOV 30, seeded yaw
-and pitch, 6 bisection steps, 500-step cap -- rendered at 1080p).  Inputs
-are parameters; the outputs stay in device memory (HBM).
+and pitch, 6 bisection steps, 500-step cap -- rendered at 1080p), over
+`--batch` frames of distinct seeded cameras (seeds 1 ..).  Inputs are
+parameters; the outputs (depth, iterations, normals) stay in device memory.
 
+The warm start's network is trained first (untimed) on the plain marcher's
+(ray, hit distance) pairs of 48 other seeded cameras at 240 x 135 (seeds
+10001 .., about 1.5 M pairs), none of them a benchmark frame.
+
+Legs (every leg marches the same frames):
 - plain: one frame per call of mb3d_hip_march, the plain FP32 march and its
-  f64 finish.
-- plain + batched: the same frames in one HIP-graph plan without the
-  network (what batching alone gives).
-- learned + batched (the value): the warm-start network, trained first
-  (untimed) on the plain marcher's (ray, hit distance) pairs of the same
-  view at 480 x 270, and `--batch` copies of the frame in one HIP-graph
-  plan (one persistent launch of each kernel for all of them); one step is
-  one replay of the plan.
+  f64 finish;
+- plain + batched: the frames in one HIP-graph plan (one persistent launch
+  of each kernel for all of them), autotuned;
+- cone + batched: plus the cone ray per tile (the reference's cone march):
+  a tile whose cone ray passes the end of the view is background, every
+  other ray marches plainly -- the plain picture bit for bit;
+- learned + batched: the network's hit jumps, each checked by the exact DE
+  at its landing and rolled back to the plain march on failure;
+- learned + cone + batched (the value): both.
+One step is one replay of the plan.
 
-Prints one JSON line: Mpixels/s of the three paths, the roofline figure of
-the learned + batched path's dominant kernel (its algorithmic FLOPs per
-launch over its HIP-event time on the library's stream), the CPU baseline
-(the f64 oracle on the host cores) and the accuracy of the plain and the
-learned march against the oracle.  Beside the metric: the painter's kernel
-on the frame's G-buffer (device time, bytes, the oracle's painter on the
-host cores), and the frame's first-bounce reflection rays through the
-secondary-ray kernels (device time, the oracle on a sample).
+Prints one JSON line: Mpixels/s of the legs, each path's accuracy against
+the f64 oracle on three of the frames, the roofline figures of the learned
+path's kernels (the FP32 march against the FP32 vector peak, the network
+against the BF16 MFMA peak, the cone rays, finish and normals against the
+FP64 vector peak; algorithmic FLOPs per launch over HIP-event time on the
+library's stream), the CPU baseline (the f64 oracle on the host cores, and
+on one core), and the 64-frame animation of mixed sizes.  Beside the
+metric: the painter, the post passes, the frame's reflection rays and the
+Monte Carlo batches.
 """
 from __future__ import annotations
 
@@ -44,25 +53,32 @@ import mb3d_hip as H  # noqa: E402
 
 PEAK_FP32 = 157.3e12  # MI355X FP32 vector, FLOP/s (published)
 PEAK_FP64 = 78.6e12   # MI355X FP64 vector, FLOP/s (published)
+PEAK_BF16 = 2.5e15    # MI355X dense BF16 MFMA, FLOP/s (published)
 PEAK_HBM = 8.0e12     # MI355X HBM3E, bytes/s (published)
 # algorithmic FLOPs per unit of work (the reference's formulas, counted by op)
 F_P8 = 60.0     # int_pow power 8 (src/formulas.rs:837-858) + the rout update
 F_BOX = 31.0    # amazing_box (src/formulas.rs:707-736) + the rout update
 F_STEP = 25.0   # one march step of march_from (src/calc.rs:2555-2640)
 F_DE = 8.0      # DE tail: sqrt, log, divide, scale, clamp
+# the network's forward pass per ray: 80 -> 64 -> 64 -> 64 -> 2, two FLOPs a multiply-add
+F_MLP = 2.0 * (80 * 64 + 64 * 64 + 64 * 64 + 64 * 2)
+TRAIN_SEEDS = tuple(range(10001, 10049))  # the network's 48 cameras: none is a benchmark frame
+ACC_FRAMES = (0, 16, 32)  # the frames whose accuracy is measured against the oracle
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic_mandelbulb_p8_1080p.json")
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="timed replays of the batch (and plain frames x batch)")
+    ap.add_argument("--steps", type=int, default=10, help="timed replays of each batched leg (plain: frames x steps)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=48, help="frames per launch of the batched paths (at most 64)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=1, help="the first frame's camera seed")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy legs")
+    ap.add_argument("--no-anim", action="store_true", help="skip the 64-frame animation leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also copy a 512 MiB device buffer once (the HBM counter scale of a profile)")
@@ -78,10 +94,36 @@ def flops_f32(c: dict) -> float:
 
 
 def flops_f64(c: dict) -> float:
-    """The finish and normals passes': the f64 iterations and DEs, and the
-    normals' FP32 stencil (unpacked FP32 issues at the FP64 rate)."""
+    """The cone rays', finish's and normals passes': the f64 iterations and
+    DEs, and the normals' FP32 stencil (unpacked FP32 issues at the FP64
+    rate)."""
     return (c["its_p8_f64"] * F_P8 + c["its_box_f64"] * F_BOX + c["de_evals_f64"] * F_DE +
             c["its_normals_f32"] * F_P8 + 6 * c["hits"] * F_DE)
+
+
+def with_opts(base, mlp=None, cone=False):
+    """A copy of `base` (tile, grid, split, refill) with the network and the
+    cone rays set."""
+    o = H.Options()
+    H.C.pointer(o)[0] = base
+    o.mlp = mlp.h if mlp is not None else None
+    o.mlp_cone = 1 if cone else 0
+    return o
+
+
+def traffic_of(kernel: str, batch: int, width: int, height: int):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy kernel's
+    counts), when it was measured on this workload; None otherwise."""
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if (t.get("batch"), t.get("width"), t.get("height")) != (batch, width, height):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return float(k["fetch_bytes"] + k["write_bytes"]) if k else None
 
 
 def main():
@@ -94,15 +136,15 @@ def main():
 
     lib = H.load()
     H.check(lib.mb3d_hip_init(0), "mb3d_hip_init")
-    sc = H.scene_bulb(a.width, a.height, seed=a.seed)
-    p = H.params_from_scene(sc)
+    scenes = [H.scene_bulb(a.width, a.height, seed=a.seed + i) for i in range(a.batch)]
+    params = [H.params_from_scene(s) for s in scenes]
+    frames = H.params_array(scenes)
     n = a.width * a.height
     dev = torch.device("cuda:0")
     depth = torch.empty(n * a.batch, dtype=torch.float32, device=dev)
     iters = torch.empty(n * a.batch, dtype=torch.int16, device=dev)
     normals = torch.empty(n * a.batch * 3, dtype=torch.float32, device=dev)
     out = H.Out(depth.data_ptr(), None, iters.data_ptr(), normals.data_ptr(), None, 1, 0)
-    plain_opts = H.options()
 
     if a.copy_calibration:
         src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)
@@ -111,83 +153,87 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    # --- the network (untimed): the plain marcher's pairs of the same view
+    # --- the network (untimed): the plain marcher's pairs of other cameras
     t = time.perf_counter()
-    tw, th = 480, a.height * 480 // a.width
-    net, curve = H.train_warm_start([H.params_from_scene(H.scene_bulb(tw, th, seed=a.seed))],
+    tw, th = 240, a.height * 240 // a.width
+    net, curve = H.train_warm_start([H.params_from_scene(H.scene_bulb(tw, th, seed=s)) for s in TRAIN_SEEDS],
                                     steps=a.train_steps, batch=8192, lr=3e-3, seed=0)
     train_s = time.perf_counter() - t
-    # hit jumps (verified) and miss jumps (opt-in: the network's view is the
-    # benchmark's own, trained at 480 x 270; its mask error is reported)
-    learn_opts = H.options(mlp=net, conf_miss=0.02)
 
-    def plain_frame():
-        st = lib.mb3d_hip_march(H.C.byref(p), None, 0, a.height, H.C.byref(out), H.C.byref(plain_opts),
-                                H.PROGRESS_FN(0), H.CANCEL_FN(0), None)
-        H.check(st, "mb3d_hip_march")
-        return H.counters()
+    # --- plain: one frame per call, every frame `steps` times
+    plain_opts = H.options()
 
-    # --- plain: one frame per call, the same number of frames as the batch leg
-    for _ in range(a.warmup):
-        plain_frame()
+    def march_one(p, o):
+        H.check(lib.mb3d_hip_march(H.C.byref(p), None, 0, a.height, H.C.byref(out), H.C.byref(o),
+                                   H.PROGRESS_FN(0), H.CANCEL_FN(0), None), "mb3d_hip_march")
+
+    for i in range(a.warmup):
+        march_one(params[i % a.batch], plain_opts)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(a.steps * a.batch):
-        cp = plain_frame()
+    for _ in range(a.steps):
+        for p in params:
+            march_one(p, plain_opts)
     torch.cuda.synchronize()
     plain_wall = time.perf_counter() - t0
+    cp = H.counters()
     plain_value = n * a.steps * a.batch / plain_wall / 1e6
 
-    # --- plain + batched: the same plan without the network (what the
-    # batching alone gives); both batched legs take the autotuner's tile,
-    # grid and iteration-cap split for their own options (untimed)
-    frames = H.params_array([sc] * a.batch)
+    # --- the batched legs: the autotuner's options for the plain batch and
+    # for the learned + cone batch (untimed); the cone-only and learned-only
+    # legs take the latter's launch options
     pb_opts, _ = H.autotune(frames, out, H.options())
-    lb_opts, _ = H.autotune(frames, out, H.options(mlp=net, conf_miss=0.02))
-    plan = H.Plan(frames, out, pb_opts)
-    for _ in range(a.warmup):
-        plan.launch(1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        plan.launch(1)
-    torch.cuda.synchronize()
-    pb_wall = time.perf_counter() - t0
-    plan.close()
-    plain_batched_value = n * a.batch * a.steps / pb_wall / 1e6
+    lc_opts, _ = H.autotune(frames, out, H.options(mlp=net, cone=True))
+    legs = {"plain_batched": with_opts(pb_opts), "cone_batched": with_opts(lc_opts, cone=True),
+            "learned_batched": with_opts(lc_opts, mlp=net), "learned_cone_batched": with_opts(lc_opts, mlp=net, cone=True)}
+    res = {}
+    for name, o in legs.items():
+        plan = H.Plan(frames, out, o)
+        for _ in range(a.warmup):
+            plan.launch(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            c = plan.launch(1)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        plan.close()
+        res[name] = {"value": n * a.batch * a.steps / wall / 1e6, "unit": "Mpixels/s",
+                     "ms_per_frame": wall * 1e3 / (a.steps * a.batch), "ms_per_step": wall * 1e3 / a.steps,
+                     "steps": c["steps"] // a.batch, "its_p8_f32": c["its_p8_f32"] // a.batch,
+                     "jumps": c["jumps"] // a.batch, "rollbacks": c["rollbacks"] // a.batch,
+                     "cone_background": c["cone_background"] // a.batch,
+                     "options": {"tile": [o.tile_w, o.tile_h], "waves_per_cu": o.waves_per_cu,
+                                 "it_split": o.it_split, "refill_min": o.refill_min}}
+    lead = res["learned_cone_batched"]
+    value = lead["value"]
 
-    # --- learned + batched: the frame `batch` times in one graph plan
-    plan = H.Plan(frames, out, lb_opts)
-    for _ in range(a.warmup):
-        plan.launch(1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        cl = plan.launch(1)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    plan.close()
-    value = n * a.batch * a.steps / wall / 1e6
-
-    # --- the dominant kernel of the learned + batched path: per-kernel
-    # HIP-event times of the same batch launched eagerly (events bracket the
-    # march -- with the inference ahead of it -- and the finish on the
-    # library's stream; a captured graph does not time its nodes), averaged
-    ms_m, ms_f, cf = [], [], None
+    # --- the kernels of the learned + cone path: HIP-event times of the same
+    # batch launched eagerly on the library's stream (a captured graph does
+    # not time its nodes), averaged
+    lo = legs["learned_cone_batched"]
+    tms = {"mlp": [], "cone": [], "march": [], "finish": []}
+    cf = None
     for _ in range(max(3, a.steps)):
-        H.check(lib.mb3d_hip_march_batch(frames, a.batch, H.C.byref(out), H.C.byref(lb_opts)), "march_batch")
+        H.check(lib.mb3d_hip_march_batch(frames, a.batch, H.C.byref(out), H.C.byref(lo)), "march_batch")
         cf = H.counters()
-        ms_m.append(cf["march_f32_ms"])
-        ms_f.append(cf["finish_f64_ms"])
-    t_march, t_finish = float(np.mean(ms_m)), float(np.mean(ms_f))
-    k_march = {"kernel": "k_mlp_propose + k_march32 (batch)", "bound": "valu-fp32", "kernel_ms": t_march,
-               "flops_per_launch": flops_f32(cf), "peak": PEAK_FP32 / 1e12}
-    k_finish = {"kernel": "k_finish64 + k_normals (batch)", "bound": "valu-fp64", "kernel_ms": t_finish,
-                "flops_per_launch": flops_f64(cf), "peak": PEAK_FP64 / 1e12}
-    for k in (k_march, k_finish):
+        for k, f in (("mlp", "mlp_ms"), ("cone", "cone_ms"), ("march", "march_f32_ms"), ("finish", "finish_f64_ms")):
+            tms[k].append(cf[f])
+    t_k = {k: float(np.mean(v)) for k, v in tms.items()}
+    items = sum(((s.width + lo.tile_w - 1) // lo.tile_w) * lo.tile_w * ((s.height + lo.tile_h - 1) // lo.tile_h) *
+                lo.tile_h for s in scenes)
+    kernels = [
+        {"kernel": "k_march32 (batch)", "bound": "valu-fp32", "kernel_ms": t_k["march"],
+         "flops_per_launch": flops_f32(cf), "peak": PEAK_FP32 / 1e12},
+        {"kernel": "k_mlp_propose (batch)", "bound": "mfma-bf16", "kernel_ms": t_k["mlp"],
+         "flops_per_launch": F_MLP * items, "peak": PEAK_BF16 / 1e12},
+        {"kernel": "k_cone + k_finish64 + k_normals (batch)", "bound": "valu-fp64",
+         "kernel_ms": t_k["cone"] + t_k["finish"], "flops_per_launch": flops_f64(cf), "peak": PEAK_FP64 / 1e12},
+    ]
+    for k in kernels:
         k["achieved"] = k["flops_per_launch"] / (k["kernel_ms"] * 1e-3) / 1e12
         k["frac"] = k["achieved"] / k["peak"]
-    dom = k_finish if t_finish >= t_march else k_march
+    dom = max(kernels, key=lambda k: k["kernel_ms"])
     roofline = {
         "bound": dom["bound"],
         "kernel": dom["kernel"],
@@ -195,14 +241,18 @@ def main():
         "peak": dom["peak"],
         "unit": "TFLOP/s",
         "frac": dom["frac"],
-        "traffic": None,
+        "traffic": traffic_of(dom["kernel"], a.batch, a.width, a.height),
         "kernel_ms": dom["kernel_ms"],
         "flops_per_launch": dom["flops_per_launch"],
-        "kernels": [k_march, k_finish],
+        "kernels": kernels,
     }
-    # --- the painter (beside the metric): the frame's G-buffer, marched once
-    # into device memory, lit and coloured on the device by k_paint (the
-    # reference's default lighting); its HIP-event time per launch
+    for k in kernels:
+        k["traffic"] = traffic_of(k["kernel"], a.batch, a.width, a.height)
+
+    # --- the painter (beside the metric): the first frame's G-buffer,
+    # marched once into device memory, lit and coloured on the device by
+    # k_paint (the reference's default lighting); its HIP-event time
+    sc, p = scenes[0], params[0]
     gbuf = torch.empty(n * 5, dtype=torch.int32, device=dev)
     rgb = torch.empty(n * 3, dtype=torch.uint8, device=dev)
     out_g = H.Out(None, None, None, None, gbuf.data_ptr(), 1, 0)
@@ -258,8 +308,8 @@ def main():
 
     # --- the Monte Carlo renderer's marches (beside the metric): every pixel's
     # primary ray with the inside probe, then a shadow ray towards one light
-    # from each surface it met (tasks and results in device memory); the k_mc
-    # batches' HIP-event time
+    # from each surface it met (tasks and results in device memory); the
+    # k_mc batches' HIP-event time
     def mc_time(t, reps):
         d_t = torch.from_numpy(t.view(np.uint8).copy()).to(dev)
         d_r = torch.empty(len(t) * H.MC_RES_DTYPE.itemsize, dtype=torch.uint8, device=dev)
@@ -281,6 +331,14 @@ def main():
                       "mrays_s": len(mc_prim) / (t_mc0 * 1e-3) / 1e6},
           "shadow": {"rays": len(mc_shad), "blocked": int((mc_sres["len"] > 0.01).sum()), "kernel_ms": t_mc1,
                      "mrays_s": len(mc_shad) / (t_mc1 * 1e-3) / 1e6, "length": 300.0}}
+    del depth, iters, normals, gbuf, rgb, d_tasks, d_hits
+
+    # --- the batched animation (the mixed-size configuration, beside the
+    # metric): 64 frames of {320x240 .. 1920x1080} along a seeded camera path
+    # around the bulb, in one plan, with the batch legs' launch options
+    anim = None
+    if not a.no_anim:
+        anim = animation_leg(a, lib, pb_opts, legs["learned_cone_batched"], net)
 
     result = {
         "metric": "Mpixels/s 1080p Mandelbulb-p8 Z-buffer; learned vs plain at equal depth error",
@@ -289,30 +347,29 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": wall * 1e3 / a.steps,
+        "ms_per_step": lead["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32/f64",
-        "data": "synthetic (seeded camera, random-initialised network trained on the plain marcher's pairs)",
+        "data": "synthetic (seeded cameras; network random-initialised, trained on the plain marcher's pairs of 48 "
+                "other cameras)",
         "config": {
-            "workload": f"mandelbulb-p8 z-buffer {a.width}x{a.height}, analytic DE, 60 its, bailout r2=16, "
-                        f"fov 30, seed {a.seed}; learned warm start + {a.batch} frames per graph launch",
-            "path": "learned+batched",
+            "workload": f"mandelbulb-p8 z-buffer {a.width}x{a.height}, analytic DE, 60 its, bailout r2=16, fov 30; "
+                        f"{a.batch} frames of distinct seeded cameras (seeds {a.seed}..{a.seed + a.batch - 1}) per "
+                        "graph launch; learned warm start (hit jumps) + cone-verified background tiles",
+            "path": "learned+cone+batched",
             "batch": a.batch,
-            "autotuned": {"learned": [lb_opts.tile_w, lb_opts.tile_h, lb_opts.waves_per_cu, lb_opts.it_split],
-                          "plain_batched": [pb_opts.tile_w, pb_opts.tile_h, pb_opts.waves_per_cu, pb_opts.it_split]},
+            "train_cameras": list(TRAIN_SEEDS),
         },
         "plain": {"value": plain_value, "unit": "Mpixels/s", "ms_per_frame": plain_wall * 1e3 / (a.steps * a.batch),
                   "kernel_ms_per_frame": cp["kernel_ms"], "march_f32_ms": cp["march_f32_ms"],
                   "finish_f64_ms": cp["finish_f64_ms"], "steps": cp["steps"], "its_p8_f32": cp["its_p8_f32"]},
-        "plain_batched": {"value": plain_batched_value, "unit": "Mpixels/s",
-                          "ms_per_frame": pb_wall * 1e3 / (a.steps * a.batch)},
+        **res,
         "speedup_vs_plain": value / plain_value,
-        "speedup_vs_plain_batched": value / plain_batched_value,
-        "learned": {"ms_per_frame": wall * 1e3 / (a.steps * a.batch), "jumps": cl["jumps"] // a.batch,
-                    "rollbacks": cl["rollbacks"] // a.batch, "steps": cl["steps"] // a.batch,
-                    "its_p8_f32": cl["its_p8_f32"] // a.batch, "train_s": train_s,
+        "speedup_vs_plain_batched": value / res["plain_batched"]["value"],
+        "speedup_of_the_network": value / res["cone_batched"]["value"],
+        "network": {"train_s": train_s, "train_pairs": int(len(TRAIN_SEEDS) * tw * th),
                     "train_loss": [float(curve[:20].mean()), float(curve[-100:].mean())]},
         "roofline": roofline,
         "paint": paint,
@@ -320,44 +377,111 @@ def main():
         "rays": rays,
         "mc": mc,
     }
+    if anim is not None:
+        result["animation"] = anim
     if not a.no_cpu:
-        result["cpu_baseline"], result["accuracy"] = cpu_leg(sc, p, a, learn_opts)
-        paint["cpu"] = paint_cpu_leg(cam, lv, paint_gbuf, result["cpu_baseline"]["cores"])
-        rays["cpu"] = rays_cpu_leg(p, tasks, ray_opts, result["cpu_baseline"]["cores"])
-        mc["cpu"] = mc_cpu_leg(p, mc_prim, mc_shad, result["cpu_baseline"]["cores"])
+        result["cpu_baseline"], result["accuracy"] = cpu_leg(scenes, params, a, legs, net)
+        cores = result["cpu_baseline"]["cores"]
+        paint["cpu"] = paint_cpu_leg(cam, lv, paint_gbuf, cores)
+        rays["cpu"] = rays_cpu_leg(p, tasks, ray_opts, cores)
+        mc["cpu"] = mc_cpu_leg(p, mc_prim, mc_shad, cores)
+        result["at_equal_depth_error"] = {k: v["meets_bar"] for k, v in result["accuracy"].items()}
     net.close()
     print(json.dumps(result))
 
 
-def cpu_leg(sc, p, a, learn_opts):
+def animation_leg(a, lib, pb_opts, lc_opts, net):
+    """The 64-frame animation of mixed sizes in one plan: plain + batched and
+    learned + cone + batched, Mpixels/s of a replay."""
+    import torch
+
+    scenes = H.animation(64, seed=a.seed)
+    frames = H.params_array(scenes)
+    n = sum(s.width * s.height for s in scenes)
+    dev = torch.device("cuda:0")
+    depth = torch.empty(n, dtype=torch.float32, device=dev)
+    iters = torch.empty(n, dtype=torch.int16, device=dev)
+    normals = torch.empty(n * 3, dtype=torch.float32, device=dev)
+    out = H.Out(depth.data_ptr(), None, iters.data_ptr(), normals.data_ptr(), None, 1, 0)
+    res = {"frames": 64, "pixels": n, "sizes": sorted({(s.width, s.height) for s in scenes})}
+    for name, o in (("plain_batched", pb_opts), ("learned_cone_batched", lc_opts)):
+        plan = H.Plan(frames, out, o)
+        plan.launch(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = max(2, a.steps // 3)
+        for _ in range(reps):
+            plan.launch(1)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        plan.close()
+        res[name] = {"value": n / wall / 1e6, "unit": "Mpixels/s", "ms_per_launch": wall * 1e3}
+    res["speedup"] = res["learned_cone_batched"]["value"] / res["plain_batched"]["value"]
+    return res
+
+
+def cpu_leg(scenes, params, a, legs, net):
     """The f64 oracle (a restatement of the reference's CPU marcher) on the
-    host cores, on the same frame, and both paths' accuracy against it."""
+    host cores, on a frame of the benchmark, and on one core on a band of
+    its rows; each path's accuracy against it on three frames."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as O
     import parity
 
-    threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
-    po = O.params(sc)
-    t = time.perf_counter()
-    ref = O.render(po, threads=threads)
-    dt = time.perf_counter() - t
+    host_threads = os.cpu_count() or 1
+    share = len(os.sched_getaffinity(0))
+    # the box's CPU share for one GPU is 16 cores (its worker-pool limit)
+    threads = a.cpu_threads or min(share, 16)
     keys = ("mask_agree", "depth_rel_p99", "depth_rel_max", "depth_ok_frac", "normal_ok_frac", "normal_deg_p99",
             "iters_equal_inner")
-    acc = {}
-    for name, o in (("plain", None), ("learned", learn_opts)):
-        g = H.march(p, opts=o)
-        m = parity.compare(g, ref, p.width, p.height, p.step_width, 2.0)
-        acc[name] = {k: m[k] for k in keys}
+    paths = {"plain": H.options(), "cone": H.options(cone=True), "learned": H.options(mlp=net),
+             "learned_cone": H.options(mlp=net, cone=True)}
+    lo = legs["learned_cone_batched"]
+    for o in paths.values():
+        o.tile_w, o.tile_h = lo.tile_w, lo.tile_h
+    acc = {k: {m: [] for m in keys} for k in paths}
+    t_full = []
+    for f in ACC_FRAMES:
+        if f >= len(scenes):
+            continue
+        t = time.perf_counter()
+        ref = O.render(O.params(scenes[f]), threads=threads)
+        t_full.append(time.perf_counter() - t)
+        p = params[f]
+        for name, o in paths.items():
+            m = parity.compare(H.march(p, opts=o), ref, p.width, p.height, p.step_width, 2.0)
+            for k in keys:
+                acc[name][k].append(m[k])
+    worst = {"mask_agree": min, "depth_rel_p99": max, "depth_rel_max": max, "depth_ok_frac": min,
+             "normal_ok_frac": min, "normal_deg_p99": max, "iters_equal_inner": min}
+    out = {}
+    for name in paths:
+        r = {k: worst[k](v) for k, v in acc[name].items()}
+        r["frames"] = [f for f in ACC_FRAMES if f < len(scenes)]
+        r["meets_bar"] = bool(r["mask_agree"] >= 0.999 and r["depth_ok_frac"] >= 0.999 and r["normal_ok_frac"] >= 0.999)
+        out[name] = r
+    # one core: a band of rows of the first frame
+    p0 = O.params(scenes[0])
+    rows = max(1, a.height // 16)
+    r0 = (a.height - rows) // 2
+    t = time.perf_counter()
+    O.render(p0, row0=r0, rows=rows, threads=1)
+    dt1 = time.perf_counter() - t
+    dt = float(np.min(t_full))
     cpu = {
-        "value": p.width * p.height / dt / 1e6,
+        "value": a.width * a.height / dt / 1e6,
         "unit": "Mpixels/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"one full {p.width}x{p.height} frame of the same workload, {dt:.2f} s, "
-                  f"{threads} threads, rows interleaved as src/render.rs:757-803",
+        "sample": f"full {a.width}x{a.height} frames of the workload ({len(t_full)} frames, the fastest "
+                  f"{dt:.2f} s), {threads} threads, rows interleaved as src/render.rs:757-803",
+        "host_threads": host_threads,
+        "affinity_threads": share,
+        "one_core": {"value": a.width * rows / dt1 / 1e6, "unit": "Mpixels/s", "cores": 1,
+                     "sample": f"rows {r0}..{r0 + rows - 1} of the first frame, {dt1:.2f} s"},
     }
-    return cpu, acc
+    return cpu, out
 
 
 def paint_cpu_leg(cam, lv, g, threads):

@@ -237,7 +237,7 @@ typedef struct mb3d_hip_options {
     /* the learned warm start (extension; NULL: the plain march).  A ray
      * whose hit probability is at least mlp_conf_hit (0: 0.5; above 1: no
      * hit jumps) starts at the predicted hit distance less mlp_margin_rel of
-     * it (0: 0.08) and mlp_margin_abs scene units.  The start is kept when
+     * it (0: 0.15) and mlp_margin_abs scene units.  The start is kept when
      * the exact DE there clears the stop distance's hand-off guard and a
      * tenth of the predicted distance left, with the iteration limit not
      * reached; otherwise the ray marches from its start plane.  A ray whose
@@ -379,9 +379,13 @@ typedef struct mb3d_hip_counters {
     uint64_t rollbacks;      /* warm starts rejected by the exact DE (overshoot) */
     double kernel_ms;        /* device time of the march kernels (HIP events) */
     double post_ms;          /* device time of the post kernels */
-    double march_f32_ms;     /* of kernel_ms: the FP32 march */
-    double finish_f64_ms;    /* of kernel_ms: the FP64 finish */
+    double march_f32_ms;     /* of kernel_ms: the FP32 march (k_march32) */
+    double finish_f64_ms;    /* of kernel_ms: the FP64 finish and the normals */
     double paint_ms;         /* device time of the last mb3d_hip_paint's kernel (HIP events) */
+    uint64_t cone_rays;      /* cone rays marched (one per tile, mlp_cone) */
+    uint64_t cone_background; /* pixels made background by their tile's cone ray */
+    double mlp_ms;           /* of kernel_ms: the warm start's network (k_mlp_propose) */
+    double cone_ms;          /* of kernel_ms: the cone rays (k_cone) */
 } mb3d_hip_counters;
 
 typedef void (*mb3d_hip_progress_fn)(void *user, int64_t done, int64_t total);

@@ -73,7 +73,9 @@ struct DevP {
     // idle lanes a wave of the FP32 march gathers before it refills them
     // (1: every turn that has an idle lane)
     int refill_min;
-    int pad_core;
+    // the cone rays (k_cone): 0 off, 1 a tile whose cone ray passes the end
+    // of the view is background, 2 also start the tile's rays at its depth
+    int cone_mode;
 
     // ---- the general kind (K_GEN) only ----
     double opt2[MB3D_HIP_MAX_SLOTS][3];
@@ -103,6 +105,7 @@ struct Cnt {
     uint32_t nrm_32 = 0;  // of p8_32 and box_32: the normals pass's FP32 stencil
     uint32_t hits = 0, overflow = 0;
     uint32_t jumps = 0, rollbacks = 0;  // warm starts kept, warm starts undone
+    uint32_t cone_rays = 0, cone_bg = 0;  // cone rays marched; pixels their cone made background
 };
 
 // ---------------------------------------------------------------------------
@@ -1093,6 +1096,23 @@ __device__ __forceinline__ void pixel_ray_tab(const DevP &P, const double2 *__re
     w2 *= nrm;
     for (int k = 0; k < 3; k++) v[k] = w0 * P.vg[0][k] + w1 * P.vg[1][k] + w2 * P.vg[2][k];
     const double fx = x, fy = P.row0 + yr;
+    for (int k = 0; k < 3; k++) c[k] = P.ystart[k] + P.vg[0][k] * fx + P.vg[1][k] * fy;
+}
+
+// The ray of the picture's point (fx, fy), fractions allowed (Marcher::ray,
+// src/calc.rs:2328-2355), Common optic: pixel_ray at non-integer pixels.
+__device__ __forceinline__ void frac_ray(const DevP &P, double fx, double fy, double c[3], double v[3]) {
+    const double cafy = (fy / (double)P.height - 0.5) * P.fov_y;
+    const double cafx = (double)((P.fovx_off - (float)(fx + 1.0)) * P.fovx_mul);
+    double sx, cx, sy, cy;
+    sincos(cafy, &sx, &cx);
+    sincos(cafx, &sy, &cy);
+    double w0 = -sy, w1 = sx, w2 = cx * cy;
+    const double nrm = 1.0 / sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+    w0 *= nrm;
+    w1 *= nrm;
+    w2 *= nrm;
+    for (int k = 0; k < 3; k++) v[k] = w0 * P.vg[0][k] + w1 * P.vg[1][k] + w2 * P.vg[2][k];
     for (int k = 0; k < 3; k++) c[k] = P.ystart[k] + P.vg[0][k] * fx + P.vg[1][k] * fy;
 }
 

@@ -81,7 +81,10 @@ struct Ctx {
     int device = -1;
     int cus = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // ev[0] start, ev[4] after the ray tables, ev[5] after the cone rays,
+    // ev[6] after the network, ev[3] after the FP32 march, ev[1] after the
+    // finish and normals, ev[2] after the post passes
+    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // grow-only device buffers
     void *d_out = nullptr;
     size_t out_bytes = 0;
@@ -311,6 +314,7 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
     d.tn_shift = d.tw_shift + __builtin_ctz((unsigned)d.tile_h);
     d.refill_min = o.refill_min > 0 ? std::min(o.refill_min, 64) : kRefillMin;
     d.rt_base = 0;
+    d.cone_mode = std::min(std::max(o.mlp_cone, 0), 2);
     d.tiles_x = (p.width + d.tile_w - 1) / d.tile_w;
     const long long tiles_y = (rows + d.tile_h - 1) / d.tile_h;
     d.npix = (long long)p.width * rows;
@@ -392,10 +396,21 @@ int dem_of(const mb3d_hip_params &p) {
 
 template <bool BATCH>
 void launch_propose(hipStream_t s, const DevP &d, const DevP *tab, int n, const MlpDev &M, const double2 *rtab,
-                    float2 *prop) {
+                    const ConeRes *cone, float2 *prop) {
     const long long groups = (d.nslots_q + 63) / 64;
     const int grid = (int)std::max<long long>(1, std::min<long long>((groups + 3) / 4, (long long)g_ctx.cus * 16));
-    hipLaunchKernelGGL((k_mlp_propose<BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, M, rtab, prop);
+    hipLaunchKernelGGL((k_mlp_propose<BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, M, rtab, cone, prop);
+}
+
+template <int KIND, int DEM, bool BATCH>
+void launch_cone(hipStream_t s, const DevP &d, const DevP *tab, int n, const double2 *rtab, ConeRes *out,
+                 unsigned long long *ctr) {
+    if constexpr (KIND == K_GEN) return;  // the FP32 march's kinds only
+    else {
+        const long long tiles = d.nslots_q >> d.tn_shift;
+        const int grid = (int)std::max<long long>(1, std::min<long long>((tiles + 255) / 256, (long long)g_ctx.cus * 8));
+        hipLaunchKernelGGL((k_cone<KIND, DEM, BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, rtab, out, ctr);
+    }
 }
 
 // the ray tables of one frame (tab NULL: `d`) or of a batch's n frames
@@ -645,8 +660,12 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     const size_t base_need = (cap * (sizeof(Cand) + sizeof(SurfRec) + sizeof(int32_t)) + 511) & ~(size_t)255;
     const size_t learn_need = M.enabled ? learn_bufs(nullptr, d.nslots_q, nullptr) : 0;
     const size_t rtab_bytes = ((size_t)ray_tab_entries(d) * sizeof(double2) + 255) & ~(size_t)255;
-    const size_t work_need = base_need + learn_need + rtab_bytes;
+    const bool cone = d.cone_mode != 0 && !f64 && !post_only;
+    const size_t cone_bytes = cone ? (size_t)(d.nslots_q >> d.tn_shift) * sizeof(ConeRes) : 0;
+    const size_t work_need = base_need + learn_need + rtab_bytes + cone_bytes;
     if (int e = ensure(&c.d_work, &c.work_bytes, work_need)) return e;
+    void *work = c.d_work;
+    unsigned long long *ctr_dev = c.d_ctr;
     Bufs b;
     std::memset(&b, 0, sizeof b);
     char *ob = (char *)c.d_out;
@@ -669,15 +688,17 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
         b.normals = out && out->normals ? (float *)(ob + L.normals) : nullptr;
         b.gbuf = (out && out->gbuf) || want_shadow || want_ao ? (uint32_t *)(ob + L.gbuf) : nullptr;
     }
-    b.cand = (Cand *)c.d_work;
-    b.surf = (SurfRec *)((char *)c.d_work + cap * sizeof(Cand));
+    b.cand = (Cand *)work;
+    b.surf = (SurfRec *)((char *)work + cap * sizeof(Cand));
     b.hits = (int32_t *)(b.surf + cap);
-    b.ctr = c.d_ctr;
-    if (M.enabled) learn_bufs((char *)c.d_work + base_need, d.nslots_q, &b);
-    double2 *rtab = (double2 *)((char *)c.d_work + base_need + learn_need);
+    b.ctr = ctr_dev;
+    if (M.enabled) learn_bufs((char *)work + base_need, d.nslots_q, &b);
+    double2 *rtab = (double2 *)((char *)work + base_need + learn_need);
     b.rtab = rtab;
+    b.cone = cone ? (ConeRes *)((char *)rtab + rtab_bytes) : nullptr;
+    if (!cone) d.cone_mode = 0;
 
-    HIP_TRY(hipMemsetAsync(c.d_ctr, 0, C_WORDS * sizeof(unsigned long long), c.stream));
+    HIP_TRY(hipMemsetAsync(ctr_dev, 0, C_WORDS * sizeof(unsigned long long), c.stream));
     if (post_only && !dev_out)
         HIP_TRY(hipMemcpyAsync(b.gbuf, post_gbuf, npix * 20, hipMemcpyHostToDevice, c.stream));
     const int kind = kind_of(p), dem = dem_of(p);
@@ -693,7 +714,16 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
             });
         } else {
             launch_ray_tab(stream, d, nullptr, 1, rtab);
-            if (M.enabled) launch_propose<false>(stream, d, nullptr, 1, M, rtab, b.prop);
+            HIP_TRY(hipEventRecord(c.ev[4], c.stream));
+            if (cone) {
+                dispatch(kind, dem, [&](auto K, auto D) {
+                    launch_cone<decltype(K)::value, decltype(D)::value, false>(stream, d, nullptr, 1, rtab, b.cone,
+                                                                              ctr_dev);
+                });
+            }
+            HIP_TRY(hipEventRecord(c.ev[5], c.stream));
+            if (M.enabled) launch_propose<false>(stream, d, nullptr, 1, M, rtab, b.cone, b.prop);
+            HIP_TRY(hipEventRecord(c.ev[6], c.stream));
             dispatch(kind, dem, [&](auto K, auto D) {
                 launch_march<decltype(K)::value, decltype(D)::value>(grid_march, stream, d, b, M);
             });
@@ -756,13 +786,20 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
         }
     }
     unsigned long long ctr[C_WORDS];
-    HIP_TRY(hipMemcpyAsync(ctr, c.d_ctr, sizeof ctr, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(hipMemcpyAsync(ctr, ctr_dev, sizeof ctr, hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(hipStreamSynchronize(c.stream));
-    float ms_march = 0, ms_post = 0, ms_a = 0;
+    float ms_march = 0, ms_post = 0, ms_a = 0, ms_mlp = 0, ms_cone = 0;
     HIP_TRY(hipEventElapsedTime(&ms_march, c.ev[0], c.ev[1]));
     HIP_TRY(hipEventElapsedTime(&ms_post, c.ev[1], c.ev[2]));
     const bool split = !post_only && !f64;
-    if (split) HIP_TRY(hipEventElapsedTime(&ms_a, c.ev[0], c.ev[3]));
+    if (split) {
+        HIP_TRY(hipEventElapsedTime(&ms_a, c.ev[0], c.ev[3]));
+        float ms_tab = 0;
+        HIP_TRY(hipEventElapsedTime(&ms_tab, c.ev[0], c.ev[4]));
+        HIP_TRY(hipEventElapsedTime(&ms_cone, c.ev[4], c.ev[5]));
+        HIP_TRY(hipEventElapsedTime(&ms_mlp, c.ev[5], c.ev[6]));
+        ms_mlp += ms_tab;  // (the ray tables, a few microseconds, with the network)
+    }
     mb3d_hip_counters &t = t_counters;
     const double paint_ms = t.paint_ms;  // the last paint's, kept
     std::memset(&t, 0, sizeof t);
@@ -781,9 +818,13 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     t.rollbacks = ctr[C_STATS + 10];
     t.its_other_f64 = ctr[C_STATS + 11];
     t.its_normals_f32 = ctr[C_STATS + 12];
+    t.cone_rays = ctr[C_STATS + 13];
+    t.cone_background = ctr[C_STATS + 14];
     t.kernel_ms = ms_march;
     t.post_ms = ms_post;
-    t.march_f32_ms = split ? ms_a : 0.0;
+    t.mlp_ms = split ? ms_mlp : 0.0;
+    t.cone_ms = split ? ms_cone : 0.0;
+    t.march_f32_ms = split ? ms_a - ms_mlp - ms_cone : 0.0;
     t.finish_f64_ms = split ? ms_march - ms_a : ms_march;
     if (progress) progress(user, 2, 2);
     // the overflow verdict (src/gpu.rs:2350-2375): the f64 finish already
@@ -808,7 +849,9 @@ struct Plan {
     Bufs b;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    // e0 start, ea after the ray tables and the cone rays, eb after the
+    // network, e1 after the FP32 march, e2 end
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, ea = nullptr, eb = nullptr;
     MlpDev M;  // the warm start (M.enabled = 0: none)
     bool n32 = true;  // the normals' stencil in FP32 (every frame allows it)
 };
@@ -823,6 +866,8 @@ void plan_free(Plan *pl) {
     if (pl->e0) (void)hipEventDestroy(pl->e0);
     if (pl->e1) (void)hipEventDestroy(pl->e1);
     if (pl->e2) (void)hipEventDestroy(pl->e2);
+    if (pl->ea) (void)hipEventDestroy(pl->ea);
+    if (pl->eb) (void)hipEventDestroy(pl->eb);
     delete pl;
 }
 
@@ -832,9 +877,17 @@ void plan_free(Plan *pl) {
 int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, stream, pl.d_ctr, (int)C_WORDS);
     launch_ray_tab(stream, pl.P0, pl.d_tab, pl.n, const_cast<double2 *>(pl.b.rtab));
-    if (pl.M.enabled) launch_propose<true>(stream, pl.P0, pl.d_tab, pl.n, pl.M, pl.b.rtab, pl.b.prop);
     const DevP *tab = pl.d_tab;
     const int n = pl.n;
+    if (pl.P0.cone_mode) {
+        dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
+            launch_cone<decltype(K)::value, decltype(D)::value, true>(stream, pl.P0, tab, n, pl.b.rtab, pl.b.cone,
+                                                                     pl.d_ctr);
+        });
+    }
+    if (split) HIP_TRY(hipEventRecord(pl.ea, stream));
+    if (pl.M.enabled) launch_propose<true>(stream, pl.P0, pl.d_tab, pl.n, pl.M, pl.b.rtab, pl.b.cone, pl.b.prop);
+    if (split) HIP_TRY(hipEventRecord(pl.eb, stream));
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
         launch_march<decltype(K)::value, decltype(D)::value, true>(pl.grid_march, stream, pl.P0, pl.b, pl.M, tab, n);
     });
@@ -906,13 +959,16 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     };
     if (hipMalloc((void **)&pl->d_tab, sizeof(DevP) * n) != hipSuccess ||
         hipMalloc(&pl->d_work, base_need + (pl->M.enabled ? learn_bufs(nullptr, q, nullptr) : 0) +
-                                   (size_t)rt * sizeof(double2)) != hipSuccess ||
+                                   (size_t)rt * sizeof(double2) +
+                                   (pl->P0.cone_mode ? (size_t)(q >> pl->P0.tn_shift) * sizeof(ConeRes) : 0)) !=
+            hipSuccess ||
         hipMalloc((void **)&pl->d_ctr, C_WORDS * sizeof(unsigned long long)) != hipSuccess)
         return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipMalloc failed for a batch"));
     if (hipMemcpy(pl->d_tab, pl->tab.data(), sizeof(DevP) * n, hipMemcpyHostToDevice) != hipSuccess)
         return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipMemcpy failed for a batch"));
     if (hipEventCreate(&pl->e0) != hipSuccess || hipEventCreate(&pl->e1) != hipSuccess ||
-        hipEventCreate(&pl->e2) != hipSuccess)
+        hipEventCreate(&pl->e2) != hipSuccess || hipEventCreate(&pl->ea) != hipSuccess ||
+        hipEventCreate(&pl->eb) != hipSuccess)
         return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipEventCreate failed"));
     std::memset(&pl->b, 0, sizeof pl->b);
     pl->b.depth = out->depth;
@@ -925,6 +981,7 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     pl->b.hits = (int32_t *)(pl->b.surf + cap);
     if (pl->M.enabled) learn_bufs((char *)pl->d_work + base_need, q, &pl->b);
     pl->b.rtab = (const double2 *)((char *)pl->d_work + base_need + (pl->M.enabled ? learn_bufs(nullptr, q, nullptr) : 0));
+    pl->b.cone = pl->P0.cone_mode ? (ConeRes *)((char *)pl->b.rtab + (size_t)rt * sizeof(double2)) : nullptr;
     pl->b.ctr = pl->d_ctr;
     if (graph) {
         if (hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -958,9 +1015,13 @@ int plan_run(Plan &pl, int repeats) {
     unsigned long long ctr[C_WORDS];
     HIP_TRY(hipMemcpyAsync(ctr, pl.d_ctr, sizeof ctr, hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(hipStreamSynchronize(c.stream));
-    float ms_all = 0, ms_a = -1.0f;
+    float ms_all = 0, ms_a = -1.0f, ms_mlp = -1.0f, ms_cone = -1.0f;
     if (hipEventElapsedTime(&ms_all, pl.e0, pl.e2) != hipSuccess) ms_all = -1.0f;
-    if (!pl.exec && repeats == 1 && hipEventElapsedTime(&ms_a, pl.e0, pl.e1) != hipSuccess) ms_a = -1.0f;
+    if (!pl.exec && repeats == 1) {
+        if (hipEventElapsedTime(&ms_a, pl.e0, pl.e1) != hipSuccess) ms_a = -1.0f;
+        if (hipEventElapsedTime(&ms_cone, pl.e0, pl.ea) != hipSuccess) ms_cone = -1.0f;
+        if (hipEventElapsedTime(&ms_mlp, pl.ea, pl.eb) != hipSuccess) ms_mlp = -1.0f;
+    }
     (void)hipGetLastError();
     mb3d_hip_counters &t = t_counters;
     const double paint_ms = t.paint_ms;  // the last paint's, kept
@@ -980,8 +1041,13 @@ int plan_run(Plan &pl, int repeats) {
     t.rollbacks = ctr[C_STATS + 10];
     t.its_other_f64 = ctr[C_STATS + 11];
     t.its_normals_f32 = ctr[C_STATS + 12];
+    t.cone_rays = ctr[C_STATS + 13];
+    t.cone_background = ctr[C_STATS + 14];
     t.kernel_ms = ms_all > 0.0f ? ms_all / repeats : -1.0;
-    t.march_f32_ms = ms_a;
+    const bool split = ms_a > 0.0f && ms_mlp >= 0.0f && ms_cone >= 0.0f;
+    t.mlp_ms = split ? ms_mlp : -1.0;
+    t.cone_ms = split ? ms_cone : -1.0;
+    t.march_f32_ms = split ? ms_a - ms_mlp - ms_cone : -1.0;
     t.finish_f64_ms = ms_a > 0.0f ? t.kernel_ms - ms_a : -1.0;
     return MB3D_HIP_OK;
 }

@@ -68,7 +68,7 @@ enum : int {
     C_POSTQ = 4,   // next item of the shadow pass
     C_AOQ = 5,     // next item of the DE AO pass
     C_SURF = 6,    // refined surface list length (the normals pass's items)
-    C_STATS = 8,   // 13 work counters from here
+    C_STATS = 8,   // 15 work counters from here
     C_WORDS = 24
 };
 
@@ -88,6 +88,17 @@ struct Bufs {
     // the frames' ray tables (k_ray_tab), read by the FP32 march and the
     // warm start's proposals
     const double2 *rtab;
+    // the cone rays' results, one per tile of the queue (k_cone)
+    struct ConeRes *cone;
+};
+
+// A tile's cone ray (k_cone): the depth up to which every ray of the tile
+// is free of the surface (+inf: the cone passed the end of the view) and the
+// dynamic fog's step count up to there.
+struct ConeRes {
+    double t;
+    float count;
+    int32_t pad;
 };
 
 // Fills the ray tables of the frames `tab[0..n)` (one frame: `tab` = &P):
@@ -128,10 +139,11 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 __device__ inline void flush_counters(const Cnt &c, unsigned long long *ctr) {
-    const unsigned long long v[13] = {c.steps,  c.de32, c.de64,     c.p8_32, c.box_32,    c.p8_64, c.box_64,
-                                      c.hits,   c.overflow, c.jumps, c.rollbacks, c.oth_64, c.nrm_32};
+    const unsigned long long v[15] = {c.steps,  c.de32, c.de64,     c.p8_32, c.box_32,    c.p8_64, c.box_64,
+                                      c.hits,   c.overflow, c.jumps, c.rollbacks, c.oth_64, c.nrm_32,
+                                      c.cone_rays, c.cone_bg};
 #pragma unroll
-    for (int k = 0; k < 13; k++) {
+    for (int k = 0; k < 15; k++) {
         const unsigned long long s = wave_sum(v[k]);
         if (lane_id() == 0 && s) atomicAdd(&ctr[C_STATS + k], s);
     }
@@ -364,7 +376,8 @@ __device__ __forceinline__ bool item_pixel(const DevP &F, int ql, int &x, int &y
 // +0.75 ms a 1080p frame at a fence pair per chunk of 256 rays).
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__restrict__ gtab, int nframes, MlpDev M,
-                                                     const double2 *__restrict__ rtab, float2 *prop) {
+                                                     const double2 *__restrict__ rtab, const ConeRes *__restrict__ cone,
+                                                     float2 *prop) {
     const GlobalFrames tab{reinterpret_cast<const char *>(gtab)};
     __shared__ __attribute__((aligned(16))) char lds[4][MLP_LDS_BYTES];
     __shared__ __attribute__((aligned(16))) mlp_s4 wl[MLP_TILES * 64];
@@ -384,6 +397,14 @@ __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__rest
             f = in_q ? frame_of(tab, nframes, h, q) : 0;
         }
         const DevP &F = BATCH ? tab[f] : P0;
+        // the items of tiles the cone rays found empty take no proposal (the
+        // march makes them background): a wave whose 64 items all lie in
+        // such tiles skips the network
+        const bool empty = !in_q || (cone && cone[q >> P0.tn_shift].t > F.zend);
+        if (__all(empty)) {
+            if (in_q) prop[q] = make_float2(0.0f, -1e30f);
+            continue;
+        }
         if (in_q && item_pixel(F, (int)(q - F.q_begin), x, yr)) {
             double cc[3], vv[3];
             pixel_ray_tab(F, rtab, x, yr, cc, vv);
@@ -393,10 +414,102 @@ __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__rest
                 d[j] = (float)(vv[j] * il);
             }
         }
+        // the network sees the ray as its point of closest approach to the
+        // origin and its direction, and answers the hit's distance past that
+        // point (a camera-independent form): back to the distance from the
+        // start point
+        const float s = -(o[0] * d[0] + o[1] * d[1] + o[2] * d[2]);
+        const float pc[3] = {o[0] + s * d[0], o[1] + s * d[1], o[2] + s * d[2]};
         float t = 0.0f, lg = 0.0f;
-        mlp_wave_infer(M, wl, o, d, lds[threadIdx.x >> 6], t, lg);
-        if (in_q) prop[q] = make_float2(t, lg);
+        mlp_wave_infer(M, wl, pc, d, lds[threadIdx.x >> 6], t, lg);
+        if (in_q) prop[q] = make_float2(t + 0.25f * s, lg);
     }
+}
+
+// The cone rays (cone_march, src/calc.rs:2357-2435, and its grid,
+// src/cone.rs:1-31), one lane per tile of the queue: the tile's coarse ray
+// from its middle, marched in f64 from the start plane while the distance
+// estimate keeps every ray of the tile -- within the spread a + slope t of
+// the coarse ray (the widest gaps of the tile's starts and directions, in
+// step widths), times the march's step factor margin -- clear of the DE
+// stop.  Its depth is a safe start for the tile's rays; a cone that passes
+// the end of the view makes the tile background (within the march's own
+// assumption that the estimate is a bound).  One level of blocks: the
+// queue's tile.
+constexpr double kConeMinStep = 0.05, kConeMinShare = 0.1, kConeMargin = 1.5;  // src/calc.rs:2659-2674
+
+template <int KIND, int DEM, bool BATCH>
+__global__ __launch_bounds__(256) void k_cone(DevP P0, const DevP *__restrict__ gtab, int nframes,
+                                              const double2 *__restrict__ rtab, ConeRes *__restrict__ out,
+                                              unsigned long long *__restrict__ ctr) {
+    const GlobalFrames tab{reinterpret_cast<const char *>(gtab)};
+    Cnt cnt;
+    const long long ntiles = P0.nslots_q >> P0.tn_shift;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < ntiles; g += stride) {
+        const long long q = g << P0.tn_shift;
+        const int fr = BATCH ? find_frame(tab, nframes, q) : 0;
+        const DevP &P = BATCH ? tab[fr] : P0;
+        int tx;
+        const int ty = div_rem((int)((q - P.q_begin) >> P.tn_shift), P.tiles_x, tx);
+        const int x0 = tx << P.tw_shift, y0 = ty * P.tile_h;  // (y0: a row of the rectangle)
+        const double fy0 = (double)(P.row0 + y0);
+        double c0[3], v0[3];
+        frac_ray(P, (double)x0 + (P.tile_w - 1.0) * 0.5, fy0 + (P.tile_h - 1.0) * 0.5, c0, v0);
+        // the tile's spread from the coarse ray, at the start and per step
+        // width of depth (step widths)
+        double a = 0.0, slope = 0.0;
+        for (int j = 0; j < P.tile_h; j++) {
+            for (int i = 0; i < P.tile_w; i++) {
+                double c[3], v[3];
+                if (x0 + i < P.width && y0 + j < P.rows)
+                    pixel_ray_tab(P, rtab, x0 + i, y0 + j, c, v);
+                else
+                    frac_ray(P, (double)(x0 + i), fy0 + j, c, v);
+                const double dc[3] = {c[0] - c0[0], c[1] - c0[1], c[2] - c0[2]};
+                const double dv[3] = {v[0] - v0[0], v[1] - v0[1], v[2] - v0[2]};
+                a = fmax(a, sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]));
+                slope = fmax(slope, sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]));
+            }
+        }
+        a /= P.step_width;
+        slope /= P.step_width;
+        const double lip = kConeMargin / (double)fminf(fmaxf(P.z_step_div, 0.05f), 1.0f);
+        const double div = 1.0 + slope * lip + (double)P.de_stop * (double)P.de_stop_factor;
+        double last_de = __builtin_inf(), last_step = 0.0, t = 0.0;
+        double c[3] = {c0[0], c0[1], c0[2]};
+        float count = 0.0f;
+        cnt.cone_rays++;
+        for (int guard = 0; guard < 2000000; guard++) {
+            const float ms = (float)((double)P.de_stop * (1.0 + t * (double)P.de_stop_factor));
+            const De64 r = de64<KIND, DEM>(P, c[0], c[1], c[2], P.max_it, 0, ms);
+            cnt.de64++;
+            cnt.p8_64 += r.p8;
+            cnt.box_64 += r.box;
+            if (r.it_result >= r.max_its) break;
+            double de = r.de;
+            if (de > last_de + last_step) de = last_de + last_step;
+            const double rsf =
+                last_step > 0.0 && last_de > de + 1e-30 ? fmin(fmax(last_step / (last_de - de), 0.5), 1.0) : 1.0;
+            const double stop = (double)ms;
+            const double reach = (a + slope * t) * lip + stop;
+            const double cap = (double)(fmaxf(ms, 0.4f) * P.mh04zsd);
+            const double sd = fmin((de - reach - (double)P.ms_de_sub * stop) * (double)P.z_step_div * rsf / div, cap);
+            const double d = fmax(0.11, (de - (double)P.ms_de_sub * stop) * (double)P.z_step_div * rsf);
+            if (sd < kConeMinStep * stop || sd < kConeMinShare * fmin(d, cap)) break;
+            if (P.dfog_on_it == 0 || r.it_result == P.dfog_on_it) count += (float)(fmin(sd, P.zend - t) / d);
+            if (t + sd > P.zend) {
+                t = __builtin_inf();
+                break;
+            }
+            t += sd;
+            for (int k = 0; k < 3; k++) c[k] += v0[k] * sd;
+            last_de = de;
+            last_step = sd;
+        }
+        out[g] = ConeRes{t, count, 0};
+    }
+    flush_counters(cnt, ctr);
 }
 
 // BATCH: the frames of a batch in one launch; `tab` holds each frame's
@@ -424,6 +537,8 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
     int nsteps = 0;
     bool start = false, first = false, jumped = false;
     float jrem = 0.0f;  // a jump's predicted distance left to the surface (march units)
+    double cone_t = 0.0;  // the lane's tile's cone depth (0: none)
+    float cone_cnt = 0.0f;
     int32_t seed = 0;
     const float fguard = P0.guard;
     WaveChunk wq, cq;
@@ -467,7 +582,15 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
                     jumped = false;
                     first = F.first_step_random != 0;
                     seed = first ? dev_pixel_seed(F.row0 + py, px) : 0;
-                    if constexpr (LEARN) {
+                    cone_t = 0.0;
+                    if (P0.cone_mode) {
+                        const ConeRes cr = B.cone[q >> P0.tn_shift];
+                        cone_t = cr.t;
+                        cone_cnt = cr.count;
+                    }
+                    // (a tile its cone ray found empty takes no network jump:
+                    // its rays' first DE is at the start plane)
+                    if (LEARN && !(cone_t > F.zend)) {
                         const float2 pr = B.prop[q];
                         const float ph = 1.0f / (1.0f + __expf(-pr.y));
                         double jump = 0.0;
@@ -509,13 +632,15 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
             // finish re-decides everything near the surface); the depth and
             // the position accumulate in f64
             bool top = true;
-            if (LEARN && jumped) {
-                // the warm start's check: a free start, clear of the
-                // hand-off guard and of a tenth of the distance the network
-                // left to the surface, is kept (a DE far under that distance
-                // means a surface nearer than predicted: one in front, or one
-                // the jump passed through); any other goes back to the start
-                // plane and marches plainly
+            if (jumped) {
+                // the warm start's check (a network jump, or a jump to the
+                // cone's depth): a free start, clear of the hand-off guard and
+                // of a tenth of the distance the network left to the surface,
+                // is kept (a DE far under that distance means a surface
+                // nearer than predicted: one in front, or one the jump passed
+                // through); any other goes back to the start plane and marches
+                // plainly (as march_pixel does when the cone's jump lands in
+                // the set, src/calc.rs:2207-2229, 2305-2319)
                 jumped = false;
                 if (!(d32 < __builtin_inff()) || it >= P.max_it || !(d32 >= ms * fguard) ||
                     !(d32 >= 0.1f * jrem)) {
@@ -523,6 +648,8 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
                     pixel_ray_tab(P, B.rtab, px, py, c, v);
                     mzz = 0.0;
                     ms = P.de_stop;
+                    scount = 0.0f;
+                    cone_t = 0.0;
                     top = false;
                 } else {
                     cnt.jumps++;
@@ -536,6 +663,28 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
             } else if (start) {
                 if (it >= P.max_it || !(d32 >= ms * fguard)) {
                     handoff = true;
+                    top = false;
+                } else if (cone_t > P.zend) {
+                    // the tile's cone ray passed the end of the view: the
+                    // background, with the dynamic fog's count of the way
+                    // (march_pixel_from, src/calc.rs:2299-2305)
+                    background = true;
+                    scount = cone_cnt;
+                    cnt.cone_bg++;
+                    top = false;
+                } else if (P.cone_mode == 2 && cone_t > mzz) {
+                    // the cone's depth: the march goes on from there, checked
+                    // as a jump (src/calc.rs:2306-2318)
+                    const double dj = cone_t - mzz;
+                    mzz = cone_t;
+                    c[0] += v[0] * dj;
+                    c[1] += v[1] * dj;
+                    c[2] += v[2] * dj;
+                    ms = P.de_stop * (1.0f + (float)mzz * P.de_stop_factor);
+                    scount = cone_cnt;
+                    jumped = true;
+                    jrem = 0.0f;
+                    cone_t = 0.0;
                     top = false;
                 } else {
                     dtmp = d32;

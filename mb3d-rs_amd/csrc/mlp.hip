@@ -504,7 +504,7 @@ void mlp_dev(const mb3d_hip_mlp *h, const mb3d_hip_options &o, MlpDev &M) {
     // jumps (unverified) only with a positive threshold
     M.conf_hit = o.mlp_conf_hit != 0.0f ? o.mlp_conf_hit : 0.5f;
     M.conf_miss = o.mlp_conf_miss > 0.0f ? o.mlp_conf_miss : -1.0f;
-    M.margin_rel = o.mlp_margin_rel > 0.0f ? o.mlp_margin_rel : 0.08f;
+    M.margin_rel = o.mlp_margin_rel > 0.0f ? o.mlp_margin_rel : 0.15f;
     M.margin_abs = o.mlp_margin_abs;
     M.enabled = 1;
 }

@@ -255,6 +255,10 @@ class Counters(C.Structure):
         ("march_f32_ms", C.c_double),
         ("finish_f64_ms", C.c_double),
         ("paint_ms", C.c_double),
+        ("cone_rays", C.c_uint64),
+        ("cone_background", C.c_uint64),
+        ("mlp_ms", C.c_double),
+        ("cone_ms", C.c_double),
     ]
 
 
@@ -851,7 +855,7 @@ def _p(a):
 def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=16, guard_band=0.25, it_split=0, mlp=None,
             conf_hit=0.0, conf_miss=0.0, margin_rel=0.0, margin_abs=0.0, refill_min=0, cone=False) -> Options:
     """Options; `mlp` (an Mlp) turns the warm start on: hit jumps with the
-    library's defaults (conf_hit 0: 0.5, above 1: off; margin_rel 0: 0.08),
+    library's defaults (conf_hit 0: 0.5, above 1: off; margin_rel 0: 0.15),
     miss jumps (unverified) only with conf_miss > 0; `cone` adds the cone
     ray per tile (a verified safe start, and background for tiles it finds
     empty).  `refill_min`: idle lanes a march wave gathers before refilling
@@ -1345,15 +1349,29 @@ def rays(p: Params, row0=0, rows=None):
     return r, sl
 
 
+def network_rays(r):
+    """The network's view of rays (n x 6: start point, unit direction): the
+    point of closest approach to the scene's origin and the direction, and
+    the distance from the start point to that point.  A camera-independent
+    form: two cameras that share a ray show it to the network alike."""
+    o = np.asarray(r[:, :3], np.float64)
+    d = np.asarray(r[:, 3:6], np.float64)
+    s = -(o * d).sum(1)
+    return np.concatenate([o + s[:, None] * d, d], 1).astype(np.float32), s
+
+
 def training_pairs(p: Params, opts: Options = None):
     """(ray, exact hit distance) pairs of every pixel of `p` from the plain
-    marcher: rays n x 6, targets n x 2 (distance in scene units / 4, hit)."""
+    marcher: rays n x 6 in the network's form (network_rays), targets n x 2
+    (the hit's distance past the point of closest approach, in scene units
+    / 4; hit)."""
     g = march(p, opts=opts, outputs=("depth64",))
     r, sl = rays(p)
     d = g["depth64"]
     hit = np.isfinite(d) & (d > 0)
     t = np.where(hit, d * sl.astype(np.float64), 0.0)
-    return r, np.stack([t / 4.0, hit.astype(np.float64)], 1).astype(np.float32)
+    nr, s = network_rays(r)
+    return nr, np.stack([np.where(hit, (t - s) / 4.0, 0.0), hit.astype(np.float64)], 1).astype(np.float32)
 
 
 def train_warm_start(params_list, steps=3000, batch=8192, lr=3e-3, seed=0, net=None):

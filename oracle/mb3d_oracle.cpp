@@ -1027,9 +1027,94 @@ void march_from(Marcher &m, Px &si, double dtmp, bool first_step, float step_cou
     si.steps = nsteps;
 }
 
-// march_pixel / march_pixel_from without cone, cut planes, inside
-// rendering or ground (src/calc.rs:2207-2322)
-Px march_pixel(Marcher &m, int x, int y) {
+// The cone march (experimental in the reference: src/cone.rs:1-31,
+// src/calc.rs:2357-2435) with one level of blocks, bw x bh pixels: the
+// block's coarse ray from the start plane.
+constexpr double kConeMinStep = 0.05, kConeMinShare = 0.1, kConeMargin = 1.5;  // src/calc.rs:2659-2674
+
+struct ConeCell {
+    double t = 0.0;  // the safe depth (step widths); +inf: the block is background
+    float count = 0.0f;  // the dynamic fog's step count up to there
+};
+
+// Marcher::ray (src/calc.rs:2328-2355), Common optic: the ray of the
+// picture's point (fx, fy), fractions allowed
+void frac_ray(const mb3d_hip_params &p, double fx, double fy, double c[3], double v[3]) {
+    double cafy = (fy / (double)p.height - 0.5) * p.fov_y;
+    double cafx = (double)((p.fovx_off - (float)(fx + 1.0)) * p.fovx_mul);
+    double w[3];
+    build_view_vector_dfov(cafy, cafx, w);
+    rot_rev(w, p.vgrads, v);
+    for (int k = 0; k < 3; k++) c[k] = p.ystart[k] + p.vgrads[0][k] * fx + p.vgrads[1][k] * fy;
+}
+
+inline double dist3(const double a[3], const double b[3]) {
+    double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    return std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+
+// cone_march (src/calc.rs:2357-2435) of the block at (bx bw, by bh), t0 = 0
+ConeCell cone_march(Marcher &m, int bx, int by, int bw, int bh) {
+    const mb3d_hip_params &p = m.p;
+    double x0 = (double)(bx * bw), y0 = (double)(by * bh);
+    double c0[3], v0[3];
+    frac_ray(p, x0 + ((double)bw - 1.0) * 0.5, y0 + ((double)bh - 1.0) * 0.5, c0, v0);
+    double a = 0.0, slope = 0.0;
+    for (int j = 0; j < bh; j++)
+        for (int i = 0; i < bw; i++) {
+            double c[3], v[3];
+            frac_ray(p, x0 + (double)i, y0 + (double)j, c, v);
+            a = std::fmax(a, dist3(c, c0));
+            slope = std::fmax(slope, dist3(v, v0));
+        }
+    a /= p.step_width;
+    slope /= p.step_width;
+    for (int k = 0; k < 3; k++) {
+        m.it.c[k] = c0[k];
+        m.vfov[k] = v0[k];
+    }
+    m.mzz = 0.0;
+    m.it.calc_sit = false;
+    m.max_its_result = p.max_it;
+    double lip = kConeMargin / (double)std::fmin(std::fmax(p.z_step_div, 0.05f), 1.0f);
+    double div = 1.0 + slope * lip + (double)p.de_stop * (double)p.de_stop_factor;
+    double last_de = INFINITY, last_step = 0.0, t = 0.0;
+    ConeCell out;
+    float count = 0.0f;
+    for (int g = 0; g < 2000000; g++) {
+        m.update_de_stop();
+        double de = m.calc_de();
+        if (m.it.it_result >= m.max_its_result) break;
+        if (de > last_de + last_step) de = last_de + last_step;
+        double rsf = last_step > 0.0 && last_de > de + 1e-30 ? std::fmin(std::fmax(last_step / (last_de - de), 0.5), 1.0)
+                                                            : 1.0;
+        double stop = (double)m.ms_de_stop;
+        double reach = (a + slope * t) * lip + stop;
+        double cap = (double)(std::fmax(m.ms_de_stop, 0.4f) * p.mh04zsd);
+        double sd = std::fmin((de - reach - (double)p.ms_de_sub * stop) * (double)p.z_step_div * rsf / div, cap);
+        double d = std::fmax(0.11, (de - (double)p.ms_de_sub * stop) * (double)p.z_step_div * rsf);
+        if (sd < kConeMinStep * stop || sd < kConeMinShare * std::fmin(d, cap)) break;
+        if (p.dfog_on_it == 0 || m.it.it_result == p.dfog_on_it) count += (float)(std::fmin(sd, p.zend - t) / d);
+        if (t + sd > p.zend) {
+            t = INFINITY;
+            break;
+        }
+        t += sd;
+        m.mzz = t;
+        m.step(sd);
+        last_de = de;
+        last_step = sd;
+    }
+    out.t = t;
+    out.count = count;
+    return out;
+}
+
+// march_pixel / march_pixel_from without cut planes, inside rendering or
+// ground (src/calc.rs:2207-2322); with a cone cell (`cone`), mode 1: its
+// background verdict, mode 2: also the march from its depth (a landing in
+// the surface marches the pixel from the start)
+Px march_pixel(Marcher &m, int x, int y, const ConeCell *cone = nullptr, int mode = 0) {
     const mb3d_hip_params &p = m.p;
     Px si;
     si.amb_shadow = 5000;
@@ -1052,6 +1137,22 @@ Px march_pixel(Marcher &m, int x, int y) {
         si.depth = 0.0;
         si.iters = (uint16_t)std::max(m.it.it_result, 0);
         return si;
+    }
+    if (cone && cone->t > 0.0) {
+        if (cone->t > p.zend) {  // background, with the count of the way (src/calc.rs:2299-2305)
+            si.shadow = (uint16_t)std::nearbyint(std::fmin(std::fmax(cone->count, 0.0f), 1023.0f));
+            return si;
+        }
+        if (mode == 2) {
+            double d = cone->t;
+            m.mzz = d;
+            m.step(d);
+            m.update_de_stop();
+            dtmp = m.calc_de();
+            if (dtmp < (double)m.ms_de_stop || m.it.it_result >= m.max_its_result) return march_pixel(m, x, y);
+            march_from(m, si, dtmp, first_step, cone->count);
+            return si;
+        }
     }
     march_from(m, si, dtmp, first_step, 0.0f);
     return si;
@@ -2765,8 +2866,61 @@ extern "C" int32_t orc_mc_batch(const mb3d_hip_params *pp, const mb3d_hip_mc_tas
     return 0;
 }
 
+namespace {
+// the cone grid of a rendering (orc_render_cone): blocks of bw x bh pixels
+struct ConeGrid {
+    int bw = 0, bh = 0, cols = 0, mode = 0;
+    std::vector<ConeCell> cells;
+    const ConeCell *at(int x, int y) const { return bw ? &cells[(size_t)(y / bh) * cols + x / bw] : nullptr; }
+};
+
+ConeGrid cone_grid(const mb3d_hip_params &p, int bw, int bh, int mode, int threads) {
+    ConeGrid g;
+    g.bw = bw;
+    g.bh = bh;
+    g.mode = mode;
+    g.cols = (p.width + bw - 1) / bw;
+    const int rows = (p.height + bh - 1) / bh;
+    g.cells.resize((size_t)g.cols * rows);
+    for_rows_interleaved(rows, nthreads(threads), [&](int by) {
+        Marcher m(p);
+        for (int bx = 0; bx < g.cols; bx++) g.cells[(size_t)by * g.cols + bx] = cone_march(m, bx, by, bw, bh);
+    });
+    return g;
+}
+
+int render_impl(const mb3d_hip_params *pp, const mb3d_hip_post_job *post, int32_t row0, int32_t rows,
+                const orc_out *out, int32_t threads, orc_counters *cnt, const ConeGrid *cone);
+}  // namespace
+
 extern "C" int32_t orc_render(const mb3d_hip_params *pp, const mb3d_hip_post_job *post, int32_t row0, int32_t rows,
                               const orc_out *out, int32_t threads, orc_counters *cnt) {
+    return render_impl(pp, post, row0, rows, out, threads, cnt, nullptr);
+}
+
+extern "C" int32_t orc_cone_grid(const mb3d_hip_params *pp, int32_t bw, int32_t bh, double *t, float *count,
+                                 int32_t threads) {
+    if (int e = check_params(pp)) return e;
+    if (bw < 1 || bh < 1) return fail("bad block size");
+    ConeGrid g = cone_grid(*pp, bw, bh, 1, threads);
+    for (size_t i = 0; i < g.cells.size(); i++) {
+        t[i] = g.cells[i].t;
+        count[i] = g.cells[i].count;
+    }
+    return 0;
+}
+
+extern "C" int32_t orc_render_cone(const mb3d_hip_params *pp, int32_t bw, int32_t bh, int32_t mode, int32_t row0,
+                                   int32_t rows, const orc_out *out, int32_t threads, orc_counters *cnt) {
+    if (int e = check_params(pp)) return e;
+    if (bw < 1 || bh < 1 || mode < 1 || mode > 2) return fail("bad cone block or mode");
+    ConeGrid g = cone_grid(*pp, bw, bh, mode, threads);
+    return render_impl(pp, nullptr, row0, rows, out, threads, cnt, &g);
+}
+
+namespace {
+int render_impl(const mb3d_hip_params *pp, const mb3d_hip_post_job *post, int32_t row0, int32_t rows,
+                const orc_out *out, int32_t threads, orc_counters *cnt, const ConeGrid *cone) {
     if (int e = check_params(pp)) return e;
     if (!out) return fail("out is NULL");
     const mb3d_hip_params &p = *pp;
@@ -2780,7 +2934,7 @@ extern "C" int32_t orc_render(const mb3d_hip_params *pp, const mb3d_hip_post_job
         uint64_t hits = 0, steps = 0;
         for (int x = 0; x < w; x++) {
             m.seed = orc_pixel_seed(y, x);
-            Px s = march_pixel(m, x, y);
+            Px s = cone ? march_pixel(m, x, y, cone->at(x, y), cone->mode) : march_pixel(m, x, y);
             hits += std::isfinite(s.depth) && s.depth > 0 ? 1 : 0;
             steps += (uint64_t)s.steps;
             px[(size_t)r * w + x] = s;
@@ -2813,6 +2967,8 @@ extern "C" int32_t orc_render(const mb3d_hip_params *pp, const mb3d_hip_post_job
     }
     return 0;
 }
+
+}  // namespace
 
 extern "C" int32_t orc_post(const mb3d_hip_params *pp, const mb3d_hip_post_job *post, int32_t row0, int32_t rows,
                             uint32_t *gbuf, int32_t threads) {

@@ -63,6 +63,17 @@ void orc_pixel_ray(const mb3d_hip_params *p, int32_t x, int32_t y, double c[3], 
 int32_t orc_render(const mb3d_hip_params *p, const mb3d_hip_post_job *post, int32_t row0,
                    int32_t rows, const orc_out *out, int32_t threads, orc_counters *cnt);
 
+/* The cone march (src/cone.rs, src/calc.rs:2357-2435) with one level of
+ * blocks of bw x bh pixels: each block's safe depth (+inf: background) and
+ * the dynamic fog's count up to it, block rows of (width + bw - 1) / bw. */
+int32_t orc_cone_grid(const mb3d_hip_params *p, int32_t bw, int32_t bh, double *t, float *count,
+                      int32_t threads);
+
+/* orc_render with the cone grid: mode 1, a block whose cone passed the end
+ * is background; mode 2, also every pixel's march from its block's depth. */
+int32_t orc_render_cone(const mb3d_hip_params *p, int32_t bw, int32_t bh, int32_t mode, int32_t row0,
+                        int32_t rows, const orc_out *out, int32_t threads, orc_counters *cnt);
+
 /* The post job alone on a G-buffer (5 words per pixel, in and out). */
 int32_t orc_post(const mb3d_hip_params *p, const mb3d_hip_post_job *post, int32_t row0,
                  int32_t rows, uint32_t *gbuf, int32_t threads);

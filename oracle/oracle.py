@@ -107,8 +107,11 @@ def params(scene) -> "H.Params":
     return p
 
 
-def render(p, post=None, row0=0, rows=None, threads=0):
-    """The oracle's G-buffer of rows [row0, row0+rows): dict of arrays."""
+def render(p, post=None, row0=0, rows=None, threads=0, cone=None):
+    """The oracle's G-buffer of rows [row0, row0+rows): dict of arrays.
+    `cone` = (block width, block height, mode): the march with the cone grid
+    (mode 1: blocks whose cone passed the end are background; 2: also every
+    pixel's march from its block's depth); no post job then."""
     rows = p.height - row0 if rows is None else rows
     n = rows * p.width
     out = {
@@ -120,12 +123,29 @@ def render(p, post=None, row0=0, rows=None, threads=0):
     }
     o = OrcOut(*(_ptr(out[k]) for k in ("depth", "iters", "normals", "gbuf", "steps")))
     cnt = OrcCounters()
-    r = load().orc_render(C.byref(p), C.byref(post) if post is not None else None, row0, rows,
-                          C.byref(o), threads, C.byref(cnt))
+    if cone is not None:
+        assert post is None
+        r = load().orc_render_cone(C.byref(p), cone[0], cone[1], cone[2], row0, rows, C.byref(o), threads,
+                                   C.byref(cnt))
+    else:
+        r = load().orc_render(C.byref(p), C.byref(post) if post is not None else None, row0, rows,
+                              C.byref(o), threads, C.byref(cnt))
     if r != 0:
         raise RuntimeError("oracle render: " + load().orc_last_error().decode())
     out["counters"] = {k: getattr(cnt, k) for k, _ in OrcCounters._fields_}
     return out
+
+
+def cone_grid(p, bw, bh, threads=0):
+    """The cone march's blocks (bw x bh pixels, one level): safe depth per
+    block (+inf: background) and the dynamic fog's count, as (rows, cols)."""
+    cols, rows = (p.width + bw - 1) // bw, (p.height + bh - 1) // bh
+    t = np.zeros(cols * rows)
+    cnt = np.zeros(cols * rows, np.float32)
+    r = load().orc_cone_grid(C.byref(p), bw, bh, _ptr(t), _ptr(cnt), threads)
+    if r != 0:
+        raise RuntimeError("oracle cone grid: " + load().orc_last_error().decode())
+    return t.reshape(rows, cols), cnt.reshape(rows, cols)
 
 
 def post(p, job, gbuf, row0=0, rows=None, threads=0):

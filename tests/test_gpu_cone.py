@@ -1,0 +1,86 @@
+"""The cone rays on the device (k_cone, the cone march of src/cone.rs and
+src/calc.rs:2357-2435 with one level of blocks, the queue's tiles) against
+the oracle's cone march and the plain march."""
+import numpy as np
+import pytest
+
+import mb3d_hip as H
+import oracle as O
+import parity
+from test_gpu_batch import device_out, host
+
+pytestmark = pytest.mark.gpu
+
+
+def scenes():
+    return [("bulb", H.scene_bulb(240, 136, seed=4)), ("bulb_numeric_f32", H.scene_bulb(240, 136, seed=5,
+                                                                                        de_mode=H.DE_REFERENCE)),
+            ("box", H.scene_box(240, 136, seed=3, step_cap=500)), ("hybrid", H.scene_hybrid(240, 136, seed=2))]
+
+
+@pytest.mark.parametrize("i", range(4))
+@pytest.mark.parametrize("tile", [(8, 8), (16, 8)])
+def test_cone_background_leaves_the_plain_picture(hip, i, tile):
+    """Mode 1 (a tile whose cone ray passes the end is background): depths,
+    iteration counts and normals equal the plain march's on the device; the
+    background pixels' step counts are the oracle's cone counts, so the
+    device's cone rays found the same empty tiles with the same fog counts."""
+    name, sc = scenes()[i]
+    p = H.params_from_scene(sc)
+    # (the numeric DE's scenes take the FP64 march automatically, which has
+    # no cone pass: the FP32 iterate is asked for)
+    prec = H.PREC_F32 if name in ("bulb_numeric_f32", "hybrid") else H.PREC_AUTO
+    plain = hip.march(p, opts=H.options(precision=prec, tile=tile))
+    cone = hip.march(p, opts=H.options(precision=prec, tile=tile, cone=True))
+    c = H.counters()
+    for k in ("depth", "depth64", "iters", "normals"):
+        assert np.array_equal(cone[k], plain[k]), (name, k)
+    ref = O.render(O.params(sc), threads=0, cone=(tile[0], tile[1], 1))
+    bg = ~np.isfinite(ref["depth"])
+    if name != "box":
+        assert c["cone_background"] > 1000, c["cone_background"]
+    assert c["cone_rays"] == ((p.width + tile[0] - 1) // tile[0]) * ((p.height + tile[1] - 1) // tile[1])
+    # the step counts of the background (the cone's counts where the cone
+    # decided, the march's elsewhere) against the oracle's
+    steps_d = (cone["gbuf"][bg, 1] >> 16) & 0x3FF
+    steps_o = (ref["gbuf"][bg, 1] >> 16) & 0x3FF
+    assert (np.abs(steps_d.astype(int) - steps_o.astype(int)) <= 1).mean() >= 0.999
+
+
+def test_cone_jumps_meet_the_bar(hip):
+    """Mode 2 (the march starts at the tile's cone depth): against the
+    oracle's march with the same cone grid, the plain bar."""
+    sc = H.scene_bulb(320, 180, seed=6)
+    p = H.params_from_scene(sc)
+    got = hip.march(p, opts=_opts_mode2())
+    c = H.counters()
+    ref = O.render(O.params(sc), threads=0, cone=(8, 8, 2))
+    m = parity.compare(got, ref, p.width, p.height, p.step_width, 2.0)
+    assert c["jumps"] > 1000, c
+    assert m["mask_agree"] >= 0.999 and m["depth_ok_frac"] >= 0.999, m
+
+
+def _opts_mode2():
+    o = H.options(cone=True)
+    o.mlp_cone = 2
+    return o
+
+
+def test_batch_with_cones_equals_single_frames(hip):
+    scenes_ = H.animation(5, seed=7, sizes=((96, 72), (160, 120), (200, 150)))
+    arr = H.params_array(scenes_)
+    n = sum(s.width * s.height for s in scenes_)
+    bufs, out = device_out(n)
+    H.check(hip.load().mb3d_hip_march_batch(arr, len(arr), H.C.byref(out), H.C.byref(H.options(cone=True))), "batch")
+    cb = H.counters()
+    got = host(bufs)
+    off = 0
+    bg = 0
+    for i, s in enumerate(scenes_):
+        m = s.width * s.height
+        one = hip.march(arr[i], opts=H.options(cone=True))
+        bg += H.counters()["cone_background"]
+        assert np.array_equal(got["depth"][off:off + m], one["depth"]), i
+        assert np.array_equal(got["gbuf"][off * 5:(off + m) * 5].view(np.uint32), one["gbuf"].ravel()), i
+        off += m
+    assert cb["cone_background"] == bg > 0
