@@ -435,6 +435,20 @@ int32_t mb3d_hip_march(const mb3d_hip_params *params, const mb3d_hip_post_job *p
                        const mb3d_hip_options *options, mb3d_hip_progress_fn progress,
                        mb3d_hip_cancel_fn cancel, void *user);
 
+/* No counterpart in the reference: mb3d_hip_march for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It marches rows [row0, row0 +
+ * rows) into `out`, which must be device memory (out->on_device = 1), with
+ * no post job and no network (options->mlp NULL); its lists, tables and
+ * counters live in a 256 MiB scratch array of the library's own device
+ * memory, held per device until the process ends.  A call needs about
+ * 180 bytes a pixel and rejects more than the array holds (about 1.3 M
+ * pixels at 256 CUs: a 1080p picture marches in two bands of rows) with
+ * MB3D_HIP_ERR_INVALID.  Calls that overlap on one device race on the
+ * array. */
+int32_t mb3d_hip_march_noalloc(const mb3d_hip_params *params, int32_t row0, int32_t rows,
+                               const mb3d_hip_out *out, const mb3d_hip_options *options);
+
 /* gpu::post_on_card (src/gpu.rs:1649): the shadow and DE-AO passes on a
  * G-buffer of rows [row0, row0 + rows) marched elsewhere (5 words per
  * pixel, in and out). */

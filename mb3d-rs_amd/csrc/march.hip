@@ -91,8 +91,18 @@ struct Ctx {
     void *d_work = nullptr;
     size_t work_bytes = 0;
     unsigned long long *d_ctr = nullptr;
+    // the device address of `scratch` (hipGetSymbolAddress at init)
+    void *scratch = nullptr;
 };
 Ctx g_ctx;
+
+// The no-allocation march's scratch (mb3d_hip_march_noalloc): its hand-off
+// list, surface records, hit list, ray tables, cone results and counters,
+// carved per call.  Each device that runs it holds the array until the
+// process ends; host code uses its device address (Ctx::scratch), never
+// this name.  Calls that overlap on one device race on it.
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 int set_error(int code, const char *msg) {
     t_error = msg;
@@ -598,6 +608,7 @@ int init_locked(int device) {
     HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     for (auto &e : c.ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipMalloc((void **)&c.d_ctr, C_WORDS * sizeof(unsigned long long)));
+    HIP_TRY(hipGetSymbolAddress(&c.scratch, HIP_SYMBOL(scratch)));
     c.device = device;
     return 0;
 }
@@ -618,9 +629,12 @@ OutLayout layout_for(size_t npix) {
     return l;
 }
 
+// `noalloc`: the march takes its buffers from the scratch array (no device
+// allocation; device outputs, no post job, no network)
 int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, int row0, int rows,
                  const mb3d_hip_out *out, const mb3d_hip_options *opt_in, mb3d_hip_progress_fn progress,
-                 mb3d_hip_cancel_fn cancel, void *user, uint32_t *post_gbuf, int post_on_device) {
+                 mb3d_hip_cancel_fn cancel, void *user, uint32_t *post_gbuf, int post_on_device,
+                 bool noalloc = false) {
     Ctx &c = g_ctx;
     if (c.device < 0) return set_error(MB3D_HIP_ERR_NOT_INIT, "mb3d_hip_init was not called");
     if (!params) return set_error(MB3D_HIP_ERR_INVALID, "params is NULL");
@@ -649,6 +663,10 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     // buffers: outputs on the device (the caller's or ours), hand-off list, hit list
     const OutLayout L = layout_for(npix);
     const bool dev_out = post_only ? post_on_device != 0 : (out && out->on_device);
+    if (noalloc && (!dev_out || post || post_only))
+        return set_error(MB3D_HIP_ERR_INVALID, "the no-allocation march writes device outputs (out->on_device = 1), "
+                                               "without a post job");
+    if (noalloc && o.mlp) return set_error(MB3D_HIP_ERR_INVALID, "the no-allocation march takes no network");
     if (!dev_out) {
         if (int e = ensure(&c.d_out, &c.out_bytes, L.total)) return e;
     }
@@ -663,9 +681,26 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     const bool cone = d.cone_mode != 0 && !f64 && !post_only;
     const size_t cone_bytes = cone ? (size_t)(d.nslots_q >> d.tn_shift) * sizeof(ConeRes) : 0;
     const size_t work_need = base_need + learn_need + rtab_bytes + cone_bytes;
-    if (int e = ensure(&c.d_work, &c.work_bytes, work_need)) return e;
-    void *work = c.d_work;
+    void *work = nullptr;
     unsigned long long *ctr_dev = c.d_ctr;
+    if (noalloc) {
+        const size_t ctr_off = (work_need + 255) & ~(size_t)255;
+        if (ctr_off + C_WORDS * sizeof(unsigned long long) > kScratchBytes) {
+            char msg[200];
+            std::snprintf(msg, sizeof msg,
+                          "the no-allocation march's scratch (%zu MiB) holds about %lld pixels a call: march the "
+                          "picture in bands of rows",
+                          kScratchBytes >> 20,
+                          (long long)((kScratchBytes - 64 * (size_t)c.cus * 32 * (sizeof(Cand) + sizeof(SurfRec) + 4)) /
+                                      (sizeof(Cand) + sizeof(SurfRec) + 4 + 32)));
+            return set_error(MB3D_HIP_ERR_INVALID, msg);
+        }
+        work = c.scratch;
+        ctr_dev = (unsigned long long *)((char *)c.scratch + ctr_off);
+    } else {
+        if (int e = ensure(&c.d_work, &c.work_bytes, work_need)) return e;
+        work = c.d_work;
+    }
     Bufs b;
     std::memset(&b, 0, sizeof b);
     char *ob = (char *)c.d_out;
@@ -698,7 +733,10 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     b.cone = cone ? (ConeRes *)((char *)rtab + rtab_bytes) : nullptr;
     if (!cone) d.cone_mode = 0;
 
-    HIP_TRY(hipMemsetAsync(ctr_dev, 0, C_WORDS * sizeof(unsigned long long), c.stream));
+    if (noalloc)
+        hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, c.stream, ctr_dev, (int)C_WORDS);  // (no memset node)
+    else
+        HIP_TRY(hipMemsetAsync(ctr_dev, 0, C_WORDS * sizeof(unsigned long long), c.stream));
     if (post_only && !dev_out)
         HIP_TRY(hipMemcpyAsync(b.gbuf, post_gbuf, npix * 20, hipMemcpyHostToDevice, c.stream));
     const int kind = kind_of(p), dem = dem_of(p);
@@ -1143,6 +1181,17 @@ extern "C" int32_t mb3d_hip_march(const mb3d_hip_params *params, const mb3d_hip_
         return march_locked(params, post, row0, rows, out, options, progress, cancel, user, nullptr, 0);
     } catch (...) {
         return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_march");
+    }
+}
+
+extern "C" int32_t mb3d_hip_march_noalloc(const mb3d_hip_params *params, int32_t row0, int32_t rows,
+                                          const mb3d_hip_out *out, const mb3d_hip_options *options) {
+    try {
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        if (!out) return set_error(MB3D_HIP_ERR_INVALID, "out is NULL");
+        return march_locked(params, nullptr, row0, rows, out, options, nullptr, nullptr, nullptr, nullptr, 0, true);
+    } catch (...) {
+        return set_error(MB3D_HIP_ERR_DEVICE, "unexpected failure in mb3d_hip_march_noalloc");
     }
 }
 

@@ -460,6 +460,8 @@ SIGNATURES = {
         [C.POINTER(Params), C.POINTER(PostJob), C.c_int32, C.c_int32, C.POINTER(Out),
          C.POINTER(Options), PROGRESS_FN, CANCEL_FN, C.c_void_p],
     ),
+    "mb3d_hip_march_noalloc": (C.c_int32, [C.POINTER(Params), C.c_int32, C.c_int32, C.POINTER(Out),
+                                           C.POINTER(Options)]),
     "mb3d_hip_post": (
         C.c_int32,
         [C.POINTER(Params), C.POINTER(PostJob), C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
