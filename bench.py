@@ -226,7 +226,8 @@ def main():
         {"kernel": "k_march32 (batch)", "bound": "valu-fp32", "kernel_ms": t_k["march"],
          "flops_per_launch": flops_f32(cf), "peak": PEAK_FP32 / 1e12},
         {"kernel": "k_mlp_propose (batch)", "bound": "mfma-bf16", "kernel_ms": t_k["mlp"],
-         "flops_per_launch": F_MLP * items, "peak": PEAK_BF16 / 1e12},
+         "flops_per_launch": F_MLP * cf["mlp_rays"], "peak": PEAK_BF16 / 1e12, "rays": cf["mlp_rays"],
+         "items": items},
         {"kernel": "k_cone + k_finish64 + k_normals (batch)", "bound": "valu-fp64",
          "kernel_ms": t_k["cone"] + t_k["finish"], "flops_per_launch": flops_f64(cf), "peak": PEAK_FP64 / 1e12},
     ]

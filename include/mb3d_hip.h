@@ -119,7 +119,8 @@ typedef struct mb3d_hip_scene {
     int32_t decomb_iterations2; /* the second part's maximum iterations */
     float decomb_smooth;        /* the smooth minima's distance (scene units) */
     int32_t part2_placed;       /* 0/1: the second part placed by part2_* (Scene::part2_place) */
-    int32_t reserved;
+    int32_t color_on_it;        /* the colour from the orbit after color_on_it - 1 iterations
+                                   (1: the start point; 0: the last DE's orbit), Scene::color_on_it */
     double rstop;              /* bailout radius (not squared); <= 0: the formulas' default */
     double zoom;
     double mid[3];
@@ -170,7 +171,7 @@ typedef struct mb3d_hip_params {
     int32_t start2, end2, repeat2, max_it2;
     int32_t de_option2, is_custom_de2, four_d2;
     int32_t placed;          /* the second part at place_* (Place, src/calc.rs:238-253) */
-    int32_t reserved;
+    int32_t color_on_it;     /* CalcParams::color_on_it (do_color_on_it, src/calc.rs:2010-2025) */
     double slot_z_mul[MB3D_HIP_MAX_SLOTS];
     double slot_scale[MB3D_HIP_MAX_SLOTS];
     double slot_min_r[MB3D_HIP_MAX_SLOTS];
@@ -386,6 +387,7 @@ typedef struct mb3d_hip_counters {
     uint64_t cone_background; /* pixels made background by their tile's cone ray */
     double mlp_ms;           /* of kernel_ms: the warm start's network (k_mlp_propose) */
     double cone_ms;          /* of kernel_ms: the cone rays (k_cone) */
+    uint64_t mlp_rays;       /* rays the network ran on (tiles the cone rays found empty skip it) */
 } mb3d_hip_counters;
 
 typedef void (*mb3d_hip_progress_fn)(void *user, int64_t done, int64_t total);

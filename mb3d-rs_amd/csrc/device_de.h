@@ -76,6 +76,12 @@ struct DevP {
     // the cone rays (k_cone): 0 off, 1 a tile whose cone ray passes the end
     // of the view is background, 2 also start the tile's rays at its depth
     int cone_mode;
+    // the surface's normal and colour modes (surface_found,
+    // src/calc.rs:2471-2517): smooth normals 0..8, normals on the DE or on
+    // the smooth iteration count, the colour option and color_on_it
+    int sm_normals, normals_on_de, color_option, color_on_it;
+    float mct_color_mul;
+    int pad_core2;
 
     // ---- the general kind (K_GEN) only ----
     double opt2[MB3D_HIP_MAX_SLOTS][3];
@@ -718,6 +724,7 @@ __device__ __forceinline__ void start64(const DevP &P, bool four, It64 &s, doubl
 // The result of one f64 DE evaluation.
 struct De64 {
     double de, rout, otrap;
+    double v[3], rold;  // the orbit's end and the radius before it (do_color)
     int it_result;
     int max_its;       // max_its_result: the ruling part's iteration limit
     int max_it_after;  // the iteration limit the evaluation leaves (set_part)
@@ -735,7 +742,10 @@ struct De64 {
 // part's own flag decides.
 template <int KIND, int DEM = DEM_NUMERIC>
 __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, double cx, double cy, double cz, int max_it,
-                                       int calc_sit, float ms) {
+                                       int calc_sit, float ms, double rstop = -1.0) {
+    // (rstop: the bailout of the evaluation, P.d_rstop unless given;
+    // do_color_on_it asks for rstop3d)
+    const double rs = rstop >= 0.0 ? rstop : P.d_rstop;
     De64 o;
     It64 s;
     s.p8 = s.box = s.oth = 0;
@@ -751,7 +761,7 @@ __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, doubl
             s.v[3] = s.rout;
         else
             s.v[3] = 1.0;
-        const int n = run64<KIND>(P, pt, s, max_it, P.d_rstop);
+        const int n = run64<KIND>(P, pt, s, max_it, rs);
         if ((pt.de_option & 0x38) == 32)
             result = sqrt(s.rout) * 0.25 * log(s.rout) / s.deriv1;
         else
@@ -762,7 +772,7 @@ __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, doubl
         o.rout = s.rout;
         o.otrap = s.otrap;
     } else {
-        const int n = run64<KIND>(P, pt, s, max_it, P.d_rstop);
+        const int n = run64<KIND>(P, pt, s, max_it, rs);
         if (calc_sit) o.smooth_it = smooth64(P, s, n);
         o.it_result = s.it_result;
         o.rout = s.rout;
@@ -786,6 +796,10 @@ __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, doubl
     const double mn = (double)ms * 0.25;
     if (result < mn || isnan(result)) result = mn;
     o.de = result;
+    o.v[0] = s.v[0];
+    o.v[1] = s.v[1];
+    o.v[2] = s.v[2];
+    o.rold = s.rold;
     o.max_its = max_it;
     o.max_it_after = max_it;
     o.p8 = s.p8;
@@ -863,9 +877,39 @@ __device__ __forceinline__ De64 de64_comb(const DevP &P, double cx, double cy, d
 
 template <int KIND, int DEM = DEM_NUMERIC>
 __device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double cz, int max_it, int calc_sit,
-                                  float ms) {
+                                  float ms, double rstop = -1.0) {
     if (KIND == K_GEN && P.decomb) return de64_comb(P, cx, cy, cz, max_it, calc_sit, ms);
-    return de64_part<KIND, KIND == K_GEN ? DEM_NUMERIC : DEM>(P, part1_of<KIND>(P), cx, cy, cz, max_it, calc_sit, ms);
+    return de64_part<KIND, KIND == K_GEN ? DEM_NUMERIC : DEM>(P, part1_of<KIND>(P), cx, cy, cz, max_it, calc_sit, ms,
+                                                              rstop);
+}
+
+// mand_function (src/calc.rs:1277-1285, Iteration::hybrid_3d,
+// src/iteration.rs:521-535): the plain iteration at c, no DE, with the
+// smooth iteration count; the result in a De64 (its `de` unset)
+template <int KIND>
+__device__ __forceinline__ De64 mand64(const DevP &P, double cx, double cy, double cz, int max_it) {
+    De64 o;
+    It64 s;
+    s.p8 = s.box = s.oth = 0;
+    s.deriv1 = 1.0;
+    s.rold = 0.0;
+    const Part64 pt = part1_of<KIND>(P);
+    start64<KIND>(P, pt.four, s, cx, cy, cz);
+    const int n = run64<KIND>(P, pt, s, max_it, P.d_rstop);
+    o.smooth_it = smooth64(P, s, n);
+    o.de = 0.0;
+    o.rout = s.rout;
+    o.otrap = s.otrap;
+    o.v[0] = s.v[0];
+    o.v[1] = s.v[1];
+    o.v[2] = s.v[2];
+    o.rold = s.rold;
+    o.it_result = s.it_result;
+    o.max_its = o.max_it_after = max_it;
+    o.p8 = s.p8;
+    o.box = s.box;
+    o.oth = s.oth;
+    return o;
 }
 
 // The reference's Marcher (src/calc.rs:1143-1170) for one ray in f64: the

@@ -126,6 +126,7 @@ mb3d_hip_counters &thread_counters() { return t_counters; }
     } while (0)
 
 int kind_of(const mb3d_hip_params &p);
+bool ext_modes(const mb3d_hip_params &p);
 
 std::string unsupported_reason(const mb3d_hip_params *p) {
     if (!p) return "no parameters";
@@ -153,9 +154,21 @@ std::string unsupported_reason(const mb3d_hip_params *p) {
     if ((p->four_d != 0) != (p->de_option >= 4 && p->de_option <= 6)) return "the 4D mode does not fit the DE option";
     if (p->de_option == 32 && kind_of(*p) == K_GEN)
         return "the analytic power-8 DE with another power or in Julia mode (not on this backend)";
-    if (p->sm_normals != 0) return "smooth normals (not on this backend yet)";
-    if (p->color_option != 0) return "colour option other than 0 (not on this backend yet)";
-    if (!p->normals_on_de) return "normals on the smooth iteration (not on this backend yet)";
+    if (p->decomb && (p->color_option != 0 || !p->normals_on_de || p->color_on_it != 0))
+        return "a colour option, colour on an iteration or normals on the smooth iteration with a DE combination "
+               "(not on this backend)";
+    return "";
+}
+
+// The secondary rays and the Monte Carlo batches keep the plain normal
+// stencil and the orbit-trap colour: scenes with the general surface modes
+// are refused there.
+std::string rays_unsupported_reason(const mb3d_hip_params *p) {
+    std::string why = unsupported_reason(p);
+    if (!why.empty()) return why;
+    if (ext_modes(*p))
+        return "smooth normals, normals on the smooth iteration or colour options in ray batches (not on this "
+               "backend)";
     return "";
 }
 
@@ -325,6 +338,11 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
     d.refill_min = o.refill_min > 0 ? std::min(o.refill_min, 64) : kRefillMin;
     d.rt_base = 0;
     d.cone_mode = std::min(std::max(o.mlp_cone, 0), 2);
+    d.sm_normals = std::min(std::max(p.sm_normals, 0), 8);
+    d.normals_on_de = p.normals_on_de != 0;
+    d.color_option = std::min(std::max(p.color_option, 0), 5);
+    d.color_on_it = std::min(std::max(p.color_on_it, 0), 255);
+    d.mct_color_mul = p.mct_color_mul;
     d.tiles_x = (p.width + d.tile_w - 1) / d.tile_w;
     const long long tiles_y = (rows + d.tile_h - 1) / d.tile_h;
     d.npix = (long long)p.width * rows;
@@ -406,10 +424,10 @@ int dem_of(const mb3d_hip_params &p) {
 
 template <bool BATCH>
 void launch_propose(hipStream_t s, const DevP &d, const DevP *tab, int n, const MlpDev &M, const double2 *rtab,
-                    const ConeRes *cone, float2 *prop) {
+                    const ConeRes *cone, float2 *prop, unsigned long long *ctr) {
     const long long groups = (d.nslots_q + 63) / 64;
     const int grid = (int)std::max<long long>(1, std::min<long long>((groups + 3) / 4, (long long)g_ctx.cus * 16));
-    hipLaunchKernelGGL((k_mlp_propose<BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, M, rtab, cone, prop);
+    hipLaunchKernelGGL((k_mlp_propose<BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, M, rtab, cone, prop, ctr);
 }
 
 template <int KIND, int DEM, bool BATCH>
@@ -449,13 +467,28 @@ void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int al
         hipLaunchKernelGGL((k_finish64<KIND, DEM, BATCH>), dim3(grid), dim3(256), lds, s, d, b, all, tab, n);
     }
 }
-// the normals pass after the finish (N32: FP32 offset DEs, analytic DEs only)
+// the surface modes beyond the plain stencil (k_normals_ext): smooth
+// normals, normals on the smooth iteration, colour options, colour on an
+// iteration
+bool ext_modes(const mb3d_hip_params &p) {
+    return p.sm_normals != 0 || !p.normals_on_de || p.color_option != 0 || p.color_on_it != 0;
+}
+
+// the normals pass after the finish (N32: FP32 offset DEs, analytic DEs
+// only; ext: k_normals_ext for the general surface modes)
 template <int KIND, int DEM, bool BATCH = false>
 void launch_normals(int grid, hipStream_t s, const DevP &d, const Bufs &b, bool f32, const DevP *tab = nullptr,
-                    int n = 1) {
+                    int n = 1, bool ext = false) {
     const size_t lds = BATCH ? (size_t)n * DEVP_CORE : 0;
     if constexpr (KIND == K_GEN) {
-        if (!BATCH) launch_gen_normals(grid, s, d, b, DEM);
+        if (!BATCH) {
+            if (ext)
+                launch_gen_normals_ext(grid, s, d, b, DEM);
+            else
+                launch_gen_normals(grid, s, d, b, DEM);
+        }
+    } else if (ext) {
+        hipLaunchKernelGGL((k_normals_ext<KIND, DEM, BATCH>), dim3(grid), dim3(256), lds, s, d, b, tab, n);
     } else if constexpr (DEM != DEM_NUMERIC) {
         if (f32)
             hipLaunchKernelGGL((k_normals<KIND, DEM, true, BATCH>), dim3(grid), dim3(256), lds, s, d, b, tab, n);
@@ -748,7 +781,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
         if (f64) {
             dispatch(kind, dem, [&](auto K, auto D) {
                 launch_finish<decltype(K)::value, decltype(D)::value>(grid_flat * 2, stream, d, b, 1);
-                launch_normals<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, false);
+                launch_normals<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, false, nullptr, 1,
+                                                                       ext_modes(p));
             });
         } else {
             launch_ray_tab(stream, d, nullptr, 1, rtab);
@@ -760,7 +794,7 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
                 });
             }
             HIP_TRY(hipEventRecord(c.ev[5], c.stream));
-            if (M.enabled) launch_propose<false>(stream, d, nullptr, 1, M, rtab, b.cone, b.prop);
+            if (M.enabled) launch_propose<false>(stream, d, nullptr, 1, M, rtab, b.cone, b.prop, ctr_dev);
             HIP_TRY(hipEventRecord(c.ev[6], c.stream));
             dispatch(kind, dem, [&](auto K, auto D) {
                 launch_march<decltype(K)::value, decltype(D)::value>(grid_march, stream, d, b, M);
@@ -768,7 +802,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
             HIP_TRY(hipEventRecord(c.ev[3], c.stream));
             dispatch(kind, dem, [&](auto K, auto D) {
                 launch_finish<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, 0);
-                launch_normals<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, fp32_stencil_ok(p));
+                launch_normals<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, fp32_stencil_ok(p),
+                                                                       nullptr, 1, ext_modes(p));
             });
         }
         HIP_TRY(hipGetLastError());
@@ -858,6 +893,7 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     t.its_normals_f32 = ctr[C_STATS + 12];
     t.cone_rays = ctr[C_STATS + 13];
     t.cone_background = ctr[C_STATS + 14];
+    t.mlp_rays = ctr[C_STATS + 15];
     t.kernel_ms = ms_march;
     t.post_ms = ms_post;
     t.mlp_ms = split ? ms_mlp : 0.0;
@@ -892,6 +928,7 @@ struct Plan {
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, ea = nullptr, eb = nullptr;
     MlpDev M;  // the warm start (M.enabled = 0: none)
     bool n32 = true;  // the normals' stencil in FP32 (every frame allows it)
+    bool ext = false;  // a frame asks for the general surface modes (k_normals_ext)
 };
 
 void plan_free(Plan *pl) {
@@ -924,7 +961,8 @@ int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
         });
     }
     if (split) HIP_TRY(hipEventRecord(pl.ea, stream));
-    if (pl.M.enabled) launch_propose<true>(stream, pl.P0, pl.d_tab, pl.n, pl.M, pl.b.rtab, pl.b.cone, pl.b.prop);
+    if (pl.M.enabled)
+        launch_propose<true>(stream, pl.P0, pl.d_tab, pl.n, pl.M, pl.b.rtab, pl.b.cone, pl.b.prop, pl.d_ctr);
     if (split) HIP_TRY(hipEventRecord(pl.eb, stream));
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
         launch_march<decltype(K)::value, decltype(D)::value, true>(pl.grid_march, stream, pl.P0, pl.b, pl.M, tab, n);
@@ -933,7 +971,7 @@ int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
         launch_finish<decltype(K)::value, decltype(D)::value, true>(pl.grid_flat, stream, pl.P0, pl.b, 0, tab, n);
         launch_normals<decltype(K)::value, decltype(D)::value, true>(pl.grid_flat, stream, pl.P0, pl.b, pl.n32, tab,
-                                                                     n);
+                                                                     n, pl.ext);
     });
     HIP_TRY(hipGetLastError());
     return 0;
@@ -963,7 +1001,10 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     Plan *pl = new Plan();
     pl->n = n;
     pl->kind = kind;
-    for (int f = 0; f < n; f++) pl->n32 = pl->n32 && fp32_stencil_ok(frames[f]);
+    for (int f = 0; f < n; f++) {
+        pl->n32 = pl->n32 && fp32_stencil_ok(frames[f]);
+        pl->ext = pl->ext || ext_modes(frames[f]);
+    }
     pl->dem = dem;
     pl->tab.resize(n);
     long long q = 0, pix = 0, rt = 0;
@@ -1081,6 +1122,7 @@ int plan_run(Plan &pl, int repeats) {
     t.its_normals_f32 = ctr[C_STATS + 12];
     t.cone_rays = ctr[C_STATS + 13];
     t.cone_background = ctr[C_STATS + 14];
+    t.mlp_rays = ctr[C_STATS + 15];
     t.kernel_ms = ms_all > 0.0f ? ms_all / repeats : -1.0;
     const bool split = ms_a > 0.0f && ms_mlp >= 0.0f && ms_cone >= 0.0f;
     t.mlp_ms = split ? ms_mlp : -1.0;
@@ -1250,7 +1292,7 @@ extern "C" int32_t mb3d_hip_march_rays(const mb3d_hip_params *params, const mb3d
         Ctx &c = g_ctx;
         if (c.device < 0) return set_error(MB3D_HIP_ERR_NOT_INIT, "mb3d_hip_init was not called");
         if (!params || !tasks || !hits || n < 0) return set_error(MB3D_HIP_ERR_INVALID, "bad argument");
-        const std::string why = unsupported_reason(params);
+        const std::string why = rays_unsupported_reason(params);
         if (!why.empty()) return set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
         RayPost post;
         const bool with_post = opts && (opts->n_lights > 0 || opts->ao);
@@ -1320,7 +1362,7 @@ extern "C" int32_t mb3d_hip_mc_batch(const mb3d_hip_params *params, const mb3d_h
         Ctx &c = g_ctx;
         if (c.device < 0) return set_error(MB3D_HIP_ERR_NOT_INIT, "mb3d_hip_init was not called");
         if (!params || !tasks || !res || n < 0) return set_error(MB3D_HIP_ERR_INVALID, "bad argument");
-        const std::string why = unsupported_reason(params);
+        const std::string why = rays_unsupported_reason(params);
         if (!why.empty()) return set_error(MB3D_HIP_UNSUPPORTED, why.c_str());
         if (n == 0) return MB3D_HIP_OK;
         if (!on_device)

@@ -31,6 +31,13 @@ void launch_gen_normals(int grid, hipStream_t s, const DevP &d, const Bufs &b, i
     });
 }
 
+void launch_gen_normals_ext(int grid, hipStream_t s, const DevP &d, const Bufs &b, int dem) {
+    with_dem(dem, [&](auto D) {
+        hipLaunchKernelGGL((k_normals_ext<K_GEN, decltype(D)::value, false>), dim3(grid), dim3(256), 0, s, d, b,
+                           nullptr, 1);
+    });
+}
+
 void launch_gen_shadow(int grid, hipStream_t s, const DevP &d, const Bufs &b, const ShadowJob &j, int dem) {
     with_dem(dem, [&](auto D) {
         hipLaunchKernelGGL((k_shadow<K_GEN, decltype(D)::value, false>), dim3(grid), dim3(256), 0, s, d, b, j);

@@ -68,7 +68,7 @@ enum : int {
     C_POSTQ = 4,   // next item of the shadow pass
     C_AOQ = 5,     // next item of the DE AO pass
     C_SURF = 6,    // refined surface list length (the normals pass's items)
-    C_STATS = 8,   // 15 work counters from here
+    C_STATS = 8,   // 16 work counters from here (the last: the network's rays)
     C_WORDS = 24
 };
 
@@ -377,7 +377,7 @@ __device__ __forceinline__ bool item_pixel(const DevP &F, int ql, int &x, int &y
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__restrict__ gtab, int nframes, MlpDev M,
                                                      const double2 *__restrict__ rtab, const ConeRes *__restrict__ cone,
-                                                     float2 *prop) {
+                                                     float2 *prop, unsigned long long *__restrict__ ctr) {
     const GlobalFrames tab{reinterpret_cast<const char *>(gtab)};
     __shared__ __attribute__((aligned(16))) char lds[4][MLP_LDS_BYTES];
     __shared__ __attribute__((aligned(16))) mlp_s4 wl[MLP_TILES * 64];
@@ -386,6 +386,7 @@ __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__rest
     const int lane = lane_id();
     const long long groups = (P0.nslots_q + 63) / 64;
     const long long waves = (long long)gridDim.x * 4;
+    unsigned long long evaluated = 0;  // rays the network ran on (the wave's, the same in every lane)
     // a uniform loop: every lane of a wave runs the same groups
     for (long long g = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); g < groups; g += waves) {
         const long long q = g * 64 + lane;
@@ -422,8 +423,10 @@ __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__rest
         const float pc[3] = {o[0] + s * d[0], o[1] + s * d[1], o[2] + s * d[2]};
         float t = 0.0f, lg = 0.0f;
         mlp_wave_infer(M, wl, pc, d, lds[threadIdx.x >> 6], t, lg);
+        evaluated += 64;
         if (in_q) prop[q] = make_float2(t + 0.25f * s, lg);
     }
+    if (lane_id() == 0 && evaluated) atomicAdd(&ctr[C_STATS + 15], evaluated);
 }
 
 // The cone rays (cone_march, src/calc.rs:2357-2435, and its grid,
@@ -1115,6 +1118,253 @@ __global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__
             cnt.hits++;
             hit = true;
             pix = o.pix;
+        }
+        const long long hs = wave_take(hit, &B.ctr[C_HITS], hq, 64, -1);
+        if (hit) B.hits[hs] = pix;
+    }
+    if (lane_id() < hq.left) B.hits[hq.base + lane_id()] = -1;
+    flush_counters(cnt, B.ctr);
+}
+
+// create_xy_vecs_from_normals (src/math.rs:218-232): two unit vectors
+// across the normal n (view frame)
+__device__ inline void xy_vecs(const double n[3], double vx[3], double vy[3]) {
+    const double d = n[1] * n[1] + n[0] * n[0];
+    if (d < 1e-50) {
+        vx[0] = 1.0, vx[1] = 0.0, vx[2] = 0.0;
+        vy[0] = 0.0, vy[1] = 1.0, vy[2] = 0.0;
+        return;
+    }
+    const double a = acos(fmin(fmax(-n[2] / sqrt(d + n[2] * n[2] + 1e-100), -1.0), 1.0)) * 0.5;
+    double sin_a, cos_a;
+    sincos(a, &sin_a, &cos_a);
+    sin_a /= sqrt(d);
+    const double dd = -n[1] * sin_a, d1 = n[0] * sin_a;
+    vx[0] = 1.0 - 2.0 * d1 * d1, vx[1] = 2.0 * dd * d1, vx[2] = 2.0 * d1 * cos_a;
+    vy[0] = vx[1], vy[1] = 1.0 - 2.0 * dd * dd, vy[2] = -2.0 * dd * cos_a;
+}
+
+// The surface values of the general modes at a refined surface point
+// (surface_found, src/calc.rs:2471-2517): the normal -- calculate_normals
+// with smooth normals 0..8 (src/calc.rs:1813-1903) or
+// calculate_normals_on_smooth_it (1910-2003) -- with its roughness, NN,
+// then do_color_on_it (2010-2025) and do_color (2068-2090) on the last
+// evaluation's orbit.  All in f64, in the reference's order; the number of
+// evaluations depends on the modes alone, so a wave's lanes stay together.
+template <int KIND, int DEM>
+__device__ void surface_ext(const DevP &P, const SurfRec &o, Cnt &cnt, double n[3], float &nn, float &rough,
+                            uint32_t &otrap_w, int &grad5) {
+    const double ct[3] = {o.c[0], o.c[1], o.c[2]};
+    const float ms = o.ms;
+    const int sm = P.sm_normals;
+    const bool on_de = P.normals_on_de != 0;
+    double noffset = (double)fminf(P.de_stop, 1.0f) * (1.0 + o.mzz * (double)P.de_stop_factor) * 0.15;
+    double c[3];
+    De64 last;
+    // one evaluation at c: the DE, or the smooth iteration count (normals
+    // on the smooth iteration); the sample's value
+    auto eval = [&](int calc_sit) -> double {
+        if (on_de) {
+            last = de64<KIND, DEM>(P, c[0], c[1], c[2], P.max_it, calc_sit, ms);
+            cnt.de64++;
+        } else {
+            last = mand64<KIND>(P, c[0], c[1], c[2], P.max_it);
+        }
+        cnt.p8_64 += last.p8;
+        cnt.box_64 += last.box;
+        cnt.oth_64 += last.oth;
+        return on_de ? last.de : (double)last.smooth_it;
+    };
+    auto at = [&](const double *base) {
+        c[0] = base[0];
+        c[1] = base[1];
+        c[2] = base[2];
+    };
+    auto addw = [&](const double *dir, double w) {
+        for (int k = 0; k < 3; k++) c[k] += dir[k] * w;
+    };
+    const double sgn = on_de ? 1.0 : -1.0;  // the smooth iteration falls outwards, the DE rises
+    at(ct);
+    double dnn = eval(1);
+    nn = last.smooth_it;
+    n[0] = n[1] = n[2] = 0.0;
+    if (sm == 8) {
+        const double ssn = noffset * 1.3333;
+        for (int a = -2; a <= 2; a++)
+            for (int b = -2; b <= 2; b++)
+                for (int cc = -2; cc <= 2; cc++) {
+                    if ((a | b | cc) == 0) continue;
+                    at(ct);
+                    addw(P.vg[2], (double)a * ssn);
+                    addw(P.vg[1], (double)b * ssn);
+                    addw(P.vg[0], (double)cc * ssn);
+                    const double d = sgn * eval(0);
+                    if (a != 0) n[2] += d / (double)a;
+                    if (b != 0) n[1] += d / (double)b;
+                    if (cc != 0) n[0] += d / (double)cc;
+                }
+        for (int k = 0; k < 3; k++) n[k] *= 0.0075;
+    } else {
+        for (int q = 0; q < 3; q++) {
+            const int axis = q == 0 ? 2 : q - 1;  // the order vg2, vg0, vg1
+            at(ct);
+            addw(P.vg[axis], noffset);
+            const double a = eval(0);
+            addw(P.vg[axis], -2.0 * noffset);
+            const double b = eval(0);
+            n[axis] = on_de ? (a - b) * 0.5 : (b - a) * 0.5;
+        }
+    }
+    rough = 0.0f;
+    if (sm > 0) {
+        noffset *= 2.0;
+        if (on_de && sm < 8) {
+            at(ct);
+            addw(P.vg[0], -noffset);
+            dnn += eval(0);
+            addw(P.vg[0], 2.0 * noffset);
+            dnn += eval(0);
+            at(ct);
+            addw(P.vg[1], -noffset);
+            dnn += eval(0);
+            addw(P.vg[1], 2.0 * noffset);
+            dnn = (dnn + eval(0)) * 0.2;
+        } else if (!on_de) {
+            double acc = (double)nn;
+            const int axes[4] = {0, 0, 1, 1};
+            const double sg[4] = {-1.0, 1.0, -1.0, 1.0};
+            for (int q = 0; q < 4; q++) {
+                at(ct);
+                addw(P.vg[axes[q]], sg[q] * noffset);
+                acc += eval(1);
+            }
+            nn = (float)(acc * 0.2);
+            dnn = (double)nn;
+        }
+        const double ssn = noffset * 3.0 / ((double)sm + 0.5);
+        const double dm = (double)sm * 2.0;
+        double vx0[3], vy0[3], dirs[2][3];
+        xy_vecs(n, vx0, vy0);
+        for (int k = 0; k < 3; k++) {
+            dirs[0][k] = vx0[0] * P.vg[0][k] + vx0[1] * P.vg[1][k] + vx0[2] * P.vg[2][k];
+            dirs[1][k] = vy0[0] * P.vg[0][k] + vy0[1] * P.vg[1][k] + vy0[2] * P.vg[2][k];
+        }
+        double s1[2], ds[2];
+        for (int w = 0; w < 2; w++) {
+            at(ct);
+            addw(dirs[w], -(double)sm * ssn);
+            s1[w] = ds[w] = 0.0;
+            for (int k = -sm; k <= sm; k++) {
+                if (k != 0) {
+                    const double v = eval(on_de ? 0 : 1);
+                    const double d = (on_de ? v - dnn : dnn - v) / (double)k;
+                    s1[w] += d;
+                    ds[w] += d * d;
+                }
+                addw(dirs[w], ssn);
+            }
+        }
+        const double dsg = ds[0] * dm - s1[0] * s1[0] + ds[1] * dm - s1[1] * s1[1];
+        const double dt2 = noffset * 0.5 / (dm * ssn);
+        const double ra = (dsg * 7.0 * dt2 * dt2 + 1e-40) / (n[0] * n[0] + n[1] * n[1] + n[2] * n[2] + 1e-40);
+        rough = (float)fmin(fmax(sqrt(fmax(ra, 0.0)) - 0.05, 0.0), 1.0);  // calc_roughness (src/calc.rs:1806-1811)
+        if (sm < 8) {
+            n[0] += s1[0] * dt2;
+            n[1] += s1[1] * dt2;
+        }
+    }
+    // do_color_on_it: the orbit from the start point, or after
+    // color_on_it - 1 iterations (with the 3D bailout)
+    double v[3] = {last.v[0], last.v[1], last.v[2]};
+    double rout = last.rout, rold = last.rold, otrap = last.otrap;
+    if (P.color_on_it == 1) {
+        v[0] = ct[0], v[1] = ct[1], v[2] = ct[2];
+    } else if (P.color_on_it > 1) {
+        const De64 r = de64<KIND, DEM>(P, ct[0], ct[1], ct[2], P.color_on_it - 1, 0, ms, P.rstop3d);
+        cnt.de64++;
+        v[0] = r.v[0], v[1] = r.v[1], v[2] = r.v[2];
+        rout = r.rout, rold = r.rold, otrap = r.otrap;
+    }
+    // do_color
+    const double pi = 3.141592653589793;
+    double sc;
+    grad5 = -1;
+    switch (P.color_option) {
+        case 1: sc = log(rout / (rold + 1.0)) * (double)P.mct_color_mul; break;
+        case 2: sc = (atan2(v[1] - ct[1], v[0] - ct[0]) + pi) * 5200.0; break;
+        case 3: sc = (atan2(v[2] - ct[2], v[0] - ct[0]) + pi) * 5200.0; break;
+        case 4: sc = (atan2(v[2] - ct[2], v[1] - ct[1]) + pi) * 5200.0; break;
+        case 5: {
+            sc = (atan2(v[0], v[1]) + pi) * 5215.0;
+            const double r = sqrt(v[0] * v[0] + v[1] * v[1] + 1e-100 + v[2] * v[2]);
+            grad5 = (int)clip15((float)((pi + asin(fmin(fmax(v[2] / r, -1.0), 1.0)) * 2.0) * 5215.0));
+            break;
+        }
+        default: sc = otrap * 4096.0; break;
+    }
+    otrap_w = clip15((float)sc);
+}
+
+// The normals pass of the general surface modes (smooth normals, normals on
+// the smooth iteration, colour options 1..5, colour on an iteration):
+// surface_ext per refined surface point, in f64, then the G-buffer words
+// and the hit list as k_normals writes them.
+template <int KIND, int DEM, bool BATCH>
+__global__ __launch_bounds__(256) void k_normals_ext(DevP P0, Bufs B, const DevP *__restrict__ gtab, int nframes) {
+    const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
+    Cnt cnt;
+    WaveChunk hq;
+    chunk_init(hq);
+    const long long n_items = (long long)B.ctr[C_SURF];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n_items; base += stride) {
+        const long long i = base + lane_id();
+        const SurfRec o = i < n_items ? B.surf[i] : SurfRec{{0, 0, 0}, 0, 0, 0, -1, 0};
+        bool hit = false;
+        int pix = 0;
+        if (o.pix >= 0) {
+            const int fr = BATCH ? o.info >> 20 : 0;
+            const DevP &P = BATCH ? tab[fr] : P0;
+            double nv[3];
+            float nn, rough;
+            uint32_t otrap_w;
+            int grad5;
+            surface_ext<KIND, DEM>(P, o, cnt, nv, nn, rough, otrap_w, grad5);
+            int16_t nq[3];
+            store_normal(nv, nq);
+            const bool de_limited = (o.info & SR_DELIM) != 0;
+            float g;
+            if (de_limited) {
+                const float lg = (float)log((double)fmaxf(P.de_stop, o.ms) * P.step_width);
+                g = 32767.0f - (nn + P.d_col_plus + P.col_var * lg) * P.mcts_m;
+            } else {
+                g = 32767.0f - nn * P.mcts_m;
+            }
+            const uint32_t si_gradient = grad5 >= 0 ? (uint32_t)grad5 : clip15(g);
+            const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(o.step_count, 0.0f), 1023.0f));
+            Ray64 R;
+            R.mzz = o.mzz;
+            uint32_t zp = R.zpos_fine(P);
+            if (P.sm_normals > 0) zp |= (uint32_t)rtef(rough * 255.0f) & 0xFFu;
+            const double l = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+            const int px = o.pix;
+            if (B.depth) B.depth[px] = (float)o.mzz;
+            if (B.depth64) B.depth64[px] = o.mzz;
+            if (B.iters) B.iters[px] = (uint16_t)(o.info & SR_ITERS);
+            if (B.normals) {
+                for (int k = 0; k < 3; k++) B.normals[3 * px + k] = l > 0 ? (float)(nv[k] / l) : 0.0f;
+            }
+            if (B.gbuf) {
+                uint32_t *w = B.gbuf + 5 * (size_t)px;
+                w[0] = (uint32_t)(uint16_t)nq[0] | ((uint32_t)(uint16_t)nq[1] << 16);
+                w[1] = (uint32_t)(uint16_t)nq[2] | (sh << 16);
+                w[2] = zp;
+                w[3] = si_gradient | (otrap_w << 16);
+                w[4] = 5000;
+            }
+            cnt.hits++;
+            hit = true;
+            pix = px;
         }
         const long long hs = wave_take(hit, &B.ctr[C_HITS], hq, 64, -1);
         if (hit) B.hits[hs] = pix;
@@ -2009,6 +2259,7 @@ __global__ void k_de_at(DevP P, const double *pts, long long n, double *de, int3
 // DEM_BOX, `f32` is always false for it (no FP32 iterate).
 void launch_gen_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all, int dem);
 void launch_gen_normals(int grid, hipStream_t s, const DevP &d, const Bufs &b, int dem);
+void launch_gen_normals_ext(int grid, hipStream_t s, const DevP &d, const Bufs &b, int dem);
 void launch_gen_shadow(int grid, hipStream_t s, const DevP &d, const Bufs &b, const ShadowJob &j, int dem);
 void launch_gen_deao(int grid, hipStream_t s, const DevP &d, const Bufs &b, const AoRays &r, int dem);
 void launch_gen_de_at(int grid, hipStream_t s, const DevP &d, const double *pts, long long n, double *de,

@@ -560,6 +560,7 @@ extern "C" int32_t mb3d_hip_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip
     p.max_it = max_it;
     p.min_it = std::min(sc->min_iterations, max_it);
     p.color_option = std::min(sc->color_option, 5);
+    p.color_on_it = std::min(std::max(sc->color_on_it, 0), 255);  // (a u8 in CalcParams, src/calc.rs:27)
     p.mct_color_mul = (float)(sc->color_mul * 512.0);
     p.dfog_on_it = sc->dfog_on_it;
     p.zcorr = zcorr;

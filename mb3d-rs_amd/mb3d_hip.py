@@ -84,7 +84,7 @@ class Scene(C.Structure):
         ("decomb_iterations2", C.c_int32),
         ("decomb_smooth", C.c_float),
         ("part2_placed", C.c_int32),
-        ("reserved", C.c_int32),
+        ("color_on_it", C.c_int32),
         ("rstop", C.c_double),
         ("zoom", C.c_double),
         ("mid", C.c_double * 3),
@@ -145,7 +145,7 @@ class Params(C.Structure):
         ("is_custom_de2", C.c_int32),
         ("four_d2", C.c_int32),
         ("placed", C.c_int32),
-        ("reserved", C.c_int32),
+        ("color_on_it", C.c_int32),
         ("slot_z_mul", S6d),
         ("slot_scale", S6d),
         ("slot_min_r", S6d),
@@ -259,6 +259,7 @@ class Counters(C.Structure):
         ("cone_background", C.c_uint64),
         ("mlp_ms", C.c_double),
         ("cone_ms", C.c_double),
+        ("mlp_rays", C.c_uint64),
     ]
 
 
@@ -708,6 +709,29 @@ def scene_bulb(width, height, seed=1, de_mode=DE_ANALYTIC_P8, step_cap=500, bin_
     s.de_mode = de_mode
     set_camera(s, vgrads_from_angles(pitch, yaw, 0.0))
     set_formulas(s, [intpow(8, 1)])
+    return s
+
+
+def scene_m3s_mandelbulb(width=800, height=600) -> Scene:
+    """The reference's scene fixture tests/data/mandelbulb.m3s (the power-8
+    Integer Power bulb close up, smooth normals 3), its keys transcribed:
+    iterations 14, zoom 1.8, mid (0, -0.35, 0), z_start -2, z_end 30,
+    rotation (-40, 30, 0) degrees, de_stop 0.8, raystep 0.4, smooth_normals
+    3, bin_search 8, Integer Power 8 with z multiplier -1 and the reference's
+    numeric DE.  (Its lights and colours are the painter's.)"""
+    s = scene_default()
+    s.width, s.height = width, height
+    s.iterations = 14
+    s.zoom = 1.8
+    s.mid[:] = [0.0, -0.35, 0.0]
+    s.z_start, s.z_end = -2.0, 30.0
+    set_camera(s, vgrads_from_angles(math.radians(-40.0), math.radians(30.0), 0.0))
+    s.de_stop = 0.8
+    s.z_step_div = 0.4
+    s.smooth_normals = 3
+    s.bin_search_steps = 8
+    s.de_mode = DE_REFERENCE
+    set_formulas(s, [intpow(8, 1, -1.0)])
     return s
 
 

@@ -360,6 +360,7 @@ struct Marcher {
     int32_t seed = 0;
     uint64_t de_evals = 0;
     int nrm_it = 0;  // the iteration count of calculate_normals' centre DE
+    float s_roughness = 0.0f;  // calc_roughness of the last smoothed normal
     bool calc_inside = false;  // the DE from inside the set (the Monte Carlo marches' inside flag)
 
     explicit Marcher(const mb3d_hip_params &pp) : p(pp) {
@@ -883,6 +884,212 @@ struct Marcher {
         return nn;
     }
 
+    // calc_roughness (src/calc.rs:1806-1811)
+    static float calc_roughness(const double n[3], double dt2, double dsg) {
+        double a = (dsg * 7.0 * dt2 * dt2 + 1e-40) / (n[0] * n[0] + n[1] * n[1] + n[2] * n[2] + 1e-40);
+        return (float)std::fmin(std::fmax(std::sqrt(std::fmax(a, 0.0)) - 0.05, 0.0), 1.0);
+    }
+
+    // The smoothing of calculate_normals and calculate_normals_on_smooth_it
+    // (src/calc.rs:1860-1901, 1958-1998): two sweeps of 2 sm samples across
+    // the normal (create_xy_vecs_from_normals) around the centre value
+    // `centre`, the roughness from their spread, and (sm < 8) the normal
+    // moved by their mean slope.  `sample(sign)` evaluates at it.c and
+    // returns the sweep's value: the DE's (value - centre), or the smooth
+    // iteration's (centre - value).
+    template <class Sample>
+    void smooth_sweeps(double n[3], const double ct1[3], double noffset, Sample sample) {
+        const auto &vg = p.vgrads;
+        const int sm = p.sm_normals;
+        double ssn = noffset * 3.0 / ((double)sm + 0.5);
+        double dm = (double)sm * 2.0;
+        double vx0[3], vy0[3], vx[3], vy[3];
+        create_xy_vecs_from_normals(n, vx0, vy0);
+        rot_rev(vx0, vg, vx);
+        rot_rev(vy0, vg, vy);
+        auto sweep = [&](const double dir[3], double &s1, double &ds) {
+            std::memcpy(it.c, ct1, 3 * sizeof(double));
+            add_weight(it.c, dir, -(double)sm * ssn);
+            s1 = ds = 0.0;
+            for (int k = -sm; k <= sm; k++) {
+                if (k != 0) {
+                    double d = sample() / (double)k;
+                    s1 += d;
+                    ds += d * d;
+                }
+                add_weight(it.c, dir, ssn);
+            }
+        };
+        double nn1, ds1, nn2, ds2;
+        sweep(vx, nn1, ds1);
+        sweep(vy, nn2, ds2);
+        double dsg = ds1 * dm - nn1 * nn1 + ds2 * dm - nn2 * nn2;
+        double dt2 = noffset * 0.5 / (dm * ssn);
+        s_roughness = calc_roughness(n, dt2, dsg);
+        if (sm < 8) {
+            n[0] += nn1 * dt2;
+            n[1] += nn2 * dt2;
+        }
+    }
+
+    // create_xy_vecs_from_normals (src/math.rs:218-232)
+    static void create_xy_vecs_from_normals(const double n[3], double vx[3], double vy[3]) {
+        double d = n[1] * n[1] + n[0] * n[0];
+        if (d < 1e-50) {
+            vx[0] = 1.0, vx[1] = 0.0, vx[2] = 0.0;
+            vy[0] = 0.0, vy[1] = 1.0, vy[2] = 0.0;
+            return;
+        }
+        double a = std::acos(std::fmin(std::fmax(-n[2] / std::sqrt(d + n[2] * n[2] + 1e-100), -1.0), 1.0)) * 0.5;
+        double sin_a = std::sin(a), cos_a = std::cos(a);
+        sin_a /= std::sqrt(d);
+        double dd = -n[1] * sin_a, d1 = n[0] * sin_a;
+        vx[0] = 1.0 - 2.0 * d1 * d1, vx[1] = 2.0 * dd * d1, vx[2] = 2.0 * d1 * cos_a;
+        vy[0] = vx[1], vy[1] = 1.0 - 2.0 * dd * dd, vy[2] = -2.0 * dd * cos_a;
+    }
+
+    // calculate_normals for any smooth_normals (src/calc.rs:1813-1903): the
+    // gradient (6 DEs, or the 124-point cube of smooth_normals 8), then the
+    // smoothing sweeps and roughness; the raw normal to n, returns NN
+    float calculate_normals_sm(double n[3]) {
+        const auto &vg = p.vgrads;
+        const int sm = p.sm_normals;
+        if (sm == 0) return calculate_normals(n);
+        it.calc_sit = true;
+        double noffset = (double)std::fmin(p.de_stop, 1.0f) * (1.0 + mzz * (double)p.de_stop_factor) * 0.15;
+        double ct1[3] = {it.c[0], it.c[1], it.c[2]};
+        double dnn = calc_de();
+        float nn = it.smooth_it;
+        nrm_it = it.it_result;
+        it.calc_sit = false;
+        n[0] = n[1] = n[2] = 0.0;
+        if (sm == 8) {
+            double ssn = noffset * 1.3333;
+            for (int a = -2; a <= 2; a++)
+                for (int b = -2; b <= 2; b++)
+                    for (int c = -2; c <= 2; c++) {
+                        if ((a | b | c) == 0) continue;
+                        std::memcpy(it.c, ct1, sizeof ct1);
+                        add_weight(it.c, vg[2], (double)a * ssn);
+                        add_weight(it.c, vg[1], (double)b * ssn);
+                        add_weight(it.c, vg[0], (double)c * ssn);
+                        double d = calc_de();
+                        if (a != 0) n[2] += d / (double)a;
+                        if (b != 0) n[1] += d / (double)b;
+                        if (c != 0) n[0] += d / (double)c;
+                    }
+            for (int k = 0; k < 3; k++) n[k] *= 0.0075;
+        } else {
+            const int order[3][2] = {{2, 2}, {0, 0}, {1, 1}};
+            for (auto &ka : order) {
+                int k = ka[0], axis = ka[1];
+                std::memcpy(it.c, ct1, sizeof ct1);
+                add_weight(it.c, vg[axis], noffset);
+                double a = calc_de();
+                add_weight(it.c, vg[axis], -2.0 * noffset);
+                n[k] = (a - calc_de()) * 0.5;
+            }
+        }
+        noffset *= 2.0;
+        if (sm < 8) {
+            std::memcpy(it.c, ct1, sizeof ct1);
+            add_weight(it.c, vg[0], -noffset);
+            dnn += calc_de();
+            add_weight(it.c, vg[0], 2.0 * noffset);
+            dnn += calc_de();
+            std::memcpy(it.c, ct1, sizeof ct1);
+            add_weight(it.c, vg[1], -noffset);
+            dnn += calc_de();
+            add_weight(it.c, vg[1], 2.0 * noffset);
+            dnn = (dnn + calc_de()) * 0.2;
+        }
+        smooth_sweeps(n, ct1, noffset, [&]() { return calc_de() - dnn; });
+        std::memcpy(it.c, ct1, sizeof ct1);
+        return nn;
+    }
+
+    // calculate_normals_on_smooth_it (src/calc.rs:1910-2003): the gradient
+    // of the smooth iteration count (plain iterations, no DE), outside
+    // rendering
+    float calculate_normals_on_smooth_it(double n[3]) {
+        const auto &vg = p.vgrads;
+        const int sm = p.sm_normals;
+        double noffset = (double)std::fmin(p.de_stop, 1.0f) * (1.0 + mzz * (double)p.de_stop_factor) * 0.15;
+        it.calc_sit = true;
+        double ct1[3] = {it.c[0], it.c[1], it.c[2]};
+        mand_function();
+        float nn = it.smooth_it;
+        nrm_it = it.it_result;
+        n[0] = n[1] = n[2] = 0.0;
+        if (sm == 8) {
+            double ssn = noffset * 1.3333;
+            for (int a = -2; a <= 2; a++)
+                for (int b = -2; b <= 2; b++)
+                    for (int c = -2; c <= 2; c++) {
+                        if ((a | b | c) == 0) continue;
+                        std::memcpy(it.c, ct1, sizeof ct1);
+                        add_weight(it.c, vg[2], (double)a * ssn);
+                        add_weight(it.c, vg[1], (double)b * ssn);
+                        add_weight(it.c, vg[0], (double)c * ssn);
+                        mand_function();
+                        double d = (double)it.smooth_it;
+                        if (a != 0) n[2] -= d / (double)a;
+                        if (b != 0) n[1] -= d / (double)b;
+                        if (c != 0) n[0] -= d / (double)c;
+                    }
+            for (int k = 0; k < 3; k++) n[k] *= 0.0075;
+        } else {
+            const int order[3][2] = {{2, 2}, {0, 0}, {1, 1}};
+            for (auto &ka : order) {
+                int k = ka[0], axis = ka[1];
+                std::memcpy(it.c, ct1, sizeof ct1);
+                add_weight(it.c, vg[axis], noffset);
+                mand_function();
+                double a = (double)it.smooth_it;
+                add_weight(it.c, vg[axis], -2.0 * noffset);
+                mand_function();
+                n[k] = ((double)it.smooth_it - a) * 0.5;
+            }
+        }
+        if (sm > 0) {
+            noffset *= 2.0;
+            double acc = (double)nn;
+            const int axes[4] = {0, 0, 1, 1};
+            const double sgn[4] = {-1.0, 1.0, -1.0, 1.0};
+            for (int q = 0; q < 4; q++) {
+                std::memcpy(it.c, ct1, sizeof ct1);
+                add_weight(it.c, vg[axes[q]], sgn[q] * noffset);
+                mand_function();
+                acc += (double)it.smooth_it;
+            }
+            nn = (float)(acc * 0.2);
+            const double nnv = (double)nn;
+            smooth_sweeps(n, ct1, noffset, [&]() {
+                mand_function();
+                return nnv - (double)it.smooth_it;
+            });
+        }
+        std::memcpy(it.c, ct1, sizeof ct1);
+        return nn;
+    }
+
+    // do_color_on_it (src/calc.rs:2010-2025): the colour from the start
+    // vector (1) or from the orbit after color_on_it - 1 iterations
+    void do_color_on_it() {
+        if (p.color_on_it == 1) {
+            it.v[0] = it.c[0];
+            it.v[1] = it.c[1];
+            it.v[2] = it.c[2];
+        } else {
+            int i = it.max_it;
+            it.max_it = p.color_on_it - 1;
+            it.rstop = p.rstop3d;
+            calc_de();
+            it.max_it = i;
+            it.rstop = p.d_rstop;
+        }
+    }
+
     // store_normal (src/calc.rs:1794-1804)
     static void store_normal(int16_t out[3], const double n[3]) {
         double d = 32767.0 / std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2] + 1e-100);
@@ -893,13 +1100,16 @@ struct Marcher {
         if (out[0] == 0 && out[1] == 0) out[2] = out[2] > 0 ? 32767 : -32767;
     }
 
-    // calc_zpos_and_rough (src/calc.rs:2194-2205), sm_normals = 0
+    // calc_zpos_and_rough (src/calc.rs:2194-2205): the depth code, and the
+    // roughness in the low byte with smooth normals
     uint32_t zpos_fine() const {
         double zz = std::max(mzz, 0.0);
         double vr = round_even(p.zc_mul * (std::sqrt(zz * p.zcorr + 1.0) - 1.0));
         int64_t v = (int64_t)vr;
         int64_t itmp = std::min<int64_t>(std::max<int64_t>(8388352 - v, 0), 8388352);
-        return (uint32_t)itmp << 8;
+        uint32_t r = (uint32_t)itmp << 8;
+        if (p.sm_normals > 0) r |= (uint32_t)std::nearbyint(s_roughness * 255.0f) & 0xFFu;
+        return r;
     }
 };
 
@@ -933,8 +1143,30 @@ void from_words(const uint32_t w[5], Px &s) {
     s.amb_shadow = (uint16_t)(w[4] & 0xFFFF);
 }
 
-// do_color with color_option 0 (src/calc.rs:2068-2090)
-void do_color(const Marcher &m, Px &si) { si.otrap = clip15((float)(m.it.otrap * 4096.0)); }
+// do_color (src/calc.rs:2068-2090): the orbit trap (0), the last two radii
+// (1), the angles of the orbit's end around the point (2-4), or its
+// spherical angles (5, which also sets the gradient)
+void do_color(const Marcher &m, Px &si) {
+    const double x = m.it.v[0], y = m.it.v[1], z = m.it.v[2];
+    const double *c = m.it.c;
+    const double pi = 3.141592653589793;
+    double s;
+    switch (m.p.color_option) {
+        case 1: s = std::log(m.it.rout / (m.it.rold + 1.0)) * (double)m.p.mct_color_mul; break;
+        case 2: s = (std::atan2(y - c[1], x - c[0]) + pi) * 5200.0; break;
+        case 3: s = (std::atan2(z - c[2], x - c[0]) + pi) * 5200.0; break;
+        case 4: s = (std::atan2(z - c[2], y - c[1]) + pi) * 5200.0; break;
+        case 5: {
+            s = (std::atan2(x, y) + pi) * 5215.0;
+            double r = std::sqrt(x * x + y * y + 1e-100 + z * z);
+            double s2 = (pi + std::asin(std::fmin(std::fmax(z / r, -1.0), 1.0)) * 2.0) * 5215.0;
+            si.si_gradient = clip15((float)s2);
+            break;
+        }
+        default: s = m.it.otrap * 4096.0; break;
+    }
+    si.otrap = clip15((float)s);
+}
 
 // surface_found (src/calc.rs:2471-2517)
 void surface_found(Marcher &m, Px &si, double dtmp, double last_step) {
@@ -948,7 +1180,7 @@ void surface_found(Marcher &m, Px &si, double dtmp, double last_step) {
     }
     si.iters = (uint16_t)std::max(m.it.it_result, 0);
     double n[3];
-    float nn = m.calculate_normals(n);
+    float nn = p.normals_on_de ? m.calculate_normals_sm(n) : m.calculate_normals_on_smooth_it(n);
     Marcher::store_normal(si.normal, n);
     double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     for (int k = 0; k < 3; k++) si.n[k] = l > 0 ? n[k] / l : 0.0;
@@ -960,6 +1192,7 @@ void surface_found(Marcher &m, Px &si, double dtmp, double last_step) {
         v = 32767.0f - nn * p.mcts_m;
     }
     si.si_gradient = clip15(v);
+    if (p.color_on_it != 0) m.do_color_on_it();
     do_color(m, si);
     si.zpos_fine = m.zpos_fine();
     si.depth = m.mzz;
@@ -1494,8 +1727,6 @@ int check_params(const mb3d_hip_params *p) {
     if (p->abi_version != MB3D_HIP_ABI_VERSION) return fail("params ABI version mismatch");
     if (p->width <= 0 || p->height <= 0) return fail("bad picture size");
     if (p->n_slots < 1 || p->n_slots > MB3D_HIP_MAX_SLOTS) return fail("bad slot count");
-    if (p->sm_normals != 0) return fail("the oracle covers smooth_normals = 0 only");
-    if (p->color_option != 0) return fail("the oracle covers color_option 0 only");
     return 0;
 }
 
@@ -1984,6 +2215,7 @@ extern "C" int32_t orc_params_from_scene(const mb3d_hip_scene *sc, mb3d_hip_para
     p.max_it = max_it;
     p.min_it = std::min(sc->min_iterations, max_it);
     p.color_option = std::min(sc->color_option, 5);
+    p.color_on_it = std::min(std::max(sc->color_on_it, 0), 255);  // (a u8 in CalcParams, src/calc.rs:27)
     p.mct_color_mul = (float)(sc->color_mul * 512.0);
     p.d_rstop = d_rstop;
     p.rstop3d = rstop3d;
@@ -2110,6 +2342,11 @@ extern "C" int32_t orc_deao_probe(const mb3d_hip_params *pp, const mb3d_hip_post
     const double info[6] = {pr.step_ao, pr.max_dist, pr.ms_ao, pr.md_d10, pr.de_mul, pr.dsm};
     for (int i = 0; i < 6; i++) out[135 + i] = info[i];
     return pr.count;
+}
+
+// create_xy_vecs_from_normals (src/math.rs:218-232), for the tests
+extern "C" void orc_xy_vecs(const double n[3], double vx[3], double vy[3]) {
+    Marcher::create_xy_vecs_from_normals(n, vx, vy);
 }
 
 extern "C" double orc_march_ray(const mb3d_hip_params *p, const double origin[3], const double dir[3], double zz,

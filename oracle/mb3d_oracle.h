@@ -87,6 +87,10 @@ int32_t orc_post(const mb3d_hip_params *p, const mb3d_hip_post_job *post, int32_
 int32_t orc_deao_probe(const mb3d_hip_params *p, const mb3d_hip_post_job *job, int32_t x, int32_t y,
                        const uint32_t rec[5], double *out, int32_t *amb);
 
+/* create_xy_vecs_from_normals (src/math.rs:218-232): two vectors across
+ * the normal n, for the smoothing sweeps of calculate_normals. */
+void orc_xy_vecs(const double n[3], double vx[3], double vy[3]);
+
 /* The march of one ray from `origin` with direction `dir` (one step long)
  * at depth `zz`, as Marcher::march_ray_from (src/calc.rs:2519-2531): the
  * depth reached (+inf: background), its iteration count and steps. */

@@ -191,6 +191,14 @@ def deao_probe(p, job, x, y, rec):
     return n, amb.value, info
 
 
+def xy_vecs(n):
+    """create_xy_vecs_from_normals: two vectors across the normal n."""
+    a = np.ascontiguousarray(n, np.float64)
+    vx, vy = np.zeros(3), np.zeros(3)
+    load().orc_xy_vecs(_ptr(a), _ptr(vx), _ptr(vy))
+    return vx, vy
+
+
 def pixel_ray(p, x, y):
     c = np.zeros(3)
     v = np.zeros(3)

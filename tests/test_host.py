@@ -207,9 +207,18 @@ def test_unsupported_scenes_name_the_reason():
     p = H.params_from_scene(s)
     assert lib.mb3d_hip_supported(C.byref(p), buf, 200) == H.OK
     s = H.scene_default()
-    s.smooth_normals = 3
+    s.smooth_normals = 3  # smooth normals, the roughness: the general normals pass
+    p = H.params_from_scene(s)
+    assert lib.mb3d_hip_supported(C.byref(p), buf, 200) == H.OK
+    # a colour option of a DE combination's ruling part: not on this backend
+    s = H.scene_preset("Integer Power", 64, 48)
+    H.set_formulas(s, [H.formula("Integer Power"), H.formula("Amazing Box", scale=-1.5)])
+    s.decomb, s.decomb_end1, s.decomb_start2, s.decomb_end2, s.decomb_repeat2 = 1, 0, 1, 1, 1
+    s.decomb_iterations2 = 20
+    s.color_option = 2
     p = H.params_from_scene(s)
     assert lib.mb3d_hip_supported(C.byref(p), buf, 200) == H.UNSUPPORTED
+    assert b"DE combination" in buf.value
     p = H.params_from_scene(H.scene_bulb(64, 48))
     assert lib.mb3d_hip_supported(C.byref(p), buf, 200) == H.OK
 
