@@ -207,8 +207,10 @@ typedef struct mb3d_hip_post_job {
     int32_t deao;            /* 0: no DE ambient occlusion */
     int32_t deao_quality;    /* 0..3 = 3, 7, 17, 33 rays (src/deao.rs:75-105); 4 = the 8-ray set (extension) */
     float deao_max_len;
-    int32_t deao_first_step_random;
-    int32_t reserved;
+    int32_t deao_first_step_random; /* the first step of each ray times 0.5..2 drawn from the row's generator,
+                                       in the CPU's order (src/deao.rs:262-270, 348-351) */
+    int32_t deao_dither;     /* 0: off, 1: 2x2, 2: 3x3 dithering of the ray directions (src/deao.rs:318-342);
+                                not with quality 4 */
 } mb3d_hip_post_job;
 
 /* Output buffers of rows [row0, row0 + rows) of the picture, each row

@@ -93,6 +93,8 @@ struct Ctx {
     unsigned long long *d_ctr = nullptr;
     // the device address of `scratch` (hipGetSymbolAddress at init)
     void *scratch = nullptr;
+    // the DE AO's ray sets of the last post job (the source of its copy)
+    AoSet ao_sets[9];
 };
 Ctx g_ctx;
 
@@ -351,8 +353,9 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
 
 // build_rays (src/deao.rs:75-105); quality 4 is the 8-ray set (extension):
 // the quality-1 geometry with 7 rays on its ring
-void build_rays(int quality, float max_len, AoRays &r) {
+void build_rays(int quality, float max_len, AoRays &r, int row_count[4] = nullptr) {
     std::memset(&r, 0, sizeof r);
+    if (row_count) std::fill(row_count, row_count + 4, 0);
     const double TAU = 6.283185307179586;
     auto push = [&](double ya, double za) {
         const double sy = std::sin(ya), cy = std::cos(ya), sz = std::sin(za), cz = std::cos(za);
@@ -378,11 +381,55 @@ void build_rays(int quality, float max_len, AoRays &r) {
         const double dt1 = y * abr;
         int n = (int)std::nearbyint(std::sin(dt1) * TAU / abr);
         if (q == 4) n = 7;
+        if (row_count) row_count[y] = n;
         for (int x = 0; x < n; x++) push(dt1, x * TAU / n);
     }
     r.d_step_mul = (float)(1.0 + std::sin(abr));
     r.d_min_a_dif = (float)std::cos(abr * 1.2);
     r.correction_weight = qg == 1 ? 0.2f : 0.1666f;
+}
+
+// The DE AO's ray sets per dither cell (src/deao.rs:318-342): for the cell
+// (x % side, y % side) the rings shifted by half a ring and half a gap per
+// side step, the straight ray dropped on the shifted rows; side = dither +
+// 1.  Without dithering the one set of build_rays.  Quality 0..3.
+void build_ao_sets(int quality, int dither, AoSet *sets) {
+    AoRays base;
+    int row_count[4];
+    build_rays(quality, 1.0f, base, row_count);
+    std::memset(sets, 0, 9 * sizeof(AoSet));
+    if (dither <= 0) {
+        std::memcpy(sets[0].dir, base.dir, sizeof base.dir);
+        sets[0].count = base.count;
+        return;
+    }
+    const double TAU = 6.283185307179586;
+    const int side = dither + 1;
+    quality = std::min(std::max(quality, 0), 3);
+    for (int yc = 0; yc < side; yc++)
+        for (int xc = 0; xc < side; xc++) {
+            AoSet &o = sets[yc * side + xc];
+            const double dt1 = (double)yc * 0.5 / (double)dither;
+            const double dt2 = (double)xc * 0.5 / (double)dither;
+            auto push = [&](double ya, double za) {
+                const double sy = std::sin(ya), cy = std::cos(ya), sz = std::sin(za), cz = std::cos(za);
+                o.dir[o.count][0] = (float)(sy * cz);
+                o.dir[o.count][1] = (float)(-sy * sz);
+                o.dir[o.count][2] = (float)(-cy);
+                o.count++;
+            };
+            if (quality == 0) {
+                for (int i = 0; i < 3; i++)
+                    push((dt1 + 0.5) * (50.0 * (3.141592653589793 / 180.0)), ((double)i + dt2) * TAU / 3.0);
+                continue;
+            }
+            const double abr = 1.5707963267948966 / ((double)quality + 0.9);
+            if (!(dt1 > 0.1)) push(0.0, 0.0);
+            for (int rr = 1; rr <= quality; rr++) {
+                const int n = row_count[rr];
+                for (int i = 0; i < n; i++) push(abr * ((double)rr + dt1 - 0.25), ((double)i + dt2) * TAU / (double)n);
+            }
+        }
 }
 
 int ensure(void **buf, size_t *have, size_t need) {
@@ -678,6 +725,10 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     if (row0 < 0 || rows <= 0 || row0 + rows > p.height) return set_error(MB3D_HIP_ERR_INVALID, "rows out of the picture");
     if (post && (post->n_lights < 0 || post->n_lights > MB3D_HIP_MAX_LIGHTS))
         return set_error(MB3D_HIP_ERR_INVALID, "bad light count");
+    if (post && post->deao && (post->deao_dither < 0 || post->deao_dither > 2))
+        return set_error(MB3D_HIP_ERR_INVALID, "bad DE AO dithering (0, 1 or 2)");
+    if (post && post->deao && post->deao_dither && post->deao_quality >= 4)
+        return set_error(MB3D_HIP_UNSUPPORTED, "dithering of the 8-ray AO set (quality 4 has no dithered form)");
     mb3d_hip_options o;
     std::memset(&o, 0, sizeof o);
     if (opt_in) o = *opt_in;
@@ -713,7 +764,11 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     const size_t rtab_bytes = ((size_t)ray_tab_entries(d) * sizeof(double2) + 255) & ~(size_t)255;
     const bool cone = d.cone_mode != 0 && !f64 && !post_only;
     const size_t cone_bytes = cone ? (size_t)(d.nslots_q >> d.tn_shift) * sizeof(ConeRes) : 0;
-    const size_t work_need = base_need + learn_need + rtab_bytes + cone_bytes;
+    const bool ao_rnd = want_ao && post->deao_first_step_random;
+    const size_t ao_sets_bytes = want_ao ? (9 * sizeof(AoSet) + 255) & ~(size_t)255 : 0;
+    const size_t ao_rnd_bytes = ao_rnd ? (npix * sizeof(uint32_t) + 255) & ~(size_t)255 : 0;
+    const size_t ao_off = base_need + learn_need + rtab_bytes + ((cone_bytes + 255) & ~(size_t)255);
+    const size_t work_need = ao_off + ao_sets_bytes + ao_rnd_bytes;
     void *work = nullptr;
     unsigned long long *ctr_dev = c.d_ctr;
     if (noalloc) {
@@ -834,9 +889,26 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
         HIP_TRY(hipGetLastError());
     }
     if (want_ao) {
-        if (post->deao_first_step_random) return set_error(MB3D_HIP_UNSUPPORTED, "DE AO with a random first step (not on this backend)");
         AoRays r;
         build_rays(post->deao_quality, post->deao_max_len, r);
+        const int side = post->deao_dither + 1;
+        build_ao_sets(post->deao_quality, post->deao_dither, c.ao_sets);
+        AoSet *dsets = (AoSet *)((char *)work + ao_off);
+        HIP_TRY(hipMemcpyAsync(dsets, c.ao_sets, (size_t)side * side * sizeof(AoSet), hipMemcpyHostToDevice,
+                               c.stream));
+        r.sets = dsets;
+        r.side = side;
+        r.rnd = nullptr;
+        if (ao_rnd) {
+            uint32_t *rnd = (uint32_t *)((char *)work + ao_off + ao_sets_bytes);
+            AoSeedJob sj;
+            std::memset(&sj, 0, sizeof sj);
+            sj.side = side;
+            for (int yc = 0; yc < side; yc++) sj.rc_y[yc] = c.ao_sets[yc * side].count;
+            hipLaunchKernelGGL(k_ao_seed, dim3(rows), dim3(256), 0, c.stream, d, (const uint32_t *)b.gbuf, rnd, sj);
+            HIP_TRY(hipGetLastError());
+            r.rnd = rnd;
+        }
         DevP d5 = d;
         d5.de_add_steps = 5;  // DeaoRows::new (src/deao.rs:254-260)
         dispatch(kind, dem, [&](auto K, auto D) {

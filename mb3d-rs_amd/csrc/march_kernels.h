@@ -114,10 +114,24 @@ static __global__ void k_ray_tab(const DevP *__restrict__ tab, DevP one, double2
 }
 
 // DE AO ray set (build_rays, src/deao.rs:75-105), computed on the host
+// One ray set of the DE AO in the frame of a surface whose normal is -z
+struct AoSet {
+    float dir[33][3];
+    int count;
+    int pad[3];
+};
+
 struct AoRays {
     float dir[33][3];
     int count;
     float d_step_mul, d_min_a_dif, correction_weight, max_len;
+    // the sets per dither cell: sets[(y % side) * side + x % side] (side =
+    // dither + 1; side 1: the set above, as sets[0])
+    const AoSet *sets;
+    int side;
+    // the random first step: each object pixel's generator state before its
+    // first ray (k_ao_seed), else NULL
+    const uint32_t *rnd;
 };
 
 struct ShadowJob {
@@ -2087,7 +2101,15 @@ template <int KIND, int DEM, bool F32>
 // against 22.1 ms at the compiler's two)
 __global__ __launch_bounds__(256, 3) void k_deao(DevP P, Bufs B, AoRays A) {
     __shared__ float s_min_ra[256 * 33];
+    __shared__ AoSet s_sets[9];
     float *min_ra = s_min_ra + threadIdx.x;  // min_ra[k * 256]
+    {
+        const int nsets = A.side * A.side;
+        const float *src = (const float *)A.sets;
+        float *dst = (float *)s_sets;
+        for (int i = threadIdx.x; i < nsets * (int)(sizeof(AoSet) / 4); i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
     Cnt cnt;
     const long long n_items = (long long)B.ctr[C_HITS];
     const double sw = P.step_width;
@@ -2095,9 +2117,11 @@ __global__ __launch_bounds__(256, 3) void k_deao(DevP P, Bufs B, AoRays A) {
         (double)A.max_len * 0.5 * sqrt((double)P.height * (double)P.height + (double)P.width * (double)P.width);
     const double de_stop = (double)P.de_stop;
     const double dsm = (double)A.d_step_mul;
-    const int rc = A.count;
-    const double de_mul = sqrt((double)rc * 0.5);
-    const double abr = 1.2 / asin(fmin(fmax(1.0 / de_mul, -1.0), 1.0));
+    // the pixel's ray set (its dither cell) and what follows from its count
+    const AoSet *set = s_sets;
+    int rc = 0;
+    double de_mul = 0.0, abr = 0.0;
+    uint32_t rs = 0;  // the row generator's state (the random first step)
     // the camera rows scaled to the step width (normalise_matrix_to)
     double rot_v[3][3];
     for (int i = 0; i < 3; i++) {
@@ -2137,7 +2161,13 @@ __global__ __launch_bounds__(256, 3) void k_deao(DevP P, Bufs B, AoRays A) {
                 const float nl = fmaxf(sqrtf(nf[0] * nf[0] + nf[1] * nf[1] + nf[2] * nf[2]), 1e-30f);
                 const float nu[3] = {nf[0] / nl, nf[1] / nl, nf[2] / nl};
                 normal_matrix(nu, rw);
-                ray_from_code(P, R, pix % P.width, P.row0 + pix / P.width, zcode);
+                const int x = pix % P.width, y = P.row0 + pix / P.width;
+                ray_from_code(P, R, x, y, zcode);
+                set = s_sets + (A.side > 1 ? (y % A.side) * A.side + x % A.side : 0);
+                rc = set->count;
+                de_mul = sqrt((double)rc * 0.5);
+                abr = 1.2 / asin(fmin(fmax(1.0 / de_mul, -1.0), 1.0));
+                if (A.rnd) rs = A.rnd[pix];
                 phase = AO_SURF;
             } else {
                 w[4] = w[4] & ~0xFFFFu;
@@ -2189,7 +2219,7 @@ __global__ __launch_bounds__(256, 3) void k_deao(DevP P, Bufs B, AoRays A) {
     ray:
         if (k >= rc) goto write;
         {
-            const float *dr = A.dir[k];
+            const float *dr = set->dir[k];
             const float v0 = rw[0][0] * dr[0] + rw[0][1] * dr[1] + rw[0][2] * dr[2];
             const float v1 = rw[1][0] * dr[0] + rw[1][1] * dr[1] + rw[1][2] * dr[2];
             const float v2 = rw[2][0] * dr[0] + rw[2][1] * dr[1] + rw[2][2] * dr[2];
@@ -2199,12 +2229,19 @@ __global__ __launch_bounds__(256, 3) void k_deao(DevP P, Bufs B, AoRays A) {
             s_tmp = 1.0;
             b_end = false;
             guard = 0;
+            if (A.rnd) {
+                // get_rand (src/deao.rs:262-265); the first step is not held to max_dist
+                rs = rs * 0x343FDu + 0x269EC3u;
+                dt1 *= (double)((rs >> 8) & 0x7FFFFFu) / (double)0x7FFFFF * 1.5 + 0.5;
+                goto ray_place;
+            }
         }
     ray_step:
         if (dt1 > max_dist) {
             dt1 = max_dist;
             b_end = true;
         }
+    ray_place:
         for (int q = 0; q < 3; q++) R.c[q] = ic[q] + sv[q] * dt1;
         cnt.steps++;
         phase = AO_D;
@@ -2222,7 +2259,8 @@ __global__ __launch_bounds__(256, 3) void k_deao(DevP P, Bufs B, AoRays A) {
                 const float max_add = 1.0f - ma;
                 for (int b = 0; b < rc; b++) {
                     if (b == a) continue;
-                    const float dt = A.dir[a][0] * A.dir[b][0] + A.dir[a][1] * A.dir[b][1] + A.dir[a][2] * A.dir[b][2];
+                    const float dt = set->dir[a][0] * set->dir[b][0] + set->dir[a][1] * set->dir[b][1] +
+                                     set->dir[a][2] * set->dir[b][2];
                     if (dt > A.d_min_a_dif) {
                         const float ac = acosf(fminf(fmaxf(dt, -1.0f), 1.0f));
                         const float ov = min_ra[b * 256] - (float)((double)ac * abr) + 1.0f;
@@ -2241,6 +2279,61 @@ __global__ __launch_bounds__(256, 3) void k_deao(DevP P, Bufs B, AoRays A) {
         if (done) have = false;
     }
     flush_counters(cnt, B.ctr);
+}
+
+// The DE AO generator's state n draws after `s` (the composition of n
+// steps of s -> 0x343FD s + 0x269EC3 mod 2^32, by squaring)
+__device__ inline uint32_t lcg_jump(uint32_t s, uint32_t n) {
+    uint32_t a = 0x343FDu, c = 0x269EC3u, ma = 1u, mc = 0u;
+    while (n) {
+        if (n & 1u) {
+            ma *= a;
+            mc = mc * a + c;
+        }
+        c *= a + 1u;
+        a *= a;
+        n >>= 1;
+    }
+    return ma * s + mc;
+}
+
+// The random first steps of the DE AO: each object pixel's generator state,
+// the row's seed (DeaoRows::row, src/deao.rs:254-262) advanced by one draw
+// per ray of every object pixel left of it, as the row's pass draws them in
+// order.  One block per row: the object flags counted by ballot, 256 pixels
+// a turn.  `rc_y`: the ray count of the rows by y % side (dithering drops
+// the straight ray on shifted rows).
+struct AoSeedJob {
+    int rc_y[3];
+    int side;
+};
+
+static __global__ __launch_bounds__(256) void k_ao_seed(DevP P, const uint32_t *gbuf, uint32_t *rnd, AoSeedJob J) {
+    __shared__ uint32_t s_cnt[4];
+    const int r = blockIdx.x;
+    const int y = P.row0 + r;
+    const uint32_t seed = (uint32_t)(0x24563487LL + ((long long)y + 1) * 0x324594A1LL);
+    const uint32_t rc = (uint32_t)J.rc_y[J.side > 1 ? y % J.side : 0];
+    const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t before = 0;
+    for (int x0 = 0; x0 < P.width; x0 += 256) {
+        const int x = x0 + (int)threadIdx.x;
+        const size_t pix = (size_t)r * P.width + x;
+        bool obj = false;
+        if (x < P.width) {
+            const uint32_t *w = gbuf + 5 * pix;
+            obj = (w[2] >> 16) < 32768u && (w[3] & 0xFFFF) < 32768u;
+        }
+        const unsigned long long m = __ballot(obj);
+        if (lane == 0) s_cnt[wv] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t off = before + (uint32_t)__popcll(m & below);
+        for (int i = 0; i < wv; i++) off += s_cnt[i];
+        if (obj) rnd[pix] = lcg_jump(seed, rc * off);
+        before += s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        __syncthreads();
+    }
 }
 
 // de_at (src/calc.rs:2695-2703) at a list of points: the f64 DE in world

@@ -199,7 +199,7 @@ class PostJob(C.Structure):
         ("deao_quality", C.c_int32),
         ("deao_max_len", C.c_float),
         ("deao_first_step_random", C.c_int32),
-        ("reserved", C.c_int32),
+        ("deao_dither", C.c_int32),
     ]
 
 
@@ -793,10 +793,12 @@ def scene_hybrid(width, height, seed=1) -> Scene:
     return s
 
 
-def post_job(lights=(), deao_quality=None, shadow_max_len_mul=1.0, params: Params = None) -> PostJob:
+def post_job(lights=(), deao_quality=None, shadow_max_len_mul=1.0, params: Params = None, deao_dither=0,
+             deao_first_step_random=False) -> PostJob:
     """A post job with directional lights given as view-space directions
     towards the light (HSvecs = RotateVectorReverse(-StepWidth * LN, M),
-    src/render.rs:2383-2417)."""
+    src/render.rs:2383-2417), and the DE AO of `DeaoParams`
+    (src/deao.rs:14-29) when `deao_quality` is given."""
     j = PostJob()
     j.n_lights = len(lights)
     if lights:
@@ -815,6 +817,8 @@ def post_job(lights=(), deao_quality=None, shadow_max_len_mul=1.0, params: Param
         j.deao = 1
         j.deao_quality = deao_quality
         j.deao_max_len = 1.0
+        j.deao_dither = deao_dither
+        j.deao_first_step_random = int(bool(deao_first_step_random))
     return j
 
 

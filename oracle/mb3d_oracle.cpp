@@ -1530,6 +1530,8 @@ inline float sdot(const float a[3], const float b[3]) { return a[0] * b[0] + a[1
 
 struct Rays {
     std::vector<std::array<float, 3>> dirs;
+    int quality = 0;
+    int row_count[4] = {0, 0, 0, 0};
     float d_step_mul, d_min_a_dif, correction_weight;
 };
 
@@ -1539,6 +1541,7 @@ Rays build_rays(int quality) {
     Rays r;
     const double TAU = 6.283185307179586;
     int q = std::min(quality, 4);
+    r.quality = q;
     if (q == 0) {
         double abr = 60.0 * (3.141592653589793 / 180.0);
         for (int x = 0; x < 3; x++) {
@@ -1560,6 +1563,7 @@ Rays build_rays(int quality) {
         double dt1 = y * abr;
         int n = (int)std::nearbyint(std::sin(dt1) * TAU / abr);
         if (q == 4) n = 7;
+        r.row_count[y] = n;
         for (int x = 0; x < n; x++) {
             std::array<float, 3> d;
             ray_dir(dt1, x * TAU / n, d.data());
@@ -1570,6 +1574,46 @@ Rays build_rays(int quality) {
     r.d_min_a_dif = (float)std::cos(abr * 1.2);
     r.correction_weight = qg == 1 ? 0.2f : 0.1666f;
     return r;
+}
+
+// The ray directions of pixel (x, y) with dithering 1 (2x2) or 2 (3x3):
+// the rings shifted by the pixel's place in its cell, and the centre ray
+// left out on the shifted rows (src/deao.rs:318-342)
+std::vector<std::array<float, 3>> dither_rays(const Rays &base, int dither, int x, int y) {
+    const double TAU = 6.283185307179586;
+    const double dt1 = (double)(y % (dither + 1)) * 0.5 / (double)dither;
+    const double dt2 = (double)(x % (dither + 1)) * 0.5 / (double)dither;
+    std::vector<std::array<float, 3>> dirs;
+    std::array<float, 3> d;
+    if (base.quality == 0) {
+        for (int i = 0; i < 3; i++) {
+            ray_dir((dt1 + 0.5) * (50.0 * (3.141592653589793 / 180.0)), ((double)i + dt2) * TAU / 3.0, d.data());
+            dirs.push_back(d);
+        }
+        return dirs;
+    }
+    const double abr = 1.5707963267948966 / ((double)base.quality + 0.9);
+    if (!(dt1 > 0.1)) {
+        ray_dir(0.0, 0.0, d.data());
+        dirs.push_back(d);
+    }
+    for (int r = 1; r <= base.quality; r++) {
+        const int n = base.row_count[r];
+        for (int i = 0; i < n; i++) {
+            ray_dir(abr * ((double)r + dt1 - 0.25), ((double)i + dt2) * TAU / (double)n, d.data());
+            dirs.push_back(d);
+        }
+    }
+    return dirs;
+}
+
+// The row's random numbers for the first AO steps: DeaoRows::row's seed and
+// get_rand (src/deao.rs:262-270), a 32-bit linear congruential generator
+int32_t deao_row_seed(int y) { return (int32_t)(uint32_t)(0x24563487LL + ((long long)y + 1) * 0x324594A1LL); }
+
+double deao_rand(int32_t &s) {
+    s = (int32_t)((uint32_t)s * 0x343FDu + 0x269EC3u);
+    return (double)(((uint32_t)s >> 8) & 0x7FFFFFu) / (double)0x7FFFFF;
 }
 
 // surface_point (src/calc.rs:2157-2192)
@@ -1609,14 +1653,16 @@ struct DeaoProbe {
     double ic[3];        // the rebuilt surface point
     double dirs[33][3];  // each ray's step in world units (step_ao-scaled directions)
     double min_ra[33];   // each ray's openness (times de_mul)
+    double first[33];    // each ray's first step over step_ao * d_step_mul (1 without the random first step)
     double step_ao, max_dist, ms_ao, md_d10, de_mul, dsm;
     int count;
 };
 
-// deao_row for one pixel, no dither, no random first step
-// (src/deao.rs:268-418); the marcher's parameters have de_add_steps = 5
+// deao_row for one pixel (src/deao.rs:268-418); the marcher's parameters
+// have de_add_steps = 5.  `rnd`: the row's generator state, which the random
+// first steps advance once per ray of every object pixel, left to right
 void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_hip_post_job &job,
-                DeaoProbe *probe = nullptr) {
+                int32_t *rnd = nullptr, DeaoProbe *probe = nullptr) {
     const mb3d_hip_params &p = m.p;
     double sw = p.step_width;
     double rot_v[3][3];
@@ -1636,7 +1682,9 @@ void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_h
         ms_de_stop = de_stop / (dsm * dsm);
     else
         ms_de_stop = ms_de_stop / (dsm * dsm);
-    int ray_count = (int)base.dirs.size();
+    const std::vector<std::array<float, 3>> dirs =
+        job.deao_dither > 0 ? dither_rays(base, job.deao_dither, x, y) : base.dirs;
+    int ray_count = (int)dirs.size();
     double de_mul = std::sqrt((double)ray_count * 0.5);
     double abr = 1.2 / std::asin(std::fmin(std::fmax(1.0 / de_mul, -1.0), 1.0));
     double md_d10 = 0.1 / (max_dist * de_mul);
@@ -1647,7 +1695,7 @@ void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_h
     normal_matrix(n, rot_w);
     std::vector<float> min_ra(ray_count, 0.0f);
     for (int k = 0; k < ray_count; k++) {
-        const float *d = base.dirs[k].data();
+        const float *d = dirs[k].data();
         float v[3] = {sdot(rot_w[0], d), sdot(rot_w[1], d), sdot(rot_w[2], d)};
         double vd[3] = {(double)v[0], (double)v[1], (double)v[2]};
         double sv[3];
@@ -1655,8 +1703,15 @@ void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_h
         double dt1 = step_ao * dsm;
         double s_tmp = 1.0;
         bool b_end = false;
+        bool b_first = job.deao_first_step_random != 0;
+        if (probe) probe->first[k] = 1.0;
         for (;;) {
-            if (dt1 > max_dist) {
+            if (b_first) {
+                b_first = false;
+                const double f = deao_rand(*rnd) * 1.5 + 0.5;
+                dt1 *= f;
+                if (probe) probe->first[k] = f;
+            } else if (dt1 > max_dist) {
                 dt1 = max_dist;
                 b_end = true;
             }
@@ -1691,7 +1746,7 @@ void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_h
             float max_add = 1.0f - min_ra[k];
             for (int k2 = 0; k2 < ray_count; k2++) {
                 if (k2 == k) continue;
-                float dt = sdot(base.dirs[k].data(), base.dirs[k2].data());
+                float dt = sdot(dirs[k].data(), dirs[k2].data());
                 if (dt > base.d_min_a_dif) {
                     float ac = std::acos(std::fmin(std::fmax(dt, -1.0f), 1.0f));
                     float overlap = min_ra[k2] - (float)((double)ac * abr) + 1.0f;
@@ -1730,6 +1785,13 @@ int check_params(const mb3d_hip_params *p) {
     return 0;
 }
 
+// The DE AO's dithering: 0..2, not with the 8-ray set
+int check_job(const mb3d_hip_post_job &job) {
+    if (job.deao && (job.deao_dither < 0 || job.deao_dither > 2)) return fail("bad DE AO dithering");
+    if (job.deao && job.deao_dither && job.deao_quality >= 4) return fail("no dithering of the 8-ray set");
+    return 0;
+}
+
 void run_post(const mb3d_hip_params &p, const mb3d_hip_post_job &job, int row0, int rows, std::vector<Px> &px,
               int threads) {
     int w = p.width;
@@ -1747,7 +1809,8 @@ void run_post(const mb3d_hip_params &p, const mb3d_hip_post_job &job, int row0, 
         for_rows_interleaved(rows, nthreads(threads), [&](int r) {
             Marcher m(pc);
             int y = row0 + r;
-            for (int x = 0; x < w; x++) deao_pixel(m, x, y, px[(size_t)r * w + x], base, job);
+            int32_t rnd = deao_row_seed(y);
+            for (int x = 0; x < w; x++) deao_pixel(m, x, y, px[(size_t)r * w + x], base, job, &rnd);
         });
     }
 }
@@ -2320,11 +2383,15 @@ extern "C" void orc_pixel_ray(const mb3d_hip_params *p, int32_t x, int32_t y, do
 // One pixel's DE AO (deao_pixel) on its card record `rec` (5 words), with
 // what it computed: out = the surface point (3), the ray steps in world
 // units (33 x 3), the openness per ray (33), then step_ao, max_dist, the
-// DE stop of the rays, md_d10, de_mul and d_step_mul; returns the ray
-// count (0: no surface), the amb word in *amb.
+// DE stop of the rays, md_d10, de_mul and d_step_mul, then each ray's first
+// step factor (33); returns the ray count (0: no surface), the amb word in
+// *amb.  `rnd`: the row generator's state before the pixel (advanced), for
+// the random first step.
 extern "C" int32_t orc_deao_probe(const mb3d_hip_params *pp, const mb3d_hip_post_job *job, int32_t x, int32_t y,
-                                  const uint32_t rec[5], double *out, int32_t *amb) {
+                                  const uint32_t rec[5], double *out, int32_t *amb, int32_t *rnd) {
     if (check_params(pp) || !job || !job->deao) return -1;
+    if (job->deao_first_step_random && !rnd) return -1;
+    if (check_job(*job)) return -1;
     mb3d_hip_params pc = *pp;
     pc.de_add_steps = 5;  // DeaoRows::new (src/deao.rs:254-260)
     Rays base = build_rays(job->deao_quality);
@@ -2333,7 +2400,7 @@ extern "C" int32_t orc_deao_probe(const mb3d_hip_params *pp, const mb3d_hip_post
     from_words(rec, si);
     DeaoProbe pr;
     std::memset(&pr, 0, sizeof pr);
-    deao_pixel(m, x, y, si, base, *job, &pr);
+    deao_pixel(m, x, y, si, base, *job, rnd, &pr);
     *amb = si.amb_shadow;
     for (int i = 0; i < 3; i++) out[i] = pr.ic[i];
     for (int k = 0; k < 33; k++)
@@ -2341,7 +2408,18 @@ extern "C" int32_t orc_deao_probe(const mb3d_hip_params *pp, const mb3d_hip_post
     for (int k = 0; k < 33; k++) out[102 + k] = pr.min_ra[k];
     const double info[6] = {pr.step_ao, pr.max_dist, pr.ms_ao, pr.md_d10, pr.de_mul, pr.dsm};
     for (int i = 0; i < 6; i++) out[135 + i] = info[i];
+    for (int k = 0; k < 33; k++) out[141 + k] = pr.first[k];
     return pr.count;
+}
+
+// The DE AO's row generator: the seed of row y, then n states and numbers
+extern "C" int32_t orc_deao_row_seed(int32_t y) { return deao_row_seed(y); }
+
+extern "C" void orc_deao_rand(int32_t seed, int32_t n, int32_t *states, double *vals) {
+    for (int i = 0; i < n; i++) {
+        vals[i] = deao_rand(seed);
+        states[i] = seed;
+    }
 }
 
 // create_xy_vecs_from_normals (src/math.rs:218-232), for the tests
@@ -3183,7 +3261,10 @@ int render_impl(const mb3d_hip_params *pp, const mb3d_hip_post_job *post, int32_
         c_box += m.it.its_box;
         c_oth += m.it.its_other;
     });
-    if (post) run_post(p, *post, row0, rows, px, threads);
+    if (post) {
+        if (int e = check_job(*post)) return e;
+        run_post(p, *post, row0, rows, px, threads);
+    }
     for (size_t i = 0; i < px.size(); i++) {
         const Px &s = px[i];
         if (out->depth) out->depth[i] = s.depth;
@@ -3211,6 +3292,7 @@ extern "C" int32_t orc_post(const mb3d_hip_params *pp, const mb3d_hip_post_job *
                             uint32_t *gbuf, int32_t threads) {
     if (int e = check_params(pp)) return e;
     if (!post || !gbuf) return fail("NULL argument");
+    if (int e = check_job(*post)) return e;
     const mb3d_hip_params &p = *pp;
     int w = p.width;
     std::vector<Px> px((size_t)rows * w);

@@ -79,13 +79,20 @@ int32_t orc_post(const mb3d_hip_params *p, const mb3d_hip_post_job *post, int32_
                  int32_t rows, uint32_t *gbuf, int32_t threads);
 
 /* One pixel's DE AO (deao_row, src/deao.rs:268-418) on its card record,
- * with what it computed, for the tests: out (141 doubles) = the rebuilt
+ * with what it computed, for the tests: out (174 doubles) = the rebuilt
  * surface point, each ray's world step (33 x 3), each ray's openness times
  * de_mul (33), then step_ao, max_dist, the rays' DE stop, md_d10, de_mul,
- * d_step_mul.  Returns the ray count (0: not a surface pixel); *amb the
- * amb_shadow word. */
+ * d_step_mul, then each ray's first step factor (33).  Returns the ray
+ * count (0: not a surface pixel); *amb the amb_shadow word.  `rnd`: the
+ * row generator's state before this pixel, advanced by its draws (needed
+ * with the random first step, else may be NULL). */
 int32_t orc_deao_probe(const mb3d_hip_params *p, const mb3d_hip_post_job *job, int32_t x, int32_t y,
-                       const uint32_t rec[5], double *out, int32_t *amb);
+                       const uint32_t rec[5], double *out, int32_t *amb, int32_t *rnd);
+
+/* The DE AO's row generator (DeaoRows::row, get_rand, src/deao.rs:262-270):
+ * the seed of picture row y; n draws from `seed`: the states and numbers. */
+int32_t orc_deao_row_seed(int32_t y);
+void orc_deao_rand(int32_t seed, int32_t n, int32_t *states, double *vals);
 
 /* create_xy_vecs_from_normals (src/math.rs:218-232): two vectors across
  * the normal n, for the smoothing sweeps of calculate_normals. */

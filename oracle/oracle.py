@@ -175,20 +175,38 @@ def de_many(p, pos, threads=0):
     return de, its
 
 
-def deao_probe(p, job, x, y, rec):
+def deao_probe(p, job, x, y, rec, rnd=None):
     """One pixel's DE AO on its card record (5 words) with what it computed:
     (ray count, amb word, dict of the surface point, each ray's world step,
-    each ray's openness times de_mul, and step_ao, max_dist, ms_ao, md_d10,
-    de_mul, dsm)."""
-    out = np.zeros(141)
+    each ray's openness times de_mul, each ray's first step factor, and
+    step_ao, max_dist, ms_ao, md_d10, de_mul, dsm).  `rnd`: the row
+    generator's state before the pixel (the random first step); the state
+    after it is in info["rnd"]."""
+    out = np.zeros(174)
     amb = C.c_int32()
     r = np.ascontiguousarray(rec, np.uint32)
-    n = load().orc_deao_probe(C.byref(p), C.byref(job), x, y, _ptr(r), _ptr(out), C.byref(amb))
+    st = C.c_int32(0 if rnd is None else int(np.int32(np.uint32(rnd & 0xFFFFFFFF))))
+    n = load().orc_deao_probe(C.byref(p), C.byref(job), x, y, _ptr(r), _ptr(out), C.byref(amb),
+                              None if rnd is None else C.byref(st))
     if n < 0:
         raise RuntimeError("orc_deao_probe failed")
     info = dict(zip(("step_ao", "max_dist", "ms_ao", "md_d10", "de_mul", "dsm"), out[135:141]))
-    info.update(point=out[:3], dirs=out[3:102].reshape(33, 3)[:n], min_ra=out[102:135][:n])
+    info.update(point=out[:3], dirs=out[3:102].reshape(33, 3)[:n], min_ra=out[102:135][:n],
+                first=out[141:174][:n], rnd=st.value)
     return n, amb.value, info
+
+
+def deao_row_seed(y):
+    """The DE AO generator's seed of picture row y."""
+    return load().orc_deao_row_seed(y)
+
+
+def deao_rand(seed, n):
+    """n draws of the DE AO's generator from `seed`: (states, numbers)."""
+    states = np.zeros(n, np.int32)
+    vals = np.zeros(n)
+    load().orc_deao_rand(int(np.int32(np.uint32(seed & 0xFFFFFFFF))), n, _ptr(states), _ptr(vals))
+    return states, vals
 
 
 def xy_vecs(n):

@@ -375,26 +375,50 @@ def _rodrigues(a, b):
     return np.eye(3) + s * K + (1.0 - c) * K @ K
 
 
-def _ao_set(quality):
+def _ao_set(quality, dt1=None, dt2=0.0):
     """The DE AO ray set of a quality in the local frame of a surface whose
     normal is (0, 0, -1) (the view looks along +z): the straight ray, then
     rings at polar angles y abr (abr = (pi / 2) / (quality + 0.9)), each
     with round(sin(y abr) 2 pi / abr) rays evenly spaced in azimuth from the
-    +x side (src/deao.rs:75-105); quality 0: three rays at 30 degrees."""
+    +x side (src/deao.rs:75-105); quality 0: three rays at 30 degrees.
+    Dithered (dt1, dt2 = the pixel's place in its cell times 1/2 / dither):
+    the rings at (y + dt1 - 1/4) abr and azimuths shifted by dt2 of a gap,
+    the straight ray dropped when dt1 > 0.1; quality 0 at (dt1 + 1/2) 50
+    degrees (src/deao.rs:318-342)."""
     out = []
     if quality == 0:
-        rings, abr = [(np.pi / 6.0, 3)], None
+        polar = np.pi / 6.0 if dt1 is None else (dt1 + 0.5) * np.radians(50.0)
+        rings, abr = [(polar, 3)], None
     else:
-        out.append((0.0, 0.0))
         abr = (np.pi / 2.0) / (quality + 0.9)
-        rings = [(y * abr, int(round(np.sin(y * abr) * 2 * np.pi / abr))) for y in range(1, quality + 1)]
+        if dt1 is None or dt1 <= 0.1:
+            out.append((0.0, 0.0))
+        shift = 0.0 if dt1 is None else dt1 - 0.25
+        rings = [((y + shift) * abr, int(round(np.sin(y * abr) * 2 * np.pi / abr))) for y in range(1, quality + 1)]
     for polar, n in rings:
-        out += [(polar, 2 * np.pi * x / n) for x in range(n)]
+        out += [(polar, 2 * np.pi * (x + dt2) / n) for x in range(n)]
     return np.array([[np.sin(a) * np.cos(z), -np.sin(a) * np.sin(z), -np.cos(a)] for a, z in out])
 
 
-@pytest.mark.parametrize("quality", [1, 2])
-def test_deao_against_brute_force(quality):
+def _object(g):
+    """The G-buffer records that are surface pixels (zpos and gradient
+    words below 32768)."""
+    return ((g[:, 2] >> 16) < 32768) & ((g[:, 3] & 0xFFFF) < 32768)
+
+
+def _row_state(p, g, x, y, draws):
+    """The DE AO generator's state before pixel (x, y): the row's seed
+    advanced by `draws` numbers for each surface pixel left of it."""
+    before = int(_object(g[y * p.width: y * p.width + x]).sum())
+    seed = O.deao_row_seed(y)
+    if before == 0:
+        return seed
+    return int(O.deao_rand(seed, before * draws)[0][-1])
+
+
+@pytest.mark.parametrize("quality,dither,rnd", [(1, 0, False), (2, 0, False), (1, 1, True), (2, 2, False),
+                                               (0, 1, True), (3, 2, True)])
+def test_deao_against_brute_force(quality, dither, rnd):
     """deao_row (src/deao.rs:268-418) per pixel, on a sample of surface
     pixels, against what it must compute:
       - the rebuilt surface point (surface_point, src/calc.rs:2157-2192)
@@ -410,25 +434,36 @@ def test_deao_against_brute_force(quality):
         (the pass's samples are sparser: median gap 0.006-0.009 here; a
         dropped DE stop makes it 0.06);
       - a pixel with every ray open has amb 0.
-    A swapped axis of the AO frame, a sign of its quaternion or a dropped DE
-    stop fails here; the walk's step rule is pinned only through the
-    openness it reaches (a doubled step passes)."""
+    Dithered, the set of the pixel's cell; with the random first step, each
+    ray's first sample at 0.5..2 of the plain first step, the walk starting
+    there (not held to the ray's length).
+    A swapped axis of the AO frame, a sign of its quaternion, a dropped DE
+    stop or swapped dither offsets fails here; the walk's step rule is
+    pinned only through the openness it reaches (a doubled step passes)."""
     sc = preset("Integer Power", 64, 48)
     sc.first_step_random = 0
     p = O.params(sc)
-    job = H.post_job(deao_quality=quality)
+    job = H.post_job(deao_quality=quality, deao_dither=dither, deao_first_step_random=rnd)
     r = O.render(p, job, threads=0)
     vg = np.array([[p.vgrads[i][k] for k in range(3)] for i in range(3)])
     axes = vg / np.linalg.norm(vg, axis=1, keepdims=True)
-    base = _ao_set(quality)
     surf = np.flatnonzero(np.isfinite(r["depth"]) & (r["depth"] > 0))
-    rng = np.random.default_rng(quality)
+    rng = np.random.default_rng(quality + 10 * dither)
     gaps = []
     for i in rng.choice(surf, size=min(30, surf.size), replace=False):
         x, y = int(i % p.width), int(i // p.width)
         rec = r["gbuf"][i]
-        n, amb, pr = O.deao_probe(p, job, x, y, rec)
+        if dither:
+            base = _ao_set(quality, (y % (dither + 1)) * 0.5 / dither, (x % (dither + 1)) * 0.5 / dither)
+        else:
+            base = _ao_set(quality)
+        st = _row_state(p, r["gbuf"], x, y, len(base)) if rnd else None
+        n, amb, pr = O.deao_probe(p, job, x, y, rec, rnd=st)
         assert n == len(base) and amb == (rec[4] & 0xFFFF)
+        if rnd:
+            assert np.all((pr["first"] >= 0.5) & (pr["first"] <= 2.0)) and np.ptp(pr["first"]) > 0
+        else:
+            assert np.all(pr["first"] == 1.0)
         c, v = O.pixel_ray(p, x, y)
         assert np.linalg.norm(pr["point"] - (c + r["depth"][i] * v)) < 0.05 * p.step_width
         nq = np.array([np.int16(rec[0] & 0xFFFF), np.int16(rec[0] >> 16), np.int16(rec[1] & 0xFFFF)], float)
@@ -441,7 +476,8 @@ def test_deao_against_brute_force(quality):
         assert np.allclose(np.linalg.norm(pr["dirs"], axis=1), p.step_width, rtol=1e-6)
         # the openness of each ray against a fine walk of the DE along it
         for k in range(n):
-            t = np.arange(pr["step_ao"] * pr["dsm"], pr["max_dist"] + 1e-9, 0.05)
+            t0 = pr["step_ao"] * pr["dsm"] * pr["first"][k]
+            t = np.concatenate([np.arange(t0, pr["max_dist"] + 1e-9, 0.05), [t0, pr["max_dist"]]])
             de, _ = O.de_many(p, pr["point"][None, :] + np.outer(t, got[k]) * p.step_width)
             walk = min(1.0, float(np.min((de - pr["ms_ao"] + t * pr["md_d10"]) / t)))
             walk = max(walk, 0.0)
@@ -453,3 +489,90 @@ def test_deao_against_brute_force(quality):
     gaps = np.abs(np.array(gaps))
     assert np.median(gaps) <= 0.025 and (gaps < 0.02).mean() >= 0.5, (np.median(gaps), (gaps < 0.02).mean())
     assert gaps.max() <= 0.3, gaps.max()
+
+
+def test_deao_generator_is_the_c_runtime_lcg():
+    """get_rand's generator (src/deao.rs:262-265) is the linear congruential
+    generator of the Microsoft C runtime's rand(): state' = 214013 state +
+    2531011 mod 2^32 (0x343FD, 0x269EC3), whose outputs, bits 16..30 of the
+    state, from srand(1) are the published 41, 18467, 6334, 26500, ...; the
+    AO's number is bits 8..30 of the state over 2^23 - 1, and a row's seed
+    is 0x24563487 + (y + 1) 0x324594A1 wrapped to 32 bits
+    (src/deao.rs:254-262)."""
+    states, vals = O.deao_rand(1, 10)
+    u = states.astype(np.int64) & 0xFFFFFFFF
+    assert list((u >> 16) & 0x7FFF) == [41, 18467, 6334, 26500, 19169, 15724, 11478, 29358, 26962, 24464]
+    assert np.array_equal(vals * 0x7FFFFF, ((u >> 8) & 0x7FFFFF).astype(float))
+    assert vals.min() >= 0.0 and vals.max() <= 1.0
+    for y in (0, 1, 1079, 4000):
+        assert O.deao_row_seed(y) & 0xFFFFFFFF == (0x24563487 + (y + 1) * 0x324594A1) & 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("dither", [0, 1])
+def test_deao_random_first_steps_draw_along_the_row(dither):
+    """deao_row draws the random first steps from one generator per row,
+    each surface pixel left to right taking one number per ray
+    (src/deao.rs:262-270, 348-351): every surface pixel of three rows gets
+    the amb word of the probe started from the row's seed advanced over the
+    pixels before it, and its rays' factors are 1.5 u + 0.5 of the draws.
+    A generator seeded per pixel, or a draw skipped on a row, fails here."""
+    sc = preset("Integer Power", 64, 48)
+    sc.first_step_random = 0
+    p = O.params(sc)
+    job = H.post_job(deao_quality=1, deao_dither=dither, deao_first_step_random=True)
+    r = O.render(p, job, threads=0)
+    plain = O.render(p, H.post_job(deao_quality=1, deao_dither=dither), threads=0)
+    g = r["gbuf"]
+    obj = _object(g)
+    amb = g[:, 4] & 0xFFFF
+    # the random steps change the picture
+    assert (amb[obj] != (plain["gbuf"][:, 4] & 0xFFFF)[obj]).mean() > 0.2
+    checked = 0
+    for y in (10, 24, 37):
+        st = O.deao_row_seed(y)
+        for x in range(p.width):
+            i = y * p.width + x
+            if not obj[i]:
+                continue
+            n, a, pr = O.deao_probe(p, job, x, y, g[i], rnd=st)
+            assert a == amb[i], (x, y)
+            _, u = O.deao_rand(st, n)
+            assert np.array_equal(pr["first"], u * 1.5 + 0.5)
+            st = pr["rnd"]
+            checked += 1
+    assert checked > 20
+
+
+def test_deao_random_first_step_past_the_ray_end():
+    """With rays shorter than their first step (deao_max_len 0.02), a ray
+    samples its random first step where it falls, not held to the ray's
+    length, and then the ray's end (src/deao.rs:348-356): its openness is
+    min(1, o(t0), o(max_dist)) with o(t) = (DE(t) - stop + t md_d10) / t,
+    the second sample skipped when the first is under 0.02."""
+    sc = preset("Integer Power", 64, 48)
+    sc.first_step_random = 0
+    p = O.params(sc)
+    job = H.post_job(deao_quality=1, deao_first_step_random=True)
+    job.deao_max_len = 0.02
+    r = O.render(p, job, threads=0)
+    surf = np.flatnonzero(np.isfinite(r["depth"]) & (r["depth"] > 0))
+    past = 0
+    for i in np.random.default_rng(5).choice(surf, size=12, replace=False):
+        x, y = int(i % p.width), int(i // p.width)
+        st = _row_state(p, r["gbuf"], x, y, 7)
+        n, amb, pr = O.deao_probe(p, job, x, y, r["gbuf"][i], rnd=st)
+        assert amb == (r["gbuf"][i][4] & 0xFFFF)
+        unit = pr["dirs"] / np.linalg.norm(pr["dirs"], axis=1, keepdims=True)
+        for k in range(n):
+            t0 = pr["step_ao"] * pr["dsm"] * pr["first"][k]
+            if t0 <= pr["max_dist"]:
+                continue
+            past += 1
+            t = np.array([t0, pr["max_dist"]])
+            de, _ = O.de_many(p, pr["point"][None, :] + np.outer(t, unit[k]) * p.step_width)
+            o = (de - pr["ms_ao"] + t * pr["md_d10"]) / t
+            s = min(1.0, o[0])
+            if s >= 0.02:
+                s = min(s, o[1])
+            assert abs(pr["min_ra"][k] / pr["de_mul"] - max(s, 0.0)) < 1e-6, (x, y, k)
+    assert past > 20
