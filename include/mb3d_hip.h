@@ -34,7 +34,10 @@ extern "C" {
 #define MB3D_HIP_OK 0
 #define MB3D_HIP_CANCELLED 1      /* the cancel callback asked to stop ("cancelled") */
 #define MB3D_HIP_UNSUPPORTED 2    /* the scene needs a feature this backend lacks: run the CPU path */
-#define MB3D_HIP_OVERFLOW 3       /* > 1/500 of the pixels overflowed single precision (src/gpu.rs:2350-2375) */
+#define MB3D_HIP_OVERFLOW 3       /* the reference's verdict for > 1/500 of the pixels overflowing single
+                                     precision (src/gpu.rs:2350-2375).  Never returned here: a ray whose FP32
+                                     DE is not finite goes on in the f64 finish, which decides as the CPU does
+                                     (counted in mb3d_hip_counters.overflow) */
 #define MB3D_HIP_ERR_INVALID (-1) /* bad argument */
 #define MB3D_HIP_ERR_DEVICE (-2)  /* HIP runtime error, no device */
 #define MB3D_HIP_ERR_NOT_INIT (-3)

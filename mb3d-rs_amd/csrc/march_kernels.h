@@ -893,8 +893,12 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
     chunk_init(fq);
     chunk_init(sq);
     for (;;) {
+        // refill as the FP32 march does: once refill_min lanes are idle, or
+        // every lane that still has work to get is
         bool need = !have && !exhausted;
-        for (int tries = 0; tries < 4 && __any(need); tries++) {
+        const int idle = __popcll(__ballot(need));
+        const bool refill = idle > 0 && (idle >= P0.refill_min || idle == __popcll(__ballot(!exhausted)));
+        for (int tries = 0; refill && tries < 4 && __any(need); tries++) {
             const long long slot = wave_take(need, &B.ctr[C_FINQ], fq, 64, n_items);
             if (!need) continue;
             if (slot < 0) {
@@ -1103,8 +1107,14 @@ __global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__
             R.ms = o.ms;
             R.calc_sit = true;
             const double noffset = (double)fminf(P.de_stop, 1.0f) * (1.0 + R.mzz * (double)P.de_stop_factor) * 0.15;
-            R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
-            const float nn = R.smooth_it;
+            // the centre DE gives the smooth iterations and the trap of the
+            // G-buffer's colour words: without a G-buffer output, nothing
+            // reads them
+            float nn = 0.0f;
+            if (B.gbuf) {
+                R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+                nn = R.smooth_it;
+            }
             R.calc_sit = false;
             for (int q = 0; q < 3; q++) {
                 const int axis = q == 0 ? 2 : q - 1;  // the order vg2, vg0, vg1
