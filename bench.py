@@ -85,20 +85,30 @@ def parse():
     return ap.parse_args()
 
 
-def flops_f32(c: dict) -> float:
-    """The FP32 march's: its iterations, DEs and steps (the normals pass's
-    FP32 stencil -- its_normals_f32, six DEs per hit -- counts with the
-    finish)."""
+def flops_march(c: dict) -> float:
+    """k_march32's: the FP32 iterations, DEs and steps (less the normals
+    pass's FP32 stencil -- its_normals_f32, six DEs per hit -- and the f64
+    finish's steps)."""
     its = c["its_p8_f32"] - c["its_normals_f32"]
-    return its * F_P8 + c["its_box_f32"] * F_BOX + (c["de_evals_f32"] - 6 * c["hits"]) * F_DE + c["steps"] * F_STEP
+    return (its * F_P8 + c["its_box_f32"] * F_BOX + (c["de_evals_f32"] - 6 * c["hits"]) * F_DE +
+            (c["steps"] - c["steps_f64"]) * F_STEP)
 
 
-def flops_f64(c: dict) -> float:
-    """The cone rays', finish's and normals passes': the f64 iterations and
-    DEs, and the normals' FP32 stencil (unpacked FP32 issues at the FP64
-    rate)."""
-    return (c["its_p8_f64"] * F_P8 + c["its_box_f64"] * F_BOX + c["de_evals_f64"] * F_DE +
-            c["its_normals_f32"] * F_P8 + 6 * c["hits"] * F_DE)
+def flops_cone(c: dict) -> float:
+    """k_cone's: the cone rays' f64 iterations and DEs."""
+    return c["its_cone_f64"] * F_P8 + c["de_cone_f64"] * F_DE
+
+
+def flops_finish(c: dict) -> float:
+    """k_finish64's: the f64 iterations, DEs and steps, less the cone rays'."""
+    return ((c["its_p8_f64"] - c["its_cone_f64"]) * F_P8 + c["its_box_f64"] * F_BOX +
+            (c["de_evals_f64"] - c["de_cone_f64"]) * F_DE + c["steps_f64"] * F_STEP)
+
+
+def flops_normals(c: dict) -> float:
+    """k_normals's FP32 stencil: six DEs per surface pixel (without a
+    G-buffer output the centre DE is not made)."""
+    return c["its_normals_f32"] * F_P8 + 6 * c["hits"] * F_DE
 
 
 def with_opts(base, mlp=None, cone=False):
@@ -212,24 +222,30 @@ def main():
     # batch launched eagerly on the library's stream (a captured graph does
     # not time its nodes), averaged
     lo = legs["learned_cone_batched"]
-    tms = {"mlp": [], "cone": [], "march": [], "finish": []}
+    tms = {"mlp": [], "cone": [], "march": [], "finish": [], "normals": []}
     cf = None
     for _ in range(max(3, a.steps)):
         H.check(lib.mb3d_hip_march_batch(frames, a.batch, H.C.byref(out), H.C.byref(lo)), "march_batch")
         cf = H.counters()
-        for k, f in (("mlp", "mlp_ms"), ("cone", "cone_ms"), ("march", "march_f32_ms"), ("finish", "finish_f64_ms")):
+        for k, f in (("mlp", "mlp_ms"), ("cone", "cone_ms"), ("march", "march_f32_ms"), ("finish", "finish_f64_ms"),
+                     ("normals", "normals_ms")):
             tms[k].append(cf[f])
     t_k = {k: float(np.mean(v)) for k, v in tms.items()}
+    t_k["finish"] -= t_k["normals"]  # (finish_f64_ms holds both passes)
     items = sum(((s.width + lo.tile_w - 1) // lo.tile_w) * lo.tile_w * ((s.height + lo.tile_h - 1) // lo.tile_h) *
                 lo.tile_h for s in scenes)
     kernels = [
         {"kernel": "k_march32 (batch)", "bound": "valu-fp32", "kernel_ms": t_k["march"],
-         "flops_per_launch": flops_f32(cf), "peak": PEAK_FP32 / 1e12},
+         "flops_per_launch": flops_march(cf), "peak": PEAK_FP32 / 1e12},
+        {"kernel": "k_finish64 (batch)", "bound": "valu-fp64", "kernel_ms": t_k["finish"],
+         "flops_per_launch": flops_finish(cf), "peak": PEAK_FP64 / 1e12},
+        {"kernel": "k_normals (batch)", "bound": "valu-fp32", "kernel_ms": t_k["normals"],
+         "flops_per_launch": flops_normals(cf), "peak": PEAK_FP32 / 1e12},
         {"kernel": "k_mlp_propose (batch)", "bound": "mfma-bf16", "kernel_ms": t_k["mlp"],
          "flops_per_launch": F_MLP * cf["mlp_rays"], "peak": PEAK_BF16 / 1e12, "rays": cf["mlp_rays"],
          "items": items},
-        {"kernel": "k_cone + k_finish64 + k_normals (batch)", "bound": "valu-fp64",
-         "kernel_ms": t_k["cone"] + t_k["finish"], "flops_per_launch": flops_f64(cf), "peak": PEAK_FP64 / 1e12},
+        {"kernel": "k_cone (batch)", "bound": "valu-fp64", "kernel_ms": t_k["cone"],
+         "flops_per_launch": flops_cone(cf), "peak": PEAK_FP64 / 1e12, "rays": cf["cone_rays"]},
     ]
     for k in kernels:
         k["achieved"] = k["flops_per_launch"] / (k["kernel_ms"] * 1e-3) / 1e12

@@ -393,6 +393,10 @@ typedef struct mb3d_hip_counters {
     double mlp_ms;           /* of kernel_ms: the warm start's network (k_mlp_propose) */
     double cone_ms;          /* of kernel_ms: the cone rays (k_cone) */
     uint64_t mlp_rays;       /* rays the network ran on (tiles the cone rays found empty skip it) */
+    uint64_t its_cone_f64;   /* of the FP64 iterations: the cone rays' (k_cone) */
+    uint64_t de_cone_f64;    /* of de_evals_f64: the cone rays' */
+    double normals_ms;       /* of finish_f64_ms: the normals pass (k_normals) */
+    uint64_t steps_f64;      /* of steps: the f64 finish's (k_finish64) */
 } mb3d_hip_counters;
 
 typedef void (*mb3d_hip_progress_fn)(void *user, int64_t done, int64_t total);

@@ -112,6 +112,8 @@ struct Cnt {
     uint32_t hits = 0, overflow = 0;
     uint32_t jumps = 0, rollbacks = 0;  // warm starts kept, warm starts undone
     uint32_t cone_rays = 0, cone_bg = 0;  // cone rays marched; pixels their cone made background
+    uint32_t cone_its = 0, cone_des = 0;  // of the f64 iterations and DEs: the cone rays'
+    uint32_t steps64 = 0;                 // of the steps: the f64 finish's
 };
 
 // ---------------------------------------------------------------------------

@@ -82,9 +82,9 @@ struct Ctx {
     int cus = 0;
     hipStream_t stream = nullptr;
     // ev[0] start, ev[4] after the ray tables, ev[5] after the cone rays,
-    // ev[6] after the network, ev[3] after the FP32 march, ev[1] after the
-    // finish and normals, ev[2] after the post passes
-    hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // ev[6] after the network, ev[3] after the FP32 march, ev[7] after the
+    // finish, ev[1] after the normals, ev[2] after the post passes
+    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // grow-only device buffers
     void *d_out = nullptr;
     size_t out_bytes = 0;
@@ -857,6 +857,9 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
             HIP_TRY(hipEventRecord(c.ev[3], c.stream));
             dispatch(kind, dem, [&](auto K, auto D) {
                 launch_finish<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, 0);
+            });
+            HIP_TRY(hipEventRecord(c.ev[7], c.stream));
+            dispatch(kind, dem, [&](auto K, auto D) {
                 launch_normals<decltype(K)::value, decltype(D)::value>(grid_flat, stream, d, b, fp32_stencil_ok(p),
                                                                        nullptr, 1, ext_modes(p));
             });
@@ -933,12 +936,13 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     unsigned long long ctr[C_WORDS];
     HIP_TRY(hipMemcpyAsync(ctr, ctr_dev, sizeof ctr, hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(hipStreamSynchronize(c.stream));
-    float ms_march = 0, ms_post = 0, ms_a = 0, ms_mlp = 0, ms_cone = 0;
+    float ms_march = 0, ms_post = 0, ms_a = 0, ms_mlp = 0, ms_cone = 0, ms_nrm = 0;
     HIP_TRY(hipEventElapsedTime(&ms_march, c.ev[0], c.ev[1]));
     HIP_TRY(hipEventElapsedTime(&ms_post, c.ev[1], c.ev[2]));
     const bool split = !post_only && !f64;
     if (split) {
         HIP_TRY(hipEventElapsedTime(&ms_a, c.ev[0], c.ev[3]));
+        HIP_TRY(hipEventElapsedTime(&ms_nrm, c.ev[7], c.ev[1]));
         float ms_tab = 0;
         HIP_TRY(hipEventElapsedTime(&ms_tab, c.ev[0], c.ev[4]));
         HIP_TRY(hipEventElapsedTime(&ms_cone, c.ev[4], c.ev[5]));
@@ -966,8 +970,12 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     t.cone_rays = ctr[C_STATS + 13];
     t.cone_background = ctr[C_STATS + 14];
     t.mlp_rays = ctr[C_STATS + 15];
+    t.its_cone_f64 = ctr[C_STATS + 16];
+    t.de_cone_f64 = ctr[C_STATS + 17];
+    t.steps_f64 = ctr[C_STATS + 18];
     t.kernel_ms = ms_march;
     t.post_ms = ms_post;
+    t.normals_ms = split ? ms_nrm : 0.0;
     t.mlp_ms = split ? ms_mlp : 0.0;
     t.cone_ms = split ? ms_cone : 0.0;
     t.march_f32_ms = split ? ms_a - ms_mlp - ms_cone : 0.0;
@@ -996,8 +1004,8 @@ struct Plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // e0 start, ea after the ray tables and the cone rays, eb after the
-    // network, e1 after the FP32 march, e2 end
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, ea = nullptr, eb = nullptr;
+    // network, e1 after the FP32 march, ef after the finish, e2 end
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, ea = nullptr, eb = nullptr, ef = nullptr;
     MlpDev M;  // the warm start (M.enabled = 0: none)
     bool n32 = true;  // the normals' stencil in FP32 (every frame allows it)
     bool ext = false;  // a frame asks for the general surface modes (k_normals_ext)
@@ -1015,6 +1023,7 @@ void plan_free(Plan *pl) {
     if (pl->e2) (void)hipEventDestroy(pl->e2);
     if (pl->ea) (void)hipEventDestroy(pl->ea);
     if (pl->eb) (void)hipEventDestroy(pl->eb);
+    if (pl->ef) (void)hipEventDestroy(pl->ef);
     delete pl;
 }
 
@@ -1042,6 +1051,9 @@ int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
     if (split) HIP_TRY(hipEventRecord(pl.e1, stream));
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
         launch_finish<decltype(K)::value, decltype(D)::value, true>(pl.grid_flat, stream, pl.P0, pl.b, 0, tab, n);
+    });
+    if (split) HIP_TRY(hipEventRecord(pl.ef, stream));
+    dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
         launch_normals<decltype(K)::value, decltype(D)::value, true>(pl.grid_flat, stream, pl.P0, pl.b, pl.n32, tab,
                                                                      n, pl.ext);
     });
@@ -1119,7 +1131,7 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
         return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipMemcpy failed for a batch"));
     if (hipEventCreate(&pl->e0) != hipSuccess || hipEventCreate(&pl->e1) != hipSuccess ||
         hipEventCreate(&pl->e2) != hipSuccess || hipEventCreate(&pl->ea) != hipSuccess ||
-        hipEventCreate(&pl->eb) != hipSuccess)
+        hipEventCreate(&pl->eb) != hipSuccess || hipEventCreate(&pl->ef) != hipSuccess)
         return fail_free(set_error(MB3D_HIP_ERR_DEVICE, "hipEventCreate failed"));
     std::memset(&pl->b, 0, sizeof pl->b);
     pl->b.depth = out->depth;
@@ -1166,10 +1178,11 @@ int plan_run(Plan &pl, int repeats) {
     unsigned long long ctr[C_WORDS];
     HIP_TRY(hipMemcpyAsync(ctr, pl.d_ctr, sizeof ctr, hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(hipStreamSynchronize(c.stream));
-    float ms_all = 0, ms_a = -1.0f, ms_mlp = -1.0f, ms_cone = -1.0f;
+    float ms_all = 0, ms_a = -1.0f, ms_mlp = -1.0f, ms_cone = -1.0f, ms_nrm = -1.0f;
     if (hipEventElapsedTime(&ms_all, pl.e0, pl.e2) != hipSuccess) ms_all = -1.0f;
     if (!pl.exec && repeats == 1) {
         if (hipEventElapsedTime(&ms_a, pl.e0, pl.e1) != hipSuccess) ms_a = -1.0f;
+        if (hipEventElapsedTime(&ms_nrm, pl.ef, pl.e2) != hipSuccess) ms_nrm = -1.0f;
         if (hipEventElapsedTime(&ms_cone, pl.e0, pl.ea) != hipSuccess) ms_cone = -1.0f;
         if (hipEventElapsedTime(&ms_mlp, pl.ea, pl.eb) != hipSuccess) ms_mlp = -1.0f;
     }
@@ -1195,7 +1208,11 @@ int plan_run(Plan &pl, int repeats) {
     t.cone_rays = ctr[C_STATS + 13];
     t.cone_background = ctr[C_STATS + 14];
     t.mlp_rays = ctr[C_STATS + 15];
+    t.its_cone_f64 = ctr[C_STATS + 16];
+    t.de_cone_f64 = ctr[C_STATS + 17];
+    t.steps_f64 = ctr[C_STATS + 18];
     t.kernel_ms = ms_all > 0.0f ? ms_all / repeats : -1.0;
+    t.normals_ms = ms_nrm >= 0.0f ? ms_nrm : -1.0;
     const bool split = ms_a > 0.0f && ms_mlp >= 0.0f && ms_cone >= 0.0f;
     t.mlp_ms = split ? ms_mlp : -1.0;
     t.cone_ms = split ? ms_cone : -1.0;

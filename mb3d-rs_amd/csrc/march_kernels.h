@@ -68,8 +68,8 @@ enum : int {
     C_POSTQ = 4,   // next item of the shadow pass
     C_AOQ = 5,     // next item of the DE AO pass
     C_SURF = 6,    // refined surface list length (the normals pass's items)
-    C_STATS = 8,   // 16 work counters from here (the last: the network's rays)
-    C_WORDS = 24
+    C_STATS = 8,   // 19 work counters from here (the 16th: the network's rays)
+    C_WORDS = 27
 };
 
 struct Bufs {
@@ -153,13 +153,14 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 }
 
 __device__ inline void flush_counters(const Cnt &c, unsigned long long *ctr) {
-    const unsigned long long v[15] = {c.steps,  c.de32, c.de64,     c.p8_32, c.box_32,    c.p8_64, c.box_64,
+    const unsigned long long v[18] = {c.steps,  c.de32, c.de64,     c.p8_32, c.box_32,    c.p8_64, c.box_64,
                                       c.hits,   c.overflow, c.jumps, c.rollbacks, c.oth_64, c.nrm_32,
-                                      c.cone_rays, c.cone_bg};
+                                      c.cone_rays, c.cone_bg, c.cone_its, c.cone_des, c.steps64};
 #pragma unroll
-    for (int k = 0; k < 15; k++) {
+    for (int k = 0; k < 18; k++) {
         const unsigned long long s = wave_sum(v[k]);
-        if (lane_id() == 0 && s) atomicAdd(&ctr[C_STATS + k], s);
+        // (word 15 is the network's ray count, k_mlp_propose's own)
+        if (lane_id() == 0 && s) atomicAdd(&ctr[C_STATS + (k < 15 ? k : k + 1)], s);
     }
 }
 
@@ -526,6 +527,8 @@ __global__ __launch_bounds__(256) void k_cone(DevP P0, const DevP *__restrict__ 
         }
         out[g] = ConeRes{t, count, 0};
     }
+    cnt.cone_its = cnt.p8_64 + cnt.box_64 + cnt.oth_64;
+    cnt.cone_des = cnt.de64;
     flush_counters(cnt, ctr);
 }
 
@@ -1035,6 +1038,7 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
             }
             nsteps++;
             cnt.steps++;
+            cnt.steps64++;
             last_step = d;
             for (int k = 0; k < 3; k++) R.c[k] += R.v[k] * d;
             R.upd(P);

@@ -260,6 +260,10 @@ class Counters(C.Structure):
         ("mlp_ms", C.c_double),
         ("cone_ms", C.c_double),
         ("mlp_rays", C.c_uint64),
+        ("its_cone_f64", C.c_uint64),
+        ("de_cone_f64", C.c_uint64),
+        ("normals_ms", C.c_double),
+        ("steps_f64", C.c_uint64),
     ]
 
 

@@ -11,23 +11,37 @@
 //   distance estimate  calc_de_part             src/calc.rs:1495-1559
 //   march              march_pixel_from, march_from  src/calc.rs:2231-2322,2533-2657
 //   bisection          bin_search, bin_search_it     src/calc.rs:1706-1792
-//   normals            calculate_normals (smooth 0)  src/calc.rs:1794-1858
-//   surface            surface_found, calc_zpos_and_rough  src/calc.rs:2194-2205,2471-2517
+//   normals            calculate_normals, smoothing sweeps, calc_roughness,
+//                      calculate_normals_on_smooth_it   src/calc.rs:1794-2006
+//   surface            surface_found, do_color, do_color_on_it,
+//                      calc_zpos_and_rough  src/calc.rs:2010-2090,2194-2205,2471-2517
+//   cone march         cone_march, Grid::safe_depth (one level of blocks)
+//                      src/calc.rs:2357-2435, src/cone.rs:66-115
 //   hard shadows       hard_shadow_row, per pixel (no row shortcut)  src/calc.rs:2741-2899
-//   DE AO              deao_row, surface_point   src/deao.rs:268-418, src/calc.rs:2157-2192
+//   DE AO              deao_row with dithering and the random first step,
+//                      surface_point   src/deao.rs:254-418, src/calc.rs:2157-2192
 //   secondary rays     SrTracer::march_ray, calc_hs, deao_1pos  src/reflect.rs:805-1090,1464-1674
 //   Monte Carlo        McMarch::run (ray, shadow, through)      src/mc.rs:488-977
 // Built with -ffp-contract=off: no fused multiply-adds, like the reference.
 //
 // Every function here is checked against the reference's known answers,
 // closed forms or brute force in tests/.  The DE AO (deao_row) per pixel:
-// its surface point against the march's depth, its ray directions against
-// a Rodrigues rotation of the ray set onto the normal, its openness against
-// a fine DE walk along each ray (test_deao_against_brute_force); the hit
-// pass's deao_1pos against deao_row on view rays (test_oracle_rays.py).  The
-// DE's size against the brute-force distance to the set, analytic and
-// numeric (test_oracle.py).  The AO walk's step rule (dt += max(DE, dt dsm))
-// is pinned only through the openness it reaches: a doubled step passes.
+// its surface point against the march's depth, its ray directions
+// (dithered too) against a Rodrigues rotation of the ray set onto the
+// normal, its openness against a fine DE walk along each ray from the
+// (random) first step (test_deao_against_brute_force); its generator
+// against the C runtime's rand() outputs and the row's draw order
+// (test_deao_random_first_steps_draw_along_the_row); the hit pass's
+// deao_1pos against deao_row on view rays (test_oracle_rays.py).  The DE's
+// size against the brute-force distance to the set, analytic and numeric
+// (test_oracle.py).  The smoothing sweeps and normals on the smooth
+// iteration against the DE gradient (test_oracle_normals.py); the cone
+// march against the reference's own cone tests (test_oracle_cone.py).
+// Parity unpinned: the AO walk's step rule (dt += max(DE, dt dsm)) is
+// pinned only through the openness it reaches (a doubled step passes);
+// colour options 1..5 and colour on an iteration (do_color,
+// do_color_on_it) are restated only: their words are checked for being
+// filled and differing from option 0, not against independent values.
 #include "mb3d_oracle.h"
 
 #include <algorithm>
