@@ -1122,43 +1122,58 @@ def counters() -> dict:
     return {k: getattr(c, k) for k, _ in Counters._fields_}
 
 
-def scene_deepzoom(width, height, seed, marcher, distance=1e-9, fov_deg=0.5) -> Scene:
-    """The deep zoom on the hybrid chain (Mandelbulb p8 x1, Amazing Box
-    scale 2 x2, 45 iterations): a coarse 64 x 48 march of the seeded hybrid
-    camera finds a surface point (a seeded surface pixel); the camera's start
-    plane sits `distance` scene units in front of it along that pixel's ray,
-    FOV `fov_deg`, the step width matched to the angular pixel footprint
-    there.  `marcher(params) -> f64 depth array` runs the coarse march (the
-    same f64 march on either side of a comparison)."""
-    coarse = scene_hybrid(64, 48, seed=seed)
-    coarse.bin_search_steps = 30
-    p = params_from_scene(coarse)
-    depth = np.asarray(marcher(p), np.float64)
-    hits = np.flatnonzero(np.isfinite(depth) & (depth > 0))
-    if hits.size == 0:
+def chain_surface_point(seed, de, reach=0.1, samples=501):
+    """A point of the hybrid chain's surface, where orbits stop escaping
+    within the 45 iterations, and the seeded direction `u` it is reached
+    from.  The chain's orbits stay bounded only in a core around the origin
+    (radius ~0.006-0.008: there the box steps multiply by 8 and the p8 step
+    shrinks); its DE shell, 0.07 units further out, holds no bounded orbit.
+    A coarse march along the seeded ray from `reach * u` towards the origin
+    samples the iteration counts (`de(params, points) -> (de, its)`,
+    its = max_it deciding inside); the first inside sample brackets the
+    surface, which is bisected to f64 resolution.  Returns (point on the
+    escaping side, u)."""
+    p = params_from_scene(scene_hybrid(64, 48, seed=seed))
+    rng = np.random.default_rng(seed + 3000)
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    start = reach * u
+    ts = np.linspace(0.0, reach, samples)
+    _, its = de(p, start[None, :] - ts[:, None] * u[None, :])
+    inside = np.flatnonzero(np.asarray(its) >= p.max_it)
+    if inside.size == 0 or inside[0] == 0:
         raise RuntimeError("the coarse march found no surface")
-    rng = np.random.default_rng(seed + 1000)
-    i = int(hits[rng.integers(hits.size)])
-    x, y = i % 64, i // 64
-    vg = np.array([[p.vgrads[r][k] for k in range(3)] for r in range(3)])
-    ystart = np.array(list(p.ystart))
-    c0 = ystart + vg[0] * x + vg[1] * y
-    cafy = (y / p.height - 0.5) * p.fov_y
-    cafx = float(np.float32((np.float32(p.fovx_off) - np.float32(x + 1)) * np.float32(p.fovx_mul)))
-    w = np.array([-math.sin(cafx), math.sin(cafy), math.cos(cafy) * math.cos(cafx)])
-    w /= np.linalg.norm(w)
-    v = w @ vg
-    surface = c0 + depth[i] * v
-    u = v / np.linalg.norm(v)
-    cam = surface - distance * u
+    lo, hi = ts[inside[0] - 1], ts[inside[0]]
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if mid <= lo or mid >= hi:
+            break
+        _, it = de(p, (start - mid * u)[None, :])
+        if it[0] >= p.max_it:
+            hi = mid
+        else:
+            lo = mid
+    return start - lo * u, u
+
+
+def scene_deepzoom(width, height, seed, de, distance=1e-9, fov_deg=0.5) -> Scene:
+    """The deep zoom on the hybrid chain (Mandelbulb p8 x1, Amazing Box
+    scale 2 x2, 45 iterations, numeric DE): the camera's start plane sits
+    `distance` scene units outside the surface point of
+    `chain_surface_point(seed, de)`, looking back along the seeded ray at
+    it, FOV `fov_deg`, the step width matched to the angular pixel
+    footprint there (1e-9 at 1080 rows: 8.1e-15 units, ~9,000 f64 ulps of
+    the coordinates and 1/58,000 of an FP32 ulp)."""
+    surface, u = chain_surface_point(seed, de)
+    cam = surface + distance * u
     s = scene_hybrid(width, height, seed=seed)
     s.fov_y = fov_deg
     sw = distance * math.radians(fov_deg) / height
     s.zoom = 2.1345 / (width * sw)
-    s.mid[:] = list(cam)
-    s.z_start = float(cam[2])
+    s.mid[:] = [float(c) for c in cam]
+    s.z_start = float(cam[2])  # CalcParams::new offsets the start plane by z_start - mid.z
     s.z_end = s.z_start + 4.0 * distance
-    set_camera(s, look_matrix(u))
+    set_camera(s, look_matrix(-u))
     return s
 
 

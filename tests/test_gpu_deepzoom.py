@@ -19,8 +19,7 @@ def oracle_march(p):
 
 
 def oracle_de(p, pts):
-    out = [O.de_at(p, q) for q in np.asarray(pts)]
-    return np.array([a for a, _ in out]), np.array([b for _, b in out])
+    return O.de_many(p, np.asarray(pts, np.float64))
 
 
 def error_map(g, r, p):
@@ -61,20 +60,71 @@ def test_bulb_deep_zoom_marches_in_fp64_and_matches_the_oracle(hip):
     report("bulb 1e-9, FP32 iterate forced", err32, mask32)
 
 
-def test_hybrid_deep_zoom_config(hip):
-    """The hybrid chain's deep zoom as configured: the camera 1e-9 in front
-    of a seeded coarse march's surface point.  That point is a DE surface of
-    the coarse picture (a pixel there is 0.07 units), and the chain's set is
-    dust at the zoomed scale, so the oracle's picture is background there --
-    and the backend's must be the same, pixel for pixel."""
-    sc = H.scene_deepzoom(192, 108, 1, oracle_march)
+def px_error(g, r, p):
+    """Depth error in step widths (pixels) where both hit."""
+    dg = np.asarray(g["depth64"], np.float64)
+    dr = np.asarray(r["depth"], np.float64)
+    both = np.isfinite(dg) & (dg > 0) & np.isfinite(dr) & (dr > 0)
+    return np.abs(dg[both] - dr[both])
+
+
+def chain_case(hip, width, height, seed, bands):
+    """Config 2 at width x height: the backend's whole picture, the oracle's
+    on `bands` (row0, rows); both in the automatic precision (FP64 here)."""
+    sc = H.scene_deepzoom(width, height, seed, oracle_de)
     p = hip.params_from_scene(sc)
-    r = O.render(O.params(sc), threads=0)
+    po = O.params(sc)
     g = hip.march(p)
-    assert g["counters"]["de_evals_f32"] == 0
-    err, mask = error_map(g, r, p)
-    report("hybrid 1e-9, automatic (FP64)", err, mask)
-    assert mask == 1.0
-    assert np.array_equal(np.isfinite(g["depth64"]), np.isfinite(r["depth"]))
-    if np.isfinite(err).any():
+    assert g["counters"]["de_evals_f32"] == 0  # the automatic precision took FP64
+    g32 = hip.march(p, opts=H.options(precision=H.PREC_F32))
+    assert g32["counters"]["de_evals_f32"] > 0
+    for row0, rows in bands:
+        r = O.render(po, row0=row0, rows=rows, threads=0)
+        sl = slice(row0 * width, (row0 + rows) * width)
+        gb = {k: (v[sl] if isinstance(v, np.ndarray) else v) for k, v in g.items()}
+        g32b = {k: (v[sl] if isinstance(v, np.ndarray) else v) for k, v in g32.items()}
+        yield p, r, gb, g32b, (row0, rows)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_hybrid_deep_zoom_config(hip, seed):
+    """Config 2: the hybrid chain at 1920 x 1080, the camera 1e-9 outside
+    the chain's surface (the boundary of its bounded orbits, found by a
+    seeded coarse march and bisected), FOV 0.5 degrees.  The surface fills
+    the view; the automatic precision marches it in FP64 (the reference's
+    needs_double rule) and matches the oracle: every mask bit, depths within
+    1e-6 of a step width (8.1e-21 units, 4e-12 relative to the depth),
+    iteration counts equal.  The FP32 iterate, forced, is reported per
+    pixel: at this zoom a pixel is 1/58,000 of an FP32 ulp of the
+    coordinates, so it cannot tell the pixels' points apart."""
+    bands = ((0, 6), (537, 6), (1074, 6))
+    for p, r, g, g32, band in chain_case(hip, 1920, 1080, seed, bands):
+        err, mask = error_map(g, r, p)
+        report(f"hybrid 1e-9 seed {seed} rows {band}, automatic (FP64)", err, mask)
+        assert np.isfinite(r["depth"]).mean() > 0.5  # the surface fills the view
+        assert mask == 1.0
         assert np.nanmax(err) < 1e-9
+        px = px_error(g, r, p)
+        print(f"  depth error in step widths: max {px.max():.3e}")
+        assert px.max() < 1e-6
+        m = parity.compare(g, r, p.width, band[1], p.step_width, 2.0)
+        assert m["iters_equal_inner"] == 1.0, m
+        err32, mask32 = error_map(g32, r, p)
+        report(f"hybrid 1e-9 seed {seed} rows {band}, FP32 iterate forced", err32, mask32)
+        px32 = px_error(g32, r, p)
+        if px32.size:
+            print(f"  FP32 depth error in step widths: p50 {np.percentile(px32, 50):.3e} "
+                  f"p99 {np.percentile(px32, 99):.3e} max {px32.max():.3e}")
+
+
+def test_hybrid_deep_zoom_whole_picture(hip):
+    """Config 2's scene at 192 x 108, every pixel against the oracle."""
+    for p, r, g, g32, band in chain_case(hip, 192, 108, 3, ((0, 108),)):
+        err, mask = error_map(g, r, p)
+        report("hybrid 1e-9 192x108 seed 3, automatic (FP64)", err, mask)
+        assert np.isfinite(r["depth"]).mean() > 0.5
+        assert mask == 1.0 and px_error(g, r, p).max() < 1e-6
+        m = parity.compare(g, r, p.width, p.height, p.step_width, 2.0)
+        assert m["iters_equal_inner"] == 1.0 and m["normal_deg_max"] < 1e-3, m
+        err32, mask32 = error_map(g32, r, p)
+        report("hybrid 1e-9 192x108 seed 3, FP32 iterate forced", err32, mask32)
