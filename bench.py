@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
-"""Benchmark of the HIP DE ray marcher on one MI355X: the learned warm
-start plus batching against the plain FP32 marcher.
+"""Benchmark of the HIP DE ray marcher on one MI355X: Mpixels/s of the
+1080p Mandelbulb Z-buffer at the plain marcher's depth error, the learned
+warm start and batching against the plain FP32 marcher.
 
 Workload: the power-8 Mandelbulb Z-buffer at 1920 x 1080 (the small
 configuration's scene -- 60 iterations, bailout r^2 = 16, analytic DE
@@ -20,19 +21,30 @@ Legs (every leg marches the same frames):
   of each kernel for all of them), autotuned;
 - cone + batched: plus the cone ray per tile (the reference's cone march):
   a tile whose cone ray passes the end of the view is background, every
-  other ray marches plainly -- the plain picture bit for bit;
+  other ray marches plainly -- the plain picture bit for bit; autotuned;
 - learned + batched: the network's hit jumps, each checked by the exact DE
   at its landing and rolled back to the plain march on failure;
-- learned + cone + batched (the value): both.
+- learned + cone + batched: both.
 One step is one replay of the plan.
 
+The value is the fastest leg whose picture meets the plain marcher's bar
+against the f64 oracle (mask, depth within 1e-4 of the scene scale and
+normals within 0.5 degrees, each on >= 99.9 % of the pixels, on three of
+the frames); `config.path` names it.  Without the oracle legs (--no-cpu)
+it is the cone + batched leg, whose picture is the plain one by
+construction.  The learned legs are reported beside it with their accuracy:
+a jump moves the march's sample points, and on a fractal the DE-stop
+decision depends on where they fall, so about 0.7 % of the surface pixels
+move by more than 1e-4 (a median of 2 pixels of depth) whatever the jump's
+margin.
+
 Prints one JSON line: Mpixels/s of the legs, each path's accuracy against
-the f64 oracle on three of the frames, the roofline figures of the learned
-path's kernels (the FP32 march against the FP32 vector peak, the network
-against the BF16 MFMA peak, the cone rays, finish and normals against the
-FP64 vector peak; algorithmic FLOPs per launch over HIP-event time on the
-library's stream), the CPU baseline (the f64 oracle on the host cores, and
-on one core), and the 64-frame animation of mixed sizes.  Beside the
+the f64 oracle, the roofline figures of the value's kernels (the FP32 march
+against the FP32 vector peak, the cone rays, finish and normals against the
+FP64 vector peak) and of the network (against the BF16 MFMA peak, on the
+learned + cone leg); algorithmic FLOPs per launch over HIP-event time on
+the library's stream; the CPU baseline (the f64 oracle on the host cores,
+and on one core), and the 64-frame animation of mixed sizes.  Beside the
 metric: the painter, the post passes, the frame's reflection rays and the
 Monte Carlo batches.
 """
@@ -189,12 +201,13 @@ def main():
     cp = H.counters()
     plain_value = n * a.steps * a.batch / plain_wall / 1e6
 
-    # --- the batched legs: the autotuner's options for the plain batch and
-    # for the learned + cone batch (untimed); the cone-only and learned-only
-    # legs take the latter's launch options
+    # --- the batched legs: the autotuner's options for the plain batch, the
+    # cone batch and the learned + cone batch (untimed); the learned-only leg
+    # takes the latter's launch options
     pb_opts, _ = H.autotune(frames, out, H.options())
+    c_opts, _ = H.autotune(frames, out, H.options(cone=True))
     lc_opts, _ = H.autotune(frames, out, H.options(mlp=net, cone=True))
-    legs = {"plain_batched": with_opts(pb_opts), "cone_batched": with_opts(lc_opts, cone=True),
+    legs = {"plain_batched": with_opts(pb_opts), "cone_batched": with_opts(c_opts, cone=True),
             "learned_batched": with_opts(lc_opts, mlp=net), "learned_cone_batched": with_opts(lc_opts, mlp=net, cone=True)}
     res = {}
     for name, o in legs.items():
@@ -215,25 +228,40 @@ def main():
                      "cone_background": c["cone_background"] // a.batch,
                      "options": {"tile": [o.tile_w, o.tile_h], "waves_per_cu": o.waves_per_cu,
                                  "it_split": o.it_split, "refill_min": o.refill_min}}
-    lead = res["learned_cone_batched"]
+    # --- each path's accuracy against the f64 oracle (three frames), and
+    # the oracle's own rate on the host cores; the value is the fastest leg
+    # that meets the plain marcher's bar
+    cpu_base = accuracy = None
+    if not a.no_cpu:
+        cpu_base, accuracy = cpu_leg(scenes, params, a, legs, net)
+    at_bar = {"plain_batched": "plain", "cone_batched": "cone", "learned_batched": "learned",
+              "learned_cone_batched": "learned_cone"}
+    if accuracy is not None:
+        ok = [k for k in legs if accuracy[at_bar[k]]["meets_bar"]]
+    else:
+        ok = ["cone_batched"]  # the plain picture by construction (tests/test_gpu_cone.py)
+    head = max(ok, key=lambda k: res[k]["value"]) if ok else "cone_batched"
+    lead = res[head]
     value = lead["value"]
 
-    # --- the kernels of the learned + cone path: HIP-event times of the same
-    # batch launched eagerly on the library's stream (a captured graph does
-    # not time its nodes), averaged
-    lo = legs["learned_cone_batched"]
-    tms = {"mlp": [], "cone": [], "march": [], "finish": [], "normals": []}
-    cf = None
-    for _ in range(max(3, a.steps)):
-        H.check(lib.mb3d_hip_march_batch(frames, a.batch, H.C.byref(out), H.C.byref(lo)), "march_batch")
-        cf = H.counters()
-        for k, f in (("mlp", "mlp_ms"), ("cone", "cone_ms"), ("march", "march_f32_ms"), ("finish", "finish_f64_ms"),
-                     ("normals", "normals_ms")):
-            tms[k].append(cf[f])
-    t_k = {k: float(np.mean(v)) for k, v in tms.items()}
-    t_k["finish"] -= t_k["normals"]  # (finish_f64_ms holds both passes)
-    items = sum(((s.width + lo.tile_w - 1) // lo.tile_w) * lo.tile_w * ((s.height + lo.tile_h - 1) // lo.tile_h) *
-                lo.tile_h for s in scenes)
+    # --- the kernels of the value's leg: HIP-event times of the same batch
+    # launched eagerly on the library's stream (a captured graph does not
+    # time its nodes), averaged; the network's on the learned + cone leg
+    def kernel_times(o):
+        tms = {"mlp": [], "cone": [], "march": [], "finish": [], "normals": []}
+        c = None
+        for _ in range(max(3, a.steps)):
+            H.check(lib.mb3d_hip_march_batch(frames, a.batch, H.C.byref(out), H.C.byref(o)), "march_batch")
+            c = H.counters()
+            for k, f in (("mlp", "mlp_ms"), ("cone", "cone_ms"), ("march", "march_f32_ms"),
+                         ("finish", "finish_f64_ms"), ("normals", "normals_ms")):
+                tms[k].append(c[f])
+        t = {k: float(np.mean(v)) for k, v in tms.items()}
+        t["finish"] -= t["normals"]  # (finish_f64_ms holds both passes)
+        return t, c
+
+    lo = legs[head]
+    t_k, cf = kernel_times(lo)
     kernels = [
         {"kernel": "k_march32 (batch)", "bound": "valu-fp32", "kernel_ms": t_k["march"],
          "flops_per_launch": flops_march(cf), "peak": PEAK_FP32 / 1e12},
@@ -241,13 +269,18 @@ def main():
          "flops_per_launch": flops_finish(cf), "peak": PEAK_FP64 / 1e12},
         {"kernel": "k_normals (batch)", "bound": "valu-fp32", "kernel_ms": t_k["normals"],
          "flops_per_launch": flops_normals(cf), "peak": PEAK_FP32 / 1e12},
-        {"kernel": "k_mlp_propose (batch)", "bound": "mfma-bf16", "kernel_ms": t_k["mlp"],
-         "flops_per_launch": F_MLP * cf["mlp_rays"], "peak": PEAK_BF16 / 1e12, "rays": cf["mlp_rays"],
-         "items": items},
-        {"kernel": "k_cone (batch)", "bound": "valu-fp64", "kernel_ms": t_k["cone"],
-         "flops_per_launch": flops_cone(cf), "peak": PEAK_FP64 / 1e12, "rays": cf["cone_rays"]},
     ]
-    for k in kernels:
+    if lo.mlp_cone:
+        kernels.append({"kernel": "k_cone (batch)", "bound": "valu-fp64", "kernel_ms": t_k["cone"],
+                        "flops_per_launch": flops_cone(cf), "peak": PEAK_FP64 / 1e12, "rays": cf["cone_rays"]})
+    ll = legs["learned_cone_batched"]
+    t_l, cl = kernel_times(ll)
+    items = sum(((s.width + ll.tile_w - 1) // ll.tile_w) * ll.tile_w * ((s.height + ll.tile_h - 1) // ll.tile_h) *
+                ll.tile_h for s in scenes)
+    net_kernel = {"kernel": "k_mlp_propose (batch)", "bound": "mfma-bf16", "kernel_ms": t_l["mlp"],
+                  "flops_per_launch": F_MLP * cl["mlp_rays"], "peak": PEAK_BF16 / 1e12, "rays": cl["mlp_rays"],
+                  "items": items, "leg": "learned_cone_batched"}
+    for k in kernels + [net_kernel]:
         k["achieved"] = k["flops_per_launch"] / (k["kernel_ms"] * 1e-3) / 1e12
         k["frac"] = k["achieved"] / k["peak"]
     dom = max(kernels, key=lambda k: k["kernel_ms"])
@@ -261,9 +294,11 @@ def main():
         "traffic": traffic_of(dom["kernel"], a.batch, a.width, a.height),
         "kernel_ms": dom["kernel_ms"],
         "flops_per_launch": dom["flops_per_launch"],
+        "leg": head,
         "kernels": kernels,
+        "network": net_kernel,
     }
-    for k in kernels:
+    for k in kernels + [net_kernel]:
         k["traffic"] = traffic_of(k["kernel"], a.batch, a.width, a.height)
 
     # --- the painter (beside the metric): the first frame's G-buffer,
@@ -355,7 +390,7 @@ def main():
     # around the bulb, in one plan, with the batch legs' launch options
     anim = None
     if not a.no_anim:
-        anim = animation_leg(a, lib, pb_opts, legs["learned_cone_batched"], net)
+        anim = animation_leg(a, lib, pb_opts, legs[head], head)
 
     result = {
         "metric": "Mpixels/s 1080p Mandelbulb-p8 Z-buffer; learned vs plain at equal depth error",
@@ -374,8 +409,8 @@ def main():
         "config": {
             "workload": f"mandelbulb-p8 z-buffer {a.width}x{a.height}, analytic DE, 60 its, bailout r2=16, fov 30; "
                         f"{a.batch} frames of distinct seeded cameras (seeds {a.seed}..{a.seed + a.batch - 1}) per "
-                        "graph launch; learned warm start (hit jumps) + cone-verified background tiles",
-            "path": "learned+cone+batched",
+                        "graph launch; the fastest leg at the plain marcher's bar against the f64 oracle",
+            "path": head.replace("_", "+"),
             "batch": a.batch,
             "train_cameras": list(TRAIN_SEEDS),
         },
@@ -385,7 +420,14 @@ def main():
         **res,
         "speedup_vs_plain": value / plain_value,
         "speedup_vs_plain_batched": value / res["plain_batched"]["value"],
-        "speedup_of_the_network": value / res["cone_batched"]["value"],
+        "learned": {
+            "leg": "learned_cone_batched",
+            "value": res["learned_cone_batched"]["value"],
+            "speedup_vs_plain": res["learned_cone_batched"]["value"] / plain_value,
+            "speedup_vs_plain_batched": res["learned_cone_batched"]["value"] / res["plain_batched"]["value"],
+            "speedup_of_the_network": res["learned_cone_batched"]["value"] / res["cone_batched"]["value"],
+            "meets_bar": None if accuracy is None else accuracy["learned_cone"]["meets_bar"],
+        },
         "network": {"train_s": train_s, "train_pairs": int(len(TRAIN_SEEDS) * tw * th),
                     "train_loss": [float(curve[:20].mean()), float(curve[-100:].mean())]},
         "roofline": roofline,
@@ -397,7 +439,7 @@ def main():
     if anim is not None:
         result["animation"] = anim
     if not a.no_cpu:
-        result["cpu_baseline"], result["accuracy"] = cpu_leg(scenes, params, a, legs, net)
+        result["cpu_baseline"], result["accuracy"] = cpu_base, accuracy
         cores = result["cpu_baseline"]["cores"]
         paint["cpu"] = paint_cpu_leg(cam, lv, paint_gbuf, cores)
         rays["cpu"] = rays_cpu_leg(p, tasks, ray_opts, cores)
@@ -407,9 +449,9 @@ def main():
     print(json.dumps(result))
 
 
-def animation_leg(a, lib, pb_opts, lc_opts, net):
+def animation_leg(a, lib, pb_opts, head_opts, head):
     """The 64-frame animation of mixed sizes in one plan: plain + batched and
-    learned + cone + batched, Mpixels/s of a replay."""
+    the value's leg, Mpixels/s of a replay."""
     import torch
 
     scenes = H.animation(64, seed=a.seed)
@@ -421,7 +463,7 @@ def animation_leg(a, lib, pb_opts, lc_opts, net):
     normals = torch.empty(n * 3, dtype=torch.float32, device=dev)
     out = H.Out(depth.data_ptr(), None, iters.data_ptr(), normals.data_ptr(), None, 1, 0)
     res = {"frames": 64, "pixels": n, "sizes": sorted({(s.width, s.height) for s in scenes})}
-    for name, o in (("plain_batched", pb_opts), ("learned_cone_batched", lc_opts)):
+    for name, o in (("plain_batched", pb_opts), (head, head_opts)):
         plan = H.Plan(frames, out, o)
         plan.launch(1)
         torch.cuda.synchronize()
@@ -433,7 +475,8 @@ def animation_leg(a, lib, pb_opts, lc_opts, net):
         wall = (time.perf_counter() - t0) / reps
         plan.close()
         res[name] = {"value": n / wall / 1e6, "unit": "Mpixels/s", "ms_per_launch": wall * 1e3}
-    res["speedup"] = res["learned_cone_batched"]["value"] / res["plain_batched"]["value"]
+    res["leg"] = head
+    res["speedup"] = res[head]["value"] / res["plain_batched"]["value"]
     return res
 
 
@@ -454,9 +497,10 @@ def cpu_leg(scenes, params, a, legs, net):
             "iters_equal_inner")
     paths = {"plain": H.options(), "cone": H.options(cone=True), "learned": H.options(mlp=net),
              "learned_cone": H.options(mlp=net, cone=True)}
-    lo = legs["learned_cone_batched"]
-    for o in paths.values():
-        o.tile_w, o.tile_h = lo.tile_w, lo.tile_h
+    of_leg = {"plain": "plain_batched", "cone": "cone_batched", "learned": "learned_batched",
+              "learned_cone": "learned_cone_batched"}
+    for k, o in paths.items():
+        o.tile_w, o.tile_h = legs[of_leg[k]].tile_w, legs[of_leg[k]].tile_h
     acc = {k: {m: [] for m in keys} for k in paths}
     t_full = []
     for f in ACC_FRAMES:
