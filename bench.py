@@ -21,7 +21,16 @@ Legs (every leg marches the same frames):
   of each kernel for all of them), autotuned;
 - cone + batched: plus the cone ray per tile (the reference's cone march):
   a tile whose cone ray passes the end of the view is background, every
-  other ray marches plainly -- the plain picture bit for bit; autotuned;
+  other ray marches plainly -- the plain picture bit for bit (depths,
+  iterations and normals; the background's fog counts are the cone's, as
+  in the reference's cone march); autotuned;
+- gated cone + batched: the network as the cone pass's gate -- it runs on
+  each tile's middle ray first, and a tile it calls a hit (--gate) marches
+  no cone ray (its rays march plainly, as a cone ray that meets the surface
+  would leave them); hit jumps off, so the plain picture bit for bit (as
+  the cone leg's: the background's fog counts from the cone where it
+  decided), and a wrong call costs time, never a pixel; the cone leg's
+  launch options;
 - learned + batched: the network's hit jumps, each checked by the exact DE
   at its landing and rolled back to the plain march on failure;
 - learned + cone + batched: both.
@@ -31,8 +40,9 @@ The value is the fastest leg whose picture meets the plain marcher's bar
 against the f64 oracle (mask, depth within 1e-4 of the scene scale and
 normals within 0.5 degrees, each on >= 99.9 % of the pixels, on three of
 the frames); `config.path` names it.  Without the oracle legs (--no-cpu)
-it is the cone + batched leg, whose picture is the plain one by
-construction.  The learned legs are reported beside it with their accuracy:
+it is the faster of the two legs whose picture is the plain one by
+construction: cone + batched, and gated cone + batched (the network as the
+cone pass's gate, hit jumps off).  The learned legs are reported beside it with their accuracy:
 a jump moves the march's sample points, and on a fractal the DE-stop
 decision depends on where they fall, so about 0.7 % of the surface pixels
 move by more than 1e-4 (a median of 2 pixels of depth) whatever the jump's
@@ -89,6 +99,8 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--seed", type=int, default=1, help="the first frame's camera seed")
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--gate", type=float, default=0.9,
+                    help="the network's hit probability above which a tile skips its cone ray (the gated leg)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy legs")
     ap.add_argument("--no-anim", action="store_true", help="skip the 64-frame animation leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -123,13 +135,16 @@ def flops_normals(c: dict) -> float:
     return c["its_normals_f32"] * F_P8 + 6 * c["hits"] * F_DE
 
 
-def with_opts(base, mlp=None, cone=False):
+def with_opts(base, mlp=None, cone=False, gate=0.0):
     """A copy of `base` (tile, grid, split, refill) with the network and the
-    cone rays set."""
+    cone rays set; `gate` > 0: the network as the cone pass's gate alone
+    (hit jumps off)."""
     o = H.Options()
     H.C.pointer(o)[0] = base
     o.mlp = mlp.h if mlp is not None else None
     o.mlp_cone = 1 if cone else 0
+    o.mlp_cone_gate = gate
+    o.mlp_conf_hit = 2.0 if gate > 0.0 else base.mlp_conf_hit
     return o
 
 
@@ -208,6 +223,7 @@ def main():
     c_opts, _ = H.autotune(frames, out, H.options(cone=True))
     lc_opts, _ = H.autotune(frames, out, H.options(mlp=net, cone=True))
     legs = {"plain_batched": with_opts(pb_opts), "cone_batched": with_opts(c_opts, cone=True),
+            "gated_cone_batched": with_opts(c_opts, mlp=net, cone=True, gate=a.gate),
             "learned_batched": with_opts(lc_opts, mlp=net), "learned_cone_batched": with_opts(lc_opts, mlp=net, cone=True)}
     res = {}
     for name, o in legs.items():
@@ -234,12 +250,14 @@ def main():
     cpu_base = accuracy = None
     if not a.no_cpu:
         cpu_base, accuracy = cpu_leg(scenes, params, a, legs, net)
-    at_bar = {"plain_batched": "plain", "cone_batched": "cone", "learned_batched": "learned",
-              "learned_cone_batched": "learned_cone"}
+    at_bar = {"plain_batched": "plain", "cone_batched": "cone", "gated_cone_batched": "gated_cone",
+              "learned_batched": "learned", "learned_cone_batched": "learned_cone"}
     if accuracy is not None:
         ok = [k for k in legs if accuracy[at_bar[k]]["meets_bar"]]
     else:
-        ok = ["cone_batched"]  # the plain picture by construction (tests/test_gpu_cone.py)
+        # the plain picture by construction (tests/test_gpu_cone.py,
+        # tests/test_gpu_learned.py)
+        ok = ["cone_batched", "gated_cone_batched"]
     head = max(ok, key=lambda k: res[k]["value"]) if ok else "cone_batched"
     lead = res[head]
     value = lead["value"]
@@ -273,6 +291,9 @@ def main():
     if lo.mlp_cone:
         kernels.append({"kernel": "k_cone (batch)", "bound": "valu-fp64", "kernel_ms": t_k["cone"],
                         "flops_per_launch": flops_cone(cf), "peak": PEAK_FP64 / 1e12, "rays": cf["cone_rays"]})
+        if lo.mlp_cone_gate > 0.0:
+            kernels[-1]["note"] = ("kernel_ms includes the gate's network pass over the tiles' middle rays "
+                                   f"({cf['mlp_rays']} rays, k_mlp_propose ahead of k_cone)")
     ll = legs["learned_cone_batched"]
     t_l, cl = kernel_times(ll)
     items = sum(((s.width + ll.tile_w - 1) // ll.tile_w) * ll.tile_w * ((s.height + ll.tile_h - 1) // ll.tile_h) *
@@ -421,12 +442,16 @@ def main():
         "speedup_vs_plain": value / plain_value,
         "speedup_vs_plain_batched": value / res["plain_batched"]["value"],
         "learned": {
-            "leg": "learned_cone_batched",
-            "value": res["learned_cone_batched"]["value"],
-            "speedup_vs_plain": res["learned_cone_batched"]["value"] / plain_value,
-            "speedup_vs_plain_batched": res["learned_cone_batched"]["value"] / res["plain_batched"]["value"],
-            "speedup_of_the_network": res["learned_cone_batched"]["value"] / res["cone_batched"]["value"],
-            "meets_bar": None if accuracy is None else accuracy["learned_cone"]["meets_bar"],
+            leg: {
+                "value": res[leg]["value"],
+                "speedup_vs_plain": res[leg]["value"] / plain_value,
+                "speedup_vs_plain_batched": res[leg]["value"] / res["plain_batched"]["value"],
+                "speedup_of_the_network": res[leg]["value"] / res["cone_batched"]["value"],
+                "meets_bar": None if accuracy is None else accuracy[at_bar[leg]]["meets_bar"],
+                "role": role,
+            }
+            for leg, role in (("gated_cone_batched", f"the cone pass's gate (p >= {a.gate}), hit jumps off"),
+                              ("learned_cone_batched", "hit jumps, checked at the landing"))
         },
         "network": {"train_s": train_s, "train_pairs": int(len(TRAIN_SEEDS) * tw * th),
                     "train_loss": [float(curve[:20].mean()), float(curve[-100:].mean())]},
@@ -495,10 +520,11 @@ def cpu_leg(scenes, params, a, legs, net):
     threads = a.cpu_threads or min(share, 16)
     keys = ("mask_agree", "depth_rel_p99", "depth_rel_max", "depth_ok_frac", "normal_ok_frac", "normal_deg_p99",
             "iters_equal_inner")
-    paths = {"plain": H.options(), "cone": H.options(cone=True), "learned": H.options(mlp=net),
-             "learned_cone": H.options(mlp=net, cone=True)}
-    of_leg = {"plain": "plain_batched", "cone": "cone_batched", "learned": "learned_batched",
-              "learned_cone": "learned_cone_batched"}
+    paths = {"plain": H.options(), "cone": H.options(cone=True),
+             "gated_cone": H.options(mlp=net, cone=True, cone_gate=a.gate, conf_hit=2.0),
+             "learned": H.options(mlp=net), "learned_cone": H.options(mlp=net, cone=True)}
+    of_leg = {"plain": "plain_batched", "cone": "cone_batched", "gated_cone": "gated_cone_batched",
+              "learned": "learned_batched", "learned_cone": "learned_cone_batched"}
     for k, o in paths.items():
         o.tile_w, o.tile_h = legs[of_leg[k]].tile_w, legs[of_leg[k]].tile_h
     acc = {k: {m: [] for m in keys} for k in paths}

@@ -261,7 +261,17 @@ typedef struct mb3d_hip_options {
      * cone that passes the end of the view makes the tile background
      * (0: off) */
     int32_t mlp_cone;
-    int32_t reserved_o[2];
+    /* extension, with the warm start and the cone rays: the network as the
+     * cone pass's gate.  The network runs once per tile, on the tile's
+     * middle ray (the cone ray's), ahead of the cone pass; a tile whose hit
+     * probability is at least mlp_cone_gate (in (0, 1); 0: off) marches no
+     * cone ray and its rays march plainly -- as for a tile whose cone ray
+     * meets a surface -- so the picture is the one without the gate.  The
+     * gain: the cone rays of tiles that hold a surface, where they decide
+     * nothing.  With hit jumps off (mlp_conf_hit above 1) the picture is
+     * the plain march's. */
+    float mlp_cone_gate;
+    int32_t reserved_o;
 } mb3d_hip_options;
 
 /* ---- painting: the G-buffer lit and coloured (src/lighting.rs; the

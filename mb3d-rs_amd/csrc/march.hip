@@ -215,6 +215,12 @@ bool tile_ok(const mb3d_hip_options &o) {
     return pow2(o.tile_w) && pow2(o.tile_h) && o.tile_w * o.tile_h <= 4096;
 }
 
+// the cone gate asks for the network and the cone rays, and a probability
+bool gate_ok(const mb3d_hip_options &o) {
+    if (o.mlp_cone_gate == 0.0f) return true;
+    return o.mlp_cone_gate > 0.0f && o.mlp_cone_gate < 1.0f && o.mlp != nullptr && o.mlp_cone != 0;
+}
+
 // entries of a frame's ray tables (k_ray_tab): its columns, then its rows
 long long ray_tab_entries(const DevP &d) { return (long long)d.width + d.rows; }
 
@@ -469,22 +475,34 @@ int dem_of(const mb3d_hip_params &p) {
     return DEM_NUMERIC;
 }
 
+// the warm start's jumps are on (hit or miss jumps: the per-item proposals
+// and the learned march); the network may be on for the cone's gate alone
+bool jumps_on(const MlpDev &M) { return M.enabled && (M.conf_hit <= 1.0f || M.conf_miss >= 0.0f); }
+
+// the network's pass: per work item (the warm start's proposals), or per
+// tile (`tiles`: the cone pass's gate)
 template <bool BATCH>
 void launch_propose(hipStream_t s, const DevP &d, const DevP *tab, int n, const MlpDev &M, const double2 *rtab,
-                    const ConeRes *cone, float2 *prop, unsigned long long *ctr) {
-    const long long groups = (d.nslots_q + 63) / 64;
+                    const ConeRes *cone, float2 *prop, unsigned long long *ctr, bool tiles = false) {
+    const long long groups = ((d.nslots_q >> (tiles ? d.tn_shift : 0)) + 63) / 64;
     const int grid = (int)std::max<long long>(1, std::min<long long>((groups + 3) / 4, (long long)g_ctx.cus * 16));
-    hipLaunchKernelGGL((k_mlp_propose<BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, M, rtab, cone, prop, ctr);
+    hipLaunchKernelGGL((k_mlp_propose<BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, M, rtab, cone, prop, ctr,
+                       tiles ? 1 : 0);
 }
 
+// the cone rays; with the network's gate (M.gate) its per-tile pass first,
+// into `prop`
 template <int KIND, int DEM, bool BATCH>
 void launch_cone(hipStream_t s, const DevP &d, const DevP *tab, int n, const double2 *rtab, ConeRes *out,
-                 unsigned long long *ctr) {
+                 unsigned long long *ctr, const MlpDev &M, float2 *prop) {
     if constexpr (KIND == K_GEN) return;  // the FP32 march's kinds only
     else {
+        const bool gate = M.enabled && M.gate && prop;
+        if (gate) launch_propose<BATCH>(s, d, tab, n, M, rtab, nullptr, prop, ctr, true);
         const long long tiles = d.nslots_q >> d.tn_shift;
         const int grid = (int)std::max<long long>(1, std::min<long long>((tiles + 255) / 256, (long long)g_ctx.cus * 8));
-        hipLaunchKernelGGL((k_cone<KIND, DEM, BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, rtab, out, ctr);
+        hipLaunchKernelGGL((k_cone<KIND, DEM, BATCH>), dim3(grid), dim3(256), 0, s, d, tab, n, rtab, out, ctr,
+                           gate ? (const float2 *)prop : nullptr, M.gate_logit);
     }
 }
 
@@ -498,7 +516,7 @@ void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b, const M
                   int n = 1) {
     const size_t lds = BATCH ? (size_t)n * DEVP_CORE : 0;
     if constexpr (KIND == K_GEN) return;  // FP64 only: the hosts never ask for it
-    else if (M.enabled)
+    else if (jumps_on(M))
         hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, true>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
     else
         hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, false>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
@@ -733,6 +751,8 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     std::memset(&o, 0, sizeof o);
     if (opt_in) o = *opt_in;
     if (!tile_ok(o)) return set_error(MB3D_HIP_ERR_INVALID, "bad tile shape (sides are powers of two up to 64)");
+    if (!gate_ok(o))
+        return set_error(MB3D_HIP_ERR_INVALID, "mlp_cone_gate is a probability in (0, 1) and needs mlp and mlp_cone");
     if (cancel && cancel(user)) return set_error(MB3D_HIP_CANCELLED, "cancelled");
     // (the K_GEN formulas have no FP32 iterate: FP64 whatever was asked)
     const bool f64 = o.precision == MB3D_HIP_PREC_F64 || (o.precision == MB3D_HIP_PREC_AUTO && needs_f64(p)) ||
@@ -845,11 +865,11 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
             if (cone) {
                 dispatch(kind, dem, [&](auto K, auto D) {
                     launch_cone<decltype(K)::value, decltype(D)::value, false>(stream, d, nullptr, 1, rtab, b.cone,
-                                                                              ctr_dev);
+                                                                              ctr_dev, M, b.prop);
                 });
             }
             HIP_TRY(hipEventRecord(c.ev[5], c.stream));
-            if (M.enabled) launch_propose<false>(stream, d, nullptr, 1, M, rtab, b.cone, b.prop, ctr_dev);
+            if (jumps_on(M)) launch_propose<false>(stream, d, nullptr, 1, M, rtab, b.cone, b.prop, ctr_dev);
             HIP_TRY(hipEventRecord(c.ev[6], c.stream));
             dispatch(kind, dem, [&](auto K, auto D) {
                 launch_march<decltype(K)::value, decltype(D)::value>(grid_march, stream, d, b, M);
@@ -1038,11 +1058,11 @@ int plan_enqueue(Plan &pl, hipStream_t stream, bool split) {
     if (pl.P0.cone_mode) {
         dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
             launch_cone<decltype(K)::value, decltype(D)::value, true>(stream, pl.P0, tab, n, pl.b.rtab, pl.b.cone,
-                                                                     pl.d_ctr);
+                                                                     pl.d_ctr, pl.M, pl.b.prop);
         });
     }
     if (split) HIP_TRY(hipEventRecord(pl.ea, stream));
-    if (pl.M.enabled)
+    if (jumps_on(pl.M))
         launch_propose<true>(stream, pl.P0, pl.d_tab, pl.n, pl.M, pl.b.rtab, pl.b.cone, pl.b.prop, pl.d_ctr);
     if (split) HIP_TRY(hipEventRecord(pl.eb, stream));
     dispatch(pl.kind, pl.dem, [&](auto K, auto D) {
@@ -1072,6 +1092,8 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     std::memset(&o, 0, sizeof o);
     if (opt_in) o = *opt_in;
     if (!tile_ok(o)) return set_error(MB3D_HIP_ERR_INVALID, "bad tile shape (sides are powers of two up to 64)");
+    if (!gate_ok(o))
+        return set_error(MB3D_HIP_ERR_INVALID, "mlp_cone_gate is a probability in (0, 1) and needs mlp and mlp_cone");
     const int kind = kind_of(frames[0]), dem = dem_of(frames[0]);
     for (int f = 0; f < n; f++) {
         const std::string why = unsupported_reason(&frames[f]);

@@ -389,45 +389,66 @@ __device__ __forceinline__ bool item_pixel(const DevP &F, int ql, int &x, int &y
 // computed in the same kernel without an agent-scope release and acquire,
 // which on this GPU write back and invalidate an XCD's whole L2 (measured:
 // +0.75 ms a 1080p frame at a fence pair per chunk of 256 rays).
+// TILES: the cone pass's gate instead -- the network on every tile's middle
+// ray (the cone ray's), written to the slot of the tile's first item.
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__restrict__ gtab, int nframes, MlpDev M,
                                                      const double2 *__restrict__ rtab, const ConeRes *__restrict__ cone,
-                                                     float2 *prop, unsigned long long *__restrict__ ctr) {
+                                                     float2 *prop, unsigned long long *__restrict__ ctr, int tiles) {
     const GlobalFrames tab{reinterpret_cast<const char *>(gtab)};
     __shared__ __attribute__((aligned(16))) char lds[4][MLP_LDS_BYTES];
     __shared__ __attribute__((aligned(16))) mlp_s4 wl[MLP_TILES * 64];
     mlp_stage_weights(M, wl);
     __syncthreads();
     const int lane = lane_id();
-    const long long groups = (P0.nslots_q + 63) / 64;
+    // the units: work items, or tiles (each its first item's slot)
+    const int ushift = tiles ? P0.tn_shift : 0;
+    const long long units = P0.nslots_q >> ushift;
+    const long long groups = (units + 63) / 64;
     const long long waves = (long long)gridDim.x * 4;
     unsigned long long evaluated = 0;  // rays the network ran on (the wave's, the same in every lane)
     // a uniform loop: every lane of a wave runs the same groups
     for (long long g = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); g < groups; g += waves) {
-        const long long q = g * 64 + lane;
+        const long long u = g * 64 + lane;
+        const long long q = u << ushift;
         float o[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 1.0f};
         int f = 0, x = 0, yr = 0;
-        const bool in_q = q < P0.nslots_q;
+        const bool in_q = u < units;
         if (BATCH) {
-            const FrameHint h = frame_hint(tab, nframes, g * 64);  // wave-uniform
+            const FrameHint h = frame_hint(tab, nframes, (g * 64) << ushift);  // wave-uniform
             f = in_q ? frame_of(tab, nframes, h, q) : 0;
         }
         const DevP &F = BATCH ? tab[f] : P0;
-        // the items of tiles the cone rays found empty take no proposal (the
-        // march makes them background): a wave whose 64 items all lie in
-        // such tiles skips the network
-        const bool empty = !in_q || (cone && cone[q >> P0.tn_shift].t > F.zend);
-        if (__all(empty)) {
-            if (in_q) prop[q] = make_float2(0.0f, -1e30f);
-            continue;
-        }
-        if (in_q && item_pixel(F, (int)(q - F.q_begin), x, yr)) {
-            double cc[3], vv[3];
-            pixel_ray_tab(F, rtab, x, yr, cc, vv);
-            const double il = 1.0 / sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
-            for (int j = 0; j < 3; j++) {
-                o[j] = (float)cc[j];
-                d[j] = (float)(vv[j] * il);
+        if (tiles) {
+            if (in_q) {
+                int tx;
+                const int ty = div_rem((int)((q - F.q_begin) >> F.tn_shift), F.tiles_x, tx);
+                double cc[3], vv[3];
+                frac_ray(F, (double)(tx << F.tw_shift) + (F.tile_w - 1.0) * 0.5,
+                         (double)(F.row0 + ty * F.tile_h) + (F.tile_h - 1.0) * 0.5, cc, vv);
+                const double il = 1.0 / sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+                for (int j = 0; j < 3; j++) {
+                    o[j] = (float)cc[j];
+                    d[j] = (float)(vv[j] * il);
+                }
+            }
+        } else {
+            // the items of tiles the cone rays found empty take no proposal
+            // (the march makes them background): a wave whose 64 items all
+            // lie in such tiles skips the network
+            const bool empty = !in_q || (cone && cone[q >> P0.tn_shift].t > F.zend);
+            if (__all(empty)) {
+                if (in_q) prop[q] = make_float2(0.0f, -1e30f);
+                continue;
+            }
+            if (in_q && item_pixel(F, (int)(q - F.q_begin), x, yr)) {
+                double cc[3], vv[3];
+                pixel_ray_tab(F, rtab, x, yr, cc, vv);
+                const double il = 1.0 / sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+                for (int j = 0; j < 3; j++) {
+                    o[j] = (float)cc[j];
+                    d[j] = (float)(vv[j] * il);
+                }
             }
         }
         // the network sees the ray as its point of closest approach to the
@@ -456,16 +477,24 @@ __global__ __launch_bounds__(256) void k_mlp_propose(DevP P0, const DevP *__rest
 // queue's tile.
 constexpr double kConeMinStep = 0.05, kConeMinShare = 0.1, kConeMargin = 1.5;  // src/calc.rs:2659-2674
 
+// gate (not NULL): the network's logit per tile (k_mlp_propose's TILES
+// pass); a tile at or above gate_logit marches no cone ray (depth 0: its
+// rays march plainly).
 template <int KIND, int DEM, bool BATCH>
 __global__ __launch_bounds__(256) void k_cone(DevP P0, const DevP *__restrict__ gtab, int nframes,
                                               const double2 *__restrict__ rtab, ConeRes *__restrict__ out,
-                                              unsigned long long *__restrict__ ctr) {
+                                              unsigned long long *__restrict__ ctr, const float2 *__restrict__ gate,
+                                              float gate_logit) {
     const GlobalFrames tab{reinterpret_cast<const char *>(gtab)};
     Cnt cnt;
     const long long ntiles = P0.nslots_q >> P0.tn_shift;
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < ntiles; g += stride) {
         const long long q = g << P0.tn_shift;
+        if (gate && gate[q].y >= gate_logit) {
+            out[g] = ConeRes{0.0, 0.0f, 0};
+            continue;
+        }
         const int fr = BATCH ? find_frame(tab, nframes, q) : 0;
         const DevP &P = BATCH ? tab[fr] : P0;
         int tx;

@@ -507,5 +507,9 @@ void mlp_dev(const mb3d_hip_mlp *h, const mb3d_hip_options &o, MlpDev &M) {
     M.margin_rel = o.mlp_margin_rel > 0.0f ? o.mlp_margin_rel : 0.15f;
     M.margin_abs = o.mlp_margin_abs;
     M.enabled = 1;
+    if (o.mlp_cone_gate > 0.0f && o.mlp_cone_gate < 1.0f && o.mlp_cone != 0) {
+        M.gate = 1;
+        M.gate_logit = std::log(o.mlp_cone_gate / (1.0f - o.mlp_cone_gate));
+    }
 }
 }  // namespace mb3d

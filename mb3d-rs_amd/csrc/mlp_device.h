@@ -52,6 +52,10 @@ struct MlpDev {
     float margin_rel;   // the jump stops short of the prediction by this share
     float margin_abs;   // ... and this many scene units
     int enabled;
+    // the cone pass's gate: a tile whose middle ray's hit logit is at least
+    // gate_logit marches no cone ray (gate 0: off)
+    int gate;
+    float gate_logit;
 };
 
 // (host) the device form of `net` with the options' thresholds; disabled

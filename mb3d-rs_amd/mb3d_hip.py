@@ -230,7 +230,8 @@ class Options(C.Structure):
         ("mlp_margin_abs", C.c_float),
         ("refill_min", C.c_int32),
         ("mlp_cone", C.c_int32),
-        ("reserved_o", C.c_int32 * 2),
+        ("mlp_cone_gate", C.c_float),
+        ("reserved_o", C.c_int32),
     ]
 
 
@@ -887,13 +888,16 @@ def _p(a):
 
 
 def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=16, guard_band=0.25, it_split=0, mlp=None,
-            conf_hit=0.0, conf_miss=0.0, margin_rel=0.0, margin_abs=0.0, refill_min=0, cone=False) -> Options:
+            conf_hit=0.0, conf_miss=0.0, margin_rel=0.0, margin_abs=0.0, refill_min=0, cone=False,
+            cone_gate=0.0) -> Options:
     """Options; `mlp` (an Mlp) turns the warm start on: hit jumps with the
     library's defaults (conf_hit 0: 0.5, above 1: off; margin_rel 0: 0.15),
     miss jumps (unverified) only with conf_miss > 0; `cone` adds the cone
     ray per tile (a verified safe start, and background for tiles it finds
-    empty).  `refill_min`: idle lanes a march wave gathers before refilling
-    (0: the library's default)."""
+    empty); `cone_gate` (with `mlp` and `cone`) skips the cone ray of a tile
+    whose middle ray the network calls a hit with at least that
+    probability.  `refill_min`: idle lanes a march wave gathers before
+    refilling (0: the library's default)."""
     o = Options()
     o.precision = precision
     o.tile_w, o.tile_h = tile
@@ -902,6 +906,7 @@ def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=16, guard_band=0.25, 
     o.it_split = it_split
     o.refill_min = refill_min
     o.mlp_cone = 1 if cone else 0
+    o.mlp_cone_gate = cone_gate
     if mlp is not None:
         o.mlp = mlp.h
         o.mlp_conf_hit, o.mlp_conf_miss = conf_hit, conf_miss

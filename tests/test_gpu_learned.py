@@ -122,3 +122,61 @@ def test_learned_batch_equals_learned_single_frames(hip, trained):
         one = hip.march(arr[i], opts=opts, outputs=("depth",))
         assert np.array_equal(got[off:off + m], one["depth"]), i
         off += m
+
+
+def test_the_cone_gate_keeps_the_plain_picture(hip, trained):
+    """The network as the cone pass's gate, hit jumps off: a tile whose
+    middle ray the network calls a hit marches no cone ray, its rays march
+    plainly -- the plain depths, iterations and normals bit for bit, with
+    fewer cone rays (the network's verified gain: a wrong call costs time,
+    never a pixel).  The G-buffer records are the plain march's or, for the
+    background of a tile whose cone ray went on, the cone's (its dynamic
+    fog count, as the reference's cone march leaves it)."""
+    sc = H.scene_bulb(640, 360, seed=5)
+    p = hip.params_from_scene(sc)
+    plain = hip.march(p)
+    cone = hip.march(p, opts=H.options(cone=True))
+    gated = hip.march(p, opts=H.options(mlp=trained, cone=True, cone_gate=0.9, conf_hit=2.0))
+    cc, cg = cone["counters"], gated["counters"]
+    for k in ("depth", "depth64", "iters", "normals"):
+        assert np.array_equal(gated[k], plain[k]), k
+    same = (gated["gbuf"] == plain["gbuf"]).all(1) | (gated["gbuf"] == cone["gbuf"]).all(1)
+    assert same.all()
+    tiles = (p.width // 8) * (p.height // 8)
+    assert cc["cone_rays"] == tiles and cg["mlp_rays"] >= tiles
+    assert cg["jumps"] == 0
+    # most tiles that hold a surface skip their cone ray; the empty ones
+    # keep theirs
+    assert cg["cone_rays"] < cc["cone_rays"] - 0.5 * (tiles - cc["cone_background"] / 64), (cc, cg)
+    assert cg["cone_background"] >= 0.97 * cc["cone_background"], (cc, cg)
+
+
+def test_the_cone_gate_in_a_batch_equals_the_plain_batch(hip, trained):
+    from test_gpu_batch import device_out, host
+
+    scenes = H.animation(4, seed=7, sizes=((160, 120), (96, 72), (200, 150)))
+    arr = H.params_array(scenes)
+    n = sum(s.width * s.height for s in scenes)
+    b0, o0 = device_out(n)
+    H.check(hip.load().mb3d_hip_march_batch(arr, len(arr), H.C.byref(o0), None), "batch")
+    want = host(b0)
+    b1, o1 = device_out(n)
+    opts = H.options(mlp=trained, cone=True, cone_gate=0.9, conf_hit=2.0)
+    plan = H.Plan(arr, o1, opts)
+    try:
+        for _ in range(2):
+            plan.launch(1)
+            got = host(b1)
+            for k in ("depth", "depth64", "iters", "normals"):
+                assert np.array_equal(got[k], want[k]), k
+    finally:
+        plan.close()
+
+
+def test_the_cone_gate_needs_the_network_and_the_cone(hip, trained):
+    sc = H.scene_bulb(64, 48, seed=1)
+    p = hip.params_from_scene(sc)
+    for o in (H.options(cone=True, cone_gate=0.9), H.options(mlp=trained, cone_gate=0.9),
+              H.options(mlp=trained, cone=True, cone_gate=1.5)):
+        with pytest.raises(RuntimeError):
+            hip.march(p, opts=o)
