@@ -37,11 +37,12 @@
 // (test_oracle.py).  The smoothing sweeps and normals on the smooth
 // iteration against the DE gradient (test_oracle_normals.py); the cone
 // march against the reference's own cone tests (test_oracle_cone.py).
+// Colour options 1..5 and colour on an iteration (do_color,
+// do_color_on_it): the orbit end they read against an independent iterate
+// of the surface point, each word inverted back to that orbit end
+// (test_oracle_normals.py).
 // Parity unpinned: the AO walk's step rule (dt += max(DE, dt dsm)) is
-// pinned only through the openness it reaches (a doubled step passes);
-// colour options 1..5 and colour on an iteration (do_color,
-// do_color_on_it) are restated only: their words are checked for being
-// filled and differing from option 0, not against independent values.
+// pinned only through the openness it reaches (a doubled step passes).
 #include "mb3d_oracle.h"
 
 #include <algorithm>
@@ -1182,6 +1183,15 @@ void do_color(const Marcher &m, Px &si) {
     si.otrap = clip15((float)s);
 }
 
+// What do_color saw at the last surface the thread found (orc_color_probe):
+// the orbit's end, the point, the radii and the two words.
+struct ColorProbe {
+    double v[3], c[3], rout, rold;
+    uint16_t otrap, gradient;
+    bool set;
+};
+thread_local ColorProbe *g_color_probe = nullptr;
+
 // surface_found (src/calc.rs:2471-2517)
 void surface_found(Marcher &m, Px &si, double dtmp, double last_step) {
     const mb3d_hip_params &p = m.p;
@@ -1208,6 +1218,18 @@ void surface_found(Marcher &m, Px &si, double dtmp, double last_step) {
     si.si_gradient = clip15(v);
     if (p.color_on_it != 0) m.do_color_on_it();
     do_color(m, si);
+    if (g_color_probe) {
+        ColorProbe &o = *g_color_probe;
+        for (int k = 0; k < 3; k++) {
+            o.v[k] = m.it.v[k];
+            o.c[k] = m.it.c[k];
+        }
+        o.rout = m.it.rout;
+        o.rold = m.it.rold;
+        o.otrap = si.otrap;
+        o.gradient = si.si_gradient;
+        o.set = true;
+    }
     si.zpos_fine = m.zpos_fine();
     si.depth = m.mzz;
 }
@@ -3225,6 +3247,31 @@ int render_impl(const mb3d_hip_params *pp, const mb3d_hip_post_job *post, int32_
 extern "C" int32_t orc_render(const mb3d_hip_params *pp, const mb3d_hip_post_job *post, int32_t row0, int32_t rows,
                               const orc_out *out, int32_t threads, orc_counters *cnt) {
     return render_impl(pp, post, row0, rows, out, threads, cnt, nullptr);
+}
+
+// One pixel's march with what do_color saw at its surface: out = the
+// orbit's end v (3), the surface point c (3), rout, rold; words = the
+// otrap and si_gradient words.  Returns 1 at a surface, 0 otherwise.
+extern "C" int32_t orc_color_probe(const mb3d_hip_params *pp, int32_t x, int32_t y, double *out, uint16_t *words) {
+    if (int e = check_params(pp)) return e;
+    if (x < 0 || y < 0 || x >= pp->width || y >= pp->height) return fail("pixel out of the picture");
+    Marcher m(*pp);
+    m.seed = orc_pixel_seed(y, x);
+    ColorProbe pr;
+    std::memset(&pr, 0, sizeof pr);
+    g_color_probe = &pr;
+    Px s = march_pixel(m, x, y);
+    g_color_probe = nullptr;
+    if (!pr.set || !(std::isfinite(s.depth) && s.depth > 0)) return 0;
+    for (int k = 0; k < 3; k++) {
+        out[k] = pr.v[k];
+        out[3 + k] = pr.c[k];
+    }
+    out[6] = pr.rout;
+    out[7] = pr.rold;
+    words[0] = pr.otrap;
+    words[1] = pr.gradient;
+    return 1;
 }
 
 extern "C" int32_t orc_cone_grid(const mb3d_hip_params *pp, int32_t bw, int32_t bh, double *t, float *count,

@@ -120,6 +120,11 @@ int32_t orc_de_many(const mb3d_hip_params *p, const double *pos, int64_t n, doub
 int32_t orc_mc_batch(const mb3d_hip_params *p, const mb3d_hip_mc_task *tasks, int64_t n, int32_t secant,
                      mb3d_hip_mc_res *res, int32_t threads);
 
+/* One pixel's march with what do_color saw at its surface (tests): out =
+ * the orbit's end v (3), the surface point c (3), rout, rold; words = the
+ * otrap and si_gradient words.  1 at a surface, 0 otherwise. */
+int32_t orc_color_probe(const mb3d_hip_params *params, int32_t x, int32_t y, double *out, uint16_t *words);
+
 /* ---- the painter (paint_oracle.cpp) ---- */
 
 /* Lighting::default (src/lighting.rs:290-392). */

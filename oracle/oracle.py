@@ -196,6 +196,21 @@ def deao_probe(p, job, x, y, rec, rnd=None):
     return n, amb.value, info
 
 
+def color_probe(p, x, y):
+    """Pixel (x, y)'s march with what do_color saw at its surface: dict of
+    the orbit's end v, the surface point c, rout, rold and the otrap and
+    gradient words; None when the pixel is not a surface."""
+    out = np.zeros(8)
+    words = np.zeros(2, np.uint16)
+    r = load().orc_color_probe(C.byref(p), x, y, _ptr(out), _ptr(words))
+    if r < 0:
+        raise RuntimeError("orc_color_probe: " + load().orc_last_error().decode())
+    if r == 0:
+        return None
+    return {"v": out[:3].copy(), "c": out[3:6].copy(), "rout": out[6], "rold": out[7], "otrap": int(words[0]),
+            "gradient": int(words[1])}
+
+
 def deao_row_seed(y):
     """The DE AO generator's seed of picture row y."""
     return load().orc_deao_row_seed(y)

@@ -13,9 +13,15 @@ mathematics fixes:
   - the reference's own scene fixture, tests/data/mandelbulb.m3s (smooth
     normals 3), renders with the reference's coverage window and a
     roughness byte.
-Colour options 1..5 and colour on an iteration are restated only (no known
-answer covers them): parity unpinned; the device is held to this
-restatement bit for bit."""
+Colour options 1..5 and colour on an iteration: the orbit end they read is
+pinned by an independent iterate of the surface point (color_on_it: the
+point after color_on_it - 1 formula steps), and each word inverts to that
+orbit end -- the angle words' cosine and sine against the normalised
+projections of v - c (options 2..4) or of v (option 5, azimuth and
+elevation), option 1's logarithm back to the radii -- so a swapped atan2
+operand, a wrong offset or scale, or a wrong orbit point fails.  The maps
+have no known answer beyond their definitions (the reference's own test
+checks only that colour on an iteration changes the colouring)."""
 import math
 
 import numpy as np
@@ -130,3 +136,67 @@ def test_colour_on_the_start_point_differs():
     assert np.array_equal(a["depth"], b["depth"])
     ob = b["gbuf"][surf, 3] >> 16
     assert (ob == round(math.pi * 5200)).all()  # atan2(0, 0) + pi
+
+
+def surface_probes(sc, n=40, seed=0):
+    p = O.params(sc)
+    r = O.render(p, threads=0)
+    surf = np.flatnonzero(np.isfinite(r["depth"]) & (r["depth"] > 0))
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in rng.choice(surf, size=min(n, surf.size), replace=False):
+        pr = O.color_probe(p, int(i % p.width), int(i // p.width))
+        assert pr is not None
+        # the probe's march is the render's: the same words at that pixel
+        assert pr["otrap"] == r["gbuf"][i, 3] >> 16, i
+        out.append((pr, r["gbuf"][i]))
+    return p, out
+
+
+@pytest.mark.parametrize("option", [1, 2, 3, 4, 5])
+def test_colour_words_invert_to_the_orbit_end(option):
+    sc = H.scene_bulb(64, 48, seed=7, de_mode=H.DE_ANALYTIC_P8)
+    sc.color_option = option
+    p, probes = surface_probes(sc)
+    for pr, g in probes:
+        v, c = pr["v"], pr["c"]
+        assert abs(v @ v - pr["rout"]) <= 1e-12 * pr["rout"]  # rout: the orbit end's squared radius
+        w = pr["otrap"]
+        assert 0 <= w <= 32767
+        if option == 1:
+            mul = float(np.float32(sc.color_mul * 512.0))
+            want = math.log(pr["rout"] / (pr["rold"] + 1.0))
+            if 0 < w < 32767:
+                assert abs(w / mul - want) <= 1.0 / mul
+        elif option in (2, 3, 4):
+            i, j = {2: (0, 1), 3: (0, 2), 4: (1, 2)}[option]
+            d = np.array([v[i] - c[i], v[j] - c[j]])
+            a = w / 5200.0 - math.pi
+            assert np.linalg.norm(np.array([math.cos(a), math.sin(a)]) - d / np.linalg.norm(d)) <= 2.0 / 5200.0
+        else:
+            rho = math.hypot(v[0], v[1])
+            a = w / 5215.0 - math.pi  # the azimuth from the y axis: atan2(x, y)
+            assert np.linalg.norm(np.array([math.sin(a), math.cos(a)]) - v[:2] / rho) <= 2.0 / 5215.0
+            e = (pr["gradient"] / 5215.0 - math.pi) / 2.0  # the elevation: asin(z / r)
+            assert abs(math.sin(e) - v[2] / np.linalg.norm(v)) <= 2.0 / 5215.0
+
+
+@pytest.mark.parametrize("k", [2, 3, 5])
+def test_colour_on_an_iteration_is_the_orbit_after_k_minus_1_steps(k):
+    """colour_on_it = k: the orbit point after k - 1 formula steps from the
+    surface point (Formula::iterate, pinned by the triplex closed form in
+    test_oracle.py), unless it escapes (rstop3d) first."""
+    sc = H.scene_bulb(64, 48, seed=7, de_mode=H.DE_ANALYTIC_P8)
+    sc.color_option = 2
+    sc.color_on_it = k
+    p, probes = surface_probes(sc, n=25, seed=k)
+    f = H.intpow(8, 1)
+    for pr, _ in probes:
+        c = pr["c"]
+        v = np.array([c[0], c[1], c[2], 0.0])
+        j = np.array([c[0], c[1], c[2], 0.0])
+        for _ in range(k - 1):
+            v, j = O.iterate(f, v, j, ade=True)
+            if v[:3] @ v[:3] > p.rstop3d:
+                break
+        assert np.array_equal(v[:3], pr["v"]), (k, v, pr["v"])
