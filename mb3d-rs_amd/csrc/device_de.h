@@ -271,7 +271,9 @@ __device__ float de32_analytic(const DevP &P, const double c[3], float ms, int &
     } else {
         const float zm8 = 8.0f * P.fz_mul[0];
         for (;;) {
-            const float r2 = x * x + y * y + z * z;
+            // (r: the squared radius of (x, y, z), from the line before the
+            // loop or the end of the last iteration)
+            const float r2 = r;
             w = 8.0f * (r2 * r2 * r2 * __builtin_amdgcn_sqrtf(r2)) * w + 1.0f;
             p8f(x, y, z, jx, jy, jz, zm8);
             it++;
@@ -541,9 +543,14 @@ __device__ __forceinline__ void call_slot64(const DevP &P, int n, It64 &s) {
     const double o[3] = {P.opt[n][0], P.opt[n][1], P.opt[n][2]};
     const int kind = P.slot_kind[n];
     if (KIND != K_GEN) {
-        if (kind == MB3D_HIP_FORMULA_INTPOW) {
+        // (K_P8 and K_BOX hold one formula in every counted slot: the other's
+        // code is left out)
+        if (KIND == K_P8 || (KIND == K_HYB && kind == MB3D_HIP_FORMULA_INTPOW)) {
             if (P.is_custom_de) {  // the analytic power-8 extension: dr <- 8 r^7 dr + 1
-                const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                // (s.rout is v's squared radius, v[0]^2 + v[1]^2 + v[2]^2 in
+                // this order: start64 and run64 keep it after every counted
+                // iteration, and these kinds count every slot)
+                const double r2 = s.rout;
                 s.deriv1 = 8.0 * (r2 * r2 * r2 * sqrt(r2)) * s.deriv1 + 1.0;
             }
             int_pow8_64(o[0], v, s.j);
