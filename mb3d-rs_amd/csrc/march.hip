@@ -24,7 +24,7 @@ namespace mb3d {
 // clears the counter words (a kernel, not a memset node: the batch plans
 // are captured into HIP graphs)
 __global__ void k_zero(unsigned long long *p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) p[i] = 0ull;
 }
 
 // the rays of the rectangle's pixels as the network takes them: start

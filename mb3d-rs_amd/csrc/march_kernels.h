@@ -60,16 +60,20 @@ struct SurfRec {
 enum : int32_t { SR_ITERS = 0xFFFF, SR_DELIM = 1 << 16 };  // SR_DELIM: the stop was the DE's, not the limit's
 
 // counter words of Bufs::ctr
+// Each queue counter has a 256-byte line of its own: device-wide atomics on
+// one line serialise, and two counters of one kernel sharing a line would
+// queue behind each other.
 enum : int {
-    C_QUEUE = 0,   // next work item of the march queue
-    C_CAND = 1,    // hand-off list length
-    C_HITS = 2,    // surface pixel list length
-    C_FINQ = 3,    // next item of the f64 finish
-    C_POSTQ = 4,   // next item of the shadow pass
-    C_AOQ = 5,     // next item of the DE AO pass
-    C_SURF = 6,    // refined surface list length (the normals pass's items)
-    C_STATS = 8,   // 19 work counters from here (the 16th: the network's rays)
-    C_WORDS = 27
+    C_LINE = 32,             // words per counter line
+    C_QUEUE = 0 * C_LINE,    // next work item of the march queue
+    C_CAND = 1 * C_LINE,     // hand-off list length
+    C_HITS = 2 * C_LINE,     // surface pixel list length
+    C_FINQ = 3 * C_LINE,     // next item of the f64 finish
+    C_POSTQ = 4 * C_LINE,    // next item of the shadow pass
+    C_AOQ = 5 * C_LINE,      // next item of the DE AO pass
+    C_SURF = 6 * C_LINE,     // refined surface list length (the normals pass's items)
+    C_STATS = 7 * C_LINE,    // 19 work counters from here (the 16th: the network's rays)
+    C_WORDS = C_STATS + 19
 };
 
 struct Bufs {
@@ -846,26 +850,28 @@ __device__ inline void store_normal(const double n[3], int16_t out[3]) {
 // the surface values of surface_found (src/calc.rs:2471-2517) into the outputs
 __device__ inline void write_surface(const DevP &P, const Bufs &B, int pix, const Ray64 &R, const double n[3], float nn,
                               bool de_limited, uint16_t iters, float step_count) {
-    int16_t nq[3];
-    store_normal(n, nq);
-    float g;
-    if (de_limited) {
-        const float lg = (float)log((double)fmaxf(P.de_stop, R.ms) * P.step_width);
-        g = 32767.0f - (nn + P.d_col_plus + P.col_var * lg) * P.mcts_m;
-    } else {
-        g = 32767.0f - nn * P.mcts_m;
-    }
-    const uint32_t si_gradient = clip15(g);
-    const uint32_t otrap = clip15((float)(R.otrap * 4096.0));  // do_color, option 0
-    const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
-    const double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
     if (B.depth) B.depth[pix] = (float)R.mzz;
     if (B.depth64) B.depth64[pix] = R.mzz;
     if (B.iters) B.iters[pix] = iters;
     if (B.normals) {
+        const double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
         for (int k = 0; k < 3; k++) B.normals[3 * pix + k] = l > 0 ? (float)(n[k] / l) : 0.0f;
     }
+    // the G-buffer's words (their f64 square root, divide and logarithm
+    // only when a G-buffer is written)
     if (B.gbuf) {
+        int16_t nq[3];
+        store_normal(n, nq);
+        float g;
+        if (de_limited) {
+            const float lg = (float)log((double)fmaxf(P.de_stop, R.ms) * P.step_width);
+            g = 32767.0f - (nn + P.d_col_plus + P.col_var * lg) * P.mcts_m;
+        } else {
+            g = 32767.0f - nn * P.mcts_m;
+        }
+        const uint32_t si_gradient = clip15(g);
+        const uint32_t otrap = clip15((float)(R.otrap * 4096.0));  // do_color, option 0
+        const uint32_t sh = (uint32_t)rtef(fminf(fmaxf(step_count, 0.0f), 1023.0f));
         uint32_t *w = B.gbuf + 5 * (size_t)pix;
         w[0] = (uint32_t)(uint16_t)nq[0] | ((uint32_t)(uint16_t)nq[1] << 16);
         w[1] = (uint32_t)(uint16_t)nq[2] | (sh << 16);
@@ -911,6 +917,9 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
     Cnt cnt;
     int fr = 0;
     const long long n_items = all_pixels ? P0.npix : (long long)B.ctr[C_CAND];
+    // the list mode's surface records: one per hand-off slot
+    if (!all_pixels && blockIdx.x == 0 && threadIdx.x == 0) B.ctr[C_SURF] = (unsigned long long)n_items;
+    long long cur = 0;  // the lane's hand-off slot
     bool have = false, exhausted = false;
     Ray64 R;
     Refine F;
@@ -938,9 +947,13 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
                 need = false;
                 continue;
             }
-            if (!all_pixels && B.cand[slot].pix < 0) continue;  // an empty slot of a wave's chunk
+            if (!all_pixels && B.cand[slot].pix < 0) {  // an empty slot of a wave's chunk
+                B.surf[slot].pix = -1;
+                continue;
+            }
             have = true;
             need = false;
+            cur = slot;
             if (!all_pixels && BATCH) fr = B.cand[slot].flags >> 8;
             const DevP &F = BATCH ? tab[fr] : P0;
             R.reset(F);
@@ -1091,7 +1104,12 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
         done = true;
     end:
         if (done) have = false;
-        const long long ss = wave_take(surf, &B.ctr[C_SURF], sq, 64, -1);
+        // the surface record: in the list mode at the ray's hand-off slot
+        // (an empty record where the ray ended without a surface), in the
+        // all-pixels mode appended
+        long long ss = cur;
+        if (all_pixels) ss = wave_take(surf, &B.ctr[C_SURF], sq, 64, -1);
+        if (done && !surf && !all_pixels) B.surf[cur].pix = -1;
         if (surf) {
             SurfRec &o = B.surf[ss];
             for (int k = 0; k < 3; k++) o.c[k] = R.c[k];
@@ -1119,14 +1137,17 @@ template <int KIND, int DEM, bool N32, bool BATCH>
 __global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__restrict__ gtab, int nframes) {
     const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
     Cnt cnt;
-    WaveChunk hq;
-    chunk_init(hq);
     const long long n = (long long)B.ctr[C_SURF];
     const long long stride = (long long)gridDim.x * blockDim.x;
-    // whole waves step together, so that the hit list's appends are uniform
-    for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+    const SurfRec none{{0, 0, 0}, 0, 0, 0, -1, 0};
+    // whole waves step together; each turn loads the next turn's record
+    // ahead of its DEs (the load's latency hidden behind them)
+    long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+    SurfRec next = base + lane_id() < n ? B.surf[base + lane_id()] : none;
+    for (; base < n; base += stride) {
         const long long i = base + lane_id();
-        const SurfRec o = i < n ? B.surf[i] : SurfRec{{0, 0, 0}, 0, 0, 0, -1, 0};
+        const SurfRec o = next;
+        next = i + stride < n ? B.surf[i + stride] : none;
         bool hit = false;
         int pix = 0;
         if (o.pix >= 0) {
@@ -1176,10 +1197,12 @@ __global__ __launch_bounds__(256) void k_normals(DevP P0, Bufs B, const DevP *__
             hit = true;
             pix = o.pix;
         }
-        const long long hs = wave_take(hit, &B.ctr[C_HITS], hq, 64, -1);
-        if (hit) B.hits[hs] = pix;
+        // the hit list: the surface records' order, -1 where a record is
+        // empty (no device-wide atomic: one per wave turn serialised the
+        // pass on a single counter)
+        if (i < n) B.hits[i] = hit ? pix : -1;
     }
-    if (lane_id() < hq.left) B.hits[hq.base + lane_id()] = -1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) B.ctr[C_HITS] = (unsigned long long)n;
     flush_counters(cnt, B.ctr);
 }
 
@@ -1370,8 +1393,6 @@ template <int KIND, int DEM, bool BATCH>
 __global__ __launch_bounds__(256) void k_normals_ext(DevP P0, Bufs B, const DevP *__restrict__ gtab, int nframes) {
     const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
     Cnt cnt;
-    WaveChunk hq;
-    chunk_init(hq);
     const long long n_items = (long long)B.ctr[C_SURF];
     const long long stride = (long long)gridDim.x * blockDim.x;
     for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n_items; base += stride) {
@@ -1423,10 +1444,12 @@ __global__ __launch_bounds__(256) void k_normals_ext(DevP P0, Bufs B, const DevP
             hit = true;
             pix = px;
         }
-        const long long hs = wave_take(hit, &B.ctr[C_HITS], hq, 64, -1);
-        if (hit) B.hits[hs] = pix;
+        // the hit list: the surface records' order, -1 where a record is
+        // empty (no device-wide atomic: one per wave turn serialised the
+        // pass on a single counter)
+        if (i < n_items) B.hits[i] = hit ? pix : -1;
     }
-    if (lane_id() < hq.left) B.hits[hq.base + lane_id()] = -1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) B.ctr[C_HITS] = (unsigned long long)n_items;
     flush_counters(cnt, B.ctr);
 }
 
