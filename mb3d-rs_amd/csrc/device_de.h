@@ -164,6 +164,13 @@ __device__ __forceinline__ void next_slot(const DevP &P, int &n, int &btmp) {
     btmp -= 1;
 }
 
+// ln x in FP32 from the hardware's base-2 logarithm (v_log_f32, ~1 ulp for
+// normal x): the FP32 march's DEs steer its steps only, the f64 finish makes
+// every decision that reaches an output, so the library logf's scaling for
+// denormal inputs and its extended-precision product with ln 2 (a dozen
+// instructions a DE) buy nothing here.
+__device__ __forceinline__ float ln_fast(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
+
 // The numeric gradient DE in FP32 (calc_de_part, src/calc.rs:1515-1545):
 // the point and its three offset points iterated in one loop.
 template <int KIND>
@@ -239,7 +246,7 @@ __device__ float de32_numeric(const DevP &P, const double c[3], float ms, int &i
         res = 0.0f;
     } else {
         const float g = (r0 - r1) * (r0 - r1) + (r0 - r2) * (r0 - r2) + (r0 - r3) * (r0 - r3);
-        res = r0 * __logf(r0) * P.d_de_scale / (__builtin_amdgcn_sqrtf(g) + P.de_offset006);
+        res = r0 * ln_fast(r0) * P.d_de_scale * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(g) + P.de_offset006);
     }
     const float mn = ms * 0.25f;
     if (!(res >= mn)) res = mn;  // also NaN
@@ -290,7 +297,7 @@ __device__ float de32_analytic(const DevP &P, const double c[3], float ms, int &
     if (DEM == DEM_BOX)
         res = __builtin_amdgcn_sqrtf(r) * __builtin_amdgcn_rcpf(fabsf(w)) * P.d_de_scale;
     else
-        res = __builtin_amdgcn_sqrtf(r) * 0.25f * __logf(r) * __builtin_amdgcn_rcpf(w) * P.d_de_scale;
+        res = __builtin_amdgcn_sqrtf(r) * 0.25f * ln_fast(r) * __builtin_amdgcn_rcpf(w) * P.d_de_scale;
     const float mn = ms * 0.25f;
     if (!(res >= mn)) res = mn;
     return res;
