@@ -200,3 +200,17 @@ def test_colour_on_an_iteration_is_the_orbit_after_k_minus_1_steps(k):
             if v[:3] @ v[:3] > p.rstop3d:
                 break
         assert np.array_equal(v[:3], pr["v"]), (k, v, pr["v"])
+
+
+def test_colour_on_iteration_2_changes_the_orbit_trap_colouring():
+    """The reference's own check (tests/render.rs:233-241): colour on
+    iteration 2 (color_on_it 3) changes the orbit-trap colouring of the
+    picture, and leaves the depths."""
+    sc = H.scene_bulb(64, 48, seed=7, de_mode=H.DE_ANALYTIC_P8)
+    a = O.render(O.params(sc), threads=0)
+    sc.color_on_it = 3
+    b = O.render(O.params(sc), threads=0)
+    surf = np.isfinite(a["depth"]) & (a["depth"] > 0)
+    assert np.array_equal(a["depth"], b["depth"])
+    ot = lambda r: int((r["gbuf"][surf, 3] >> 16).astype(np.uint64).sum())
+    assert ot(a) != ot(b)
