@@ -326,7 +326,10 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
     d.col_var = p.col_var_de_stop_mul;
     d.fovx_off = p.fovx_off;
     d.fovx_mul = p.fovx_mul;
-    d.guard = 1.0f + (o.guard_band > 0.0f ? o.guard_band : 0.25f);
+    // (0.05: the FP32 DE's error near the surface is ~1e-5 of it at the
+    // zooms that take the FP32 march; 0.25 and 0.02 gave the same accuracy
+    // against the oracle on the 1080p bulb, and 0.05 takes 6 % less time)
+    d.guard = 1.0f + (o.guard_band > 0.0f ? o.guard_band : 0.05f);
     d.f_d_rstop = (float)p.d_rstop;
     d.f_rstop3d = (float)p.rstop3d;
     d.d_rstop = p.d_rstop;

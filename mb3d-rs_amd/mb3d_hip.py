@@ -887,7 +887,7 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
-def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=16, guard_band=0.25, it_split=0, mlp=None,
+def options(precision=PREC_AUTO, tile=(8, 8), waves_per_cu=16, guard_band=0.0, it_split=0, mlp=None,
             conf_hit=0.0, conf_miss=0.0, margin_rel=0.0, margin_abs=0.0, refill_min=0, cone=False,
             cone_gate=0.0) -> Options:
     """Options; `mlp` (an Mlp) turns the warm start on: hit jumps with the
