@@ -508,10 +508,14 @@ __global__ __launch_bounds__(256) void k_cone(DevP P0, const DevP *__restrict__ 
         double c0[3], v0[3];
         frac_ray(P, (double)x0 + (P.tile_w - 1.0) * 0.5, fy0 + (P.tile_h - 1.0) * 0.5, c0, v0);
         // the tile's spread from the coarse ray, at the start and per step
-        // width of depth (step widths)
+        // width of depth (step widths): the widest gaps are the corners'
+        // (the starts are affine in the pixel, and the directions' angle
+        // from the middle one grows with both offsets), so the four corner
+        // rays give the maxima over the tile's rays
         double a = 0.0, slope = 0.0;
-        for (int j = 0; j < P.tile_h; j++) {
-            for (int i = 0; i < P.tile_w; i++) {
+        const int sj = P.tile_h > 1 ? P.tile_h - 1 : 1, si = P.tile_w > 1 ? P.tile_w - 1 : 1;
+        for (int j = 0; j < P.tile_h; j += sj) {
+            for (int i = 0; i < P.tile_w; i += si) {
                 double c[3], v[3];
                 if (x0 + i < P.width && y0 + j < P.rows)
                     pixel_ray_tab(P, rtab, x0 + i, y0 + j, c, v);
