@@ -175,8 +175,9 @@ def note(what: str) -> None:
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}): every box of this build has one GPU, "
-              "and the marcher shards by pixel bands or frames per GPU (see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for {a.gpus}): the marcher shards by pixel bands or whole "
+              "frames per GPU, with no exchange between GPUs, so the rate of N GPUs is N times one GPU's share",
+              file=sys.stderr)
         sys.exit(2)
     import torch
 
@@ -549,7 +550,7 @@ def cpu_leg(scenes, params, a, legs, net):
 
     host_threads = os.cpu_count() or 1
     share = len(os.sched_getaffinity(0))
-    # the box's CPU share for one GPU is 16 cores (its worker-pool limit)
+    # a host's CPU share for one GPU: at most 16 cores
     threads = a.cpu_threads or min(share, 16)
     keys = ("mask_agree", "depth_rel_p99", "depth_rel_max", "depth_ok_frac", "normal_ok_frac", "normal_deg_p99",
             "iters_equal_inner")

@@ -1,5 +1,5 @@
 """Parity metrics of a G-buffer against the f64 oracle's (test and bench
-infrastructure).  Tolerances (BASELINE): hit/miss mask agreement >= 99.9 %,
+infrastructure).  Tolerances (the target's): hit/miss mask agreement >= 99.9 %,
 relative depth error <= 1e-4 of the scene scale, normal error < 0.5 degrees,
 iteration counts equal away from silhouette pixels (a pixel whose 8
 neighbours do not all share its mask)."""

@@ -1,5 +1,5 @@
 """Parity of the HIP marcher (through the C ABI) with the f64 CPU oracle on
-the same seeded scenes.  Tolerances (BASELINE): mask agreement >= 99.9 %,
+the same seeded scenes.  Tolerances (the target's): mask agreement >= 99.9 %,
 relative depth error <= 1e-4 of the scene scale (bulb 2, box 4), normal
 error < 0.5 degrees, iteration counts equal away from silhouettes.  The
 depth and normal bars hold for >= 99.9 % of the pixels both sides hit (the
