@@ -176,7 +176,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU (asked for {a.gpus}): the marcher shards by pixel bands or whole "
-              "frames per GPU, with no exchange between GPUs, so the rate of N GPUs is N times one GPU's share",
+              "frames per GPU, each GPU marching its own share with no exchange between GPUs",
               file=sys.stderr)
         sys.exit(2)
     import torch
