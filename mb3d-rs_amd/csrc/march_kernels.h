@@ -72,7 +72,8 @@ enum : int {
     C_POSTQ = 4 * C_LINE,    // next item of the shadow pass
     C_AOQ = 5 * C_LINE,      // next item of the DE AO pass
     C_SURF = 6 * C_LINE,     // refined surface list length (the normals pass's items)
-    C_STATS = 7 * C_LINE,    // 19 work counters from here (the 16th: the network's rays)
+    C_CONEQ = 7 * C_LINE,    // next tile of the cone rays
+    C_STATS = 8 * C_LINE,    // 19 work counters from here (the 16th: the network's rays)
     C_WORDS = C_STATS + 19
 };
 
@@ -484,6 +485,10 @@ constexpr double kConeMinStep = 0.05, kConeMinShare = 0.1, kConeMargin = 1.5;  /
 // gate (not NULL): the network's logit per tile (k_mlp_propose's TILES
 // pass); a tile at or above gate_logit marches no cone ray (depth 0: its
 // rays march plainly).
+// Persistent: one cone ray per lane, one f64 DE per turn, and a lane whose
+// ray has ended takes the next tile of the queue (C_CONEQ, chunks of 64)
+// once 8 of the wave's lanes are idle, so that a wave does not wait for its
+// longest ray.
 template <int KIND, int DEM, bool BATCH>
 __global__ __launch_bounds__(256) void k_cone(DevP P0, const DevP *__restrict__ gtab, int nframes,
                                               const double2 *__restrict__ rtab, ConeRes *__restrict__ out,
@@ -492,56 +497,90 @@ __global__ __launch_bounds__(256) void k_cone(DevP P0, const DevP *__restrict__ 
     const GlobalFrames tab{reinterpret_cast<const char *>(gtab)};
     Cnt cnt;
     const long long ntiles = P0.nslots_q >> P0.tn_shift;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < ntiles; g += stride) {
-        const long long q = g << P0.tn_shift;
-        if (gate && gate[q].y >= gate_logit) {
-            out[g] = ConeRes{0.0, 0.0f, 0};
-            continue;
-        }
-        const int fr = BATCH ? find_frame(tab, nframes, q) : 0;
-        const DevP &P = BATCH ? tab[fr] : P0;
-        int tx;
-        const int ty = div_rem((int)((q - P.q_begin) >> P.tn_shift), P.tiles_x, tx);
-        const int x0 = tx << P.tw_shift, y0 = ty * P.tile_h;  // (y0: a row of the rectangle)
-        const double fy0 = (double)(P.row0 + y0);
-        double c0[3], v0[3];
-        frac_ray(P, (double)x0 + (P.tile_w - 1.0) * 0.5, fy0 + (P.tile_h - 1.0) * 0.5, c0, v0);
-        // the tile's spread from the coarse ray, at the start and per step
-        // width of depth (step widths): the widest gaps are the corners'
-        // (the starts are affine in the pixel, and the directions' angle
-        // from the middle one grows with both offsets), so the four corner
-        // rays give the maxima over the tile's rays
-        double a = 0.0, slope = 0.0;
-        const int sj = P.tile_h > 1 ? P.tile_h - 1 : 1, si = P.tile_w > 1 ? P.tile_w - 1 : 1;
-        for (int j = 0; j < P.tile_h; j += sj) {
-            for (int i = 0; i < P.tile_w; i += si) {
-                double c[3], v[3];
-                if (x0 + i < P.width && y0 + j < P.rows)
-                    pixel_ray_tab(P, rtab, x0 + i, y0 + j, c, v);
-                else
-                    frac_ray(P, (double)(x0 + i), fy0 + j, c, v);
-                const double dc[3] = {c[0] - c0[0], c[1] - c0[1], c[2] - c0[2]};
-                const double dv[3] = {v[0] - v0[0], v[1] - v0[1], v[2] - v0[2]};
-                a = fmax(a, sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]));
-                slope = fmax(slope, sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]));
+    WaveChunk tq;
+    chunk_init(tq);
+    bool have = false, exhausted = false;
+    long long g = 0;  // the lane's tile
+    int fr = 0;
+    double c[3] = {0, 0, 0}, v0[3] = {0, 0, 0};
+    double a = 0.0, slope = 0.0, lip = 0.0, div = 1.0;
+    double last_de = 0.0, last_step = 0.0, t = 0.0;
+    float count = 0.0f;
+    uint32_t guard = 0;
+    for (;;) {
+        bool need = !have && !exhausted;
+        const int idle = __popcll(__ballot(need));
+        const bool refill = idle > 0 && (idle >= 8 || idle == __popcll(__ballot(!exhausted)));
+        for (int tries = 0; refill && tries < 4 && __any(need); tries++) {
+            const long long gi = wave_take(need, &ctr[C_CONEQ], tq, 64, ntiles);
+            if (!need) continue;
+            if (gi < 0) {
+                exhausted = true;
+                need = false;
+                continue;
             }
+            const long long q = gi << P0.tn_shift;
+            if (gate && gate[q].y >= gate_logit) {
+                out[gi] = ConeRes{0.0, 0.0f, 0};
+                continue;
+            }
+            need = false;
+            have = true;
+            g = gi;
+            fr = BATCH ? find_frame(tab, nframes, q) : 0;
+            const DevP &P = BATCH ? tab[fr] : P0;
+            int tx;
+            const int ty = div_rem((int)((q - P.q_begin) >> P.tn_shift), P.tiles_x, tx);
+            const int x0 = tx << P.tw_shift, y0 = ty * P.tile_h;  // (y0: a row of the rectangle)
+            const double fy0 = (double)(P.row0 + y0);
+            double c0[3];
+            frac_ray(P, (double)x0 + (P.tile_w - 1.0) * 0.5, fy0 + (P.tile_h - 1.0) * 0.5, c0, v0);
+            // the tile's spread from the coarse ray, at the start and per
+            // step width of depth (step widths): the widest gaps are the
+            // corners' (the starts are affine in the pixel, and the
+            // directions' angle from the middle one grows with both
+            // offsets), so the four corner rays give the maxima over the
+            // tile's rays
+            a = 0.0;
+            slope = 0.0;
+            const int sj = P.tile_h > 1 ? P.tile_h - 1 : 1, si = P.tile_w > 1 ? P.tile_w - 1 : 1;
+            for (int j = 0; j < P.tile_h; j += sj) {
+                for (int i = 0; i < P.tile_w; i += si) {
+                    double cc[3], vv[3];
+                    if (x0 + i < P.width && y0 + j < P.rows)
+                        pixel_ray_tab(P, rtab, x0 + i, y0 + j, cc, vv);
+                    else
+                        frac_ray(P, (double)(x0 + i), fy0 + j, cc, vv);
+                    const double dc[3] = {cc[0] - c0[0], cc[1] - c0[1], cc[2] - c0[2]};
+                    const double dv[3] = {vv[0] - v0[0], vv[1] - v0[1], vv[2] - v0[2]};
+                    a = fmax(a, sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]));
+                    slope = fmax(slope, sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]));
+                }
+            }
+            a /= P.step_width;
+            slope /= P.step_width;
+            lip = kConeMargin / (double)fminf(fmaxf(P.z_step_div, 0.05f), 1.0f);
+            div = 1.0 + slope * lip + (double)P.de_stop * (double)P.de_stop_factor;
+            last_de = __builtin_inf();
+            last_step = 0.0;
+            t = 0.0;
+            for (int k = 0; k < 3; k++) c[k] = c0[k];
+            count = 0.0f;
+            guard = 0;
+            cnt.cone_rays++;
         }
-        a /= P.step_width;
-        slope /= P.step_width;
-        const double lip = kConeMargin / (double)fminf(fmaxf(P.z_step_div, 0.05f), 1.0f);
-        const double div = 1.0 + slope * lip + (double)P.de_stop * (double)P.de_stop_factor;
-        double last_de = __builtin_inf(), last_step = 0.0, t = 0.0;
-        double c[3] = {c0[0], c0[1], c0[2]};
-        float count = 0.0f;
-        cnt.cone_rays++;
-        for (int guard = 0; guard < 2000000; guard++) {
-            const float ms = (float)((double)P.de_stop * (1.0 + t * (double)P.de_stop_factor));
-            const De64 r = de64<KIND, DEM>(P, c[0], c[1], c[2], P.max_it, 0, ms);
-            cnt.de64++;
-            cnt.p8_64 += r.p8;
-            cnt.box_64 += r.box;
-            if (r.it_result >= r.max_its) break;
+        if (__all(exhausted && !have)) break;
+        if (!have) continue;
+        const DevP &P = BATCH ? tab[fr] : P0;
+        // one step of the cone ray
+        bool done = ++guard > 2000000u;
+        const float ms = (float)((double)P.de_stop * (1.0 + t * (double)P.de_stop_factor));
+        const De64 r = de64<KIND, DEM>(P, c[0], c[1], c[2], P.max_it, 0, ms);
+        cnt.de64++;
+        cnt.p8_64 += r.p8;
+        cnt.box_64 += r.box;
+        if (!done && r.it_result >= r.max_its) done = true;
+        if (!done) {
             double de = r.de;
             if (de > last_de + last_step) de = last_de + last_step;
             const double rsf =
@@ -551,18 +590,25 @@ __global__ __launch_bounds__(256) void k_cone(DevP P0, const DevP *__restrict__ 
             const double cap = (double)(fmaxf(ms, 0.4f) * P.mh04zsd);
             const double sd = fmin((de - reach - (double)P.ms_de_sub * stop) * (double)P.z_step_div * rsf / div, cap);
             const double d = fmax(0.11, (de - (double)P.ms_de_sub * stop) * (double)P.z_step_div * rsf);
-            if (sd < kConeMinStep * stop || sd < kConeMinShare * fmin(d, cap)) break;
-            if (P.dfog_on_it == 0 || r.it_result == P.dfog_on_it) count += (float)(fmin(sd, P.zend - t) / d);
-            if (t + sd > P.zend) {
-                t = __builtin_inf();
-                break;
+            if (sd < kConeMinStep * stop || sd < kConeMinShare * fmin(d, cap)) {
+                done = true;
+            } else {
+                if (P.dfog_on_it == 0 || r.it_result == P.dfog_on_it) count += (float)(fmin(sd, P.zend - t) / d);
+                if (t + sd > P.zend) {
+                    t = __builtin_inf();
+                    done = true;
+                } else {
+                    t += sd;
+                    for (int k = 0; k < 3; k++) c[k] += v0[k] * sd;
+                    last_de = de;
+                    last_step = sd;
+                }
             }
-            t += sd;
-            for (int k = 0; k < 3; k++) c[k] += v0[k] * sd;
-            last_de = de;
-            last_step = sd;
         }
-        out[g] = ConeRes{t, count, 0};
+        if (done) {
+            out[g] = ConeRes{t, count, 0};
+            have = false;
+        }
     }
     cnt.cone_its = cnt.p8_64 + cnt.box_64 + cnt.oth_64;
     cnt.cone_des = cnt.de64;
