@@ -740,6 +740,46 @@ def scene_m3s_mandelbulb(width=800, height=600) -> Scene:
     return s
 
 
+def scene_m3s_quaternion_julia(width=640, height=480) -> Scene:
+    """The reference's scene fixture tests/data/quaternion_julia.m3s (the 4D
+    Quaternion Julia set through the 4D hybrid loop with a 4D rotation), its
+    keys transcribed: iterations 12, zoom 0.8, z_start -3, rotation (23, 17,
+    0) degrees, rotation_4d (0, 20, 0) degrees, Julia at (-0.2, 0.6, 0.2, 0),
+    Quaternion with yw multiplier -1 and w add 0.  (Its colours are the
+    painter's.)"""
+    s = scene_default()
+    s.width, s.height = width, height
+    s.iterations = 12
+    s.zoom = 0.8
+    s.z_start = -3.0
+    set_camera(s, vgrads_from_angles(math.radians(23.0), math.radians(17.0), 0.0))
+    s.rot_4d[:] = [0.0, math.radians(20.0), 0.0]
+    s.julia = 1
+    s.julia_c[:] = [-0.2, 0.6, 0.2, 0.0]
+    set_formulas(s, [formula("Quaternion", yw_mul=-1.0, w_add=0.0)])
+    return s
+
+
+def scene_m3s_interpolation_boxes(width=640, height=480) -> Scene:
+    """The reference's scene fixture tests/data/interpolation_boxes.m3s (two
+    Amazing Boxes blended 60 : 40 in every iteration, MB3D's interpolation
+    hybrid, analytic DE), its keys transcribed: iterations 30, zoom 0.35,
+    z_start -8, rotation (25, -35, 0) degrees, smooth normals 2; Amazing Box
+    (scale -1.5, min_r 0.5, fold 1) x2 and Amazing Box (scale 2.2, min_r 0.5,
+    fold 1) x1."""
+    s = scene_default()
+    s.width, s.height = width, height
+    s.iterations = 30
+    s.interpolation = 1
+    s.interpolation_w[:] = [0.6, 0.4]
+    s.zoom = 0.35
+    s.z_start = -8.0
+    set_camera(s, vgrads_from_angles(math.radians(25.0), math.radians(-35.0), 0.0))
+    s.smooth_normals = 2
+    set_formulas(s, [amazing_box(-1.5, 0.5, 1.0, 2), amazing_box(2.2, 0.5, 1.0, 1)])
+    return s
+
+
 def _box_de(p, scale, min_r, fold, its):
     """Analytic Amazing Box DE |z|/|dr| at a point (harness: camera placement)."""
     z = np.array(p, float)

@@ -61,3 +61,44 @@ def compare(gpu: dict, ref: dict, w: int, h: int, step_width: float, scale: floa
         "iters_equal_inner": float(it_eq.mean()) if it_eq.size else 1.0,
         "inner": int(inner.sum()),
     }
+
+
+def card_metrics(g_dev, g_ref) -> dict:
+    """The reference's own card-against-CPU measures (gbuffer_matches_cpu,
+    src/gpu.rs:4178-4244) on two G-buffers of 5-word card records: same
+    background/object share, colour values (orbit trap, 15 bits) off by more
+    than 500, flag bits (bit 15 of the trap and the gradient) that differ,
+    mean |zpos difference| (of 32768), mean normal cosine and the share of
+    normals off by more than 25 degrees (cosine < 0.9)."""
+    a = np.asarray(g_dev, np.uint32).reshape(-1, 5)
+    b = np.asarray(g_ref, np.uint32).reshape(-1, 5)
+    bg_a, bg_b = (a[:, 2] >> 16) >= 32768, (b[:, 2] >> 16) >= 32768
+    both = ~bg_a & ~bg_b
+    ot_a, ot_b = (a[both, 3] >> 16), (b[both, 3] >> 16)
+    gr_a, gr_b = (a[both, 3] & 0xFFFF), (b[both, 3] & 0xFFFF)
+    col_off = np.abs((ot_a & 0x7FFF).astype(np.int64) - (ot_b & 0x7FFF).astype(np.int64)) > 500
+    flag_off = (((ot_a ^ ot_b) & 0x8000) != 0) | (((gr_a ^ gr_b) & 0x8000) != 0)
+    zd = np.abs((a[both, 2] >> 16).astype(np.int64) - (b[both, 2] >> 16).astype(np.int64))
+
+    def normals(g):
+        n0 = (g[:, 0] & 0xFFFF).astype(np.uint16).view(np.int16).astype(np.float64)
+        n1 = (g[:, 0] >> 16).astype(np.uint16).view(np.int16).astype(np.float64)
+        n2 = (g[:, 1] & 0xFFFF).astype(np.uint16).view(np.int16).astype(np.float64)
+        return np.stack([n0, n1, n2], 1) / 32767.0
+
+    dots = np.sum(normals(a[both]) * normals(b[both]), 1)
+    nb = max(int(both.sum()), 1)
+    return {
+        "same": float((bg_a == bg_b).mean()),
+        "colour_off": float(col_off.sum()) / nb,
+        "flags_off": float(flag_off.sum()) / nb,
+        "zpos_diff": float(zd.sum()) / nb,
+        "normal_cos": float(dots.sum()) / nb,
+        "normals_bad": float((dots < 0.9).sum()) / nb,
+    }
+
+
+def card_bar(m: dict) -> bool:
+    """gbuffer_matches_cpu's asserts."""
+    return (m["same"] > 0.99 and m["colour_off"] < 0.03 and m["flags_off"] < 0.01 and m["zpos_diff"] < 20.0
+            and m["normal_cos"] > 0.97 and m["normals_bad"] < 0.03)

@@ -1,0 +1,53 @@
+"""The reference's own card-against-CPU scenes (src/gpu.rs:3967-3984): its
+scene fixtures tests/data/mandelbulb.m3s (the power-8 bulb close up, smooth
+normals 3), quaternion_julia.m3s (the 4D hybrid loop with a 4D rotation) and
+interpolation_boxes.m3s (two Amazing Boxes blended 60 : 40, smooth normals
+2), and the 4D interpolation of two Quaternions the reference builds from
+the second, each at half size as the reference renders them.  The backend's
+G-buffer is held to the reference's own tolerances for its card
+(gbuffer_matches_cpu, src/gpu.rs:4178-4244) and to this backend's plain bar
+against the f64 oracle."""
+import numpy as np
+import pytest
+
+import mb3d_hip as H
+import oracle as O
+import parity
+
+pytestmark = pytest.mark.gpu
+
+
+def quaternion_interpolation(width, height):
+    """interpolation_matches_cpu's second scene: quaternion_julia.m3s with
+    interpolation 0.5, 0.5 and a second Quaternion (yw multiplier -1, w add
+    0.3)."""
+    s = H.scene_m3s_quaternion_julia(width, height)
+    s.interpolation = 1
+    s.interpolation_w[:] = [0.5, 0.5]
+    H.set_formulas(s, [H.formula("Quaternion", yw_mul=-1.0, w_add=0.0), H.formula("Quaternion", yw_mul=-1.0, w_add=0.3)])
+    return s
+
+
+CASES = {
+    "mandelbulb.m3s": (lambda: H.scene_m3s_mandelbulb(400, 300), 2.0),
+    "quaternion_julia.m3s": (lambda: H.scene_m3s_quaternion_julia(320, 240), 2.0),
+    "interpolation_boxes.m3s": (lambda: H.scene_m3s_interpolation_boxes(320, 240), 4.0),
+    "quaternion interpolation": (lambda: quaternion_interpolation(320, 240), 2.0),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_reference_fixture_matches_the_oracle(hip, name):
+    make, scale = CASES[name]
+    sc = make()
+    p = hip.params_from_scene(sc)
+    g = hip.march(p)
+    r = O.render(O.params(sc), threads=0)
+    assert np.isfinite(r["depth"]).mean() > 0.05
+    cm = parity.card_metrics(g["gbuf"], r["gbuf"])
+    print(name, cm)
+    assert parity.card_bar(cm), cm
+    m = parity.compare(g, r, p.width, p.height, p.step_width, scale)
+    print(name, m)
+    assert m["mask_agree"] >= 0.999, m
+    assert m["depth_ok_frac"] >= 0.999 and m["normal_ok_frac"] >= 0.999, m
