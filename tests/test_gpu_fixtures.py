@@ -51,3 +51,25 @@ def test_reference_fixture_matches_the_oracle(hip, name):
     print(name, m)
     assert m["mask_agree"] >= 0.999, m
     assert m["depth_ok_frac"] >= 0.999 and m["normal_ok_frac"] >= 0.999, m
+
+
+def test_deao_on_the_mandelbulb_fixture(hip):
+    """deao_inside_matches_cpu's outside case (src/gpu.rs:4125-4156):
+    mandelbulb.m3s at 0.4 of its size with the default DE AO (quality 1,
+    full length); the reference holds its card to a mean |AO difference|
+    under 1 % and under 1 % of the pixels off by more than 10 %; the device
+    equals the oracle's AO words."""
+    sc = H.scene_m3s_mandelbulb(320, 240)
+    p = hip.params_from_scene(sc)
+    job = H.post_job(deao_quality=1)
+    g = hip.march(p, post=job)
+    r = O.render(O.params(sc), job, threads=0)
+    surf = (g["gbuf"][:, 2] >> 16) < 32768
+    assert surf.sum() > 100
+    ag = (g["gbuf"][surf, 4] & 0xFFFF).astype(np.float64) / 16383.0
+    ao = (r["gbuf"][surf, 4] & 0xFFFF).astype(np.float64) / 16383.0
+    d = np.abs(ag - ao)
+    print(f"DE AO on mandelbulb.m3s: {surf.sum()} pixels, mean difference {d.mean():.2e}, "
+          f"equal {(d == 0).mean():.5f}")
+    assert d.mean() < 0.01 and (d > 0.1).mean() < 0.01
+    assert (d == 0).mean() >= 0.999
