@@ -29,9 +29,13 @@
 // its surface point against the march's depth, its ray directions
 // (dithered too) against a Rodrigues rotation of the ray set onto the
 // normal, its openness against a fine DE walk along each ray from the
-// (random) first step (test_deao_against_brute_force); its generator
-// against the C runtime's rand() outputs and the row's draw order
-// (test_deao_random_first_steps_draw_along_the_row); the hit pass's
+// (random) first step (test_deao_against_brute_force); each walk's
+// samples against the DE evaluated apart at their points, each step the
+// larger of that DE and the walk's growth, the end at the openness floor or
+// the ray's length
+// (test_deao_walk_steps_by_the_larger_of_the_de_and_its_growth); its
+// generator against the C runtime's rand() outputs and the row's draw
+// order (test_deao_random_first_steps_draw_along_the_row); the hit pass's
 // deao_1pos against deao_row on view rays (test_oracle_rays.py).  The DE's
 // size against the brute-force distance to the set, analytic and numeric
 // (test_oracle.py).  The smoothing sweeps and normals on the smooth
@@ -41,8 +45,6 @@
 // do_color_on_it): the orbit end they read against an independent iterate
 // of the surface point, each word inverted back to that orbit end
 // (test_oracle_normals.py).
-// Parity unpinned: the AO walk's step rule (dt += max(DE, dt dsm)) is
-// pinned only through the openness it reaches (a doubled step passes).
 #include "mb3d_oracle.h"
 
 #include <algorithm>
@@ -1692,6 +1694,10 @@ struct DeaoProbe {
     double first[33];    // each ray's first step over step_ao * d_step_mul (1 without the random first step)
     double step_ao, max_dist, ms_ao, md_d10, de_mul, dsm;
     int count;
+    // the walk of one ray (trace_ray >= 0): its samples' distances and DEs
+    int trace_ray = -1, trace_cap = 0, trace_n = 0;
+    double *trace_t = nullptr, *trace_de = nullptr;
+    float ms_walk = 0.0f;  // the marcher's DE stop during the walks (the DE's floor is a quarter of it)
 };
 
 // deao_row for one pixel (src/deao.rs:268-418); the marcher's parameters
@@ -1754,6 +1760,10 @@ void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_h
             std::memcpy(m.it.c, ic, sizeof ic);
             add_weight(m.it.c, sv, dt1);
             double dt2 = m.calc_de();
+            if (probe && k == probe->trace_ray && probe->trace_n < probe->trace_cap) {
+                probe->trace_t[probe->trace_n] = dt1;
+                probe->trace_de[probe->trace_n++] = dt2;
+            }
             s_tmp = std::fmin(s_tmp, (dt2 - ms_de_stop + dt1 * md_d10) / dt1);
             if (s_tmp < 0.02) break;
             dt1 += std::fmax(dt2, dt1 * dsm);
@@ -1774,6 +1784,7 @@ void deao_pixel(Marcher &m, int x, int y, Px &si, const Rays &base, const mb3d_h
         probe->de_mul = de_mul;
         probe->dsm = dsm;
         probe->count = ray_count;
+        probe->ms_walk = m.ms_de_stop;
     }
     double amount = 0.0;
     for (int k = 0; k < ray_count; k++) {
@@ -2434,8 +2445,7 @@ extern "C" int32_t orc_deao_probe(const mb3d_hip_params *pp, const mb3d_hip_post
     Marcher m(pc);
     Px si;
     from_words(rec, si);
-    DeaoProbe pr;
-    std::memset(&pr, 0, sizeof pr);
+    DeaoProbe pr{};
     deao_pixel(m, x, y, si, base, *job, rnd, &pr);
     *amb = si.amb_shadow;
     for (int i = 0; i < 3; i++) out[i] = pr.ic[i];
@@ -2446,6 +2456,35 @@ extern "C" int32_t orc_deao_probe(const mb3d_hip_params *pp, const mb3d_hip_post
     for (int i = 0; i < 6; i++) out[135 + i] = info[i];
     for (int k = 0; k < 33; k++) out[141 + k] = pr.first[k];
     return pr.count;
+}
+
+// The walk of ray `ray` of one pixel's DE AO (deao_pixel on the card record
+// `rec`): the samples' distances along the ray in steps (t) and the DE the
+// walk took at each (de), at most `cap`; out2 = the marcher's DE stop during
+// the walk (the DE's floor is a quarter of it) and the ray's openness;
+// returns the sample count (-1: bad arguments or no such ray).
+extern "C" int32_t orc_deao_walk(const mb3d_hip_params *pp, const mb3d_hip_post_job *job, int32_t x, int32_t y,
+                                 const uint32_t rec[5], int32_t *rnd, int32_t ray, int32_t cap, double *t,
+                                 double *de, double out2[2]) {
+    if (check_params(pp) || !job || !job->deao || ray < 0 || cap <= 0 || !t || !de) return -1;
+    if (job->deao_first_step_random && !rnd) return -1;
+    if (check_job(*job)) return -1;
+    mb3d_hip_params pc = *pp;
+    pc.de_add_steps = 5;  // DeaoRows::new (src/deao.rs:254-260)
+    Rays base = build_rays(job->deao_quality);
+    Marcher m(pc);
+    Px si;
+    from_words(rec, si);
+    DeaoProbe pr{};
+    pr.trace_ray = ray;
+    pr.trace_cap = cap;
+    pr.trace_t = t;
+    pr.trace_de = de;
+    deao_pixel(m, x, y, si, base, *job, rnd, &pr);
+    if (ray >= pr.count) return -1;
+    out2[0] = (double)pr.ms_walk;
+    out2[1] = pr.min_ra[ray] / pr.de_mul;
+    return pr.trace_n;
 }
 
 // The DE AO's row generator: the seed of row y, then n states and numbers

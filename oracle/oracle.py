@@ -196,6 +196,23 @@ def deao_probe(p, job, x, y, rec, rnd=None):
     return n, amb.value, info
 
 
+def deao_walk(p, job, x, y, rec, ray, rnd=None, cap=4096):
+    """The walk of one DE AO ray of pixel (x, y) on its card record: (t, de,
+    ms_walk, openness) -- the samples' distances in steps, the DE the walk
+    took at each, the marcher's DE stop during the walk and the ray's
+    openness (min_ra / de_mul).  `rnd` as for deao_probe."""
+    t = np.zeros(cap)
+    de = np.zeros(cap)
+    out2 = np.zeros(2)
+    r = np.ascontiguousarray(rec, np.uint32)
+    st = C.c_int32(0 if rnd is None else int(np.int32(np.uint32(rnd & 0xFFFFFFFF))))
+    n = load().orc_deao_walk(C.byref(p), C.byref(job), x, y, _ptr(r), None if rnd is None else C.byref(st),
+                             ray, cap, _ptr(t), _ptr(de), _ptr(out2))
+    if n < 0:
+        raise RuntimeError("orc_deao_walk failed")
+    return t[:n].copy(), de[:n].copy(), float(out2[0]), float(out2[1])
+
+
 def color_probe(p, x, y):
     """Pixel (x, y)'s march with what do_color saw at its surface: dict of
     the orbit's end v, the surface point c, rout, rold and the otrap and

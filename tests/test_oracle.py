@@ -439,7 +439,7 @@ def test_deao_against_brute_force(quality, dither, rnd):
     there (not held to the ray's length).
     A swapped axis of the AO frame, a sign of its quaternion, a dropped DE
     stop or swapped dither offsets fails here; the walk's step rule is
-    pinned only through the openness it reaches (a doubled step passes)."""
+    pinned sample by sample in the next test."""
     sc = preset("Integer Power", 64, 48)
     sc.first_step_random = 0
     p = O.params(sc)
@@ -489,6 +489,68 @@ def test_deao_against_brute_force(quality, dither, rnd):
     gaps = np.abs(np.array(gaps))
     assert np.median(gaps) <= 0.025 and (gaps < 0.02).mean() >= 0.5, (np.median(gaps), (gaps < 0.02).mean())
     assert gaps.max() <= 0.3, gaps.max()
+
+
+@pytest.mark.parametrize("quality,rnd", [(1, False), (2, True), (3, False)])
+def test_deao_walk_steps_by_the_larger_of_the_de_and_its_growth(quality, rnd):
+    """The walk along each DE AO ray (src/deao.rs:357-382), sample by
+    sample, against the DE evaluated apart at each sample's point (de_many,
+    a fresh marcher on the point surface + t * ray step):
+      - the first sample at step_ao * dsm (times the ray's random factor);
+      - the walk's DE at each sample equals the DE there;
+      - each next sample lies the larger of that DE and dsm times the
+        distance walked further on (src/deao.rs:379), held at max_dist;
+      - the walk ends at the first sample whose running openness
+        min((DE - stop + t md) / t) falls under 0.02, or at max_dist, and
+        the ray's openness is that running minimum;
+      - the samples at most 2 + log(max_dist / t0) / log(1 + dsm): the
+        walk grows at least geometrically.
+    The smaller of the two instead of the larger, a step of the DE alone or
+    of the growth alone, a doubled step, an assignment in place of the
+    increment, or a walk not held at max_dist fails here; the openness
+    the walk reaches is held to a fine walk in
+    test_deao_against_brute_force."""
+    sc = preset("Integer Power", 64, 48)
+    sc.first_step_random = 0
+    p = O.params(sc)
+    job = H.post_job(deao_quality=quality, deao_dither=0, deao_first_step_random=rnd)
+    r = O.render(p, job, threads=0)
+    surf = np.flatnonzero(np.isfinite(r["depth"]) & (r["depth"] > 0))
+    rng = np.random.default_rng(100 + quality)
+    base = _ao_set(quality)
+    checked = long_walks = geometric = 0
+    for i in rng.choice(surf, size=min(12, surf.size), replace=False):
+        x, y = int(i % p.width), int(i // p.width)
+        rec = r["gbuf"][i]
+        st = _row_state(p, r["gbuf"], x, y, len(base)) if rnd else None
+        n, _, pr = O.deao_probe(p, job, x, y, rec, rnd=st)
+        for k in range(n):
+            t, de, ms_walk, opn = O.deao_walk(p, job, x, y, rec, k, rnd=st)
+            assert t.size >= 1
+            t0 = pr["step_ao"] * pr["dsm"] * pr["first"][k]
+            assert t[0] == pytest.approx(t0, rel=1e-12)
+            pts = pr["point"][None, :] + t[:, None] * pr["dirs"][k][None, :]
+            de_apart, _ = O.de_many(p, pts)
+            floor = 0.25 * max(ms_walk, float(p.de_stop))
+            free = de_apart > floor
+            assert np.allclose(de[free], de_apart[free], rtol=1e-12, atol=0.0), (x, y, k)
+            assert np.all(de[~free] <= floor * (1 + 1e-6))
+            dsm, dmax = pr["dsm"], pr["max_dist"]
+            for j in range(t.size - 1):
+                want = t[j] + max(de[j], t[j] * dsm)
+                if want > dmax:
+                    want = dmax
+                assert t[j + 1] == pytest.approx(want, rel=1e-12), (x, y, k, j)
+                geometric += de[j] < t[j] * dsm
+            run = np.minimum.accumulate(np.minimum(1.0, (de - pr["ms_ao"] + t * pr["md_d10"]) / t))
+            assert np.all(run[:-1] >= 0.02) and np.all(t[:-1] < dmax), (x, y, k)
+            assert run[-1] < 0.02 or t[-1] == dmax
+            assert max(run[-1], 0.0) == pytest.approx(opn, rel=1e-6, abs=1e-7)
+            assert t.size <= 2 + np.log(dmax / t0) / np.log1p(dsm)
+            checked += 1
+            long_walks += t.size >= 4
+    # both branches of the step rule and long walks were exercised
+    assert checked >= 30 and long_walks >= checked // 4 and geometric > 0, (checked, long_walks, geometric)
 
 
 def test_deao_generator_is_the_c_runtime_lcg():
