@@ -403,6 +403,52 @@ __device__ __forceinline__ void int_pow64(int power, double zmul, double v[4], c
         int_pow_low64(power, zmul, v, j);
 }
 
+// The f64 square root and reciprocal from the hardware's estimates
+// (v_rsq_f64, v_rcp_f64) refined by Newton steps (Goldschmidt's for the
+// root): within an ulp or two of the correctly rounded values, in about half
+// the instructions of the IEEE sequences, which scale, classify and fix up
+// every operand.  For the FP32 march's f64 finish: its decisions need f64
+// resolution, not the reference's rounding of every operation (that is the
+// FP64 mode's, which keeps the exact forms).
+__device__ __forceinline__ double sqrt_fast64(double x) {
+    if (!(x >= 0x1p-900 && x <= 0x1p900)) return sqrt(x);  // 0, tiny, huge, inf, NaN: the exact form
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    const double d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
+// 1 / d for a normal positive d below 2^900
+__device__ __forceinline__ double rcp_fast64(double d) {
+    double y = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-d, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+// int_pow8_64 with contractions and the refined hardware root and
+// reciprocal (the finish's form; the divisor rr^2 + 1e-40 lies in
+// [1e-40, 2^16] for an orbit that has not escaped)
+__device__ __forceinline__ void int_pow8_64_fast(double zmul, double v[4], const double j[4]) {
+#pragma clang fp contract(fast)
+    const double x = v[0], y = v[1], z = v[2];
+    const double xx = x * x, yy = y * y, zz = z * z;
+    const double r = xx + yy;
+    const double rr = r * r;
+    const double zzzz = zz * zz;
+    const double t = (zzzz - 6.0 * r * zz + rr) * (zz - r) * sqrt_fast64(r);
+    v[2] = -(t * z * 8.0 * zmul) + j[2];
+    const double num = (rr * 70.0 + zzzz) * zzzz - 28.0 * zz * r * (zzzz + rr);
+    const double a = num * rcp_fast64(rr * rr + 1e-40) + 1.0;
+    const double xxxx = xx * xx, yyyy = yy * yy;
+    v[1] = (yyyy * (7.0 * xx - yy) + xxxx * (xx - 7.0 * yy)) * 8.0 * x * y * a + j[1];
+    v[0] = a * (xxxx * xxxx + yyyy * (70.0 * xxxx + yyyy) - 28.0 * (xx * yy * (yyyy + xxxx))) + j[0];
+}
+
 // amazing_box (src/formulas.rs:713-736)
 __device__ __forceinline__ void box64(double scale, double min_r, double fold, double v[4], const double j[4],
                                       bool ade) {
@@ -417,6 +463,28 @@ __device__ __forceinline__ void box64(double scale, double min_r, double fold, d
         mul = scale;
     else
         mul = scale / r;
+    if (ade) v[3] *= mul;
+    v[2] = z * mul + j[2];
+    v[1] = y * mul + j[1];
+    v[0] = x * mul + j[0];
+}
+
+// box64 with contractions and the refined hardware reciprocal (the
+// finish's form; r lies in [min_r^2, 1] where it divides)
+__device__ __forceinline__ void box64_fast(double scale, double min_r, double fold, double v[4], const double j[4],
+                                           bool ade) {
+#pragma clang fp contract(fast)
+    const double x = bfold64(v[0], fold), y = bfold64(v[1], fold), z = bfold64(v[2], fold);
+    const double r = z * z + y * y + x * x;
+    const double m = fmax(min_r, 1e-40);
+    const double sqr_min_r = m * m;
+    double mul;
+    if (r < sqr_min_r)
+        mul = scale * rcp_fast64(sqr_min_r);
+    else if (1.0 < r)
+        mul = scale;
+    else
+        mul = scale * rcp_fast64(r);
     if (ade) v[3] *= mul;
     v[2] = z * mul + j[2];
     v[1] = y * mul + j[1];
@@ -542,7 +610,9 @@ __device__ __forceinline__ void aexion64(double pw, double z_mul, double power_c
 // Formula::iterate via call_slot (src/iteration.rs:325-345,
 // src/formulas.rs:576-641).  The power-8 and box kinds keep their two
 // formulas inline; K_GEN selects among all built-ins.
-template <int KIND>
+// FAST: the power-8 and box kinds' two formulas in their fast forms (the
+// finish of the FP32 march); K_GEN keeps the exact forms.
+template <int KIND, bool FAST = false>
 __device__ __forceinline__ void call_slot64(const DevP &P, int n, It64 &s) {
     double *v = s.v;
     // the options as values (a pointer into the kernel's parameter block
@@ -558,12 +628,21 @@ __device__ __forceinline__ void call_slot64(const DevP &P, int n, It64 &s) {
                 // this order: start64 and run64 keep it after every counted
                 // iteration, and these kinds count every slot)
                 const double r2 = s.rout;
-                s.deriv1 = 8.0 * (r2 * r2 * r2 * sqrt(r2)) * s.deriv1 + 1.0;
+                if (FAST)
+                    s.deriv1 = __builtin_fma(8.0 * (r2 * r2 * r2 * sqrt_fast64(r2)), s.deriv1, 1.0);
+                else
+                    s.deriv1 = 8.0 * (r2 * r2 * r2 * sqrt(r2)) * s.deriv1 + 1.0;
             }
-            int_pow8_64(o[0], v, s.j);
+            if (FAST)
+                int_pow8_64_fast(o[0], v, s.j);
+            else
+                int_pow8_64(o[0], v, s.j);
             s.p8++;
         } else {
-            box64(o[0], o[1], o[2], v, s.j, P.is_custom_de != 0);
+            if (FAST)
+                box64_fast(o[0], o[1], o[2], v, s.j, P.is_custom_de != 0);
+            else
+                box64(o[0], o[1], o[2], v, s.j, P.is_custom_de != 0);
             s.box++;
         }
         return;
@@ -677,7 +756,7 @@ __device__ __forceinline__ Part64 part1_of(const DevP &P) {
 
 // run_with (src/iteration.rs:268-315), counted slots, 3D or 4D squared
 // radius; returns the index of the last slot used
-template <int KIND>
+template <int KIND, bool FAST = false>
 __device__ __forceinline__ int run64(const DevP &P, const Part64 &pt, It64 &s, int max_it, double rstop) {
     if (KIND == K_GEN && P.ipol) return run_ipol64(P, s, max_it, rstop);
     int n = pt.start;
@@ -693,7 +772,7 @@ __device__ __forceinline__ int run64(const DevP &P, const Part64 &pt, It64 &s, i
             btmp = P.slot_iters[n];
             if (++guard > 64) return n;
         }
-        call_slot64<KIND>(P, n, s);
+        call_slot64<KIND, FAST>(P, n, s);
         btmp -= 1;
         if (KIND == K_GEN && P.slot_unc[n]) {  // a slot that does not count (src/iteration.rs:294-300)
             if (++uncounted_guard > 100000) return n;
@@ -702,6 +781,8 @@ __device__ __forceinline__ int run64(const DevP &P, const Part64 &pt, It64 &s, i
         s.it_result += 1;
         if (KIND == K_GEN && pt.four)
             s.rout = s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2] + s.v[3] * s.v[3];
+        else if (FAST && KIND != K_GEN)
+            s.rout = __builtin_fma(s.v[2], s.v[2], __builtin_fma(s.v[1], s.v[1], s.v[0] * s.v[0]));
         else
             s.rout = s.v[0] * s.v[0] + s.v[1] * s.v[1] + s.v[2] * s.v[2];
         if (s.rout < s.otrap) s.otrap = s.rout;
@@ -756,7 +837,7 @@ struct De64 {
 // DEM: the DE mode when the kernel knows it (DEM_P8 and DEM_BOX are
 // analytic: the numeric gradient's code is left out), DEM_NUMERIC when the
 // part's own flag decides.
-template <int KIND, int DEM = DEM_NUMERIC>
+template <int KIND, int DEM = DEM_NUMERIC, bool FAST = false>
 __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, double cx, double cy, double cz, int max_it,
                                        int calc_sit, float ms, double rstop = -1.0) {
     // (rstop: the bailout of the evaluation, P.d_rstop unless given;
@@ -777,7 +858,7 @@ __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, doubl
             s.v[3] = s.rout;
         else
             s.v[3] = 1.0;
-        const int n = run64<KIND>(P, pt, s, max_it, rs);
+        const int n = run64<KIND, FAST>(P, pt, s, max_it, rs);
         if ((pt.de_option & 0x38) == 32)
             result = sqrt(s.rout) * 0.25 * log(s.rout) / s.deriv1;
         else
@@ -788,7 +869,7 @@ __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, doubl
         o.rout = s.rout;
         o.otrap = s.otrap;
     } else {
-        const int n = run64<KIND>(P, pt, s, max_it, rs);
+        const int n = run64<KIND, FAST>(P, pt, s, max_it, rs);
         if (calc_sit) o.smooth_it = smooth64(P, s, n);
         o.it_result = s.it_result;
         o.rout = s.rout;
@@ -802,7 +883,7 @@ __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, doubl
             double g[3];
             for (int k = 0; k < 3; k++) {
                 start64<KIND>(P, pt.four, s, k == 0 ? cx + off : cx, k == 1 ? cy + off : cy, k == 2 ? cz + off : cz);
-                run64<KIND>(P, pt, s, it0, P.rstop3d);
+                run64<KIND, FAST>(P, pt, s, it0, P.rstop3d);
                 g[k] = (buf_rout - s.rout) * (buf_rout - s.rout);
             }
             result = buf_rout * log(buf_rout) * pt.dds / (sqrt(g[0] + g[1] + g[2]) + (double)P.de_offset006);
@@ -891,12 +972,14 @@ __device__ __forceinline__ De64 de64_comb(const DevP &P, double cx, double cy, d
     return o;
 }
 
-template <int KIND, int DEM = DEM_NUMERIC>
+// FAST: the power-8 and box kinds' iterates in their fast forms
+// (int_pow8_64_fast, box64_fast) for the FP32 march's finish and cone rays
+template <int KIND, int DEM = DEM_NUMERIC, bool FAST = false>
 __device__ __forceinline__ De64 de64(const DevP &P, double cx, double cy, double cz, int max_it, int calc_sit,
                                   float ms, double rstop = -1.0) {
     if (KIND == K_GEN && P.decomb) return de64_comb(P, cx, cy, cz, max_it, calc_sit, ms);
-    return de64_part<KIND, KIND == K_GEN ? DEM_NUMERIC : DEM>(P, part1_of<KIND>(P), cx, cy, cz, max_it, calc_sit, ms,
-                                                              rstop);
+    return de64_part<KIND, KIND == K_GEN ? DEM_NUMERIC : DEM, FAST && KIND != K_GEN>(P, part1_of<KIND>(P), cx, cy, cz,
+                                                                                     max_it, calc_sit, ms, rstop);
 }
 
 // mand_function (src/calc.rs:1277-1285, Iteration::hybrid_3d,

@@ -531,8 +531,12 @@ void launch_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int al
     const size_t lds = BATCH ? (size_t)n * DEVP_CORE : 0;
     if constexpr (KIND == K_GEN) {
         if (!BATCH) launch_gen_finish(grid, s, d, b, all, DEM);  // FP64 only, never batched
+    } else if (BATCH || !all) {
+        // the FP32 march's finish: the fast f64 forms
+        hipLaunchKernelGGL((k_finish64<KIND, DEM, BATCH, true>), dim3(grid), dim3(256), lds, s, d, b, all, tab, n);
     } else {
-        hipLaunchKernelGGL((k_finish64<KIND, DEM, BATCH>), dim3(grid), dim3(256), lds, s, d, b, all, tab, n);
+        // the FP64 mode: the reference's operation order
+        hipLaunchKernelGGL((k_finish64<KIND, DEM, false, false>), dim3(grid), dim3(256), lds, s, d, b, all, tab, n);
     }
 }
 // the surface modes beyond the plain stencil (k_normals_ext): smooth

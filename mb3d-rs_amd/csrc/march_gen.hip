@@ -19,8 +19,8 @@ static void with_dem(int dem, F &&f) {
 
 void launch_gen_finish(int grid, hipStream_t s, const DevP &d, const Bufs &b, int all, int dem) {
     with_dem(dem, [&](auto D) {
-        hipLaunchKernelGGL((k_finish64<K_GEN, decltype(D)::value, false>), dim3(grid), dim3(256), 0, s, d, b, all,
-                           nullptr, 1);
+        hipLaunchKernelGGL((k_finish64<K_GEN, decltype(D)::value, false, false>), dim3(grid), dim3(256), 0, s, d, b,
+                           all, nullptr, 1);
     });
 }
 

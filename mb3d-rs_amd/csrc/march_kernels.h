@@ -575,7 +575,7 @@ __global__ __launch_bounds__(256) void k_cone(DevP P0, const DevP *__restrict__ 
         // one step of the cone ray
         bool done = ++guard > 2000000u;
         const float ms = (float)((double)P.de_stop * (1.0 + t * (double)P.de_stop_factor));
-        const De64 r = de64<KIND, DEM>(P, c[0], c[1], c[2], P.max_it, 0, ms);
+        const De64 r = de64<KIND, DEM, true>(P, c[0], c[1], c[2], P.max_it, 0, ms);
         cnt.de64++;
         cnt.p8_64 += r.p8;
         cnt.box_64 += r.box;
@@ -960,7 +960,7 @@ enum FinPhase : int { FP_START, FP_RESUME, FP_BACK, FP_STEP, FP_REF };
 // surface list, and k_normals computes the normal, colour and G-buffer
 // words.  So that every phase here makes the same f64 DE: one DE site per
 // turn.
-template <int KIND, int DEM, bool BATCH>
+template <int KIND, int DEM, bool BATCH, bool FAST>
 __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixels, const DevP *__restrict__ gtab,
                                                   int nframes) {
     const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
@@ -1047,7 +1047,7 @@ __global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixel
         const DevP &P = BATCH ? tab[fr] : P0;
         bool surf = false, done = false;
         if (!have) goto end;
-        R.take(de64<KIND, DEM>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
+        R.take(de64<KIND, DEM, FAST>(P, R.c[0], R.c[1], R.c[2], R.max_it, R.calc_sit, R.ms), cnt);
         switch (phase) {
             case FP_START:
                 if (R.it >= R.max_its || R.de < (double)R.ms) {
