@@ -310,7 +310,10 @@ def main():
     lo = legs[head]
     t_k, cf = kernel_times(lo)
     kernels = [
-        {"kernel": "k_march32 (batch)", "bound": "valu-fp32", "kernel_ms": t_k["march"],
+        # (with the cone rays the FP32 march is two kernels: k_cone_bg, the
+        # background tiles' start DEs, then k_march32 on the other tiles)
+        {"kernel": "k_cone_bg + k_march32 (batch)" if lo.mlp_cone else "k_march32 (batch)", "bound": "valu-fp32",
+         "kernel_ms": t_k["march"],
          "flops_per_launch": flops_march(cf), "peak": PEAK_FP32 / 1e12},
         {"kernel": "k_finish64 (batch)", "bound": "valu-fp64", "kernel_ms": t_k["finish"],
          "flops_per_launch": flops_finish(cf), "peak": PEAK_FP64 / 1e12},

@@ -118,18 +118,27 @@ struct Cnt {
 
 // ---------------------------------------------------------------------------
 // FP32 formula iterations
+//
+// The FMAs are written out (fmaf) rather than left to contraction: which
+// products the compiler would fuse depends on the code around an inlined
+// call, so two kernels (a batch's and a single frame's) could round the same
+// DE differently; written out, every kernel computes the same bits.
 
 __device__ __forceinline__ void p8f(float &x, float &y, float &z, float jx, float jy, float jz, float zmul8) {
-#pragma clang fp contract(fast)
     const float xx = x * x, yy = y * y, zz = z * z;
-    const float r = xx + yy, rr = r * r, zzzz = zz * zz;
-    const float t = (zzzz - 6.0f * r * zz + rr) * (zz - r) * __builtin_amdgcn_sqrtf(r);
-    const float nz = jz - t * z * zmul8;
-    const float num = (rr * 70.0f + zzzz) * zzzz - 28.0f * zz * r * (zzzz + rr);
-    const float a = num * __builtin_amdgcn_rcpf(rr * rr + 1e-37f) + 1.0f;
+    const float r = fmaf(x, x, yy), rr = r * r, zzzz = zz * zz;
+    // (zzzz - 6 r zz + rr) (zz - r) sqrt(r)
+    const float t = fmaf(-6.0f * r, zz, zzzz + rr) * (zz - r) * __builtin_amdgcn_sqrtf(r);
+    const float nz = fmaf(-(t * z), zmul8, jz);
+    // (70 rr + zzzz) zzzz - 28 zz r (zzzz + rr)
+    const float num = fmaf(fmaf(rr, 70.0f, zzzz), zzzz, -28.0f * (zz * r) * (zzzz + rr));
+    const float a = fmaf(num, __builtin_amdgcn_rcpf(fmaf(rr, rr, 1e-37f)), 1.0f);
     const float xxxx = xx * xx, yyyy = yy * yy;
-    const float ny = (yyyy * (7.0f * xx - yy) + xxxx * (xx - 7.0f * yy)) * 8.0f * x * y * a + jy;
-    const float nx = a * (xxxx * xxxx + yyyy * (70.0f * xxxx + yyyy) - 28.0f * (xx * yy * (yyyy + xxxx))) + jx;
+    // (yyyy (7 xx - yy) + xxxx (xx - 7 yy)) 8 x y a + jy
+    const float ny = fmaf(fmaf(yyyy, fmaf(7.0f, xx, -yy), xxxx * fmaf(-7.0f, yy, xx)) * (8.0f * x * y), a, jy);
+    // a (xxxx^2 + yyyy (70 xxxx + yyyy) - 28 xx yy (yyyy + xxxx)) + jx
+    const float nx =
+        fmaf(a, fmaf(xxxx, xxxx, fmaf(yyyy, fmaf(70.0f, xxxx, yyyy), -28.0f * (xx * yy) * (yyyy + xxxx))), jx);
     x = nx;
     y = ny;
     z = nz;
@@ -137,16 +146,18 @@ __device__ __forceinline__ void p8f(float &x, float &y, float &z, float jx, floa
 
 __device__ __forceinline__ float bfoldf(float x, float f) { return fabsf(x + f) - (fabsf(x - f) + x); }
 
+// x^2 + y^2 + z^2 with two FMAs
+__device__ __forceinline__ float sq3f(float x, float y, float z) { return fmaf(z, z, fmaf(y, y, x * x)); }
+
 // amazing_box in FP32; returns the derivative factor
 __device__ __forceinline__ float boxf(float &x, float &y, float &z, float jx, float jy, float jz, float fold,
                                      float scale, float min_r2, float scale_min_r2) {
-#pragma clang fp contract(fast)
     const float a = bfoldf(x, fold), b = bfoldf(y, fold), c = bfoldf(z, fold);
-    const float r = c * c + b * b + a * a;
+    const float r = sq3f(c, b, a);
     const float mul = r < min_r2 ? scale_min_r2 : (1.0f < r ? scale : scale * __builtin_amdgcn_rcpf(r));
-    x = a * mul + jx;
-    y = b * mul + jy;
-    z = c * mul + jz;
+    x = fmaf(a, mul, jx);
+    y = fmaf(b, mul, jy);
+    z = fmaf(c, mul, jz);
     return mul;
 }
 
@@ -175,7 +186,6 @@ __device__ __forceinline__ float ln_fast(float x) { return __builtin_amdgcn_logf
 // the point and its three offset points iterated in one loop.
 template <int KIND>
 __device__ float de32_numeric(const DevP &P, const double c[3], float ms, int &it_out, Cnt &cnt) {
-#pragma clang fp contract(fast)
     const double off = (double)P.de_offset;
     float x[4], y[4], z[4], jx[4], jy[4], jz[4];
     jx[0] = (float)c[0];
@@ -221,10 +231,10 @@ __device__ float de32_numeric(const DevP &P, const double c[3], float ms, int &i
             cnt.box_32 += 1u + a1 + a2 + a3;
         }
         it++;
-        r0 = x[0] * x[0] + y[0] * y[0] + z[0] * z[0];
-        const float q1 = x[1] * x[1] + y[1] * y[1] + z[1] * z[1];
-        const float q2 = x[2] * x[2] + y[2] * y[2] + z[2] * z[2];
-        const float q3 = x[3] * x[3] + y[3] * y[3] + z[3] * z[3];
+        r0 = sq3f(x[0], y[0], z[0]);
+        const float q1 = sq3f(x[1], y[1], z[1]);
+        const float q2 = sq3f(x[2], y[2], z[2]);
+        const float q3 = sq3f(x[3], y[3], z[3]);
         if (a1) {
             r1 = q1;
             a1 = !(q1 > P.f_rstop3d);
@@ -245,7 +255,7 @@ __device__ float de32_numeric(const DevP &P, const double c[3], float ms, int &i
     if (r0 < 1e-37f) {
         res = 0.0f;
     } else {
-        const float g = (r0 - r1) * (r0 - r1) + (r0 - r2) * (r0 - r2) + (r0 - r3) * (r0 - r3);
+        const float g = sq3f(r0 - r1, r0 - r2, r0 - r3);
         res = r0 * ln_fast(r0) * P.d_de_scale * __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(g) + P.de_offset006);
     }
     const float mn = ms * 0.25f;
@@ -258,10 +268,9 @@ __device__ float de32_numeric(const DevP &P, const double c[3], float ms, int &i
 template <int DEM>
 __device__ float de32_analytic(const DevP &P, const double c[3], float ms, int &it_out, Cnt &cnt,
                                float *otrap = nullptr) {
-#pragma clang fp contract(fast)
     const float jx = (float)c[0], jy = (float)c[1], jz = (float)c[2];
     float x = jx, y = jy, z = jz;
-    float r = x * x + y * y + z * z;
+    float r = sq3f(x, y, z);
     float ot = r;
     float w = 1.0f;
     int it = 0;
@@ -270,7 +279,7 @@ __device__ float de32_analytic(const DevP &P, const double c[3], float ms, int &
         for (;;) {
             w *= boxf(x, y, z, jx, jy, jz, fo, sc, m2, sm);
             it++;
-            r = x * x + y * y + z * z;
+            r = sq3f(x, y, z);
             if (otrap) ot = fminf(ot, r);
             if (it >= P.max_it || r > P.f_d_rstop) break;
         }
@@ -281,10 +290,10 @@ __device__ float de32_analytic(const DevP &P, const double c[3], float ms, int &
             // (r: the squared radius of (x, y, z), from the line before the
             // loop or the end of the last iteration)
             const float r2 = r;
-            w = 8.0f * (r2 * r2 * r2 * __builtin_amdgcn_sqrtf(r2)) * w + 1.0f;
+            w = fmaf(8.0f * (r2 * r2 * r2 * __builtin_amdgcn_sqrtf(r2)), w, 1.0f);
             p8f(x, y, z, jx, jy, jz, zm8);
             it++;
-            r = x * x + y * y + z * z;
+            r = sq3f(x, y, z);
             if (otrap) ot = fminf(ot, r);
             if (it >= P.max_it || r > P.f_d_rstop) break;
         }
@@ -430,23 +439,26 @@ __device__ __forceinline__ double rcp_fast64(double d) {
     return __builtin_fma(y, e, y);
 }
 
-// int_pow8_64 with contractions and the refined hardware root and
-// reciprocal (the finish's form; the divisor rr^2 + 1e-40 lies in
+// int_pow8_64 with FMAs (written out, as p8f's) and the refined hardware
+// root and reciprocal (the finish's form; the divisor rr^2 + 1e-40 lies in
 // [1e-40, 2^16] for an orbit that has not escaped)
 __device__ __forceinline__ void int_pow8_64_fast(double zmul, double v[4], const double j[4]) {
-#pragma clang fp contract(fast)
     const double x = v[0], y = v[1], z = v[2];
     const double xx = x * x, yy = y * y, zz = z * z;
-    const double r = xx + yy;
+    const double r = __builtin_fma(x, x, yy);
     const double rr = r * r;
     const double zzzz = zz * zz;
-    const double t = (zzzz - 6.0 * r * zz + rr) * (zz - r) * sqrt_fast64(r);
-    v[2] = -(t * z * 8.0 * zmul) + j[2];
-    const double num = (rr * 70.0 + zzzz) * zzzz - 28.0 * zz * r * (zzzz + rr);
-    const double a = num * rcp_fast64(rr * rr + 1e-40) + 1.0;
+    const double t = __builtin_fma(-6.0 * r, zz, zzzz + rr) * (zz - r) * sqrt_fast64(r);
+    v[2] = __builtin_fma(-(t * z), 8.0 * zmul, j[2]);
+    const double num = __builtin_fma(__builtin_fma(rr, 70.0, zzzz), zzzz, -28.0 * (zz * r) * (zzzz + rr));
+    const double a = __builtin_fma(num, rcp_fast64(__builtin_fma(rr, rr, 1e-40)), 1.0);
     const double xxxx = xx * xx, yyyy = yy * yy;
-    v[1] = (yyyy * (7.0 * xx - yy) + xxxx * (xx - 7.0 * yy)) * 8.0 * x * y * a + j[1];
-    v[0] = a * (xxxx * xxxx + yyyy * (70.0 * xxxx + yyyy) - 28.0 * (xx * yy * (yyyy + xxxx))) + j[0];
+    v[1] = __builtin_fma(
+        __builtin_fma(yyyy, __builtin_fma(7.0, xx, -yy), xxxx * __builtin_fma(-7.0, yy, xx)) * (8.0 * x * y), a, j[1]);
+    v[0] = __builtin_fma(
+        a,
+        __builtin_fma(xxxx, xxxx, __builtin_fma(yyyy, __builtin_fma(70.0, xxxx, yyyy), -28.0 * (xx * yy) * (yyyy + xxxx))),
+        j[0]);
 }
 
 // amazing_box (src/formulas.rs:713-736)
@@ -469,13 +481,12 @@ __device__ __forceinline__ void box64(double scale, double min_r, double fold, d
     v[0] = x * mul + j[0];
 }
 
-// box64 with contractions and the refined hardware reciprocal (the
-// finish's form; r lies in [min_r^2, 1] where it divides)
+// box64 with FMAs and the refined hardware reciprocal (the finish's form;
+// r lies in [min_r^2, 1] where it divides)
 __device__ __forceinline__ void box64_fast(double scale, double min_r, double fold, double v[4], const double j[4],
                                            bool ade) {
-#pragma clang fp contract(fast)
     const double x = bfold64(v[0], fold), y = bfold64(v[1], fold), z = bfold64(v[2], fold);
-    const double r = z * z + y * y + x * x;
+    const double r = __builtin_fma(x, x, __builtin_fma(y, y, z * z));
     const double m = fmax(min_r, 1e-40);
     const double sqr_min_r = m * m;
     double mul;
@@ -486,9 +497,9 @@ __device__ __forceinline__ void box64_fast(double scale, double min_r, double fo
     else
         mul = scale * rcp_fast64(r);
     if (ade) v[3] *= mul;
-    v[2] = z * mul + j[2];
-    v[1] = y * mul + j[1];
-    v[0] = x * mul + j[0];
+    v[2] = __builtin_fma(z, mul, j[2]);
+    v[1] = __builtin_fma(y, mul, j[1]);
+    v[0] = __builtin_fma(x, mul, j[0]);
 }
 
 // int_pow2_scale (src/formulas.rs:739-752), the bulb of Bulbox

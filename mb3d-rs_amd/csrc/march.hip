@@ -514,15 +514,26 @@ void launch_ray_tab(hipStream_t s, const DevP &d, const DevP *tab, int n, double
     hipLaunchKernelGGL(k_ray_tab, dim3(8, tab ? n : 1), dim3(256), 0, s, tab, d, rtab);
 }
 
+// the FP32 march; with the cone rays k_cone_bg first: the background tiles'
+// pixels, and the list of tiles the march takes
 template <int KIND, int DEM, bool BATCH = false>
 void launch_march(int grid, hipStream_t s, const DevP &d, const Bufs &b, const MlpDev &M, const DevP *tab = nullptr,
                   int n = 1) {
     const size_t lds = BATCH ? (size_t)n * DEVP_CORE : 0;
-    if constexpr (KIND == K_GEN) return;  // FP64 only: the hosts never ask for it
-    else if (jumps_on(M))
-        hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, true>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
-    else
-        hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, false>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
+    if constexpr (KIND == K_GEN) {
+        return;  // FP64 only: the hosts never ask for it
+    } else {
+        if (d.cone_mode) {
+            const long long groups = ((d.nslots_q >> d.tn_shift) + 63) / 64;  // 64 tiles a wave
+            const int gbg =
+                (int)std::max<long long>(1, std::min<long long>((groups + 3) / 4, (long long)g_ctx.cus * 8));
+            hipLaunchKernelGGL((k_cone_bg<KIND, DEM, BATCH>), dim3(gbg), dim3(256), lds, s, d, b, tab, n);
+        }
+        if (jumps_on(M))
+            hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, true>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
+        else
+            hipLaunchKernelGGL((k_march32<KIND, DEM, BATCH, false>), dim3(grid), dim3(256), lds, s, d, b, tab, n, M);
+    }
 }
 // (the general kind's kernels are instantiated in march_gen.hip)
 template <int KIND, int DEM, bool BATCH = false>
@@ -790,6 +801,9 @@ int march_locked(const mb3d_hip_params *params, const mb3d_hip_post_job *post, i
     const size_t learn_need = M.enabled ? learn_bufs(nullptr, d.nslots_q, nullptr) : 0;
     const size_t rtab_bytes = ((size_t)ray_tab_entries(d) * sizeof(double2) + 255) & ~(size_t)255;
     const bool cone = d.cone_mode != 0 && !f64 && !post_only;
+    // (the march's tile list packs a tile index in LIVE_TILE_BITS bits)
+    if (cone && (d.nslots_q >> d.tn_shift) >= (1ll << LIVE_TILE_BITS))
+        return set_error(MB3D_HIP_ERR_INVALID, "with the cone rays a picture holds at most 2^26 tiles");
     const size_t cone_bytes = cone ? (size_t)(d.nslots_q >> d.tn_shift) * sizeof(ConeRes) : 0;
     const bool ao_rnd = want_ao && post->deao_first_step_random;
     const size_t ao_sets_bytes = want_ao ? (9 * sizeof(AoSet) + 255) & ~(size_t)255 : 0;
@@ -1134,6 +1148,10 @@ int plan_build(const mb3d_hip_params *frames, int n, const mb3d_hip_out *out, co
     if (pix >= (1ll << 31)) {
         plan_free(pl);
         return set_error(MB3D_HIP_ERR_INVALID, "a batch holds at most 2^31 pixels");
+    }
+    if (pl->tab[0].cone_mode != 0 && (q >> pl->tab[0].tn_shift) >= (1ll << LIVE_TILE_BITS)) {
+        plan_free(pl);
+        return set_error(MB3D_HIP_ERR_INVALID, "with the cone rays a batch holds at most 2^26 tiles");
     }
     pl->npix = pix;
     mlp_dev(o.mlp, o, pl->M);

@@ -73,7 +73,8 @@ enum : int {
     C_AOQ = 5 * C_LINE,      // next item of the DE AO pass
     C_SURF = 6 * C_LINE,     // refined surface list length (the normals pass's items)
     C_CONEQ = 7 * C_LINE,    // next tile of the cone rays
-    C_STATS = 8 * C_LINE,    // 19 work counters from here (the 16th: the network's rays)
+    C_LIVE = 8 * C_LINE,     // tiles the cone rays leave to the march (k_cone_bg's list)
+    C_STATS = 9 * C_LINE,    // 19 work counters from here (the 16th: the network's rays)
     C_WORDS = C_STATS + 19
 };
 
@@ -99,12 +100,16 @@ struct Bufs {
 
 // A tile's cone ray (k_cone): the depth up to which every ray of the tile
 // is free of the surface (+inf: the cone passed the end of the view) and the
-// dynamic fog's step count up to there.
+// dynamic fog's step count up to there.  `live` is not the tile's: record s
+// holds entry s of the list of tiles the march takes (k_cone_bg writes it
+// after reading the records' depths and counts; the tile's index in the
+// queue | its frame << 26).
 struct ConeRes {
     double t;
     float count;
-    int32_t pad;
+    int32_t live;
 };
+constexpr int LIVE_TILE_BITS = 26;
 
 // Fills the ray tables of the frames `tab[0..n)` (one frame: `tab` = &P):
 // block row blockIdx.y takes frame blockIdx.y's width + rows entries.
@@ -615,6 +620,96 @@ __global__ __launch_bounds__(256) void k_cone(DevP P0, const DevP *__restrict__ 
     flush_counters(cnt, ctr);
 }
 
+// The tiles whose cone ray passed the end of the view, each by one whole
+// wave: every pixel's start DE (march_pixel_from's check for a start in the
+// set, src/calc.rs:2263-2290, as k_march32 makes it), then the background
+// with the cone's fog count (src/calc.rs:2299-2305), written by the wave as
+// the tile's rows, or the pixel handed to the f64 finish when its start is
+// in the set.  Every other tile is listed for the march (ConeRes::live,
+// C_LIVE), so the march's persistent waves refill their lanes only with rays
+// that march.  A wave takes 64 tiles at a time, lane l deciding tile g0 + l.
+template <int KIND, int DEM, bool BATCH>
+__global__ __launch_bounds__(256) void k_cone_bg(DevP P0, Bufs B, const DevP *__restrict__ gtab, int nframes) {
+    const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
+    Cnt cnt;
+    const long long ntiles = P0.nslots_q >> P0.tn_shift;
+    const int tsize = 1 << P0.tn_shift;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    const float fguard = P0.guard;
+    for (long long g0 = wave * 64; g0 < ntiles; g0 += nwaves * 64) {
+        const long long gi = g0 + lane_id();
+        bool bg = false, live = false;
+        int fr = 0;
+        float count = 0.0f;
+        if (gi < ntiles) {
+            const ConeRes cr = B.cone[gi];
+            fr = BATCH ? find_frame(tab, nframes, gi << P0.tn_shift) : 0;
+            const DevP &F = BATCH ? tab[fr] : P0;
+            bg = cr.t > F.zend;
+            live = !bg;
+            count = cr.count;
+        }
+        const long long s = wave_append(live, &B.ctr[C_LIVE]);
+        if (live) B.cone[s].live = (int32_t)((uint32_t)gi | ((uint32_t)fr << LIVE_TILE_BITS));
+        // the background tiles, one at a time with the whole wave
+        unsigned long long m = __ballot(bg);
+        while (m) {
+            const int l = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const long long tq = (g0 + l) << P0.tn_shift;  // the tile's first item
+            const int tfr = __shfl(fr, l, 64);
+            const float tcount = __shfl(count, l, 64);
+            const DevP &F = BATCH ? tab[tfr] : P0;
+            for (int k0 = 0; k0 < tsize; k0 += 64) {
+                const int k = k0 + lane_id();
+                bool handoff = false, first = false;
+                int x = 0, yr = 0, pix = 0;
+                int32_t seed = 0;
+                double c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+                if (k < tsize && item_pixel(F, (int)(tq + k - F.q_begin), x, yr)) {
+                    pix = (int)(F.pix_base + (long long)yr * F.width + x);
+                    pixel_ray_tab(F, B.rtab, x, yr, c, v);
+                    first = F.first_step_random != 0;
+                    seed = first ? dev_pixel_seed(F.row0 + yr, x) : 0;
+                    int it = 0;
+                    const float ms = F.de_stop;
+                    const float d32 = de32<KIND, DEM>(F, c, ms, it, cnt);
+                    if (!(d32 < __builtin_inff())) {  // single precision overflow: f64 decides
+                        cnt.overflow++;
+                        handoff = true;
+                    } else if (it >= F.max_it || !(d32 >= ms * fguard)) {
+                        handoff = true;
+                    } else {
+                        cnt.cone_bg++;
+                        write_background(F, B, pix, tcount);
+                    }
+                }
+                // (an exact append: these hand-offs are rare, and the list's
+                // capacity allows empty chunk tails for the march's waves only)
+                const long long slot = wave_append(handoff, &B.ctr[C_CAND]);
+                if (handoff) {
+                    Cand &o = B.cand[slot];
+                    for (int i = 0; i < 3; i++) {
+                        o.c[i] = c[i];
+                        o.v[i] = v[i];
+                    }
+                    o.mzz = 0.0;
+                    o.last_step = 0.0;
+                    o.last_de = 0.0;
+                    o.rsf = 1.0f;
+                    o.step_count = 0.0f;
+                    o.pix = pix;
+                    o.flags = CF_START | (first ? CF_FIRST : 0) | (tfr << 8);
+                    o.nsteps = 0;
+                    o.seed = seed;
+                }
+            }
+        }
+    }
+    flush_counters(cnt, B.ctr);
+}
+
 // BATCH: the frames of a batch in one launch; `tab` holds each frame's
 // parameters, P0 the batch's queue totals and the shared tile shape.
 // LEARN: the warm start (B.prop holds k_mlp_propose's proposals).  A ray
@@ -648,8 +743,12 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
     chunk_init(wq);
     chunk_init(cq);
     FrameHint hint = {0, 0, 0};
+    // the queue: every tile's items, or with the cone rays the items of the
+    // tiles k_cone_bg listed (the others are background or handed off)
+    const bool listed = P0.cone_mode != 0;
+    const long long n_items = listed ? (long long)B.ctr[C_LIVE] << P0.tn_shift : P0.nslots_q;
     auto on_chunk = [&](long long base, int) {
-        if (BATCH) hint = frame_hint(tab, nframes, base);
+        if (BATCH && !listed) hint = frame_hint(tab, nframes, base);
     };
     for (;;) {
         // refill: the idle lanes take the next work items from the wave's
@@ -659,15 +758,23 @@ __global__ __launch_bounds__(256) void k_march32(DevP P0, Bufs B, const DevP *__
         const int idle = __popcll(__ballot(need));
         const bool refill = idle > 0 && (idle >= P0.refill_min || idle == __popcll(__ballot(!exhausted)));
         for (int tries = 0; refill && tries < 4 && __any(need); tries++) {
-            const long long q = wave_take(need, &B.ctr[C_QUEUE], wq, 256, P0.nslots_q, on_chunk);
+            const long long qi = wave_take(need, &B.ctr[C_QUEUE], wq, 256, n_items, on_chunk);
             if (need) {
                 int x = 0, yr = 0;
-                if (q < 0) {
+                if (qi < 0) {
                     exhausted = true;
                     need = false;
                     continue;
                 }
-                fr = BATCH ? frame_of(tab, nframes, hint, q) : 0;
+                long long q = qi;  // the item of the tile queue
+                if (listed) {
+                    const uint32_t e = (uint32_t)B.cone[qi >> P0.tn_shift].live;
+                    q = ((long long)(e & ((1u << LIVE_TILE_BITS) - 1u)) << P0.tn_shift) |
+                        (qi & ((1 << P0.tn_shift) - 1));
+                    fr = BATCH ? (int)(e >> LIVE_TILE_BITS) : 0;
+                } else {
+                    fr = BATCH ? frame_of(tab, nframes, hint, q) : 0;
+                }
                 const DevP &F = BATCH ? tab[fr] : P0;
                 if (item_pixel(F, (int)(q - F.q_begin), x, yr)) {
                     have = true;
