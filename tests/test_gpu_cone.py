@@ -66,21 +66,48 @@ def _opts_mode2():
     return o
 
 
-def test_batch_with_cones_equals_single_frames(hip):
+@pytest.mark.parametrize("tile,mode", [((8, 8), 1), ((16, 16), 1), ((8, 8), 2)])
+def test_batch_with_cones_equals_single_frames(hip, tile, mode):
+    """A batch of uneven frames with the cone rays (k_cone, k_cone_bg's
+    background tiles and tile list, the listed march) equals the frames
+    marched one by one: 16 x 16 tiles take four wave passes a background
+    tile, and mode 2's jumps start from the listed tiles' cone depths."""
     scenes_ = H.animation(5, seed=7, sizes=((96, 72), (160, 120), (200, 150)))
     arr = H.params_array(scenes_)
     n = sum(s.width * s.height for s in scenes_)
     bufs, out = device_out(n)
-    H.check(hip.load().mb3d_hip_march_batch(arr, len(arr), H.C.byref(out), H.C.byref(H.options(cone=True))), "batch")
+
+    def opts():
+        o = H.options(cone=True, tile=tile)
+        o.mlp_cone = mode
+        return o
+
+    H.check(hip.load().mb3d_hip_march_batch(arr, len(arr), H.C.byref(out), H.C.byref(opts())), "batch")
     cb = H.counters()
     got = host(bufs)
     off = 0
     bg = 0
     for i, s in enumerate(scenes_):
         m = s.width * s.height
-        one = hip.march(arr[i], opts=H.options(cone=True))
+        one = hip.march(arr[i], opts=opts())
         bg += H.counters()["cone_background"]
         assert np.array_equal(got["depth"][off:off + m], one["depth"]), i
         assert np.array_equal(got["gbuf"][off * 5:(off + m) * 5].view(np.uint32), one["gbuf"].ravel()), i
         off += m
     assert cb["cone_background"] == bg > 0
+
+
+@pytest.mark.parametrize("tile", [(8, 8), (16, 16)])
+def test_cone_background_tiles_hand_off_starts_in_the_guard(hip, tile):
+    """With a guard band so wide that every start's DE lies inside it, each
+    pixel of a background tile goes from k_cone_bg to the f64 finish at its
+    start (as k_march32 hands off a start in the guard), and the picture --
+    every output word -- is the plain march's with the same guard."""
+    sc = H.scene_bulb(240, 136, seed=4)  # (136 rows: ragged 16 x 16 tiles)
+    p = H.params_from_scene(sc)
+    plain = hip.march(p, opts=H.options(tile=tile, guard_band=1e4))
+    cone = hip.march(p, opts=H.options(tile=tile, guard_band=1e4, cone=True))
+    c = H.counters()
+    assert c["cone_rays"] > 0 and c["cone_background"] == 0, c
+    for k in ("depth", "depth64", "iters", "normals", "gbuf"):
+        assert np.array_equal(cone[k], plain[k]), k
