@@ -1253,6 +1253,12 @@ __device__ __forceinline__ void pixel_ray_tab(const DevP &P, const double2 *__re
     for (int k = 0; k < 3; k++) c[k] = P.ystart[k] + P.vg[0][k] * fx + P.vg[1][k] * fy;
 }
 
+// pixel_ray_tab's start point alone (the same operations: the same bits)
+__device__ __forceinline__ void pixel_start(const DevP &P, int x, int yr, double c[3]) {
+    const double fx = x, fy = P.row0 + yr;
+    for (int k = 0; k < 3; k++) c[k] = P.ystart[k] + P.vg[0][k] * fx + P.vg[1][k] * fy;
+}
+
 // The ray of the picture's point (fx, fy), fractions allowed (Marcher::ray,
 // src/calc.rs:2328-2355), Common optic: pixel_ray at non-integer pixels.
 __device__ __forceinline__ void frac_ray(const DevP &P, double fx, double fy, double c[3], double v[3]) {

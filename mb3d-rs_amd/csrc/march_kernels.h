@@ -669,9 +669,8 @@ __global__ __launch_bounds__(256) void k_cone_bg(DevP P0, Bufs B, const DevP *__
                 double c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
                 if (k < tsize && item_pixel(F, (int)(tq + k - F.q_begin), x, yr)) {
                     pix = (int)(F.pix_base + (long long)yr * F.width + x);
-                    pixel_ray_tab(F, B.rtab, x, yr, c, v);
-                    first = F.first_step_random != 0;
-                    seed = first ? dev_pixel_seed(F.row0 + yr, x) : 0;
+                    // the start point (the direction only for a hand-off)
+                    pixel_start(F, x, yr, c);
                     int it = 0;
                     const float ms = F.de_stop;
                     const float d32 = de32<KIND, DEM>(F, c, ms, it, cnt);
@@ -689,6 +688,9 @@ __global__ __launch_bounds__(256) void k_cone_bg(DevP P0, Bufs B, const DevP *__
                 // capacity allows empty chunk tails for the march's waves only)
                 const long long slot = wave_append(handoff, &B.ctr[C_CAND]);
                 if (handoff) {
+                    pixel_ray_tab(F, B.rtab, x, yr, c, v);
+                    first = F.first_step_random != 0;
+                    seed = first ? dev_pixel_seed(F.row0 + yr, x) : 0;
                     Cand &o = B.cand[slot];
                     for (int i = 0; i < 3; i++) {
                         o.c[i] = c[i];
