@@ -623,9 +623,10 @@ __global__ __launch_bounds__(256) void k_cone(DevP P0, const DevP *__restrict__ 
 // The tiles whose cone ray passed the end of the view, each by one whole
 // wave: every pixel's start DE (march_pixel_from's check for a start in the
 // set, src/calc.rs:2263-2290, as k_march32 makes it), then the background
-// with the cone's fog count (src/calc.rs:2299-2305), written by the wave as
-// the tile's rows, or the pixel handed to the f64 finish when its start is
-// in the set.  Every other tile is listed for the march (ConeRes::live,
+// with the cone's fog count (src/calc.rs:2299-2305), written by the wave in
+// whole rows (of four 8 x 8 tiles side by side where they are background
+// together), or the pixel handed to the f64 finish when its start is in the
+// set.  Every other tile is listed for the march (ConeRes::live,
 // C_LIVE), so the march's persistent waves refill their lanes only with rays
 // that march.  A wave takes 64 tiles at a time, lane l deciding tile g0 + l.
 template <int KIND, int DEM, bool BATCH>
@@ -652,61 +653,90 @@ __global__ __launch_bounds__(256) void k_cone_bg(DevP P0, Bufs B, const DevP *__
         }
         const long long s = wave_append(live, &B.ctr[C_LIVE]);
         if (live) B.cone[s].live = (int32_t)((uint32_t)gi | ((uint32_t)fr << LIVE_TILE_BITS));
-        // the background tiles, one at a time with the whole wave
-        unsigned long long m = __ballot(bg);
+        // one pixel of a background tile per lane (item q of frame tfr; `in`:
+        // the lane has one): the start DE, then the background or a hand-off.
+        // Called with the whole wave.
+        auto bg_pixel = [&](int tfr, long long q, bool in, float tcount) {
+            const DevP &F = BATCH ? tab[tfr] : P0;
+            bool handoff = false, first = false;
+            int x = 0, yr = 0, pix = 0;
+            int32_t seed = 0;
+            double c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+            if (in && item_pixel(F, (int)(q - F.q_begin), x, yr)) {
+                pix = (int)(F.pix_base + (long long)yr * F.width + x);
+                // the start point (the direction only for a hand-off)
+                pixel_start(F, x, yr, c);
+                int it = 0;
+                const float ms = F.de_stop;
+                const float d32 = de32<KIND, DEM>(F, c, ms, it, cnt);
+                if (!(d32 < __builtin_inff())) {  // single precision overflow: f64 decides
+                    cnt.overflow++;
+                    handoff = true;
+                } else if (it >= F.max_it || !(d32 >= ms * fguard)) {
+                    handoff = true;
+                } else {
+                    cnt.cone_bg++;
+                    write_background(F, B, pix, tcount);
+                }
+            }
+            // (an exact append: these hand-offs are rare, and the list's
+            // capacity allows empty chunk tails for the march's waves only)
+            const long long slot = wave_append(handoff, &B.ctr[C_CAND]);
+            if (handoff) {
+                pixel_ray_tab(F, B.rtab, x, yr, c, v);
+                first = F.first_step_random != 0;
+                seed = first ? dev_pixel_seed(F.row0 + yr, x) : 0;
+                Cand &o = B.cand[slot];
+                for (int i = 0; i < 3; i++) {
+                    o.c[i] = c[i];
+                    o.v[i] = v[i];
+                }
+                o.mzz = 0.0;
+                o.last_step = 0.0;
+                o.last_de = 0.0;
+                o.rsf = 1.0f;
+                o.step_count = 0.0f;
+                o.pix = pix;
+                o.flags = CF_START | (first ? CF_FIRST : 0) | (tfr << 8);
+                o.nsteps = 0;
+                o.seed = seed;
+            }
+        };
+        const unsigned long long bgm = __ballot(bg);
+        // Quads of four background 8 x 8 tiles side by side, the first at a
+        // multiple of 4 in its row (rows of a multiple of 4 tiles hold whole
+        // quads, frames whole rows): the wave takes them as rows of 32
+        // pixels, so that its depth and normal writes fill whole 128-byte
+        // lines.  Lane l of pass p: row 2p + l / 32, column l % 32.
+        unsigned long long quads = 0;
+        if (P0.tile_w == 8 && P0.tile_h == 8) {
+            bool head = false;
+            if ((lane_id() & 3) == 0 && ((bgm >> lane_id()) & 15ull) == 15ull) {
+                const DevP &F = BATCH ? tab[fr] : P0;
+                const long long tf = gi - (F.q_begin >> P0.tn_shift);  // the tile's index in its frame
+                head = (F.tiles_x & 3) == 0 && (tf & 3) == 0;
+            }
+            quads = __ballot(head);
+        }
+        for (unsigned long long qm = quads; qm; qm &= qm - 1) {
+            const int l = __ffsll((long long)qm) - 1;
+            const int tfr = __shfl(fr, l, 64);
+            const int col = lane_id() & 31, t = col >> 3;
+            const float tcount = __shfl(count, l + t, 64);
+            for (int p = 0; p < 4; p++) {
+                const int r = 2 * p + (lane_id() >> 5);
+                bg_pixel(tfr, ((g0 + l + t) << P0.tn_shift) + r * 8 + (col & 7), true, tcount);
+            }
+        }
+        // the other background tiles one at a time: lane l takes item l
+        unsigned long long m = bgm & ~(quads | quads << 1 | quads << 2 | quads << 3);
         while (m) {
             const int l = __ffsll((long long)m) - 1;
             m &= m - 1;
             const long long tq = (g0 + l) << P0.tn_shift;  // the tile's first item
             const int tfr = __shfl(fr, l, 64);
             const float tcount = __shfl(count, l, 64);
-            const DevP &F = BATCH ? tab[tfr] : P0;
-            for (int k0 = 0; k0 < tsize; k0 += 64) {
-                const int k = k0 + lane_id();
-                bool handoff = false, first = false;
-                int x = 0, yr = 0, pix = 0;
-                int32_t seed = 0;
-                double c[3] = {0, 0, 0}, v[3] = {0, 0, 0};
-                if (k < tsize && item_pixel(F, (int)(tq + k - F.q_begin), x, yr)) {
-                    pix = (int)(F.pix_base + (long long)yr * F.width + x);
-                    // the start point (the direction only for a hand-off)
-                    pixel_start(F, x, yr, c);
-                    int it = 0;
-                    const float ms = F.de_stop;
-                    const float d32 = de32<KIND, DEM>(F, c, ms, it, cnt);
-                    if (!(d32 < __builtin_inff())) {  // single precision overflow: f64 decides
-                        cnt.overflow++;
-                        handoff = true;
-                    } else if (it >= F.max_it || !(d32 >= ms * fguard)) {
-                        handoff = true;
-                    } else {
-                        cnt.cone_bg++;
-                        write_background(F, B, pix, tcount);
-                    }
-                }
-                // (an exact append: these hand-offs are rare, and the list's
-                // capacity allows empty chunk tails for the march's waves only)
-                const long long slot = wave_append(handoff, &B.ctr[C_CAND]);
-                if (handoff) {
-                    pixel_ray_tab(F, B.rtab, x, yr, c, v);
-                    first = F.first_step_random != 0;
-                    seed = first ? dev_pixel_seed(F.row0 + yr, x) : 0;
-                    Cand &o = B.cand[slot];
-                    for (int i = 0; i < 3; i++) {
-                        o.c[i] = c[i];
-                        o.v[i] = v[i];
-                    }
-                    o.mzz = 0.0;
-                    o.last_step = 0.0;
-                    o.last_de = 0.0;
-                    o.rsf = 1.0f;
-                    o.step_count = 0.0f;
-                    o.pix = pix;
-                    o.flags = CF_START | (first ? CF_FIRST : 0) | (tfr << 8);
-                    o.nsteps = 0;
-                    o.seed = seed;
-                }
-            }
+            for (int k0 = 0; k0 < tsize; k0 += 64) bg_pixel(tfr, tq + k0 + lane_id(), k0 + lane_id() < tsize, tcount);
         }
     }
     flush_counters(cnt, B.ctr);

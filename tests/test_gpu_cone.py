@@ -13,12 +13,15 @@ pytestmark = pytest.mark.gpu
 
 
 def scenes():
+    # (bulb_wide: rows of 40 tiles of 8 x 8, so k_cone_bg takes background
+    # tiles four side by side as well as one at a time)
     return [("bulb", H.scene_bulb(240, 136, seed=4)), ("bulb_numeric_f32", H.scene_bulb(240, 136, seed=5,
                                                                                         de_mode=H.DE_REFERENCE)),
-            ("box", H.scene_box(240, 136, seed=3, step_cap=500)), ("hybrid", H.scene_hybrid(240, 136, seed=2))]
+            ("box", H.scene_box(240, 136, seed=3, step_cap=500)), ("hybrid", H.scene_hybrid(240, 136, seed=2)),
+            ("bulb_wide", H.scene_bulb(320, 184, seed=8))]
 
 
-@pytest.mark.parametrize("i", range(4))
+@pytest.mark.parametrize("i", range(5))
 @pytest.mark.parametrize("tile", [(8, 8), (16, 8)])
 def test_cone_background_leaves_the_plain_picture(hip, i, tile):
     """Mode 1 (a tile whose cone ray passes the end is background): depths,
@@ -103,7 +106,7 @@ def test_cone_background_tiles_hand_off_starts_in_the_guard(hip, tile):
     pixel of a background tile goes from k_cone_bg to the f64 finish at its
     start (as k_march32 hands off a start in the guard), and the picture --
     every output word -- is the plain march's with the same guard."""
-    sc = H.scene_bulb(240, 136, seed=4)  # (136 rows: ragged 16 x 16 tiles)
+    sc = H.scene_bulb(256, 136, seed=4)  # (rows of 32 tiles of 8 x 8, taken in fours; 136 rows: ragged 16 x 16)
     p = H.params_from_scene(sc)
     plain = hip.march(p, opts=H.options(tile=tile, guard_band=1e4))
     cone = hip.march(p, opts=H.options(tile=tile, guard_band=1e4, cone=True))
