@@ -124,21 +124,28 @@ struct Cnt {
 // call, so two kernels (a batch's and a single frame's) could round the same
 // DE differently; written out, every kernel computes the same bits.
 
-__device__ __forceinline__ void p8f(float &x, float &y, float &z, float jx, float jy, float jz, float zmul8) {
+// The power-8 triplex step (the HybridIntP8 polynomial, src/formulas.rs:
+// 837-858) in FP32, arranged for fewer operations with the same values up to
+// rounding.  With X = x^2, Y = y^2, R = X + Y, Z = z^2:
+//   x^8 - 28 x^6 y^2 + 70 x^4 y^4 - 28 x^2 y^6 + y^8 = R^4 - 32 R^2 XY + 128 (XY)^2
+//   x^6 - 7 x^4 y^2 + 7 x^2 y^4 - y^6 = (X - Y)(R^2 - 8 XY)
+//   and the polar factor a = 1 + num / R^4 = (r2^4 - 32 r2^2 ZR + 128 (ZR)^2) / R^4
+//   (the same Chebyshev identity in R and Z, r2 = R + Z = |v|^2),
+// so r2, the squared radius the caller already holds, replaces the fourth
+// powers.  (Checked against the f64 reference form on random points: the
+// same error distribution as that form evaluated in FP32.)
+__device__ __forceinline__ void p8f(float &x, float &y, float &z, float jx, float jy, float jz, float zmul8,
+                                    float r2) {
     const float xx = x * x, yy = y * y, zz = z * z;
-    const float r = fmaf(x, x, yy), rr = r * r, zzzz = zz * zz;
-    // (zzzz - 6 r zz + rr) (zz - r) sqrt(r)
-    const float t = fmaf(-6.0f * r, zz, zzzz + rr) * (zz - r) * __builtin_amdgcn_sqrtf(r);
+    const float R = fmaf(x, x, yy), rr = R * R, zr = zz * R;
+    // (Z^2 - 6 ZR + R^2) (Z - R) sqrt(R)
+    const float t = fmaf(-6.0f, zr, fmaf(zz, zz, rr)) * (zz - R) * __builtin_amdgcn_sqrtf(R);
     const float nz = fmaf(-(t * z), zmul8, jz);
-    // (70 rr + zzzz) zzzz - 28 zz r (zzzz + rr)
-    const float num = fmaf(fmaf(rr, 70.0f, zzzz), zzzz, -28.0f * (zz * r) * (zzzz + rr));
-    const float a = fmaf(num, __builtin_amdgcn_rcpf(fmaf(rr, rr, 1e-37f)), 1.0f);
-    const float xxxx = xx * xx, yyyy = yy * yy;
-    // (yyyy (7 xx - yy) + xxxx (xx - 7 yy)) 8 x y a + jy
-    const float ny = fmaf(fmaf(yyyy, fmaf(7.0f, xx, -yy), xxxx * fmaf(-7.0f, yy, xx)) * (8.0f * x * y), a, jy);
-    // a (xxxx^2 + yyyy (70 xxxx + yyyy) - 28 xx yy (yyyy + xxxx)) + jx
-    const float nx =
-        fmaf(a, fmaf(xxxx, xxxx, fmaf(yyyy, fmaf(70.0f, xxxx, yyyy), -28.0f * (xx * yy) * (yyyy + xxxx))), jx);
+    const float s2 = r2 * r2;
+    const float a = fmaf(s2, s2, zr * fmaf(128.0f, zr, -32.0f * s2)) * __builtin_amdgcn_rcpf(fmaf(rr, rr, 1e-37f));
+    const float p2 = xx * yy;
+    const float ny = fmaf((xx - yy) * fmaf(-8.0f, p2, rr) * (x * y), 8.0f * a, jy);
+    const float nx = fmaf(a, fmaf(rr, rr, p2 * fmaf(128.0f, p2, -32.0f * rr)), jx);
     x = nx;
     y = ny;
     z = nz;
@@ -207,6 +214,9 @@ __device__ float de32_numeric(const DevP &P, const double c[3], float ms, int &i
         z[k] = jz[k];
     }
     float r0 = 0.f, r1 = 0.f, r2 = 0.f, r3 = 0.f;
+    float cur[4];  // each point's squared radius now (p8f's r2)
+#pragma unroll
+    for (int k = 0; k < 4; k++) cur[k] = sq3f(x[k], y[k], z[k]);
     bool a1 = true, a2 = true, a3 = true;
     int it = 0;
     int n = 0, btmp = P.slot_iters[0];
@@ -222,7 +232,7 @@ __device__ float de32_numeric(const DevP &P, const double c[3], float ms, int &i
         if (kind == K_P8) {
             const float zm = KIND == K_HYB ? 8.0f * P.fz_mul[slot] : zm8;
 #pragma unroll
-            for (int k = 0; k < 4; k++) p8f(x[k], y[k], z[k], jx[k], jy[k], jz[k], zm);
+            for (int k = 0; k < 4; k++) p8f(x[k], y[k], z[k], jx[k], jy[k], jz[k], zm, cur[k]);
             cnt.p8_32 += 1u + a1 + a2 + a3;
         } else {
             const float fo = P.ffold[slot], sc = P.fscale[slot], m2 = P.fmin_r2[slot], sm = P.fscale_min_r2[slot];
@@ -235,6 +245,10 @@ __device__ float de32_numeric(const DevP &P, const double c[3], float ms, int &i
         const float q1 = sq3f(x[1], y[1], z[1]);
         const float q2 = sq3f(x[2], y[2], z[2]);
         const float q3 = sq3f(x[3], y[3], z[3]);
+        cur[0] = r0;
+        cur[1] = q1;
+        cur[2] = q2;
+        cur[3] = q3;
         if (a1) {
             r1 = q1;
             a1 = !(q1 > P.f_rstop3d);
@@ -291,7 +305,7 @@ __device__ float de32_analytic(const DevP &P, const double c[3], float ms, int &
             // loop or the end of the last iteration)
             const float r2 = r;
             w = fmaf(8.0f * (r2 * r2 * r2 * __builtin_amdgcn_sqrtf(r2)), w, 1.0f);
-            p8f(x, y, z, jx, jy, jz, zm8);
+            p8f(x, y, z, jx, jy, jz, zm8, r2);
             it++;
             r = sq3f(x, y, z);
             if (otrap) ot = fminf(ot, r);
