@@ -453,26 +453,22 @@ __device__ __forceinline__ double rcp_fast64(double d) {
     return __builtin_fma(y, e, y);
 }
 
-// int_pow8_64 with FMAs (written out, as p8f's) and the refined hardware
-// root and reciprocal (the finish's form; the divisor rr^2 + 1e-40 lies in
+// int_pow8_64 in the form of p8f (the Chebyshev identities, the fourth
+// powers from r2 = |v|^2), with FMAs and the refined hardware root and
+// reciprocal: the finish's form (the divisor R^4 + 1e-40 lies in
 // [1e-40, 2^16] for an orbit that has not escaped)
-__device__ __forceinline__ void int_pow8_64_fast(double zmul, double v[4], const double j[4]) {
+__device__ __forceinline__ void int_pow8_64_fast(double zmul, double v[4], const double j[4], double r2) {
     const double x = v[0], y = v[1], z = v[2];
     const double xx = x * x, yy = y * y, zz = z * z;
-    const double r = __builtin_fma(x, x, yy);
-    const double rr = r * r;
-    const double zzzz = zz * zz;
-    const double t = __builtin_fma(-6.0 * r, zz, zzzz + rr) * (zz - r) * sqrt_fast64(r);
+    const double R = __builtin_fma(x, x, yy), rr = R * R, zr = zz * R;
+    const double t = __builtin_fma(-6.0, zr, __builtin_fma(zz, zz, rr)) * (zz - R) * sqrt_fast64(R);
     v[2] = __builtin_fma(-(t * z), 8.0 * zmul, j[2]);
-    const double num = __builtin_fma(__builtin_fma(rr, 70.0, zzzz), zzzz, -28.0 * (zz * r) * (zzzz + rr));
-    const double a = __builtin_fma(num, rcp_fast64(__builtin_fma(rr, rr, 1e-40)), 1.0);
-    const double xxxx = xx * xx, yyyy = yy * yy;
-    v[1] = __builtin_fma(
-        __builtin_fma(yyyy, __builtin_fma(7.0, xx, -yy), xxxx * __builtin_fma(-7.0, yy, xx)) * (8.0 * x * y), a, j[1]);
-    v[0] = __builtin_fma(
-        a,
-        __builtin_fma(xxxx, xxxx, __builtin_fma(yyyy, __builtin_fma(70.0, xxxx, yyyy), -28.0 * (xx * yy) * (yyyy + xxxx))),
-        j[0]);
+    const double s2 = r2 * r2;
+    const double a = __builtin_fma(s2, s2, zr * __builtin_fma(128.0, zr, -32.0 * s2)) *
+                     rcp_fast64(__builtin_fma(rr, rr, 1e-40));
+    const double p2 = xx * yy;
+    v[1] = __builtin_fma((xx - yy) * __builtin_fma(-8.0, p2, rr) * (x * y), 8.0 * a, j[1]);
+    v[0] = __builtin_fma(a, __builtin_fma(rr, rr, p2 * __builtin_fma(128.0, p2, -32.0 * rr)), j[0]);
 }
 
 // amazing_box (src/formulas.rs:713-736)
@@ -659,7 +655,7 @@ __device__ __forceinline__ void call_slot64(const DevP &P, int n, It64 &s) {
                     s.deriv1 = 8.0 * (r2 * r2 * r2 * sqrt(r2)) * s.deriv1 + 1.0;
             }
             if (FAST)
-                int_pow8_64_fast(o[0], v, s.j);
+                int_pow8_64_fast(o[0], v, s.j, s.rout);
             else
                 int_pow8_64(o[0], v, s.j);
             s.p8++;
