@@ -241,14 +241,18 @@ def main():
     else:
         def tuned(o):
             return H.autotune(frames, out, o)
+    note("autotuning the plain batch")
     pb_opts, _ = tuned(H.options())
+    note("autotuning the cone batch")
     c_opts, _ = tuned(H.options(cone=True))
+    note("autotuning the learned + cone batch")
     lc_opts, _ = tuned(H.options(mlp=net, cone=True))
     legs = {"plain_batched": with_opts(pb_opts), "cone_batched": with_opts(c_opts, cone=True),
             "gated_cone_batched": with_opts(c_opts, mlp=net, cone=True, gate=a.gate),
             "learned_batched": with_opts(lc_opts, mlp=net), "learned_cone_batched": with_opts(lc_opts, mlp=net, cone=True)}
     res = {}
     for name, o in legs.items():
+        note(f"the {name} leg")
         plan = H.Plan(frames, out, o)
         for _ in range(a.warmup):
             plan.launch(1)
