@@ -100,13 +100,15 @@ def test_batch_with_cones_equals_single_frames(hip, tile, mode):
     assert cb["cone_background"] == bg > 0
 
 
-@pytest.mark.parametrize("tile", [(8, 8), (16, 16)])
-def test_cone_background_tiles_hand_off_starts_in_the_guard(hip, tile):
+@pytest.mark.parametrize("tile,rnd", [((8, 8), False), ((16, 16), False), ((8, 8), True)])
+def test_cone_background_tiles_hand_off_starts_in_the_guard(hip, tile, rnd):
     """With a guard band so wide that every start's DE lies inside it, each
     pixel of a background tile goes from k_cone_bg to the f64 finish at its
     start (as k_march32 hands off a start in the guard), and the picture --
-    every output word -- is the plain march's with the same guard."""
+    every output word -- is the plain march's with the same guard; with the
+    random first step the hand-off carries the pixel's generator state."""
     sc = H.scene_bulb(256, 136, seed=4)  # (rows of 32 tiles of 8 x 8, taken in fours; 136 rows: ragged 16 x 16)
+    sc.first_step_random = 1 if rnd else 0
     p = H.params_from_scene(sc)
     plain = hip.march(p, opts=H.options(tile=tile, guard_band=1e4))
     cone = hip.march(p, opts=H.options(tile=tile, guard_band=1e4, cone=True))
