@@ -1099,9 +1099,15 @@ enum FinPhase : int { FP_START, FP_RESUME, FP_BACK, FP_STEP, FP_REF };
 // surface list, and k_normals computes the normal, colour and G-buffer
 // words.  So that every phase here makes the same f64 DE: one DE site per
 // turn.
+// The batches' finish (BATCH and FAST) is built for four waves per SIMD
+// (128 VGPRs, 16 of them spilled): at the compiler's three (155 VGPRs) its
+// waves waited a third of their cycles, and four hide more of that latency
+// (6.60 -> 6.24 ms a 48-frame launch).  A single frame's finish spills more
+// at four (41 VGPRs) and was slower there (0.84 -> 1.15 ms with the normals).
 template <int KIND, int DEM, bool BATCH, bool FAST>
-__global__ __launch_bounds__(256) void k_finish64(DevP P0, Bufs B, int all_pixels, const DevP *__restrict__ gtab,
-                                                  int nframes) {
+__global__ __launch_bounds__(256, (BATCH && FAST) ? 4 : 1) void k_finish64(DevP P0, Bufs B, int all_pixels,
+                                                                           const DevP *__restrict__ gtab,
+                                                                           int nframes) {
     const LdsFrames tab = BATCH ? stage_frames(gtab, nframes) : LdsFrames{nullptr};
     Cnt cnt;
     int fr = 0;
