@@ -880,10 +880,18 @@ __device__ __forceinline__ De64 de64_part(const DevP &P, const Part64 &pt, doubl
         else
             s.v[3] = 1.0;
         const int n = run64<KIND, FAST>(P, pt, s, max_it, rs);
-        if ((pt.de_option & 0x38) == 32)
+        if (FAST) {
+            // (the refined reciprocal where its operand allows it, the
+            // IEEE quotient otherwise: the derivative can reach 2^900)
+            const double den = (pt.de_option & 0x38) == 32 ? s.deriv1 : fabs(s.v[3]);
+            const double inv = den >= 0x1p-900 && den <= 0x1p900 ? rcp_fast64(den) : 1.0 / den;
+            result = (pt.de_option & 0x38) == 32 ? sqrt_fast64(s.rout) * 0.25 * log(s.rout) * inv
+                                                 : sqrt_fast64(s.rout) * inv;
+        } else if ((pt.de_option & 0x38) == 32) {
             result = sqrt(s.rout) * 0.25 * log(s.rout) / s.deriv1;
-        else
+        } else {
             result = sqrt(s.rout) / fabs(s.v[3]);
+        }
         if (calc_sit) o.smooth_it = smooth64(P, s, n);
         result = result * pt.dds;
         o.it_result = s.it_result;
