@@ -239,7 +239,7 @@ typedef struct mb3d_hip_options {
     int32_t tile_w, tile_h; /* pixel tile of the work queue (default 8 x 8; powers of two up to 64) */
     int32_t waves_per_cu;  /* persistent grid: waves per CU (default 16) */
     int32_t it_split;      /* extension: FP32 march steps after which a ray goes on in the f64 finish (0: never) */
-    float guard_band;      /* FP32 stop decisions within (1 + guard_band) x DE stop go to FP64 (0: 0.05) */
+    float guard_band;      /* FP32 stop decisions within (1 + guard_band) x DE stop go to FP64 (0: 0.02) */
     /* the learned warm start (extension; NULL: the plain march).  A ray
      * whose hit probability is at least mlp_conf_hit (0: 0.5; above 1: no
      * hit jumps) starts at the predicted hit distance less mlp_margin_rel of

@@ -329,7 +329,7 @@ void fill_devp(const mb3d_hip_params &p, const mb3d_hip_options &o, int row0, in
     // (0.05: the FP32 DE's error near the surface is ~1e-5 of it at the
     // zooms that take the FP32 march; 0.25 and 0.02 gave the same accuracy
     // against the oracle on the 1080p bulb, and 0.05 takes 6 % less time)
-    d.guard = 1.0f + (o.guard_band > 0.0f ? o.guard_band : 0.05f);
+    d.guard = 1.0f + (o.guard_band > 0.0f ? o.guard_band : 0.02f);
     d.f_d_rstop = (float)p.d_rstop;
     d.f_rstop3d = (float)p.rstop3d;
     d.d_rstop = p.d_rstop;
